@@ -1,0 +1,288 @@
+"""MI355X-native hot path of DoubleAdapParabolicFEM: assembly, equilibrated-flux
+estimator and MINRES for the space-time heat equation.
+
+This package is a thin ctypes mirror of the C ABI in ``include/apf.h``
+(library ``libapf.so``, HIP kernels for gfx950).  The public objects follow the
+reference's call chain (algorithm/adaptive_driver.hpp:317-336 ->
+compute_adaptive_test_space_iteration, src/finite_element/time_slabs/
+time_slab_adaptive_loop.hpp:727):
+
+* :class:`Space`   -- X/Y spaces, DOF numbering, K pattern, vertex patches
+                      (host integer work, no GPU needed);
+* :class:`Context` -- one HIP device + stream;
+* :class:`System`  -- device-resident K, b, solution and estimator buffers:
+  ``assemble`` (assemble_main_system_blocks), ``minres``
+  (solve_minres_with_parabolic_graph_norm), ``estimate`` (slab
+  reconstruction + solve_all_patches + flux indicators).
+
+There is no CPU fallback: if ``libapf.so`` is missing the import of the
+library raises, and device calls raise when no HIP device is present.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libapf.so")
+
+APF_OK, APF_BAD_ARG, APF_NOT_CONVERGED, APF_RESIDUAL_TOO_LARGE, APF_SINGULAR, APF_HIP_ERROR, APF_NO_DEVICE = range(7)
+APF_FP64, APF_FP32_REFINE = 0, 1
+APF_ASSEMBLE_KRON, APF_ASSEMBLE_SCATTER = 0, 1
+
+_lib = None
+
+
+class APFError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[apf status {code}] {msg}")
+        self.code = code
+
+
+class NotConverged(APFError):
+    """Mirror of la::concepts::PreconditionedSolveFailure
+    (src/linear_algebra/preconditioners/solve_failure.hpp:7-25)."""
+
+    def __init__(self, code, msg, stats=None):
+        super().__init__(code, msg)
+        self.stats = stats
+
+
+class _MeshDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("p", ctypes.c_int), ("n_vertices", ctypes.c_int),
+                ("vertices", ctypes.POINTER(ctypes.c_double)), ("n_cells", ctypes.c_int),
+                ("cells", ctypes.POINTER(ctypes.c_int)), ("n_times", ctypes.c_int),
+                ("times", ctypes.POINTER(ctypes.c_double))]
+
+
+class _SpaceInfo(ctypes.Structure):
+    _fields_ = [("nY", ctypes.c_int64), ("nX", ctypes.c_int64), ("n", ctypes.c_int64), ("nnz", ctypes.c_int64),
+                ("dim", ctypes.c_int), ("p", ctypes.c_int), ("Ns", ctypes.c_int), ("Nt", ctypes.c_int),
+                ("nf", ctypes.c_int), ("n_nodes", ctypes.c_int), ("n_space_cells", ctypes.c_int),
+                ("n_intervals", ctypes.c_int), ("n_vertices", ctypes.c_int), ("nnz_spatial", ctypes.c_int),
+                ("n_patches", ctypes.c_int), ("max_patch_cells", ctypes.c_int), ("max_patch_flux", ctypes.c_int),
+                ("max_patch_scalar", ctypes.c_int), ("n_bottom_points", ctypes.c_int)]
+
+
+class _Coeff(ctypes.Structure):
+    _fields_ = [("M", ctypes.POINTER(ctypes.c_double)), ("ell", ctypes.c_double),
+                ("u0_bottom", ctypes.POINTER(ctypes.c_double))]
+
+
+class _MinresStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("true_relative_residual", ctypes.c_double), ("estimate", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+class _EstimateOut(ctypes.Structure):
+    _fields_ = [("lambda_y2", ctypes.c_double), ("trace2", ctypes.c_double), ("eta2", ctypes.c_double),
+                ("flux2", ctypes.c_double), ("recon2", ctypes.c_double), ("y_est2", ctypes.c_double),
+                ("div_residual_l2", ctypes.c_double), ("max_patch_residual", ctypes.c_double),
+                ("seconds_patches", ctypes.c_double), ("seconds_diagnostics", ctypes.c_double)]
+
+
+EXPORTED = {
+    "apf_space_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(_MeshDesc), ctypes.c_int]),
+    "apf_space_destroy": (None, [ctypes.c_void_p]),
+    "apf_space_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_SpaceInfo)]),
+    "apf_space_bottom_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_space_cell_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_space_free_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_space_pattern": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_space_patch_sizes": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "apf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "apf_ctx_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    "apf_ctx_destroy": (None, [ctypes.c_void_p]),
+    "apf_system_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_system_destroy": (None, [ctypes.c_void_p]),
+    "apf_assemble_main": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Coeff), ctypes.c_int]),
+    "apf_system_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_system_set_solution": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_minres_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(_MinresStats)]),
+    "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_system_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+
+def lib():
+    """Load libapf.so (raises if it has not been built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing; build it with `python -m doubleadapparabolicfem_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTED.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _check(code, ctx=None, what=""):
+    if code != APF_OK:
+        msg = lib().apf_last_error(ctx).decode(errors="replace")
+        raise APFError(code, f"{what}: {msg}" if what else msg)
+
+
+@dataclass
+class Mesh:
+    """Tensor space-time mesh: spatial vertices [nv, dim], cells [ns, dim+1], times [m+1]."""
+    dim: int
+    vertices: np.ndarray
+    cells: np.ndarray
+    times: np.ndarray
+
+
+class Space:
+    """X/Y spaces of degree p on a tensor mesh, with the CSR pattern of K."""
+
+    def __init__(self, mesh: Mesh, p: int, pattern: bool = True):
+        L = lib()
+        self.mesh = mesh
+        self._v = np.ascontiguousarray(mesh.vertices, dtype=np.float64).reshape(-1)
+        self._c = np.ascontiguousarray(mesh.cells, dtype=np.int32).reshape(-1)
+        self._t = np.ascontiguousarray(mesh.times, dtype=np.float64)
+        md = _MeshDesc(mesh.dim, p, len(self._v) // mesh.dim, self._v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                       len(self._c) // (mesh.dim + 1), self._c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       len(self._t), self._t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        h = ctypes.c_void_p()
+        _check(L.apf_space_create(ctypes.byref(h), ctypes.byref(md), 1 if pattern else 0), None, "apf_space_create")
+        self.handle = h
+        info = _SpaceInfo()
+        _check(L.apf_space_get_info(h, ctypes.byref(info)))
+        self.info = {f: getattr(info, f) for f, _ in _SpaceInfo._fields_}
+        self.p = p
+        self.dim = mesh.dim
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.apf_space_destroy(self.handle)
+            self.handle = None
+
+    def __getattr__(self, k):
+        info = self.__dict__.get("info")
+        if info is not None and k in info:
+            return info[k]
+        raise AttributeError(k)
+
+    def bottom_points(self):
+        nb = self.info["n_bottom_points"]
+        out = np.zeros((self.info["n_space_cells"], nb, self.dim))
+        _check(lib().apf_space_bottom_points(self.handle, _ptr(out)))
+        return out
+
+    def cell_free(self):
+        out = np.zeros((self.info["n_space_cells"], self.info["Ns"]), dtype=np.int32)
+        _check(lib().apf_space_cell_free(self.handle, _ptr(out)))
+        return out
+
+    def free_coords(self):
+        out = np.zeros((self.info["nf"], self.dim))
+        _check(lib().apf_space_free_coords(self.handle, _ptr(out)))
+        return out
+
+    def pattern(self):
+        rowptr = np.zeros(self.info["n"] + 1, dtype=np.int64)
+        col = np.zeros(self.info["nnz"], dtype=np.int32)
+        _check(lib().apf_space_pattern(self.handle, _ptr(rowptr), _ptr(col)))
+        return rowptr, col
+
+    def patch_sizes(self):
+        n = self.info["n_patches"]
+        nf, ns_, nc, bd = (np.zeros(n, dtype=np.int32) for _ in range(4))
+        _check(lib().apf_space_patch_sizes(self.handle, _ptr(nf), _ptr(ns_), _ptr(nc), _ptr(bd)))
+        return nf, ns_, nc, bd
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().apf_ctx_create(ctypes.byref(h), device), None, "apf_ctx_create")
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.apf_ctx_destroy(self.handle)
+            self.handle = None
+
+
+@dataclass
+class Coefficients:
+    """M per spatial cell ([ns, dim, dim] or None = identity), constant ell, u0 callable on [n, dim]."""
+    M: np.ndarray | None
+    ell: float
+    u0: object
+
+
+class System:
+    def __init__(self, ctx: Context, space: Space):
+        self.ctx, self.space = ctx, space
+        h = ctypes.c_void_p()
+        _check(lib().apf_system_create(ctx.handle, space.handle, ctypes.byref(h)), ctx.handle, "apf_system_create")
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.apf_system_destroy(self.handle)
+            self.handle = None
+
+    def assemble(self, coeff: Coefficients, mode: int = APF_ASSEMBLE_KRON):
+        sp = self.space
+        bp = sp.bottom_points()
+        u0b = np.ascontiguousarray(coeff.u0(bp.reshape(-1, sp.dim)).reshape(bp.shape[:2]), dtype=np.float64)
+        M = None if coeff.M is None else np.ascontiguousarray(coeff.M, dtype=np.float64).reshape(-1)
+        c = _Coeff(M.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if M is not None else None, float(coeff.ell),
+                   u0b.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        self._keep = (M, u0b)
+        _check(lib().apf_assemble_main(self.handle, ctypes.byref(c), mode), self.ctx.handle, "apf_assemble_main")
+
+    def download(self, values=True, rhs=True):
+        v = np.zeros(self.space.nnz) if values else None
+        b = np.zeros(self.space.n) if rhs else None
+        _check(lib().apf_system_download(self.handle, _ptr(v), _ptr(b)), self.ctx.handle)
+        return v, b
+
+    def set_solution(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        _check(lib().apf_system_set_solution(self.handle, _ptr(x)), self.ctx.handle)
+
+    def minres(self, tol=1e-10, max_it=20000, check_interval=100, precond=APF_FP64, x0=None, raise_on_fail=True):
+        st = _MinresStats()
+        x = np.zeros(self.space.n)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        code = lib().apf_minres_solve(self.handle, tol, max_it, check_interval, precond, _ptr(x0), _ptr(x),
+                                      ctypes.byref(st))
+        stats = {f: getattr(st, f) for f, _ in _MinresStats._fields_}
+        if code == APF_NOT_CONVERGED and raise_on_fail:
+            raise NotConverged(code, lib().apf_last_error(self.ctx.handle).decode(), stats)
+        if code not in (APF_OK, APF_NOT_CONVERGED):
+            _check(code, self.ctx.handle, "apf_minres_solve")
+        return x, stats
+
+    def estimate(self, prec=APF_FP64, refine_steps=2, per_cell=True):
+        out = _EstimateOut()
+        n = self.space.n_space_cells * self.space.n_intervals
+        fl = np.zeros(n) if per_cell else None
+        rs = np.zeros(n) if per_cell else None
+        rc = np.zeros(n) if per_cell else None
+        _check(lib().apf_estimate(self.handle, prec, refine_steps, ctypes.byref(out), _ptr(fl), _ptr(rs), _ptr(rc)),
+               self.ctx.handle, "apf_estimate")
+        res = {f: getattr(out, f) for f, _ in _EstimateOut._fields_}
+        res.update(flux_by_cell=fl, resid_by_cell=rs, recon_by_cell=rc)
+        return res
+
+    def timings(self):
+        t = np.zeros(4)
+        _check(lib().apf_system_timings(self.handle, _ptr(t)))
+        return dict(assembly_ms=t[0], patch_ms=t[1], diagnostics_ms=t[2], spmv_ms=t[3])

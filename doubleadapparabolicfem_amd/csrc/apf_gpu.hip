@@ -1,0 +1,1076 @@
+// C ABI + HIP kernels for gfx950: saddle-point assembly, CSR SpMV, MINRES.
+// The estimator kernels live in apf_estimator.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/apf.h"
+#include "apf_internal.hpp"
+
+using namespace apf;
+
+thread_local std::string g_last_error;
+
+static int set_error(apf_ctx* ctx, int code, const std::string& msg)
+{
+    g_last_error = msg;
+    if (ctx) ctx->error = msg;
+    return code;
+}
+
+
+// ============================================================== spatial elements
+// One thread per (spatial cell, i, j) local pair.  Computes the spatial element
+// stiffness K_c = sum_ab G_ab Khat_ab (G = |det J| J^-1 M J^-T) and mass
+// M_c = |det J| Mhat and scatter-adds them into the spatial CSR (the small
+// matrices the tensor expansion reads).  Restates the element integrals of
+// src/finite_element/assembly/detail/local_bilinear_forms.hpp:24-81 with the
+// geometry of src/finite_element/geometry/cell_geometry.hpp:161-207.
+__global__ void k_spatial_elements(int dim, int ns, int Ns, const double* __restrict__ verts,
+                                   const int* __restrict__ cells, const double* __restrict__ Mcell,
+                                   const int* __restrict__ cell_free, const int16_t* __restrict__ rank,
+                                   const int* __restrict__ sptr, const double* __restrict__ Khat,
+                                   const double* __restrict__ Mhat, const double* __restrict__ mhat,
+                                   int nb, const double* __restrict__ bw, const double* __restrict__ bphi,
+                                   const double* __restrict__ u0b, double* __restrict__ Ksp,
+                                   double* __restrict__ Msp, double* __restrict__ msp, double* __restrict__ gsp,
+                                   double* __restrict__ Kdiag_cell)
+{
+    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t total = static_cast<int64_t>(ns) * Ns * Ns;
+    if (tid >= total) return;
+    const int c = static_cast<int>(tid / (Ns * Ns));
+    const int ij = static_cast<int>(tid % (Ns * Ns));
+    const int i = ij / Ns, j = ij % Ns;
+    CellMetric g = cell_metric(dim, c, verts, cells, Mcell);
+    const int fi = cell_free[c * Ns + i];
+    const int fj = cell_free[c * Ns + j];
+    double kij = 0.0;
+    for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) kij += g.G[a * 2 + b] * Khat[((a * dim + b) * Ns + i) * Ns + j];
+    const double mij = g.det * Mhat[i * Ns + j];
+    if (fi >= 0 && fj >= 0) {
+        const int slot = sptr[fi] + rank[(static_cast<int64_t>(c) * Ns + i) * Ns + j];
+        atomicAdd(&Ksp[slot], kij);
+        atomicAdd(&Msp[slot], mij);
+    }
+    if (j == 0 && fi >= 0) {
+        atomicAdd(&msp[fi], g.det * mhat[i]);
+        // g = g_scale * int_{t=0} u0 phi_i, g_scale = -1
+        // (src/finite_element/assembly/detail/local_trace_linear_forms.hpp:22-52)
+        double s = 0.0;
+        for (int q = 0; q < nb; ++q) s += bw[q] * u0b[static_cast<int64_t>(c) * nb + q] * bphi[q * Ns + i];
+        atomicAdd(&gsp[fi], -g.det * s);
+    }
+}
+
+// ============================================================== tensor expansion of K
+// K entries on a tensor mesh factor as (spatial matrix entry) x (time table
+// entry); one wavefront slice writes consecutive nonzeros of a row, so the
+// fill is a coalesced stream of stores at the HBM write roofline.
+//   A_y((k,l,s),(k,l',s'))      = K(s,s') h_k Mt[l][l']
+//   B((kp+i,s),(k,l,s'))        = M(s,s') Dt[i][l] + K(s,s') h_k Mt[i][l]
+//   C((0,s),(0,s'))             = -M(s,s')   (t = 0 layer)
+template <int LANES>
+__global__ void __launch_bounds__(256) k_fill_kron(int64_t nY, int64_t n, int nf, int p, int m,
+                                                   const int64_t* __restrict__ rowptr,
+                                                   const int* __restrict__ sptr, const double* __restrict__ Ksp,
+                                                   const double* __restrict__ Msp, const double* __restrict__ times,
+                                                   TimeTables tt, double* __restrict__ val)
+{
+    const int Nt = p + 1;
+    const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / LANES;
+    const int lane = threadIdx.x % LANES;
+    if (row >= n) return;
+    const int64_t r0 = rowptr[row];
+    const int64_t len = rowptr[row + 1] - r0;
+    if (row < nY) {
+        const int s = static_cast<int>(row % nf);
+        const int64_t kl = row / nf;
+        const int k = static_cast<int>(kl / Nt), l = static_cast<int>(kl % Nt);
+        const double h = times[k + 1] - times[k];
+        const int sb = sptr[s];
+        const int deg = sptr[s + 1] - sb;
+        for (int64_t e = lane; e < len; e += LANES) {
+            const int pos = static_cast<int>(e);
+            double v;
+            if (pos < Nt * deg) {
+                const int lp = pos / deg, r = pos - lp * deg;
+                v = Ksp[sb + r] * h * tt.Mt[l * Nt + lp];
+            } else {
+                const int q = pos - Nt * deg;
+                const int it = q / deg, r = q - it * deg;
+                v = Msp[sb + r] * tt.Dt[it * Nt + l] + Ksp[sb + r] * h * tt.Mt[it * Nt + l];
+            }
+            __builtin_nontemporal_store(v, &val[r0 + e]);
+        }
+    } else {
+        const int64_t x = row - nY;
+        const int tn = static_cast<int>(x / nf);
+        const int s = static_cast<int>(x % nf);
+        const int sb = sptr[s];
+        const int deg = sptr[s + 1] - sb;
+        int k0, nk;
+        time_node_intervals(tn, p, m, k0, nk);
+        for (int64_t e = lane; e < len; e += LANES) {
+            const int pos = static_cast<int>(e);
+            double v;
+            const int blk = pos / (Nt * deg);
+            if (blk < nk) {
+                const int k = k0 + blk;
+                const int it = tn - k * p;
+                const int q = pos - blk * Nt * deg;
+                const int l = q / deg, r = q - l * deg;
+                const double h = times[k + 1] - times[k];
+                v = Msp[sb + r] * tt.Dt[it * Nt + l] + Ksp[sb + r] * h * tt.Mt[it * Nt + l];
+            } else {
+                // C block (t = 0 layer): -M(s,s') between the t = 0 nodes, structural zeros otherwise
+                const int q = pos - nk * Nt * deg;
+                const int tq = q / deg, r = q - tq * deg;
+                v = (tn == 0 && tq == 0) ? -Msp[sb + r] : 0.0;
+            }
+            __builtin_nontemporal_store(v, &val[r0 + e]);
+        }
+    }
+}
+
+__global__ void k_fill_rhs(int64_t nY, int64_t n, int nf, int p, double ell, const double* __restrict__ times,
+                           TimeTables tt, const double* __restrict__ msp, const double* __restrict__ gsp,
+                           double* __restrict__ b)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int Nt = p + 1;
+    if (i < nY) {
+        const int s = static_cast<int>(i % nf);
+        const int64_t kl = i / nf;
+        const int k = static_cast<int>(kl / Nt), l = static_cast<int>(kl % Nt);
+        b[i] = ell * msp[s] * (times[k + 1] - times[k]) * tt.mt[l];
+    } else {
+        const int64_t x = i - nY;
+        b[i] = (x < nf) ? gsp[x] : 0.0;
+    }
+}
+
+// ============================================================== element scatter path
+// Reference-structured path: one thread per (space-time cell, local i, local j)
+// computes A_loc(i,j) = K_c(is,js) h Mt(it,jt) and
+// B_loc(i,j) = M_c(is,js) Dt(it,jt) + K_c(is,js) h Mt(it,jt) and scatter-adds
+// them into K with FP64 atomics through structural slots
+// (src/finite_element/assembly/scatter.hpp:66-115; slot idea of
+// two_pass_full_saddle_2d.hpp:421-459).
+__global__ void k_element_scatter(int dim, int ns, int Ns, int p, int m, int nf, int64_t nY,
+                                  const double* __restrict__ verts, const int* __restrict__ cells,
+                                  const double* __restrict__ Mcell, const int* __restrict__ cell_free,
+                                  const int16_t* __restrict__ rank, const int* __restrict__ sptr,
+                                  const int64_t* __restrict__ rowptr, const double* __restrict__ times,
+                                  const double* __restrict__ Khat, const double* __restrict__ Mhat,
+                                  TimeTables tt, double* __restrict__ val)
+{
+    const int Nt = p + 1;
+    const int n = Ns * Nt;
+    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t total = static_cast<int64_t>(m) * ns * n * n;
+    if (tid >= total) return;
+    const int64_t cell = tid / (n * n);
+    const int IJ = static_cast<int>(tid % (n * n));
+    const int I = IJ / n, J = IJ % n;
+    const int c = static_cast<int>(cell % ns), k = static_cast<int>(cell / ns);
+    const int is = I / Nt, it = I % Nt, js = J / Nt, jt = J % Nt;
+    const int fi = cell_free[c * Ns + is], fj = cell_free[c * Ns + js];
+    if (fi < 0 || fj < 0) return;
+    CellMetric g = cell_metric(dim, c, verts, cells, Mcell);
+    double kij = 0.0;
+    for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) kij += g.G[a * 2 + b] * Khat[((a * dim + b) * Ns + is) * Ns + js];
+    const double mij = g.det * Mhat[is * Ns + js];
+    const double h = times[k + 1] - times[k];
+    const int rk = rank[(static_cast<int64_t>(c) * Ns + is) * Ns + js];  // js in adj(is)
+    const int rk_t = rank[(static_cast<int64_t>(c) * Ns + js) * Ns + is]; // is in adj(js)
+    const int degi = sptr[fi + 1] - sptr[fi];
+    const int degj = sptr[fj + 1] - sptr[fj];
+    // A_y: row Y(k,it,fi), col Y(k,jt,fj)
+    {
+        const int64_t row = (static_cast<int64_t>(k) * Nt + it) * nf + fi;
+        atomicAdd(&val[rowptr[row] + jt * degi + rk], kij * h * tt.Mt[it * Nt + jt]);
+    }
+    const double bval = mij * tt.Dt[it * Nt + jt] + kij * h * tt.Mt[it * Nt + jt];
+    // B: row X(kp+it, fi), col Y(k, jt, fj)
+    {
+        const int64_t row = nY + (static_cast<int64_t>(k) * p + it) * nf + fi;
+        const int kk = (it == 0 && k > 0) ? 1 : 0;
+        atomicAdd(&val[rowptr[row] + (kk * Nt + jt) * degi + rk], bval);
+    }
+    // B^T: row Y(k, jt, fj), col X(kp+it, fi)
+    {
+        const int64_t row = (static_cast<int64_t>(k) * Nt + jt) * nf + fj;
+        atomicAdd(&val[rowptr[row] + (Nt + it) * degj + rk_t], bval);
+    }
+    // C on the t = 0 layer: -int_{t=0} psi_i psi_j
+    if (k == 0 && it == 0 && jt == 0) {
+        const int64_t row = nY + fi;
+        atomicAdd(&val[rowptr[row] + Nt * degi + rk], -mij);
+    }
+}
+
+// ============================================================== SpMV
+template <int LANES>
+__global__ void __launch_bounds__(256) k_spmv(int64_t n, const int64_t* __restrict__ rowptr,
+                                              const int* __restrict__ col, const double* __restrict__ val,
+                                              const double* __restrict__ x, double* __restrict__ y)
+{
+    const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / LANES;
+    const int lane = threadIdx.x % LANES;
+    double sum = 0.0;
+    if (row < n) {
+        const int64_t b = rowptr[row], e = rowptr[row + 1];
+        for (int64_t j = b + lane; j < e; j += LANES) {
+            const double a = __builtin_nontemporal_load(&val[j]);
+            const int cj = __builtin_nontemporal_load(&col[j]);
+            sum += a * x[cj];
+        }
+    }
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, LANES);
+    if (row < n && lane == 0) y[row] = sum;
+}
+
+// ============================================================== reductions
+__device__ inline double block_sum(double v)
+{
+    __shared__ double sh[64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < static_cast<int>(blockDim.x / 64); ++i) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+// deterministic final reduction of per-block partials (single block)
+__device__ inline double reduce_partials(const double* part, int nparts)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+    return block_sum(s);
+}
+
+__global__ void k_residual_parts(int64_t n, const double* __restrict__ b, const double* __restrict__ Kx,
+                                 double* __restrict__ part)
+{
+    double s_r = 0.0, s_b = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double r = b[i] - Kx[i];
+        s_r += r * r;
+        s_b += b[i] * b[i];
+    }
+    const double a = block_sum(s_r);
+    const double c = block_sum(s_b);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[gridDim.x + blockIdx.x] = c;
+    }
+}
+
+__global__ void k_residual_final(const double* part, int nparts, double* out)
+{
+    const double a = reduce_partials(part, nparts);
+    const double c = reduce_partials(part + nparts, nparts);
+    if (threadIdx.x == 0) { out[0] = a; out[1] = c; }
+}
+
+// ============================================================== preconditioner
+// Block-diagonal preconditioner P = blockdiag(P_Y, P_X) in the spirit of
+// ParabolicGraphNormBlockPreconditioner
+// (src/linear_algebra/eigen_backend/preconditioners/parabolic_graph_norm_minres.hpp:46-181),
+// which factors A_y and H = -C + B_dt diag(A_y)^-1 B_dt^T + A_x exactly
+// (src/linear_algebra/preconditioners/schur_approximations.hpp:233-287).
+// Here each block keeps the full time coupling of one spatial node and only
+// the spatial diagonal:
+//   P_Y(k,s)  = K(s,s) h_k Mt                     ((p+1) x (p+1), inverse closed form)
+//   P_X(s)    = K(s,s) Mt_glob + Ms2(s) T_dd + M(s,s) e0 e0^T   (banded, mp+1)
+// with Ms2(s) = sum_s' M(s,s')^2 / K(s',s') and T_dd the time factor of
+// B_dt diag(A_y)^-1 B_dt^T.  Both are SPD; P_X is band-Cholesky factored.
+__global__ void k_precond_setup(int nf, int p, int m, const int* __restrict__ sptr, const int* __restrict__ scol,
+                                const double* __restrict__ Ksp, const double* __restrict__ Msp,
+                                const double* __restrict__ times, TimeTables tt, double* __restrict__ kdiag,
+                                double* __restrict__ band /* [nf][T][p+1] */, float* __restrict__ band32)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nf) return;
+    const int Nt = p + 1;
+    const int T = m * p + 1;
+    const int bw = p + 1;
+    double kd = 0.0, md = 0.0, ms2 = 0.0;
+    for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
+        const int t = scol[e];
+        if (t == s) { kd = Ksp[e]; md = Msp[e]; }
+    }
+    for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
+        const int t = scol[e];
+        double kt = 0.0;
+        for (int f = sptr[t]; f < sptr[t + 1]; ++f)
+            if (scol[f] == t) kt = Ksp[f];
+        ms2 += Msp[e] * Msp[e] / kt;
+    }
+    kdiag[s] = kd;
+    // band storage: B[tn][d] = H(tn, tn - d), d = 0..p  (lower band)
+    double* Bm = band + static_cast<int64_t>(s) * T * bw;
+    for (int i = 0; i < T * bw; ++i) Bm[i] = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const double h = times[k + 1] - times[k];
+        for (int i = 0; i < Nt; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double tdd = 0.0;
+                for (int l = 0; l < Nt; ++l) tdd += tt.Dt[i * Nt + l] * tt.Dt[j * Nt + l] / (h * tt.Mt[l * Nt + l]);
+                const double v = kd * h * tt.Mt[i * Nt + j] + ms2 * tdd;
+                Bm[(k * p + i) * bw + (i - j)] += v;
+            }
+    }
+    Bm[0] += md;  // -C at (0,0)
+    // band Cholesky in place: L(tn, tn-d)
+    for (int j = 0; j < T; ++j) {
+        double d = Bm[j * bw + 0];
+        for (int q = 1; q <= p && j - q >= 0; ++q) d -= Bm[j * bw + q] * Bm[j * bw + q];
+        d = sqrt(d);
+        Bm[j * bw + 0] = d;
+        for (int i = j + 1; i <= j + p && i < T; ++i) {
+            // L(i,j) = (H(i,j) - sum_q L(i,j-q) L(j,j-q)) / L(j,j)
+            double v = Bm[i * bw + (i - j)];
+            for (int q = 1; q <= p && j - q >= 0 && (i - (j - q)) <= p; ++q)
+                v -= Bm[i * bw + (i - (j - q))] * Bm[j * bw + q];
+            Bm[i * bw + (i - j)] = v / d;
+        }
+    }
+    float* B32 = band32 + static_cast<int64_t>(s) * T * bw;
+    for (int i = 0; i < T * bw; ++i) B32[i] = static_cast<float>(Bm[i]);
+}
+
+// Apply P^-1 to r (one thread per (k, s) Y block and per spatial node X line),
+// also forms t = t - coef * r2 first (fused MINRES update) and accumulates
+// dot(t, z) partials.  `mode`: 0 plain apply, 1 fused update.
+template <typename PT>
+__global__ void k_precond_apply(int nf, int p, int m, int64_t nY, const double* __restrict__ times,
+                                TimeTables tt, const double* __restrict__ kdiag, const PT* __restrict__ band,
+                                double* __restrict__ t, const double* __restrict__ r2,
+                                const MinresScalars* __restrict__ st, int fused, double* __restrict__ z,
+                                double* __restrict__ part)
+{
+    const int Nt = p + 1;
+    const int T = m * p + 1;
+    const int bw = p + 1;
+    const int64_t nyb = static_cast<int64_t>(m) * nf;  // Y blocks
+    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    double dot = 0.0;
+    double coef = 0.0;
+    if (fused) coef = st->alfa / st->beta;
+    if (tid < nyb) {
+        const int k = static_cast<int>(tid / nf), s = static_cast<int>(tid % nf);
+        const double scale = 1.0 / (kdiag[s] * (times[k + 1] - times[k]));
+        PT rv[5];
+        for (int l = 0; l < Nt; ++l) {
+            const int64_t idx = (static_cast<int64_t>(k) * Nt + l) * nf + s;
+            double v = t[idx];
+            if (fused) { v -= coef * r2[idx]; t[idx] = v; }
+            rv[l] = static_cast<PT>(v);
+        }
+        for (int l = 0; l < Nt; ++l) {
+            PT acc = 0;
+            for (int q = 0; q < Nt; ++q) acc += static_cast<PT>(tt.Mtinv[l * Nt + q]) * rv[q];
+            const int64_t idx = (static_cast<int64_t>(k) * Nt + l) * nf + s;
+            const double zv = static_cast<double>(acc) * scale;
+            z[idx] = zv;
+            dot += zv * static_cast<double>(t[idx]);
+        }
+    } else if (tid < nyb + nf) {
+        const int s = static_cast<int>(tid - nyb);
+        const PT* L = band + static_cast<int64_t>(s) * T * bw;
+        // forward: L y = r ; z holds y in place
+        for (int j = 0; j < T; ++j) {
+            const int64_t idx = nY + static_cast<int64_t>(j) * nf + s;
+            double v = t[idx];
+            if (fused) { v -= coef * r2[idx]; t[idx] = v; }
+            PT acc = static_cast<PT>(v);
+            for (int q = 1; q <= p && j - q >= 0; ++q)
+                acc -= L[j * bw + q] * static_cast<PT>(z[nY + static_cast<int64_t>(j - q) * nf + s]);
+            z[idx] = static_cast<double>(acc / L[j * bw + 0]);
+        }
+        // backward: L^T z = y
+        for (int j = T - 1; j >= 0; --j) {
+            const int64_t idx = nY + static_cast<int64_t>(j) * nf + s;
+            PT acc = static_cast<PT>(z[idx]);
+            for (int q = 1; q <= p && j + q < T; ++q)
+                acc -= L[(j + q) * bw + q] * static_cast<PT>(z[nY + static_cast<int64_t>(j + q) * nf + s]);
+            const double zv = static_cast<double>(acc / L[j * bw + 0]);
+            z[idx] = zv;
+            dot += zv * t[idx];
+        }
+    }
+    const double bs = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// ============================================================== MINRES vector kernels
+// Paige-Saunders MINRES (the algorithm behind Eigen::MINRES used at
+// src/linear_algebra/eigen_backend/preconditioners/parabolic_graph_norm_minres.hpp:240-243),
+// with all scalar recurrences kept on the device so the iteration needs no
+// host synchronisation.
+__global__ void k_minres_vec1(int64_t n, const MinresScalars* __restrict__ st, const double* __restrict__ y,
+                              double* __restrict__ t, const double* __restrict__ r1, double* __restrict__ v,
+                              double* __restrict__ part)
+{
+    const double s = 1.0 / st->beta;
+    const double c = (st->itn >= 1) ? st->beta / st->oldb : 0.0;
+    double dot = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double vi = s * y[i];
+        const double ti = s * t[i] - c * r1[i];
+        v[i] = vi;
+        t[i] = ti;
+        dot += vi * ti;
+    }
+    const double bs = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+__global__ void k_minres_alfa(MinresScalars* st, const double* part, int nparts)
+{
+    const double a = reduce_partials(part, nparts);
+    if (threadIdx.x == 0) st->alfa = a;
+}
+
+__global__ void k_minres_scal2(MinresScalars* st, const double* part, int nparts)
+{
+    const double b2 = reduce_partials(part, nparts);
+    if (threadIdx.x != 0) return;
+    MinresScalars s = *st;
+    s.oldb = s.beta;
+    s.beta = sqrt(fmax(b2, 0.0));
+    s.oldeps = s.epsln;
+    s.delta = s.cs * s.dbar + s.sn * s.alfa;
+    const double gbar = s.sn * s.dbar - s.cs * s.alfa;
+    s.epsln = s.sn * s.beta;
+    s.dbar = -s.cs * s.beta;
+    double gamma = hypot(gbar, s.beta);
+    if (gamma < 1e-300) gamma = 1e-300;
+    s.gamma = gamma;
+    s.cs = gbar / gamma;
+    s.sn = s.beta / gamma;
+    s.phi = s.cs * s.phibar;
+    s.phibar = s.sn * s.phibar;
+    s.itn += 1;
+    *st = s;
+}
+
+__global__ void k_minres_vec3(int64_t n, const MinresScalars* __restrict__ st, const double* __restrict__ v,
+                              const double* __restrict__ w2, const double* __restrict__ w,
+                              double* __restrict__ wnew /* aliases old w1 */, double* __restrict__ x)
+{
+    const double oe = st->oldeps, de = st->delta, ig = 1.0 / st->gamma, ph = st->phi;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double wn = (v[i] - oe * w2[i] - de * w[i]) * ig;
+        wnew[i] = wn;
+        x[i] += ph * wn;
+    }
+}
+
+__global__ void k_minres_init(MinresScalars* st, const double* part, int nparts)
+{
+    const double b2 = reduce_partials(part, nparts);
+    if (threadIdx.x != 0) return;
+    MinresScalars s{};
+    s.beta = sqrt(fmax(b2, 0.0));
+    s.beta1 = s.beta;
+    s.oldb = 0.0;
+    s.dbar = 0.0;
+    s.epsln = 0.0;
+    s.phibar = s.beta;
+    s.cs = -1.0;
+    s.sn = 0.0;
+    s.itn = 0;
+    *st = s;
+}
+
+__global__ void k_axpby(int64_t n, double a, const double* x, double b, double* y)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        y[i] = a * x[i] + b * y[i];
+}
+
+// ============================================================== host side
+namespace apf {
+
+static void upload(void* dst, const void* src, size_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+
+
+void System::spmv(const double* x, double* y)
+{
+    const int64_t n = space->n;
+    constexpr int L = 16;
+    const int64_t threads = n * L;
+    const int block = 256;
+    const int64_t grid = (threads + block - 1) / block;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spmv<L>), dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, n,
+                       d_rowptr, d_col, d_val, x, y);
+}
+
+}  // namespace apf
+
+static TimeTables make_time_tables(const RefTables& T)
+{
+    TimeTables tt{};
+    const int Nt = T.Nt;
+    for (int i = 0; i < Nt * Nt; ++i) { tt.Mt[i] = T.Mt[i]; tt.Dt[i] = T.Dt[i]; }
+    for (int i = 0; i < Nt; ++i) tt.mt[i] = T.mt[i];
+    std::vector<double> inv(T.Mt);
+    invert_dense(Nt, inv);
+    for (int i = 0; i < Nt * Nt; ++i) tt.Mtinv[i] = inv[i];
+    return tt;
+}
+
+extern "C" {
+
+const char* apf_last_error(const apf_ctx* ctx)
+{
+    if (ctx && !ctx->error.empty()) return ctx->error.c_str();
+    return g_last_error.c_str();
+}
+
+int apf_space_create(apf_space** out, const apf_mesh_desc* md, int want_pattern)
+{
+    if (!out || !md) return set_error(nullptr, APF_BAD_ARG, "apf_space_create: null argument");
+    try {
+        TensorMesh mesh;
+        mesh.dim = md->dim;
+        if (md->dim != 1 && md->dim != 2) return set_error(nullptr, APF_BAD_ARG, "apf_space_create: dim must be 1 or 2");
+        if (md->n_vertices <= 0 || md->n_cells <= 0 || md->n_times < 2 || !md->vertices || !md->cells || !md->times)
+            return set_error(nullptr, APF_BAD_ARG, "apf_space_create: empty mesh");
+        mesh.vertices.assign(md->vertices, md->vertices + static_cast<size_t>(md->n_vertices) * md->dim);
+        mesh.cells.assign(md->cells, md->cells + static_cast<size_t>(md->n_cells) * (md->dim + 1));
+        mesh.times.assign(md->times, md->times + md->n_times);
+        auto sp = std::make_unique<apf_space>();
+        build_space(sp->S, mesh, md->p);
+        if (mesh.dim == 2) {
+            // counter-clockwise orientation is required by the Piola signs
+            for (int c = 0; c < mesh.ns(); ++c) {
+                const double* a = &mesh.vertices[2 * mesh.cells[3 * c]];
+                const double* b = &mesh.vertices[2 * mesh.cells[3 * c + 1]];
+                const double* d = &mesh.vertices[2 * mesh.cells[3 * c + 2]];
+                const double det = (b[0] - a[0]) * (d[1] - a[1]) - (d[0] - a[0]) * (b[1] - a[1]);
+                if (!(det > 0.0))
+                    return set_error(nullptr, APF_BAD_ARG,
+                                     "apf_space_create: triangle " + std::to_string(c) + " is not counter-clockwise");
+            }
+        }
+        sp->T = make_ref_tables(mesh.dim, md->p);
+        if (want_pattern) build_pattern(sp->S);
+        sp->has_pattern = want_pattern != 0;
+        *out = sp.release();
+        return APF_OK;
+    } catch (const std::exception& e) {
+        return set_error(nullptr, APF_BAD_ARG, std::string("apf_space_create: ") + e.what());
+    }
+}
+
+void apf_space_destroy(apf_space* s) { delete s; }
+
+int apf_space_get_info(const apf_space* s, apf_space_info* info)
+{
+    if (!s || !info) return set_error(nullptr, APF_BAD_ARG, "apf_space_get_info: null argument");
+    const Space& S = s->S;
+    info->nY = S.nY;
+    info->nX = S.nX;
+    info->n = S.n;
+    info->nnz = S.nnz;
+    info->dim = S.dim;
+    info->p = S.p;
+    info->Ns = S.Ns;
+    info->Nt = S.Nt;
+    info->nf = S.nf;
+    info->n_nodes = S.n_nodes;
+    info->n_space_cells = S.mesh.ns();
+    info->n_intervals = S.mesh.m();
+    info->n_vertices = S.mesh.nv();
+    info->nnz_spatial = S.sptr.empty() ? 0 : S.sptr.back();
+    const PatchTopology& P = S.patches;
+    info->n_patches = P.n_patches;
+    int mc = 0, mf = 0, ms = 0;
+    for (int a = 0; a < P.n_patches; ++a) {
+        mc = std::max(mc, P.cell_ptr[a + 1] - P.cell_ptr[a]);
+        mf = std::max(mf, P.n_flux[a]);
+        ms = std::max(ms, P.n_scalar[a]);
+    }
+    info->max_patch_cells = mc;
+    info->max_patch_flux = mf;
+    info->max_patch_scalar = ms;
+    info->n_bottom_points = s->T.nb;
+    return APF_OK;
+}
+
+int apf_space_bottom_points(const apf_space* s, double* xy)
+{
+    if (!s || !xy) return set_error(nullptr, APF_BAD_ARG, "apf_space_bottom_points: null argument");
+    const Space& S = s->S;
+    const int d = S.dim, nb = s->T.nb;
+    for (int c = 0; c < S.mesh.ns(); ++c) {
+        const int* V = &S.mesh.cells[c * (d + 1)];
+        for (int q = 0; q < nb; ++q) {
+            if (d == 1) {
+                const double a = S.mesh.vertices[V[0]], b = S.mesh.vertices[V[1]];
+                xy[static_cast<size_t>(c) * nb + q] = a + (b - a) * s->T.bx[q];
+            } else {
+                const double* v0 = &S.mesh.vertices[2 * V[0]];
+                const double* v1 = &S.mesh.vertices[2 * V[1]];
+                const double* v2 = &S.mesh.vertices[2 * V[2]];
+                const double xi = s->T.bx[2 * q], eta = s->T.bx[2 * q + 1];
+                double* o = &xy[(static_cast<size_t>(c) * nb + q) * 2];
+                o[0] = v0[0] + xi * (v1[0] - v0[0]) + eta * (v2[0] - v0[0]);
+                o[1] = v0[1] + xi * (v1[1] - v0[1]) + eta * (v2[1] - v0[1]);
+            }
+        }
+    }
+    return APF_OK;
+}
+
+int apf_space_cell_free(const apf_space* s, int* out)
+{
+    if (!s || !out) return set_error(nullptr, APF_BAD_ARG, "apf_space_cell_free: null argument");
+    std::copy(s->S.cell_free.begin(), s->S.cell_free.end(), out);
+    return APF_OK;
+}
+
+int apf_space_free_coords(const apf_space* s, double* out)
+{
+    if (!s || !out) return set_error(nullptr, APF_BAD_ARG, "apf_space_free_coords: null argument");
+    const Space& S = s->S;
+    const int d = S.dim, Ns = S.Ns;
+    // node coordinates from the cells that own them
+    const auto tuples_xy = [&](int c, int i, double* o) {
+        const int* V = &S.mesh.cells[c * (d + 1)];
+        if (d == 1) {
+            const double a = S.mesh.vertices[V[0]], b = S.mesh.vertices[V[1]];
+            o[0] = a + (b - a) * static_cast<double>(i) / S.p;
+            return;
+        }
+        int k = 0;
+        for (int cc = 0; cc <= S.p; ++cc)
+            for (int bb = 0; bb <= S.p - cc; ++bb) {
+                if (k == i) {
+                    const int aa = S.p - bb - cc;
+                    for (int q = 0; q < 2; ++q)
+                        o[q] = (aa * S.mesh.vertices[2 * V[0] + q] + bb * S.mesh.vertices[2 * V[1] + q] +
+                                cc * S.mesh.vertices[2 * V[2] + q]) / S.p;
+                }
+                ++k;
+            }
+    };
+    for (int c = 0; c < S.mesh.ns(); ++c)
+        for (int i = 0; i < Ns; ++i) {
+            const int f = S.cell_free[c * Ns + i];
+            if (f >= 0) tuples_xy(c, i, &out[static_cast<size_t>(f) * d]);
+        }
+    return APF_OK;
+}
+
+int apf_space_pattern(const apf_space* s, int64_t* rowptr, int* col)
+{
+    if (!s) return set_error(nullptr, APF_BAD_ARG, "apf_space_pattern: null space");
+    if (!s->has_pattern) return set_error(nullptr, APF_BAD_ARG, "apf_space_pattern: space built without pattern");
+    if (rowptr) std::copy(s->S.rowptr.begin(), s->S.rowptr.end(), rowptr);
+    if (col) std::copy(s->S.col.begin(), s->S.col.end(), col);
+    return APF_OK;
+}
+
+int apf_space_patch_sizes(const apf_space* s, int* n_flux, int* n_scalar, int* n_cells, int* boundary)
+{
+    if (!s) return set_error(nullptr, APF_BAD_ARG, "apf_space_patch_sizes: null space");
+    const PatchTopology& P = s->S.patches;
+    for (int a = 0; a < P.n_patches; ++a) {
+        if (n_flux) n_flux[a] = P.n_flux[a];
+        if (n_scalar) n_scalar[a] = P.n_scalar[a];
+        if (n_cells) n_cells[a] = P.cell_ptr[a + 1] - P.cell_ptr[a];
+        if (boundary) boundary[a] = P.boundary[a];
+    }
+    return APF_OK;
+}
+
+int apf_ctx_create(apf_ctx** out, int device)
+{
+    if (!out) return set_error(nullptr, APF_BAD_ARG, "apf_ctx_create: null argument");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+        return set_error(nullptr, APF_NO_DEVICE, "apf_ctx_create: no HIP device " + std::to_string(device));
+    auto ctx = std::make_unique<apf_ctx>();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) return set_error(nullptr, APF_HIP_ERROR, "apf_ctx_create: hipSetDevice failed");
+    if (hipStreamCreate(&ctx->stream) != hipSuccess)
+        return set_error(nullptr, APF_HIP_ERROR, "apf_ctx_create: hipStreamCreate failed");
+    *out = ctx.release();
+    return APF_OK;
+}
+
+void apf_ctx_destroy(apf_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
+{
+    if (!ctx || !s || !out) return set_error(ctx, APF_BAD_ARG, "apf_system_create: null argument");
+    if (!s->has_pattern) return set_error(ctx, APF_BAD_ARG, "apf_system_create: space needs the K pattern");
+    auto sys = std::make_unique<apf_system>();
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        sys->ctx = ctx;
+        sys->space = &s->S;
+        sys->tables = &s->T;
+        sys->stream = ctx->stream;
+        const Space& S = s->S;
+        const RefTables& T = s->T;
+        const int ns = S.mesh.ns(), d = S.dim, Ns = S.Ns;
+        sys->tt = make_time_tables(T);
+        sys->d_verts = dalloc<double>(sys.get(), S.mesh.vertices.size());
+        upload(sys->d_verts, S.mesh.vertices.data(), S.mesh.vertices.size() * sizeof(double));
+        sys->d_cells = dalloc<int>(sys.get(), S.mesh.cells.size());
+        upload(sys->d_cells, S.mesh.cells.data(), S.mesh.cells.size() * sizeof(int));
+        sys->d_times = dalloc<double>(sys.get(), S.mesh.times.size());
+        upload(sys->d_times, S.mesh.times.data(), S.mesh.times.size() * sizeof(double));
+        sys->d_cell_free = dalloc<int>(sys.get(), S.cell_free.size());
+        upload(sys->d_cell_free, S.cell_free.data(), S.cell_free.size() * sizeof(int));
+        sys->d_rank = dalloc<int16_t>(sys.get(), S.rank.size());
+        upload(sys->d_rank, S.rank.data(), S.rank.size() * sizeof(int16_t));
+        sys->d_sptr = dalloc<int>(sys.get(), S.sptr.size());
+        upload(sys->d_sptr, S.sptr.data(), S.sptr.size() * sizeof(int));
+        sys->d_scol = dalloc<int>(sys.get(), S.scol.size());
+        upload(sys->d_scol, S.scol.data(), S.scol.size() * sizeof(int));
+        sys->d_Mcell = dalloc<double>(sys.get(), static_cast<size_t>(ns) * d * d);
+        sys->d_u0b = dalloc<double>(sys.get(), static_cast<size_t>(ns) * T.nb);
+        auto up_tab = [&](const std::vector<double>& v) {
+            double* p = dalloc<double>(sys.get(), v.size());
+            upload(p, v.data(), v.size() * sizeof(double));
+            return p;
+        };
+        sys->d_Khat = up_tab(T.Khat);
+        sys->d_Mhat = up_tab(T.Mhat);
+        sys->d_mhat = up_tab(T.mhat);
+        sys->d_bw = up_tab(T.bw);
+        sys->d_bphi = up_tab(T.bphi);
+        const size_t nnzs = S.scol.size();
+        sys->d_Ksp = dalloc<double>(sys.get(), nnzs);
+        sys->d_Msp = dalloc<double>(sys.get(), nnzs);
+        sys->d_msp = dalloc<double>(sys.get(), S.nf);
+        sys->d_gsp = dalloc<double>(sys.get(), S.nf);
+        sys->d_rowptr = dalloc<int64_t>(sys.get(), S.rowptr.size());
+        upload(sys->d_rowptr, S.rowptr.data(), S.rowptr.size() * sizeof(int64_t));
+        sys->d_col = dalloc<int>(sys.get(), S.col.size());
+        upload(sys->d_col, S.col.data(), S.col.size() * sizeof(int));
+        sys->d_val = dalloc<double>(sys.get(), S.col.size());
+        sys->d_b = dalloc<double>(sys.get(), S.n);
+        sys->d_x = dalloc<double>(sys.get(), S.n);
+        HIP_TRY(hipMemset(sys->d_x, 0, S.n * sizeof(double)));
+        for (int i = 0; i < 8; ++i) sys->d_vec[i] = dalloc<double>(sys.get(), S.n);
+        sys->nparts = 1024;
+        sys->d_part = dalloc<double>(sys.get(), 4 * sys->nparts);
+        sys->d_scal = dalloc<MinresScalars>(sys.get(), 1);
+        sys->d_res = dalloc<double>(sys.get(), 4);
+        const int Tn = S.mesh.m() * S.p + 1;
+        sys->d_kdiag = dalloc<double>(sys.get(), S.nf);
+        sys->d_band = dalloc<double>(sys.get(), static_cast<size_t>(S.nf) * Tn * (S.p + 1));
+        sys->d_band32 = dalloc<float>(sys.get(), static_cast<size_t>(S.nf) * Tn * (S.p + 1));
+        (void)Ns;
+        HIP_TRY(hipEventCreate(&sys->ev0));
+        HIP_TRY(hipEventCreate(&sys->ev1));
+        estimator_alloc(sys.get());
+        *out = sys.release();
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(ctx, APF_HIP_ERROR, std::string("apf_system_create: ") + e.what());
+    } catch (const std::exception& e) {
+        return set_error(ctx, APF_BAD_ARG, std::string("apf_system_create: ") + e.what());
+    }
+}
+
+void apf_system_destroy(apf_system* sys)
+{
+    if (!sys) return;
+    (void)hipSetDevice(sys->ctx->device);
+    (void)hipStreamSynchronize(sys->stream);
+    for (void* p : sys->allocs) (void)hipFree(p);
+    if (sys->ev0) (void)hipEventDestroy(sys->ev0);
+    if (sys->ev1) (void)hipEventDestroy(sys->ev1);
+    delete sys;
+}
+
+int apf_assemble_main(apf_system* sys, const apf_coeff* coeff, int mode)
+{
+    if (!sys || !coeff) return set_error(sys ? sys->ctx : nullptr, APF_BAD_ARG, "apf_assemble_main: null argument");
+    if (!coeff->u0_bottom) return set_error(sys->ctx, APF_BAD_ARG, "apf_assemble_main: u0_bottom is required");
+    if (mode != APF_ASSEMBLE_KRON && mode != APF_ASSEMBLE_SCATTER)
+        return set_error(sys->ctx, APF_BAD_ARG, "apf_assemble_main: unknown mode");
+    try {
+        HIP_TRY(hipSetDevice(sys->ctx->device));
+        const Space& S = *sys->space;
+        const RefTables& T = *sys->tables;
+        const int ns = S.mesh.ns(), d = S.dim;
+        std::vector<double> M(static_cast<size_t>(ns) * d * d, 0.0);
+        if (coeff->M) {
+            std::copy(coeff->M, coeff->M + M.size(), M.begin());
+            for (int c = 0; c < ns; ++c) {
+                const double* m = &M[static_cast<size_t>(c) * d * d];
+                const double det = (d == 1) ? m[0] : m[0] * m[3] - m[1] * m[2];
+                if (!(m[0] > 0.0 && det > 0.0) || (d == 2 && std::fabs(m[1] - m[2]) > 1e-14 * std::fabs(m[0])))
+                    return set_error(sys->ctx, APF_BAD_ARG,
+                                     "apf_assemble_main: M is not symmetric positive definite on cell " +
+                                         std::to_string(c));
+            }
+        } else {
+            for (int c = 0; c < ns; ++c)
+                for (int a = 0; a < d; ++a) M[static_cast<size_t>(c) * d * d + a * d + a] = 1.0;
+        }
+        sys->Mhost = M;
+        sys->ell = coeff->ell;
+        upload(sys->d_Mcell, M.data(), M.size() * sizeof(double));
+        upload(sys->d_u0b, coeff->u0_bottom, static_cast<size_t>(ns) * T.nb * sizeof(double));
+        hipStream_t st = sys->stream;
+        HIP_TRY(hipEventRecord(sys->ev0, st));
+        const size_t nnzs = S.scol.size();
+        HIP_TRY(hipMemsetAsync(sys->d_Ksp, 0, nnzs * sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(sys->d_Msp, 0, nnzs * sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(sys->d_msp, 0, S.nf * sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(sys->d_gsp, 0, S.nf * sizeof(double), st));
+        {
+            const int64_t total = static_cast<int64_t>(ns) * S.Ns * S.Ns;
+            const int block = 256;
+            hipLaunchKernelGGL(k_spatial_elements, dim3(static_cast<unsigned>((total + block - 1) / block)), dim3(block),
+                               0, st, d, ns, S.Ns, sys->d_verts, sys->d_cells, sys->d_Mcell, sys->d_cell_free,
+                               sys->d_rank, sys->d_sptr, sys->d_Khat, sys->d_Mhat, sys->d_mhat, T.nb, sys->d_bw,
+                               sys->d_bphi, sys->d_u0b, sys->d_Ksp, sys->d_Msp, sys->d_msp, sys->d_gsp, nullptr);
+        }
+        if (mode == APF_ASSEMBLE_KRON) {
+            constexpr int L = 32;
+            const int64_t threads = S.n * L;
+            const int block = 256;
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fill_kron<L>), dim3(static_cast<unsigned>((threads + block - 1) / block)),
+                               dim3(block), 0, st, S.nY, S.n, S.nf, S.p, S.mesh.m(), sys->d_rowptr, sys->d_sptr,
+                               sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_val);
+        } else {
+            HIP_TRY(hipMemsetAsync(sys->d_val, 0, S.col.size() * sizeof(double), st));
+            const int nloc = S.Ns * S.Nt;
+            const int64_t total = static_cast<int64_t>(S.mesh.m()) * ns * nloc * nloc;
+            const int block = 256;
+            hipLaunchKernelGGL(k_element_scatter, dim3(static_cast<unsigned>((total + block - 1) / block)), dim3(block), 0,
+                               st, d, ns, S.Ns, S.p, S.mesh.m(), S.nf, S.nY, sys->d_verts, sys->d_cells,
+                               sys->d_Mcell, sys->d_cell_free, sys->d_rank, sys->d_sptr, sys->d_rowptr,
+                               sys->d_times, sys->d_Khat, sys->d_Mhat, sys->tt, sys->d_val);
+        }
+        {
+            const int block = 256;
+            hipLaunchKernelGGL(k_fill_rhs, dim3(static_cast<unsigned>((S.n + block - 1) / block)), dim3(block), 0, st,
+                               S.nY, S.n, S.nf, S.p, coeff->ell, sys->d_times, sys->tt, sys->d_msp, sys->d_gsp,
+                               sys->d_b);
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(sys->ev1, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+        sys->t_assembly_ms = ms;
+        sys->assembled = true;
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_assemble_main: ") + e.what());
+    }
+}
+
+int apf_system_download(apf_system* sys, double* K_values, double* b)
+{
+    if (!sys) return set_error(nullptr, APF_BAD_ARG, "apf_system_download: null system");
+    try {
+        HIP_TRY(hipSetDevice(sys->ctx->device));
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+        if (K_values) HIP_TRY(hipMemcpy(K_values, sys->d_val, sys->space->col.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (b) HIP_TRY(hipMemcpy(b, sys->d_b, sys->space->n * sizeof(double), hipMemcpyDeviceToHost));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_system_download: ") + e.what());
+    }
+}
+
+int apf_system_set_solution(apf_system* sys, const double* x)
+{
+    if (!sys || !x) return set_error(sys ? sys->ctx : nullptr, APF_BAD_ARG, "apf_system_set_solution: null argument");
+    try {
+        HIP_TRY(hipSetDevice(sys->ctx->device));
+        HIP_TRY(hipMemcpy(sys->d_x, x, sys->space->n * sizeof(double), hipMemcpyHostToDevice));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_system_set_solution: ") + e.what());
+    }
+}
+
+int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval, int precond_prec,
+                     const double* x0, double* x_out, apf_minres_stats* stats)
+{
+    if (!sys) return set_error(nullptr, APF_BAD_ARG, "apf_minres_solve: null system");
+    if (!sys->assembled) return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: assemble first");
+    if (!(tol > 0.0) || max_it < 1 || check_interval < 1)
+        return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: bad tolerance / iteration limits");
+    if (precond_prec != APF_FP64 && precond_prec != APF_FP32_REFINE)
+        return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: unknown preconditioner precision");
+    try {
+        HIP_TRY(hipSetDevice(sys->ctx->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        const Space& S = *sys->space;
+        hipStream_t st = sys->stream;
+        const int64_t n = S.n;
+        const int block = 256;
+        const int nparts = sys->nparts;
+        const unsigned vgrid = static_cast<unsigned>(nparts);
+        // preconditioner setup
+        hipLaunchKernelGGL(k_precond_setup, dim3((S.nf + 127) / 128), dim3(128), 0, st, S.nf, S.p, S.mesh.m(),
+                           sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_kdiag,
+                           sys->d_band, sys->d_band32);
+        const int64_t npb = static_cast<int64_t>(S.mesh.m()) * S.nf + S.nf;
+        const unsigned pgrid = static_cast<unsigned>((npb + block - 1) / block);
+        if (pgrid > static_cast<unsigned>(4 * nparts)) throw std::runtime_error("too many preconditioner blocks");
+        auto precond = [&](double* t, const double* r2, int fused, double* z) {
+            if (precond_prec == APF_FP64)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_precond_apply<double>), dim3(pgrid), dim3(block), 0, st, S.nf, S.p,
+                                   S.mesh.m(), S.nY, sys->d_times, sys->tt, sys->d_kdiag, sys->d_band, t, r2,
+                                   sys->d_scal, fused, z, sys->d_part);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_precond_apply<float>), dim3(pgrid), dim3(block), 0, st, S.nf, S.p,
+                                   S.mesh.m(), S.nY, sys->d_times, sys->tt, sys->d_kdiag, sys->d_band32, t, r2,
+                                   sys->d_scal, fused, z, sys->d_part);
+        };
+        double* x = sys->d_x;
+        double* r1 = sys->d_vec[0];
+        double* r2 = sys->d_vec[1];
+        double* t = sys->d_vec[2];
+        double* y = sys->d_vec[3];
+        double* v = sys->d_vec[4];
+        double* w = sys->d_vec[5];
+        double* w1 = sys->d_vec[6];
+        double* w2 = sys->d_vec[7];
+        if (x0) HIP_TRY(hipMemcpyAsync(x, x0, n * sizeof(double), hipMemcpyHostToDevice, st));
+        else HIP_TRY(hipMemsetAsync(x, 0, n * sizeof(double), st));
+        // r1 = b - K x
+        sys->spmv(x, r1);
+        hipLaunchKernelGGL(k_axpby, dim3(vgrid), dim3(block), 0, st, n, 1.0, sys->d_b, -1.0, r1);
+        HIP_TRY(hipMemcpyAsync(r2, r1, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemsetAsync(w, 0, n * sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(w2, 0, n * sizeof(double), st));
+        // y = P^-1 r1, beta1 = sqrt(r1 . y)   (t := r1 copy for the apply kernel)
+        HIP_TRY(hipMemcpyAsync(t, r1, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        precond(t, nullptr, 0, y);
+        hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, static_cast<int>(pgrid));
+        // |b|
+        auto true_residual = [&](double& rel) {
+            sys->spmv(x, t);
+            hipLaunchKernelGGL(k_residual_parts, dim3(vgrid), dim3(block), 0, st, n, sys->d_b, t, sys->d_part);
+            hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
+            double h[2];
+            HIP_TRY(hipMemcpyAsync(h, sys->d_res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            rel = (h[1] > 0.0) ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+        };
+        double rel = 0.0;
+        true_residual(rel);
+        int it = 0;
+        bool conv = rel <= tol;
+        MinresScalars hs{};
+        // t must be re-zeroed: the residual check used it as scratch; the loop overwrites it.
+        while (!conv && it < max_it) {
+            const int batch = std::min(check_interval, max_it - it);
+            for (int b = 0; b < batch; ++b) {
+                // t = K y ; (v, t) update ; alfa
+                sys->spmv(y, t);
+                hipLaunchKernelGGL(k_minres_vec1, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, y, t, r1, v,
+                                   sys->d_part);
+                hipLaunchKernelGGL(k_minres_alfa, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, nparts);
+                // t -= alfa/beta r2 ; y = P^-1 t ; beta^2 = t . y
+                precond(t, r2, 1, y);
+                hipLaunchKernelGGL(k_minres_scal2, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part,
+                                   static_cast<int>(pgrid));
+                // rotate r1 <- r2 <- t
+                double* tmp = r1;
+                r1 = r2;
+                r2 = t;
+                t = tmp;
+                // w update and x += phi w
+                hipLaunchKernelGGL(k_minres_vec3, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, v, w2, w, w1, x);
+                double* wt = w1;
+                w1 = w2;
+                w2 = w;
+                w = wt;
+            }
+            it += batch;
+            HIP_TRY(hipGetLastError());
+            // true residual (t is free scratch here)
+            true_residual(rel);
+            HIP_TRY(hipMemcpy(&hs, sys->d_scal, sizeof(hs), hipMemcpyDeviceToHost));
+            conv = rel <= tol;
+            if (!std::isfinite(rel)) break;
+        }
+        HIP_TRY(hipMemcpy(&hs, sys->d_scal, sizeof(hs), hipMemcpyDeviceToHost));
+        if (x_out) HIP_TRY(hipMemcpy(x_out, x, n * sizeof(double), hipMemcpyDeviceToHost));
+        // one timed SpMV for the roofline record
+        HIP_TRY(hipEventRecord(sys->ev0, st));
+        sys->spmv(x, t);
+        HIP_TRY(hipEventRecord(sys->ev1, st));
+        HIP_TRY(hipEventSynchronize(sys->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+        sys->t_spmv_ms = ms;
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) {
+            stats->iterations = it;
+            stats->converged = conv ? 1 : 0;
+            stats->true_relative_residual = rel;
+            stats->estimate = hs.beta1 > 0 ? hs.phibar / hs.beta1 : 0.0;
+            stats->seconds = secs;
+        }
+        if (!conv)
+            return set_error(sys->ctx, APF_NOT_CONVERGED,
+                             "apf_minres_solve: not converged after " + std::to_string(it) +
+                                 " iterations, true relative residual " + std::to_string(rel));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_minres_solve: ") + e.what());
+    } catch (const std::exception& e) {
+        return set_error(sys->ctx, APF_BAD_ARG, std::string("apf_minres_solve: ") + e.what());
+    }
+}
+
+int apf_system_timings(apf_system* sys, double* ms4)
+{
+    if (!sys || !ms4) return set_error(nullptr, APF_BAD_ARG, "apf_system_timings: null argument");
+    ms4[0] = sys->t_assembly_ms;
+    ms4[1] = sys->t_patch_ms;
+    ms4[2] = sys->t_diag_ms;
+    ms4[3] = sys->t_spmv_ms;
+    return APF_OK;
+}
+
+}  // extern "C"

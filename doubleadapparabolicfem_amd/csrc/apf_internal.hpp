@@ -1,0 +1,214 @@
+// Internal types shared by the HIP translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "apf_space.hpp"
+#include "apf_tables.hpp"
+
+struct apf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;  // default priority (no priority streams)
+    std::string error;
+};
+
+struct apf_space {
+    apf::Space S;
+    apf::RefTables T;
+    bool has_pattern = false;
+};
+
+namespace apf {
+
+struct HipFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// temporal tables, Nt <= 5 (p <= 4), passed by value to kernels
+struct TimeTables {
+    double Mt[25];
+    double Dt[25];     // Dt[i][j] = int l_i' l_j
+    double Mtinv[25];
+    double mt[5];
+};
+
+struct MinresScalars {
+    double beta, beta1, oldb, alfa, dbar, epsln, oldeps, delta, gamma, cs, sn, phi, phibar;
+    int itn;
+};
+
+// Per-cell affine metric (restates CellGeometry<Space,dim>::make,
+// src/finite_element/geometry/cell_geometry.hpp:161-207):
+//   det = |det J|,  G = det J^-1 M J^-T  (stiffness metric),
+//   H = J^T M^-1 J / det  (contravariant-Piola mass metric),  Jm = J.
+struct CellMetric {
+    double det;
+    double G[4];
+    double H[4];
+    double J[4];
+};
+
+__host__ __device__ inline CellMetric cell_metric(int dim, int c, const double* verts, const int* cells,
+                                                  const double* Mcell)
+{
+    CellMetric g{};
+    if (dim == 1) {
+        const double h = verts[cells[2 * c + 1]] - verts[cells[2 * c]];
+        const double ah = h < 0 ? -h : h;
+        const double m = Mcell[c];
+        g.det = ah;
+        g.G[0] = m / ah;
+        g.H[0] = ah / m;
+        g.J[0] = h;
+        return g;
+    }
+    const int v0 = cells[3 * c], v1 = cells[3 * c + 1], v2 = cells[3 * c + 2];
+    const double J00 = verts[2 * v1] - verts[2 * v0], J01 = verts[2 * v2] - verts[2 * v0];
+    const double J10 = verts[2 * v1 + 1] - verts[2 * v0 + 1], J11 = verts[2 * v2 + 1] - verts[2 * v0 + 1];
+    const double det = J00 * J11 - J01 * J10;
+    const double ad = det < 0 ? -det : det;
+    const double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;  // J^-1
+    const double* M = Mcell + 4 * c;
+    // G = ad * Ji M Ji^T
+    const double a00 = i00 * M[0] + i01 * M[2], a01 = i00 * M[1] + i01 * M[3];
+    const double a10 = i10 * M[0] + i11 * M[2], a11 = i10 * M[1] + i11 * M[3];
+    g.G[0] = ad * (a00 * i00 + a01 * i01);
+    g.G[1] = ad * (a00 * i10 + a01 * i11);
+    g.G[2] = ad * (a10 * i00 + a11 * i01);
+    g.G[3] = ad * (a10 * i10 + a11 * i11);
+    // H = J^T M^-1 J / ad
+    const double md = M[0] * M[3] - M[1] * M[2];
+    const double n00 = M[3] / md, n01 = -M[1] / md, n10 = -M[2] / md, n11 = M[0] / md;
+    const double b00 = n00 * J00 + n01 * J10, b01 = n00 * J01 + n01 * J11;
+    const double b10 = n10 * J00 + n11 * J10, b11 = n10 * J01 + n11 * J11;
+    g.H[0] = (J00 * b00 + J10 * b10) / ad;
+    g.H[1] = (J00 * b01 + J10 * b11) / ad;
+    g.H[2] = (J01 * b00 + J11 * b10) / ad;
+    g.H[3] = (J01 * b01 + J11 * b11) / ad;
+    g.det = ad;
+    g.J[0] = J00; g.J[1] = J01; g.J[2] = J10; g.J[3] = J11;
+    return g;
+}
+
+struct EstimatorBuffers {
+    // device tables (estimator)
+    double* RTab = nullptr;
+    double* Bhat = nullptr;
+    double* qbar = nullptr;
+    double* R = nullptr;
+    double* Pv = nullptr;
+    double* Q = nullptr;
+    double* E = nullptr;
+    double* Phi = nullptr;
+    double* Khat = nullptr;
+    double* Mhat = nullptr;
+    double* dbary = nullptr;
+    double* Wt = nullptr;     // [Nt][Nt] int l_l L_k
+    double* Wdt = nullptr;    // [Nt][Nt] int l_l' L_k
+    double* Mdd = nullptr;
+    // patch topology
+    int* p_cell_ptr = nullptr;
+    int* p_cells = nullptr;
+    int* p_lv = nullptr;
+    int* p_fmap = nullptr;
+    int8_t* p_fsign = nullptr;
+    int* p_nflux = nullptr;
+    int* p_nscal = nullptr;
+    uint8_t* p_boundary = nullptr;
+    int* order = nullptr;       // patch processing order (large first)
+    int* cell_patch_pos = nullptr;
+    // per (space-time cell, local vertex) flux contributions [ncell][dim+1][Nf][Nt]
+    double* sig = nullptr;
+    // per space-time cell outputs
+    double* flux = nullptr;
+    double* resid = nullptr;
+    double* recon = nullptr;
+    double* lam2 = nullptr;
+    double* trace = nullptr;
+    double* patch_res = nullptr;  // per patch max residual
+    // scratch per resident workgroup
+    double* scratch = nullptr;
+    size_t scratch_per_wg = 0;
+    int n_wg = 0;
+};
+
+struct System {
+    apf_ctx* ctx = nullptr;
+    const Space* space = nullptr;
+    const RefTables* tables = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    TimeTables tt{};
+    std::vector<double> Mhost;
+    double ell = 0.0;
+    // mesh / numbering
+    double* d_verts = nullptr;
+    int* d_cells = nullptr;
+    double* d_times = nullptr;
+    int* d_cell_free = nullptr;
+    int16_t* d_rank = nullptr;
+    int* d_sptr = nullptr;
+    int* d_scol = nullptr;
+    double* d_Mcell = nullptr;
+    double* d_u0b = nullptr;
+    // tables
+    double* d_Khat = nullptr;
+    double* d_Mhat = nullptr;
+    double* d_mhat = nullptr;
+    double* d_bw = nullptr;
+    double* d_bphi = nullptr;
+    // spatial matrices
+    double* d_Ksp = nullptr;
+    double* d_Msp = nullptr;
+    double* d_msp = nullptr;
+    double* d_gsp = nullptr;
+    // K
+    int64_t* d_rowptr = nullptr;
+    int* d_col = nullptr;
+    double* d_val = nullptr;
+    double* d_b = nullptr;
+    double* d_x = nullptr;
+    double* d_vec[8] = {};
+    int nparts = 0;
+    double* d_part = nullptr;
+    MinresScalars* d_scal = nullptr;
+    double* d_res = nullptr;
+    double* d_kdiag = nullptr;
+    double* d_band = nullptr;
+    float* d_band32 = nullptr;
+    EstimatorBuffers est;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float t_assembly_ms = 0.f, t_patch_ms = 0.f, t_diag_ms = 0.f, t_spmv_ms = 0.f;
+    bool assembled = false;
+
+    void spmv(const double* x, double* y);
+};
+
+void estimator_alloc(System* sys);
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            throw ::apf::HipFailure(std::string(#expr) + ": " + hipGetErrorString(_e));  \
+    } while (0)
+
+template <typename T>
+inline T* dalloc(System* sys, size_t count)
+{
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc(&p, count * sizeof(T)));
+    sys->allocs.push_back(p);
+    return static_cast<T*>(p);
+}
+
+}  // namespace apf
+
+struct apf_system : apf::System {};

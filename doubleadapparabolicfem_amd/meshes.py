@@ -1,0 +1,88 @@
+"""Structured tensor space-time meshes and problem data of the hot-path workloads.
+
+These are the active meshes the reference reaches by uniform refinement
+(algorithm/support/space_time_2d.hpp:91-114 roots, NVB bisection with the
+union-jack outcome; algorithm/support/space_time_1d.hpp:14-24) and the
+problem data of algorithm/examples/space_time_{1d,2d}/initial_data_*_examples.hpp.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import Coefficients, Mesh
+
+
+def interval_mesh(n: int, times) -> Mesh:
+    x = np.arange(n + 1) / n
+    cells = np.stack([np.arange(n), np.arange(1, n + 1)], axis=1)
+    return Mesh(1, x[:, None], cells.astype(np.int32), np.asarray(times, dtype=np.float64))
+
+
+def union_jack_mesh(n: int, times) -> Mesh:
+    """n x n squares of (0,1)^2; square (i,j) is cut along the diagonal through
+    its corner with even i+j (counter-clockwise triangles)."""
+    j, i = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    verts = np.stack([i.ravel() / n, j.ravel() / n], axis=1)
+    vid = lambda a, b: b * (n + 1) + a
+    cells = np.zeros((2 * n * n, 3), dtype=np.int32)
+    t = 0
+    for jj in range(n):
+        for ii in range(n):
+            if (ii + jj) % 2 == 0:
+                cells[t] = (vid(ii, jj), vid(ii + 1, jj), vid(ii + 1, jj + 1))
+                cells[t + 1] = (vid(ii, jj), vid(ii + 1, jj + 1), vid(ii, jj + 1))
+            else:
+                cells[t] = (vid(ii, jj), vid(ii + 1, jj), vid(ii, jj + 1))
+                cells[t + 1] = (vid(ii + 1, jj), vid(ii + 1, jj + 1), vid(ii, jj + 1))
+            t += 2
+    return Mesh(2, verts, cells, np.asarray(times, dtype=np.float64))
+
+
+def criss_cross_mesh(n: int, times) -> Mesh:
+    """n x n squares each cut by both diagonals (4 triangles around a centre vertex)."""
+    vid = lambda a, b: b * (n + 1) + a
+    verts = [(a / n, b / n) for b in range(n + 1) for a in range(n + 1)]
+    nv0 = len(verts)
+    cells = []
+    for jj in range(n):
+        for ii in range(n):
+            c = nv0 + jj * n + ii
+            verts.append(((ii + 0.5) / n, (jj + 0.5) / n))
+            a, b, cc, d = vid(ii, jj), vid(ii + 1, jj), vid(ii + 1, jj + 1), vid(ii, jj + 1)
+            cells += [(a, b, c), (b, cc, c), (cc, d, c), (d, a, c)]
+    return Mesh(2, np.array(verts), np.array(cells, dtype=np.int32), np.asarray(times, dtype=np.float64))
+
+
+def uniform_mesh(dim: int, outer: int) -> Mesh:
+    """Active mesh of uniform outer iteration `outer` of the reference's
+    uniform_standard runs (one uniform level from the roots):
+    1+1D: 2^(outer+1) rectangles per direction;
+    2+1D: outer 0 -> unit square cut by both diagonals, one interval;
+          outer >= 1 -> union-jack 2^(outer+1) squares per side, 2^outer intervals."""
+    if dim == 1:
+        n = 2 ** (outer + 1)
+        return interval_mesh(n, np.linspace(0.0, 1.0, n + 1))
+    if outer == 0:
+        return criss_cross_mesh(1, np.array([0.0, 1.0]))
+    n = 2 ** (outer + 1)
+    return union_jack_mesh(n, np.linspace(0.0, 1.0, 2 ** outer + 1))
+
+
+def smooth_initial(dim: int) -> Coefficients:
+    """M = I, ell = 0, u0 = prod_i sin(pi x_i) (smooth_initial examples)."""
+    if dim == 1:
+        return Coefficients(None, 0.0, lambda x: np.sin(np.pi * x[:, 0]))
+    return Coefficients(None, 0.0, lambda x: np.sin(np.pi * x[:, 0]) * np.sin(np.pi * x[:, 1]))
+
+
+def checkerboard_kappa(mesh: Mesh, k: int = 8, seed: int = 4) -> np.ndarray:
+    """M = kappa(x) I, kappa piecewise constant on a k x k spatial checkerboard,
+    kappa = 10^U(-1,1) per square (std::mt19937_64-independent: numpy PCG64 with `seed`)."""
+    rng = np.random.default_rng(seed)
+    vals = 10.0 ** rng.uniform(-1.0, 1.0, size=(k, k))
+    cen = mesh.vertices[mesh.cells].mean(axis=1)
+    ix = np.minimum((cen[:, 0] * k).astype(int), k - 1)
+    iy = np.minimum((cen[:, 1] * k).astype(int), k - 1)
+    kap = vals[iy, ix]
+    d = mesh.dim
+    return kap[:, None, None] * np.eye(d)[None, :, :]

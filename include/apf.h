@@ -1,0 +1,169 @@
+/*
+ * apf.h -- C ABI of the MI355X (gfx950) hot path: saddle-point assembly,
+ * equilibrated-flux estimator and preconditioned MINRES for the space-time
+ * heat equation of Rsmeets99/DoubleAdapParabolicFEM.
+ *
+ * Plain pointers and sizes only.  The host adapter (C++ or Python/ctypes)
+ * owns every host array; an apf_ctx owns device memory.  Calls on one ctx are
+ * not thread-safe.  Every function returns APF_OK (0) or an error code and, on
+ * error, leaves a message readable with apf_last_error().
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference repository):
+ *   apf_space_create      -- FESpace::initialize + DOF distribution + the
+ *                            two-pass symbolic pattern
+ *                            (src/finite_element/fespace/fespace.hpp:111-758,
+ *                             src/finite_element/assembly/main_system/
+ *                             two_pass_full_saddle_2d.hpp:326-421) and the
+ *                            vertex-patch sets
+ *                            (src/finite_element/time_slabs/
+ *                             time_slab_vertex_patch_builder.hpp:24-140)
+ *   apf_assemble_main     -- assemble_main_system_blocks
+ *                            (src/finite_element/system/
+ *                             assemble_saddle_problem.hpp:131-392) incl.
+ *                            compose_saddle_point_matrix
+ *                            (src/linear_algebra/system/saddle_point_system.hpp:105-118)
+ *   apf_minres_solve      -- SaddlePreconditionedSolveDispatcher<Backend>::solve ->
+ *                            solve_minres_with_parabolic_graph_norm
+ *                            (src/linear_algebra/eigen_backend/preconditioners/
+ *                             solve_saddle_system.hpp:32-121,
+ *                             parabolic_graph_norm_minres.hpp:232-369)
+ *   apf_estimate          -- compute_adaptive_test_space_iteration, estimator part
+ *                            (src/finite_element/time_slabs/
+ *                             time_slab_adaptive_loop.hpp:833-2026):
+ *                            solve_all_slabs, solve_all_patches
+ *                            (time_slab_equilibrated_flux_reconstruction.hpp:2515),
+ *                            compute_equilibrated_flux_error_squared_by_source_cell_*
+ *                            (time_slab_equilibrated_flux_indicators.hpp:34-525),
+ *                            compute_Y_error_squared_by_source_cell
+ *                            (time_slab_error_norms.hpp:371-537) and the X-marking
+ *                            components (algorithm/detail/indicator_utilities.hpp:444-472)
+ */
+#ifndef APF_H
+#define APF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum apf_status {
+    APF_OK = 0,
+    APF_BAD_ARG = 1,            /* std::invalid_argument / std::runtime_error on bad input */
+    APF_NOT_CONVERGED = 2,      /* PreconditionedSolveFailure (MINRES) */
+    APF_RESIDUAL_TOO_LARGE = 3, /* patch residual acceptance failed (dense_local_error_blocks_2d.hpp:923-949) */
+    APF_SINGULAR = 4,           /* non-SPD patch factor */
+    APF_HIP_ERROR = 5,
+    APF_NO_DEVICE = 6
+};
+
+enum apf_precision { APF_FP64 = 0, APF_FP32_REFINE = 1 };
+enum apf_assembly_mode { APF_ASSEMBLE_KRON = 0, APF_ASSEMBLE_SCATTER = 1 };
+
+typedef struct apf_ctx apf_ctx;
+typedef struct apf_space apf_space;
+typedef struct apf_system apf_system;
+
+/* Tensor space-time mesh: a conforming spatial mesh (intervals in 1D,
+ * counter-clockwise triangles in 2D) times a time partition. */
+typedef struct {
+    int dim;                 /* 1 or 2 */
+    int p;                   /* polynomial degree in space and time, 1..4 (estimator: 1..3) */
+    int n_vertices;
+    const double* vertices;  /* [n_vertices * dim] */
+    int n_cells;
+    const int* cells;        /* [n_cells * (dim + 1)] */
+    int n_times;
+    const double* times;     /* [n_times], strictly increasing */
+} apf_mesh_desc;
+
+typedef struct {
+    int64_t nY, nX, n, nnz;  /* sizes of Y, X, K = [A_y B^T; B C] and nnz(K) */
+    int dim, p, Ns, Nt;      /* local spatial / temporal Lagrange basis sizes */
+    int nf;                  /* free spatial nodes */
+    int n_nodes, n_space_cells, n_intervals, n_vertices;
+    int nnz_spatial;         /* spatial adjacency entries */
+    int n_patches;           /* spatial vertex patches (one per vertex, shared by all slabs) */
+    int max_patch_cells, max_patch_flux, max_patch_scalar;
+    int n_bottom_points;     /* bottom (t = 0) quadrature points per spatial cell */
+} apf_space_info;
+
+/* Problem data.  M is one SPD tensor per spatial cell (piecewise constant,
+ * time independent), ell a constant load, u0 sampled at the bottom quadrature
+ * points (apf_space_bottom_points) of every spatial cell. */
+typedef struct {
+    const double* M;          /* [n_space_cells * dim * dim] row-major, NULL -> identity */
+    double ell;
+    const double* u0_bottom;  /* [n_space_cells * n_bottom_points] */
+} apf_coeff;
+
+typedef struct {
+    int iterations;
+    int converged;
+    double true_relative_residual;  /* ||K x - b||_2 / ||b||_2 */
+    double estimate;                /* MINRES preconditioned residual estimate */
+    double seconds;
+} apf_minres_stats;
+
+typedef struct {
+    double lambda_y2, trace2, eta2;          /* ||lambda||_Y^2, ||u0 - u(0)||^2, eta^2 */
+    double flux2, recon2, y_est2;            /* estimator components */
+    double div_residual_l2;
+    double max_patch_residual;               /* max_i |K s - b|_inf / max(1, |b|_inf) over patches */
+    double seconds_patches, seconds_diagnostics;
+} apf_estimate_out;
+
+/* ---- host-side integer work (no GPU needed) ---- */
+int apf_space_create(apf_space** out, const apf_mesh_desc* mesh, int build_pattern);
+void apf_space_destroy(apf_space* s);
+int apf_space_get_info(const apf_space* s, apf_space_info* info);
+/* physical bottom quadrature points [n_space_cells][nb][dim] */
+int apf_space_bottom_points(const apf_space* s, double* xy);
+/* spatial free-node id per local basis function [n_space_cells][Ns] (-1 = Dirichlet) */
+int apf_space_cell_free(const apf_space* s, int* out);
+/* spatial node coordinates of free nodes [nf][dim] */
+int apf_space_free_coords(const apf_space* s, double* out);
+/* CSR pattern of K: rowptr [n+1], col [nnz] */
+int apf_space_pattern(const apf_space* s, int64_t* rowptr, int* col);
+/* patch topology: per patch n_flux, n_scalar, n_cells, boundary flag */
+int apf_space_patch_sizes(const apf_space* s, int* n_flux, int* n_scalar, int* n_cells, int* boundary);
+const char* apf_last_error(const apf_ctx* ctx); /* ctx may be NULL: last error of this thread */
+
+/* ---- device ---- */
+int apf_ctx_create(apf_ctx** out, int device);
+void apf_ctx_destroy(apf_ctx* ctx);
+int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out);
+void apf_system_destroy(apf_system* sys);
+
+/* Assemble K (CSR values), f, g on the device. */
+int apf_assemble_main(apf_system* sys, const apf_coeff* coeff, int mode);
+/* Copy K values [nnz] and b = [f; g] [n] to the host (either may be NULL). */
+int apf_system_download(apf_system* sys, double* K_values, double* b);
+
+/* Preconditioned MINRES on K x = b (x0 may be NULL).  The preconditioner is
+ * block diagonal: diag-in-space / exact-in-time blocks of A_y and of the
+ * parabolic graph-norm Schur approximation, applied in FP64 (precond_prec =
+ * APF_FP64) or FP32 (APF_FP32_REFINE).  Accepts when the true relative
+ * residual <= tol, checked every check_interval iterations. */
+int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval, int precond_prec,
+                     const double* x0, double* x_out, apf_minres_stats* stats);
+/* Upload a solution x = [lambda; u] [n] (e.g. from another solver). */
+int apf_system_set_solution(apf_system* sys, const double* x);
+
+/* Estimator on the current solution: slab reconstruction, one mixed patch
+ * problem per (slab, vertex), per-cell flux / residual / reconstruction terms.
+ * prec: APF_FP64 (FP64 Cholesky) or APF_FP32_REFINE (FP32 factors, FP64
+ * residual, `refine_steps` refinement steps).  Per-cell outputs (may be NULL)
+ * are indexed by space-time cell k * n_space_cells + c. */
+int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_estimate_out* out,
+                 double* flux_by_cell, double* resid_by_cell, double* recon_by_cell);
+
+/* Device timing of the last call of each phase in milliseconds (HIP events):
+ * [0] assembly, [1] patch kernel, [2] diagnostics, [3] one SpMV (last). */
+int apf_system_timings(apf_system* sys, double* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APF_H */

@@ -84,7 +84,18 @@ class SaddleBlocks:
     Ax: sp.csr_matrix   # nX x nX, int M grad_x psi_i . grad_x psi_j
 
     def K(self):
-        return sp.bmat([[self.A, self.B.T], [self.B, self.C]], format="csr")
+        """Compose [A_y B^T; B C] keeping explicit zeros (full pattern)."""
+        A = self.A.tocoo()
+        B = self.B.tocoo()
+        C = self.C.tocoo()
+        nY = A.shape[0]
+        r = np.concatenate([A.row, B.col, B.row + nY, C.row + nY])
+        c = np.concatenate([A.col, B.row + nY, B.col, C.col + nY])
+        v = np.concatenate([A.data, B.data, B.data, C.data])
+        n = nY + C.shape[0]
+        K = sp.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+        K.sort_indices()
+        return K
 
     def rhs(self):
         return np.concatenate([self.f, self.g])
@@ -132,12 +143,26 @@ def local_trace(stm: SpaceTimeMesh, p: int, prob: Problem, g_scale: float = -1.0
     return C, g, xq, meas, vals
 
 
-def _scatter(loc, rmap, cmap, shape):
+def _triplets(loc, rmap, cmap):
     nc, nr, ncol = loc.shape
     R = np.repeat(rmap[:, :, None], ncol, axis=2)
     Cc = np.repeat(cmap[:, None, :], nr, axis=1)
     mask = (R >= 0) & (Cc >= 0)
-    return sp.coo_matrix((loc[mask], (R[mask], Cc[mask])), shape=shape).tocsr()
+    return R[mask], Cc[mask], loc[mask]
+
+
+def _build(trips, shape):
+    """Sum duplicate triplets; explicit zeros are kept, so the pattern is the
+    full element-coupling pattern (sparse '+' would drop exact zeros)."""
+    if not trips:
+        return sp.csr_matrix(shape)
+    r = np.concatenate([t[0] for t in trips])
+    c = np.concatenate([t[1] for t in trips])
+    v = np.concatenate([t[2] for t in trips])
+    M = sp.coo_matrix((v, (r, c)), shape=shape).tocsr()
+    M.sum_duplicates()
+    M.sort_indices()
+    return M
 
 
 def _scatter_vec(loc, rmap, n):
@@ -150,24 +175,22 @@ def _scatter_vec(loc, rmap, n):
 def assemble(stm: SpaceTimeMesh, dofs: SpaceTimeDofs, prob: Problem, chunk: int = 8192) -> SaddleBlocks:
     p = dofs.p
     nX, nY = dofs.nX, dofs.nY
-    A = sp.csr_matrix((nY, nY))
-    B = sp.csr_matrix((nX, nY))
-    Bdt = sp.csr_matrix((nX, nY))
-    Ax = sp.csr_matrix((nX, nX))
+    tA, tB, tBdt, tAx = [], [], [], []
     f = np.zeros(nY)
     for c0 in range(0, stm.n_cells, chunk):
         cells = np.arange(c0, min(c0 + chunk, stm.n_cells))
         Al, Bl, fl, Bdtl = local_blocks(stm, p, prob, cells)
         ym = dofs.y_map[cells]
         xm = dofs.x_map[cells]
-        A = A + _scatter(Al, ym, ym, (nY, nY))
-        B = B + _scatter(Bl, xm, ym, (nX, nY))
-        Bdt = Bdt + _scatter(Bdtl, xm, ym, (nX, nY))
-        Ax = Ax + _scatter(Al, xm, xm, (nX, nX))
+        tA.append(_triplets(Al, ym, ym))
+        tB.append(_triplets(Bl, xm, ym))
+        tBdt.append(_triplets(Bdtl, xm, ym))
+        tAx.append(_triplets(Al, xm, xm))
         f += _scatter_vec(fl, ym, nY)
     Cl, gl, *_ = local_trace(stm, p, prob)
     ns = stm.space.n_cells
     xm0 = dofs.x_map[:ns]
-    C = _scatter(Cl, xm0, xm0, (nX, nX))
+    C = _build([_triplets(Cl, xm0, xm0)], (nX, nX))
     g = _scatter_vec(gl, xm0, nX)
-    return SaddleBlocks(A.tocsr(), B.tocsr(), C.tocsr(), f, g, Bdt.tocsr(), Ax.tocsr())
+    return SaddleBlocks(_build(tA, (nY, nY)), _build(tB, (nX, nY)), C, f, g,
+                        _build(tBdt, (nX, nY)), _build(tAx, (nX, nX)))

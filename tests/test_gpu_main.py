@@ -1,0 +1,80 @@
+"""GPU parity of the main-system path (assembly of K, f, g and MINRES)
+against the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import doubleadapparabolicfem_amd as apf
+from doubleadapparabolicfem_amd import meshes
+from oracle import assembly as OA, estimator as OE
+
+from gpu_helpers import oracle_setup
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(1, 1, 3), (1, 2, 2), (1, 3, 1), (2, 1, 1), (2, 2, 1), (2, 3, 0), (2, 2, 2)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return apf.Context(0)
+
+
+def _coeff(dim, kind):
+    if kind == "smooth":
+        return meshes.smooth_initial(dim)
+    return None
+
+
+@pytest.mark.parametrize("mode", [apf.APF_ASSEMBLE_KRON, apf.APF_ASSEMBLE_SCATTER], ids=["kron", "scatter"])
+@pytest.mark.parametrize("dim,p,outer", CASES)
+def test_assembly_matches_oracle(ctx, dim, p, outer, mode):
+    m = meshes.uniform_mesh(dim, outer)
+    coeff = meshes.smooth_initial(dim)
+    if dim == 2 and outer >= 1:
+        coeff = apf.Coefficients(meshes.checkerboard_kappa(m, 4, seed=7), 0.3, coeff.u0)
+    print(f"assembly dim={dim} p={p} outer={outer} mode={mode}", flush=True)
+    sp = apf.Space(m, p)
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(coeff, mode)
+    vals, b = sysm.download()
+    stm, od, prob = oracle_setup(m, p, coeff)
+    S = OA.assemble(stm, od, prob)
+    K = S.K()
+    rowptr, col = sp.pattern()
+    assert np.array_equal(K.indices, col)
+    scale = np.abs(K.data).max()
+    err = np.abs(vals - K.data)
+    # 1e-12 relative per entry; structural zeros / cancellation measured against the matrix scale
+    assert np.all(err <= 1e-12 * np.abs(K.data) + 1e-14 * scale), err.max()
+    rhs = S.rhs()
+    assert np.all(np.abs(b - rhs) <= 1e-12 * np.abs(rhs) + 1e-15 * np.abs(rhs).max())
+
+
+@pytest.mark.parametrize("prec", [apf.APF_FP64, apf.APF_FP32_REFINE], ids=["pc64", "pc32"])
+@pytest.mark.parametrize("dim,p,outer", [(1, 1, 3), (1, 2, 3), (2, 1, 2), (2, 2, 1), (2, 3, 1)])
+def test_minres_matches_direct_solve(ctx, dim, p, outer, prec):
+    m = meshes.uniform_mesh(dim, outer)
+    coeff = meshes.smooth_initial(dim)
+    print(f"minres dim={dim} p={p} outer={outer} prec={prec}", flush=True)
+    sp = apf.Space(m, p)
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(coeff)
+    x, st = sysm.minres(tol=1e-10, max_it=20000, check_interval=50, precond=prec)
+    assert st["converged"] and st["true_relative_residual"] <= 1e-10
+    stm, od, prob = oracle_setup(m, p, coeff)
+    S = OA.assemble(stm, od, prob)
+    xd = OE.main_solve(S)
+    K = S.K()
+    b = S.rhs()
+    assert np.linalg.norm(K @ x - b) <= 1.01e-10 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+def test_minres_failure_is_reported(ctx):
+    m = meshes.uniform_mesh(2, 2)
+    sp = apf.Space(m, 2)
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(meshes.smooth_initial(2))
+    with pytest.raises(apf.NotConverged) as ei:
+        sysm.minres(tol=1e-12, max_it=3, check_interval=3)
+    assert ei.value.stats["iterations"] == 3
