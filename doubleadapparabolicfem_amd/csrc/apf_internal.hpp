@@ -112,6 +112,9 @@ struct EstimatorBuffers {
     double* Wt = nullptr;     // [Nt][Nt] int l_l L_k
     double* Wdt = nullptr;    // [Nt][Nt] int l_l' L_k
     double* Mdd = nullptr;
+    double* Lnorm_ = nullptr;
+    double* Mt = nullptr;
+    double* Igrad = nullptr;
     // patch topology
     int* p_cell_ptr = nullptr;
     int* p_cells = nullptr;
@@ -132,10 +135,14 @@ struct EstimatorBuffers {
     double* lam2 = nullptr;
     double* trace = nullptr;
     double* patch_res = nullptr;  // per patch max residual
+    double* lt = nullptr;         // slab-reconstructed lambda
+    double* sums = nullptr;
     // scratch per resident workgroup
     double* scratch = nullptr;
     size_t scratch_per_wg = 0;
     int n_wg = 0;
+    int nmax = 0;
+    int chunk = 0;
 };
 
 struct System {

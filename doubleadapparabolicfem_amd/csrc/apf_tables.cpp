@@ -364,6 +364,7 @@ RefTables make_ref_tables(int dim, int p)
     T.Q.assign(static_cast<size_t>(NV) * Nq * Ns, 0.0);
     T.E.assign(static_cast<size_t>(D) * Nq * Ns, 0.0);
     T.Phi.assign(static_cast<size_t>(Nq) * Ns, 0.0);
+    std::vector<double> Gphi(static_cast<size_t>(Nf) * D * D * Ns, 0.0);  // int s_j,a d_b phi_m
 
     std::vector<double> phi(Ns), gphi(static_cast<size_t>(Ns) * 2), fv(static_cast<size_t>(Nf) * 2),
         fd(Nf), q(Nq), lam(NV);
@@ -406,6 +407,11 @@ RefTables make_ref_tables(int dim, int p)
                 for (int b = 0; b < D; ++b) T.E[(b * Nq + i) * Ns + m] += w * gphi[m * 2 + b] * q[i];
             }
         }
+        for (int j = 0; j < Nf; ++j)
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b)
+                    for (int mm = 0; mm < Ns; ++mm)
+                        Gphi[((j * D + a) * D + b) * Ns + mm] += w * fv[j * 2 + a] * gphi[mm * 2 + b];
         for (int v = 0; v < NV; ++v) {
             for (int i = 0; i < Nf; ++i)
                 for (int m = 0; m < Ns; ++m) {
@@ -425,6 +431,24 @@ RefTables make_ref_tables(int dim, int p)
     } else {
         const Rule2D r = duffy(2 * p + 4);
         for (size_t k = 0; k < r.w.size(); ++k) accumulate(r.x[k], r.y[k], r.w[k]);
+    }
+
+    // coefficients of e_a d_b phi_m in the flux basis: L2(flux) projection,
+    // exact because e_a d_b phi_m lies in the local flux space
+    {
+        std::vector<double> Mrt(static_cast<size_t>(Nf) * Nf, 0.0);
+        for (int a = 0; a < D; ++a)
+            for (int i = 0; i < Nf; ++i)
+                for (int j = 0; j < Nf; ++j) Mrt[i * Nf + j] += T.RTab[((a * D + a) * Nf + i) * Nf + j];
+        if (!invert_dense(Nf, Mrt)) throw std::runtime_error("make_ref_tables: singular flux mass");
+        T.Igrad.assign(static_cast<size_t>(Nf) * D * D * Ns, 0.0);
+        for (int j = 0; j < Nf; ++j)
+            for (int ab = 0; ab < D * D; ++ab)
+                for (int mm = 0; mm < Ns; ++mm) {
+                    double s = 0.0;
+                    for (int i = 0; i < Nf; ++i) s += Mrt[j * Nf + i] * Gphi[(i * D * D + ab) * Ns + mm];
+                    T.Igrad[(j * D * D + ab) * Ns + mm] = s;
+                }
     }
 
     // temporal tables

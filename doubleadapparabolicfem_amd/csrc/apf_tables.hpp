@@ -86,6 +86,7 @@ struct RefTables {
     std::vector<double> Lnorm;  // [Nt]                 int L_k^2 = 1/(2k+1)
     std::vector<double> dbary;  // [dim+1][dim]         grad of barycentric lam_v
     std::vector<double> Phi;    // [Nq][Ns]             int q_i phi_m
+    std::vector<double> Igrad;  // [Nf][dim][dim][Ns]   flux-basis coefficients of e_a d_b phi_m
     // facet structure of the local flux basis: local dof = facet*Nfm + moment
     std::vector<int> facet_v;   // [Nfc][2] local vertices of facet (2D: counter-clockwise)
 };
