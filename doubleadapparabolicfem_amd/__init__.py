@@ -32,6 +32,7 @@ LIB_PATH = os.path.join(_HERE, "libapf.so")
 APF_OK, APF_BAD_ARG, APF_NOT_CONVERGED, APF_RESIDUAL_TOO_LARGE, APF_SINGULAR, APF_HIP_ERROR, APF_NO_DEVICE = range(7)
 APF_FP64, APF_FP32_REFINE = 0, 1
 APF_ASSEMBLE_KRON, APF_ASSEMBLE_SCATTER = 0, 1
+APF_PRECOND_DIAGONAL, APF_PRECOND_MULTIGRID = 0, 1
 
 _lib = None
 
@@ -94,6 +95,7 @@ EXPORTED = {
     "apf_space_free_coords": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_space_pattern": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "apf_space_patch_sizes": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "apf_space_add_coarse_mesh": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_MeshDesc)]),
     "apf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "apf_ctx_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
     "apf_ctx_destroy": (None, [ctypes.c_void_p]),
@@ -104,6 +106,11 @@ EXPORTED = {
     "apf_system_set_solution": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_minres_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(_MinresStats)]),
+    "apf_minres_solve_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.POINTER(_MinresStats)]),
+    "apf_system_mg_setup_seconds": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "apf_bench_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "apf_system_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -144,21 +151,34 @@ class Mesh:
     times: np.ndarray
 
 
-class Space:
-    """X/Y spaces of degree p on a tensor mesh, with the CSR pattern of K."""
+def _mesh_desc(mesh: Mesh, p: int):
+    v = np.ascontiguousarray(mesh.vertices, dtype=np.float64).reshape(-1)
+    c = np.ascontiguousarray(mesh.cells, dtype=np.int32).reshape(-1)
+    t = np.ascontiguousarray(mesh.times if mesh.times is not None else [0.0, 1.0], dtype=np.float64)
+    md = _MeshDesc(mesh.dim, p, len(v) // mesh.dim, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                   len(c) // (mesh.dim + 1), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                   len(t), t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return md, (v, c, t)
 
-    def __init__(self, mesh: Mesh, p: int, pattern: bool = True):
+
+class Space:
+    """X/Y spaces of degree p on a tensor mesh, with the CSR pattern of K.
+
+    ``coarse``: optional nested coarser spatial meshes (fine to coarse) for the
+    multigrid MINRES preconditioner."""
+
+    def __init__(self, mesh: Mesh, p: int, pattern: bool = True, coarse=()):
         L = lib()
         self.mesh = mesh
-        self._v = np.ascontiguousarray(mesh.vertices, dtype=np.float64).reshape(-1)
-        self._c = np.ascontiguousarray(mesh.cells, dtype=np.int32).reshape(-1)
-        self._t = np.ascontiguousarray(mesh.times, dtype=np.float64)
-        md = _MeshDesc(mesh.dim, p, len(self._v) // mesh.dim, self._v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                       len(self._c) // (mesh.dim + 1), self._c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                       len(self._t), self._t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        md, self._keep = _mesh_desc(mesh, p)
         h = ctypes.c_void_p()
         _check(L.apf_space_create(ctypes.byref(h), ctypes.byref(md), 1 if pattern else 0), None, "apf_space_create")
         self.handle = h
+        self.n_coarse = 0
+        for cm in coarse:
+            cmd, keep = _mesh_desc(cm, p)
+            _check(L.apf_space_add_coarse_mesh(h, ctypes.byref(cmd)), None, "apf_space_add_coarse_mesh")
+            self.n_coarse += 1
         info = _SpaceInfo()
         _check(L.apf_space_get_info(h, ctypes.byref(info)))
         self.info = {f: getattr(info, f) for f, _ in _SpaceInfo._fields_}
@@ -257,12 +277,16 @@ class System:
         x = np.ascontiguousarray(x, dtype=np.float64)
         _check(lib().apf_system_set_solution(self.handle, _ptr(x)), self.ctx.handle)
 
-    def minres(self, tol=1e-10, max_it=20000, check_interval=100, precond=APF_FP64, x0=None, raise_on_fail=True):
+    def minres(self, tol=1e-10, max_it=20000, check_interval=100, precond=APF_FP64, x0=None, raise_on_fail=True,
+               kind=None):
+        """Preconditioned MINRES; kind None -> multigrid when coarse meshes were given."""
         st = _MinresStats()
         x = np.zeros(self.space.n)
         x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
-        code = lib().apf_minres_solve(self.handle, tol, max_it, check_interval, precond, _ptr(x0), _ptr(x),
-                                      ctypes.byref(st))
+        if kind is None:
+            kind = APF_PRECOND_MULTIGRID if self.space.n_coarse else APF_PRECOND_DIAGONAL
+        code = lib().apf_minres_solve_ex(self.handle, tol, max_it, check_interval, kind, precond, _ptr(x0), _ptr(x),
+                                         ctypes.byref(st))
         stats = {f: getattr(st, f) for f, _ in _MinresStats._fields_}
         if code == APF_NOT_CONVERGED and raise_on_fail:
             raise NotConverged(code, lib().apf_last_error(self.ctx.handle).decode(), stats)
@@ -281,6 +305,16 @@ class System:
         res = {f: getattr(out, f) for f, _ in _EstimateOut._fields_}
         res.update(flux_by_cell=fl, resid_by_cell=rs, recon_by_cell=rc)
         return res
+
+    def bench_spmv(self, reps=20):
+        v = ctypes.c_double()
+        _check(lib().apf_bench_spmv(self.handle, reps, ctypes.byref(v)), self.ctx.handle)
+        return v.value
+
+    def mg_setup_seconds(self):
+        v = ctypes.c_double()
+        _check(lib().apf_system_mg_setup_seconds(self.handle, ctypes.byref(v)))
+        return v.value
 
     def timings(self):
         t = np.zeros(4)

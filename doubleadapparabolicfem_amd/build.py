@@ -21,11 +21,11 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics"]
 
-SOURCES = ["apf_tables.cpp", "apf_space.cpp", "apf_gpu.hip", "apf_estimator.hip"]
+SOURCES = ["apf_tables.cpp", "apf_space.cpp", "apf_mg.cpp", "apf_gpu.hip", "apf_estimator.hip", "apf_mg.hip"]
 
 
 def _compile(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    obj = os.path.join(BUILD, src.replace(".", "_") + ".o")
     cmd = [HIPCC] + COMMON + ["-I", CSRC]
     if src.endswith(".hip"):
         cmd += ["-x", "hip"] + HIP_FLAGS

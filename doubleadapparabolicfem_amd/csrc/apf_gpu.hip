@@ -14,6 +14,7 @@
 
 #include "../../include/apf.h"
 #include "apf_internal.hpp"
+#include "apf_mg.hpp"
 
 using namespace apf;
 
@@ -446,6 +447,15 @@ __global__ void k_minres_vec1(int64_t n, const MinresScalars* __restrict__ st, c
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
+__global__ void k_minres_upd(int64_t n, const MinresScalars* __restrict__ st, double* __restrict__ t,
+                             const double* __restrict__ r2)
+{
+    const double c = st->alfa / st->beta;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        t[i] -= c * r2[i];
+}
+
 __global__ void k_minres_alfa(MinresScalars* st, const double* part, int nparts)
 {
     const double a = reduce_partials(part, nparts);
@@ -745,6 +755,7 @@ int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
         sys->ctx = ctx;
         sys->space = &s->S;
         sys->tables = &s->T;
+        sys->H = s->H.coarse.empty() ? nullptr : &s->H;
         sys->stream = ctx->stream;
         const Space& S = s->S;
         const RefTables& T = s->T;
@@ -897,6 +908,7 @@ int apf_assemble_main(apf_system* sys, const apf_coeff* coeff, int mode)
         HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
         sys->t_assembly_ms = ms;
         sys->assembled = true;
+        ++sys->assembly_version;
         return APF_OK;
     } catch (const HipFailure& e) {
         return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_assemble_main: ") + e.what());
@@ -932,12 +944,25 @@ int apf_system_set_solution(apf_system* sys, const double* x)
 int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval, int precond_prec,
                      const double* x0, double* x_out, apf_minres_stats* stats)
 {
+    const int kind = (sys && sys->H) ? APF_PRECOND_MULTIGRID : APF_PRECOND_DIAGONAL;
+    return apf_minres_solve_ex(sys, tol, max_it, check_interval, kind, precond_prec, x0, x_out, stats);
+}
+
+int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_interval, int precond_kind,
+                        int precond_prec, const double* x0, double* x_out, apf_minres_stats* stats)
+{
     if (!sys) return set_error(nullptr, APF_BAD_ARG, "apf_minres_solve: null system");
     if (!sys->assembled) return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: assemble first");
     if (!(tol > 0.0) || max_it < 1 || check_interval < 1)
         return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: bad tolerance / iteration limits");
     if (precond_prec != APF_FP64 && precond_prec != APF_FP32_REFINE)
         return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: unknown preconditioner precision");
+    if (precond_kind != APF_PRECOND_DIAGONAL && precond_kind != APF_PRECOND_MULTIGRID)
+        return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: unknown preconditioner kind");
+    if (precond_kind == APF_PRECOND_MULTIGRID && !sys->H)
+        return set_error(sys->ctx, APF_BAD_ARG,
+                         "apf_minres_solve: the multigrid preconditioner needs nested coarse meshes "
+                         "(apf_space_add_coarse_mesh)");
     try {
         HIP_TRY(hipSetDevice(sys->ctx->device));
         const auto t0 = std::chrono::steady_clock::now();
@@ -947,22 +972,57 @@ int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval
         const int block = 256;
         const int nparts = sys->nparts;
         const unsigned vgrid = static_cast<unsigned>(nparts);
-        // preconditioner setup
-        hipLaunchKernelGGL(k_precond_setup, dim3((S.nf + 127) / 128), dim3(128), 0, st, S.nf, S.p, S.mesh.m(),
-                           sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_kdiag,
-                           sys->d_band, sys->d_band32);
-        const int64_t npb = static_cast<int64_t>(S.mesh.m()) * S.nf + S.nf;
-        const unsigned pgrid = static_cast<unsigned>((npb + block - 1) / block);
-        if (pgrid > static_cast<unsigned>(4 * nparts)) throw std::runtime_error("too many preconditioner blocks");
-        auto precond = [&](double* t, const double* r2, int fused, double* z) {
-            if (precond_prec == APF_FP64)
+        const bool use_mg = precond_kind == APF_PRECOND_MULTIGRID;
+        const bool mg32 = precond_prec == APF_FP32_REFINE;
+        int pparts = nparts;
+        unsigned pgrid = 0;
+        if (use_mg) {
+            // (re)build the multigrid operators for the current assembly
+            uint64_t& ver = mg32 ? sys->mg32_version : sys->mg64_version;
+            if (ver != sys->assembly_version) {
+                const auto ts = std::chrono::steady_clock::now();
+                std::vector<double> K(S.scol.size()), M(S.scol.size());
+                HIP_TRY(hipMemcpyAsync(K.data(), sys->d_Ksp, K.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipMemcpyAsync(M.data(), sys->d_Msp, M.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                MGSetupHost hs;
+                setup_mg(S, *sys->H, K, M, sys->tables->Mt, sys->tables->Dt, hs);
+                const int nyc = S.mesh.m() * S.Nt;
+                if (mg32) {
+                    if (!sys->mg32) sys->mg32 = std::make_unique<MGDevice<float>>();
+                    sys->mg32->build(sys, hs, nyc);
+                } else {
+                    if (!sys->mg64) sys->mg64 = std::make_unique<MGDevice<double>>();
+                    sys->mg64->build(sys, hs, nyc);
+                }
+                HIP_TRY(hipStreamSynchronize(st));
+                ver = sys->assembly_version;
+                sys->t_mg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+            }
+        } else {
+            hipLaunchKernelGGL(k_precond_setup, dim3((S.nf + 127) / 128), dim3(128), 0, st, S.nf, S.p, S.mesh.m(),
+                               sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_kdiag,
+                               sys->d_band, sys->d_band32);
+            const int64_t npb = static_cast<int64_t>(S.mesh.m()) * S.nf + S.nf;
+            pgrid = static_cast<unsigned>((npb + block - 1) / block);
+            if (pgrid > static_cast<unsigned>(4 * nparts)) throw std::runtime_error("too many preconditioner blocks");
+            pparts = static_cast<int>(pgrid);
+        }
+        // z = P^-1 t and dot(z, t) partials in d_part; `r2` non-null fuses t -= alfa/beta r2 first
+        auto precond = [&](double* t, const double* r2, double* z) {
+            if (use_mg) {
+                if (r2) hipLaunchKernelGGL(k_minres_upd, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, t, r2);
+                if (mg32) sys->mg32->apply(st, S, t, z, sys->d_part, nparts);
+                else sys->mg64->apply(st, S, t, z, sys->d_part, nparts);
+            } else if (precond_prec == APF_FP64) {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_precond_apply<double>), dim3(pgrid), dim3(block), 0, st, S.nf, S.p,
                                    S.mesh.m(), S.nY, sys->d_times, sys->tt, sys->d_kdiag, sys->d_band, t, r2,
-                                   sys->d_scal, fused, z, sys->d_part);
-            else
+                                   sys->d_scal, r2 ? 1 : 0, z, sys->d_part);
+            } else {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_precond_apply<float>), dim3(pgrid), dim3(block), 0, st, S.nf, S.p,
                                    S.mesh.m(), S.nY, sys->d_times, sys->tt, sys->d_kdiag, sys->d_band32, t, r2,
-                                   sys->d_scal, fused, z, sys->d_part);
+                                   sys->d_scal, r2 ? 1 : 0, z, sys->d_part);
+            }
         };
         double* x = sys->d_x;
         double* r1 = sys->d_vec[0];
@@ -981,11 +1041,10 @@ int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval
         HIP_TRY(hipMemcpyAsync(r2, r1, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIP_TRY(hipMemsetAsync(w, 0, n * sizeof(double), st));
         HIP_TRY(hipMemsetAsync(w2, 0, n * sizeof(double), st));
-        // y = P^-1 r1, beta1 = sqrt(r1 . y)   (t := r1 copy for the apply kernel)
+        // y = P^-1 r1, beta1 = sqrt(r1 . y)
         HIP_TRY(hipMemcpyAsync(t, r1, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        precond(t, nullptr, 0, y);
-        hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, static_cast<int>(pgrid));
-        // |b|
+        precond(t, nullptr, y);
+        hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, pparts);
         auto true_residual = [&](double& rel) {
             sys->spmv(x, t);
             hipLaunchKernelGGL(k_residual_parts, dim3(vgrid), dim3(block), 0, st, n, sys->d_b, t, sys->d_part);
@@ -1000,25 +1059,21 @@ int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval
         int it = 0;
         bool conv = rel <= tol;
         MinresScalars hs{};
-        // t must be re-zeroed: the residual check used it as scratch; the loop overwrites it.
         while (!conv && it < max_it) {
             const int batch = std::min(check_interval, max_it - it);
             for (int b = 0; b < batch; ++b) {
-                // t = K y ; (v, t) update ; alfa
+                // t = K y ; v = y / beta ; t = t / beta - (beta / oldb) r1 ; alfa = v . t
                 sys->spmv(y, t);
                 hipLaunchKernelGGL(k_minres_vec1, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, y, t, r1, v,
                                    sys->d_part);
                 hipLaunchKernelGGL(k_minres_alfa, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, nparts);
                 // t -= alfa/beta r2 ; y = P^-1 t ; beta^2 = t . y
-                precond(t, r2, 1, y);
-                hipLaunchKernelGGL(k_minres_scal2, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part,
-                                   static_cast<int>(pgrid));
-                // rotate r1 <- r2 <- t
+                precond(t, r2, y);
+                hipLaunchKernelGGL(k_minres_scal2, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, pparts);
                 double* tmp = r1;
                 r1 = r2;
                 r2 = t;
                 t = tmp;
-                // w update and x += phi w
                 hipLaunchKernelGGL(k_minres_vec3, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, v, w2, w, w1, x);
                 double* wt = w1;
                 w1 = w2;
@@ -1027,15 +1082,12 @@ int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval
             }
             it += batch;
             HIP_TRY(hipGetLastError());
-            // true residual (t is free scratch here)
             true_residual(rel);
-            HIP_TRY(hipMemcpy(&hs, sys->d_scal, sizeof(hs), hipMemcpyDeviceToHost));
             conv = rel <= tol;
             if (!std::isfinite(rel)) break;
         }
         HIP_TRY(hipMemcpy(&hs, sys->d_scal, sizeof(hs), hipMemcpyDeviceToHost));
         if (x_out) HIP_TRY(hipMemcpy(x_out, x, n * sizeof(double), hipMemcpyDeviceToHost));
-        // one timed SpMV for the roofline record
         HIP_TRY(hipEventRecord(sys->ev0, st));
         sys->spmv(x, t);
         HIP_TRY(hipEventRecord(sys->ev1, st));
@@ -1061,6 +1113,55 @@ int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval
     } catch (const std::exception& e) {
         return set_error(sys->ctx, APF_BAD_ARG, std::string("apf_minres_solve: ") + e.what());
     }
+}
+
+int apf_space_add_coarse_mesh(apf_space* s, const apf_mesh_desc* md)
+{
+    if (!s || !md) return set_error(nullptr, APF_BAD_ARG, "apf_space_add_coarse_mesh: null argument");
+    try {
+        if (md->dim != s->S.dim) return set_error(nullptr, APF_BAD_ARG, "apf_space_add_coarse_mesh: dim mismatch");
+        TensorMesh mesh;
+        mesh.dim = md->dim;
+        mesh.vertices.assign(md->vertices, md->vertices + static_cast<size_t>(md->n_vertices) * md->dim);
+        mesh.cells.assign(md->cells, md->cells + static_cast<size_t>(md->n_cells) * (md->dim + 1));
+        mesh.times = {0.0, 1.0};
+        Space cs;
+        build_space(cs, mesh, s->S.p);
+        s->H.coarse.push_back(std::move(cs));
+        build_mg_prolongations(s->S, s->H);
+        return APF_OK;
+    } catch (const std::exception& e) {
+        if (!s->H.coarse.empty()) s->H.coarse.pop_back();
+        return set_error(nullptr, APF_BAD_ARG, std::string("apf_space_add_coarse_mesh: ") + e.what());
+    }
+}
+
+int apf_bench_spmv(apf_system* sys, int reps, double* ms_avg)
+{
+    if (!sys || !ms_avg || reps < 1) return set_error(nullptr, APF_BAD_ARG, "apf_bench_spmv: bad argument");
+    if (!sys->assembled) return set_error(sys->ctx, APF_BAD_ARG, "apf_bench_spmv: assemble first");
+    try {
+        HIP_TRY(hipSetDevice(sys->ctx->device));
+        hipStream_t st = sys->stream;
+        sys->spmv(sys->d_x, sys->d_vec[2]);  // warm
+        HIP_TRY(hipEventRecord(sys->ev0, st));
+        for (int r = 0; r < reps; ++r) sys->spmv(sys->d_x, sys->d_vec[2 + (r & 1)]);
+        HIP_TRY(hipEventRecord(sys->ev1, st));
+        HIP_TRY(hipEventSynchronize(sys->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+        *ms_avg = static_cast<double>(ms) / reps;
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_bench_spmv: ") + e.what());
+    }
+}
+
+int apf_system_mg_setup_seconds(apf_system* sys, double* sec)
+{
+    if (!sys || !sec) return set_error(nullptr, APF_BAD_ARG, "apf_system_mg_setup_seconds: null argument");
+    *sec = sys->t_mg_setup_s;
+    return APF_OK;
 }
 
 int apf_system_timings(apf_system* sys, double* ms4)

@@ -9,6 +9,10 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
+#include "apf_mg.hpp"
+#include "apf_mg_dev.hpp"
 #include "apf_space.hpp"
 #include "apf_tables.hpp"
 
@@ -22,6 +26,8 @@ struct apf_space {
     apf::Space S;
     apf::RefTables T;
     bool has_pattern = false;
+    apf::MGHierarchyHost H;   // nested coarser spatial meshes for the multigrid preconditioner
+    bool mg_ready = false;
 };
 
 namespace apf {
@@ -190,6 +196,11 @@ struct System {
     double* d_band = nullptr;
     float* d_band32 = nullptr;
     EstimatorBuffers est;
+    const MGHierarchyHost* H = nullptr;
+    std::unique_ptr<MGDevice<double>> mg64;
+    std::unique_ptr<MGDevice<float>> mg32;
+    uint64_t assembly_version = 0, mg64_version = 0, mg32_version = 0;
+    double t_mg_setup_s = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float t_assembly_ms = 0.f, t_patch_ms = 0.f, t_diag_ms = 0.f, t_spmv_ms = 0.f;
     bool assembled = false;

@@ -1,0 +1,433 @@
+// Host setup of the space-time multigrid preconditioner (see apf_mg.hpp).
+#include "apf_mg.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <random>
+#include <stdexcept>
+#include <string>
+
+#include "apf_tables.hpp"
+
+namespace apf {
+
+namespace {
+
+// coordinates of local basis node i of spatial cell c
+void node_coords(const Space& S, int c, int i, double* out)
+{
+    const int d = S.dim;
+    const int* V = &S.mesh.cells[c * (d + 1)];
+    if (d == 1) {
+        const double a = S.mesh.vertices[V[0]], b = S.mesh.vertices[V[1]];
+        out[0] = a + (b - a) * static_cast<double>(i) / S.p;
+        return;
+    }
+    int k = 0;
+    for (int cc = 0; cc <= S.p; ++cc)
+        for (int bb = 0; bb <= S.p - cc; ++bb) {
+            if (k == i) {
+                const int aa = S.p - bb - cc;
+                for (int q = 0; q < 2; ++q)
+                    out[q] = (aa * S.mesh.vertices[2 * V[0] + q] + bb * S.mesh.vertices[2 * V[1] + q] +
+                              cc * S.mesh.vertices[2 * V[2] + q]) / S.p;
+                return;
+            }
+            ++k;
+        }
+}
+
+// reference coordinates of physical point x in cell c
+void to_reference(const Space& S, int c, const double* x, double* xi)
+{
+    const int d = S.dim;
+    const int* V = &S.mesh.cells[c * (d + 1)];
+    if (d == 1) {
+        const double a = S.mesh.vertices[V[0]], b = S.mesh.vertices[V[1]];
+        xi[0] = (x[0] - a) / (b - a);
+        return;
+    }
+    const double* v0 = &S.mesh.vertices[2 * V[0]];
+    const double* v1 = &S.mesh.vertices[2 * V[1]];
+    const double* v2 = &S.mesh.vertices[2 * V[2]];
+    const double J00 = v1[0] - v0[0], J01 = v2[0] - v0[0], J10 = v1[1] - v0[1], J11 = v2[1] - v0[1];
+    const double det = J00 * J11 - J01 * J10;
+    const double dx = x[0] - v0[0], dy = x[1] - v0[1];
+    xi[0] = (J11 * dx - J01 * dy) / det;
+    xi[1] = (-J10 * dx + J00 * dy) / det;
+}
+
+bool inside(int d, const double* xi, double tol)
+{
+    if (d == 1) return xi[0] >= -tol && xi[0] <= 1.0 + tol;
+    return xi[0] >= -tol && xi[1] >= -tol && xi[0] + xi[1] <= 1.0 + tol;
+}
+
+struct Locator {
+    const Space* S;
+    double lo[2], hi[2];
+    int nb[2];
+    std::vector<std::vector<int>> buckets;
+    explicit Locator(const Space& sp) : S(&sp)
+    {
+        const int d = sp.dim, nv = sp.mesh.nv(), nc = sp.mesh.ns();
+        for (int q = 0; q < 2; ++q) { lo[q] = 0.0; hi[q] = 1.0; nb[q] = 1; }
+        for (int q = 0; q < d; ++q) {
+            lo[q] = 1e300;
+            hi[q] = -1e300;
+            for (int v = 0; v < nv; ++v) {
+                lo[q] = std::min(lo[q], sp.mesh.vertices[v * d + q]);
+                hi[q] = std::max(hi[q], sp.mesh.vertices[v * d + q]);
+            }
+            nb[q] = std::max(1, static_cast<int>(std::pow(static_cast<double>(nc), 1.0 / d)));
+        }
+        buckets.assign(static_cast<size_t>(nb[0]) * nb[1], {});
+        for (int c = 0; c < nc; ++c) {
+            double bl[2] = {1e300, 1e300}, bh[2] = {-1e300, -1e300};
+            for (int v = 0; v <= d; ++v)
+                for (int q = 0; q < d; ++q) {
+                    const double x = sp.mesh.vertices[sp.mesh.cells[c * (d + 1) + v] * d + q];
+                    bl[q] = std::min(bl[q], x);
+                    bh[q] = std::max(bh[q], x);
+                }
+            int i0[2] = {0, 0}, i1[2] = {0, 0};
+            for (int q = 0; q < d; ++q) {
+                i0[q] = bucket(q, bl[q] - 1e-12);
+                i1[q] = bucket(q, bh[q] + 1e-12);
+            }
+            for (int a = i0[0]; a <= i1[0]; ++a)
+                for (int b = i0[1]; b <= i1[1]; ++b) buckets[static_cast<size_t>(b) * nb[0] + a].push_back(c);
+        }
+    }
+    int bucket(int q, double x) const
+    {
+        const double w = (hi[q] - lo[q]) > 0 ? (hi[q] - lo[q]) : 1.0;
+        int i = static_cast<int>(std::floor((x - lo[q]) / w * nb[q]));
+        return std::min(std::max(i, 0), nb[q] - 1);
+    }
+    int find(const double* x) const
+    {
+        const int d = S->dim;
+        const int a = bucket(0, x[0]);
+        const int b = (d == 2) ? bucket(1, x[1]) : 0;
+        for (int c : buckets[static_cast<size_t>(b) * nb[0] + a]) {
+            double xi[2];
+            to_reference(*S, c, x, xi);
+            if (inside(d, xi, 1e-10)) return c;
+        }
+        return -1;
+    }
+};
+
+void eval_basis(int d, int p, const double* xi, double* v)
+{
+    if (d == 1) segment_lagrange(p, xi[0], v, nullptr);
+    else triangle_lagrange(p, xi[0], xi[1], v, nullptr, nullptr);
+}
+
+CSR transpose(const CSR& A)
+{
+    CSR T;
+    T.nrows = A.ncols;
+    T.ncols = A.nrows;
+    T.ptr.assign(T.nrows + 1, 0);
+    for (int c : A.col) T.ptr[c + 1]++;
+    for (int i = 0; i < T.nrows; ++i) T.ptr[i + 1] += T.ptr[i];
+    T.col.resize(A.col.size());
+    T.val.resize(A.val.size());
+    std::vector<int> pos(T.ptr.begin(), T.ptr.end() - 1);
+    for (int r = 0; r < A.nrows; ++r)
+        for (int e = A.ptr[r]; e < A.ptr[r + 1]; ++e) {
+            const int q = pos[A.col[e]]++;
+            T.col[q] = r;
+            T.val[q] = A.val[e];
+        }
+    return T;
+}
+
+// C = R A P restricted to the coarse adjacency pattern (sptr, scol) of the
+// coarse space; the Galerkin product of nested Lagrange spaces lives there.
+std::vector<double> galerkin(const CSR& R, const std::vector<int>& fptr, const std::vector<int>& fcol,
+                             const std::vector<double>& A, const CSR& P, const std::vector<int>& cptr,
+                             const std::vector<int>& ccol)
+{
+    const int nc = R.nrows;
+    std::vector<double> out(ccol.size(), 0.0);
+    std::vector<double> acc(nc, 0.0);
+    std::vector<int> mark(nc, -1), touched;
+    for (int i = 0; i < nc; ++i) {
+        touched.clear();
+        for (int e = R.ptr[i]; e < R.ptr[i + 1]; ++e) {
+            const int r = R.col[e];
+            const double wr = R.val[e];
+            for (int f = fptr[r]; f < fptr[r + 1]; ++f) {
+                const int k = fcol[f];
+                const double a = wr * A[f];
+                for (int g = P.ptr[k]; g < P.ptr[k + 1]; ++g) {
+                    const int j = P.col[g];
+                    if (mark[j] != i) { mark[j] = i; acc[j] = 0.0; touched.push_back(j); }
+                    acc[j] += a * P.val[g];
+                }
+            }
+        }
+        for (int e = cptr[i]; e < cptr[i + 1]; ++e) out[e] = (mark[ccol[e]] == i) ? acc[ccol[e]] : 0.0;
+        for (int j : touched) {
+            bool found = std::binary_search(ccol.begin() + cptr[i], ccol.begin() + cptr[i + 1], j);
+            if (!found && std::fabs(acc[j]) > 1e-12)
+                throw std::runtime_error("setup_mg: Galerkin product leaves the coarse pattern (meshes not nested?)");
+        }
+    }
+    return out;
+}
+
+double lmax_bound(int n, const std::vector<int>& ptr, const std::vector<int>& col, const std::vector<double>& A,
+                  const std::vector<double>& d)
+{
+    // Gershgorin bound and a power-iteration estimate; the smaller one with a margin
+    double gers = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) s += std::fabs(A[e]);
+        gers = std::max(gers, s / d[i]);
+    }
+    std::vector<double> x(n), y(n);
+    std::mt19937_64 rng(12345);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (auto& v : x) v = U(rng);
+    double lam = 0.0;
+    for (int it = 0; it < 60; ++it) {
+        double nrm = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int e = ptr[i]; e < ptr[i + 1]; ++e) s += A[e] * x[col[e]];
+            y[i] = s / d[i];
+            nrm += y[i] * y[i];
+        }
+        nrm = std::sqrt(nrm);
+        double xn = 0.0;
+        for (int i = 0; i < n; ++i) xn += x[i] * x[i];
+        lam = nrm / std::sqrt(xn);
+        for (int i = 0; i < n; ++i) x[i] = y[i] / nrm;
+    }
+    return std::min(gers, 1.2 * lam);
+}
+
+}  // namespace
+
+void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V)
+{
+    V.assign(static_cast<size_t>(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; ++i) {
+            diag += A[i * n + i] * A[i * n + i];
+            for (int j = i + 1; j < n; ++j) off += A[i * n + j] * A[i * n + j];
+        }
+        if (off <= 1e-30 * diag || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double app = A[p * n + p], aqq = A[q * n + q];
+                const double tau = (aqq - app) / (2.0 * apq);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - s * akq;
+                    A[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - s * aqk;
+                    A[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - s * vkq;
+                    V[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; ++i) w[i] = A[i * n + i];
+}
+
+void build_mg_prolongations(const Space& fine, MGHierarchyHost& H)
+{
+    H.P.clear();
+    const int L = static_cast<int>(H.coarse.size()) + 1;
+    for (int l = 0; l + 1 < L; ++l) {
+        const Space& F = (l == 0) ? fine : H.coarse[l - 1];
+        const Space& Cs = H.coarse[l];
+        if (F.dim != Cs.dim || F.p != Cs.p) throw std::runtime_error("mg: coarse space dim/p mismatch");
+        if (Cs.nf >= F.nf) throw std::runtime_error("mg: coarse meshes must be ordered fine to coarse");
+        const int d = F.dim, Ns = F.Ns;
+        Locator loc(Cs);
+        std::vector<int> parent(F.mesh.ns(), -1);
+        for (int c = 0; c < F.mesh.ns(); ++c) {
+            double cen[2] = {0.0, 0.0};
+            for (int v = 0; v <= d; ++v)
+                for (int q = 0; q < d; ++q) cen[q] += F.mesh.vertices[F.mesh.cells[c * (d + 1) + v] * d + q] / (d + 1);
+            parent[c] = loc.find(cen);
+            if (parent[c] < 0) throw std::runtime_error("mg: fine cell " + std::to_string(c) + " has no coarse parent");
+        }
+        CSR P;
+        P.nrows = F.nf;
+        P.ncols = Cs.nf;
+        std::vector<int> owner(F.nf, -1), owner_loc(F.nf, -1);
+        for (int c = 0; c < F.mesh.ns(); ++c)
+            for (int i = 0; i < Ns; ++i) {
+                const int f = F.cell_free[c * Ns + i];
+                if (f >= 0 && owner[f] < 0) { owner[f] = c; owner_loc[f] = i; }
+            }
+        std::vector<std::vector<std::pair<int, double>>> rows(F.nf);
+        std::vector<double> phi(Cs.Ns);
+        for (int f = 0; f < F.nf; ++f) {
+            double x[2], xi[2];
+            node_coords(F, owner[f], owner_loc[f], x);
+            const int cc = parent[owner[f]];
+            to_reference(Cs, cc, x, xi);
+            if (!inside(d, xi, 1e-9)) throw std::runtime_error("mg: fine node outside its parent cell (not nested)");
+            eval_basis(d, F.p, xi, phi.data());
+            for (int j = 0; j < Cs.Ns; ++j) {
+                const int cf = Cs.cell_free[cc * Cs.Ns + j];
+                if (cf >= 0 && std::fabs(phi[j]) > 1e-13) rows[f].push_back({cf, phi[j]});
+            }
+            std::sort(rows[f].begin(), rows[f].end());
+        }
+        P.ptr.assign(F.nf + 1, 0);
+        for (int f = 0; f < F.nf; ++f) P.ptr[f + 1] = P.ptr[f] + static_cast<int>(rows[f].size());
+        for (int f = 0; f < F.nf; ++f)
+            for (auto& e : rows[f]) { P.col.push_back(e.first); P.val.push_back(e.second); }
+        H.P.push_back(std::move(P));
+    }
+}
+
+void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double>& Ksp,
+              const std::vector<double>& Msp, const std::vector<double>& Mt, const std::vector<double>& Dt,
+              MGSetupHost& out)
+{
+    const int L = static_cast<int>(H.coarse.size()) + 1;
+    out.levels.assign(L, MGLevelHost());
+    for (int l = 0; l < L; ++l) {
+        MGLevelHost& lv = out.levels[l];
+        const Space& sp = (l == 0) ? S : H.coarse[l - 1];
+        lv.nf = sp.nf;
+        lv.sptr = sp.sptr;
+        lv.scol = sp.scol;
+        if (l == 0) {
+            lv.K = Ksp;
+            lv.M = Msp;
+        } else {
+            const MGLevelHost& f = out.levels[l - 1];
+            const CSR& P = H.P[l - 1];
+            const CSR R = transpose(P);
+            lv.K = galerkin(R, f.sptr, f.scol, f.K, P, lv.sptr, lv.scol);
+            lv.M = galerkin(R, f.sptr, f.scol, f.M, P, lv.sptr, lv.scol);
+        }
+        lv.dK.assign(lv.nf, 0.0);
+        lv.dM.assign(lv.nf, 0.0);
+        for (int i = 0; i < lv.nf; ++i)
+            for (int e = lv.sptr[i]; e < lv.sptr[i + 1]; ++e)
+                if (lv.scol[e] == i) { lv.dK[i] = lv.K[e]; lv.dM[i] = lv.M[e]; }
+        for (int i = 0; i < lv.nf; ++i)
+            if (!(lv.dK[i] > 0.0) || !(lv.dM[i] > 0.0)) throw std::runtime_error("setup_mg: non-positive diagonal");
+        lv.lmax = std::max(lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK),
+                           lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM));
+        if (l + 1 < L) {
+            lv.P = H.P[l];
+            lv.R = transpose(H.P[l]);
+        }
+    }
+    // time factors
+    const int p = S.p, Nt = S.Nt, m = S.mesh.m();
+    const int T = m * p + 1;
+    out.T = T;
+    std::vector<double> T1(static_cast<size_t>(T) * T, 0.0), T3(static_cast<size_t>(T) * T, 0.0);
+    std::vector<double> Mti(Mt);
+    invert_dense(Nt, Mti);
+    std::vector<double> DMD(static_cast<size_t>(Nt) * Nt, 0.0);  // Dt Mt^-1 Dt^T
+    for (int i = 0; i < Nt; ++i)
+        for (int j = 0; j < Nt; ++j) {
+            double s = 0.0;
+            for (int a = 0; a < Nt; ++a)
+                for (int b = 0; b < Nt; ++b) s += Dt[i * Nt + a] * Mti[a * Nt + b] * Dt[j * Nt + b];
+            DMD[i * Nt + j] = s;
+        }
+    out.MtInvH.assign(static_cast<size_t>(m) * Nt * Nt, 0.0);
+    for (int k = 0; k < m; ++k) {
+        const double h = S.mesh.times[k + 1] - S.mesh.times[k];
+        for (int i = 0; i < Nt; ++i)
+            for (int j = 0; j < Nt; ++j) {
+                T3[static_cast<size_t>(k * p + i) * T + (k * p + j)] += h * Mt[i * Nt + j];
+                T1[static_cast<size_t>(k * p + i) * T + (k * p + j)] += DMD[i * Nt + j] / h;
+                out.MtInvH[(static_cast<size_t>(k) * Nt + i) * Nt + j] = Mti[i * Nt + j] / h;
+            }
+    }
+    // generalised eigenproblem T1 w = theta T3 w:  T3 = L L^T, C = L^-1 T1 L^-T
+    std::vector<double> Lc(static_cast<size_t>(T) * T, 0.0);
+    for (int j = 0; j < T; ++j) {
+        double s = T3[static_cast<size_t>(j) * T + j];
+        for (int k = 0; k < j; ++k) s -= Lc[static_cast<size_t>(j) * T + k] * Lc[static_cast<size_t>(j) * T + k];
+        if (!(s > 0.0)) throw std::runtime_error("setup_mg: time mass not SPD");
+        Lc[static_cast<size_t>(j) * T + j] = std::sqrt(s);
+        for (int i = j + 1; i < T; ++i) {
+            double t = T3[static_cast<size_t>(i) * T + j];
+            for (int k = 0; k < j; ++k) t -= Lc[static_cast<size_t>(i) * T + k] * Lc[static_cast<size_t>(j) * T + k];
+            Lc[static_cast<size_t>(i) * T + j] = t / Lc[static_cast<size_t>(j) * T + j];
+        }
+    }
+    std::vector<double> Li(Lc);
+    invert_dense(T, Li);
+    std::vector<double> C(static_cast<size_t>(T) * T, 0.0), tmp(static_cast<size_t>(T) * T, 0.0);
+    for (int i = 0; i < T; ++i)
+        for (int j = 0; j < T; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < T; ++k) s += Li[static_cast<size_t>(i) * T + k] * T1[static_cast<size_t>(k) * T + j];
+            tmp[static_cast<size_t>(i) * T + j] = s;
+        }
+    for (int i = 0; i < T; ++i)
+        for (int j = 0; j < T; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < T; ++k) s += tmp[static_cast<size_t>(i) * T + k] * Li[static_cast<size_t>(j) * T + k];
+            C[static_cast<size_t>(i) * T + j] = s;
+        }
+    for (int i = 0; i < T; ++i)
+        for (int j = i + 1; j < T; ++j) {
+            const double a = 0.5 * (C[static_cast<size_t>(i) * T + j] + C[static_cast<size_t>(j) * T + i]);
+            C[static_cast<size_t>(i) * T + j] = C[static_cast<size_t>(j) * T + i] = a;
+        }
+    std::vector<double> theta, Q;
+    sym_eig(T, C, theta, Q);
+    out.W.assign(static_cast<size_t>(T) * T, 0.0);  // W = L^-T Q
+    for (int i = 0; i < T; ++i)
+        for (int j = 0; j < T; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < T; ++k) s += Li[static_cast<size_t>(k) * T + i] * Q[static_cast<size_t>(k) * T + j];
+            out.W[static_cast<size_t>(i) * T + j] = s;
+        }
+    out.shift.resize(T);
+    for (int i = 0; i < T; ++i) out.shift[i] = std::sqrt(std::max(theta[i], 0.0));
+    // coarsest inverses for shift 0 and each time-mode shift
+    const MGLevelHost& cl = out.levels.back();
+    const int nc = cl.nf;
+    if (nc > 4096) throw std::runtime_error("setup_mg: coarsest level too large (" + std::to_string(nc) +
+                                             " nodes); pass coarser nested meshes");
+    out.shifts_all.clear();
+    out.shifts_all.push_back(0.0);
+    for (int i = 0; i < T; ++i) out.shifts_all.push_back(out.shift[i]);
+    const int nsft = static_cast<int>(out.shifts_all.size());
+    out.coarse_inv.assign(static_cast<size_t>(nsft) * nc * nc, 0.0);
+    for (int q = 0; q < nsft; ++q) {
+        std::vector<double> A(static_cast<size_t>(nc) * nc, 0.0);
+        for (int i = 0; i < nc; ++i)
+            for (int e = cl.sptr[i]; e < cl.sptr[i + 1]; ++e)
+                A[static_cast<size_t>(i) * nc + cl.scol[e]] = cl.K[e] + out.shifts_all[q] * cl.M[e];
+        if (!invert_dense(nc, A)) throw std::runtime_error("setup_mg: singular coarsest operator");
+        std::copy(A.begin(), A.end(), out.coarse_inv.begin() + static_cast<size_t>(q) * nc * nc);
+    }
+}
+
+}  // namespace apf

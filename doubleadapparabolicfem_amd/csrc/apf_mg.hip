@@ -1,0 +1,369 @@
+// Device side of the space-time multigrid preconditioner (see apf_mg.hpp).
+//
+// All spatial work is multi-column: a block Z[nf][ld] holds one column per
+// time dof (Y block, shift 0) or per time eigenmode (X block, shift s_i), so
+// every sparse product streams the spatial matrix once for all columns.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "apf_internal.hpp"
+#include "apf_mg.hpp"
+#include "apf_mg_dev.hpp"
+
+namespace apf {
+
+namespace {
+
+// Y = (K + s_c M) X   (mode 0)   or   Y = B - (K + s_c M) X   (mode 1), columns [0, nc)
+template <typename R>
+__global__ void __launch_bounds__(256) k_mg_apply(int nf, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                  const R* __restrict__ K, const R* __restrict__ M,
+                                                  const R* __restrict__ sig, int nc, int ld, const R* __restrict__ X,
+                                                  const R* __restrict__ B, R* __restrict__ Y, int mode)
+{
+    const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (row >= nf) return;
+    const int b = ptr[row], e = ptr[row + 1];
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+        const int c = c0 + lane;
+        if (c >= nc) break;
+        const R s = sig[c];
+        R acc = 0;
+        for (int q = b; q < e; ++q) {
+            const int j = col[q];
+            acc += (K[q] + s * M[q]) * X[static_cast<int64_t>(j) * ld + c];
+        }
+        const int64_t o = static_cast<int64_t>(row) * ld + c;
+        Y[o] = mode ? B[o] - acc : acc;
+    }
+}
+
+// Chebyshev start: r = B - A x (given as r), d = D^-1 r / theta, x = x0 (+ nothing)
+template <typename R>
+__global__ void k_cheb_start(int nf, int nc, int ld, const R* __restrict__ dK, const R* __restrict__ dM,
+                             const R* __restrict__ sig, R inv_theta, const R* __restrict__ r, R* __restrict__ d)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<int64_t>(nf) * nc) return;
+    const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
+    const int64_t o = static_cast<int64_t>(i) * ld + c;
+    d[o] = r[o] * inv_theta / (dK[i] + sig[c] * dM[i]);
+}
+
+// x += d ; r -= Ad ; d = c1 d + c2 D^-1 r
+template <typename R>
+__global__ void k_cheb_step(int nf, int nc, int ld, const R* __restrict__ dK, const R* __restrict__ dM,
+                            const R* __restrict__ sig, R c1, R c2, R* __restrict__ x, R* __restrict__ r,
+                            R* __restrict__ d, const R* __restrict__ Ad, int last)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<int64_t>(nf) * nc) return;
+    const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
+    const int64_t o = static_cast<int64_t>(i) * ld + c;
+    const R dv = d[o];
+    x[o] += dv;
+    const R rv = r[o] - Ad[o];
+    r[o] = rv;
+    if (!last) d[o] = c1 * dv + c2 * rv / (dK[i] + sig[c] * dM[i]);
+}
+
+// Y = A X for a plain CSR (prolongation / restriction), optionally accumulated
+template <typename R>
+__global__ void __launch_bounds__(256) k_csr_mm(int nrows, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                const R* __restrict__ val, int nc, int ldx, const R* __restrict__ X,
+                                                int ldy, R* __restrict__ Y, int accumulate)
+{
+    const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (row >= nrows) return;
+    const int b = ptr[row], e = ptr[row + 1];
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+        const int c = c0 + lane;
+        if (c >= nc) break;
+        R acc = 0;
+        for (int q = b; q < e; ++q) acc += val[q] * X[static_cast<int64_t>(col[q]) * ldx + c];
+        const int64_t o = static_cast<int64_t>(row) * ldy + c;
+        Y[o] = accumulate ? Y[o] + acc : acc;
+    }
+}
+
+// coarsest level: X[:,c] = inv[shift_idx[c]] B[:,c]
+template <typename R>
+__global__ void k_coarse_solve(int n, int nc, int ld, const R* __restrict__ inv, const int* __restrict__ sidx,
+                               const R* __restrict__ B, R* __restrict__ X)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<int64_t>(n) * nc) return;
+    const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
+    const R* A = inv + static_cast<int64_t>(sidx[c]) * n * n + static_cast<int64_t>(i) * n;
+    R acc = 0;
+    for (int j = 0; j < n; ++j) acc += A[j] * B[static_cast<int64_t>(j) * ld + c];
+    X[static_cast<int64_t>(i) * ld + c] = acc;
+}
+
+// gather: Z[s][i] = sum_tn r_X[tn nf + s] W[tn][i]  (i < T)  and  Z[s][T + c] = r_Y[c nf + s]
+template <typename R>
+__global__ void k_mg_gather(int nf, int T, int nyc, int ld, int64_t nY, const double* __restrict__ r,
+                            const double* __restrict__ W, R* __restrict__ Z)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int ncol = T + nyc;
+    if (t >= static_cast<int64_t>(nf) * ncol) return;
+    const int c = static_cast<int>(t % ncol), s = static_cast<int>(t / ncol);
+    double v;
+    if (c < T) {
+        v = 0.0;
+        for (int tn = 0; tn < T; ++tn) v += r[nY + static_cast<int64_t>(tn) * nf + s] * W[tn * T + c];
+    } else {
+        v = r[static_cast<int64_t>(c - T) * nf + s];
+    }
+    Z[static_cast<int64_t>(s) * ld + c] = static_cast<R>(v);
+}
+
+// scatter back: z_X[tn nf + s] = sum_i Z[s][i] W[tn][i] ; z_Y[(k Nt + l) nf + s] = sum_l' Minv_k[l][l'] Z[s][T + k Nt + l']
+// also accumulates dot(z, r) partials
+template <typename R>
+__global__ void k_mg_scatter(int nf, int T, int Nt, int m, int ld, int64_t nY, const R* __restrict__ Z,
+                             const double* __restrict__ W, const double* __restrict__ MtInvH,
+                             const double* __restrict__ r, double* __restrict__ z, double* __restrict__ part)
+{
+    const int64_t n = nY + static_cast<int64_t>(T) * nf;
+    double dot = 0.0;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double v = 0.0;
+        if (t < nY) {
+            const int s = static_cast<int>(t % nf);
+            const int kl = static_cast<int>(t / nf);
+            const int k = kl / Nt, l = kl % Nt;
+            const R* zr = Z + static_cast<int64_t>(s) * ld + T + k * Nt;
+            const double* Mi = MtInvH + (static_cast<int64_t>(k) * Nt + l) * Nt;
+            for (int q = 0; q < Nt; ++q) v += Mi[q] * static_cast<double>(zr[q]);
+        } else {
+            const int64_t x = t - nY;
+            const int tn = static_cast<int>(x / nf), s = static_cast<int>(x % nf);
+            const R* zr = Z + static_cast<int64_t>(s) * ld;
+            for (int i = 0; i < T; ++i) v += static_cast<double>(zr[i]) * W[tn * T + i];
+        }
+        z[t] = v;
+        dot += v * r[t];
+    }
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < static_cast<int>(blockDim.x / 64); ++w) s += sh[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+template <typename R>
+R* mg_alloc(System* sys, size_t n)
+{
+    return dalloc<R>(sys, n);
+}
+
+template <typename R>
+std::vector<R> conv(const std::vector<double>& v)
+{
+    return std::vector<R>(v.begin(), v.end());
+}
+
+template <typename T>
+void up(T* dst, const std::vector<T>& src)
+{
+    if (!src.empty()) HIP_TRY(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+inline unsigned grid_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
+
+}  // namespace
+
+template <typename R>
+void MGDevice<R>::build(System* sys, const MGSetupHost& H, int nyc_)
+{
+    if (ready) {
+        refresh(H);
+        return;
+    }
+    const int L = static_cast<int>(H.levels.size());
+    T = H.T;
+    nyc = nyc_;
+    ncol = T + nyc;
+    ld = ncol;
+    levels.resize(L);
+    for (int l = 0; l < L; ++l) {
+        const MGLevelHost& h = H.levels[l];
+        Level& d = levels[l];
+        d.nf = h.nf;
+        d.lmax = h.lmax;
+        d.ptr = mg_alloc<int>(sys, h.sptr.size());
+        up(d.ptr, h.sptr);
+        d.col = mg_alloc<int>(sys, h.scol.size());
+        up(d.col, h.scol);
+        d.K = mg_alloc<R>(sys, h.K.size());
+        up(d.K, conv<R>(h.K));
+        d.M = mg_alloc<R>(sys, h.M.size());
+        up(d.M, conv<R>(h.M));
+        d.dK = mg_alloc<R>(sys, h.nf);
+        up(d.dK, conv<R>(h.dK));
+        d.dM = mg_alloc<R>(sys, h.nf);
+        up(d.dM, conv<R>(h.dM));
+        const size_t blk = static_cast<size_t>(h.nf) * ld;
+        d.X = mg_alloc<R>(sys, blk);
+        d.B = mg_alloc<R>(sys, blk);
+        d.Rr = mg_alloc<R>(sys, blk);
+        d.D = mg_alloc<R>(sys, blk);
+        d.Tm = mg_alloc<R>(sys, blk);
+        if (l + 1 < L) {
+            d.Pn = h.P.nrows;
+            d.Pptr = mg_alloc<int>(sys, h.P.ptr.size());
+            up(d.Pptr, h.P.ptr);
+            d.Pcol = mg_alloc<int>(sys, h.P.col.size());
+            up(d.Pcol, h.P.col);
+            d.Pval = mg_alloc<R>(sys, h.P.val.size());
+            up(d.Pval, conv<R>(h.P.val));
+            d.Rn = h.R.nrows;
+            d.Rptr = mg_alloc<int>(sys, h.R.ptr.size());
+            up(d.Rptr, h.R.ptr);
+            d.Rcol = mg_alloc<int>(sys, h.R.col.size());
+            up(d.Rcol, h.R.col);
+            d.Rval = mg_alloc<R>(sys, h.R.val.size());
+            up(d.Rval, conv<R>(h.R.val));
+        }
+    }
+    // per-column shifts: X columns carry sqrt(theta_i), Y columns 0
+    std::vector<R> sig(ncol, 0), zero(ncol, 0);
+    std::vector<int> sidx(ncol, 0), zidx(ncol, 0);
+    for (int i = 0; i < T; ++i) {
+        sig[i] = static_cast<R>(H.shift[i]);
+        sidx[i] = 1 + i;
+    }
+    d_sig = mg_alloc<R>(sys, ncol);
+    up(d_sig, sig);
+    d_zero = mg_alloc<R>(sys, ncol);
+    up(d_zero, zero);
+    d_sidx = mg_alloc<int>(sys, ncol);
+    up(d_sidx, sidx);
+    d_zidx = mg_alloc<int>(sys, ncol);
+    up(d_zidx, zidx);
+    nc_coarse = H.levels.back().nf;
+    d_cinv = mg_alloc<R>(sys, H.coarse_inv.size());
+    up(d_cinv, conv<R>(H.coarse_inv));
+    d_W = mg_alloc<double>(sys, H.W.size());
+    up(d_W, H.W);
+    d_MtInvH = mg_alloc<double>(sys, H.MtInvH.size());
+    up(d_MtInvH, H.MtInvH);
+    d_Z = mg_alloc<R>(sys, static_cast<size_t>(levels[0].nf) * ld);
+    d_Z2 = mg_alloc<R>(sys, static_cast<size_t>(levels[0].nf) * ld);
+    ready = true;
+}
+
+template <typename R>
+void MGDevice<R>::refresh(const MGSetupHost& H)
+{
+    for (size_t l = 0; l < levels.size(); ++l) {
+        const MGLevelHost& h = H.levels[l];
+        Level& d = levels[l];
+        d.lmax = h.lmax;
+        up(d.K, conv<R>(h.K));
+        up(d.M, conv<R>(h.M));
+        up(d.dK, conv<R>(h.dK));
+        up(d.dM, conv<R>(h.dM));
+    }
+    std::vector<R> sig(ncol, 0);
+    for (int i = 0; i < T; ++i) sig[i] = static_cast<R>(H.shift[i]);
+    up(d_sig, sig);
+    up(d_cinv, conv<R>(H.coarse_inv));
+    up(d_W, H.W);
+    up(d_MtInvH, H.MtInvH);
+}
+
+template <typename R>
+void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_start)
+{
+    Level& d = levels[l];
+    const R lmax = static_cast<R>(d.lmax);
+    const R lmin = lmax / static_cast<R>(cheb_ratio);
+    const R theta = (lmax + lmin) / 2, delta = (lmax - lmin) / 2;
+    const R sigma1 = theta / delta;
+    const int64_t nel = static_cast<int64_t>(d.nf) * nc;
+    const int blk = 256;
+    const unsigned rows_grid = grid_for(d.nf, 4);
+    if (zero_start) {
+        // x = 0, r = B
+        HIP_TRY(hipMemsetAsync(d.X, 0, static_cast<size_t>(d.nf) * ld * sizeof(R), st));
+        HIP_TRY(hipMemcpyAsync(d.Rr, d.B, static_cast<size_t>(d.nf) * ld * sizeof(R), hipMemcpyDeviceToDevice, st));
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col, d.K,
+                           d.M, sig, nc, ld, d.X, d.B, d.Rr, 1);
+    }
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc, ld, d.dK,
+                       d.dM, sig, static_cast<R>(1) / theta, d.Rr, d.D);
+    R rho = 1 / sigma1;
+    for (int k = 0; k < nu; ++k) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col, d.K,
+                           d.M, sig, nc, ld, d.D, d.D, d.Tm, 0);
+        const R rho_n = 1 / (2 * sigma1 - rho);
+        const R c1 = rho_n * rho, c2 = 2 * rho_n / delta;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_step<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc, ld,
+                           d.dK, d.dM, sig, c1, c2, d.X, d.Rr, d.D, d.Tm, (k == nu - 1) ? 1 : 0);
+        rho = rho_n;
+    }
+}
+
+// V-cycle on level l: B_l -> X_l (columns [0, nc))
+template <typename R>
+void MGDevice<R>::vcycle(hipStream_t st, int l, int nc, const R* sig, const int* sidx)
+{
+    Level& d = levels[l];
+    const int L = static_cast<int>(levels.size());
+    if (l == L - 1) {
+        const int64_t nel = static_cast<int64_t>(d.nf) * nc;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_coarse_solve<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, d.nf, nc, ld,
+                           d_cinv, sidx, d.B, d.X);
+        return;
+    }
+    Level& c = levels[l + 1];
+    smooth(st, l, nc, sig, true);  // leaves r = B - A x in Rr
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R>), dim3(grid_for(d.Rn, 4)), dim3(256), 0, st, d.Rn, d.Rptr, d.Rcol,
+                       d.Rval, nc, ld, d.Rr, ld, c.B, 0);
+    vcycle(st, l + 1, nc, sig, sidx);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R>), dim3(grid_for(d.Pn, 4)), dim3(256), 0, st, d.Pn, d.Pptr, d.Pcol,
+                       d.Pval, nc, ld, c.X, ld, d.X, 1);
+    smooth(st, l, nc, sig, false);
+}
+
+// z = P^-1 r for the saddle-point vector r = [r_Y; r_X]; writes dot(z, r) partials.
+template <typename R>
+void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts)
+{
+    Level& f = levels[0];
+    const int nf = f.nf;
+    const int64_t nel = static_cast<int64_t>(nf) * ncol;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, T, nyc, ld,
+                       S.nY, r, d_W, f.B);
+    // stage 1: all columns, Y columns with shift 0, X columns with shift s_i
+    vcycle(st, 0, ncol, d_sig, d_sidx);
+    // stage 2 (X columns): apply K, then the shifted V-cycle again
+    HIP_TRY(hipMemcpyAsync(d_Z, f.X, static_cast<size_t>(nf) * ld * sizeof(R), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(grid_for(nf, 4)), dim3(256), 0, st, nf, f.ptr, f.col, f.K,
+                       f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
+    vcycle(st, 0, T, d_sig, d_sidx);
+    // X columns from the second V-cycle, Y columns from the first
+    HIP_TRY(hipMemcpy2DAsync(d_Z, ld * sizeof(R), f.X, ld * sizeof(R), T * sizeof(R), nf, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R>), dim3(nparts), dim3(256), 0, st, nf, T, S.Nt, S.mesh.m(), ld,
+                       S.nY, d_Z, d_W, d_MtInvH, r, z, part);
+}
+
+template struct MGDevice<double>;
+template struct MGDevice<float>;
+
+}  // namespace apf

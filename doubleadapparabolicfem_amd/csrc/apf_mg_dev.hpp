@@ -1,0 +1,61 @@
+// Device state of the space-time multigrid preconditioner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "apf_mg.hpp"
+
+namespace apf {
+
+struct System;
+
+template <typename R>
+struct MGDevice {
+    struct Level {
+        int nf = 0;
+        double lmax = 1.0;
+        int* ptr = nullptr;
+        int* col = nullptr;
+        R* K = nullptr;
+        R* M = nullptr;
+        R* dK = nullptr;
+        R* dM = nullptr;
+        R* X = nullptr;   // solution block
+        R* B = nullptr;   // right-hand side block
+        R* Rr = nullptr;  // residual
+        R* D = nullptr;   // Chebyshev direction
+        R* Tm = nullptr;  // A D
+        int Pn = 0, Rn = 0;
+        int* Pptr = nullptr;
+        int* Pcol = nullptr;
+        R* Pval = nullptr;
+        int* Rptr = nullptr;
+        int* Rcol = nullptr;
+        R* Rval = nullptr;
+    };
+    std::vector<Level> levels;
+    int T = 0, nyc = 0, ncol = 0, ld = 0;
+    int nu = 2;              // Chebyshev degree (pre and post)
+    double cheb_ratio = 4.0; // smoothing interval [lmax / ratio, lmax]
+    R* d_sig = nullptr;
+    R* d_zero = nullptr;
+    int* d_sidx = nullptr;
+    int* d_zidx = nullptr;
+    int nc_coarse = 0;
+    R* d_cinv = nullptr;
+    double* d_W = nullptr;
+    double* d_MtInvH = nullptr;
+    R* d_Z = nullptr;
+    R* d_Z2 = nullptr;
+    bool ready = false;
+
+    void build(System* sys, const MGSetupHost& H, int nyc);
+    void refresh(const MGSetupHost& H);
+    void smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_start);
+    void vcycle(hipStream_t st, int l, int nc, const R* sig, const int* sidx);
+    void apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts);
+};
+
+}  // namespace apf

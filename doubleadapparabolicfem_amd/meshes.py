@@ -68,6 +68,26 @@ def uniform_mesh(dim: int, outer: int) -> Mesh:
     return union_jack_mesh(n, np.linspace(0.0, 1.0, 2 ** outer + 1))
 
 
+def uniform_hierarchy(dim: int, outer: int):
+    """Nested coarser spatial meshes of uniform_mesh(dim, outer), fine to coarse
+    (for the multigrid preconditioner): halve the resolution down to 2 cells
+    per direction."""
+    out = []
+    if dim == 1:
+        n = 2 ** (outer + 1) // 2
+        while n >= 2:
+            out.append(interval_mesh(n, [0.0, 1.0]))
+            n //= 2
+        return out
+    if outer == 0:
+        return out
+    n = 2 ** (outer + 1) // 2
+    while n >= 2:
+        out.append(union_jack_mesh(n, [0.0, 1.0]))
+        n //= 2
+    return out
+
+
 def smooth_initial(dim: int) -> Coefficients:
     """M = I, ell = 0, u0 = prod_i sin(pi x_i) (smooth_initial examples)."""
     if dim == 1:

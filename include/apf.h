@@ -60,6 +60,7 @@ enum apf_status {
 
 enum apf_precision { APF_FP64 = 0, APF_FP32_REFINE = 1 };
 enum apf_assembly_mode { APF_ASSEMBLE_KRON = 0, APF_ASSEMBLE_SCATTER = 1 };
+enum apf_precond_kind { APF_PRECOND_DIAGONAL = 0, APF_PRECOND_MULTIGRID = 1 };
 
 typedef struct apf_ctx apf_ctx;
 typedef struct apf_space apf_space;
@@ -126,6 +127,11 @@ int apf_space_cell_free(const apf_space* s, int* out);
 int apf_space_free_coords(const apf_space* s, double* out);
 /* CSR pattern of K: rowptr [n+1], col [nnz] */
 int apf_space_pattern(const apf_space* s, int64_t* rowptr, int* col);
+/* Append the next coarser nested spatial mesh (same dim; every cell of the
+ * current coarsest mesh must lie in one cell of the new mesh) for the
+ * multigrid preconditioner of apf_minres_solve.  Call fine-to-coarse before
+ * apf_system_create. */
+int apf_space_add_coarse_mesh(apf_space* s, const apf_mesh_desc* coarse);
 /* patch topology: per patch n_flux, n_scalar, n_cells, boundary flag */
 int apf_space_patch_sizes(const apf_space* s, int* n_flux, int* n_scalar, int* n_cells, int* boundary);
 const char* apf_last_error(const apf_ctx* ctx); /* ctx may be NULL: last error of this thread */
@@ -148,6 +154,17 @@ int apf_system_download(apf_system* sys, double* K_values, double* b);
  * residual <= tol, checked every check_interval iterations. */
 int apf_minres_solve(apf_system* sys, double tol, int max_it, int check_interval, int precond_prec,
                      const double* x0, double* x_out, apf_minres_stats* stats);
+/* Same with an explicit preconditioner: APF_PRECOND_DIAGONAL (time-exact,
+ * space-diagonal blocks) or APF_PRECOND_MULTIGRID (Kronecker/time-eigenmode
+ * blocks with spatial multigrid V-cycles; needs apf_space_add_coarse_mesh).
+ * apf_minres_solve picks the multigrid one when coarse meshes are present. */
+int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_interval, int precond_kind,
+                        int precond_prec, const double* x0, double* x_out, apf_minres_stats* stats);
+/* Average device time of `reps` back-to-back SpMVs y = K x (HIP events on the
+ * system stream), for the HBM roofline of the MINRES inner loop. */
+int apf_bench_spmv(apf_system* sys, int reps, double* ms_avg);
+/* Host seconds of the last multigrid setup (Galerkin operators, time eigenmodes). */
+int apf_system_mg_setup_seconds(apf_system* sys, double* sec);
 /* Upload a solution x = [lambda; u] [n] (e.g. from another solver). */
 int apf_system_set_solution(apf_system* sys, const double* x);
 
