@@ -110,6 +110,8 @@ EXPORTED = {
                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(_MinresStats)]),
     "apf_system_mg_setup_seconds": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "apf_system_use_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "apf_bench_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_bench_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -231,6 +233,12 @@ class Context:
         _check(lib().apf_ctx_create(ctypes.byref(h), device), None, "apf_ctx_create")
         self.handle = h
 
+    def bench_stream(self, nbytes=1 << 30, reps=10):
+        """Copy-kernel bandwidth in GB/s (read + write)."""
+        v = ctypes.c_double()
+        _check(lib().apf_bench_stream(self.handle, int(nbytes), reps, ctypes.byref(v)), self.handle)
+        return v.value
+
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
             _lib.apf_ctx_destroy(self.handle)
@@ -305,6 +313,9 @@ class System:
         res = {f: getattr(out, f) for f, _ in _EstimateOut._fields_}
         res.update(flux_by_cell=fl, resid_by_cell=rs, recon_by_cell=rc)
         return res
+
+    def use_graph(self, on=True):
+        _check(lib().apf_system_use_graph(self.handle, 1 if on else 0), self.ctx.handle)
 
     def bench_spmv(self, reps=20):
         v = ctypes.c_double()

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -223,20 +224,43 @@ __global__ void k_element_scatter(int dim, int ns, int Ns, int p, int m, int nf,
 }
 
 // ============================================================== SpMV
-template <int LANES>
+// CSR SpMV y = K x, LANES lanes per row, U independent (value, column, x)
+// load triples in flight per lane; values and column indices are streamed once
+// (non-temporal), x is gathered through the cache.
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8);
+// the remap gives every XCD one contiguous range of rows, so the x entries a
+// range gathers (a few spatial blocks of 8 nf bytes) stay in that XCD's L2.
+__device__ inline unsigned xcd_block(unsigned b, unsigned nblocks)
+{
+    const unsigned per = nblocks / 8u;  // nblocks is a multiple of 8
+    return (b % 8u) * per + b / 8u;
+}
+
+template <int LANES, int U>
 __global__ void __launch_bounds__(256) k_spmv(int64_t n, const int64_t* __restrict__ rowptr,
                                               const int* __restrict__ col, const double* __restrict__ val,
                                               const double* __restrict__ x, double* __restrict__ y)
 {
-    const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / LANES;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t row = (static_cast<int64_t>(lb) * blockDim.x + threadIdx.x) / LANES;
     const int lane = threadIdx.x % LANES;
     double sum = 0.0;
     if (row < n) {
-        const int64_t b = rowptr[row], e = rowptr[row + 1];
-        for (int64_t j = b + lane; j < e; j += LANES) {
-            const double a = __builtin_nontemporal_load(&val[j]);
-            const int cj = __builtin_nontemporal_load(&col[j]);
-            sum += a * x[cj];
+        const int64_t b = rowptr[row];
+        const int len = static_cast<int>(rowptr[row + 1] - b);
+        const double* vr = val + b;
+        const int* cr = col + b;
+        for (int j0 = 0; j0 < len; j0 += LANES * U) {
+            double a[U];
+            int c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = j0 + lane + u * LANES;
+                a[u] = (j < len) ? __builtin_nontemporal_load(vr + j) : 0.0;
+                c[u] = (j < len) ? __builtin_nontemporal_load(cr + j) : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) sum += a[u] * x[c[u]];
         }
     }
 #pragma unroll
@@ -534,12 +558,22 @@ static void upload(void* dst, const void* src, size_t bytes)
 void System::spmv(const double* x, double* y)
 {
     const int64_t n = space->n;
-    constexpr int L = 16;
-    const int64_t threads = n * L;
     const int block = 256;
-    const int64_t grid = (threads + block - 1) / block;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spmv<L>), dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, n,
-                       d_rowptr, d_col, d_val, x, y);
+    auto launch = [&](auto kern, int lanes) {
+        int64_t grid = (n * lanes + block - 1) / block;
+        grid = (grid + 7) / 8 * 8;  // whole XCD rounds for xcd_block
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, n, d_rowptr, d_col, d_val,
+                           x, y);
+    };
+    switch (spmv_lanes) {
+    case 8: launch(k_spmv<8, 4>, 8); break;
+    case 16: launch(k_spmv<16, 2>, 16); break;
+    case 17: launch(k_spmv<16, 4>, 16); break;
+    case 32: launch(k_spmv<32, 2>, 32); break;
+    case 33: launch(k_spmv<32, 1>, 32); break;
+    case 64: launch(k_spmv<64, 1>, 64); break;
+    default: launch(k_spmv<16, 4>, 16); break;
+    }
 }
 
 }  // namespace apf
@@ -755,6 +789,8 @@ int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
         sys->ctx = ctx;
         sys->space = &s->S;
         sys->tables = &s->T;
+        if (const char* e = std::getenv("APF_SPMV_LANES")) sys->spmv_lanes = std::atoi(e);
+        if (const char* e = std::getenv("APF_MINRES_GRAPH")) sys->use_graph = std::atoi(e) != 0;
         sys->H = s->H.coarse.empty() ? nullptr : &s->H;
         sys->stream = ctx->stream;
         const Space& S = s->S;
@@ -973,6 +1009,7 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
         const int nparts = sys->nparts;
         const unsigned vgrid = static_cast<unsigned>(nparts);
         const bool use_mg = precond_kind == APF_PRECOND_MULTIGRID;
+        const bool use_graph_ = sys->use_graph && max_it >= 3;
         const bool mg32 = precond_prec == APF_FP32_REFINE;
         int pparts = nparts;
         unsigned pgrid = 0;
@@ -999,6 +1036,8 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
                 ver = sys->assembly_version;
                 sys->t_mg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
             }
+            pparts = mg32 ? sys->mg32->n_partials(S.nf) : sys->mg64->n_partials(S.nf);
+            if (pparts > 4 * nparts) throw std::runtime_error("too many multigrid transpose tiles");
         } else {
             hipLaunchKernelGGL(k_precond_setup, dim3((S.nf + 127) / 128), dim3(128), 0, st, S.nf, S.p, S.mesh.m(),
                                sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_kdiag,
@@ -1059,26 +1098,54 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
         int it = 0;
         bool conv = rel <= tol;
         MinresScalars hs{};
+        // One MINRES iteration; the buffer rotations have period 3, so three
+        // iterations form a fixed launch sequence that is captured once in a
+        // hipGraph and replayed (no host synchronisation inside).
+        auto iteration = [&]() {
+            // t = K y ; v = y / beta ; t = t / beta - (beta / oldb) r1 ; alfa = v . t
+            sys->spmv(y, t);
+            hipLaunchKernelGGL(k_minres_vec1, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, y, t, r1, v,
+                               sys->d_part);
+            hipLaunchKernelGGL(k_minres_alfa, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, nparts);
+            // t -= alfa/beta r2 ; y = P^-1 t ; beta^2 = t . y
+            precond(t, r2, y);
+            hipLaunchKernelGGL(k_minres_scal2, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, pparts);
+            double* tmp = r1;
+            r1 = r2;
+            r2 = t;
+            t = tmp;
+            hipLaunchKernelGGL(k_minres_vec3, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, v, w2, w, w1, x);
+            double* wt = w1;
+            w1 = w2;
+            w2 = w;
+            w = wt;
+        };
+        hipGraphExec_t gexec = nullptr;
+        hipGraph_t graph = nullptr;
+        if (use_graph_) {
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int q = 0; q < 3; ++q) iteration();
+            HIP_TRY(hipStreamEndCapture(st, &graph));
+            HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        struct GraphGuard {
+            hipGraphExec_t& e;
+            hipGraph_t& g;
+            ~GraphGuard()
+            {
+                if (e) (void)hipGraphExecDestroy(e);
+                if (g) (void)hipGraphDestroy(g);
+            }
+        } guard{gexec, graph};
         while (!conv && it < max_it) {
-            const int batch = std::min(check_interval, max_it - it);
-            for (int b = 0; b < batch; ++b) {
-                // t = K y ; v = y / beta ; t = t / beta - (beta / oldb) r1 ; alfa = v . t
-                sys->spmv(y, t);
-                hipLaunchKernelGGL(k_minres_vec1, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, y, t, r1, v,
-                                   sys->d_part);
-                hipLaunchKernelGGL(k_minres_alfa, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, nparts);
-                // t -= alfa/beta r2 ; y = P^-1 t ; beta^2 = t . y
-                precond(t, r2, y);
-                hipLaunchKernelGGL(k_minres_scal2, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, pparts);
-                double* tmp = r1;
-                r1 = r2;
-                r2 = t;
-                t = tmp;
-                hipLaunchKernelGGL(k_minres_vec3, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, v, w2, w, w1, x);
-                double* wt = w1;
-                w1 = w2;
-                w2 = w;
-                w = wt;
+            int batch = std::min(check_interval, max_it - it);
+            if (gexec) {
+                batch = std::max(3, (batch / 3) * 3);
+                if (batch > max_it - it) batch = ((max_it - it) / 3) * 3;
+                if (batch == 0) break;
+                for (int b = 0; b < batch; b += 3) HIP_TRY(hipGraphLaunch(gexec, st));
+            } else {
+                for (int b = 0; b < batch; ++b) iteration();
             }
             it += batch;
             HIP_TRY(hipGetLastError());
@@ -1155,6 +1222,58 @@ int apf_bench_spmv(apf_system* sys, int reps, double* ms_avg)
     } catch (const HipFailure& e) {
         return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_bench_spmv: ") + e.what());
     }
+}
+
+__global__ void k_stream_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        b[i] = a[i];
+}
+
+int apf_bench_stream(apf_ctx* ctx, int64_t bytes, int reps, double* gbs)
+{
+    if (!ctx || !gbs || bytes < 8 || reps < 1) return set_error(ctx, APF_BAD_ARG, "apf_bench_stream: bad argument");
+    double *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int64_t n = bytes / 8;
+        HIP_TRY(hipMalloc(&a, n * sizeof(double)));
+        HIP_TRY(hipMalloc(&b, n * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(a, 0, n * sizeof(double), ctx->stream));
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        int cus = 0;
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        const unsigned grid = static_cast<unsigned>(cus) * 8u;
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, n, a, b);
+        HIP_TRY(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, n, a, b);
+        HIP_TRY(hipEventRecord(e1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *gbs = 2.0 * static_cast<double>(n) * 8.0 * reps / (ms * 1e-3) / 1e9;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+        return set_error(ctx, APF_HIP_ERROR, std::string("apf_bench_stream: ") + e.what());
+    }
+}
+
+int apf_system_use_graph(apf_system* sys, int on)
+{
+    if (!sys) return set_error(nullptr, APF_BAD_ARG, "apf_system_use_graph: null system");
+    sys->use_graph = on != 0;
+    return APF_OK;
 }
 
 int apf_system_mg_setup_seconds(apf_system* sys, double* sec)

@@ -204,6 +204,8 @@ struct System {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float t_assembly_ms = 0.f, t_patch_ms = 0.f, t_diag_ms = 0.f, t_spmv_ms = 0.f;
     bool assembled = false;
+    bool use_graph = true;   // capture the MINRES iteration in a hipGraph
+    int spmv_lanes = 16;     // lanes per CSR row in k_spmv
 
     void spmv(const double* x, double* y);
 };

@@ -18,28 +18,56 @@ namespace apf {
 
 namespace {
 
-// Y = (K + s_c M) X   (mode 0)   or   Y = B - (K + s_c M) X   (mode 1), columns [0, nc)
+// Y = (K + s_c M) X   (mode 0)   or   Y = B - (K + s_c M) X   (mode 1), columns [0, nc).
+// One wavefront per row; the row's entries are loaded once into registers
+// (chunks of 32) and reused for every column pass.
 template <typename R>
 __global__ void __launch_bounds__(256) k_mg_apply(int nf, const int* __restrict__ ptr, const int* __restrict__ col,
                                                   const R* __restrict__ K, const R* __restrict__ M,
                                                   const R* __restrict__ sig, int nc, int ld, const R* __restrict__ X,
                                                   const R* __restrict__ B, R* __restrict__ Y, int mode)
 {
-    const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
     const int lane = threadIdx.x % 64;
     if (row >= nf) return;
     const int b = ptr[row], e = ptr[row + 1];
-    for (int c0 = 0; c0 < nc; c0 += 64) {
-        const int c = c0 + lane;
-        if (c >= nc) break;
-        const R s = sig[c];
-        R acc = 0;
-        for (int q = b; q < e; ++q) {
-            const int j = col[q];
-            acc += (K[q] + s * M[q]) * X[static_cast<int64_t>(j) * ld + c];
+    constexpr int NP = 4;  // column passes of 64 handled together
+    for (int cg = 0; cg < nc; cg += NP * 64) {
+        R acc[NP];
+        R sg[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            acc[q] = 0;
+            const int c = cg + q * 64 + lane;
+            sg[q] = (c < nc) ? sig[c] : R(0);
         }
-        const int64_t o = static_cast<int64_t>(row) * ld + c;
-        Y[o] = mode ? B[o] - acc : acc;
+        for (int e0 = b; e0 < e; e0 += 64) {
+            // stage up to 64 entries of the row in registers, one per lane
+            const int my = e0 + lane;
+            const int cj = (my < e) ? col[my] : 0;
+            const R kv = (my < e) ? K[my] : R(0);
+            const R mv = (my < e) ? M[my] : R(0);
+            const int cnt = min(64, e - e0);
+            for (int t = 0; t < cnt; ++t) {
+                const int j = __shfl(cj, t, 64);
+                const R kk = __shfl(kv, t, 64);
+                const R mm = __shfl(mv, t, 64);
+                const R* xr = X + static_cast<int64_t>(j) * ld;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    const int c = cg + q * 64 + lane;
+                    if (c < nc) acc[q] += (kk + sg[q] * mm) * xr[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const int c = cg + q * 64 + lane;
+            if (c < nc) {
+                const int64_t o = static_cast<int64_t>(row) * ld + c;
+                Y[o] = mode ? B[o] - acc[q] : acc[q];
+            }
+        }
     }
 }
 
@@ -53,6 +81,33 @@ __global__ void k_cheb_start(int nf, int nc, int ld, const R* __restrict__ dK, c
     const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
     const int64_t o = static_cast<int64_t>(i) * ld + c;
     d[o] = r[o] * inv_theta / (dK[i] + sig[c] * dM[i]);
+}
+
+// zero start: x = 0, r = B, d = D^-1 B / theta (no memset/memcpy nodes: graph safe)
+template <typename R>
+__global__ void k_cheb_start_zero(int nf, int nc, int ld, const R* __restrict__ dK, const R* __restrict__ dM,
+                                  const R* __restrict__ sig, R inv_theta, const R* __restrict__ B, R* __restrict__ x,
+                                  R* __restrict__ r, R* __restrict__ d)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<int64_t>(nf) * nc) return;
+    const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
+    const int64_t o = static_cast<int64_t>(i) * ld + c;
+    const R b = B[o];
+    x[o] = 0;
+    r[o] = b;
+    d[o] = b * inv_theta / (dK[i] + sig[c] * dM[i]);
+}
+
+// Y[:, 0:nc] = X[:, 0:nc]
+template <typename R>
+__global__ void k_block_copy(int nf, int nc, int ld, const R* __restrict__ X, R* __restrict__ Y)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<int64_t>(nf) * nc) return;
+    const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
+    const int64_t o = static_cast<int64_t>(i) * ld + c;
+    Y[o] = X[o];
 }
 
 // x += d ; r -= Ad ; d = c1 d + c2 D^-1 r
@@ -106,52 +161,76 @@ __global__ void k_coarse_solve(int n, int nc, int ld, const R* __restrict__ inv,
     X[static_cast<int64_t>(i) * ld + c] = acc;
 }
 
-// gather: Z[s][i] = sum_tn r_X[tn nf + s] W[tn][i]  (i < T)  and  Z[s][T + c] = r_Y[c nf + s]
+// gather: Z[s][i] = sum_tn r_X[tn nf + s] W[tn][i]  (i < T)  and  Z[s][T + c] = r_Y[c nf + s].
+// A block owns 64 consecutive spatial rows: it reads the vector coalesced along
+// s into LDS and writes Z rows coalesced along the columns.
 template <typename R>
-__global__ void k_mg_gather(int nf, int T, int nyc, int ld, int64_t nY, const double* __restrict__ r,
-                            const double* __restrict__ W, R* __restrict__ Z)
+__global__ void __launch_bounds__(256) k_mg_gather(int nf, int T, int nyc, int ld, int64_t nY, int th,
+                                                   const double* __restrict__ r, const double* __restrict__ W,
+                                                   R* __restrict__ Z)
 {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    extern __shared__ double lds_g[];
+    double* tile = lds_g;  // [ncol][th]
     const int ncol = T + nyc;
-    if (t >= static_cast<int64_t>(nf) * ncol) return;
-    const int c = static_cast<int>(t % ncol), s = static_cast<int>(t / ncol);
-    double v;
-    if (c < T) {
-        v = 0.0;
-        for (int tn = 0; tn < T; ++tn) v += r[nY + static_cast<int64_t>(tn) * nf + s] * W[tn * T + c];
-    } else {
-        v = r[static_cast<int64_t>(c - T) * nf + s];
+    const int s0 = blockIdx.x * th;
+    const int ns = min(th, nf - s0);
+    for (int e = threadIdx.x; e < ncol * th; e += blockDim.x) {
+        const int c = e / th, sl = e % th;
+        double v = 0.0;
+        if (sl < ns) v = (c < T) ? r[nY + static_cast<int64_t>(c) * nf + s0 + sl]
+                                 : r[static_cast<int64_t>(c - T) * nf + s0 + sl];
+        tile[c * th + sl] = v;
     }
-    Z[static_cast<int64_t>(s) * ld + c] = static_cast<R>(v);
+    __syncthreads();
+    for (int e = threadIdx.x; e < ns * ncol; e += blockDim.x) {
+        const int sl = e / ncol, c = e % ncol;
+        double v;
+        if (c < T) {
+            v = 0.0;
+            for (int tn = 0; tn < T; ++tn) v += tile[tn * th + sl] * W[tn * T + c];
+        } else {
+            v = tile[c * th + sl];
+        }
+        Z[static_cast<int64_t>(s0 + sl) * ld + c] = static_cast<R>(v);
+    }
 }
 
-// scatter back: z_X[tn nf + s] = sum_i Z[s][i] W[tn][i] ; z_Y[(k Nt + l) nf + s] = sum_l' Minv_k[l][l'] Z[s][T + k Nt + l']
-// also accumulates dot(z, r) partials
+// scatter back: z_X[tn nf + s] = sum_i Z[s][i] W[tn][i] ; z_Y[(k Nt + l) nf + s] = sum_l' Minv_k[l][l'] Z[s][T + k Nt + l'],
+// and dot(z, r) partials (one per block).
 template <typename R>
-__global__ void k_mg_scatter(int nf, int T, int Nt, int m, int ld, int64_t nY, const R* __restrict__ Z,
-                             const double* __restrict__ W, const double* __restrict__ MtInvH,
-                             const double* __restrict__ r, double* __restrict__ z, double* __restrict__ part)
+__global__ void __launch_bounds__(256) k_mg_scatter(int nf, int T, int Nt, int m, int ld, int64_t nY, int th,
+                                                    const R* __restrict__ Z, const double* __restrict__ W,
+                                                    const double* __restrict__ MtInvH, const double* __restrict__ r,
+                                                    double* __restrict__ z, double* __restrict__ part)
 {
-    const int64_t n = nY + static_cast<int64_t>(T) * nf;
+    extern __shared__ double lds_s[];
+    const int ncol = T + m * Nt;
+    double* tile = lds_s;  // [ncol][th]
+    const int s0 = blockIdx.x * th;
+    const int ns = min(th, nf - s0);
+    for (int e = threadIdx.x; e < ns * ncol; e += blockDim.x) {
+        const int sl = e / ncol, c = e % ncol;
+        tile[c * th + sl] = static_cast<double>(Z[static_cast<int64_t>(s0 + sl) * ld + c]);
+    }
+    __syncthreads();
     double dot = 0.0;
-    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
-         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    for (int e = threadIdx.x; e < ncol * th; e += blockDim.x) {
+        const int c = e / th, sl = e % th;
+        if (sl >= ns) continue;
+        int64_t idx;
         double v = 0.0;
-        if (t < nY) {
-            const int s = static_cast<int>(t % nf);
-            const int kl = static_cast<int>(t / nf);
-            const int k = kl / Nt, l = kl % Nt;
-            const R* zr = Z + static_cast<int64_t>(s) * ld + T + k * Nt;
-            const double* Mi = MtInvH + (static_cast<int64_t>(k) * Nt + l) * Nt;
-            for (int q = 0; q < Nt; ++q) v += Mi[q] * static_cast<double>(zr[q]);
+        if (c < T) {
+            idx = nY + static_cast<int64_t>(c) * nf + s0 + sl;  // c plays tn
+            for (int i = 0; i < T; ++i) v += tile[i * th + sl] * W[c * T + i];
         } else {
-            const int64_t x = t - nY;
-            const int tn = static_cast<int>(x / nf), s = static_cast<int>(x % nf);
-            const R* zr = Z + static_cast<int64_t>(s) * ld;
-            for (int i = 0; i < T; ++i) v += static_cast<double>(zr[i]) * W[tn * T + i];
+            const int kl = c - T;
+            const int k = kl / Nt, l = kl % Nt;
+            idx = static_cast<int64_t>(kl) * nf + s0 + sl;
+            const double* Mi = MtInvH + (static_cast<int64_t>(k) * Nt + l) * Nt;
+            for (int q = 0; q < Nt; ++q) v += Mi[q] * tile[(T + k * Nt + q) * th + sl];
         }
-        z[t] = v;
-        dot += v * r[t];
+        z[idx] = v;
+        dot += v * r[idx];
     }
     __shared__ double sh[4];
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
@@ -298,15 +377,14 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_
     const int blk = 256;
     const unsigned rows_grid = grid_for(d.nf, 4);
     if (zero_start) {
-        // x = 0, r = B
-        HIP_TRY(hipMemsetAsync(d.X, 0, static_cast<size_t>(d.nf) * ld * sizeof(R), st));
-        HIP_TRY(hipMemcpyAsync(d.Rr, d.B, static_cast<size_t>(d.nf) * ld * sizeof(R), hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start_zero<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc,
+                           ld, d.dK, d.dM, sig, static_cast<R>(1) / theta, d.B, d.X, d.Rr, d.D);
     } else {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col, d.K,
                            d.M, sig, nc, ld, d.X, d.B, d.Rr, 1);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc, ld,
+                           d.dK, d.dM, sig, static_cast<R>(1) / theta, d.Rr, d.D);
     }
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc, ld, d.dK,
-                       d.dM, sig, static_cast<R>(1) / theta, d.Rr, d.D);
     R rho = 1 / sigma1;
     for (int k = 0; k < nu; ++k) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col, d.K,
@@ -348,19 +426,40 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
     Level& f = levels[0];
     const int nf = f.nf;
     const int64_t nel = static_cast<int64_t>(nf) * ncol;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, T, nyc, ld,
-                       S.nY, r, d_W, f.B);
+    const int th = tile_rows();
+    const unsigned tiles = static_cast<unsigned>((nf + th - 1) / th);
+    const size_t lds = static_cast<size_t>(ncol) * th * sizeof(double);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R>), dim3(tiles), dim3(256), lds, st, nf, T, nyc, ld, S.nY, th, r,
+                       d_W, f.B);
     // stage 1: all columns, Y columns with shift 0, X columns with shift s_i
     vcycle(st, 0, ncol, d_sig, d_sidx);
     // stage 2 (X columns): apply K, then the shifted V-cycle again
-    HIP_TRY(hipMemcpyAsync(d_Z, f.X, static_cast<size_t>(nf) * ld * sizeof(R), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, ncol, ld, f.X,
+                       d_Z);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(grid_for(nf, 4)), dim3(256), 0, st, nf, f.ptr, f.col, f.K,
                        f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
     vcycle(st, 0, T, d_sig, d_sidx);
     // X columns from the second V-cycle, Y columns from the first
-    HIP_TRY(hipMemcpy2DAsync(d_Z, ld * sizeof(R), f.X, ld * sizeof(R), T * sizeof(R), nf, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R>), dim3(nparts), dim3(256), 0, st, nf, T, S.Nt, S.mesh.m(), ld,
-                       S.nY, d_Z, d_W, d_MtInvH, r, z, part);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(static_cast<int64_t>(nf) * T, 256)), dim3(256), 0,
+                       st, nf, T, ld, f.X, d_Z);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R>), dim3(tiles), dim3(256), lds, st, nf, T, S.Nt, S.mesh.m(),
+                       ld, S.nY, th, d_Z, d_W, d_MtInvH, r, z, part);
+}
+
+template <typename R>
+int MGDevice<R>::tile_rows() const
+{
+    // rows per transpose tile: the LDS tile ncol x th doubles stays within 64 KiB
+    int th = 65536 / (ncol * static_cast<int>(sizeof(double)));
+    th = std::max(1, std::min(64, th));
+    return th;
+}
+
+template <typename R>
+int MGDevice<R>::n_partials(int nf) const
+{
+    const int th = tile_rows();
+    return (nf + th - 1) / th;
 }
 
 template struct MGDevice<double>;

@@ -56,6 +56,8 @@ struct MGDevice {
     void smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_start);
     void vcycle(hipStream_t st, int l, int nc, const R* sig, const int* sidx);
     void apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts);
+    int tile_rows() const;
+    int n_partials(int nf) const;  // dot partials written by apply()
 };
 
 }  // namespace apf

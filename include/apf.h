@@ -163,6 +163,12 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
 /* Average device time of `reps` back-to-back SpMVs y = K x (HIP events on the
  * system stream), for the HBM roofline of the MINRES inner loop. */
 int apf_bench_spmv(apf_system* sys, int reps, double* ms_avg);
+/* Device copy bandwidth (read + write bytes / s, in GB/s) of a plain copy kernel
+ * over `bytes` bytes, `reps` launches: the measured HBM ceiling and the
+ * calibration kernel for counter-based traffic. */
+int apf_bench_stream(apf_ctx* ctx, int64_t bytes, int reps, double* gbs);
+/* Capture the MINRES iteration in a hipGraph (default on) or launch eagerly. */
+int apf_system_use_graph(apf_system* sys, int on);
 /* Host seconds of the last multigrid setup (Galerkin operators, time eigenmodes). */
 int apf_system_mg_setup_seconds(apf_system* sys, double* sec);
 /* Upload a solution x = [lambda; u] [n] (e.g. from another solver). */

@@ -78,3 +78,39 @@ def test_minres_failure_is_reported(ctx):
     with pytest.raises(apf.NotConverged) as ei:
         sysm.minres(tol=1e-12, max_it=3, check_interval=3)
     assert ei.value.stats["iterations"] == 3
+
+
+def test_minres_graph_replays_match_eager(ctx):
+    """The hipGraph of three MINRES iterations is replayed many times with eager
+    true-residual checks in between; every replay must reproduce the eager
+    launch sequence, so the final iterate is bitwise identical."""
+    m = meshes.uniform_mesh(2, 3)
+    sp = apf.Space(m, 2, coarse=meshes.uniform_hierarchy(2, 3))
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(meshes.smooth_initial(2))
+    sysm.use_graph(True)
+    xg, sg = sysm.minres(tol=1e-10, max_it=300, check_interval=9)
+    sysm.use_graph(False)
+    xe, se = sysm.minres(tol=1e-10, max_it=300, check_interval=9)
+    assert sg["iterations"] == se["iterations"]
+    assert np.array_equal(xg, xe)
+
+
+@pytest.mark.parametrize("prec", [apf.APF_FP64, apf.APF_FP32_REFINE], ids=["mg64", "mg32"])
+@pytest.mark.parametrize("dim,p,outer", [(1, 1, 5), (1, 2, 4), (2, 1, 3), (2, 2, 3), (2, 3, 2)])
+def test_multigrid_minres_converges_and_matches(ctx, dim, p, outer, prec):
+    m = meshes.uniform_mesh(dim, outer)
+    coeff = meshes.smooth_initial(dim)
+    if dim == 2:
+        coeff = apf.Coefficients(meshes.checkerboard_kappa(m, 4, seed=11), 0.0, coeff.u0)
+    print(f"mg minres dim={dim} p={p} outer={outer} prec={prec}", flush=True)
+    sp = apf.Space(m, p, coarse=meshes.uniform_hierarchy(dim, outer))
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(coeff)
+    x, st = sysm.minres(tol=1e-10, max_it=600, check_interval=10, precond=prec)
+    assert st["converged"] and st["true_relative_residual"] <= 1e-10
+    assert st["iterations"] <= 150
+    stm, od, prob = oracle_setup(m, p, coeff)
+    S = OA.assemble(stm, od, prob)
+    xd = OE.main_solve(S)
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
