@@ -53,6 +53,7 @@ struct EstParams {
     int64_t scratch_stride;   // doubles per workgroup
     int nmax;
     int refine_steps;
+    int n_lds;                // largest patch order whose inverse is kept in LDS
 };
 
 struct EstTables {
@@ -214,27 +215,31 @@ __device__ inline void gather_cell(const EstParams& P, int k, int c, const int* 
     uu = u;
 }
 
+constexpr int PATCH_THREADS = 1024;
+
 template <typename FT>
-__global__ void __launch_bounds__(256) k_patch(EstParams P, EstTables T, PatchTopo topo, const double* verts,
+__global__ void __launch_bounds__(PATCH_THREADS) k_patch(EstParams P, EstTables T, PatchTopo topo, const double* verts,
                                                const int* cells, const double* Mcell, const int* cell_free,
                                                const double* times, const double* lt, const double* x,
                                                double* scratch, double* sig, double* patch_res)
 {
     extern __shared__ double lds[];
-    __shared__ double shv[8];
-    __shared__ int shi[8];
+    __shared__ double shv[PATCH_THREADS / 64];
+    __shared__ int shi[PATCH_THREADS / 64];
     const int Nf = P.Nf, Nq = P.Nq, Nt = P.Nt, Ns = P.Ns, dim = P.dim;
     double* base = scratch + static_cast<int64_t>(blockIdx.x) * P.scratch_stride;
     const int nmax = P.nmax;
     const int C = P.chunk * Nt;
     double* Kd = base;                                            // nmax^2
-    FT* Ki = reinterpret_cast<FT*>(base + static_cast<int64_t>(nmax) * nmax);  // nmax^2
+    FT* Ki_g = reinterpret_cast<FT*>(base + static_cast<int64_t>(nmax) * nmax);  // nmax^2
     double* RHS = base + 2 * static_cast<int64_t>(nmax) * nmax;   // C * nmax (column major)
     double* X = RHS + static_cast<int64_t>(C) * nmax;
     double* Rs = X + static_cast<int64_t>(C) * nmax;
     int* perm = reinterpret_cast<int*>(Rs + static_cast<int64_t>(C) * nmax);        // nmax ints
     FT* fac = reinterpret_cast<FT*>(Rs + static_cast<int64_t>(C) * nmax + nmax);    // nmax FT
     double* stage = lds;  // [chunk][Ns][Nt][2]
+    // the inverse lives in LDS when it fits (n <= P.n_lds), else in the scratch
+    FT* Ki_l = reinterpret_cast<FT*>(lds + static_cast<int64_t>(P.chunk) * Ns * Nt * 2);
 
     for (int item = blockIdx.x; item < P.n_items; item += gridDim.x) {
         const int a = topo.order[P.per_slab ? item / P.m : item];
@@ -279,6 +284,7 @@ __global__ void __launch_bounds__(256) k_patch(EstParams P, EstTables T, PatchTo
             __syncthreads();
         }
         // ---- invert in FT
+        FT* Ki = (n <= P.n_lds) ? Ki_l : Ki_g;
         for (int64_t e = threadIdx.x; e < static_cast<int64_t>(n) * n; e += blockDim.x) Ki[e] = static_cast<FT>(Kd[e]);
         __syncthreads();
         gauss_jordan_inverse<FT>(Ki, n, perm, fac, shv, shi);
@@ -351,34 +357,60 @@ __global__ void __launch_bounds__(256) k_patch(EstParams P, EstTables T, PatchTo
                 }
                 __syncthreads();
             }
-            // ---- X = Ki RHS (Ki symmetric: read Ki[j][i] for coalescing)
-            for (int64_t e = threadIdx.x; e < static_cast<int64_t>(ncol) * n; e += blockDim.x) {
-                const int col = static_cast<int>(e / n), i = static_cast<int>(e % n);
-                const double* b = RHS + static_cast<int64_t>(col) * n;
-                FT acc = 0;
-                for (int j = 0; j < n; ++j) acc += Ki[static_cast<int64_t>(j) * n + i] * static_cast<FT>(b[j]);
-                X[e] = static_cast<double>(acc);
+            // ---- X = Ki RHS, residual Rs = RHS - Kd X (FP64), FT corrections.
+            // Both matrices are symmetric: reading column i as row i keeps the
+            // loads coalesced; each thread owns one row i and 4 columns.
+            const int ncg = (ncol + 3) / 4;
+            for (int e = threadIdx.x; e < ncg * n; e += blockDim.x) {
+                const int cg = e / n, i = e % n;
+                const int c0 = cg * 4;
+                FT acc[4] = {0, 0, 0, 0};
+                for (int j = 0; j < n; ++j) {
+                    const FT kij = Ki[static_cast<int64_t>(j) * n + i];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (c0 + q < ncol) acc[q] += kij * static_cast<FT>(RHS[static_cast<int64_t>(c0 + q) * n + j]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (c0 + q < ncol) X[static_cast<int64_t>(c0 + q) * n + i] = static_cast<double>(acc[q]);
             }
             __syncthreads();
             const int steps = P.refine_steps;
             for (int it = 0; it <= steps; ++it) {
                 // residual in FP64
-                for (int64_t e = threadIdx.x; e < static_cast<int64_t>(ncol) * n; e += blockDim.x) {
-                    const int col = static_cast<int>(e / n), i = static_cast<int>(e % n);
-                    const double* xc = X + static_cast<int64_t>(col) * n;
-                    double acc = RHS[e];
-                    for (int j = 0; j < n; ++j) acc -= Kd[static_cast<int64_t>(j) * n + i] * xc[j];
-                    Rs[e] = acc;
+                for (int e = threadIdx.x; e < ncg * n; e += blockDim.x) {
+                    const int cg = e / n, i = e % n;
+                    const int c0 = cg * 4;
+                    double acc[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = (c0 + q < ncol) ? RHS[static_cast<int64_t>(c0 + q) * n + i] : 0.0;
+                    for (int j = 0; j < n; ++j) {
+                        const double kij = Kd[static_cast<int64_t>(j) * n + i];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (c0 + q < ncol) acc[q] -= kij * X[static_cast<int64_t>(c0 + q) * n + j];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (c0 + q < ncol) Rs[static_cast<int64_t>(c0 + q) * n + i] = acc[q];
                 }
                 __syncthreads();
                 if (it == steps) break;
                 // correction in FT
-                for (int64_t e = threadIdx.x; e < static_cast<int64_t>(ncol) * n; e += blockDim.x) {
-                    const int col = static_cast<int>(e / n), i = static_cast<int>(e % n);
-                    const double* rc = Rs + static_cast<int64_t>(col) * n;
-                    FT acc = 0;
-                    for (int j = 0; j < n; ++j) acc += Ki[static_cast<int64_t>(j) * n + i] * static_cast<FT>(rc[j]);
-                    X[e] += static_cast<double>(acc);
+                for (int e = threadIdx.x; e < ncg * n; e += blockDim.x) {
+                    const int cg = e / n, i = e % n;
+                    const int c0 = cg * 4;
+                    FT acc[4] = {0, 0, 0, 0};
+                    for (int j = 0; j < n; ++j) {
+                        const FT kij = Ki[static_cast<int64_t>(j) * n + i];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (c0 + q < ncol) acc[q] += kij * static_cast<FT>(Rs[static_cast<int64_t>(c0 + q) * n + j]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (c0 + q < ncol) X[static_cast<int64_t>(c0 + q) * n + i] += static_cast<double>(acc[q]);
                 }
                 __syncthreads();
             }
@@ -707,14 +739,26 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         // slab by slab; with the main solve converged it equals lambda.
         HIP_TRY(hipMemcpyAsync(B.lt, sys->d_x, S.nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIP_TRY(hipMemsetAsync(B.patch_res, 0, S.patches.n_patches * sizeof(double), st));
-        const size_t lds = static_cast<size_t>(B.chunk) * S.Ns * S.Nt * 2 * sizeof(double);
+        const size_t stage_bytes = static_cast<size_t>(B.chunk) * S.Ns * S.Nt * 2 * sizeof(double);
+        const size_t ft_bytes = (prec == APF_FP64) ? sizeof(double) : sizeof(float);
+        const size_t lds_cap = 150 * 1024;
+        int n_lds = static_cast<int>(std::sqrt(static_cast<double>(lds_cap - stage_bytes) / ft_bytes));
+        n_lds = std::min(n_lds, B.nmax);
+        P.n_lds = n_lds;
+        const size_t lds = stage_bytes + static_cast<size_t>(n_lds) * n_lds * ft_bytes;
+        if (prec == APF_FP64)
+            HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_patch<double>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        else
+            HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_patch<float>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
         HIP_TRY(hipEventRecord(sys->ev0, st));
         if (prec == APF_FP64)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch<double>), dim3(B.n_wg), dim3(256), lds, st, P, Tb, topo,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch<double>), dim3(B.n_wg), dim3(PATCH_THREADS), lds, st, P, Tb, topo,
                                sys->d_verts, sys->d_cells, sys->d_Mcell, sys->d_cell_free, sys->d_times, B.lt,
                                sys->d_x, B.scratch, B.sig, B.patch_res);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch<float>), dim3(B.n_wg), dim3(256), lds, st, P, Tb, topo,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch<float>), dim3(B.n_wg), dim3(PATCH_THREADS), lds, st, P, Tb, topo,
                                sys->d_verts, sys->d_cells, sys->d_Mcell, sys->d_cell_free, sys->d_times, B.lt,
                                sys->d_x, B.scratch, B.sig, B.patch_res);
         HIP_TRY(hipGetLastError());

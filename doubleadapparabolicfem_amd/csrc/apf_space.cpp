@@ -109,10 +109,14 @@ void build_space(Space& S, const TensorMesh& mesh, int p)
             }
         }
     }
+    // free ids by first appearance over (cell, local node): neighbours get
+    // nearby ids, which keeps the x gathers of the SpMV cache-local
     S.free_index.assign(S.n_nodes, -1);
     S.nf = 0;
-    for (int i = 0; i < S.n_nodes; ++i)
-        if (!S.node_boundary[i]) S.free_index[i] = S.nf++;
+    for (size_t e = 0; e < S.cell_nodes.size(); ++e) {
+        const int nid = S.cell_nodes[e];
+        if (!S.node_boundary[nid] && S.free_index[nid] < 0) S.free_index[nid] = S.nf++;
+    }
     S.cell_free.resize(S.cell_nodes.size());
     for (size_t i = 0; i < S.cell_nodes.size(); ++i) S.cell_free[i] = S.free_index[S.cell_nodes[i]];
 

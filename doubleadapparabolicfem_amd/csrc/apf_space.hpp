@@ -8,7 +8,8 @@
 // and uses its own (documented) numbering:
 //   spatial nodes: vertices, then (p-1) nodes per edge (edges sorted by
 //     (min, max) vertex id, nodes ordered from the min vertex), then interior
-//     nodes per triangle in reference tuple order; free = not on dOmega;
+//     nodes per triangle in reference tuple order; free = not on dOmega,
+//     numbered by first appearance over (cell, local node);
 //   Y dof (k, l, s) = (k (p+1) + l) nf + s          (dG in time)
 //   X dof (tn, s)   = nY + tn nf + s,  tn = k p + l (continuous in time)
 //   K unknown order [lambda (Y); u (X)] as in

@@ -19,7 +19,8 @@ Numbering (shared bit-for-bit with the product's host code):
 * spatial nodes: vertices, then (p-1) nodes per edge (edges sorted by
   (min vertex, max vertex), nodes ordered from the min vertex), then interior
   nodes per triangle in reference tuple order; free = not on the boundary,
-  numbered in node order;
+  numbered by first appearance when scanning cells in order and their local
+  nodes in reference order (keeps neighbours close in memory);
 * X dof = time_node * n_free + free_node, time_node = k*p + l;
 * Y dof = (k*(p+1) + l) * n_free + free_node.
 """
@@ -116,13 +117,8 @@ def spatial_dofs(mesh: SpatialMesh, p: int) -> SpatialDofs:
         bvert = (xs == xs.min()) | (xs == xs.max())
         boundary = np.zeros(n_nodes, dtype=bool)
         boundary[:nv] = bvert
-        free_index = -np.ones(n_nodes, dtype=np.int64)
-        nf = 0
-        for i in range(n_nodes):
-            if not boundary[i]:
-                free_index[i] = nf
-                nf += 1
-        return SpatialDofs(p, n_nodes, cell_nodes, boundary, free_index, nf, coords,
+        free_index = _first_appearance(cell_nodes, boundary, n_nodes)
+        return SpatialDofs(p, n_nodes, cell_nodes, boundary, free_index, int((free_index >= 0).sum()), coords,
                            None, bvert, None)
 
     cells = mesh.cells
@@ -167,11 +163,20 @@ def spatial_dofs(mesh: SpatialMesh, p: int) -> SpatialDofs:
             cell_nodes[c, k] = nid
             coords[nid] = (t[0] * mesh.vertices[V[0]] + t[1] * mesh.vertices[V[1]]
                            + t[2] * mesh.vertices[V[2]]) / p
-    free_index = -np.ones(n_nodes, dtype=np.int64)
-    free = np.nonzero(~boundary)[0]
-    free_index[free] = np.arange(len(free))
-    return SpatialDofs(p, n_nodes, cell_nodes, boundary, free_index, len(free), coords,
+    free_index = _first_appearance(cell_nodes, boundary, n_nodes)
+    return SpatialDofs(p, n_nodes, cell_nodes, boundary, free_index, int((free_index >= 0).sum()), coords,
                        edges, bvert, bedge)
+
+
+def _first_appearance(cell_nodes, boundary, n_nodes):
+    """Free ids in order of first appearance over (cell, local node)."""
+    free_index = -np.ones(n_nodes, dtype=np.int64)
+    nf = 0
+    for nid in cell_nodes.ravel():
+        if not boundary[nid] and free_index[nid] < 0:
+            free_index[nid] = nf
+            nf += 1
+    return free_index
 
 
 @dataclass
