@@ -27,6 +27,12 @@ sys.path.insert(0, ROOT)
 METRIC = "assembly+estimator DOF/s & MINRES s/solve; mixed vs FP64 at 1e-10 rel err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+PATCH_KERNEL = "k_patch<float> (vertex-patch mixed solves, FP32 inverse + FP64 residual refinement)"
+# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+# calibration) from the rocprofv3 --pmc runs of `bench.py --profile` on the
+# same code, summarised in profiles/counters_c2.md.
+MEASURED_TRAFFIC = {}
 
 
 def workload(name):
@@ -53,6 +59,18 @@ def workload(name):
                     desc="2+1D p=3, union-jack 64x64 x 16 intervals, kappa piecewise constant 10^U(-1,1) "
                          "on an 8x8 checkerboard")
     raise SystemExit(f"unknown config {name}")
+
+
+def patch_flops(sp):
+    """Algorithmic flops of the patch solves of one estimate: per spatial vertex
+    patch of size N (flux + scalar + mean-zero multiplier, one Legendre time
+    mode) one LU, (2/3) N^3, and one forward/back solve, 2 N^2, per (slab, time
+    mode) right-hand side.  The patch matrix is the same for every slab and
+    mode of a tensor mesh, so it is factorised once."""
+    nf, ns_, _, bd = sp.patch_sizes()
+    N = (nf + ns_ + (bd == 0)).astype(np.float64)
+    cols = sp.n_intervals * (sp.p + 1)
+    return float(np.sum(2.0 / 3.0 * N ** 3 + 2.0 * N ** 2 * cols))
 
 
 def cpu_baseline(min_s=10.0):
@@ -87,8 +105,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-spmv", action="store_true",
-                    help="assemble once, run 20 SpMVs and one 1 GiB device copy (counter calibration)")
+    ap.add_argument("--profile", action="store_true",
+                    help="one assembly, MINRES solve, mixed and FP64 estimate, 20 SpMVs and a 1 GiB device "
+                         "copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
@@ -103,13 +122,17 @@ def main():
     sysm = apf.System(ctx, sp)
     n_dofs = sp.nX + sp.nY
 
-    if args.profile_spmv:
+    if args.profile:
+        # counter runs: one pass of each hot-path kernel, then the calibration copy
         sysm.assemble(W["coeff"])
+        sysm.minres(tol=1e-10, max_it=5000, check_interval=10)
+        sysm.estimate(apf.APF_FP32_REFINE, 2, per_cell=False)
+        sysm.estimate(apf.APF_FP64, 0, per_cell=False)
         ms = sysm.bench_spmv(20)
         gbs = ctx.bench_stream(1 << 30, 3)
         print(json.dumps({"spmv_ms": ms, "spmv_algorithmic_bytes": 12 * sp.nnz + 20 * sp.n, "nnz": sp.nnz,
                           "n": sp.n, "copy_kernel_bytes_read": 1 << 30, "copy_kernel_bytes_written": 1 << 30,
-                          "copy_GBs": gbs}))
+                          "copy_GBs": gbs, "patch_flops": patch_flops(sp)}))
         return
 
     def step(prec=apf.APF_FP32_REFINE):
@@ -140,11 +163,16 @@ def main():
     rel = {k: abs(est_mixed[k] - est64[k]) / abs(est64[k]) for k in ("flux2", "eta2", "lambda_y2", "y_est2")}
     value = n_dofs / ((t_asm + t_est) * 1e-3)
 
+    # MFMA/FP32 roofline of the patch kernel (dominant kernel of `value`)
+    pflops = patch_flops(sp)
+    patch_ms = est_mixed["seconds_patches"] * 1e3
+    patch_tflops = pflops / (patch_ms * 1e-3) / 1e12
     # HBM roofline of the MINRES SpMV: algorithmic bytes 12 nnz + 20 n
     spmv_ms = sysm.bench_spmv(50)
     stream_gbs = ctx.bench_stream(1 << 30, 10)
     spmv_bytes = 12 * sp.nnz + 20 * sp.n
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC.get(args.config, {})
     out = {
         "metric": METRIC,
         "value": value,
@@ -161,10 +189,15 @@ def main():
         "config": {"workload": W["desc"], "p": W["p"], "n_X": sp.nX, "n_Y": sp.nY, "nnz_K": sp.nnz,
                    "space_time_cells": sp.n_space_cells * sp.n_intervals,
                    "vertex_patches": sp.n_patches * sp.n_intervals},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv (MINRES CSR SpMV, FP64 values, int32 columns)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms,
-                     "measured_copy_GBs": stream_gbs, "frac_of_measured_copy": achieved / stream_gbs},
+        "roofline": {"bound": "mfma", "kernel": PATCH_KERNEL, "achieved": patch_tflops,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": patch_tflops / FP32_PEAK_TFLOPS,
+                     "traffic": traffic.get("patch"), "algorithmic_flops_per_launch": pflops,
+                     "avg_launch_ms": patch_ms},
+        "roofline_minres_spmv": {"bound": "hbm", "kernel": "k_spmv (MINRES CSR SpMV, FP64 values, int32 columns)",
+                                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get("spmv"),
+                                 "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms,
+                                 "measured_copy_GBs": stream_gbs, "frac_of_measured_copy": achieved / stream_gbs},
         "minres_s_per_solve": t_minres,
         "minres_iterations": its,
         "minres_true_rel_residual": rec[-1][2]["true_relative_residual"],

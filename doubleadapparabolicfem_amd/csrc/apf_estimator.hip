@@ -49,11 +49,7 @@ struct EstParams {
     int ns, m, nf;
     int64_t nY;
     double ell;
-    int n_items, per_slab, chunk;
-    int64_t scratch_stride;   // doubles per workgroup
-    int nmax;
     int refine_steps;
-    int n_lds;                // largest patch order whose inverse is kept in LDS
 };
 
 struct EstTables {
@@ -112,81 +108,121 @@ __device__ inline void atomic_max_pos(double* addr, double v)
     atomicMax(reinterpret_cast<unsigned long long*>(addr), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
-// argmax_{r in [k, n)} |A[r][k]| over the workgroup
-template <typename FT>
-__device__ inline int block_argmax_col(const FT* A, int n, int k, double* shv, int* shi)
-{
-    double best = -1.0;
-    int bi = k;
-    for (int r = k + threadIdx.x; r < n; r += blockDim.x) {
-        const double v = fabs(static_cast<double>(A[static_cast<int64_t>(r) * n + k]));
-        if (v > best) { best = v; bi = r; }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_down(best, off, 64);
-        const int oi = __shfl_down(bi, off, 64);
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-    }
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    __syncthreads();
-    if (l == 0) { shv[w] = best; shi[w] = bi; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 1; i < static_cast<int>(blockDim.x / 64); ++i)
-            if (shv[i] > shv[0] || (shv[i] == shv[0] && shi[i] < shi[0])) { shv[0] = shv[i]; shi[0] = shi[i]; }
-    }
-    __syncthreads();
-    const int r = shi[0];
-    __syncthreads();
-    return r;
-}
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 
-// In-place Gauss-Jordan inversion with partial (row) pivoting; returns the
-// smallest pivot magnitude seen.
 template <typename FT>
-__device__ double gauss_jordan_inverse(FT* A, int n, int* perm, FT* fac, double* shv, int* shi)
+struct Vec4;
+template <>
+struct Vec4<float> {
+    using type = v4f;
+};
+template <>
+struct Vec4<double> {
+    using type = v4d;
+};
+
+// D += A B for one 16x16x4 tile: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; element r of the result is D[drow(l / 16, r)][l % 16],
+// with drow = 4 (l / 16) + r for FP32 and l / 16 + 4 r for FP64 (measured lane
+// layouts of v_mfma_f32_16x16x4f32 and v_mfma_f64_16x16x4f64 on gfx950).
+__device__ inline v4f mfma_16x16x4(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+__device__ inline v4d mfma_16x16x4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+template <typename FT>
+__device__ constexpr int drow(int lk, int r);
+template <>
+__device__ constexpr int drow<float>(int lk, int r) { return 4 * lk + r; }
+template <>
+__device__ constexpr int drow<double>(int lk, int r) { return lk + 4 * r; }
+
+// Patch slots in processing order q (largest patch first): order[q] is the
+// patch, n[q] its order, np[q] the order padded to a multiple of 16, off2[q]
+// the offset of its np x np matrix slots and off1[q] of its np-row vectors.
+struct PatchSlots {
+    const int* n;
+    const int* np;
+    const int64_t* off2;
+    const int64_t* off1;
+};
+
+// In-place Gauss-Jordan inversion of the n x n matrix A (row stride ld) with
+// partial row pivoting (ties to the lowest row).  Lanes own columns
+// (j = lane + 64 t), waves own rows (i = wave + nw s).  Wave 0 picks the pivot,
+// swaps it into place and scales it; then every wave eliminates its rows: two
+// workgroup barriers per step, no integer division in the update.
+template <typename FT>
+__device__ void gj_invert(FT* A, int n, int ld, int* perm)
 {
-    double minpiv = 1e300;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nch = (n + 255) / 256;
     for (int k = 0; k < n; ++k) {
-        const int pr = block_argmax_col(A, n, k, shv, shi);
-        if (threadIdx.x == 0) perm[k] = pr;
-        if (pr != k)
-            for (int j = threadIdx.x; j < n; j += blockDim.x) {
-                const FT t = A[static_cast<int64_t>(k) * n + j];
-                A[static_cast<int64_t>(k) * n + j] = A[static_cast<int64_t>(pr) * n + j];
-                A[static_cast<int64_t>(pr) * n + j] = t;
+        if (wave == 0) {
+            double best = -1.0, bval = 0.0;
+            int bi = n;
+            for (int r = k + lane; r < n; r += 64) {
+                const double v = static_cast<double>(A[static_cast<int64_t>(r) * ld + k]);
+                if (fabs(v) > best) {
+                    best = fabs(v);
+                    bval = v;
+                    bi = r;
+                }
             }
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = A[static_cast<int64_t>(i) * n + k];
-        __syncthreads();
-        const FT d = fac[k];
-        minpiv = fmin(minpiv, fabs(static_cast<double>(d)));
-        const FT inv = static_cast<FT>(1) / d;
-        for (int j = threadIdx.x; j < n; j += blockDim.x) {
-            const FT a = (j == k) ? static_cast<FT>(1) : A[static_cast<int64_t>(k) * n + j];
-            A[static_cast<int64_t>(k) * n + j] = a * inv;
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off, 64);
+                const double ov = __shfl_xor(bval, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bval = ov;
+                    bi = oi;
+                }
+            }
+            const int pr = bi;
+            const FT inv = static_cast<FT>(1) / static_cast<FT>(bval);
+            for (int j = lane; j < n; j += 64) {
+                const FT v = A[static_cast<int64_t>(pr) * ld + j];
+                if (pr != k) A[static_cast<int64_t>(pr) * ld + j] = A[static_cast<int64_t>(k) * ld + j];
+                A[static_cast<int64_t>(k) * ld + j] = ((j == k) ? static_cast<FT>(1) : v) * inv;
+            }
+            if (lane == 0) perm[k] = pr;
         }
         __syncthreads();
-        const int64_t nn = static_cast<int64_t>(n) * n;
-        for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) {
-            const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
-            if (i == k) continue;
-            const FT a = (j == k) ? static_cast<FT>(0) : A[e];
-            A[e] = a - fac[i] * A[static_cast<int64_t>(k) * n + j];
+        // the column block holding k goes last: its update overwrites A[i][k]
+        const int kc = k >> 8;
+        for (int cc = 1; cc <= nch; ++cc) {
+            const int jb = ((kc + cc) % nch) * 256;
+            FT prow[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int j = jb + lane + 64 * t;
+                prow[t] = (j < n) ? A[static_cast<int64_t>(k) * ld + j] : static_cast<FT>(0);
+            }
+            for (int i = wave; i < n; i += nw) {
+                if (i == k) continue;
+                FT* row = A + static_cast<int64_t>(i) * ld;
+                const FT f = row[k];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int j = jb + lane + 64 * t;
+                    if (j < n) row[j] = ((j == k) ? static_cast<FT>(0) : row[j]) - f * prow[t];
+                }
+            }
         }
         __syncthreads();
     }
-    for (int k = n - 1; k >= 0; --k) {
-        const int pr = perm[k];
-        if (pr != k)
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const FT t = A[static_cast<int64_t>(i) * n + k];
-                A[static_cast<int64_t>(i) * n + k] = A[static_cast<int64_t>(i) * n + pr];
-                A[static_cast<int64_t>(i) * n + pr] = t;
+    // undo the row interchanges as column interchanges; each thread owns whole rows
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        FT* row = A + static_cast<int64_t>(i) * ld;
+        for (int k = n - 1; k >= 0; --k) {
+            const int pr = perm[k];
+            if (pr != k) {
+                const FT t = row[k];
+                row[k] = row[pr];
+                row[pr] = t;
             }
-        __syncthreads();
+        }
     }
-    return minpiv;
+    __syncthreads();
 }
 
 struct Metric2 {
@@ -198,7 +234,10 @@ __device__ inline Metric2 metric_of(int dim, int c, const double* verts, const i
     const CellMetric g = cell_metric(dim, c, verts, cells, Mcell);
     Metric2 r;
     r.det = g.det;
-    for (int i = 0; i < 4; ++i) { r.G[i] = g.G[i]; r.H[i] = g.H[i]; }
+    for (int i = 0; i < 4; ++i) {
+        r.G[i] = g.G[i];
+        r.H[i] = g.H[i];
+    }
     return r;
 }
 
@@ -208,241 +247,349 @@ __device__ inline void gather_cell(const EstParams& P, int k, int c, const int* 
                                    const double* x, int m_, int l, double& th, double& uu)
 {
     const int s = cell_free[c * P.Ns + m_];
-    if (s < 0) { th = 0.0; uu = 0.0; return; }
+    if (s < 0) {
+        th = 0.0;
+        uu = 0.0;
+        return;
+    }
     const double lam = lt[(static_cast<int64_t>(k) * P.Nt + l) * P.nf + s];
     const double u = x[P.nY + (static_cast<int64_t>(k) * P.p + l) * P.nf + s];
     th = lam + u;
     uu = u;
 }
 
-constexpr int PATCH_THREADS = 1024;
+constexpr int PATCH_THREADS = 256;
 
+// Phase 1, one workgroup per spatial vertex patch of one size bucket: the FP64
+// patch matrix of one time mode (flux mass, divergence coupling, mean-zero row;
+// dense_local_error_blocks_2d.hpp:719-768, 812-879) into its slot of Kd, exactly
+// symmetric and zero-padded to np x np, and its inverse in FT into the slot of
+// Ki (Gauss-Jordan in LDS when in_lds, else in place in Ki).
 template <typename FT>
-__global__ void __launch_bounds__(PATCH_THREADS) k_patch(EstParams P, EstTables T, PatchTopo topo, const double* verts,
-                                               const int* cells, const double* Mcell, const int* cell_free,
-                                               const double* times, const double* lt, const double* x,
-                                               double* scratch, double* sig, double* patch_res)
+__global__ void __launch_bounds__(PATCH_THREADS)
+    k_patch_factor(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, int in_lds, const double* verts,
+                   const int* cells, const double* Mcell, double* Kd, FT* Ki)
 {
     extern __shared__ double lds[];
-    __shared__ double shv[PATCH_THREADS / 64];
-    __shared__ int shi[PATCH_THREADS / 64];
-    const int Nf = P.Nf, Nq = P.Nq, Nt = P.Nt, Ns = P.Ns, dim = P.dim;
-    double* base = scratch + static_cast<int64_t>(blockIdx.x) * P.scratch_stride;
-    const int nmax = P.nmax;
-    const int C = P.chunk * Nt;
-    double* Kd = base;                                            // nmax^2
-    FT* Ki_g = reinterpret_cast<FT*>(base + static_cast<int64_t>(nmax) * nmax);  // nmax^2
-    double* RHS = base + 2 * static_cast<int64_t>(nmax) * nmax;   // C * nmax (column major)
-    double* X = RHS + static_cast<int64_t>(C) * nmax;
-    double* Rs = X + static_cast<int64_t>(C) * nmax;
-    int* perm = reinterpret_cast<int*>(Rs + static_cast<int64_t>(C) * nmax);        // nmax ints
-    FT* fac = reinterpret_cast<FT*>(Rs + static_cast<int64_t>(C) * nmax + nmax);    // nmax FT
-    double* stage = lds;  // [chunk][Ns][Nt][2]
-    // the inverse lives in LDS when it fits (n <= P.n_lds), else in the scratch
-    FT* Ki_l = reinterpret_cast<FT*>(lds + static_cast<int64_t>(P.chunk) * Ns * Nt * 2);
+    const int Nf = P.Nf, Nq = P.Nq, dim = P.dim;
+    const int q = q0 + blockIdx.x;
+    const int a = topo.order[q];
+    const int n = ps.n[q], np = ps.np[q];
+    const int nF = topo.nflux[a];
+    const bool interior = topo.boundary[a] == 0;
+    const int cb = topo.cell_ptr[a], ce = topo.cell_ptr[a + 1];
+    double* K = Kd + ps.off2[q];
+    FT* Kg = Ki + ps.off2[q];
+    const int64_t nn = static_cast<int64_t>(np) * np;
+    for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) K[e] = 0.0;
+    __syncthreads();
+    for (int ci = cb; ci < ce; ++ci) {
+        const int c = topo.cells[ci];
+        const Metric2 g = metric_of(dim, c, verts, cells, Mcell);
+        const int* fm = topo.fmap + static_cast<int64_t>(ci) * Nf;
+        const int8_t* fs = topo.fsign + static_cast<int64_t>(ci) * Nf;
+        // flux mass: int M^-1 s_i . s_j = sum_ab H_ab RTab_ab[i][j], i <= j mirrored
+        for (int e = threadIdx.x; e < Nf * Nf; e += blockDim.x) {
+            const int i = e / Nf, j = e % Nf;
+            if (j < i) continue;
+            const int fi = fm[i], fj = fm[j];
+            if (fi < 0 || fj < 0) continue;
+            double v = 0.0;
+            for (int ab = 0; ab < dim * dim; ++ab)
+                v += g.H[(ab / dim) * 2 + (ab % dim)] * T.RTab[(static_cast<int64_t>(ab) * Nf + i) * Nf + j];
+            v *= static_cast<double>(fs[i] * fs[j]);
+            K[static_cast<int64_t>(fi) * np + fj] += v;
+            if (i != j) K[static_cast<int64_t>(fj) * np + fi] += v;
+        }
+        // divergence coupling int q_i div s_j (orthonormal scalars) and mean-zero row
+        const int r0 = nF + (ci - cb) * Nq;
+        for (int e = threadIdx.x; e < Nq * Nf; e += blockDim.x) {
+            const int qq = e / Nf, j = e % Nf;
+            const int fj = fm[j];
+            if (fj < 0) continue;
+            const double v = fs[j] * T.Bhat[qq * Nf + j];
+            K[static_cast<int64_t>(r0 + qq) * np + fj] = v;
+            K[static_cast<int64_t>(fj) * np + r0 + qq] = v;
+        }
+        if (interior)
+            for (int qq = threadIdx.x; qq < Nq; qq += blockDim.x) {
+                const double v = g.det * T.qbar[qq];
+                K[static_cast<int64_t>(r0 + qq) * np + (n - 1)] = v;
+                K[static_cast<int64_t>(n - 1) * np + r0 + qq] = v;
+            }
+        __syncthreads();
+    }
+    FT* A = in_lds ? reinterpret_cast<FT*>(lds) : Kg;
+    int* perm = reinterpret_cast<int*>(in_lds ? reinterpret_cast<char*>(lds) + nn * sizeof(FT)
+                                              : reinterpret_cast<char*>(lds));
+    for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) A[e] = static_cast<FT>(K[e]);
+    __syncthreads();
+    gj_invert<FT>(A, n, np, perm);
+    if (in_lds)
+        for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) Kg[e] = A[e];
+}
 
-    for (int item = blockIdx.x; item < P.n_items; item += gridDim.x) {
-        const int a = topo.order[P.per_slab ? item / P.m : item];
-        const int k_begin = P.per_slab ? item % P.m : 0;
-        const int k_end = P.per_slab ? k_begin + 1 : P.m;
-        const int nF = topo.nflux[a], nS = topo.nscal[a];
-        const bool interior = topo.boundary[a] == 0;
-        const int n = nF + nS + (interior ? 1 : 0);
-        const int cb = topo.cell_ptr[a], ce = topo.cell_ptr[a + 1];
-        // ---- build the FP64 patch matrix (one time mode; shared by all modes and slabs)
-        for (int64_t e = threadIdx.x; e < static_cast<int64_t>(n) * n; e += blockDim.x) Kd[e] = 0.0;
-        __syncthreads();
-        for (int ci = cb; ci < ce; ++ci) {
-            const int c = topo.cells[ci];
-            const Metric2 g = metric_of(dim, c, verts, cells, Mcell);
-            const int* fm = topo.fmap + static_cast<int64_t>(ci) * Nf;
-            const int8_t* fs = topo.fsign + static_cast<int64_t>(ci) * Nf;
-            for (int e = threadIdx.x; e < Nf * Nf; e += blockDim.x) {
-                const int i = e / Nf, j = e % Nf;
-                const int fi = fm[i], fj = fm[j];
-                if (fi < 0 || fj < 0) continue;
-                double v = 0.0;
-                for (int ab = 0; ab < dim * dim; ++ab)
-                    v += g.H[(ab / dim) * 2 + (ab % dim)] * T.RTab[(static_cast<int64_t>(ab) * Nf + i) * Nf + j];
-                Kd[static_cast<int64_t>(fi) * n + fj] += static_cast<double>(fs[i] * fs[j]) * v;
+// Phase 2, one workgroup per (patch, group of sg slabs): right-hand sides of
+// the patch problems of every slab and time mode in the group
+// (vec_f_2d.hpp:27-78, vec_g_2d.hpp:30-95), in the Legendre time basis,
+// mode kk scaled by 1 / (h |L_kk|^2), into RHS[off1 * ncols + row * ncols + col],
+// col = slab * Nt + kk.  Flux rows shared by two cells accumulate with LDS
+// atomics; rows n..np-1 stay zero.
+__global__ void __launch_bounds__(PATCH_THREADS)
+    k_patch_rhs(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, int sg, int nsg, int nc_max,
+                const double* verts, const int* cells, const double* Mcell, const int* cell_free, const double* times,
+                const double* lt, const double* x, double* RHS)
+{
+    extern __shared__ double lds[];
+    const int Nf = P.Nf, Nq = P.Nq, Nt = P.Nt, Ns = P.Ns, dim = P.dim;
+    const int q = q0 + blockIdx.x / nsg;
+    const int k0 = (blockIdx.x % nsg) * sg;
+    const int a = topo.order[q];
+    const int np = ps.np[q];
+    const int nsl = min(sg, P.m - k0);
+    const int ncols = P.m * Nt;
+    const int wcol = sg * Nt;
+    const int nF = topo.nflux[a];
+    const int cb = topo.cell_ptr[a], nc = topo.cell_ptr[a + 1] - cb;
+    double* acc = lds;                                              // [np][sg * Nt]
+    double* stage = acc + static_cast<int64_t>(np) * wcol;         // [nc][sg][Ns * Nt][2]
+    double* geo = stage + static_cast<int64_t>(nc_max) * sg * Ns * Nt * 2;  // [nc][4]
+    for (int e = threadIdx.x; e < np * wcol; e += blockDim.x) acc[e] = 0.0;
+    const int per_cell = nsl * Ns * Nt;
+    for (int e = threadIdx.x; e < nc * per_cell; e += blockDim.x) {
+        const int ci = e / per_cell, r = e % per_cell;
+        const int sl = r / (Ns * Nt), r2 = r % (Ns * Nt);
+        const int c = topo.cells[cb + ci];
+        double th, uu;
+        gather_cell(P, k0 + sl, c, cell_free, lt, x, r2 / Nt, r2 % Nt, th, uu);
+        double* s = stage + ((static_cast<int64_t>(ci) * sg + sl) * Ns * Nt + r2) * 2;
+        s[0] = th;
+        s[1] = uu;
+    }
+    for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) {
+        const int c = topo.cells[cb + ci];
+        const int v = topo.lv[cb + ci];
+        const Metric2 g = metric_of(dim, c, verts, cells, Mcell);
+        // grad psi_a pulled through the stiffness metric: gv = G grad^lam_v
+        double gv[2] = {0.0, 0.0};
+        for (int a2 = 0; a2 < dim; ++a2)
+            for (int b = 0; b < dim; ++b) gv[a2] += g.G[a2 * 2 + b] * T.dbary[v * dim + b];
+        geo[ci * 4 + 0] = g.det;
+        geo[ci * 4 + 1] = gv[0];
+        geo[ci * 4 + 2] = gv[1];
+    }
+    __syncthreads();
+    const int rows = (Nf + Nq) * Nt;
+    for (int e = threadIdx.x; e < nc * nsl * rows; e += blockDim.x) {
+        const int ci = e / (nsl * rows), r = e % (nsl * rows);
+        const int sl = r / rows, r2 = r % rows;
+        const int row = r2 / Nt, kk = r2 % Nt;
+        const int v = topo.lv[cb + ci];
+        const int k = k0 + sl;
+        const double h = times[k + 1] - times[k];
+        const double scale = 1.0 / (h * T.Lnorm[kk]);
+        const double* st = stage + (static_cast<int64_t>(ci) * sg + sl) * Ns * Nt * 2;
+        const double det = geo[ci * 4 + 0];
+        if (row < Nf) {
+            const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + row];
+            if (fi < 0) continue;
+            // F = -h R_v Theta W
+            double s = 0.0;
+            for (int mm = 0; mm < Ns; ++mm) {
+                double tw = 0.0;
+                for (int l = 0; l < Nt; ++l) tw += st[(mm * Nt + l) * 2] * T.W[l * Nt + kk];
+                s += T.R[(static_cast<int64_t>(v) * Nf + row) * Ns + mm] * tw;
             }
-            const int r0 = nF + (ci - cb) * Nq;
-            for (int e = threadIdx.x; e < Nq * Nf; e += blockDim.x) {
-                const int q = e / Nf, j = e % Nf;
-                const int fj = fm[j];
-                if (fj < 0) continue;
-                const double v = fs[j] * T.Bhat[q * Nf + j];
-                Kd[static_cast<int64_t>(r0 + q) * n + fj] = v;
-                Kd[static_cast<int64_t>(fj) * n + r0 + q] = v;
-            }
-            if (interior)
-                for (int q = threadIdx.x; q < Nq; q += blockDim.x) {
-                    const double v = g.det * T.qbar[q];
-                    Kd[static_cast<int64_t>(r0 + q) * n + (n - 1)] = v;
-                    Kd[static_cast<int64_t>(n - 1) * n + r0 + q] = v;
+            const int8_t sgn = topo.fsign[static_cast<int64_t>(cb + ci) * Nf + row];
+            atomicAdd(&acc[fi * wcol + sl * Nt + kk], -h * sgn * s * scale);
+        } else {
+            const int qq = row - Nf;
+            double s = (kk == 0) ? P.ell * det * h * T.Pv[v * Nq + qq] : 0.0;
+            double du = 0.0, dth = 0.0;
+            for (int mm = 0; mm < Ns; ++mm) {
+                double uw = 0.0, tw = 0.0;
+                for (int l = 0; l < Nt; ++l) {
+                    uw += st[(mm * Nt + l) * 2 + 1] * T.Wd[l * Nt + kk];
+                    tw += st[(mm * Nt + l) * 2] * T.W[l * Nt + kk];
                 }
-            __syncthreads();
+                du += T.Q[(static_cast<int64_t>(v) * Nq + qq) * Ns + mm] * uw;
+                double eb = 0.0;
+                for (int b = 0; b < dim; ++b)
+                    eb += geo[ci * 4 + 1 + b] * T.E[(static_cast<int64_t>(b) * Nq + qq) * Ns + mm];
+                dth += eb * tw;
+            }
+            s += -det * du - h * dth;
+            acc[(nF + ci * Nq + qq) * wcol + sl * Nt + kk] = s * scale;
         }
-        // ---- invert in FT
-        FT* Ki = (n <= P.n_lds) ? Ki_l : Ki_g;
-        for (int64_t e = threadIdx.x; e < static_cast<int64_t>(n) * n; e += blockDim.x) Ki[e] = static_cast<FT>(Kd[e]);
+    }
+    __syncthreads();
+    const int ncw = nsl * Nt;
+    double* out = RHS + ps.off1[q] * ncols + static_cast<int64_t>(k0) * Nt;
+    for (int e = threadIdx.x; e < np * ncw; e += blockDim.x) {
+        const int row = e / ncw, c = e % ncw;
+        out[static_cast<int64_t>(row) * ncols + c] = acc[row * wcol + c];
+    }
+}
+
+// Phase 3, one workgroup per (patch, block of whole slabs): X = Ki B on MFMA in
+// FT, then refine_steps rounds of R = B - Kd X (FP64 MFMA) and X += Ki R (FT);
+// the residual of the final X gives the acceptance statistic
+// |K x - b|_inf / max(1, |b|_inf) per (slab, vertex) problem
+// (dense_local_error_blocks_2d.hpp:923-949).  Both matrices are symmetric, so
+// the MFMA A operand A[i][k] is read as M[k][i] (coalesced over i).  The flux
+// coefficients go to sig[(cell, local vertex)].
+template <typename FT>
+__global__ void __launch_bounds__(PATCH_THREADS)
+    k_patch_solve(EstParams P, PatchTopo topo, PatchSlots ps, int q0, int ncg, int cpw, const double* Kd, const FT* Ki,
+                  const double* RHS, double* sig, double* patch_res)
+{
+    using V4 = typename Vec4<FT>::type;
+    extern __shared__ double lds[];
+    __shared__ unsigned long long colmax[64];  // [0, 32): |R| per column, [32, 64): |b| per column
+    const int Nf = P.Nf, Nt = P.Nt;
+    const int q = q0 + blockIdx.x / ncg;
+    const int c0 = (blockIdx.x % ncg) * cpw;
+    const int a = topo.order[q];
+    const int np = ps.np[q];
+    const int ncols = P.m * Nt;
+    const int ncw = min(cpw, ncols - c0);
+    const double* Kq = Kd + ps.off2[q];
+    const FT* Iq = Ki + ps.off2[q];
+    const double* Bq = RHS + ps.off1[q] * ncols + c0;
+    double* Xs = lds;                                                  // [np][32]
+    FT* Rs = reinterpret_cast<FT*>(Xs + static_cast<int64_t>(np) * 32);  // [np][32]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nrt = np >> 4;
+    for (int e = threadIdx.x; e < np * 32; e += blockDim.x) {
+        const int r = e >> 5, c = e & 31;
+        Rs[e] = (c < ncw) ? static_cast<FT>(Bq[static_cast<int64_t>(r) * ncols + c]) : static_cast<FT>(0);
+    }
+    if (threadIdx.x < 64) colmax[threadIdx.x] = 0ull;
+    __syncthreads();
+    // X = Ki B
+    for (int rt = wave; rt < nrt; rt += nw) {
+        V4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
+        const FT* ip = Iq + rt * 16 + li;
+        for (int kb = 0; kb < np; kb += 16)
+#pragma unroll
+            for (int k = kb; k < kb + 16; k += 4) {
+                const FT av = ip[static_cast<int64_t>(k + lk) * np];
+                d0 = mfma_16x16x4(av, Rs[(k + lk) * 32 + li], d0);
+                d1 = mfma_16x16x4(av, Rs[(k + lk) * 32 + 16 + li], d1);
+            }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rt * 16 + drow<FT>(lk, r);
+            Xs[row * 32 + li] = static_cast<double>(d0[r]);
+            Xs[row * 32 + 16 + li] = static_cast<double>(d1[r]);
+        }
+    }
+    __syncthreads();
+    const int steps = P.refine_steps;
+    for (int it = 0;; ++it) {
+        const bool last = it == steps;
+        // R = B - Kd X in FP64
+        for (int rt = wave; rt < nrt; rt += nw) {
+            v4d d0, d1;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rt * 16 + drow<double>(lk, r);
+                d0[r] = (li < ncw) ? Bq[static_cast<int64_t>(row) * ncols + li] : 0.0;
+                d1[r] = (16 + li < ncw) ? Bq[static_cast<int64_t>(row) * ncols + 16 + li] : 0.0;
+            }
+            double bm0 = 0.0, bm1 = 0.0;
+            if (last) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    bm0 = fmax(bm0, fabs(d0[r]));
+                    bm1 = fmax(bm1, fabs(d1[r]));
+                }
+            }
+            const double* kp = Kq + rt * 16 + li;
+            for (int kb = 0; kb < np; kb += 16)
+#pragma unroll
+                for (int k = kb; k < kb + 16; k += 4) {
+                    const double av = -kp[static_cast<int64_t>(k + lk) * np];
+                    d0 = mfma_16x16x4(av, Xs[(k + lk) * 32 + li], d0);
+                    d1 = mfma_16x16x4(av, Xs[(k + lk) * 32 + 16 + li], d1);
+                }
+            if (last) {
+                double rm0 = 0.0, rm1 = 0.0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    rm0 = fmax(rm0, fabs(d0[r]));
+                    rm1 = fmax(rm1, fabs(d1[r]));
+                }
+                for (int off = 16; off < 64; off <<= 1) {
+                    rm0 = fmax(rm0, __shfl_xor(rm0, off, 64));
+                    rm1 = fmax(rm1, __shfl_xor(rm1, off, 64));
+                    bm0 = fmax(bm0, __shfl_xor(bm0, off, 64));
+                    bm1 = fmax(bm1, __shfl_xor(bm1, off, 64));
+                }
+                if (lk == 0) {
+                    // non-negative doubles order like their bit patterns
+                    atomicMax(&colmax[li], static_cast<unsigned long long>(__double_as_longlong(rm0)));
+                    atomicMax(&colmax[16 + li], static_cast<unsigned long long>(__double_as_longlong(rm1)));
+                    atomicMax(&colmax[32 + li], static_cast<unsigned long long>(__double_as_longlong(bm0)));
+                    atomicMax(&colmax[48 + li], static_cast<unsigned long long>(__double_as_longlong(bm1)));
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = rt * 16 + drow<double>(lk, r);
+                    Rs[row * 32 + li] = static_cast<FT>(d0[r]);
+                    Rs[row * 32 + 16 + li] = static_cast<FT>(d1[r]);
+                }
+            }
+        }
         __syncthreads();
-        gauss_jordan_inverse<FT>(Ki, n, perm, fac, shv, shi);
-        // ---- right-hand sides, solve, refine, write contributions, chunk of slabs at a time
+        if (last) break;
+        // X += Ki R in FT
+        for (int rt = wave; rt < nrt; rt += nw) {
+            V4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
+            const FT* ip = Iq + rt * 16 + li;
+            for (int kb = 0; kb < np; kb += 16)
+#pragma unroll
+                for (int k = kb; k < kb + 16; k += 4) {
+                    const FT av = ip[static_cast<int64_t>(k + lk) * np];
+                    d0 = mfma_16x16x4(av, Rs[(k + lk) * 32 + li], d0);
+                    d1 = mfma_16x16x4(av, Rs[(k + lk) * 32 + 16 + li], d1);
+                }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rt * 16 + drow<FT>(lk, r);
+                Xs[row * 32 + li] += static_cast<double>(d0[r]);
+                Xs[row * 32 + 16 + li] += static_cast<double>(d1[r]);
+            }
+        }
+        __syncthreads();
+    }
+    // acceptance per (slab, vertex) problem: the Nt mode columns of one slab
+    if (threadIdx.x == 0) {
         double worst = 0.0;
-        for (int k0 = k_begin; k0 < k_end; k0 += P.chunk) {
-            const int nsl = min(P.chunk, k_end - k0);
-            const int ncol = nsl * Nt;
-            for (int64_t e = threadIdx.x; e < static_cast<int64_t>(ncol) * n; e += blockDim.x) RHS[e] = 0.0;
-            __syncthreads();
-            for (int ci = cb; ci < ce; ++ci) {
-                const int c = topo.cells[ci];
-                const int v = topo.lv[ci];
-                const Metric2 g = metric_of(dim, c, verts, cells, Mcell);
-                const int* fm = topo.fmap + static_cast<int64_t>(ci) * Nf;
-                const int8_t* fs = topo.fsign + static_cast<int64_t>(ci) * Nf;
-                // stage theta and u of this cell for every slab of the chunk
-                for (int e = threadIdx.x; e < nsl * Ns * Nt; e += blockDim.x) {
-                    const int sl = e / (Ns * Nt), r = e % (Ns * Nt);
-                    const int mm = r / Nt, l = r % Nt;
-                    double th, uu;
-                    gather_cell(P, k0 + sl, c, cell_free, lt, x, mm, l, th, uu);
-                    stage[e * 2] = th;
-                    stage[e * 2 + 1] = uu;
-                }
-                __syncthreads();
-                // grad psi_a pulled through the stiffness metric: gv = G grad^lam_v
-                double gv[2] = {0.0, 0.0};
-                for (int a2 = 0; a2 < dim; ++a2)
-                    for (int b = 0; b < dim; ++b) gv[a2] += g.G[a2 * 2 + b] * T.dbary[v * dim + b];
-                const int r0 = nF + (ci - cb) * Nq;
-                const int nwork = nsl * (Nf + Nq) * Nt;
-                for (int e = threadIdx.x; e < nwork; e += blockDim.x) {
-                    const int sl = e / ((Nf + Nq) * Nt);
-                    const int r = e % ((Nf + Nq) * Nt);
-                    const int row = r / Nt, kk = r % Nt;
-                    const int k = k0 + sl;
-                    const double h = times[k + 1] - times[k];
-                    const double scale = 1.0 / (h * T.Lnorm[kk]);
-                    const double* st = stage + static_cast<int64_t>(sl) * Ns * Nt * 2;
-                    if (row < Nf) {
-                        const int fi = fm[row];
-                        if (fi < 0) continue;
-                        // F = -h R_v Theta W
-                        double acc = 0.0;
-                        for (int mm = 0; mm < Ns; ++mm) {
-                            double tw = 0.0;
-                            for (int l = 0; l < Nt; ++l) tw += st[(mm * Nt + l) * 2] * T.W[l * Nt + kk];
-                            acc += T.R[(static_cast<int64_t>(v) * Nf + row) * Ns + mm] * tw;
-                        }
-                        RHS[static_cast<int64_t>(sl * Nt + kk) * n + fi] += -h * fs[row] * acc * scale;
-                    } else {
-                        const int q = row - Nf;
-                        double acc = (kk == 0) ? P.ell * g.det * h * T.Pv[v * Nq + q] : 0.0;
-                        double du = 0.0, dth = 0.0;
-                        for (int mm = 0; mm < Ns; ++mm) {
-                            double uw = 0.0, tw = 0.0;
-                            for (int l = 0; l < Nt; ++l) {
-                                uw += st[(mm * Nt + l) * 2 + 1] * T.Wd[l * Nt + kk];
-                                tw += st[(mm * Nt + l) * 2] * T.W[l * Nt + kk];
-                            }
-                            du += T.Q[(static_cast<int64_t>(v) * Nq + q) * Ns + mm] * uw;
-                            double eb = 0.0;
-                            for (int b = 0; b < dim; ++b) eb += gv[b] * T.E[(static_cast<int64_t>(b) * Nq + q) * Ns + mm];
-                            dth += eb * tw;
-                        }
-                        acc += -g.det * du - h * dth;
-                        RHS[static_cast<int64_t>(sl * Nt + kk) * n + r0 + q] = acc * scale;
-                    }
-                }
-                __syncthreads();
+        for (int s0 = 0; s0 < ncw; s0 += Nt) {
+            double rm = 0.0, bm = 0.0;
+            for (int c = s0; c < s0 + Nt; ++c) {
+                rm = fmax(rm, __longlong_as_double(static_cast<long long>(colmax[c])));
+                bm = fmax(bm, __longlong_as_double(static_cast<long long>(colmax[32 + c])));
             }
-            // ---- X = Ki RHS, residual Rs = RHS - Kd X (FP64), FT corrections.
-            // Both matrices are symmetric: reading column i as row i keeps the
-            // loads coalesced; each thread owns one row i and 4 columns.
-            const int ncg = (ncol + 3) / 4;
-            for (int e = threadIdx.x; e < ncg * n; e += blockDim.x) {
-                const int cg = e / n, i = e % n;
-                const int c0 = cg * 4;
-                FT acc[4] = {0, 0, 0, 0};
-                for (int j = 0; j < n; ++j) {
-                    const FT kij = Ki[static_cast<int64_t>(j) * n + i];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        if (c0 + q < ncol) acc[q] += kij * static_cast<FT>(RHS[static_cast<int64_t>(c0 + q) * n + j]);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (c0 + q < ncol) X[static_cast<int64_t>(c0 + q) * n + i] = static_cast<double>(acc[q]);
-            }
-            __syncthreads();
-            const int steps = P.refine_steps;
-            for (int it = 0; it <= steps; ++it) {
-                // residual in FP64
-                for (int e = threadIdx.x; e < ncg * n; e += blockDim.x) {
-                    const int cg = e / n, i = e % n;
-                    const int c0 = cg * 4;
-                    double acc[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[q] = (c0 + q < ncol) ? RHS[static_cast<int64_t>(c0 + q) * n + i] : 0.0;
-                    for (int j = 0; j < n; ++j) {
-                        const double kij = Kd[static_cast<int64_t>(j) * n + i];
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            if (c0 + q < ncol) acc[q] -= kij * X[static_cast<int64_t>(c0 + q) * n + j];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        if (c0 + q < ncol) Rs[static_cast<int64_t>(c0 + q) * n + i] = acc[q];
-                }
-                __syncthreads();
-                if (it == steps) break;
-                // correction in FT
-                for (int e = threadIdx.x; e < ncg * n; e += blockDim.x) {
-                    const int cg = e / n, i = e % n;
-                    const int c0 = cg * 4;
-                    FT acc[4] = {0, 0, 0, 0};
-                    for (int j = 0; j < n; ++j) {
-                        const FT kij = Ki[static_cast<int64_t>(j) * n + i];
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            if (c0 + q < ncol) acc[q] += kij * static_cast<FT>(Rs[static_cast<int64_t>(c0 + q) * n + j]);
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        if (c0 + q < ncol) X[static_cast<int64_t>(c0 + q) * n + i] += static_cast<double>(acc[q]);
-                }
-                __syncthreads();
-            }
-            // acceptance statistic: |K x - b|_inf / max(1, |b|_inf)
-            // (dense_local_error_blocks_2d.hpp:923-949)
-            double rmax = 0.0, bmax = 0.0;
-            for (int64_t e = threadIdx.x; e < static_cast<int64_t>(ncol) * n; e += blockDim.x) {
-                rmax = fmax(rmax, fabs(Rs[e]));
-                bmax = fmax(bmax, fabs(RHS[e]));
-            }
-            rmax = block_max(rmax, shv);
-            bmax = block_max(bmax, shv);
-            worst = fmax(worst, rmax / fmax(1.0, bmax));
-            // ---- write per (space-time cell, local vertex) flux coefficients
-            for (int ci = cb; ci < ce; ++ci) {
-                const int c = topo.cells[ci];
-                const int v = topo.lv[ci];
-                const int* fm = topo.fmap + static_cast<int64_t>(ci) * Nf;
-                const int8_t* fs = topo.fsign + static_cast<int64_t>(ci) * Nf;
-                for (int e = threadIdx.x; e < nsl * Nf * Nt; e += blockDim.x) {
-                    const int sl = e / (Nf * Nt), r = e % (Nf * Nt);
-                    const int i = r / Nt, kk = r % Nt;
-                    const int fi = fm[i];
-                    const double val = (fi >= 0) ? fs[i] * X[static_cast<int64_t>(sl * Nt + kk) * n + fi] : 0.0;
-                    const int64_t cell = static_cast<int64_t>(k0 + sl) * P.ns + c;
-                    sig[((cell * P.NV + v) * Nf + i) * Nt + kk] = val;
-                }
-            }
-            __syncthreads();
+            worst = fmax(worst, rm / fmax(1.0, bm));
         }
-        if (threadIdx.x == 0) atomic_max_pos(&patch_res[a], worst);
-        __syncthreads();
+        atomic_max_pos(&patch_res[a], worst);
+    }
+    // flux coefficients per (space-time cell, local vertex)
+    const int cb = topo.cell_ptr[a], nc = topo.cell_ptr[a + 1] - cb;
+    for (int e = threadIdx.x; e < nc * Nf * ncw; e += blockDim.x) {
+        const int ci = e / (Nf * ncw), r = e % (Nf * ncw);
+        const int i = r / ncw, c = r % ncw;
+        const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + i];
+        const double val =
+            (fi >= 0) ? topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i] * Xs[fi * 32 + c] : 0.0;
+        const int col = c0 + c;
+        const int k = col / Nt, kk = col % Nt;
+        const int64_t cell = static_cast<int64_t>(k) * P.ns + topo.cells[cb + ci];
+        sig[((cell * P.NV + topo.lv[cb + ci]) * Nf + i) * Nt + kk] = val;
     }
 }
 
@@ -652,16 +799,44 @@ void estimator_alloc(System* sys)
     B.p_nscal = upi(P.n_scalar);
     B.p_boundary = dalloc<uint8_t>(sys, P.boundary.size());
     HIP_TRY(hipMemcpy(B.p_boundary, P.boundary.data(), P.boundary.size(), hipMemcpyHostToDevice));
-    // largest patches first for load balance
+    // processing order: largest patches first; slots padded to multiples of
+    // 16 (the MFMA tile), one size bucket per padded order
+    std::vector<int> pn(P.n_patches);
+    for (int a = 0; a < P.n_patches; ++a) pn[a] = P.n_flux[a] + P.n_scalar[a] + (P.boundary[a] ? 0 : 1);
     std::vector<int> order(P.n_patches);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return P.n_flux[a] + P.n_scalar[a] > P.n_flux[b] + P.n_scalar[b];
-    });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pn[a] > pn[b]; });
     B.order = upi(order);
-    int nmax = 0;
-    for (int a = 0; a < P.n_patches; ++a) nmax = std::max(nmax, P.n_flux[a] + P.n_scalar[a] + 1);
-    B.nmax = nmax;
+    std::vector<int> sn(P.n_patches), snp(P.n_patches);
+    std::vector<int64_t> off2(P.n_patches), off1(P.n_patches);
+    int64_t o2 = 0, o1 = 0;
+    B.buckets.clear();
+    for (int q = 0; q < P.n_patches; ++q) {
+        const int a = order[q];
+        sn[q] = pn[a];
+        snp[q] = (pn[a] + 15) / 16 * 16;
+        off2[q] = o2;
+        off1[q] = o1;
+        o2 += static_cast<int64_t>(snp[q]) * snp[q];
+        o1 += snp[q];
+        const int nc = P.cell_ptr[a + 1] - P.cell_ptr[a];
+        if (B.buckets.empty() || B.buckets.back().np != snp[q]) B.buckets.push_back({q, 0, snp[q], 0});
+        B.buckets.back().count++;
+        B.buckets.back().nc_max = std::max(B.buckets.back().nc_max, nc);
+    }
+    B.s_n = upi(sn);
+    B.s_np = upi(snp);
+    auto up64 = [&](const std::vector<int64_t>& v) {
+        int64_t* p = dalloc<int64_t>(sys, v.size());
+        if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        return p;
+    };
+    B.s_off2 = up64(off2);
+    B.s_off1 = up64(off1);
+    B.nmax = P.n_patches ? snp[0] : 0;
+    B.Kd = dalloc<double>(sys, static_cast<size_t>(o2));
+    B.Ki = dalloc<double>(sys, static_cast<size_t>(o2));
+    B.RHS = dalloc<double>(sys, static_cast<size_t>(o1) * S.mesh.m() * T.Nt);
     const int64_t ncell = static_cast<int64_t>(S.mesh.m()) * S.mesh.ns();
     B.sig = dalloc<double>(sys, static_cast<size_t>(ncell) * (S.dim + 1) * T.Nf * T.Nt);
     B.flux = dalloc<double>(sys, ncell);
@@ -672,10 +847,6 @@ void estimator_alloc(System* sys)
     B.patch_res = dalloc<double>(sys, P.n_patches);
     B.lt = dalloc<double>(sys, S.nY);
     B.sums = dalloc<double>(sys, 8);
-    B.chunk = std::min(S.mesh.m(), 32);
-    const int C = B.chunk * T.Nt;
-    B.scratch_per_wg = 2 * static_cast<size_t>(nmax) * nmax + 3 * static_cast<size_t>(C) * nmax + 2 * nmax + 8;
-    B.n_wg = 0;  // allocated on first use
 }
 
 }  // namespace apf
@@ -702,14 +873,6 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
     try {
         HIP_TRY(hipSetDevice(sys->ctx->device));
         hipStream_t st = sys->stream;
-        if (B.n_wg == 0) {
-            int dev = sys->ctx->device, cus = 0;
-            HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-            const int64_t items = S.patches.n_patches;
-            B.n_wg = static_cast<int>(std::min<int64_t>(items, 4LL * cus));
-            const size_t bytes = B.scratch_per_wg * sizeof(double) * B.n_wg;
-            B.scratch = dalloc<double>(sys, bytes / sizeof(double));
-        }
         EstParams P{};
         P.dim = S.dim;
         P.p = S.p;
@@ -725,11 +888,6 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         P.nf = S.nf;
         P.nY = S.nY;
         P.ell = sys->ell;
-        P.per_slab = 0;
-        P.n_items = S.patches.n_patches;
-        P.chunk = B.chunk;
-        P.scratch_stride = static_cast<int64_t>(B.scratch_per_wg);
-        P.nmax = B.nmax;
         P.refine_steps = (prec == APF_FP32_REFINE) ? refine_steps : 0;
         EstTables Tb{B.RTab, B.Bhat, B.qbar, B.R, B.Pv, B.Q, B.E, B.Phi, B.Khat, B.dbary, B.Wt, B.Wdt, B.Lnorm_, B.Mt,
                      B.Igrad, sys->d_bw, sys->d_bphi};
@@ -739,28 +897,54 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         // slab by slab; with the main solve converged it equals lambda.
         HIP_TRY(hipMemcpyAsync(B.lt, sys->d_x, S.nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIP_TRY(hipMemsetAsync(B.patch_res, 0, S.patches.n_patches * sizeof(double), st));
-        const size_t stage_bytes = static_cast<size_t>(B.chunk) * S.Ns * S.Nt * 2 * sizeof(double);
         const size_t ft_bytes = (prec == APF_FP64) ? sizeof(double) : sizeof(float);
-        const size_t lds_cap = 150 * 1024;
-        int n_lds = static_cast<int>(std::sqrt(static_cast<double>(lds_cap - stage_bytes) / ft_bytes));
-        n_lds = std::min(n_lds, B.nmax);
-        P.n_lds = n_lds;
-        const size_t lds = stage_bytes + static_cast<size_t>(n_lds) * n_lds * ft_bytes;
-        if (prec == APF_FP64)
-            HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_patch<double>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        else
-            HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_patch<float>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        const size_t lds_cap = 160 * 1024;
+        if (static_cast<size_t>(B.nmax) * 32 * (sizeof(double) + ft_bytes) > lds_cap)
+            return fail(APF_BAD_ARG, "apf_estimate: patch order " + std::to_string(B.nmax) +
+                                         " exceeds the batched patch-solve limit of this build");
+        PatchSlots ps{B.s_n, B.s_np, B.s_off2, B.s_off1};
+        const int ncols = P.m * P.Nt;
+        const int cpw = (32 / P.Nt) * P.Nt;
+        const int ncg = (ncols + cpw - 1) / cpw;
+        auto set_lds = [&](const void* fn, size_t bytes) {
+            HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+        };
+        auto launch = [&](auto tag) {
+            using FT = decltype(tag);
+            FT* Ki = reinterpret_cast<FT*>(B.Ki);
+            for (const auto& bk : B.buckets) {
+                const size_t npb = static_cast<size_t>(bk.np);
+                const size_t in_lds_bytes = npb * npb * sizeof(FT) + npb * sizeof(int);
+                const int in_lds = in_lds_bytes <= lds_cap ? 1 : 0;
+                const size_t lf = in_lds ? in_lds_bytes : npb * sizeof(int);
+                set_lds(reinterpret_cast<const void*>(&k_patch_factor<FT>), lf);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor<FT>), dim3(bk.count), dim3(PATCH_THREADS), lf, st, P,
+                                   Tb, topo, ps, bk.q0, in_lds, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
+            }
+            for (const auto& bk : B.buckets) {
+                int sg = std::min(8, P.m);
+                auto lr = [&](int g) {
+                    return (static_cast<size_t>(bk.np) * g * P.Nt +
+                            static_cast<size_t>(bk.nc_max) * g * P.Ns * P.Nt * 2 + bk.nc_max * 4) * sizeof(double);
+                };
+                while (sg > 1 && lr(sg) > 64 * 1024) --sg;
+                if (lr(sg) > lds_cap) throw std::runtime_error("patch right-hand side block exceeds LDS");
+                const int nsg = (P.m + sg - 1) / sg;
+                set_lds(reinterpret_cast<const void*>(&k_patch_rhs), lr(sg));
+                hipLaunchKernelGGL(k_patch_rhs, dim3(bk.count * nsg), dim3(PATCH_THREADS), lr(sg), st, P, Tb, topo, ps,
+                                   bk.q0, sg, nsg, bk.nc_max, sys->d_verts, sys->d_cells, sys->d_Mcell,
+                                   sys->d_cell_free, sys->d_times, B.lt, sys->d_x, B.RHS);
+            }
+            for (const auto& bk : B.buckets) {
+                const size_t ls = static_cast<size_t>(bk.np) * 32 * (sizeof(double) + sizeof(FT));
+                set_lds(reinterpret_cast<const void*>(&k_patch_solve<FT>), ls);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_solve<FT>), dim3(bk.count * ncg), dim3(PATCH_THREADS), ls,
+                                   st, P, topo, ps, bk.q0, ncg, cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res);
+            }
+        };
         HIP_TRY(hipEventRecord(sys->ev0, st));
-        if (prec == APF_FP64)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch<double>), dim3(B.n_wg), dim3(PATCH_THREADS), lds, st, P, Tb, topo,
-                               sys->d_verts, sys->d_cells, sys->d_Mcell, sys->d_cell_free, sys->d_times, B.lt,
-                               sys->d_x, B.scratch, B.sig, B.patch_res);
-        else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch<float>), dim3(B.n_wg), dim3(PATCH_THREADS), lds, st, P, Tb, topo,
-                               sys->d_verts, sys->d_cells, sys->d_Mcell, sys->d_cell_free, sys->d_times, B.lt,
-                               sys->d_x, B.scratch, B.sig, B.patch_res);
+        if (prec == APF_FP64) launch(double{});
+        else launch(float{});
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(sys->ev1, st));
         const int64_t ncell = static_cast<int64_t>(P.m) * P.ns;
@@ -808,5 +992,7 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         return APF_OK;
     } catch (const HipFailure& e) {
         return fail(APF_HIP_ERROR, std::string("apf_estimate: ") + e.what());
+    } catch (const std::exception& e) {
+        return fail(APF_BAD_ARG, std::string("apf_estimate: ") + e.what());
     }
 }

@@ -143,12 +143,19 @@ struct EstimatorBuffers {
     double* patch_res = nullptr;  // per patch max residual
     double* lt = nullptr;         // slab-reconstructed lambda
     double* sums = nullptr;
-    // scratch per resident workgroup
-    double* scratch = nullptr;
-    size_t scratch_per_wg = 0;
-    int n_wg = 0;
+    // patch slots in processing order (largest first) and size buckets
+    int* s_n = nullptr;
+    int* s_np = nullptr;
+    int64_t* s_off2 = nullptr;
+    int64_t* s_off1 = nullptr;
+    double* Kd = nullptr;    // FP64 patch matrices, np x np per patch
+    double* Ki = nullptr;    // inverses (FP64, or FP32 in the first half of each slot)
+    double* RHS = nullptr;   // right-hand sides, np x (slabs * Nt) per patch
+    struct Bucket {
+        int q0, count, np, nc_max;
+    };
+    std::vector<Bucket> buckets;
     int nmax = 0;
-    int chunk = 0;
 };
 
 struct System {
