@@ -33,6 +33,7 @@
 #include <cmath>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/apf.h"
@@ -260,26 +261,16 @@ __device__ inline void gather_cell(const EstParams& P, int k, int c, const int* 
 
 constexpr int PATCH_THREADS = 256;
 
-// Phase 1, one workgroup per spatial vertex patch of one size bucket: the FP64
-// patch matrix of one time mode (flux mass, divergence coupling, mean-zero row;
-// dense_local_error_blocks_2d.hpp:719-768, 812-879) into its slot of Kd, exactly
-// symmetric and zero-padded to np x np, and its inverse in FT into the slot of
-// Ki (Gauss-Jordan in LDS when in_lds, else in place in Ki).
-template <typename FT>
-__global__ void __launch_bounds__(PATCH_THREADS)
-    k_patch_factor(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, int in_lds, const double* verts,
-                   const int* cells, const double* Mcell, double* Kd, FT* Ki)
+// FP64 patch matrix of one time mode (flux mass, divergence coupling,
+// mean-zero row; dense_local_error_blocks_2d.hpp:719-768, 812-879) of patch a
+// into K (np x np, zero padded, exactly symmetric).  Ends with a barrier.
+__device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const PatchTopo& topo, int a, int n, int np,
+                                   const double* verts, const int* cells, const double* Mcell, double* K)
 {
-    extern __shared__ double lds[];
     const int Nf = P.Nf, Nq = P.Nq, dim = P.dim;
-    const int q = q0 + blockIdx.x;
-    const int a = topo.order[q];
-    const int n = ps.n[q], np = ps.np[q];
     const int nF = topo.nflux[a];
     const bool interior = topo.boundary[a] == 0;
     const int cb = topo.cell_ptr[a], ce = topo.cell_ptr[a + 1];
-    double* K = Kd + ps.off2[q];
-    FT* Kg = Ki + ps.off2[q];
     const int64_t nn = static_cast<int64_t>(np) * np;
     for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) K[e] = 0.0;
     __syncthreads();
@@ -319,6 +310,24 @@ __global__ void __launch_bounds__(PATCH_THREADS)
             }
         __syncthreads();
     }
+}
+
+// Phase 1 (large patches), one workgroup per spatial vertex patch of one size
+// bucket: the patch matrix into its slot of Kd and its inverse in FT into the
+// slot of Ki (Gauss-Jordan in LDS when in_lds, else in place in Ki).
+template <typename FT>
+__global__ void __launch_bounds__(PATCH_THREADS)
+    k_patch_factor(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, int in_lds, const double* verts,
+                   const int* cells, const double* Mcell, double* Kd, FT* Ki)
+{
+    extern __shared__ double lds[];
+    const int q = q0 + blockIdx.x;
+    const int a = topo.order[q];
+    const int n = ps.n[q], np = ps.np[q];
+    double* K = Kd + ps.off2[q];
+    FT* Kg = Ki + ps.off2[q];
+    const int64_t nn = static_cast<int64_t>(np) * np;
+    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
     FT* A = in_lds ? reinterpret_cast<FT*>(lds) : Kg;
     int* perm = reinterpret_cast<int*>(in_lds ? reinterpret_cast<char*>(lds) + nn * sizeof(FT)
                                               : reinterpret_cast<char*>(lds));
@@ -327,6 +336,165 @@ __global__ void __launch_bounds__(PATCH_THREADS)
     gj_invert<FT>(A, n, np, perm);
     if (in_lds)
         for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) Kg[e] = A[e];
+}
+
+__device__ inline float lane_value(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ inline double lane_value(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), src));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), src));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+__device__ inline double first_lane(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b)));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32)));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Phase 1 (patches of order <= min(NW * RPW, 64 * CPL)): the patch matrix
+// lives in registers, row i = wave + NW il in A[il][.], column
+// j = lane + 64 t in A[.][t].  Gauss-Jordan with partial pivoting among the
+// rows not yet used as pivots (ties to the lowest row) and no row interchange:
+// step k scales pivot row r_k, broadcasts it through LDS and eliminates column
+// k from every other row, the multiplier A[i][k] coming from lane k % 64 by
+// readlane.  After n steps A holds T with inv(K)[k][r_k'] = T[r_k][k'], which is
+// how the inverse is written to the Ki slot.  Two barriers per step; the
+// pivot candidates of column k+1 are found while column k is eliminated.
+template <typename FT, int NW, int RPW, int CPL>
+__global__ void __launch_bounds__(NW * 64)
+    k_patch_factor_reg(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, const double* verts,
+                       const int* cells, const double* Mcell, double* Kd, FT* Ki)
+{
+    __shared__ FT prow_s[64 * CPL];
+    __shared__ double cabs[NW], cval[NW];
+    __shared__ int cidx[NW];
+    __shared__ int pivrow[64 * CPL], rinv[64 * CPL];
+    const int q = q0 + blockIdx.x;
+    const int a = topo.order[q];
+    const int n = ps.n[q], np = ps.np[q];
+    double* K = Kd + ps.off2[q];
+    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
+    // wave, pivot row and multipliers are wave-uniform: keep them in scalar registers
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    FT A[RPW][CPL];
+    unsigned used = 0;  // bit il: row wave + NW il is a pivot row already, or padding
+#pragma unroll
+    for (int il = 0; il < RPW; ++il) {
+        const int i = wave + NW * il;
+        if (i >= n) used |= 1u << il;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int j = lane + 64 * t;
+            A[il][t] = (i < n && j < n) ? static_cast<FT>(K[static_cast<int64_t>(i) * np + j]) : static_cast<FT>(0);
+        }
+    }
+    // pivot candidate of this wave for column kc: largest |A[i][kc]| over unused rows
+    auto candidates = [&](int kc) {
+        const int tk = kc >> 6, lk = kc & 63;
+        double best = -1.0, bval = 0.0;
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int il = 0; il < RPW; ++il) {
+            if (used & (1u << il)) continue;
+            FT v = 0;
+#pragma unroll
+            for (int t = 0; t < CPL; ++t)
+                if (t == tk) v = A[il][t];
+            const double av = fabs(static_cast<double>(v));
+            if (av > best) {
+                best = av;
+                bval = static_cast<double>(v);
+                bi = wave + NW * il;
+            }
+        }
+        best = lane_value(best, lk);
+        bval = lane_value(bval, lk);
+        bi = __builtin_amdgcn_readlane(bi, lk);
+        if (lane == 0) {
+            cabs[wave] = best;
+            cval[wave] = bval;
+            cidx[wave] = bi;
+        }
+    };
+    candidates(0);
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        double best = -1.0, pv = 0.0;
+        int r = 0x7fffffff;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const double b = cabs[w];
+            const int ri = cidx[w];
+            if (b > best || (b == best && ri < r)) {
+                best = b;
+                pv = cval[w];
+                r = ri;
+            }
+        }
+        r = __builtin_amdgcn_readfirstlane(r);
+        pv = first_lane(pv);
+        if (wave == r % NW) {
+            const int ilr = r / NW;
+            const FT inv = static_cast<FT>(1) / static_cast<FT>(pv);
+#pragma unroll
+            for (int il = 0; il < RPW; ++il)
+                if (il == ilr) {
+#pragma unroll
+                    for (int t = 0; t < CPL; ++t) {
+                        const int j = lane + 64 * t;
+                        const FT v = ((j == k) ? static_cast<FT>(1) : A[il][t]) * inv;
+                        A[il][t] = v;
+                        prow_s[j] = v;
+                    }
+                    used |= 1u << il;
+                }
+        }
+        if (threadIdx.x == 0) {
+            pivrow[k] = r;
+            rinv[r] = k;
+        }
+        __syncthreads();
+        FT pr[CPL];
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) pr[t] = prow_s[lane + 64 * t];
+        const int tk = k >> 6, lk = k & 63;
+#pragma unroll
+        for (int il = 0; il < RPW; ++il) {
+            const int i = wave + NW * il;
+            if (i == r || i >= n) continue;
+            FT ck = 0;
+#pragma unroll
+            for (int t = 0; t < CPL; ++t)
+                if (t == tk) ck = A[il][t];
+            const FT f = lane_value(ck, lk);
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) {
+                FT v = A[il][t];
+                if (t == tk && lane == lk) v = static_cast<FT>(0);
+                A[il][t] = v - f * pr[t];
+            }
+        }
+        if (k + 1 < n) candidates(k + 1);
+        __syncthreads();
+    }
+    FT* Kg = Ki + ps.off2[q];
+#pragma unroll
+    for (int il = 0; il < RPW; ++il) {
+        const int i = wave + NW * il;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int j = lane + 64 * t;
+            if (i < n && j < n) Kg[static_cast<int64_t>(rinv[i]) * np + pivrow[j]] = A[il][t];
+            else if (i < np && j < np) Kg[static_cast<int64_t>(i) * np + j] = static_cast<FT>(0);
+        }
+    }
 }
 
 // Phase 2, one workgroup per (patch, group of sg slabs): right-hand sides of
@@ -578,7 +746,7 @@ __global__ void __launch_bounds__(PATCH_THREADS)
         }
         atomic_max_pos(&patch_res[a], worst);
     }
-    // flux coefficients per (space-time cell, local vertex)
+    // flux coefficients per (spatial cell, local vertex, flux dof) and column
     const int cb = topo.cell_ptr[a], nc = topo.cell_ptr[a + 1] - cb;
     for (int e = threadIdx.x; e < nc * Nf * ncw; e += blockDim.x) {
         const int ci = e / (Nf * ncw), r = e % (Nf * ncw);
@@ -586,130 +754,198 @@ __global__ void __launch_bounds__(PATCH_THREADS)
         const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + i];
         const double val =
             (fi >= 0) ? topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i] * Xs[fi * 32 + c] : 0.0;
-        const int col = c0 + c;
-        const int k = col / Nt, kk = col % Nt;
-        const int64_t cell = static_cast<int64_t>(k) * P.ns + topo.cells[cb + ci];
-        sig[((cell * P.NV + topo.lv[cb + ci]) * Nf + i) * Nt + kk] = val;
+        const int64_t sc = topo.cells[cb + ci];
+        sig[((sc * P.NV + topo.lv[cb + ci]) * Nf + i) * ncols + c0 + c] = val;
     }
 }
+
+template <int DIM, int PP>
+struct LocalSizes {
+    static constexpr int NS = DIM == 1 ? PP + 1 : (PP + 1) * (PP + 2) / 2;  // spatial Lagrange
+    static constexpr int NT = PP + 1;                                        // time modes
+    static constexpr int NF = DIM == 1 ? PP + 2 : (PP + 1) * (PP + 3);      // local flux
+    static constexpr int NQ = NS;                                            // local scalar
+    static constexpr int NV = DIM + 1;
+};
 
 // Per space-time cell terms, in exact coefficient form:
 //   flux^2  = sum_k h |L_k|^2 d_k^T A_c d_k,  d = s + coef(M grad theta) (mismatch in the flux basis)
 //   resid^2 = sum_k det h |L_k|^2 |rho_k|^2,   rho = projection of ell - d_t u - div s
 //   recon^2 = h sum_{l,l'} Mt_{ll'} e_l^T K_c e_l',  e = lambda - lambda_tilde
 //   lam^2   = same with lambda,  trace^2 = int_{t=0} (u0 - u)^2 on the first layer.
-__global__ void k_diagnostics(EstParams P, EstTables T, const double* verts, const int* cells, const double* Mcell,
-                              const int* cell_free, const double* times, const double* lam, const double* lt,
-                              const double* x, const double* sig, const double* u0b, int nb, double* flux,
-                              double* resid, double* recon, double* lam2, double* trace)
+// One workgroup per spatial cell c, one thread per (slab, time mode) column of
+// a chunk of slabs.  The cell's flux mass A_c, gradient projection IG and
+// stiffness K_c are formed once in LDS and shared by every slab; the flux
+// coefficients sig[(c, v, i)][slab * NT + mode] are read coalesced.
+template <int DIM, int PP>
+__global__ void __launch_bounds__(128)
+    k_diag(EstParams P, EstTables T, const double* verts, const int* cells, const double* Mcell, const int* cell_free,
+           const double* times, const double* lam, const double* lt, const double* x, const double* sig,
+           const double* u0b, int nb, double* flux, double* resid, double* recon, double* lam2, double* trace)
 {
-    const int64_t cell = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (cell >= static_cast<int64_t>(P.m) * P.ns) return;
-    const int c = static_cast<int>(cell % P.ns), k = static_cast<int>(cell / P.ns);
-    const int Nf = P.Nf, Nq = P.Nq, Nt = P.Nt, Ns = P.Ns, dim = P.dim;
-    const double h = times[k + 1] - times[k];
-    const Metric2 g = metric_of(dim, c, verts, cells, Mcell);
-    constexpr int MAXNS = 15, MAXNT = 5, MAXNF = 24, MAXNQ = 15;
-    double th[MAXNS][MAXNT], uu[MAXNS][MAXNT], la[MAXNS][MAXNT], el[MAXNS][MAXNT];
-    for (int mm = 0; mm < Ns; ++mm) {
-        const int s = cell_free[c * Ns + mm];
-        for (int l = 0; l < Nt; ++l) {
-            if (s < 0) { th[mm][l] = uu[mm][l] = la[mm][l] = el[mm][l] = 0.0; continue; }
-            const int64_t yi = (static_cast<int64_t>(k) * Nt + l) * P.nf + s;
-            const double u = x[P.nY + (static_cast<int64_t>(k) * P.p + l) * P.nf + s];
-            th[mm][l] = lt[yi] + u;
-            uu[mm][l] = u;
-            la[mm][l] = lam[yi];
-            el[mm][l] = lam[yi] - lt[yi];
-        }
+    using L = LocalSizes<DIM, PP>;
+    constexpr int NS = L::NS, NT = L::NT, NF = L::NF, NQ = L::NQ, NV = L::NV;
+    constexpr int KC = 128 / NT;  // slabs per chunk
+    __shared__ double Ac[NF * NF], IG[NF * NS], Kc[NS * NS];
+    __shared__ double Wt[NT * NT], Wdt[NT * NT], Mtt[NT * NT], Ln[NT], Ph[NQ * NS], Bh[NQ * NF], qb[NQ];
+    __shared__ double st[KC][NS][NT][4];  // theta, u, lambda, lambda - lambda_tilde
+    __shared__ double colf[KC * NT], colr[KC * NT];
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int ncols = P.m * NT;
+    const Metric2 g = metric_of(DIM, c, verts, cells, Mcell);
+    for (int e = tid; e < NF * NF; e += blockDim.x) {
+        const int i = e / NF, j = e % NF;
+        double a = 0.0;
+        for (int ab = 0; ab < DIM * DIM; ++ab)
+            a += g.H[(ab / DIM) * 2 + (ab % DIM)] * T.RTab[(static_cast<int64_t>(ab) * NF + i) * NF + j];
+        Ac[e] = a;
     }
-    // Legendre modes of theta
-    double thL[MAXNS][MAXNT];
-    for (int mm = 0; mm < Ns; ++mm)
-        for (int kk = 0; kk < Nt; ++kk) {
-            double a = 0.0;
-            for (int l = 0; l < Nt; ++l) a += th[mm][l] * T.W[l * Nt + kk];
-            thL[mm][kk] = a / T.Lnorm[kk];
-        }
-    double fl = 0.0, rs = 0.0;
-    for (int kk = 0; kk < Nt; ++kk) {
-        double d[MAXNF];
-        for (int i = 0; i < Nf; ++i) {
-            double s = 0.0;
-            for (int v = 0; v < P.NV; ++v) s += sig[((cell * P.NV + v) * Nf + i) * Nt + kk];
-            double cg = 0.0;
-            for (int ab = 0; ab < dim * dim; ++ab) {
-                const double gab = g.G[(ab / dim) * 2 + (ab % dim)];
-                double t = 0.0;
-                for (int mm = 0; mm < Ns; ++mm) t += T.Igrad[(static_cast<int64_t>(i) * dim * dim + ab) * Ns + mm] * thL[mm][kk];
-                cg += gab * t;
+    for (int e = tid; e < NF * NS; e += blockDim.x) {
+        const int i = e / NS, mm = e % NS;
+        double a = 0.0;
+        for (int ab = 0; ab < DIM * DIM; ++ab)
+            a += g.G[(ab / DIM) * 2 + (ab % DIM)] * T.Igrad[(static_cast<int64_t>(i) * DIM * DIM + ab) * NS + mm];
+        IG[e] = a;
+    }
+    for (int e = tid; e < NS * NS; e += blockDim.x) {
+        const int i = e / NS, j = e % NS;
+        double a = 0.0;
+        for (int ab = 0; ab < DIM * DIM; ++ab)
+            a += g.G[(ab / DIM) * 2 + (ab % DIM)] * T.Khat[(static_cast<int64_t>(ab) * NS + i) * NS + j];
+        Kc[e] = a;
+    }
+    for (int e = tid; e < NT * NT; e += blockDim.x) {
+        Wt[e] = T.W[e];
+        Wdt[e] = T.Wd[e];
+        Mtt[e] = T.Mt[e];
+    }
+    for (int e = tid; e < NT; e += blockDim.x) Ln[e] = T.Lnorm[e];
+    for (int e = tid; e < NQ * NS; e += blockDim.x) Ph[e] = T.Phi[e];
+    for (int e = tid; e < NQ * NF; e += blockDim.x) Bh[e] = T.Bhat[e];
+    for (int e = tid; e < NQ; e += blockDim.x) qb[e] = T.qbar[e];
+    for (int k0 = 0; k0 < P.m; k0 += KC) {
+        const int nsl = min(KC, P.m - k0);
+        __syncthreads();
+        for (int e = tid; e < nsl * NS * NT; e += blockDim.x) {
+            const int sl = e / (NS * NT), mm = (e / NT) % NS, l = e % NT;
+            const int k = k0 + sl;
+            const int s = cell_free[c * NS + mm];
+            double th = 0.0, uu = 0.0, la = 0.0, el = 0.0;
+            if (s >= 0) {
+                const int64_t yi = (static_cast<int64_t>(k) * NT + l) * P.nf + s;
+                uu = x[P.nY + (static_cast<int64_t>(k) * PP + l) * P.nf + s];
+                th = lt[yi] + uu;
+                la = lam[yi];
+                el = lam[yi] - lt[yi];
             }
-            d[i] = s + cg;
+            st[sl][mm][l][0] = th;
+            st[sl][mm][l][1] = uu;
+            st[sl][mm][l][2] = la;
+            st[sl][mm][l][3] = el;
         }
-        double q2 = 0.0;
-        for (int i = 0; i < Nf; ++i) {
-            double row = 0.0;
-            for (int j = 0; j < Nf; ++j) {
+        __syncthreads();
+        if (tid < nsl * NT) {
+            const int sl = tid / NT, kk = tid % NT;
+            const int k = k0 + sl;
+            const int col = k * NT + kk;
+            const double h = times[k + 1] - times[k];
+            double thL[NS];
+#pragma unroll
+            for (int mm = 0; mm < NS; ++mm) {
                 double a = 0.0;
-                for (int ab = 0; ab < dim * dim; ++ab)
-                    a += g.H[(ab / dim) * 2 + (ab % dim)] * T.RTab[(static_cast<int64_t>(ab) * Nf + i) * Nf + j];
-                row += a * d[j];
+#pragma unroll
+                for (int l = 0; l < NT; ++l) a += st[sl][mm][l][0] * Wt[l * NT + kk];
+                thL[mm] = a / Ln[kk];
             }
-            q2 += d[i] * row;
-        }
-        fl += h * T.Lnorm[kk] * q2;
-        // divergence residual coefficients
-        double r2 = 0.0;
-        for (int q = 0; q < Nq; ++q) {
-            double rho = (kk == 0) ? P.ell * T.qbar[q] : 0.0;
-            double dtu = 0.0;
-            for (int mm = 0; mm < Ns; ++mm) {
-                double uw = 0.0;
-                for (int l = 0; l < Nt; ++l) uw += uu[mm][l] * T.Wd[l * Nt + kk];
-                dtu += T.Phi[q * Ns + mm] * uw;
+            // d = s + coef(M grad theta): flux mismatch in the local flux basis
+            double sv[NF], d[NF];
+#pragma unroll
+            for (int i = 0; i < NF; ++i) {
+                double a = 0.0;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) a += sig[((static_cast<int64_t>(c) * NV + v) * NF + i) * ncols + col];
+                sv[i] = a;
+                double cg = 0.0;
+#pragma unroll
+                for (int mm = 0; mm < NS; ++mm) cg += IG[i * NS + mm] * thL[mm];
+                d[i] = a + cg;
             }
-            rho -= dtu / (T.Lnorm[kk] * h);
-            double dv = 0.0;
-            for (int j = 0; j < Nf; ++j) {
-                double s = 0.0;
-                for (int v = 0; v < P.NV; ++v) s += sig[((cell * P.NV + v) * Nf + j) * Nt + kk];
-                dv += T.Bhat[q * Nf + j] * s;
+            double q2 = 0.0;
+#pragma unroll
+            for (int i = 0; i < NF; ++i) {
+                double row = 0.0;
+#pragma unroll
+                for (int j = 0; j < NF; ++j) row += Ac[i * NF + j] * d[j];
+                q2 += d[i] * row;
             }
-            rho -= dv / g.det;
-            r2 += rho * rho;
-        }
-        rs += g.det * h * T.Lnorm[kk] * r2;
-    }
-    (void)MAXNQ;
-    // Y-norm terms with the spatial stiffness K_c = sum_ab G_ab Khat_ab
-    double lq = 0.0, eq = 0.0;
-    for (int i = 0; i < Ns; ++i)
-        for (int j = 0; j < Ns; ++j) {
-            double kij = 0.0;
-            for (int ab = 0; ab < dim * dim; ++ab)
-                kij += g.G[(ab / dim) * 2 + (ab % dim)] * T.Khat[(static_cast<int64_t>(ab) * Ns + i) * Ns + j];
-            double tl = 0.0, te = 0.0;
-            for (int l = 0; l < Nt; ++l)
-                for (int l2 = 0; l2 < Nt; ++l2) {
-                    tl += la[i][l] * T.Mt[l * Nt + l2] * la[j][l2];
-                    te += el[i][l] * T.Mt[l * Nt + l2] * el[j][l2];
+            // divergence residual ell - d_t u - div s in the orthonormal scalar basis
+            double r2 = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                double rho = (kk == 0) ? P.ell * qb[q] : 0.0;
+                double dtu = 0.0;
+#pragma unroll
+                for (int mm = 0; mm < NS; ++mm) {
+                    double uw = 0.0;
+#pragma unroll
+                    for (int l = 0; l < NT; ++l) uw += st[sl][mm][l][1] * Wdt[l * NT + kk];
+                    dtu += Ph[q * NS + mm] * uw;
                 }
-            lq += kij * tl;
-            eq += kij * te;
+                rho -= dtu / (Ln[kk] * h);
+                double dv = 0.0;
+#pragma unroll
+                for (int j = 0; j < NF; ++j) dv += Bh[q * NF + j] * sv[j];
+                rho -= dv / g.det;
+                r2 += rho * rho;
+            }
+            colf[tid] = h * Ln[kk] * q2;
+            colr[tid] = g.det * h * Ln[kk] * r2;
         }
-    double tr = 0.0;
-    if (k == 0)
-        for (int qb = 0; qb < nb; ++qb) {
-            double uv = 0.0;
-            for (int mm = 0; mm < Ns; ++mm) uv += uu[mm][0] * T.bphi[qb * Ns + mm];
-            const double dd = u0b[static_cast<int64_t>(c) * nb + qb] - uv;
-            tr += T.bw[qb] * g.det * dd * dd;
+        __syncthreads();
+        if (tid < nsl) {
+            const int sl = tid, k = k0 + sl;
+            const double h = times[k + 1] - times[k];
+            // Y-norm terms with K_c and the Lagrange time mass matrix
+            double lq = 0.0, eq = 0.0;
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    double tl = 0.0, te = 0.0;
+#pragma unroll
+                    for (int l = 0; l < NT; ++l)
+#pragma unroll
+                        for (int l2 = 0; l2 < NT; ++l2) {
+                            tl += st[sl][i][l][2] * Mtt[l * NT + l2] * st[sl][j][l2][2];
+                            te += st[sl][i][l][3] * Mtt[l * NT + l2] * st[sl][j][l2][3];
+                        }
+                    lq += Kc[i * NS + j] * tl;
+                    eq += Kc[i * NS + j] * te;
+                }
+            double fsum = 0.0, rsum = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < NT; ++kk) {
+                fsum += colf[sl * NT + kk];
+                rsum += colr[sl * NT + kk];
+            }
+            double tr = 0.0;
+            if (k == 0)
+                for (int qq = 0; qq < nb; ++qq) {
+                    double uv = 0.0;
+#pragma unroll
+                    for (int mm = 0; mm < NS; ++mm) uv += st[0][mm][0][1] * T.bphi[qq * NS + mm];
+                    const double dd = u0b[static_cast<int64_t>(c) * nb + qq] - uv;
+                    tr += T.bw[qq] * g.det * dd * dd;
+                }
+            const int64_t cell = static_cast<int64_t>(k) * P.ns + c;
+            flux[cell] = fsum;
+            resid[cell] = rsum;
+            recon[cell] = h * eq;
+            lam2[cell] = h * lq;
+            trace[cell] = tr;
         }
-    flux[cell] = fl;
-    resid[cell] = rs;
-    recon[cell] = h * eq;
-    lam2[cell] = h * lq;
-    trace[cell] = tr;
+    }
 }
 
 __global__ void k_sum5(int64_t n, const double* a, const double* b, const double* c, const double* d,
@@ -913,6 +1149,24 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
             using FT = decltype(tag);
             FT* Ki = reinterpret_cast<FT*>(B.Ki);
             for (const auto& bk : B.buckets) {
+                // register-resident Gauss-Jordan where the matrix fits the register file
+                auto reg = [&](auto kern, int threads) {
+                    hipLaunchKernelGGL(kern, dim3(bk.count), dim3(threads), 0, st, P, Tb, topo, ps, bk.q0,
+                                       sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
+                };
+                if (bk.np <= 64) {
+                    reg(k_patch_factor_reg<FT, 4, 16, 1>, 256);
+                    continue;
+                }
+                if (bk.np <= 128) {
+                    reg(k_patch_factor_reg<FT, 4, 32, 2>, 256);
+                    continue;
+                }
+                if constexpr (std::is_same<FT, float>::value)
+                    if (bk.np <= 256) {
+                        reg(k_patch_factor_reg<FT, 8, 32, 4>, 512);
+                        continue;
+                    }
                 const size_t npb = static_cast<size_t>(bk.np);
                 const size_t in_lds_bytes = npb * npb * sizeof(FT) + npb * sizeof(int);
                 const int in_lds = in_lds_bytes <= lds_cap ? 1 : 0;
@@ -952,9 +1206,23 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         HIP_TRY(hipEventCreate(&e2));
         HIP_TRY(hipEventCreate(&e3));
         HIP_TRY(hipEventRecord(e2, st));
-        hipLaunchKernelGGL(k_diagnostics, dim3(static_cast<unsigned>((ncell + 127) / 128)), dim3(128), 0, st, P, Tb,
-                           sys->d_verts, sys->d_cells, sys->d_Mcell, sys->d_cell_free, sys->d_times, sys->d_x, B.lt,
-                           sys->d_x, B.sig, sys->d_u0b, T.nb, B.flux, B.resid, B.recon, B.lam2, B.trace);
+        {
+            auto diag = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(P.ns)), dim3(128), 0, st, P, Tb, sys->d_verts,
+                                   sys->d_cells, sys->d_Mcell, sys->d_cell_free, sys->d_times, sys->d_x, B.lt, sys->d_x,
+                                   B.sig, sys->d_u0b, T.nb, B.flux, B.resid, B.recon, B.lam2, B.trace);
+            };
+            const int key = S.dim * 10 + S.p;
+            switch (key) {
+                case 11: diag(k_diag<1, 1>); break;
+                case 12: diag(k_diag<1, 2>); break;
+                case 13: diag(k_diag<1, 3>); break;
+                case 21: diag(k_diag<2, 1>); break;
+                case 22: diag(k_diag<2, 2>); break;
+                case 23: diag(k_diag<2, 3>); break;
+                default: throw std::runtime_error("estimator diagnostics: unsupported (dim, p)");
+            }
+        }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(e3, st));
         const int nparts = 256;
