@@ -109,6 +109,15 @@ __device__ inline void atomic_max_pos(double* addr, double v)
     atomicMax(reinterpret_cast<unsigned long long*>(addr), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+template <int DIM, int PP>
+struct LocalSizes {
+    static constexpr int NS = DIM == 1 ? PP + 1 : (PP + 1) * (PP + 2) / 2;  // spatial Lagrange
+    static constexpr int NT = PP + 1;                                        // time modes
+    static constexpr int NF = DIM == 1 ? PP + 2 : (PP + 1) * (PP + 3);      // local flux
+    static constexpr int NQ = NS;                                            // local scalar
+    static constexpr int NV = DIM + 1;
+};
+
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -501,98 +510,116 @@ __global__ void __launch_bounds__(NW * 64)
 // the patch problems of every slab and time mode in the group
 // (vec_f_2d.hpp:27-78, vec_g_2d.hpp:30-95), in the Legendre time basis,
 // mode kk scaled by 1 / (h |L_kk|^2), into RHS[off1 * ncols + row * ncols + col],
-// col = slab * Nt + kk.  Flux rows shared by two cells accumulate with LDS
-// atomics; rows n..np-1 stay zero.
+// col = slab * NT + kk.  The time projections Theta W and U W' of every
+// (cell, slab) are formed once while staging; flux rows shared by two cells
+// accumulate with LDS atomics; rows n..np-1 stay zero.
+template <int DIM, int PP>
 __global__ void __launch_bounds__(PATCH_THREADS)
     k_patch_rhs(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, int sg, int nsg, int nc_max,
                 const double* verts, const int* cells, const double* Mcell, const int* cell_free, const double* times,
                 const double* lt, const double* x, double* RHS)
 {
+    using L = LocalSizes<DIM, PP>;
+    constexpr int NS = L::NS, NT = L::NT, NF = L::NF, NQ = L::NQ, NV = L::NV;
+    __shared__ double Rt[NV * NF * NS], Qt[NV * NQ * NS], Et[DIM * NQ * NS], Pt[NV * NQ];
+    __shared__ double Wt[NT * NT], Wdt[NT * NT], Ln[NT];
     extern __shared__ double lds[];
-    const int Nf = P.Nf, Nq = P.Nq, Nt = P.Nt, Ns = P.Ns, dim = P.dim;
     const int q = q0 + blockIdx.x / nsg;
     const int k0 = (blockIdx.x % nsg) * sg;
     const int a = topo.order[q];
     const int np = ps.np[q];
     const int nsl = min(sg, P.m - k0);
-    const int ncols = P.m * Nt;
-    const int wcol = sg * Nt;
+    const int ncols = P.m * NT;
+    const int wcol = sg * NT;
     const int nF = topo.nflux[a];
     const int cb = topo.cell_ptr[a], nc = topo.cell_ptr[a + 1] - cb;
-    double* acc = lds;                                              // [np][sg * Nt]
-    double* stage = acc + static_cast<int64_t>(np) * wcol;         // [nc][sg][Ns * Nt][2]
-    double* geo = stage + static_cast<int64_t>(nc_max) * sg * Ns * Nt * 2;  // [nc][4]
-    for (int e = threadIdx.x; e < np * wcol; e += blockDim.x) acc[e] = 0.0;
-    const int per_cell = nsl * Ns * Nt;
-    for (int e = threadIdx.x; e < nc * per_cell; e += blockDim.x) {
-        const int ci = e / per_cell, r = e % per_cell;
-        const int sl = r / (Ns * Nt), r2 = r % (Ns * Nt);
-        const int c = topo.cells[cb + ci];
-        double th, uu;
-        gather_cell(P, k0 + sl, c, cell_free, lt, x, r2 / Nt, r2 % Nt, th, uu);
-        double* s = stage + ((static_cast<int64_t>(ci) * sg + sl) * Ns * Nt + r2) * 2;
-        s[0] = th;
-        s[1] = uu;
+    double* acc = lds;                                                  // [np][sg * NT]
+    double* stage = acc + static_cast<int64_t>(np) * wcol;             // [nc][sg][NS][NT][2]: Theta W, U W'
+    double* geo = stage + static_cast<int64_t>(nc_max) * sg * NS * NT * 2;  // [nc][4]: det, gv
+    for (int e = threadIdx.x; e < NV * NF * NS; e += blockDim.x) Rt[e] = T.R[e];
+    for (int e = threadIdx.x; e < NV * NQ * NS; e += blockDim.x) Qt[e] = T.Q[e];
+    for (int e = threadIdx.x; e < DIM * NQ * NS; e += blockDim.x) Et[e] = T.E[e];
+    for (int e = threadIdx.x; e < NV * NQ; e += blockDim.x) Pt[e] = T.Pv[e];
+    for (int e = threadIdx.x; e < NT * NT; e += blockDim.x) {
+        Wt[e] = T.W[e];
+        Wdt[e] = T.Wd[e];
     }
+    for (int e = threadIdx.x; e < NT; e += blockDim.x) Ln[e] = T.Lnorm[e];
+    for (int e = threadIdx.x; e < np * wcol; e += blockDim.x) acc[e] = 0.0;
     for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) {
         const int c = topo.cells[cb + ci];
         const int v = topo.lv[cb + ci];
-        const Metric2 g = metric_of(dim, c, verts, cells, Mcell);
+        const Metric2 g = metric_of(DIM, c, verts, cells, Mcell);
         // grad psi_a pulled through the stiffness metric: gv = G grad^lam_v
         double gv[2] = {0.0, 0.0};
-        for (int a2 = 0; a2 < dim; ++a2)
-            for (int b = 0; b < dim; ++b) gv[a2] += g.G[a2 * 2 + b] * T.dbary[v * dim + b];
+        for (int a2 = 0; a2 < DIM; ++a2)
+            for (int b = 0; b < DIM; ++b) gv[a2] += g.G[a2 * 2 + b] * T.dbary[v * DIM + b];
         geo[ci * 4 + 0] = g.det;
         geo[ci * 4 + 1] = gv[0];
         geo[ci * 4 + 2] = gv[1];
     }
     __syncthreads();
-    const int rows = (Nf + Nq) * Nt;
-    for (int e = threadIdx.x; e < nc * nsl * rows; e += blockDim.x) {
-        const int ci = e / (nsl * rows), r = e % (nsl * rows);
-        const int sl = r / rows, r2 = r % rows;
-        const int row = r2 / Nt, kk = r2 % Nt;
-        const int v = topo.lv[cb + ci];
+    for (int e = threadIdx.x; e < nc * nsl * NS; e += blockDim.x) {
+        const int ci = e / (nsl * NS), r = e % (nsl * NS);
+        const int sl = r / NS, mm = r % NS;
+        const int c = topo.cells[cb + ci];
         const int k = k0 + sl;
-        const double h = times[k + 1] - times[k];
-        const double scale = 1.0 / (h * T.Lnorm[kk]);
-        const double* st = stage + (static_cast<int64_t>(ci) * sg + sl) * Ns * Nt * 2;
-        const double det = geo[ci * 4 + 0];
-        if (row < Nf) {
-            const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + row];
-            if (fi < 0) continue;
-            // F = -h R_v Theta W
-            double s = 0.0;
-            for (int mm = 0; mm < Ns; ++mm) {
-                double tw = 0.0;
-                for (int l = 0; l < Nt; ++l) tw += st[(mm * Nt + l) * 2] * T.W[l * Nt + kk];
-                s += T.R[(static_cast<int64_t>(v) * Nf + row) * Ns + mm] * tw;
+        double th[NT], uu[NT];
+#pragma unroll
+        for (int l = 0; l < NT; ++l) gather_cell(P, k, c, cell_free, lt, x, mm, l, th[l], uu[l]);
+        double* s = stage + ((static_cast<int64_t>(ci) * sg + sl) * NS + mm) * NT * 2;
+#pragma unroll
+        for (int kk = 0; kk < NT; ++kk) {
+            double tw = 0.0, uw = 0.0;
+#pragma unroll
+            for (int l = 0; l < NT; ++l) {
+                tw += th[l] * Wt[l * NT + kk];
+                uw += uu[l] * Wdt[l * NT + kk];
             }
-            const int8_t sgn = topo.fsign[static_cast<int64_t>(cb + ci) * Nf + row];
-            atomicAdd(&acc[fi * wcol + sl * Nt + kk], -h * sgn * s * scale);
-        } else {
-            const int qq = row - Nf;
-            double s = (kk == 0) ? P.ell * det * h * T.Pv[v * Nq + qq] : 0.0;
-            double du = 0.0, dth = 0.0;
-            for (int mm = 0; mm < Ns; ++mm) {
-                double uw = 0.0, tw = 0.0;
-                for (int l = 0; l < Nt; ++l) {
-                    uw += st[(mm * Nt + l) * 2 + 1] * T.Wd[l * Nt + kk];
-                    tw += st[(mm * Nt + l) * 2] * T.W[l * Nt + kk];
-                }
-                du += T.Q[(static_cast<int64_t>(v) * Nq + qq) * Ns + mm] * uw;
-                double eb = 0.0;
-                for (int b = 0; b < dim; ++b)
-                    eb += geo[ci * 4 + 1 + b] * T.E[(static_cast<int64_t>(b) * Nq + qq) * Ns + mm];
-                dth += eb * tw;
-            }
-            s += -det * du - h * dth;
-            acc[(nF + ci * Nq + qq) * wcol + sl * Nt + kk] = s * scale;
+            s[kk * 2] = tw;
+            s[kk * 2 + 1] = uw;
         }
     }
     __syncthreads();
-    const int ncw = nsl * Nt;
-    double* out = RHS + ps.off1[q] * ncols + static_cast<int64_t>(k0) * Nt;
+    constexpr int ROWS = (NF + NQ) * NT;
+    for (int e = threadIdx.x; e < nc * nsl * ROWS; e += blockDim.x) {
+        const int ci = e / (nsl * ROWS), r = e % (nsl * ROWS);
+        const int sl = r / ROWS, r2 = r % ROWS;
+        const int row = r2 / NT, kk = r2 % NT;
+        const int v = topo.lv[cb + ci];
+        const int k = k0 + sl;
+        const double h = times[k + 1] - times[k];
+        const double scale = 1.0 / (h * Ln[kk]);
+        const double* st = stage + (static_cast<int64_t>(ci) * sg + sl) * NS * NT * 2;
+        if (row < NF) {
+            const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * NF + row];
+            if (fi < 0) continue;
+            // F = -h R_v (Theta W)
+            double sum = 0.0;
+#pragma unroll
+            for (int mm = 0; mm < NS; ++mm) sum += Rt[(v * NF + row) * NS + mm] * st[(mm * NT + kk) * 2];
+            const int8_t sgn = topo.fsign[static_cast<int64_t>(cb + ci) * NF + row];
+            atomicAdd(&acc[fi * wcol + sl * NT + kk], -h * sgn * sum * scale);
+        } else {
+            const int qq = row - NF;
+            const double det = geo[ci * 4 + 0];
+            double sum = (kk == 0) ? P.ell * det * h * Pt[v * NQ + qq] : 0.0;
+            double du = 0.0, dth = 0.0;
+#pragma unroll
+            for (int mm = 0; mm < NS; ++mm) {
+                du += Qt[(v * NQ + qq) * NS + mm] * st[(mm * NT + kk) * 2 + 1];
+                double eb = 0.0;
+#pragma unroll
+                for (int b = 0; b < DIM; ++b) eb += geo[ci * 4 + 1 + b] * Et[(b * NQ + qq) * NS + mm];
+                dth += eb * st[(mm * NT + kk) * 2];
+            }
+            sum += -det * du - h * dth;
+            acc[(nF + ci * NQ + qq) * wcol + sl * NT + kk] = sum * scale;
+        }
+    }
+    __syncthreads();
+    const int ncw = nsl * NT;
+    double* out = RHS + ps.off1[q] * ncols + static_cast<int64_t>(k0) * NT;
     for (int e = threadIdx.x; e < np * ncw; e += blockDim.x) {
         const int row = e / ncw, c = e % ncw;
         out[static_cast<int64_t>(row) * ncols + c] = acc[row * wcol + c];
@@ -758,15 +785,6 @@ __global__ void __launch_bounds__(PATCH_THREADS)
         sig[((sc * P.NV + topo.lv[cb + ci]) * Nf + i) * ncols + c0 + c] = val;
     }
 }
-
-template <int DIM, int PP>
-struct LocalSizes {
-    static constexpr int NS = DIM == 1 ? PP + 1 : (PP + 1) * (PP + 2) / 2;  // spatial Lagrange
-    static constexpr int NT = PP + 1;                                        // time modes
-    static constexpr int NF = DIM == 1 ? PP + 2 : (PP + 1) * (PP + 3);      // local flux
-    static constexpr int NQ = NS;                                            // local scalar
-    static constexpr int NV = DIM + 1;
-};
 
 // Per space-time cell terms, in exact coefficient form:
 //   flux^2  = sum_k h |L_k|^2 d_k^T A_c d_k,  d = s + coef(M grad theta) (mismatch in the flux basis)
@@ -1184,10 +1202,21 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
                 while (sg > 1 && lr(sg) > 64 * 1024) --sg;
                 if (lr(sg) > lds_cap) throw std::runtime_error("patch right-hand side block exceeds LDS");
                 const int nsg = (P.m + sg - 1) / sg;
-                set_lds(reinterpret_cast<const void*>(&k_patch_rhs), lr(sg));
-                hipLaunchKernelGGL(k_patch_rhs, dim3(bk.count * nsg), dim3(PATCH_THREADS), lr(sg), st, P, Tb, topo, ps,
-                                   bk.q0, sg, nsg, bk.nc_max, sys->d_verts, sys->d_cells, sys->d_Mcell,
-                                   sys->d_cell_free, sys->d_times, B.lt, sys->d_x, B.RHS);
+                auto rhs = [&](auto kern) {
+                    set_lds(reinterpret_cast<const void*>(kern), lr(sg));
+                    hipLaunchKernelGGL(kern, dim3(bk.count * nsg), dim3(PATCH_THREADS), lr(sg), st, P, Tb, topo, ps,
+                                       bk.q0, sg, nsg, bk.nc_max, sys->d_verts, sys->d_cells, sys->d_Mcell,
+                                       sys->d_cell_free, sys->d_times, B.lt, sys->d_x, B.RHS);
+                };
+                switch (S.dim * 10 + S.p) {
+                    case 11: rhs(k_patch_rhs<1, 1>); break;
+                    case 12: rhs(k_patch_rhs<1, 2>); break;
+                    case 13: rhs(k_patch_rhs<1, 3>); break;
+                    case 21: rhs(k_patch_rhs<2, 1>); break;
+                    case 22: rhs(k_patch_rhs<2, 2>); break;
+                    case 23: rhs(k_patch_rhs<2, 3>); break;
+                    default: throw std::runtime_error("patch right-hand sides: unsupported (dim, p)");
+                }
             }
             for (const auto& bk : B.buckets) {
                 const size_t ls = static_cast<size_t>(bk.np) * 32 * (sizeof(double) + sizeof(FT));
