@@ -28,7 +28,6 @@ METRIC = "assembly+estimator DOF/s & MINRES s/solve; mixed vs FP64 at 1e-10 rel 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-PATCH_KERNEL = "k_patch<float> (vertex-patch mixed solves, FP32 inverse + FP64 residual refinement)"
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
 # calibration) from the rocprofv3 --pmc runs of `bench.py --profile` on the
 # same code, summarised in profiles/counters_c2.md.
@@ -61,16 +60,46 @@ def workload(name):
     raise SystemExit(f"unknown config {name}")
 
 
-def patch_flops(sp):
-    """Algorithmic flops of the patch solves of one estimate: per spatial vertex
-    patch of size N (flux + scalar + mean-zero multiplier, one Legendre time
-    mode) one LU, (2/3) N^3, and one forward/back solve, 2 N^2, per (slab, time
-    mode) right-hand side.  The patch matrix is the same for every slab and
-    mode of a tensor mesh, so it is factorised once."""
+def patch_work(sp, steps):
+    """Algorithmic work of one estimate's patch phases (n = order of a spatial
+    vertex patch problem of one Legendre time mode: flux + scalar + mean-zero
+    multiplier; the matrix is shared by every slab and mode of a tensor mesh,
+    so each patch is inverted once and applied to cols = slabs x modes
+    right-hand sides):
+      factor: Gauss-Jordan inversion, 2 n^3 flops;
+      solve:  X = K^-1 B and `steps` refinement rounds, each one FP64
+              residual product and one correction product, plus the final
+              FP64 residual: (1 + steps) products in the inverse's precision
+              and (1 + steps) FP64 products, 2 n^2 cols flops each."""
     nf, ns_, _, bd = sp.patch_sizes()
-    N = (nf + ns_ + (bd == 0)).astype(np.float64)
+    n = (nf + ns_ + (bd == 0)).astype(np.float64)
     cols = sp.n_intervals * (sp.p + 1)
-    return float(np.sum(2.0 / 3.0 * N ** 3 + 2.0 * N ** 2 * cols))
+    prod = float(np.sum(2.0 * n ** 2 * cols))
+    return {"factor": float(np.sum(2.0 * n ** 3)), "solve_ft": (1 + steps) * prod, "solve_fp64": (1 + steps) * prod,
+            "lu_and_solve": float(np.sum(2.0 / 3.0 * n ** 3 + 2.0 * n ** 2 * cols))}
+
+
+def phase_rooflines(sp, ph, steps, mixed):
+    """Roofline entries of the patch kernels from their measured times."""
+    w = patch_work(sp, steps)
+    ft_peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
+    out = {}
+    t = ph["factor_ms"] * 1e-3
+    ach = w["factor"] / t / 1e12
+    out["factor"] = {"bound": "mfma", "kernel": "k_patch_factor_reg (Gauss-Jordan, " + ("FP32" if mixed else "FP64")
+                     + " VALU)", "achieved": ach, "peak": ft_peak, "unit": "TFLOP/s", "frac": ach / ft_peak,
+                     "algorithmic_flops_per_launch": w["factor"], "avg_launch_ms": ph["factor_ms"]}
+    t = ph["solve_ms"] * 1e-3
+    flops = w["solve_ft"] + w["solve_fp64"]
+    # time at peak: products in the inverse's precision at its MFMA peak, FP64 residuals at the FP64 peak
+    t_ideal = w["solve_ft"] / (ft_peak * 1e12) + w["solve_fp64"] / (FP64_PEAK_TFLOPS * 1e12)
+    peak = flops / t_ideal / 1e12
+    ach = flops / t / 1e12
+    out["solve"] = {"bound": "mfma", "kernel": "k_patch_solve (MFMA 16x16x4: " + ("FP32" if mixed else "FP64")
+                    + " products, FP64 residuals)", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                    "frac": ach / peak, "algorithmic_flops_per_launch": flops, "avg_launch_ms": ph["solve_ms"]}
+    out["rhs"] = {"kernel": "k_patch_rhs", "avg_launch_ms": ph["rhs_ms"]}
+    return out
 
 
 def cpu_baseline(min_s=10.0):
@@ -132,7 +161,7 @@ def main():
         gbs = ctx.bench_stream(1 << 30, 3)
         print(json.dumps({"spmv_ms": ms, "spmv_algorithmic_bytes": 12 * sp.nnz + 20 * sp.n, "nnz": sp.nnz,
                           "n": sp.n, "copy_kernel_bytes_read": 1 << 30, "copy_kernel_bytes_written": 1 << 30,
-                          "copy_GBs": gbs, "patch_flops": patch_flops(sp)}))
+                          "copy_GBs": gbs, "patch_work": patch_work(sp, 2)}))
         return
 
     def step(prec=apf.APF_FP32_REFINE):
@@ -143,7 +172,7 @@ def main():
         t_minres = time.perf_counter() - t0
         est = sysm.estimate(prec, 2, per_cell=False)
         t_est = (est["seconds_patches"] + est["seconds_diagnostics"]) * 1e3
-        return t_asm, t_minres, st, t_est, est
+        return t_asm, t_minres, st, t_est, est, sysm.phase_timings()
 
     for _ in range(args.warmup):
         step()
@@ -163,10 +192,12 @@ def main():
     rel = {k: abs(est_mixed[k] - est64[k]) / abs(est64[k]) for k in ("flux2", "eta2", "lambda_y2", "y_est2")}
     value = n_dofs / ((t_asm + t_est) * 1e-3)
 
-    # MFMA/FP32 roofline of the patch kernel (dominant kernel of `value`)
-    pflops = patch_flops(sp)
-    patch_ms = est_mixed["seconds_patches"] * 1e3
-    patch_tflops = pflops / (patch_ms * 1e-3) / 1e12
+    # rooflines of the patch phases (the dominant kernels of `value`)
+    ph_mixed = {k: float(np.mean([r[5][k] for r in rec])) for k in rec[0][5]}
+    ph64 = sysm.phase_timings()
+    roof = phase_rooflines(sp, ph_mixed, 2, True)
+    roof64 = phase_rooflines(sp, ph64, 0, False)
+    dom = max(("factor", "solve"), key=lambda k: roof[k]["avg_launch_ms"])
     # HBM roofline of the MINRES SpMV: algorithmic bytes 12 nnz + 20 n
     spmv_ms = sysm.bench_spmv(50)
     stream_gbs = ctx.bench_stream(1 << 30, 10)
@@ -189,10 +220,9 @@ def main():
         "config": {"workload": W["desc"], "p": W["p"], "n_X": sp.nX, "n_Y": sp.nY, "nnz_K": sp.nnz,
                    "space_time_cells": sp.n_space_cells * sp.n_intervals,
                    "vertex_patches": sp.n_patches * sp.n_intervals},
-        "roofline": {"bound": "mfma", "kernel": PATCH_KERNEL, "achieved": patch_tflops,
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": patch_tflops / FP32_PEAK_TFLOPS,
-                     "traffic": traffic.get("patch"), "algorithmic_flops_per_launch": pflops,
-                     "avg_launch_ms": patch_ms},
+        "roofline": dict(roof[dom], traffic=traffic.get(dom)),
+        "roofline_patch_phases_mixed": roof,
+        "roofline_patch_phases_fp64": roof64,
         "roofline_minres_spmv": {"bound": "hbm", "kernel": "k_spmv (MINRES CSR SpMV, FP64 values, int32 columns)",
                                  "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get("spmv"),

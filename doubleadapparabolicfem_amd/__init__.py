@@ -116,6 +116,7 @@ EXPORTED = {
     "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "apf_system_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_estimate_phase_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 
@@ -331,3 +332,9 @@ class System:
         t = np.zeros(4)
         _check(lib().apf_system_timings(self.handle, _ptr(t)))
         return dict(assembly_ms=t[0], patch_ms=t[1], diagnostics_ms=t[2], spmv_ms=t[3])
+
+    def phase_timings(self):
+        """Device milliseconds of the last estimate's patch phases."""
+        t = np.zeros(3)
+        _check(lib().apf_estimate_phase_timings(self.handle, _ptr(t)))
+        return dict(factor_ms=t[0], rhs_ms=t[1], solve_ms=t[2])

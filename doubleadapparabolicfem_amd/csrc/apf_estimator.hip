@@ -626,24 +626,57 @@ __global__ void __launch_bounds__(PATCH_THREADS)
     }
 }
 
-// Phase 3, one workgroup per (patch, block of whole slabs): X = Ki B on MFMA in
-// FT, then refine_steps rounds of R = B - Kd X (FP64 MFMA) and X += Ki R (FT);
-// the residual of the final X gives the acceptance statistic
-// |K x - b|_inf / max(1, |b|_inf) per (slab, vertex) problem
-// (dense_local_error_blocks_2d.hpp:923-949).  Both matrices are symmetric, so
-// the MFMA A operand A[i][k] is read as M[k][i] (coalesced over i).  The flux
-// coefficients go to sig[(cell, local vertex)].
-template <typename FT>
-__global__ void __launch_bounds__(PATCH_THREADS)
-    k_patch_solve(EstParams P, PatchTopo topo, PatchSlots ps, int q0, int ncg, int cpw, const double* Kd, const FT* Ki,
-                  const double* RHS, double* sig, double* patch_res)
+constexpr int SOLVE_THREADS = 512;
+
+// acc[ct] += M(rows rt*16.., k) B(k, cols ct*16..) over k < np, M read as
+// M[k][i] (symmetric), B from LDS with row stride ldb.  The A operands of 16
+// k-steps are loaded together so one memory latency covers 16 * CT MFMAs.
+template <int CT, typename MT, typename BT, typename V4>
+__device__ inline void tile_product(const MT* mp, int np, const BT* Bs, int ldb, int li, int lk, V4 (&acc)[CT],
+                                    bool negate)
+{
+    for (int kb = 0; kb < np; kb += 64) {
+        MT av[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int k = kb + 4 * t + lk;
+            av[t] = (kb + 4 * t < np) ? mp[static_cast<int64_t>(k) * np] : static_cast<MT>(0);
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            if (kb + 4 * t >= np) break;
+            const int k = kb + 4 * t + lk;
+            const MT a = negate ? -av[t] : av[t];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+                acc[ct] = mfma_16x16x4(a, static_cast<MT>(Bs[k * ldb + ct * 16 + li]), acc[ct]);
+        }
+    }
+}
+
+// Phase 3, one workgroup per (patch, block of 16 CT columns holding whole
+// slabs): X = Ki B on MFMA in FT, then refine_steps rounds of R = B - Kd X
+// (FP64 MFMA) and X += Ki R (FT); the residual of the final X gives the
+// acceptance statistic |K x - b|_inf / max(1, |b|_inf) per (slab, vertex)
+// problem (dense_local_error_blocks_2d.hpp:923-949).  Each wave owns row tiles
+// rt = wave + 8 j.  The flux coefficients go to sig.
+template <typename FT, int CT>
+__global__ void __launch_bounds__(SOLVE_THREADS)
+    k_patch_solve(EstParams P, PatchTopo topo, PatchSlots ps, int q0, int count, int ncg, int cpw, const double* Kd,
+                  const FT* Ki, const double* RHS, double* sig, double* patch_res)
 {
     using V4 = typename Vec4<FT>::type;
+    constexpr int CW = 16 * CT;
     extern __shared__ double lds[];
-    __shared__ unsigned long long colmax[64];  // [0, 32): |R| per column, [32, 64): |b| per column
+    __shared__ unsigned long long colmax[2 * CW];  // [0, CW): |R| per column, [CW, 2 CW): |b| per column
     const int Nf = P.Nf, Nt = P.Nt;
-    const int q = q0 + blockIdx.x / ncg;
-    const int c0 = (blockIdx.x % ncg) * cpw;
+    // Workgroups are dealt to the 8 XCDs round-robin: keep every column block
+    // of one patch on the same XCD so its matrices are read into one L2.
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int qb = (slot / ncg) * 8 + xcd;
+    if (qb >= count) return;
+    const int q = q0 + qb;
+    const int c0 = (slot % ncg) * cpw;
     const int a = topo.order[q];
     const int np = ps.np[q];
     const int ncols = P.m * Nt;
@@ -651,34 +684,27 @@ __global__ void __launch_bounds__(PATCH_THREADS)
     const double* Kq = Kd + ps.off2[q];
     const FT* Iq = Ki + ps.off2[q];
     const double* Bq = RHS + ps.off1[q] * ncols + c0;
-    double* Xs = lds;                                                  // [np][32]
-    FT* Rs = reinterpret_cast<FT*>(Xs + static_cast<int64_t>(np) * 32);  // [np][32]
+    double* Xs = lds;                                                  // [np][CW]
+    FT* Rs = reinterpret_cast<FT*>(Xs + static_cast<int64_t>(np) * CW);  // [np][CW]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int nrt = np >> 4;
-    for (int e = threadIdx.x; e < np * 32; e += blockDim.x) {
-        const int r = e >> 5, c = e & 31;
+    for (int e = threadIdx.x; e < np * CW; e += blockDim.x) {
+        const int r = e / CW, c = e % CW;
         Rs[e] = (c < ncw) ? static_cast<FT>(Bq[static_cast<int64_t>(r) * ncols + c]) : static_cast<FT>(0);
     }
-    if (threadIdx.x < 64) colmax[threadIdx.x] = 0ull;
+    for (int e = threadIdx.x; e < 2 * CW; e += blockDim.x) colmax[e] = 0ull;
     __syncthreads();
     // X = Ki B
     for (int rt = wave; rt < nrt; rt += nw) {
-        V4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
-        const FT* ip = Iq + rt * 16 + li;
-        for (int kb = 0; kb < np; kb += 16)
+        V4 acc[CT];
 #pragma unroll
-            for (int k = kb; k < kb + 16; k += 4) {
-                const FT av = ip[static_cast<int64_t>(k + lk) * np];
-                d0 = mfma_16x16x4(av, Rs[(k + lk) * 32 + li], d0);
-                d1 = mfma_16x16x4(av, Rs[(k + lk) * 32 + 16 + li], d1);
-            }
+        for (int ct = 0; ct < CT; ++ct) acc[ct] = V4{0, 0, 0, 0};
+        tile_product<CT>(Iq + rt * 16 + li, np, Rs, CW, li, lk, acc, false);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = rt * 16 + drow<FT>(lk, r);
-            Xs[row * 32 + li] = static_cast<double>(d0[r]);
-            Xs[row * 32 + 16 + li] = static_cast<double>(d1[r]);
-        }
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Xs[(rt * 16 + drow<FT>(lk, r)) * CW + ct * 16 + li] = static_cast<double>(acc[ct][r]);
     }
     __syncthreads();
     const int steps = P.refine_steps;
@@ -686,77 +712,58 @@ __global__ void __launch_bounds__(PATCH_THREADS)
         const bool last = it == steps;
         // R = B - Kd X in FP64
         for (int rt = wave; rt < nrt; rt += nw) {
-            v4d d0, d1;
+            v4d acc[CT];
+            double bm[CT];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = rt * 16 + drow<double>(lk, r);
-                d0[r] = (li < ncw) ? Bq[static_cast<int64_t>(row) * ncols + li] : 0.0;
-                d1[r] = (16 + li < ncw) ? Bq[static_cast<int64_t>(row) * ncols + 16 + li] : 0.0;
-            }
-            double bm0 = 0.0, bm1 = 0.0;
-            if (last) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    bm0 = fmax(bm0, fabs(d0[r]));
-                    bm1 = fmax(bm1, fabs(d1[r]));
-                }
-            }
-            const double* kp = Kq + rt * 16 + li;
-            for (int kb = 0; kb < np; kb += 16)
-#pragma unroll
-                for (int k = kb; k < kb + 16; k += 4) {
-                    const double av = -kp[static_cast<int64_t>(k + lk) * np];
-                    d0 = mfma_16x16x4(av, Xs[(k + lk) * 32 + li], d0);
-                    d1 = mfma_16x16x4(av, Xs[(k + lk) * 32 + 16 + li], d1);
-                }
-            if (last) {
-                double rm0 = 0.0, rm1 = 0.0;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    rm0 = fmax(rm0, fabs(d0[r]));
-                    rm1 = fmax(rm1, fabs(d1[r]));
-                }
-                for (int off = 16; off < 64; off <<= 1) {
-                    rm0 = fmax(rm0, __shfl_xor(rm0, off, 64));
-                    rm1 = fmax(rm1, __shfl_xor(rm1, off, 64));
-                    bm0 = fmax(bm0, __shfl_xor(bm0, off, 64));
-                    bm1 = fmax(bm1, __shfl_xor(bm1, off, 64));
-                }
-                if (lk == 0) {
-                    // non-negative doubles order like their bit patterns
-                    atomicMax(&colmax[li], static_cast<unsigned long long>(__double_as_longlong(rm0)));
-                    atomicMax(&colmax[16 + li], static_cast<unsigned long long>(__double_as_longlong(rm1)));
-                    atomicMax(&colmax[32 + li], static_cast<unsigned long long>(__double_as_longlong(bm0)));
-                    atomicMax(&colmax[48 + li], static_cast<unsigned long long>(__double_as_longlong(bm1)));
-                }
-            } else {
+            for (int ct = 0; ct < CT; ++ct) {
+                const int col = ct * 16 + li;
+                bm[ct] = 0.0;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = rt * 16 + drow<double>(lk, r);
-                    Rs[row * 32 + li] = static_cast<FT>(d0[r]);
-                    Rs[row * 32 + 16 + li] = static_cast<FT>(d1[r]);
+                    acc[ct][r] = (col < ncw) ? Bq[static_cast<int64_t>(row) * ncols + col] : 0.0;
+                    bm[ct] = fmax(bm[ct], fabs(acc[ct][r]));
                 }
+            }
+            tile_product<CT>(Kq + rt * 16 + li, np, Xs, CW, li, lk, acc, true);
+            if (last) {
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    double rm = 0.0, bb = bm[ct];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) rm = fmax(rm, fabs(acc[ct][r]));
+                    for (int off = 16; off < 64; off <<= 1) {
+                        rm = fmax(rm, __shfl_xor(rm, off, 64));
+                        bb = fmax(bb, __shfl_xor(bb, off, 64));
+                    }
+                    if (lk == 0) {
+                        // non-negative doubles order like their bit patterns
+                        atomicMax(&colmax[ct * 16 + li], static_cast<unsigned long long>(__double_as_longlong(rm)));
+                        atomicMax(&colmax[CW + ct * 16 + li],
+                                  static_cast<unsigned long long>(__double_as_longlong(bb)));
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        Rs[(rt * 16 + drow<double>(lk, r)) * CW + ct * 16 + li] = static_cast<FT>(acc[ct][r]);
             }
         }
         __syncthreads();
         if (last) break;
         // X += Ki R in FT
         for (int rt = wave; rt < nrt; rt += nw) {
-            V4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
-            const FT* ip = Iq + rt * 16 + li;
-            for (int kb = 0; kb < np; kb += 16)
+            V4 acc[CT];
 #pragma unroll
-                for (int k = kb; k < kb + 16; k += 4) {
-                    const FT av = ip[static_cast<int64_t>(k + lk) * np];
-                    d0 = mfma_16x16x4(av, Rs[(k + lk) * 32 + li], d0);
-                    d1 = mfma_16x16x4(av, Rs[(k + lk) * 32 + 16 + li], d1);
-                }
+            for (int ct = 0; ct < CT; ++ct) acc[ct] = V4{0, 0, 0, 0};
+            tile_product<CT>(Iq + rt * 16 + li, np, Rs, CW, li, lk, acc, false);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = rt * 16 + drow<FT>(lk, r);
-                Xs[row * 32 + li] += static_cast<double>(d0[r]);
-                Xs[row * 32 + 16 + li] += static_cast<double>(d1[r]);
-            }
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    Xs[(rt * 16 + drow<FT>(lk, r)) * CW + ct * 16 + li] += static_cast<double>(acc[ct][r]);
         }
         __syncthreads();
     }
@@ -767,7 +774,7 @@ __global__ void __launch_bounds__(PATCH_THREADS)
             double rm = 0.0, bm = 0.0;
             for (int c = s0; c < s0 + Nt; ++c) {
                 rm = fmax(rm, __longlong_as_double(static_cast<long long>(colmax[c])));
-                bm = fmax(bm, __longlong_as_double(static_cast<long long>(colmax[32 + c])));
+                bm = fmax(bm, __longlong_as_double(static_cast<long long>(colmax[CW + c])));
             }
             worst = fmax(worst, rm / fmax(1.0, bm));
         }
@@ -780,7 +787,7 @@ __global__ void __launch_bounds__(PATCH_THREADS)
         const int i = r / ncw, c = r % ncw;
         const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + i];
         const double val =
-            (fi >= 0) ? topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i] * Xs[fi * 32 + c] : 0.0;
+            (fi >= 0) ? topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i] * Xs[fi * CW + c] : 0.0;
         const int64_t sc = topo.cells[cb + ci];
         sig[((sc * P.NV + topo.lv[cb + ci]) * Nf + i) * ncols + c0 + c] = val;
     }
@@ -1153,13 +1160,11 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         HIP_TRY(hipMemsetAsync(B.patch_res, 0, S.patches.n_patches * sizeof(double), st));
         const size_t ft_bytes = (prec == APF_FP64) ? sizeof(double) : sizeof(float);
         const size_t lds_cap = 160 * 1024;
-        if (static_cast<size_t>(B.nmax) * 32 * (sizeof(double) + ft_bytes) > lds_cap)
+        if (static_cast<size_t>(B.nmax) * 16 * (sizeof(double) + ft_bytes) > lds_cap - 4096)
             return fail(APF_BAD_ARG, "apf_estimate: patch order " + std::to_string(B.nmax) +
                                          " exceeds the batched patch-solve limit of this build");
         PatchSlots ps{B.s_n, B.s_np, B.s_off2, B.s_off1};
         const int ncols = P.m * P.Nt;
-        const int cpw = (32 / P.Nt) * P.Nt;
-        const int ncg = (ncols + cpw - 1) / cpw;
         auto set_lds = [&](const void* fn, size_t bytes) {
             HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
         };
@@ -1193,6 +1198,7 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor<FT>), dim3(bk.count), dim3(PATCH_THREADS), lf, st, P,
                                    Tb, topo, ps, bk.q0, in_lds, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
             }
+            HIP_TRY(hipEventRecord(sys->evp[1], st));
             for (const auto& bk : B.buckets) {
                 int sg = std::min(8, P.m);
                 auto lr = [&](int g) {
@@ -1218,11 +1224,31 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
                     default: throw std::runtime_error("patch right-hand sides: unsupported (dim, p)");
                 }
             }
+            HIP_TRY(hipEventRecord(sys->evp[2], st));
             for (const auto& bk : B.buckets) {
-                const size_t ls = static_cast<size_t>(bk.np) * 32 * (sizeof(double) + sizeof(FT));
-                set_lds(reinterpret_cast<const void*>(&k_patch_solve<FT>), ls);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_solve<FT>), dim3(bk.count * ncg), dim3(PATCH_THREADS), ls,
-                                   st, P, topo, ps, bk.q0, ncg, cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res);
+                // widest column block (16 CT columns, CT <= 6) whose X / R tiles fit the LDS
+                int CT = 6;
+                while (CT > 1 && static_cast<size_t>(bk.np) * 16 * CT * (sizeof(double) + sizeof(FT)) > lds_cap - 4096)
+                    --CT;
+                CT = std::min(CT, (ncols + 15) / 16);
+                const int cpw = std::max(P.Nt, (16 * CT / P.Nt) * P.Nt);
+                if (cpw > 16 * CT) throw std::runtime_error("patch solve: a slab's modes exceed the column block");
+                const int ncg = (ncols + cpw - 1) / cpw;
+                const size_t ls = static_cast<size_t>(bk.np) * 16 * CT * (sizeof(double) + sizeof(FT));
+                const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * ncg);
+                auto solve = [&](auto kern) {
+                    set_lds(reinterpret_cast<const void*>(kern), ls);
+                    hipLaunchKernelGGL(kern, dim3(grid), dim3(SOLVE_THREADS), ls, st, P, topo, ps, bk.q0, bk.count,
+                                       ncg, cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res);
+                };
+                switch (CT) {
+                    case 1: solve(k_patch_solve<FT, 1>); break;
+                    case 2: solve(k_patch_solve<FT, 2>); break;
+                    case 3: solve(k_patch_solve<FT, 3>); break;
+                    case 4: solve(k_patch_solve<FT, 4>); break;
+                    case 5: solve(k_patch_solve<FT, 5>); break;
+                    default: solve(k_patch_solve<FT, 6>); break;
+                }
             }
         };
         HIP_TRY(hipEventRecord(sys->ev0, st));
@@ -1265,6 +1291,9 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         HIP_TRY(hipStreamSynchronize(st));
         float ms1 = 0.f, ms2 = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms1, sys->ev0, sys->ev1));
+        HIP_TRY(hipEventElapsedTime(&sys->t_factor_ms, sys->ev0, sys->evp[1]));
+        HIP_TRY(hipEventElapsedTime(&sys->t_rhs_ms, sys->evp[1], sys->evp[2]));
+        HIP_TRY(hipEventElapsedTime(&sys->t_solve_ms, sys->evp[2], sys->ev1));
         HIP_TRY(hipEventElapsedTime(&ms2, e2, e3));
         (void)hipEventDestroy(e2);
         (void)hipEventDestroy(e3);

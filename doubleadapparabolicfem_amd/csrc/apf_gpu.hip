@@ -849,6 +849,7 @@ int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
         (void)Ns;
         HIP_TRY(hipEventCreate(&sys->ev0));
         HIP_TRY(hipEventCreate(&sys->ev1));
+        for (auto& e : sys->evp) HIP_TRY(hipEventCreate(&e));
         estimator_alloc(sys.get());
         *out = sys.release();
         return APF_OK;
@@ -867,6 +868,8 @@ void apf_system_destroy(apf_system* sys)
     for (void* p : sys->allocs) (void)hipFree(p);
     if (sys->ev0) (void)hipEventDestroy(sys->ev0);
     if (sys->ev1) (void)hipEventDestroy(sys->ev1);
+    for (auto& e : sys->evp)
+        if (e) (void)hipEventDestroy(e);
     delete sys;
 }
 
@@ -1291,6 +1294,15 @@ int apf_system_timings(apf_system* sys, double* ms4)
     ms4[1] = sys->t_patch_ms;
     ms4[2] = sys->t_diag_ms;
     ms4[3] = sys->t_spmv_ms;
+    return APF_OK;
+}
+
+int apf_estimate_phase_timings(apf_system* sys, double* ms3)
+{
+    if (!sys || !ms3) return set_error(nullptr, APF_BAD_ARG, "apf_estimate_phase_timings: null argument");
+    ms3[0] = sys->t_factor_ms;
+    ms3[1] = sys->t_rhs_ms;
+    ms3[2] = sys->t_solve_ms;
     return APF_OK;
 }
 

@@ -209,7 +209,9 @@ struct System {
     uint64_t assembly_version = 0, mg64_version = 0, mg32_version = 0;
     double t_mg_setup_s = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evp[4] = {nullptr, nullptr, nullptr, nullptr};  // estimator phase boundaries
     float t_assembly_ms = 0.f, t_patch_ms = 0.f, t_diag_ms = 0.f, t_spmv_ms = 0.f;
+    float t_factor_ms = 0.f, t_rhs_ms = 0.f, t_solve_ms = 0.f;
     bool assembled = false;
     bool use_graph = true;   // capture the MINRES iteration in a hipGraph
     int spmv_lanes = 16;     // lanes per CSR row in k_spmv

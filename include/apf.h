@@ -185,6 +185,10 @@ int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_estimate_out* 
 /* Device timing of the last call of each phase in milliseconds (HIP events):
  * [0] assembly, [1] patch kernel, [2] diagnostics, [3] one SpMV (last). */
 int apf_system_timings(apf_system* sys, double* ms4);
+/* Device time of the three patch phases of the last apf_estimate in
+ * milliseconds: [0] patch matrices and inverses, [1] right-hand sides,
+ * [2] batched solves with refinement. */
+int apf_estimate_phase_timings(apf_system* sys, double* ms3);
 
 #ifdef __cplusplus
 }
