@@ -86,6 +86,12 @@ class _EstimateOut(ctypes.Structure):
                 ("seconds_patches", ctypes.c_double), ("seconds_diagnostics", ctypes.c_double)]
 
 
+class _DoerflerDiag(ctypes.Structure):
+    _fields_ = [("total", ctypes.c_double), ("target", ctypes.c_double), ("cutoff_value", ctypes.c_double),
+                ("cutoff_margin", ctypes.c_double), ("max_indicator", ctypes.c_double),
+                ("cutoff_index", ctypes.c_int64), ("near_tie_count", ctypes.c_int64)]
+
+
 EXPORTED = {
     "apf_space_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(_MeshDesc), ctypes.c_int]),
     "apf_space_destroy": (None, [ctypes.c_void_p]),
@@ -117,6 +123,12 @@ EXPORTED = {
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "apf_system_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_estimate_phase_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_combine_estimator": (ctypes.c_int, [ctypes.c_int64] + [ctypes.c_void_p] * 4),
+    "apf_doerfler_mark": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(_DoerflerDiag)]),
+    "apf_y_estimator_threshold": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.POINTER(ctypes.c_double)]),
 }
 
 
@@ -338,3 +350,40 @@ class System:
         t = np.zeros(3)
         _check(lib().apf_estimate_phase_timings(self.handle, _ptr(t)))
         return dict(factor_ms=t[0], rhs_ms=t[1], solve_ms=t[2])
+
+
+def combine_estimator(flux2, recon2, resid2=None):
+    """estimator^2 = flux^2 + reconstruction^2 per cell (host C++;
+    time_slab_estimator_combination.hpp:88-250).  Raises APFError on a negative
+    or non-finite squared quantity."""
+    f = np.ascontiguousarray(flux2, dtype=np.float64)
+    r = np.ascontiguousarray(recon2, dtype=np.float64)
+    d = None if resid2 is None else np.ascontiguousarray(resid2, dtype=np.float64)
+    if r.shape != f.shape or (d is not None and d.shape != f.shape):
+        raise ValueError("combine_estimator: shape mismatch")
+    out = np.empty_like(f)
+    _check(lib().apf_combine_estimator(f.size, _ptr(f), _ptr(r), _ptr(d), _ptr(out)))
+    return out
+
+
+def doerfler_mark(values, theta, ids=None, near_tie_tolerance=0.0):
+    """Deterministic Doerfler marking (host C++; time_slab_cellwise_errors.hpp:156-280).
+    Returns (marked ids in marking order, diagnostics dict)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    i = None if ids is None else np.ascontiguousarray(ids, dtype=np.int64)
+    if i is not None and i.shape != v.shape:
+        raise ValueError("doerfler_mark: ids and values differ in length")
+    marked = np.empty(max(v.size, 1), dtype=np.int64)
+    nm = ctypes.c_int64()
+    dg = _DoerflerDiag()
+    _check(lib().apf_doerfler_mark(v.size, _ptr(i), _ptr(v), float(theta), float(near_tie_tolerance), _ptr(marked),
+                                   ctypes.byref(nm), ctypes.byref(dg)))
+    diag = {name: getattr(dg, name) for name, _ in _DoerflerDiag._fields_}
+    return marked[:nm.value].copy(), diag
+
+
+def y_estimator_threshold(rho, lambda_y2, trace2):
+    """rho^2 (lambda_Y^2 + (1 + 1/(rho+1/2)^2) trace^2) (indicator_utilities.hpp:475-492)."""
+    out = ctypes.c_double()
+    _check(lib().apf_y_estimator_threshold(float(rho), float(lambda_y2), float(trace2), ctypes.byref(out)))
+    return out.value

@@ -190,6 +190,30 @@ int apf_system_timings(apf_system* sys, double* ms4);
  * [2] batched solves with refinement. */
 int apf_estimate_phase_timings(apf_system* sys, double* ms3);
 
+/* ---- marking (host; no GPU needed) ---- */
+typedef struct {
+    double total, target;          /* Neumaier total over ids ascending, theta * total */
+    double cutoff_value, cutoff_margin, max_indicator;
+    int64_t cutoff_index;          /* position of the last marked entry (n if none) */
+    int64_t near_tie_count;        /* entries within near_tie_tolerance of the cutoff value */
+} apf_doerfler_diag;
+
+/* estimator^2 = flux^2 + reconstruction^2 per cell; every input (resid2 may be
+ * NULL) must be finite and >= -1e-14.  Replaces
+ * combine_time_slab_estimator_squared_errors_deterministic
+ * (src/finite_element/time_slabs/time_slab_estimator_combination.hpp:88-250). */
+int apf_combine_estimator(int64_t n, const double* flux2, const double* recon2, const double* resid2, double* est2);
+/* Deterministic Doerfler marking: cell ids (NULL -> 0..n-1, unique) sorted by
+ * value descending, ties to the lower id, marked until the compensated
+ * cumulative sum reaches theta * total; marked[] receives the ids in marking
+ * order.  Replaces CellwiseSquaredError::doerfler_marking_deterministic
+ * (src/finite_element/time_slabs/time_slab_cellwise_errors.hpp:156-280). */
+int apf_doerfler_mark(int64_t n, const int64_t* ids, const double* values, double theta, double near_tie_tolerance,
+                      int64_t* marked, int64_t* n_marked, apf_doerfler_diag* diag);
+/* rho^2 (lambda_Y^2 + (1 + 1/(rho + 1/2)^2) trace^2).  Replaces
+ * compute_y_estimator_threshold_squared (algorithm/detail/indicator_utilities.hpp:475-492). */
+int apf_y_estimator_threshold(double rho, double lambda_y2, double trace2, double* out);
+
 #ifdef __cplusplus
 }
 #endif
