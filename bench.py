@@ -28,10 +28,10 @@ METRIC = "assembly+estimator DOF/s & MINRES s/solve; mixed vs FP64 at 1e-10 rel 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-# calibration) from the rocprofv3 --pmc runs of `bench.py --profile` on the
-# same code, summarised in profiles/counters_c2.md.
-MEASURED_TRAFFIC = {}
+# HBM bytes per launch (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors a
+# plain copy kernel of known size reads) from the rocprofv3 --pmc runs of
+# `bench.py --profile`, summarised in profiles/.
+MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9}}
 
 
 def workload(name):
@@ -54,7 +54,7 @@ def workload(name):
         coeff = meshes.smooth_initial(2)
         coeff = apf.Coefficients(meshes.checkerboard_kappa(m, 8, 4), 0.0, coeff.u0)
         return dict(mesh=m, p=3, coarse=[meshes.union_jack_mesh(n, [0.0, 1.0]) for n in (32, 16, 8, 4, 2)],
-                    coeff=coeff, anchor=None,
+                    coeff=coeff, anchor=None, precond=apf.APF_FP32_REFINE,
                     desc="2+1D p=3, union-jack 64x64 x 16 intervals, kappa piecewise constant 10^U(-1,1) "
                          "on an 8x8 checkerboard")
     raise SystemExit(f"unknown config {name}")
@@ -168,7 +168,8 @@ def main():
         sysm.assemble(W["coeff"])
         t_asm = sysm.timings()["assembly_ms"]
         t0 = time.perf_counter()
-        x, st = sysm.minres(tol=1e-10, max_it=5000, check_interval=10)
+        x, st = sysm.minres(tol=1e-10, max_it=5000, check_interval=10, precond=W.get("precond", apf.APF_FP64),
+                            download=False)
         t_minres = time.perf_counter() - t0
         est = sysm.estimate(prec, 2, per_cell=False)
         t_est = (est["seconds_patches"] + est["seconds_diagnostics"]) * 1e3

@@ -299,10 +299,11 @@ class System:
         _check(lib().apf_system_set_solution(self.handle, _ptr(x)), self.ctx.handle)
 
     def minres(self, tol=1e-10, max_it=20000, check_interval=100, precond=APF_FP64, x0=None, raise_on_fail=True,
-               kind=None):
-        """Preconditioned MINRES; kind None -> multigrid when coarse meshes were given."""
+               kind=None, download=True):
+        """Preconditioned MINRES; kind None -> multigrid when coarse meshes were given.  With
+        download=False the solution stays on the device (returned x is None)."""
         st = _MinresStats()
-        x = np.zeros(self.space.n)
+        x = np.zeros(self.space.n) if download else None
         x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         if kind is None:
             kind = APF_PRECOND_MULTIGRID if self.space.n_coarse else APF_PRECOND_DIAGONAL
