@@ -171,7 +171,7 @@ __device__ void gj_invert(FT* A, int n, int ld, int* perm)
             int bi = n;
             for (int r = k + lane; r < n; r += 64) {
                 const double v = static_cast<double>(A[static_cast<int64_t>(r) * ld + k]);
-                if (fabs(v) > best) {
+                if (!(fabs(v) <= best)) {  // a NaN entry wins, so the pivot row is always a real row
                     best = fabs(v);
                     bval = v;
                     bi = r;
@@ -181,13 +181,13 @@ __device__ void gj_invert(FT* A, int n, int ld, int* perm)
                 const double ob = __shfl_xor(best, off, 64);
                 const double ov = __shfl_xor(bval, off, 64);
                 const int oi = __shfl_xor(bi, off, 64);
-                if (ob > best || (ob == best && oi < bi)) {
+                if (!(ob <= best) || (ob == best && oi < bi)) {
                     best = ob;
                     bval = ov;
                     bi = oi;
                 }
             }
-            const int pr = bi;
+            const int pr = min(max(bi, k), n - 1);
             const FT inv = static_cast<FT>(1) / static_cast<FT>(bval);
             for (int j = lane; j < n; j += 64) {
                 const FT v = A[static_cast<int64_t>(pr) * ld + j];
@@ -367,44 +367,33 @@ __device__ inline double first_lane(double v)
     return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
-// Phase 1 (patches of order <= min(NW * RPW, 64 * CPL)): the patch matrix
-// lives in registers, row i = wave + NW il in A[il][.], column
-// j = lane + 64 t in A[.][t].  Gauss-Jordan with partial pivoting among the
-// rows not yet used as pivots (ties to the lowest row) and no row interchange:
-// step k scales pivot row r_k, broadcasts it through LDS and eliminates column
-// k from every other row, the multiplier A[i][k] coming from lane k % 64 by
-// readlane.  After n steps A holds T with inv(K)[k][r_k'] = T[r_k][k'], which is
-// how the inverse is written to the Ki slot.  Two barriers per step; the
-// pivot candidates of column k+1 are found while column k is eliminated.
+// Shared state of the register-resident Gauss-Jordan below.
+template <typename FT, int NW, int CPL>
+struct GJShared {
+    FT prow[64 * CPL];
+    double cabs[NW], cval[NW];
+    int cidx[NW];
+    int pivrow[64 * CPL], rinv[64 * CPL];
+};
+
+// In-place Gauss-Jordan inversion of an n x n matrix held in registers, row
+// i = wave + NW il in A[il][.], column j = lane + 64 t in A[.][t]
+// (n <= min(NW RPW, 64 CPL)).  Partial pivoting among the rows not yet used as
+// pivots (ties to the lowest row), no row interchange: step k scales pivot row
+// r_k, broadcasts it through LDS and eliminates column k from every other row,
+// the multiplier A[i][k] coming from lane k % 64 by readlane.  Afterwards A
+// holds T with inv[k][r_k'] = T[r_k][k'] (sh.pivrow[k] = r_k, sh.rinv[r_k] = k).
+// Two barriers per step; the pivot candidates of column k+1 are found while
+// column k is eliminated.  Wave id, pivot row and multipliers are
+// wave-uniform and kept in scalar registers.
 template <typename FT, int NW, int RPW, int CPL>
-__global__ void __launch_bounds__(NW * 64)
-    k_patch_factor_reg(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, const double* verts,
-                       const int* cells, const double* Mcell, double* Kd, FT* Ki)
+__device__ void gj_registers(FT (&A)[RPW][CPL], int n, GJShared<FT, NW, CPL>& sh)
 {
-    __shared__ FT prow_s[64 * CPL];
-    __shared__ double cabs[NW], cval[NW];
-    __shared__ int cidx[NW];
-    __shared__ int pivrow[64 * CPL], rinv[64 * CPL];
-    const int q = q0 + blockIdx.x;
-    const int a = topo.order[q];
-    const int n = ps.n[q], np = ps.np[q];
-    double* K = Kd + ps.off2[q];
-    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
-    // wave, pivot row and multipliers are wave-uniform: keep them in scalar registers
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    FT A[RPW][CPL];
-    unsigned used = 0;  // bit il: row wave + NW il is a pivot row already, or padding
+    unsigned used = 0;  // bit il: row wave + NW il is a pivot row already, or beyond n
 #pragma unroll
-    for (int il = 0; il < RPW; ++il) {
-        const int i = wave + NW * il;
-        if (i >= n) used |= 1u << il;
-#pragma unroll
-        for (int t = 0; t < CPL; ++t) {
-            const int j = lane + 64 * t;
-            A[il][t] = (i < n && j < n) ? static_cast<FT>(K[static_cast<int64_t>(i) * np + j]) : static_cast<FT>(0);
-        }
-    }
-    // pivot candidate of this wave for column kc: largest |A[i][kc]| over unused rows
+    for (int il = 0; il < RPW; ++il)
+        if (wave + NW * il >= n) used |= 1u << il;
     auto candidates = [&](int kc) {
         const int tk = kc >> 6, lk = kc & 63;
         double best = -1.0, bval = 0.0;
@@ -417,7 +406,7 @@ __global__ void __launch_bounds__(NW * 64)
             for (int t = 0; t < CPL; ++t)
                 if (t == tk) v = A[il][t];
             const double av = fabs(static_cast<double>(v));
-            if (av > best) {
+            if (!(av <= best)) {  // a NaN entry wins, so the pivot row is always a real row
                 best = av;
                 bval = static_cast<double>(v);
                 bi = wave + NW * il;
@@ -427,11 +416,12 @@ __global__ void __launch_bounds__(NW * 64)
         bval = lane_value(bval, lk);
         bi = __builtin_amdgcn_readlane(bi, lk);
         if (lane == 0) {
-            cabs[wave] = best;
-            cval[wave] = bval;
-            cidx[wave] = bi;
+            sh.cabs[wave] = best;
+            sh.cval[wave] = bval;
+            sh.cidx[wave] = bi;
         }
     };
+    for (int i = threadIdx.x; i < 64 * CPL; i += blockDim.x) sh.rinv[i] = i;
     candidates(0);
     __syncthreads();
     for (int k = 0; k < n; ++k) {
@@ -439,15 +429,16 @@ __global__ void __launch_bounds__(NW * 64)
         int r = 0x7fffffff;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            const double b = cabs[w];
-            const int ri = cidx[w];
-            if (b > best || (b == best && ri < r)) {
+            const double b = sh.cabs[w];
+            const int ri = sh.cidx[w];
+            if (!(b <= best) || (b == best && ri < r)) {
                 best = b;
-                pv = cval[w];
+                pv = sh.cval[w];
                 r = ri;
             }
         }
-        r = __builtin_amdgcn_readfirstlane(r);
+        // every index below stays inside the matrix whatever the data
+        r = __builtin_amdgcn_readfirstlane(min(max(r, 0), n - 1));
         pv = first_lane(pv);
         if (wave == r % NW) {
             const int ilr = r / NW;
@@ -460,19 +451,19 @@ __global__ void __launch_bounds__(NW * 64)
                         const int j = lane + 64 * t;
                         const FT v = ((j == k) ? static_cast<FT>(1) : A[il][t]) * inv;
                         A[il][t] = v;
-                        prow_s[j] = v;
+                        sh.prow[j] = v;
                     }
                     used |= 1u << il;
                 }
         }
         if (threadIdx.x == 0) {
-            pivrow[k] = r;
-            rinv[r] = k;
+            sh.pivrow[k] = r;
+            sh.rinv[r] = k;
         }
         __syncthreads();
         FT pr[CPL];
 #pragma unroll
-        for (int t = 0; t < CPL; ++t) pr[t] = prow_s[lane + 64 * t];
+        for (int t = 0; t < CPL; ++t) pr[t] = sh.prow[lane + 64 * t];
         const int tk = k >> 6, lk = k & 63;
 #pragma unroll
         for (int il = 0; il < RPW; ++il) {
@@ -493,16 +484,210 @@ __global__ void __launch_bounds__(NW * 64)
         if (k + 1 < n) candidates(k + 1);
         __syncthreads();
     }
-    FT* Kg = Ki + ps.off2[q];
+}
+
+// Operand staging of wg_gemm: k-chunks of both factors (M, N <= 128).
+struct GemmShared {
+    static constexpr int KC = 16;
+    double a[128][KC + 1];
+    double b[KC][128 + 4];
+};
+
+// C(i, j) = sum_k A(i, k) B(k, j) for i < M, j < N (M, N <= 128), handed to
+// store(i, j, value), with blockDim.x * TPT >= 32 * 32.  k-chunks of both
+// factors are staged in LDS with all their loads in flight, then every thread
+// updates its TPT 4 x 4 register tiles from LDS.
+template <int TPT, typename FA, typename FB, typename FS>
+__device__ void wg_gemm(int M, int N, int K, FA A, FB B, FS store, GemmShared& g)
+{
+    constexpr int KC = GemmShared::KC;
+    const int mt = (M + 3) / 4, nt = (N + 3) / 4;
+    double acc[TPT][4][4] = {};
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        const int kc = min(KC, K - k0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < M * KC; e += blockDim.x) {
+            const int i = e / KC, kk = e % KC;
+            g.a[i][kk] = (kk < kc) ? A(i, k0 + kk) : 0.0;
+        }
+        for (int e = threadIdx.x; e < N * KC; e += blockDim.x) {
+            const int j = e / KC, kk = e % KC;
+            g.b[kk][j] = (kk < kc) ? B(k0 + kk, j) : 0.0;
+        }
+        __syncthreads();
 #pragma unroll
-    for (int il = 0; il < RPW; ++il) {
-        const int i = wave + NW * il;
+        for (int u = 0; u < TPT; ++u) {
+            const int tile = threadIdx.x + u * blockDim.x;
+            if (tile < mt * nt) {
+            const int i0 = (tile % mt) * 4, j0 = (tile / mt) * 4;
+            for (int kk = 0; kk < kc; ++kk) {
+                double av[4], bv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) av[r] = (i0 + r < M) ? g.a[i0 + r][kk] : 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) bv[c] = (j0 + c < N) ? g.b[kk][j0 + c] : 0.0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[u][r][c] += av[r] * bv[c];
+            }
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+        const int tile = threadIdx.x + u * blockDim.x;
+        const int i0 = (tile % mt) * 4, j0 = (tile / mt) * 4;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (tile < mt * nt && i0 + r < M && j0 + c < N) store(i0 + r, j0 + c, acc[u][r][c]);
+    }
+}
+
+// Phase 1 (patches of padded order <= min(NW RPW, 64 CPL)): the whole patch
+// matrix in registers, inverted by one register-resident Gauss-Jordan in FT,
+// written to the FT inverse slot with the pivot permutation folded in.
+template <typename FT, int NW, int RPW, int CPL>
+__global__ void __launch_bounds__(NW * 64)
+    k_patch_factor_full(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, const double* verts,
+                        const int* cells, const double* Mcell, double* Kd, FT* Ki)
+{
+    __shared__ GJShared<FT, NW, CPL> sh;
+    const int q = q0 + blockIdx.x;
+    const int a = topo.order[q];
+    const int n = ps.n[q], np = ps.np[q];
+    double* K = Kd + ps.off2[q];
+    FT* Kg = Ki + ps.off2[q];
+    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    FT R[RPW][CPL];
+#pragma unroll
+    for (int il = 0; il < RPW; ++il)
 #pragma unroll
         for (int t = 0; t < CPL; ++t) {
-            const int j = lane + 64 * t;
-            if (i < n && j < n) Kg[static_cast<int64_t>(rinv[i]) * np + pivrow[j]] = A[il][t];
-            else if (i < np && j < np) Kg[static_cast<int64_t>(i) * np + j] = static_cast<FT>(0);
+            const int i = wave + NW * il, j = lane + 64 * t;
+            R[il][t] = (i < n && j < n) ? static_cast<FT>(K[static_cast<int64_t>(i) * np + j]) : static_cast<FT>(0);
         }
+    gj_registers<FT, NW, RPW, CPL>(R, n, sh);
+#pragma unroll
+    for (int il = 0; il < RPW; ++il)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int i = wave + NW * il, j = lane + 64 * t;
+            if (i < n && j < n)
+                Kg[static_cast<int64_t>(sh.rinv[i]) * np + sh.pivrow[j]] = R[il][t];
+            else if (i < np && j < np)
+                Kg[static_cast<int64_t>(i) * np + j] = static_cast<FT>(0);
+        }
+}
+
+// Phase 1 (patches whose flux block and Schur block both have order <=
+// min(NW RPW, 64 CPL)).  With the patch unknowns ordered [flux; scalar; mean
+// multiplier] the matrix is K = [A C^T; C D], A the SPD flux mass.  Block
+// elimination gives
+//   K^-1 = [A^-1 + G H^T, -H; -H^T, S^-1],  G = A^-1 C^T,  S = D - C G,  H^T = S^-1 G^T,
+// so the dense inversions shrink from order n to nF (A) and n - nF (S), both
+// register-resident Gauss-Jordan; the products between them are workgroup
+// GEMMs on the FP64 inverse slot: A^-1 in the top-left block, G in the
+// top-right, S and then S^-1 in the bottom-right, H^T in the bottom-left.
+// Everything is FP64: the top-left block is A^-1 minus a matrix of similar
+// size (the inverse on the divergence-free fluxes), and an FP32 A^-1 loses
+// too much there.  With packed FP32 off, FP64 and FP32 VALU rates are equal.
+// For FT = float the result is also rounded into Ki32 for the FP32 products.
+template <typename FT, int NW, int RPW, int CPL>
+__global__ void __launch_bounds__(NW * 64)
+    k_patch_factor_schur(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, const double* verts,
+                         const int* cells, const double* Mcell, double* Kd, double* Ki, float* Ki32)
+{
+    __shared__ GJShared<double, NW, CPL> sh;
+    __shared__ GemmShared gs;
+    const int q = q0 + blockIdx.x;
+    const int a = topo.order[q];
+    const int n = ps.n[q], np = ps.np[q];
+    const int nF = topo.nflux[a], m = n - nF;
+    double* K = Kd + ps.off2[q];
+    double* Kg = Ki + ps.off2[q];
+    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double R[RPW][CPL];
+    // 1. A^-1 into the top-left block
+#pragma unroll
+    for (int il = 0; il < RPW; ++il)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int i = wave + NW * il, j = lane + 64 * t;
+            R[il][t] = (i < nF && j < nF) ? K[static_cast<int64_t>(i) * np + j] : 0.0;
+        }
+    gj_registers<double, NW, RPW, CPL>(R, nF, sh);
+#pragma unroll
+    for (int il = 0; il < RPW; ++il)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int i = wave + NW * il, j = lane + 64 * t;
+            if (i < nF && j < nF) Kg[static_cast<int64_t>(sh.rinv[i]) * np + sh.pivrow[j]] = R[il][t];
+        }
+    __syncthreads();
+    // 2. G = A^-1 C^T  (nF x m, top-right)
+    wg_gemm<32 * 32 / (NW * 64)>(
+        nF, m, nF, [&](int i, int k) { return Kg[static_cast<int64_t>(i) * np + k]; },
+        [&](int k, int s) { return K[static_cast<int64_t>(nF + s) * np + k]; },
+        [&](int i, int s, double v) { Kg[static_cast<int64_t>(i) * np + nF + s] = v; }, gs);
+    __syncthreads();
+    // 3. S = D - C G  (m x m, bottom-right)
+    wg_gemm<32 * 32 / (NW * 64)>(
+        m, m, nF, [&](int s, int k) { return K[static_cast<int64_t>(nF + s) * np + k]; },
+        [&](int k, int s2) { return Kg[static_cast<int64_t>(k) * np + nF + s2]; },
+        [&](int s, int s2, double v) {
+            const int64_t e = static_cast<int64_t>(nF + s) * np + nF + s2;
+            Kg[e] = K[e] - v;
+        }, gs);
+    __syncthreads();
+    // 4. S^-1 in place
+#pragma unroll
+    for (int il = 0; il < RPW; ++il)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int i = wave + NW * il, j = lane + 64 * t;
+            R[il][t] = (i < m && j < m) ? Kg[static_cast<int64_t>(nF + i) * np + nF + j] : 0.0;
+        }
+    gj_registers<double, NW, RPW, CPL>(R, m, sh);
+#pragma unroll
+    for (int il = 0; il < RPW; ++il)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int i = wave + NW * il, j = lane + 64 * t;
+            if (i < m && j < m) Kg[static_cast<int64_t>(nF + sh.rinv[i]) * np + nF + sh.pivrow[j]] = R[il][t];
+        }
+    __syncthreads();
+    // 5. H^T = S^-1 G^T  (m x nF, bottom-left)
+    wg_gemm<32 * 32 / (NW * 64)>(
+        m, nF, m, [&](int s, int k) { return Kg[static_cast<int64_t>(nF + s) * np + nF + k]; },
+        [&](int k, int j) { return Kg[static_cast<int64_t>(j) * np + nF + k]; },
+        [&](int s, int j, double v) { Kg[static_cast<int64_t>(nF + s) * np + j] = v; }, gs);
+    __syncthreads();
+    // 6. top-left += G H^T
+    wg_gemm<32 * 32 / (NW * 64)>(
+        nF, nF, m, [&](int i, int k) { return Kg[static_cast<int64_t>(i) * np + nF + k]; },
+        [&](int k, int j) { return Kg[static_cast<int64_t>(nF + k) * np + j]; },
+        [&](int i, int j, double v) { Kg[static_cast<int64_t>(i) * np + j] += v; }, gs);
+    __syncthreads();
+    // 7. off-diagonal blocks -H, -H^T; zero padding rows and columns
+    for (int e = threadIdx.x; e < m * nF; e += blockDim.x) {
+        const int s = e / nF, j = e % nF;
+        const double v = -Kg[static_cast<int64_t>(nF + s) * np + j];
+        Kg[static_cast<int64_t>(nF + s) * np + j] = v;
+        Kg[static_cast<int64_t>(j) * np + nF + s] = v;
+    }
+    for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
+        const int i = e / np, j = e % np;
+        if (i >= n || j >= n) Kg[e] = 0.0;
+    }
+    if constexpr (std::is_same<FT, float>::value) {
+        __syncthreads();
+        float* K32 = Ki32 + ps.off2[q];
+        for (int e = threadIdx.x; e < np * np; e += blockDim.x) K32[e] = static_cast<float>(Kg[e]);
     }
 }
 
@@ -1081,9 +1266,10 @@ void estimator_alloc(System* sys)
         o2 += static_cast<int64_t>(snp[q]) * snp[q];
         o1 += snp[q];
         const int nc = P.cell_ptr[a + 1] - P.cell_ptr[a];
-        if (B.buckets.empty() || B.buckets.back().np != snp[q]) B.buckets.push_back({q, 0, snp[q], 0});
+        if (B.buckets.empty() || B.buckets.back().np != snp[q]) B.buckets.push_back({q, 0, snp[q], 0, 0});
         B.buckets.back().count++;
         B.buckets.back().nc_max = std::max(B.buckets.back().nc_max, nc);
+        B.buckets.back().blk_max = std::max({B.buckets.back().blk_max, P.n_flux[a], pn[a] - P.n_flux[a]});
     }
     B.s_n = upi(sn);
     B.s_np = upi(snp);
@@ -1097,7 +1283,12 @@ void estimator_alloc(System* sys)
     B.nmax = P.n_patches ? snp[0] : 0;
     B.Kd = dalloc<double>(sys, static_cast<size_t>(o2));
     B.Ki = dalloc<double>(sys, static_cast<size_t>(o2));
+    B.Ki32 = dalloc<float>(sys, static_cast<size_t>(o2));
     B.RHS = dalloc<double>(sys, static_cast<size_t>(o1) * S.mesh.m() * T.Nt);
+    HIP_TRY(hipMemset(B.Kd, 0, static_cast<size_t>(o2) * sizeof(double)));
+    HIP_TRY(hipMemset(B.Ki, 0, static_cast<size_t>(o2) * sizeof(double)));
+    HIP_TRY(hipMemset(B.Ki32, 0, static_cast<size_t>(o2) * sizeof(float)));
+    HIP_TRY(hipMemset(B.RHS, 0, static_cast<size_t>(o1) * S.mesh.m() * T.Nt * sizeof(double)));
     const int64_t ncell = static_cast<int64_t>(S.mesh.m()) * S.mesh.ns();
     B.sig = dalloc<double>(sys, static_cast<size_t>(ncell) * (S.dim + 1) * T.Nf * T.Nt);
     B.flux = dalloc<double>(sys, ncell);
@@ -1170,26 +1361,26 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         };
         auto launch = [&](auto tag) {
             using FT = decltype(tag);
-            FT* Ki = reinterpret_cast<FT*>(B.Ki);
+            FT* Ki = std::is_same<FT, float>::value ? reinterpret_cast<FT*>(B.Ki32) : reinterpret_cast<FT*>(B.Ki);
             for (const auto& bk : B.buckets) {
-                // register-resident Gauss-Jordan where the matrix fits the register file
-                auto reg = [&](auto kern, int threads) {
-                    hipLaunchKernelGGL(kern, dim3(bk.count), dim3(threads), 0, st, P, Tb, topo, ps, bk.q0,
-                                       sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
-                };
+                // one register-resident Gauss-Jordan where the whole matrix fits the register file,
+                // else block elimination with two of them
                 if (bk.np <= 64) {
-                    reg(k_patch_factor_reg<FT, 4, 16, 1>, 256);
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_full<FT, 4, 16, 1>), dim3(bk.count), dim3(256), 0,
+                                       st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
                     continue;
                 }
                 if (bk.np <= 128) {
-                    reg(k_patch_factor_reg<FT, 4, 32, 2>, 256);
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_full<FT, 4, 32, 2>), dim3(bk.count), dim3(256), 0,
+                                       st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
                     continue;
                 }
-                if constexpr (std::is_same<FT, float>::value)
-                    if (bk.np <= 256) {
-                        reg(k_patch_factor_reg<FT, 8, 32, 4>, 512);
-                        continue;
-                    }
+                if (bk.blk_max <= 128) {
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_schur<FT, 8, 16, 2>), dim3(bk.count), dim3(512), 0,
+                                       st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, B.Ki,
+                                       B.Ki32);
+                    continue;
+                }
                 const size_t npb = static_cast<size_t>(bk.np);
                 const size_t in_lds_bytes = npb * npb * sizeof(FT) + npb * sizeof(int);
                 const int in_lds = in_lds_bytes <= lds_cap ? 1 : 0;

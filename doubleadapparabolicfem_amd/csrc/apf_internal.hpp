@@ -149,10 +149,12 @@ struct EstimatorBuffers {
     int64_t* s_off2 = nullptr;
     int64_t* s_off1 = nullptr;
     double* Kd = nullptr;    // FP64 patch matrices, np x np per patch
-    double* Ki = nullptr;    // inverses (FP64, or FP32 in the first half of each slot)
+    double* Ki = nullptr;    // FP64 inverses (and block-elimination workspace)
+    float* Ki32 = nullptr;   // FP32 inverses of the mixed path
     double* RHS = nullptr;   // right-hand sides, np x (slabs * Nt) per patch
     struct Bucket {
         int q0, count, np, nc_max;
+        int blk_max;  // largest flux block or Schur block order in the bucket
     };
     std::vector<Bucket> buckets;
     int nmax = 0;
