@@ -410,24 +410,61 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
         }
     out.shift.resize(T);
     for (int i = 0; i < T; ++i) out.shift[i] = std::sqrt(std::max(theta[i], 0.0));
-    // coarsest inverses for shift 0 and each time-mode shift
+    // coarsest inverses for shift 0 and each time-mode shift, per connected
+    // component when the components are contiguous node ranges
     const MGLevelHost& cl = out.levels.back();
     const int nc = cl.nf;
-    if (nc > 4096) throw std::runtime_error("setup_mg: coarsest level too large (" + std::to_string(nc) +
-                                             " nodes); pass coarser nested meshes");
+    std::vector<int> comp(nc, -1);
+    int ncomp = 0;
+    std::vector<int> stack;
+    for (int s0 = 0; s0 < nc; ++s0) {
+        if (comp[s0] >= 0) continue;
+        comp[s0] = ncomp;
+        stack.assign(1, s0);
+        while (!stack.empty()) {
+            const int i = stack.back();
+            stack.pop_back();
+            for (int e = cl.sptr[i]; e < cl.sptr[i + 1]; ++e)
+                if (comp[cl.scol[e]] < 0) {
+                    comp[cl.scol[e]] = ncomp;
+                    stack.push_back(cl.scol[e]);
+                }
+        }
+        ++ncomp;
+    }
+    bool contiguous = true;
+    for (int i = 1; i < nc; ++i)
+        if (comp[i] != comp[i - 1] && comp[i] != comp[i - 1] + 1) contiguous = false;
+    if (!contiguous) std::fill(comp.begin(), comp.end(), 0), ncomp = nc > 0 ? 1 : 0;
+    out.node_comp = comp;
+    out.comp_start.assign(ncomp + 1, nc);
+    for (int i = nc - 1; i >= 0; --i) out.comp_start[comp[i]] = i;
+    out.blk_off.assign(ncomp, 0);
+    out.blk_total = 0;
+    int bmax = 0;
+    for (int c = 0; c < ncomp; ++c) {
+        const int b = out.comp_start[c + 1] - out.comp_start[c];
+        out.blk_off[c] = out.blk_total;
+        out.blk_total += static_cast<int64_t>(b) * b;
+        bmax = std::max(bmax, b);
+    }
+    if (bmax > 4096) throw std::runtime_error("setup_mg: coarsest level too large (" + std::to_string(bmax) +
+                                              " coupled nodes); pass coarser nested meshes");
     out.shifts_all.clear();
     out.shifts_all.push_back(0.0);
     for (int i = 0; i < T; ++i) out.shifts_all.push_back(out.shift[i]);
     const int nsft = static_cast<int>(out.shifts_all.size());
-    out.coarse_inv.assign(static_cast<size_t>(nsft) * nc * nc, 0.0);
-    for (int q = 0; q < nsft; ++q) {
-        std::vector<double> A(static_cast<size_t>(nc) * nc, 0.0);
-        for (int i = 0; i < nc; ++i)
-            for (int e = cl.sptr[i]; e < cl.sptr[i + 1]; ++e)
-                A[static_cast<size_t>(i) * nc + cl.scol[e]] = cl.K[e] + out.shifts_all[q] * cl.M[e];
-        if (!invert_dense(nc, A)) throw std::runtime_error("setup_mg: singular coarsest operator");
-        std::copy(A.begin(), A.end(), out.coarse_inv.begin() + static_cast<size_t>(q) * nc * nc);
-    }
+    out.coarse_inv.assign(static_cast<size_t>(nsft) * out.blk_total, 0.0);
+    for (int q = 0; q < nsft; ++q)
+        for (int c = 0; c < ncomp; ++c) {
+            const int s = out.comp_start[c], b = out.comp_start[c + 1] - s;
+            std::vector<double> A(static_cast<size_t>(b) * b, 0.0);
+            for (int i = 0; i < b; ++i)
+                for (int e = cl.sptr[s + i]; e < cl.sptr[s + i + 1]; ++e)
+                    A[static_cast<size_t>(i) * b + (cl.scol[e] - s)] = cl.K[e] + out.shifts_all[q] * cl.M[e];
+            if (!invert_dense(b, A)) throw std::runtime_error("setup_mg: singular coarsest operator");
+            std::copy(A.begin(), A.end(), out.coarse_inv.begin() + static_cast<size_t>(q) * out.blk_total + out.blk_off[c]);
+        }
 }
 
 }  // namespace apf

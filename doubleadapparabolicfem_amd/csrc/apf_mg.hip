@@ -149,15 +149,19 @@ __global__ void __launch_bounds__(256) k_csr_mm(int nrows, const int* __restrict
 
 // coarsest level: X[:,c] = inv[shift_idx[c]] B[:,c]
 template <typename R>
-__global__ void k_coarse_solve(int n, int nc, int ld, const R* __restrict__ inv, const int* __restrict__ sidx,
+__global__ void k_coarse_solve(int n, int nc, int ld, const R* __restrict__ inv, int64_t blk_total,
+                               const int* __restrict__ comp_start, const int* __restrict__ node_comp,
+                               const int64_t* __restrict__ blk_off, const int* __restrict__ sidx,
                                const R* __restrict__ B, R* __restrict__ X)
 {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<int64_t>(n) * nc) return;
     const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
-    const R* A = inv + static_cast<int64_t>(sidx[c]) * n * n + static_cast<int64_t>(i) * n;
+    const int comp = node_comp[i];
+    const int s = comp_start[comp], b = comp_start[comp + 1] - s;
+    const R* A = inv + static_cast<int64_t>(sidx[c]) * blk_total + blk_off[comp] + static_cast<int64_t>(i - s) * b;
     R acc = 0;
-    for (int j = 0; j < n; ++j) acc += A[j] * B[static_cast<int64_t>(j) * ld + c];
+    for (int j = 0; j < b; ++j) acc += A[j] * B[static_cast<int64_t>(s + j) * ld + c];
     X[static_cast<int64_t>(i) * ld + c] = acc;
 }
 
@@ -334,8 +338,15 @@ void MGDevice<R>::build(System* sys, const MGSetupHost& H, int nyc_)
     d_zidx = mg_alloc<int>(sys, ncol);
     up(d_zidx, zidx);
     nc_coarse = H.levels.back().nf;
+    blk_total = H.blk_total;
     d_cinv = mg_alloc<R>(sys, H.coarse_inv.size());
     up(d_cinv, conv<R>(H.coarse_inv));
+    d_comp_start = mg_alloc<int>(sys, H.comp_start.size());
+    up(d_comp_start, H.comp_start);
+    d_node_comp = mg_alloc<int>(sys, std::max<size_t>(H.node_comp.size(), 1));
+    if (!H.node_comp.empty()) up(d_node_comp, H.node_comp);
+    d_blk_off = mg_alloc<int64_t>(sys, std::max<size_t>(H.blk_off.size(), 1));
+    if (!H.blk_off.empty()) up(d_blk_off, H.blk_off);
     d_W = mg_alloc<double>(sys, H.W.size());
     up(d_W, H.W);
     d_MtInvH = mg_alloc<double>(sys, H.MtInvH.size());
@@ -406,7 +417,7 @@ void MGDevice<R>::vcycle(hipStream_t st, int l, int nc, const R* sig, const int*
     if (l == L - 1) {
         const int64_t nel = static_cast<int64_t>(d.nf) * nc;
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_coarse_solve<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, d.nf, nc, ld,
-                           d_cinv, sidx, d.B, d.X);
+                           d_cinv, blk_total, d_comp_start, d_node_comp, d_blk_off, sidx, d.B, d.X);
         return;
     }
     Level& c = levels[l + 1];

@@ -59,9 +59,15 @@ struct MGSetupHost {
     std::vector<double> W;             // [T][T] generalised eigenvectors (column i = mode i)
     std::vector<double> shift;          // [T] sqrt(theta_i)
     std::vector<double> MtInvH;        // [m][Nt][Nt] (h_k Mt)^-1
-    // coarsest-level dense inverses for each distinct shift: 0 and shift[i]
-    std::vector<double> shifts_all;     // [ns] distinct shifts, shifts_all[0] = 0
-    std::vector<double> coarse_inv;     // [ns][nfc][nfc]
+    // coarsest-level dense inverses for each distinct shift (0 and shift[i]),
+    // one block per connected component of the coarsest graph (components are
+    // contiguous node ranges, e.g. independent problems; else one block)
+    std::vector<double> shifts_all;     // [nsft] distinct shifts, shifts_all[0] = 0
+    std::vector<int> comp_start;        // [ncomp + 1] node range of each block
+    std::vector<int> node_comp;         // [nfc] block of each node
+    std::vector<int64_t> blk_off;       // [ncomp] offset of each block inside one shift
+    int64_t blk_total = 0;              // sum of squared block sizes
+    std::vector<double> coarse_inv;     // [nsft][blk_total]
 };
 
 // Galerkin coarse operators, smoother bounds, time eigen-decomposition,

@@ -44,7 +44,11 @@ struct MGDevice {
     int* d_sidx = nullptr;
     int* d_zidx = nullptr;
     int nc_coarse = 0;
-    R* d_cinv = nullptr;
+    int64_t blk_total = 0;
+    R* d_cinv = nullptr;       // [shift][blk_total] block-diagonal coarsest inverses
+    int* d_comp_start = nullptr;
+    int* d_node_comp = nullptr;
+    int64_t* d_blk_off = nullptr;
     double* d_W = nullptr;
     double* d_MtInvH = nullptr;
     R* d_Z = nullptr;

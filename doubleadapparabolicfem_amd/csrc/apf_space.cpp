@@ -45,10 +45,15 @@ void build_space(Space& S, const TensorMesh& mesh, int p)
         }
     S.cell_nodes.assign(static_cast<size_t>(ns) * Ns, -1);
     if (d == 1) {
-        double xmin = mesh.vertices[0], xmax = mesh.vertices[0];
-        for (int i = 0; i < nv; ++i) { xmin = std::min(xmin, mesh.vertices[i]); xmax = std::max(xmax, mesh.vertices[i]); }
+        // boundary vertices are those of exactly one cell: the two ends of an
+        // interval, or of every segment when the mesh holds several disjoint ones
+        std::vector<int> deg(nv, 0);
+        for (int c = 0; c < ns; ++c) {
+            ++deg[mesh.cells[c * 2]];
+            ++deg[mesh.cells[c * 2 + 1]];
+        }
         S.vertex_boundary.assign(nv, 0);
-        for (int i = 0; i < nv; ++i) S.vertex_boundary[i] = (mesh.vertices[i] == xmin || mesh.vertices[i] == xmax);
+        for (int i = 0; i < nv; ++i) S.vertex_boundary[i] = deg[i] == 1;
         S.n_nodes = nv + ns * (p - 1);
         S.node_boundary.assign(S.n_nodes, 0);
         for (int i = 0; i < nv; ++i) S.node_boundary[i] = S.vertex_boundary[i];

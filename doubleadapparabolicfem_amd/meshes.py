@@ -106,3 +106,56 @@ def checkerboard_kappa(mesh: Mesh, k: int = 8, seed: int = 4) -> np.ndarray:
     kap = vals[iy, ix]
     d = mesh.dim
     return kap[:, None, None] * np.eye(d)[None, :, :]
+
+
+def segment_batch_mesh(nx, times) -> Mesh:
+    """Independent 1+1D problems in one mesh: problem i lives on the segment
+    [2i, 2i+1] cut into nx[i] equal intervals; segments share no vertex, so the
+    problems decouple and each segment's ends are its Dirichlet boundary."""
+    verts, cells, off = [], [], 0
+    for i, n in enumerate(nx):
+        verts.append(2.0 * i + np.arange(n + 1) / n)
+        cells.append(off + np.stack([np.arange(n), np.arange(1, n + 1)], axis=1))
+        off += n + 1
+    return Mesh(1, np.concatenate(verts)[:, None], np.concatenate(cells).astype(np.int32),
+                np.asarray(times, dtype=np.float64))
+
+
+def segment_batch_hierarchy(nx, levels):
+    """Nested coarser batch meshes (every segment halved `levels` times)."""
+    out = []
+    for lev in range(1, levels + 1):
+        out.append(segment_batch_mesh([n >> lev for n in nx], [0.0, 1.0]))
+    return out
+
+
+def segment_batch_problems(n_problems=256, n_t=32, seed0=5000, dof_range=(1e3, 1e5), multiple=8):
+    """Synthetic batch of independent 1+1D heat problems: per problem i (numpy
+    PCG64 seeded with seed0 + i) a target X-DOF count log-uniform in dof_range,
+    kappa = 10^U(-1,1), u0 = sum_{k=1..5} c_k sin(k pi x) with c_k ~ N(0,1),
+    ell = 0; all share the uniform time grid of n_t intervals.  Returns
+    (nx per problem, kappa per problem, coefficients [n_problems, 5])."""
+    nx, kap, coef = [], [], []
+    for i in range(n_problems):
+        rng = np.random.default_rng(seed0 + i)
+        target = np.exp(rng.uniform(np.log(dof_range[0]), np.log(dof_range[1])))
+        n = max(multiple, int(round((target / (n_t + 1) + 1) / multiple)) * multiple)
+        nx.append(n)
+        kap.append(10.0 ** rng.uniform(-1.0, 1.0))
+        coef.append(rng.standard_normal(5))
+    return nx, np.array(kap), np.array(coef)
+
+
+def segment_batch_coefficients(mesh: Mesh, nx, kappa, coef) -> Coefficients:
+    """M = kappa_i per cell of segment i; u0 = sum_k c_ik sin(k pi (x - 2i))."""
+    seg_of_cell = np.repeat(np.arange(len(nx)), nx)
+    M = kappa[seg_of_cell][:, None, None] * np.ones((1, 1, 1))
+
+    def u0(x):
+        i = np.floor(x[:, 0] / 2.0).astype(int)
+        xl = x[:, 0] - 2.0 * i
+        k = np.arange(1, coef.shape[1] + 1)
+        return np.sum(coef[i] * np.sin(np.pi * xl[:, None] * k[None, :]), axis=1)
+
+    return Coefficients(M, 0.0, u0)
+

@@ -113,8 +113,9 @@ def spatial_dofs(mesh: SpatialMesh, p: int) -> SpatialDofs:
                 nid = nv + c * (p - 1) + (j - 1)
                 cell_nodes[c, j] = nid
                 coords[nid] = mesh.vertices[v0] + (mesh.vertices[v1] - mesh.vertices[v0]) * j / p
-        xs = mesh.vertices[:, 0]
-        bvert = (xs == xs.min()) | (xs == xs.max())
+        # boundary = vertices of exactly one cell (the interval ends; every
+        # segment's ends when the mesh holds several disjoint segments)
+        bvert = np.bincount(mesh.cells.ravel(), minlength=nv) == 1
         boundary = np.zeros(n_nodes, dtype=bool)
         boundary[:nv] = bvert
         free_index = _first_appearance(cell_nodes, boundary, n_nodes)
