@@ -33,6 +33,7 @@ APF_OK, APF_BAD_ARG, APF_NOT_CONVERGED, APF_RESIDUAL_TOO_LARGE, APF_SINGULAR, AP
 APF_FP64, APF_FP32_REFINE = 0, 1
 APF_ASSEMBLE_KRON, APF_ASSEMBLE_SCATTER = 0, 1
 APF_PRECOND_DIAGONAL, APF_PRECOND_MULTIGRID = 0, 1
+APF_SLAB_FROM_SOLUTION, APF_SLAB_SOLVE = 0, 1
 
 _lib = None
 
@@ -83,7 +84,9 @@ class _EstimateOut(ctypes.Structure):
     _fields_ = [("lambda_y2", ctypes.c_double), ("trace2", ctypes.c_double), ("eta2", ctypes.c_double),
                 ("flux2", ctypes.c_double), ("recon2", ctypes.c_double), ("y_est2", ctypes.c_double),
                 ("div_residual_l2", ctypes.c_double), ("max_patch_residual", ctypes.c_double),
-                ("seconds_patches", ctypes.c_double), ("seconds_diagnostics", ctypes.c_double)]
+                ("seconds_patches", ctypes.c_double), ("seconds_diagnostics", ctypes.c_double),
+                ("slab_iterations", ctypes.c_int), ("slab_relative_residual", ctypes.c_double),
+                ("seconds_slab", ctypes.c_double)]
 
 
 class _DoerflerDiag(ctypes.Structure):
@@ -121,6 +124,9 @@ EXPORTED = {
     "apf_bench_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_estimate_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(_EstimateOut), ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
     "apf_system_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_estimate_phase_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_combine_estimator": (ctypes.c_int, [ctypes.c_int64] + [ctypes.c_void_p] * 4),
@@ -316,14 +322,16 @@ class System:
             _check(code, self.ctx.handle, "apf_minres_solve")
         return x, stats
 
-    def estimate(self, prec=APF_FP64, refine_steps=2, per_cell=True):
+    def estimate(self, prec=APF_FP64, refine_steps=2, per_cell=True, slab=APF_SLAB_FROM_SOLUTION):
+        """Equilibrated-flux estimator on the current solution; slab=APF_SLAB_SOLVE
+        solves the slab reconstruction instead of taking lambda."""
         out = _EstimateOut()
         n = self.space.n_space_cells * self.space.n_intervals
         fl = np.zeros(n) if per_cell else None
         rs = np.zeros(n) if per_cell else None
         rc = np.zeros(n) if per_cell else None
-        _check(lib().apf_estimate(self.handle, prec, refine_steps, ctypes.byref(out), _ptr(fl), _ptr(rs), _ptr(rc)),
-               self.ctx.handle, "apf_estimate")
+        _check(lib().apf_estimate_ex(self.handle, prec, refine_steps, slab, ctypes.byref(out), _ptr(fl), _ptr(rs),
+                                     _ptr(rc)), self.ctx.handle, "apf_estimate")
         res = {f: getattr(out, f) for f, _ in _EstimateOut._fields_}
         res.update(flux_by_cell=fl, resid_by_cell=rs, recon_by_cell=rc)
         return res

@@ -1306,6 +1306,13 @@ void estimator_alloc(System* sys)
 extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_estimate_out* out, double* flux_by_cell,
                             double* resid_by_cell, double* recon_by_cell)
 {
+    return apf_estimate_ex(sys, prec, refine_steps, APF_SLAB_FROM_SOLUTION, out, flux_by_cell, resid_by_cell,
+                           recon_by_cell);
+}
+
+extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int slab_mode, apf_estimate_out* out,
+                               double* flux_by_cell, double* resid_by_cell, double* recon_by_cell)
+{
     if (!sys || !out) {
         g_last_error = "apf_estimate: null argument";
         return APF_BAD_ARG;
@@ -1318,6 +1325,8 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
     if (!sys->assembled) return fail(APF_BAD_ARG, "apf_estimate: assemble and solve first");
     if (prec != APF_FP64 && prec != APF_FP32_REFINE) return fail(APF_BAD_ARG, "apf_estimate: unknown precision");
     if (refine_steps < 0 || refine_steps > 8) return fail(APF_BAD_ARG, "apf_estimate: refine_steps must be 0..8");
+    if (slab_mode != APF_SLAB_FROM_SOLUTION && slab_mode != APF_SLAB_SOLVE)
+        return fail(APF_BAD_ARG, "apf_estimate: unknown slab reconstruction mode");
     const Space& S = *sys->space;
     const RefTables& T = *sys->tables;
     if (S.p > 3) return fail(APF_BAD_ARG, "apf_estimate: the estimator supports p <= 3");
@@ -1347,7 +1356,14 @@ extern "C" int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_est
         // slab reconstruction: on a tensor mesh the slab space is Y restricted to
         // the slab, so lambda_tilde solves the lambda rows of the main system
         // slab by slab; with the main solve converged it equals lambda.
-        HIP_TRY(hipMemcpyAsync(B.lt, sys->d_x, S.nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+        out->slab_iterations = 0;
+        out->slab_relative_residual = -1.0;
+        const auto t_slab = std::chrono::steady_clock::now();
+        if (slab_mode == APF_SLAB_SOLVE)
+            slab_reconstruct(sys, B.lt, 1e-13, 100, &out->slab_iterations, &out->slab_relative_residual);
+        else
+            HIP_TRY(hipMemcpyAsync(B.lt, sys->d_x, S.nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+        out->seconds_slab = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_slab).count();
         HIP_TRY(hipMemsetAsync(B.patch_res, 0, S.patches.n_patches * sizeof(double), st));
         const size_t ft_bytes = (prec == APF_FP64) ? sizeof(double) : sizeof(float);
         const size_t lds_cap = 160 * 1024;

@@ -560,6 +560,88 @@ static void upload(void* dst, const void* src, size_t bytes)
 }
 
 
+// (Re)build the multigrid preconditioner for the current assembly: Galerkin
+// operators and time eigenmodes on the host, device copies (FP32 or FP64).
+void mg_prepare(System* sys, bool mg32)
+{
+    const Space& S = *sys->space;
+    hipStream_t st = sys->stream;
+    uint64_t& ver = mg32 ? sys->mg32_version : sys->mg64_version;
+    if (ver == sys->assembly_version) return;
+    const auto ts = std::chrono::steady_clock::now();
+    std::vector<double> K(S.scol.size()), M(S.scol.size());
+    HIP_TRY(hipMemcpyAsync(K.data(), sys->d_Ksp, K.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(M.data(), sys->d_Msp, M.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    MGSetupHost hs;
+    setup_mg(S, *sys->H, K, M, sys->tables->Mt, sys->tables->Dt, hs);
+    const int nyc = S.mesh.m() * S.Nt;
+    if (mg32) {
+        if (!sys->mg32) sys->mg32 = std::make_unique<MGDevice<float>>();
+        sys->mg32->build(sys, hs, nyc);
+    } else {
+        if (!sys->mg64) sys->mg64 = std::make_unique<MGDevice<double>>();
+        sys->mg64->build(sys, hs, nyc);
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    ver = sys->assembly_version;
+    sys->t_mg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+}
+
+// Slab reconstruction (time_slabs/time_slab_reconstruction.hpp:584-735): on a
+// tensor mesh the slab space is Y restricted to the slab, so lambda_tilde
+// solves  A_y lt = f - B^T u  slab by slab (A_y is block diagonal over the
+// slabs).  Preconditioned Richardson from lt = lambda with
+// P_Y = K^-1 (x) (h_k Mt)^-1, one multigrid V-cycle per spatial inverse, for
+// all slabs at once, until |f - B^T u - A_y lt| <= tol |f - B^T u|.
+void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel)
+{
+    const Space& S = *sys->space;
+    if (!sys->H) throw std::runtime_error("slab reconstruction needs the nested coarse meshes of the multigrid");
+    mg_prepare(sys, false);
+    hipStream_t st = sys->stream;
+    const int64_t n = S.n, nY = S.nY;
+    const int nparts = sys->nparts;
+    double* t = sys->d_vec[2];
+    double* y = sys->d_vec[3];
+    double* r = sys->d_vec[0];
+    double* z = sys->d_vec[1];
+    auto resid2 = [&](const double* Kt) {
+        hipLaunchKernelGGL(k_residual_parts, dim3(nparts), dim3(256), 0, st, nY, sys->d_b, Kt, sys->d_part);
+        hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
+        double h[2];
+        HIP_TRY(hipMemcpyAsync(h, sys->d_res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return h[0];
+    };
+    // |f - B^T u|: the lambda rows of K [0; u]
+    HIP_TRY(hipMemsetAsync(t, 0, nY * sizeof(double), st));
+    HIP_TRY(hipMemcpyAsync(t + nY, sys->d_x + nY, (n - nY) * sizeof(double), hipMemcpyDeviceToDevice, st));
+    sys->spmv(t, y);
+    const double g2 = resid2(y);
+    HIP_TRY(hipMemcpyAsync(lt, sys->d_x, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+    int it = 0;
+    double rr = 0.0, prev = 0.0;
+    for (;; ++it) {
+        HIP_TRY(hipMemcpyAsync(t, lt, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+        sys->spmv(t, y);
+        const double r2 = resid2(y);
+        rr = g2 > 0.0 ? std::sqrt(r2 / g2) : std::sqrt(r2);
+        // stop at the tolerance, or at the round-off floor (less than 10% gained)
+        if (!(rr > tol) || it >= max_it || (it >= 2 && rr > 0.9 * prev)) break;
+        prev = rr;
+        // r = [f - B^T u - A_y lt; 0],  lt += P_Y r
+        HIP_TRY(hipMemcpyAsync(r, sys->d_b, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -1.0, y, 1.0, r);
+        HIP_TRY(hipMemsetAsync(r + nY, 0, (n - nY) * sizeof(double), st));
+        sys->mg64->apply(st, S, r, z, sys->d_part, nparts);
+        hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, 1.0, z, 1.0, lt);
+    }
+    HIP_TRY(hipGetLastError());
+    *its = it;
+    *rel = rr;
+}
+
 void System::spmv(const double* x, double* y)
 {
     const int64_t n = space->n;
@@ -1018,28 +1100,7 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
         int pparts = nparts;
         unsigned pgrid = 0;
         if (use_mg) {
-            // (re)build the multigrid operators for the current assembly
-            uint64_t& ver = mg32 ? sys->mg32_version : sys->mg64_version;
-            if (ver != sys->assembly_version) {
-                const auto ts = std::chrono::steady_clock::now();
-                std::vector<double> K(S.scol.size()), M(S.scol.size());
-                HIP_TRY(hipMemcpyAsync(K.data(), sys->d_Ksp, K.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-                HIP_TRY(hipMemcpyAsync(M.data(), sys->d_Msp, M.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-                HIP_TRY(hipStreamSynchronize(st));
-                MGSetupHost hs;
-                setup_mg(S, *sys->H, K, M, sys->tables->Mt, sys->tables->Dt, hs);
-                const int nyc = S.mesh.m() * S.Nt;
-                if (mg32) {
-                    if (!sys->mg32) sys->mg32 = std::make_unique<MGDevice<float>>();
-                    sys->mg32->build(sys, hs, nyc);
-                } else {
-                    if (!sys->mg64) sys->mg64 = std::make_unique<MGDevice<double>>();
-                    sys->mg64->build(sys, hs, nyc);
-                }
-                HIP_TRY(hipStreamSynchronize(st));
-                ver = sys->assembly_version;
-                sys->t_mg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
-            }
+            mg_prepare(sys, mg32);
             pparts = mg32 ? sys->mg32->n_partials(S.nf) : sys->mg64->n_partials(S.nf);
             if (pparts > 4 * nparts) throw std::runtime_error("too many multigrid transpose tiles");
         } else {

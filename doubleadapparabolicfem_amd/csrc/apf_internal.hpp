@@ -222,6 +222,8 @@ struct System {
 };
 
 void estimator_alloc(System* sys);
+void mg_prepare(System* sys, bool mg32);
+void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel);
 
 #define HIP_TRY(expr)                                                                    \
     do {                                                                                 \

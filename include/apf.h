@@ -61,6 +61,9 @@ enum apf_status {
 enum apf_precision { APF_FP64 = 0, APF_FP32_REFINE = 1 };
 enum apf_assembly_mode { APF_ASSEMBLE_KRON = 0, APF_ASSEMBLE_SCATTER = 1 };
 enum apf_precond_kind { APF_PRECOND_DIAGONAL = 0, APF_PRECOND_MULTIGRID = 1 };
+/* slab reconstruction: lambda_tilde = lambda of the current solution (exact when the
+ * main solve is converged), or solved slab by slab to 1e-13 relative */
+enum apf_slab_mode { APF_SLAB_FROM_SOLUTION = 0, APF_SLAB_SOLVE = 1 };
 
 typedef struct apf_ctx apf_ctx;
 typedef struct apf_space apf_space;
@@ -113,6 +116,9 @@ typedef struct {
     double div_residual_l2;
     double max_patch_residual;               /* max_i |K s - b|_inf / max(1, |b|_inf) over patches */
     double seconds_patches, seconds_diagnostics;
+    int slab_iterations;                     /* APF_SLAB_SOLVE: preconditioned iterations */
+    double slab_relative_residual;           /* |f - B^T u - A_y lt| / |f - B^T u| (-1: not solved) */
+    double seconds_slab;
 } apf_estimate_out;
 
 /* ---- host-side integer work (no GPU needed) ---- */
@@ -181,6 +187,12 @@ int apf_system_set_solution(apf_system* sys, const double* x);
  * are indexed by space-time cell k * n_space_cells + c. */
 int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_estimate_out* out,
                  double* flux_by_cell, double* resid_by_cell, double* recon_by_cell);
+/* Same with the slab reconstruction mode (apf_slab_mode).  APF_SLAB_SOLVE
+ * solves A_slab lt = f_slab - (B^T u)_slab for every slab
+ * (time_slabs/time_slab_reconstruction.hpp:584-735) with the multigrid's
+ * slab preconditioner; it needs apf_space_add_coarse_mesh. */
+int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int slab_mode, apf_estimate_out* out,
+                    double* flux_by_cell, double* resid_by_cell, double* recon_by_cell);
 
 /* Device timing of the last call of each phase in milliseconds (HIP events):
  * [0] assembly, [1] patch kernel, [2] diagnostics, [3] one SpMV (last). */

@@ -73,3 +73,30 @@ def test_gpu_pipeline_reproduces_published_anchor(ctx, rec):
     for key, col in (("lambda_y2", "lambda_y_squared"), ("eta2", "eta_squared"), ("flux2", "y_flux_squared"),
                      ("y_est2", "y_estimator_squared")):
         assert _rel(est[key], rec[col]) <= 6e-6, (key, est[key], rec[col])
+
+
+@pytest.mark.parametrize("dim,p,outer", [(1, 1, 3), (1, 2, 2), (2, 2, 1), (2, 1, 2)])
+def test_slab_reconstruction_solve_matches_oracle(ctx, dim, p, outer):
+    """lambda perturbed by 10%: the slab solve recovers the oracle's per-slab
+    direct solves (lambda_tilde depends on u only), so the reconstruction term
+    ||lambda - lambda_tilde||_Y^2 and the flux terms agree per cell."""
+    m = meshes.uniform_mesh(dim, outer)
+    coeff = meshes.smooth_initial(dim)
+    print(f"slab solve dim={dim} p={p} outer={outer}", flush=True)
+    stm, od, prob = oracle_setup(m, p, coeff)
+    S = OA.assemble(stm, od, prob)
+    x = OE.main_solve(S)
+    rng = np.random.default_rng(7)
+    x[:od.nY] *= 1.0 + 0.1 * rng.standard_normal(od.nY)
+    ref = OE.estimate(stm, od, prob, S, x)
+    sp = apf.Space(m, p, coarse=meshes.uniform_hierarchy(dim, outer))
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(coeff)
+    sysm.set_solution(x)
+    est = sysm.estimate(apf.APF_FP64, 0, slab=apf.APF_SLAB_SOLVE)
+    assert est["slab_relative_residual"] <= 1e-12, est["slab_relative_residual"]
+    assert ref.recon2 > 1e-6 * ref.lambda_y2
+    for key, rv in (("recon2", ref.recon2), ("flux2", ref.flux2), ("y_est2", ref.y_est2)):
+        assert _rel(est[key], rv) <= 1e-10, (key, est[key], rv)
+    rc = est["recon_by_cell"]
+    assert np.all(np.abs(rc - ref.recon_by_cell) <= 1e-9 * np.abs(ref.recon_by_cell) + 1e-13 * ref.recon2)
