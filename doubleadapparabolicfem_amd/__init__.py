@@ -85,7 +85,8 @@ class _EstimateOut(ctypes.Structure):
                 ("flux2", ctypes.c_double), ("recon2", ctypes.c_double), ("y_est2", ctypes.c_double),
                 ("div_residual_l2", ctypes.c_double), ("max_patch_residual", ctypes.c_double),
                 ("seconds_patches", ctypes.c_double), ("seconds_diagnostics", ctypes.c_double),
-                ("slab_iterations", ctypes.c_int), ("slab_relative_residual", ctypes.c_double),
+                ("extra_refinements", ctypes.c_int), ("slab_iterations", ctypes.c_int),
+                ("slab_relative_residual", ctypes.c_double),
                 ("seconds_slab", ctypes.c_double)]
 
 

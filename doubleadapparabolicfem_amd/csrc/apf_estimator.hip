@@ -281,44 +281,64 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
     const bool interior = topo.boundary[a] == 0;
     const int cb = topo.cell_ptr[a], ce = topo.cell_ptr[a + 1];
     const int64_t nn = static_cast<int64_t>(np) * np;
+    // cell metrics once per cell (patches of up to 64 cells; larger ones recompute)
+    constexpr int GMAX = 64;
+    __shared__ double gmH[GMAX][5];
+    const int nc = ce - cb;
+    for (int ci = threadIdx.x; ci < min(nc, GMAX); ci += blockDim.x) {
+        const Metric2 g = metric_of(dim, topo.cells[cb + ci], verts, cells, Mcell);
+        for (int u = 0; u < 4; ++u) gmH[ci][u] = g.H[u];
+        gmH[ci][4] = g.det;
+    }
     for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) K[e] = 0.0;
     __syncthreads();
-    for (int ci = cb; ci < ce; ++ci) {
-        const int c = topo.cells[ci];
-        const Metric2 g = metric_of(dim, c, verts, cells, Mcell);
+    // every cell at once: a flux-flux entry gets at most two contributions (the
+    // cells of an edge), so the FP64 atomic sums are exact-order independent
+    const int per_cell = Nf * Nf + Nq * Nf + Nq;
+    for (int e = threadIdx.x; e < nc * per_cell; e += blockDim.x) {
+        const int ci = cb + e / per_cell, w = e % per_cell;
         const int* fm = topo.fmap + static_cast<int64_t>(ci) * Nf;
         const int8_t* fs = topo.fsign + static_cast<int64_t>(ci) * Nf;
-        // flux mass: int M^-1 s_i . s_j = sum_ab H_ab RTab_ab[i][j], i <= j mirrored
-        for (int e = threadIdx.x; e < Nf * Nf; e += blockDim.x) {
-            const int i = e / Nf, j = e % Nf;
+        if (w < Nf * Nf) {
+            // flux mass: int M^-1 s_i . s_j = sum_ab H_ab RTab_ab[i][j], i <= j mirrored
+            const int i = w / Nf, j = w % Nf;
             if (j < i) continue;
             const int fi = fm[i], fj = fm[j];
             if (fi < 0 || fj < 0) continue;
+            double H[4];
+            if (ci - cb < GMAX) {
+                for (int u = 0; u < 4; ++u) H[u] = gmH[ci - cb][u];
+            } else {
+                const Metric2 g = metric_of(dim, topo.cells[ci], verts, cells, Mcell);
+                for (int u = 0; u < 4; ++u) H[u] = g.H[u];
+            }
             double v = 0.0;
             for (int ab = 0; ab < dim * dim; ++ab)
-                v += g.H[(ab / dim) * 2 + (ab % dim)] * T.RTab[(static_cast<int64_t>(ab) * Nf + i) * Nf + j];
+                v += H[(ab / dim) * 2 + (ab % dim)] * T.RTab[(static_cast<int64_t>(ab) * Nf + i) * Nf + j];
             v *= static_cast<double>(fs[i] * fs[j]);
-            K[static_cast<int64_t>(fi) * np + fj] += v;
-            if (i != j) K[static_cast<int64_t>(fj) * np + fi] += v;
-        }
-        // divergence coupling int q_i div s_j (orthonormal scalars) and mean-zero row
-        const int r0 = nF + (ci - cb) * Nq;
-        for (int e = threadIdx.x; e < Nq * Nf; e += blockDim.x) {
-            const int qq = e / Nf, j = e % Nf;
+            atomicAdd(&K[static_cast<int64_t>(fi) * np + fj], v);
+            if (i != j) atomicAdd(&K[static_cast<int64_t>(fj) * np + fi], v);
+        } else if (w < Nf * Nf + Nq * Nf) {
+            // divergence coupling int q_i div s_j (orthonormal scalars)
+            const int w2 = w - Nf * Nf;
+            const int qq = w2 / Nf, j = w2 % Nf;
             const int fj = fm[j];
             if (fj < 0) continue;
+            const int r0 = nF + (ci - cb) * Nq;
             const double v = fs[j] * T.Bhat[qq * Nf + j];
             K[static_cast<int64_t>(r0 + qq) * np + fj] = v;
             K[static_cast<int64_t>(fj) * np + r0 + qq] = v;
+        } else if (interior) {
+            // mean-zero row
+            const int qq = w - Nf * Nf - Nq * Nf;
+            const int r0 = nF + (ci - cb) * Nq;
+            const double det = (ci - cb < GMAX) ? gmH[ci - cb][4] : metric_of(dim, topo.cells[ci], verts, cells, Mcell).det;
+            const double v = det * T.qbar[qq];
+            K[static_cast<int64_t>(r0 + qq) * np + (n - 1)] = v;
+            K[static_cast<int64_t>(n - 1) * np + r0 + qq] = v;
         }
-        if (interior)
-            for (int qq = threadIdx.x; qq < Nq; qq += blockDim.x) {
-                const double v = g.det * T.qbar[qq];
-                K[static_cast<int64_t>(r0 + qq) * np + (n - 1)] = v;
-                K[static_cast<int64_t>(n - 1) * np + r0 + qq] = v;
-            }
-        __syncthreads();
     }
+    __syncthreads();
 }
 
 // Phase 1 (large patches), one workgroup per spatial vertex patch of one size
@@ -581,6 +601,197 @@ __global__ void __launch_bounds__(NW * 64)
             else if (i < np && j < np)
                 Kg[static_cast<int64_t>(i) * np + j] = static_cast<FT>(0);
         }
+}
+
+// Largest value over the wave (all 64 lanes) by DPP row shifts and row
+// broadcasts, no LDS: row_shr 1, 2, 3, 4, 8 within rows of 16, then
+// row_bcast 15 and 31 across rows; lane 63 holds the result.
+__device__ inline unsigned wave_umax(unsigned v)
+{
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x113, 0xf, 0xf, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xe, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xc, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false)));
+    return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
+// Phase 1, blocked (FP32, padded order np <= 64 RPL): Gauss-Jordan in
+// 16-column panels on the matrix in LDS.  Wave 0 eliminates a panel in
+// registers (lane l owns rows l + 64 s, all 16 panel columns; partial pivoting
+// among unused rows, no row interchange), so a panel step needs no workgroup
+// barrier and every row's multiplier is in its own lane.  The panel's
+// accumulated transform E (identity outside the pivot columns R, which the
+// eliminated panel P holds) is then applied to the other columns by all waves
+// as one rank-16 MFMA update  A[:, T] += (P - I_R) A[R, T]  with A[R, T]
+// copied first.  Same arithmetic as the unblocked elimination, reordered.
+template <int RPL>
+__global__ void __launch_bounds__(256)
+    k_patch_factor_blocked(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, const double* verts,
+                           const int* cells, const double* Mcell, double* Kd, float* Ki)
+{
+    constexpr int NPM = 64 * RPL;  // largest padded order
+    __shared__ float A[NPM * NPM];
+    __shared__ float Rb[16 * NPM];
+    __shared__ int pivrow[NPM], rinv[NPM];
+    __shared__ double dsc[NPM];
+    const int q = q0 + blockIdx.x;
+    const int a = topo.order[q];
+    const int n = ps.n[q], np = ps.np[q];
+    double* K = Kd + ps.off2[q];
+    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
+    // symmetric equilibration D K D, d_i = 1 / sqrt(max_j |K_ij|): the flux,
+    // divergence and mean rows differ in scale by powers of h, and an FP32
+    // Gauss-Jordan inverse of the scaled matrix is several times more accurate
+    // 8 loads in flight per thread
+    for (int e0 = threadIdx.x; e0 < np * np; e0 += 8 * blockDim.x) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            v[u] = (e < np * np) ? K[e] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < np * np) A[e] = static_cast<float>(fabs(v[u]));
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        float mx = 0.0f;
+        for (int j = 0; j < np; ++j) mx = fmaxf(mx, A[i * np + j]);
+        dsc[i] = (mx > 0.0f) ? 1.0 / sqrt(static_cast<double>(mx)) : 1.0;
+        rinv[i] = i;
+    }
+    __syncthreads();
+    for (int e0 = threadIdx.x; e0 < np * np; e0 += 8 * blockDim.x) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            v[u] = (e < np * np) ? K[e] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < np * np) A[e] = static_cast<float>(dsc[e / np] * v[u] * dsc[e % np]);
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nrt = np >> 4;
+    unsigned used = 0;  // wave 0: bit s = row lane + 64 s used as pivot (or >= n)
+#pragma unroll
+    for (int s = 0; s < RPL; ++s)
+        if (lane + 64 * s >= n) used |= 1u << s;
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        const int kc = min(16, n - k0);
+        if (wave == 0) {
+            float Pn[RPL][16];
+#pragma unroll
+            for (int s = 0; s < RPL; ++s)
+#pragma unroll
+                for (int c = 0; c < 16; ++c) Pn[s][c] = A[(lane + 64 * s) * np + k0 + c];
+            for (int kk = 0; kk < kc; ++kk) {
+                // pivot: largest |A[i][k]| over unused rows, as one 32-bit key
+                // per candidate (|v| with its 7 low mantissa bits replaced by
+                // 127 - i: a pivot within 1 - 2^-16 of the largest is as good,
+                // ties go to the lowest row, a NaN wins) and one DPP max
+                unsigned key = 0u;
+                float cur[RPL];
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) {
+                    float v = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        if (c == kk) v = Pn[s][c];
+                    cur[s] = v;
+                    const unsigned kv = (__float_as_uint(fabsf(v)) & ~0x7Fu) | static_cast<unsigned>(127 - (lane + 64 * s));
+                    if (!(used & (1u << s))) key = max(key, kv);
+                }
+                key = wave_umax(key);
+                const int r = min(max(127 - static_cast<int>(key & 0x7Fu), 0), n - 1);
+                const int owner = r & 63, sr = r >> 6;
+                float pvl = 0.0f;
+#pragma unroll
+                for (int s = 0; s < RPL; ++s)
+                    if (s == sr) pvl = cur[s];
+                const float inv = 1.0f / __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pvl), owner));
+                float prow[16];
+#pragma unroll
+                for (int s = 0; s < RPL; ++s)
+                    if (s == sr) {
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) {
+                            const float v = ((c == kk) ? 1.0f : Pn[s][c]) * inv;
+                            if (lane == owner) Pn[s][c] = v;
+                            prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), owner));
+                        }
+                    }
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) {
+                    const int i = lane + 64 * s;
+                    if (i == r || i >= n) continue;
+                    float f = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        if (c == kk) f = Pn[s][c];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) Pn[s][c] = ((c == kk) ? 0.0f : Pn[s][c]) - f * prow[c];
+                }
+                if (lane == owner) used |= 1u << sr;
+                if (lane == 0) {
+                    pivrow[k0 + kk] = r;
+                    rinv[r] = k0 + kk;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < RPL; ++s)
+#pragma unroll
+                for (int c = 0; c < 16; ++c) A[(lane + 64 * s) * np + k0 + c] = Pn[s][c];
+        }
+        __syncthreads();
+        // pivot rows outside the panel, before the update
+        for (int e = threadIdx.x; e < 16 * np; e += blockDim.x) {
+            const int c = e / np, j = e % np;
+            Rb[e] = (c < kc && (j < k0 || j >= k0 + 16)) ? A[pivrow[k0 + c] * np + j] : 0.0f;
+        }
+        __syncthreads();
+        // A[:, T] += (P - I_R) Rb on MFMA; the panel's own column tile is left alone
+        const int pt = k0 >> 4;
+        for (int tile = wave; tile < nrt * nrt; tile += 4) {
+            const int rt = tile % nrt, ct = tile / nrt;
+            if (ct == pt) continue;
+            v4f acc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = A[(rt * 16 + drow<float>(lk, r)) * np + ct * 16 + li];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int c = 4 * t + lk, i = rt * 16 + li;
+                float av = (c < kc) ? A[i * np + k0 + c] : 0.0f;
+                if (c < kc && i == pivrow[k0 + c]) av -= 1.0f;
+                acc = mfma_16x16x4(av, Rb[c * np + ct * 16 + li], acc);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) A[(rt * 16 + drow<float>(lk, r)) * np + ct * 16 + li] = acc[r];
+        }
+        __syncthreads();
+    }
+    // K^-1 = D (D K D)^-1 D, with the pivot permutation folded in
+    float* Kg = Ki + ps.off2[q];
+    for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
+        const int i = e / np, j = e % np;
+        if (i < n && j < n) {
+            const int r = rinv[i], c = pivrow[j];
+            Kg[static_cast<int64_t>(r) * np + c] = static_cast<float>(dsc[r] * static_cast<double>(A[e]) * dsc[c]);
+        } else {
+            Kg[e] = 0.0f;
+        }
+    }
 }
 
 // Phase 1 (patches whose flux block and Schur block both have order <=
@@ -848,7 +1059,7 @@ __device__ inline void tile_product(const MT* mp, int np, const BT* Bs, int ldb,
 template <typename FT, int CT>
 __global__ void __launch_bounds__(SOLVE_THREADS)
     k_patch_solve(EstParams P, PatchTopo topo, PatchSlots ps, int q0, int count, int ncg, int cpw, const double* Kd,
-                  const FT* Ki, const double* RHS, double* sig, double* patch_res)
+                  const FT* Ki, const double* RHS, double* sig, double* patch_res, int* extra_rounds)
 {
     using V4 = typename Vec4<FT>::type;
     constexpr int CW = 16 * CT;
@@ -878,7 +1089,6 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
         const int r = e / CW, c = e % CW;
         Rs[e] = (c < ncw) ? static_cast<FT>(Bq[static_cast<int64_t>(r) * ncols + c]) : static_cast<FT>(0);
     }
-    for (int e = threadIdx.x; e < 2 * CW; e += blockDim.x) colmax[e] = 0ull;
     __syncthreads();
     // X = Ki B
     for (int rt = wave; rt < nrt; rt += nw) {
@@ -892,9 +1102,19 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
             for (int r = 0; r < 4; ++r) Xs[(rt * 16 + drow<FT>(lk, r)) * CW + ct * 16 + li] = static_cast<double>(acc[ct][r]);
     }
     __syncthreads();
+    // refine_steps rounds, then more (at most MAX_EXTRA) while the acceptance
+    // statistic of this block's (slab, vertex) problems exceeds 1e-10
+    constexpr int MAX_EXTRA = 4;
+    __shared__ int s_done;
+    __shared__ double s_worst;
     const int steps = P.refine_steps;
+    int extra = 0;
     for (int it = 0;; ++it) {
-        const bool last = it == steps;
+        const bool check = it >= steps;
+        if (check) {
+            for (int e = threadIdx.x; e < 2 * CW; e += blockDim.x) colmax[e] = 0ull;
+            __syncthreads();
+        }
         // R = B - Kd X in FP64
         for (int rt = wave; rt < nrt; rt += nw) {
             v4d acc[CT];
@@ -911,7 +1131,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
                 }
             }
             tile_product<CT>(Kq + rt * 16 + li, np, Xs, CW, li, lk, acc, true);
-            if (last) {
+            if (check) {
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) {
                     double rm = 0.0, bb = bm[ct];
@@ -928,16 +1148,33 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
                                   static_cast<unsigned long long>(__double_as_longlong(bb)));
                     }
                 }
-            } else {
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        Rs[(rt * 16 + drow<double>(lk, r)) * CW + ct * 16 + li] = static_cast<FT>(acc[ct][r]);
             }
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    Rs[(rt * 16 + drow<double>(lk, r)) * CW + ct * 16 + li] = static_cast<FT>(acc[ct][r]);
         }
         __syncthreads();
-        if (last) break;
+        if (check) {
+            // acceptance per (slab, vertex) problem: the Nt mode columns of one slab
+            if (threadIdx.x == 0) {
+                double worst = 0.0;
+                for (int s0 = 0; s0 < ncw; s0 += Nt) {
+                    double rm = 0.0, bm = 0.0;
+                    for (int c = s0; c < s0 + Nt; ++c) {
+                        rm = fmax(rm, __longlong_as_double(static_cast<long long>(colmax[c])));
+                        bm = fmax(bm, __longlong_as_double(static_cast<long long>(colmax[CW + c])));
+                    }
+                    worst = fmax(worst, rm / fmax(1.0, bm));
+                }
+                s_worst = worst;
+                s_done = (worst <= 1e-10 || extra >= MAX_EXTRA) ? 1 : 0;
+            }
+            __syncthreads();
+            if (s_done) break;
+            ++extra;
+        }
         // X += Ki R in FT
         for (int rt = wave; rt < nrt; rt += nw) {
             V4 acc[CT];
@@ -952,18 +1189,9 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
         }
         __syncthreads();
     }
-    // acceptance per (slab, vertex) problem: the Nt mode columns of one slab
     if (threadIdx.x == 0) {
-        double worst = 0.0;
-        for (int s0 = 0; s0 < ncw; s0 += Nt) {
-            double rm = 0.0, bm = 0.0;
-            for (int c = s0; c < s0 + Nt; ++c) {
-                rm = fmax(rm, __longlong_as_double(static_cast<long long>(colmax[c])));
-                bm = fmax(bm, __longlong_as_double(static_cast<long long>(colmax[CW + c])));
-            }
-            worst = fmax(worst, rm / fmax(1.0, bm));
-        }
-        atomic_max_pos(&patch_res[a], worst);
+        atomic_max_pos(&patch_res[a], s_worst);
+        if (extra) atomicAdd(extra_rounds, extra);
     }
     // flux coefficients per (spatial cell, local vertex, flux dof) and column
     const int cb = topo.cell_ptr[a], nc = topo.cell_ptr[a + 1] - cb;
@@ -1299,6 +1527,7 @@ void estimator_alloc(System* sys)
     B.patch_res = dalloc<double>(sys, P.n_patches);
     B.lt = dalloc<double>(sys, S.nY);
     B.sums = dalloc<double>(sys, 8);
+    B.extra_rounds = dalloc<int>(sys, 1);
 }
 
 }  // namespace apf
@@ -1365,6 +1594,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             HIP_TRY(hipMemcpyAsync(B.lt, sys->d_x, S.nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         out->seconds_slab = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_slab).count();
         HIP_TRY(hipMemsetAsync(B.patch_res, 0, S.patches.n_patches * sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(B.extra_rounds, 0, sizeof(int), st));
         const size_t ft_bytes = (prec == APF_FP64) ? sizeof(double) : sizeof(float);
         const size_t lds_cap = 160 * 1024;
         if (static_cast<size_t>(B.nmax) * 16 * (sizeof(double) + ft_bytes) > lds_cap - 4096)
@@ -1381,6 +1611,21 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             for (const auto& bk : B.buckets) {
                 // one register-resident Gauss-Jordan where the whole matrix fits the register file,
                 // else block elimination with two of them
+                if constexpr (std::is_same<FT, float>::value) {
+                    // FP32: blocked Gauss-Jordan with MFMA panel updates
+                    if (bk.np <= 64) {
+                        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_blocked<1>), dim3(bk.count), dim3(256), 0,
+                                           st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd,
+                                           Ki);
+                        continue;
+                    }
+                    if (bk.np <= 128) {
+                        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_blocked<2>), dim3(bk.count), dim3(256), 0,
+                                           st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd,
+                                           Ki);
+                        continue;
+                    }
+                }
                 if (bk.np <= 64) {
                     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_full<FT, 4, 16, 1>), dim3(bk.count), dim3(256), 0,
                                        st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
@@ -1446,7 +1691,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                 auto solve = [&](auto kern) {
                     set_lds(reinterpret_cast<const void*>(kern), ls);
                     hipLaunchKernelGGL(kern, dim3(grid), dim3(SOLVE_THREADS), ls, st, P, topo, ps, bk.q0, bk.count,
-                                       ncg, cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res);
+                                       ncg, cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res, B.extra_rounds);
                 };
                 switch (CT) {
                     case 1: solve(k_patch_solve<FT, 1>); break;
@@ -1494,7 +1739,9 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, st, S.patches.n_patches, B.patch_res, B.sums + 5);
         HIP_TRY(hipGetLastError());
         double h[6];
+        int extra_rounds = 0;
         HIP_TRY(hipMemcpyAsync(h, B.sums, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(&extra_rounds, B.extra_rounds, sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         float ms1 = 0.f, ms2 = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms1, sys->ev0, sys->ev1));
@@ -1514,6 +1761,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         out->eta2 = h[3] + h[4];
         out->y_est2 = h[0] + h[2];
         out->max_patch_residual = h[5];
+        out->extra_refinements = extra_rounds;
         out->seconds_patches = ms1 * 1e-3;
         out->seconds_diagnostics = ms2 * 1e-3;
         if (flux_by_cell) HIP_TRY(hipMemcpy(flux_by_cell, B.flux, ncell * sizeof(double), hipMemcpyDeviceToHost));

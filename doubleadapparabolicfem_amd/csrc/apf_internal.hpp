@@ -143,6 +143,7 @@ struct EstimatorBuffers {
     double* patch_res = nullptr;  // per patch max residual
     double* lt = nullptr;         // slab-reconstructed lambda
     double* sums = nullptr;
+    int* extra_rounds = nullptr;  // refinement rounds beyond refine_steps (all blocks)
     // patch slots in processing order (largest first) and size buckets
     int* s_n = nullptr;
     int* s_np = nullptr;

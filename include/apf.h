@@ -116,6 +116,9 @@ typedef struct {
     double div_residual_l2;
     double max_patch_residual;               /* max_i |K s - b|_inf / max(1, |b|_inf) over patches */
     double seconds_patches, seconds_diagnostics;
+    int extra_refinements;                   /* refinement rounds beyond refine_steps, summed over
+                                                patch blocks (each block refines until its problems pass
+                                                the 1e-10 acceptance, at most 4 more rounds) */
     int slab_iterations;                     /* APF_SLAB_SOLVE: preconditioned iterations */
     double slab_relative_residual;           /* |f - B^T u - A_y lt| / |f - B^T u| (-1: not solved) */
     double seconds_slab;
@@ -182,9 +185,10 @@ int apf_system_set_solution(apf_system* sys, const double* x);
 
 /* Estimator on the current solution: slab reconstruction, one mixed patch
  * problem per (slab, vertex), per-cell flux / residual / reconstruction terms.
- * prec: APF_FP64 (FP64 Cholesky) or APF_FP32_REFINE (FP32 factors, FP64
- * residual, `refine_steps` refinement steps).  Per-cell outputs (may be NULL)
- * are indexed by space-time cell k * n_space_cells + c. */
+ * prec: APF_FP64 (FP64 inverse) or APF_FP32_REFINE (FP32 inverse, FP64
+ * residuals, at least `refine_steps` refinement steps, more where a patch
+ * misses the 1e-10 acceptance).  Per-cell outputs (may be NULL) are indexed by
+ * space-time cell k * n_space_cells + c. */
 int apf_estimate(apf_system* sys, int prec, int refine_steps, apf_estimate_out* out,
                  double* flux_by_cell, double* resid_by_cell, double* recon_by_cell);
 /* Same with the slab reconstruction mode (apf_slab_mode).  APF_SLAB_SOLVE
