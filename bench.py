@@ -31,7 +31,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors a
 # plain copy kernel of known size reads) from the rocprofv3 --pmc runs of
 # `bench.py --profile`, summarised in profiles/.
-MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9}}
+MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9, "solve": 2.40e9, "factor": 1.32e9}}
 
 
 def workload(name):
@@ -94,8 +94,10 @@ def phase_rooflines(sp, ph, steps, mixed):
     out = {}
     t = ph["factor_ms"] * 1e-3
     ach = w["factor"] / t / 1e12
-    out["factor"] = {"bound": "mfma", "kernel": "k_patch_factor_reg (Gauss-Jordan, " + ("FP32" if mixed else "FP64")
-                     + " VALU)", "achieved": ach, "peak": ft_peak, "unit": "TFLOP/s", "frac": ach / ft_peak,
+    name = ("k_patch_factor_blocked (Gauss-Jordan: FP32 VALU panels, MFMA rank-16 updates)" if mixed
+            else "k_patch_factor_full (register Gauss-Jordan, FP64 VALU)")
+    out["factor"] = {"bound": "mfma", "kernel": name, "achieved": ach, "peak": ft_peak, "unit": "TFLOP/s",
+                     "frac": ach / ft_peak,
                      "algorithmic_flops_per_launch": w["factor"], "avg_launch_ms": ph["factor_ms"]}
     t = ph["solve_ms"] * 1e-3
     flops = w["solve_ft"] + w["solve_fp64"]
