@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -277,6 +278,9 @@ void MGDevice<R>::build(System* sys, const MGSetupHost& H, int nyc_)
         return;
     }
     const int L = static_cast<int>(H.levels.size());
+    // tuning knobs: Chebyshev degree and smoothing interval ratio
+    if (const char* e = std::getenv("APF_MG_DEGREE")) nu = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("APF_MG_RATIO")) cheb_ratio = std::max(1.5, std::atof(e));
     T = H.T;
     nyc = nyc_;
     ncol = T + nyc;
