@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 #include <string>
 #include <type_traits>
@@ -1085,9 +1086,19 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int nrt = np >> 4;
-    for (int e = threadIdx.x; e < np * CW; e += blockDim.x) {
-        const int r = e / CW, c = e % CW;
-        Rs[e] = (c < ncw) ? static_cast<FT>(Bq[static_cast<int64_t>(r) * ncols + c]) : static_cast<FT>(0);
+    // B into LDS, 8 loads in flight per thread
+    for (int e0 = threadIdx.x; e0 < np * CW; e0 += 8 * blockDim.x) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x, r = e / CW, c = e % CW;
+            v[u] = (e < np * CW && c < ncw) ? Bq[static_cast<int64_t>(r) * ncols + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < np * CW) Rs[e] = static_cast<FT>(v[u]);
+        }
     }
     __syncthreads();
     // X = Ki B
@@ -1193,16 +1204,16 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
         atomic_max_pos(&patch_res[a], s_worst);
         if (extra) atomicAdd(extra_rounds, extra);
     }
-    // flux coefficients per (spatial cell, local vertex, flux dof) and column
+    // flux coefficients per (spatial cell, local vertex, flux dof) and column:
+    // one (cell, flux dof) row per wave step, lanes along the contiguous columns
     const int cb = topo.cell_ptr[a], nc = topo.cell_ptr[a + 1] - cb;
-    for (int e = threadIdx.x; e < nc * Nf * ncw; e += blockDim.x) {
-        const int ci = e / (Nf * ncw), r = e % (Nf * ncw);
-        const int i = r / ncw, c = r % ncw;
+    for (int row = wave; row < nc * Nf; row += nw) {
+        const int ci = row / Nf, i = row - ci * Nf;
         const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + i];
-        const double val =
-            (fi >= 0) ? topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i] * Xs[fi * CW + c] : 0.0;
+        const double sg = topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i];
         const int64_t sc = topo.cells[cb + ci];
-        sig[((sc * P.NV + topo.lv[cb + ci]) * Nf + i) * ncols + c0 + c] = val;
+        double* out = sig + ((sc * P.NV + topo.lv[cb + ci]) * Nf + i) * ncols + c0;
+        for (int c = lane; c < ncw; c += 64) out[c] = (fi >= 0) ? sg * Xs[fi * CW + c] : 0.0;
     }
 }
 
@@ -1680,6 +1691,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             for (const auto& bk : B.buckets) {
                 // widest column block (16 CT columns, CT <= 6) whose X / R tiles fit the LDS
                 int CT = 6;
+                if (const char* e = std::getenv("APF_SOLVE_CT_MAX")) CT = std::min(CT, std::max(1, std::atoi(e)));
                 while (CT > 1 && static_cast<size_t>(bk.np) * 16 * CT * (sizeof(double) + sizeof(FT)) > lds_cap - 4096)
                     --CT;
                 CT = std::min(CT, (ncols + 15) / 16);
