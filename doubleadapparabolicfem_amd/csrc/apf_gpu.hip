@@ -76,71 +76,94 @@ __global__ void k_spatial_elements(int dim, int ns, int Ns, const double* __rest
 
 // ============================================================== tensor expansion of K
 // K entries on a tensor mesh factor as (spatial matrix entry) x (time table
-// entry); one wavefront slice writes consecutive nonzeros of a row, so the
-// fill is a coalesced stream of stores at the HBM write roofline.
+// entry):
 //   A_y((k,l,s),(k,l',s'))      = K(s,s') h_k Mt[l][l']
 //   B((kp+i,s),(k,l,s'))        = M(s,s') Dt[i][l] + K(s,s') h_k Mt[i][l]
 //   C((0,s),(0,s'))             = -M(s,s')   (t = 0 layer)
-template <int LANES>
-__global__ void __launch_bounds__(256) k_fill_kron(int64_t nY, int64_t n, int nf, int p, int m,
-                                                   const int64_t* __restrict__ rowptr,
+// so row r = (.., s) is a run of segments, each a copy of spatial row s
+// (deg(s) entries) scaled as  aM M(s,:) + (K(s,:) h) aK.  A block owns
+// FILL_ROWS consecutive rows: it tabulates (h, aK, aM) per (row, segment) in
+// LDS, then streams the rows' nonzeros as one contiguous range, thread e of the
+// block writing entries e, e + 256, ..  Stores are whole cache lines except at
+// the two ends of the range and no lane idles at row ends.
+constexpr int FILL_ROWS = 64;
+
+template <int NT>
+__global__ void __launch_bounds__(256) k_fill_kron(int nY, int n, int nf, int m, const int64_t* __restrict__ rowptr,
                                                    const int* __restrict__ sptr, const double* __restrict__ Ksp,
                                                    const double* __restrict__ Msp, const double* __restrict__ times,
                                                    TimeTables tt, double* __restrict__ val)
 {
-    const int Nt = p + 1;
-    const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / LANES;
-    const int lane = threadIdx.x % LANES;
-    if (row >= n) return;
-    const int64_t r0 = rowptr[row];
-    const int64_t len = rowptr[row + 1] - r0;
-    if (row < nY) {
-        const int s = static_cast<int>(row % nf);
-        const int64_t kl = row / nf;
-        const int k = static_cast<int>(kl / Nt), l = static_cast<int>(kl % Nt);
-        const double h = times[k + 1] - times[k];
-        const int sb = sptr[s];
-        const int deg = sptr[s + 1] - sb;
-        for (int64_t e = lane; e < len; e += LANES) {
-            const int pos = static_cast<int>(e);
-            double v;
-            if (pos < Nt * deg) {
-                const int lp = pos / deg, r = pos - lp * deg;
-                v = Ksp[sb + r] * h * tt.Mt[l * Nt + lp];
-            } else {
-                const int q = pos - Nt * deg;
-                const int it = q / deg, r = q - it * deg;
-                v = Msp[sb + r] * tt.Dt[it * Nt + l] + Ksp[sb + r] * h * tt.Mt[it * Nt + l];
+    constexpr int P = NT - 1, SEG = 3 * NT;
+    __shared__ int s_start[FILL_ROWS + 1], s_sb[FILL_ROWS], s_deg[FILL_ROWS];
+    __shared__ float s_rcp[FILL_ROWS];
+    __shared__ double s_h[FILL_ROWS * SEG], s_aK[FILL_ROWS * SEG], s_aM[FILL_ROWS * SEG];
+    const int r0 = blockIdx.x * FILL_ROWS;
+    const int nrows = min(FILL_ROWS, n - r0);
+    const int64_t base = rowptr[r0];
+    for (int t = threadIdx.x; t <= nrows; t += blockDim.x) s_start[t] = static_cast<int>(rowptr[r0 + t] - base);
+    for (int t = threadIdx.x; t < nrows * SEG; t += blockDim.x) {
+        const int i = t / SEG, g = t - i * SEG;
+        const int row = r0 + i;
+        double h = 0.0, aK = 0.0, aM = 0.0;
+        int s;
+        if (row < nY) {
+            // Y row (k, l, s): segments l' = 0..NT-1 of A_y, then i = 0..NT-1 of B^T
+            s = row % nf;
+            const int kl = row / nf, k = kl / NT, l = kl - k * NT;
+            h = times[k + 1] - times[k];
+            if (g < NT) {
+                aK = tt.Mt[l * NT + g];
+            } else if (g < 2 * NT) {
+                const int it = g - NT;
+                aK = tt.Mt[it * NT + l];
+                aM = tt.Dt[it * NT + l];
             }
-            __builtin_nontemporal_store(v, &val[r0 + e]);
-        }
-    } else {
-        const int64_t x = row - nY;
-        const int tn = static_cast<int>(x / nf);
-        const int s = static_cast<int>(x % nf);
-        const int sb = sptr[s];
-        const int deg = sptr[s + 1] - sb;
-        int k0, nk;
-        time_node_intervals(tn, p, m, k0, nk);
-        for (int64_t e = lane; e < len; e += LANES) {
-            const int pos = static_cast<int>(e);
-            double v;
-            const int blk = pos / (Nt * deg);
+        } else {
+            // X row (tn, s): segments (interval k0 + blk, l), then the C block (tq) when tn <= p
+            const int x = row - nY, tn = x / nf;
+            s = x - tn * nf;
+            int k0, nk;
+            time_node_intervals(tn, P, m, k0, nk);
+            const int blk = g / NT, l = g - blk * NT;
             if (blk < nk) {
-                const int k = k0 + blk;
-                const int it = tn - k * p;
-                const int q = pos - blk * Nt * deg;
-                const int l = q / deg, r = q - l * deg;
-                const double h = times[k + 1] - times[k];
-                v = Msp[sb + r] * tt.Dt[it * Nt + l] + Ksp[sb + r] * h * tt.Mt[it * Nt + l];
-            } else {
-                // C block (t = 0 layer): -M(s,s') between the t = 0 nodes, structural zeros otherwise
-                const int q = pos - nk * Nt * deg;
-                const int tq = q / deg, r = q - tq * deg;
-                v = (tn == 0 && tq == 0) ? -Msp[sb + r] : 0.0;
+                const int k = k0 + blk, it = tn - k * P;
+                h = times[k + 1] - times[k];
+                aK = tt.Mt[it * NT + l];
+                aM = tt.Dt[it * NT + l];
+            } else if (blk == nk) {
+                aM = (tn == 0 && l == 0) ? -1.0 : 0.0;
             }
-            __builtin_nontemporal_store(v, &val[r0 + e]);
         }
+        s_h[t] = h;
+        s_aK[t] = aK;
+        s_aM[t] = aM;
+        if (g == 0) {
+            const int sb = sptr[s], deg = sptr[s + 1] - sb;
+            s_sb[i] = sb;
+            s_deg[i] = deg;
+            s_rcp[i] = 1.0f / static_cast<float>(deg);
+        }
+    }
+    __syncthreads();
+    const int len = s_start[nrows];
+    double* __restrict__ out = val + base;
+    int cur = 0;
+    for (int e = threadIdx.x; e < len; e += blockDim.x) {
+        while (e >= s_start[cur + 1]) ++cur;
+        const int pos = e - s_start[cur], deg = s_deg[cur];
+        // pos / deg through the float reciprocal, corrected by one step (pos < 2^20)
+        int g = static_cast<int>(static_cast<float>(pos) * s_rcp[cur]);
+        int r = pos - g * deg;
+        if (r < 0) {
+            --g;
+            r += deg;
+        } else if (r >= deg) {
+            ++g;
+            r -= deg;
+        }
+        const int j = s_sb[cur] + r, t = cur * SEG + g;
+        out[e] = Msp[j] * s_aM[t] + (Ksp[j] * s_h[t]) * s_aK[t];
     }
 }
 
@@ -1001,12 +1024,27 @@ int apf_assemble_main(apf_system* sys, const apf_coeff* coeff, int mode)
                                sys->d_bphi, sys->d_u0b, sys->d_Ksp, sys->d_Msp, sys->d_msp, sys->d_gsp, nullptr);
         }
         if (mode == APF_ASSEMBLE_KRON) {
-            constexpr int L = 32;
-            const int64_t threads = S.n * L;
-            const int block = 256;
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fill_kron<L>), dim3(static_cast<unsigned>((threads + block - 1) / block)),
-                               dim3(block), 0, st, S.nY, S.n, S.nf, S.p, S.mesh.m(), sys->d_rowptr, sys->d_sptr,
-                               sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_val);
+            // row indices fit int (build_pattern rejects n >= 2^31)
+            const dim3 grid(static_cast<unsigned>((S.n + FILL_ROWS - 1) / FILL_ROWS)), block(256);
+            const int nY = static_cast<int>(S.nY), n = static_cast<int>(S.n), m = S.mesh.m();
+            switch (S.Nt) {
+            case 2:
+                hipLaunchKernelGGL(k_fill_kron<2>, grid, block, 0, st, nY, n, S.nf, m, sys->d_rowptr, sys->d_sptr,
+                                   sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_val);
+                break;
+            case 3:
+                hipLaunchKernelGGL(k_fill_kron<3>, grid, block, 0, st, nY, n, S.nf, m, sys->d_rowptr, sys->d_sptr,
+                                   sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_val);
+                break;
+            case 4:
+                hipLaunchKernelGGL(k_fill_kron<4>, grid, block, 0, st, nY, n, S.nf, m, sys->d_rowptr, sys->d_sptr,
+                                   sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_val);
+                break;
+            default:
+                hipLaunchKernelGGL(k_fill_kron<5>, grid, block, 0, st, nY, n, S.nf, m, sys->d_rowptr, sys->d_sptr,
+                                   sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_val);
+                break;
+            }
         } else {
             HIP_TRY(hipMemsetAsync(sys->d_val, 0, S.col.size() * sizeof(double), st));
             const int nloc = S.Ns * S.Nt;
