@@ -1056,11 +1056,14 @@ __device__ inline void tile_product(const MT* mp, int np, const BT* Bs, int ldb,
 // (FP64 MFMA) and X += Ki R (FT); the residual of the final X gives the
 // acceptance statistic |K x - b|_inf / max(1, |b|_inf) per (slab, vertex)
 // problem (dense_local_error_blocks_2d.hpp:923-949).  Each wave owns row tiles
-// rt = wave + 8 j.  The flux coefficients go to sig.
-template <typename FT, int CT>
+// rt = wave + 8 j.  The flux coefficients go to sig.  X and R live in LDS, or,
+// with GWS (patches whose tiles exceed the LDS), in this workgroup's slice of
+// the global workspace ws (np x CW doubles, then np x CW FT).
+template <typename FT, int CT, bool GWS>
 __global__ void __launch_bounds__(SOLVE_THREADS)
     k_patch_solve(EstParams P, PatchTopo topo, PatchSlots ps, int q0, int count, int ncg, int cpw, const double* Kd,
-                  const FT* Ki, const double* RHS, double* sig, double* patch_res, int* extra_rounds)
+                  const FT* Ki, const double* RHS, double* sig, double* patch_res, int* extra_rounds, double* ws,
+                  int64_t ws_stride)
 {
     using V4 = typename Vec4<FT>::type;
     constexpr int CW = 16 * CT;
@@ -1081,7 +1084,8 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
     const double* Kq = Kd + ps.off2[q];
     const FT* Iq = Ki + ps.off2[q];
     const double* Bq = RHS + ps.off1[q] * ncols + c0;
-    double* Xs = lds;                                                  // [np][CW]
+    double* Xs = lds;  // [np][CW]
+    if constexpr (GWS) Xs = ws + (static_cast<int64_t>(qb) * ncg + slot % ncg) * ws_stride;
     FT* Rs = reinterpret_cast<FT*>(Xs + static_cast<int64_t>(np) * CW);  // [np][CW]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -1441,6 +1445,36 @@ __global__ void k_max_reduce(int n, const double* a, double* out)
 
 namespace apf {
 
+// Column blocking of the patch solve at padded order np: the widest block of
+// 16 CT columns (CT <= 6, whole slabs) whose X / R tiles fit the LDS.  When
+// even 16 columns do not fit, the tiles go to a global workspace instead.
+struct SolveShape {
+    int CT, cpw, ncg;
+    bool gws;
+    size_t lds_bytes;   // dynamic LDS per workgroup
+    int64_t ws_stride;  // workspace doubles per workgroup (gws only)
+};
+
+SolveShape solve_shape(int np, int ncols, int Nt, size_t ft_bytes)
+{
+    const size_t cap = 160 * 1024 - 4096;
+    auto tiles = [&](int ct) { return static_cast<size_t>(np) * 16 * ct * (sizeof(double) + ft_bytes); };
+    SolveShape sh{};
+    int CT = 6;
+    if (const char* e = std::getenv("APF_SOLVE_CT_MAX")) CT = std::min(CT, std::max(1, std::atoi(e)));
+    sh.gws = tiles(1) > cap;
+    if (!sh.gws)
+        while (CT > 1 && tiles(CT) > cap) --CT;
+    CT = std::min(CT, (ncols + 15) / 16);
+    sh.CT = CT;
+    sh.cpw = std::max(Nt, (16 * CT / Nt) * Nt);
+    if (sh.cpw > 16 * CT) throw std::runtime_error("patch solve: a slab's modes exceed the column block");
+    sh.ncg = (ncols + sh.cpw - 1) / sh.cpw;
+    sh.lds_bytes = sh.gws ? 0 : tiles(CT);
+    sh.ws_stride = sh.gws ? static_cast<int64_t>((tiles(CT) + sizeof(double) - 1) / sizeof(double)) : 0;
+    return sh;
+}
+
 void estimator_alloc(System* sys)
 {
     const Space& S = *sys->space;
@@ -1528,6 +1562,16 @@ void estimator_alloc(System* sys)
     HIP_TRY(hipMemset(B.Ki, 0, static_cast<size_t>(o2) * sizeof(double)));
     HIP_TRY(hipMemset(B.Ki32, 0, static_cast<size_t>(o2) * sizeof(float)));
     HIP_TRY(hipMemset(B.RHS, 0, static_cast<size_t>(o1) * S.mesh.m() * T.Nt * sizeof(double)));
+    // one workspace for the buckets whose solve tiles exceed the LDS (buckets
+    // run one after another on the stream), sized for either precision
+    int64_t ws = 0;
+    for (const auto& bk : B.buckets)
+        for (size_t ft : {sizeof(double), sizeof(float)}) {
+            const SolveShape sh = solve_shape(bk.np, S.mesh.m() * T.Nt, T.Nt, ft);
+            if (sh.gws) ws = std::max(ws, static_cast<int64_t>(bk.count) * sh.ncg * sh.ws_stride);
+        }
+    B.solve_ws = ws ? dalloc<double>(sys, static_cast<size_t>(ws)) : nullptr;
+    B.solve_ws_doubles = ws;
     const int64_t ncell = static_cast<int64_t>(S.mesh.m()) * S.mesh.ns();
     B.sig = dalloc<double>(sys, static_cast<size_t>(ncell) * (S.dim + 1) * T.Nf * T.Nt);
     B.flux = dalloc<double>(sys, ncell);
@@ -1606,11 +1650,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         out->seconds_slab = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_slab).count();
         HIP_TRY(hipMemsetAsync(B.patch_res, 0, S.patches.n_patches * sizeof(double), st));
         HIP_TRY(hipMemsetAsync(B.extra_rounds, 0, sizeof(int), st));
-        const size_t ft_bytes = (prec == APF_FP64) ? sizeof(double) : sizeof(float);
         const size_t lds_cap = 160 * 1024;
-        if (static_cast<size_t>(B.nmax) * 16 * (sizeof(double) + ft_bytes) > lds_cap - 4096)
-            return fail(APF_BAD_ARG, "apf_estimate: patch order " + std::to_string(B.nmax) +
-                                         " exceeds the batched patch-solve limit of this build");
         PatchSlots ps{B.s_n, B.s_np, B.s_off2, B.s_off1};
         const int ncols = P.m * P.Nt;
         auto set_lds = [&](const void* fn, size_t bytes) {
@@ -1689,29 +1729,34 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             }
             HIP_TRY(hipEventRecord(sys->evp[2], st));
             for (const auto& bk : B.buckets) {
-                // widest column block (16 CT columns, CT <= 6) whose X / R tiles fit the LDS
-                int CT = 6;
-                if (const char* e = std::getenv("APF_SOLVE_CT_MAX")) CT = std::min(CT, std::max(1, std::atoi(e)));
-                while (CT > 1 && static_cast<size_t>(bk.np) * 16 * CT * (sizeof(double) + sizeof(FT)) > lds_cap - 4096)
-                    --CT;
-                CT = std::min(CT, (ncols + 15) / 16);
-                const int cpw = std::max(P.Nt, (16 * CT / P.Nt) * P.Nt);
-                if (cpw > 16 * CT) throw std::runtime_error("patch solve: a slab's modes exceed the column block");
-                const int ncg = (ncols + cpw - 1) / cpw;
-                const size_t ls = static_cast<size_t>(bk.np) * 16 * CT * (sizeof(double) + sizeof(FT));
-                const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * ncg);
+                const SolveShape sh = solve_shape(bk.np, ncols, P.Nt, sizeof(FT));
+                const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * sh.ncg);
+                if (sh.gws && static_cast<int64_t>(bk.count) * sh.ncg * sh.ws_stride > B.solve_ws_doubles)
+                    throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
                 auto solve = [&](auto kern) {
-                    set_lds(reinterpret_cast<const void*>(kern), ls);
-                    hipLaunchKernelGGL(kern, dim3(grid), dim3(SOLVE_THREADS), ls, st, P, topo, ps, bk.q0, bk.count,
-                                       ncg, cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res, B.extra_rounds);
+                    if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
+                    hipLaunchKernelGGL(kern, dim3(grid), dim3(SOLVE_THREADS), sh.lds_bytes, st, P, topo, ps, bk.q0,
+                                       bk.count, sh.ncg, sh.cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res, B.extra_rounds,
+                                       B.solve_ws, sh.ws_stride);
                 };
-                switch (CT) {
-                    case 1: solve(k_patch_solve<FT, 1>); break;
-                    case 2: solve(k_patch_solve<FT, 2>); break;
-                    case 3: solve(k_patch_solve<FT, 3>); break;
-                    case 4: solve(k_patch_solve<FT, 4>); break;
-                    case 5: solve(k_patch_solve<FT, 5>); break;
-                    default: solve(k_patch_solve<FT, 6>); break;
+                if (sh.gws) {
+                    switch (sh.CT) {
+                        case 1: solve(k_patch_solve<FT, 1, true>); break;
+                        case 2: solve(k_patch_solve<FT, 2, true>); break;
+                        case 3: solve(k_patch_solve<FT, 3, true>); break;
+                        case 4: solve(k_patch_solve<FT, 4, true>); break;
+                        case 5: solve(k_patch_solve<FT, 5, true>); break;
+                        default: solve(k_patch_solve<FT, 6, true>); break;
+                    }
+                    continue;
+                }
+                switch (sh.CT) {
+                    case 1: solve(k_patch_solve<FT, 1, false>); break;
+                    case 2: solve(k_patch_solve<FT, 2, false>); break;
+                    case 3: solve(k_patch_solve<FT, 3, false>); break;
+                    case 4: solve(k_patch_solve<FT, 4, false>); break;
+                    case 5: solve(k_patch_solve<FT, 5, false>); break;
+                    default: solve(k_patch_solve<FT, 6, false>); break;
                 }
             }
         };

@@ -153,6 +153,10 @@ struct EstimatorBuffers {
     double* Ki = nullptr;    // FP64 inverses (and block-elimination workspace)
     float* Ki32 = nullptr;   // FP32 inverses of the mixed path
     double* RHS = nullptr;   // right-hand sides, np x (slabs * Nt) per patch
+    // X / R tiles of the patch solves whose 16-column tiles exceed the LDS
+    // (one [np][16 CT] FP64 + FT pair per workgroup of the largest such bucket)
+    double* solve_ws = nullptr;
+    int64_t solve_ws_doubles = 0;
     struct Bucket {
         int q0, count, np, nc_max;
         int blk_max;  // largest flux block or Schur block order in the bucket

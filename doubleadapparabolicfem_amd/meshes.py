@@ -159,3 +159,53 @@ def segment_batch_coefficients(mesh: Mesh, nx, kappa, coef) -> Coefficients:
 
     return Coefficients(M, 0.0, u0)
 
+
+
+def fan_mesh(q: int, times, seed: int = 3, smin: int = 1, smax: int = 16) -> Mesh:
+    """Fan-patch mesh of (0,1)^2: q x q squares, every square side split into s
+    equal segments (s uniform in smin..smax per side, numpy PCG64 with `seed`),
+    and every square a fan of triangles around its centre vertex.  The centre's
+    vertex patch has K = (sum of its four side counts) cells, 4..64 at the
+    default range; the side and corner patches have 4..8.  The mesh is
+    conforming: both squares of a shared side see the same split."""
+    rng = np.random.default_rng(seed)
+    sh = rng.integers(smin, smax + 1, size=(q + 1, q))  # side y = j/q, x in [i/q, (i+1)/q]
+    sv = rng.integers(smin, smax + 1, size=(q + 1, q))  # side x = i/q, y in [j/q, (j+1)/q]
+    verts = [(i / q, j / q) for j in range(q + 1) for i in range(q + 1)]
+    corner = lambda i, j: j * (q + 1) + i
+    hp, vp = {}, {}
+    for j in range(q + 1):
+        for i in range(q):
+            s = sh[j, i]
+            hp[j, i] = list(range(len(verts), len(verts) + s - 1))
+            verts += [((i + k / s) / q, j / q) for k in range(1, s)]
+    for i in range(q + 1):
+        for j in range(q):
+            s = sv[i, j]
+            vp[i, j] = list(range(len(verts), len(verts) + s - 1))
+            verts += [(i / q, (j + k / s) / q) for k in range(1, s)]
+    cells = []
+    for j in range(q):
+        for i in range(q):
+            c = len(verts)
+            verts.append(((i + 0.5) / q, (j + 0.5) / q))
+            ring = ([corner(i, j)] + hp[j, i] + [corner(i + 1, j)] + vp[i + 1, j] + [corner(i + 1, j + 1)]
+                    + hp[j + 1, i][::-1] + [corner(i, j + 1)] + vp[i, j][::-1])
+            cells += [(ring[k], ring[(k + 1) % len(ring)], c) for k in range(len(ring))]
+    return Mesh(2, np.array(verts), np.array(cells, dtype=np.int32), np.asarray(times, dtype=np.float64))
+
+
+def graded_times(lmin: int = 2, lmax: int = 10) -> np.ndarray:
+    """Time grid of [0,1] graded toward t = 0: uniform intervals of length
+    2^-lmin, the first one bisected toward 0 down to length 2^-lmax."""
+    head = [0.0] + [2.0 ** -l for l in range(lmax, lmin - 1, -1)]
+    tail = [k / 2 ** lmin for k in range(2, 2 ** lmin + 1)]
+    return np.array(head + tail)
+
+
+def indicator_initial() -> Coefficients:
+    """M = I, ell = 0, u0 = indicator of [0.25, 0.75]^2."""
+    def u0(x):
+        inside = (x[:, 0] >= 0.25) & (x[:, 0] <= 0.75) & (x[:, 1] >= 0.25) & (x[:, 1] <= 0.75)
+        return inside.astype(np.float64)
+    return Coefficients(None, 0.0, u0)

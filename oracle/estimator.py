@@ -298,9 +298,14 @@ def estimate(stm: SpaceTimeMesh, dofs: SpaceTimeDofs, prob: Problem, S: SaddleBl
                 Fst = (Fm[:, :, None, :] * LB.Lt[:, None, :, None]).reshape(nq, nF, d)
                 Fdst = (Fd[:, :, None] * LB.Lt[:, None, :]).reshape(nq, nF)
                 Sst = (Sm[:, :, None] * LB.Lt[:, None, :]).reshape(nq, nS)
-                MiF = np.einsum("ab,qib->qia", Minv[s], Fst)
-                A += np.einsum("q,qia,qja->ij", dmu, Fst, MiF)
-                Bm += np.einsum("q,qi,qj->ij", dmu, Sst, Fdst)
+                # the cell's own columns only (space-time index = patch dof * Nt + time mode)
+                fp = np.array(sorted({pi for (_, pi, _) in pt.flux_map[ci]}), dtype=np.int64)
+                fc = (fp[:, None] * Nt + np.arange(Nt)[None, :]).ravel()
+                sc = ((pt.scalar_off[ci] + np.arange(nsl))[:, None] * Nt + np.arange(Nt)[None, :]).ravel()
+                Fc = Fst[:, fc]
+                MiF = np.einsum("ab,qib->qia", Minv[s], Fc)
+                A[np.ix_(fc, fc)] += np.einsum("q,qia,qja->ij", dmu, Fc, MiF, optimize=True)
+                Bm[np.ix_(sc, fc)] += np.einsum("q,qi,qj->ij", dmu, Sst[:, sc], Fdst[:, fc], optimize=True)
                 f += np.einsum("q,qia,qa->i", dmu, Fst, -psi[:, None] * grad_theta)
                 Mgt = grad_theta @ prob.M[s].T
                 gval = psi * (prob.ell - dtu) - Mgt @ gpsi
