@@ -9,7 +9,7 @@ mean device time of assembly + estimator per step; MINRES seconds per solve
 are reported beside it.  Inputs are resident in HBM when the timed region
 starts (the mesh, pattern and coefficients are uploaded once).
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c1|c4|c5]
+    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5]
 
 Prints one JSON line.
 """
@@ -66,6 +66,73 @@ def workload(name):
                          "n_x log-uniform for 1e3-1e5 X DOFs rounded to multiples of 64, 32 shared time "
                          "intervals, kappa 10^U(-1,1), u0 = 5 random sine modes, seeds 5000+i")
     raise SystemExit(f"unknown config {name}")
+
+
+def run_c3(args):
+    """Fan-patch workload: q = 8 squares per side, side splits uniform in 1..16
+    (seed 3), so the 64 fan centres have K in 4..64 cells; time graded toward
+    t = 0 (intervals 2^-10 .. 2^-2, then 1/4); u0 = indicator of [0.25,0.75]^2.
+    The same mesh at p = 1, 2, 3 (every patch of each degree); value is the sum
+    of DOFs over the sum of assembly + mixed estimator device times."""
+    import doubleadapparabolicfem_amd as apf
+    from doubleadapparabolicfem_amd import meshes
+
+    m = meshes.fan_mesh(8, meshes.graded_times(2, 10), seed=3)
+    coeff = meshes.indicator_initial()
+    ctx = apf.Context(0)
+    per_p, tot_dofs, tot_ms, wall = {}, 0, 0.0, 0.0
+    for p in (1, 2, 3):
+        sp = apf.Space(m, p, coarse=meshes.fan_hierarchy(8))
+        sysm = apf.System(ctx, sp)
+        nf, ns_, nc, bd = sp.patch_sizes()
+        order = nf + ns_ + (bd == 0)
+        rec = []
+        for i in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            sysm.assemble(coeff)
+            t_asm = sysm.timings()["assembly_ms"]
+            t1 = time.perf_counter()
+            x, st = sysm.minres(tol=1e-10, max_it=20000, check_interval=20, download=False)
+            t_minres = time.perf_counter() - t1
+            est = sysm.estimate(apf.APF_FP32_REFINE, 2, per_cell=False)
+            if i >= args.warmup:
+                wall += time.perf_counter() - t0
+                rec.append((t_asm, t_minres, st["iterations"], (est["seconds_patches"] + est["seconds_diagnostics"]) * 1e3,
+                            sysm.phase_timings(), est))
+        est64 = sysm.estimate(apf.APF_FP64, 0, per_cell=False)
+        ph64 = sysm.phase_timings()
+        t_asm = float(np.mean([r[0] for r in rec]))
+        t_est = float(np.mean([r[3] for r in rec]))
+        ph = {k: float(np.mean([r[4][k] for r in rec])) for k in rec[0][4]}
+        est = rec[-1][5]
+        n_dofs = sp.nX + sp.nY
+        tot_dofs += n_dofs
+        tot_ms += t_asm + t_est
+        per_p[f"p{p}"] = {
+            "n_X": sp.nX, "n_Y": sp.nY, "patches_per_slab": int(order.size), "max_patch_cells": int(nc.max()),
+            "max_patch_order": int(order.max()), "assembly_ms": t_asm, "estimator_ms_mixed": t_est,
+            "estimator_ms_fp64": (est64["seconds_patches"] + est64["seconds_diagnostics"]) * 1e3,
+            "phases_mixed_ms": ph, "phases_fp64_ms": ph64,
+            "minres_s_per_solve": float(np.mean([r[1] for r in rec])), "minres_iterations": int(rec[-1][2]),
+            "minres_preconditioner": "multigrid over criss-cross 8x8, union-jack 8x8 .. 2x2",
+            "mixed_vs_fp64_rel_err_flux2": abs(est["flux2"] - est64["flux2"]) / abs(est64["flux2"]),
+            "max_patch_residual": est["max_patch_residual"], "extra_refinements": est["extra_refinements"],
+            "roofline_patch_phases_mixed": phase_rooflines(sp, ph, 2, True)}
+    dom_p = max(per_p, key=lambda k: per_p[k]["estimator_ms_mixed"])
+    roof = per_p[dom_p]["roofline_patch_phases_mixed"]
+    dom = max(("factor", "solve"), key=lambda k: roof[k]["avg_launch_ms"])
+    out = {
+        "metric": METRIC, "value": tot_dofs / (tot_ms * 1e-3), "unit": "DOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2+1D fan-patch mesh: 8x8 squares, side splits U{1..16} (seed 3), fan centres "
+                               "with K in 4..64 cells; 12 intervals graded toward t=0 (2^-10 .. 1/4); "
+                               "u0 = indicator of [0.25,0.75]^2; p = 1, 2, 3 in turn",
+                   "space_cells": int(m.cells.shape[0]), "intervals": int(m.times.size - 1)},
+        "roofline": dict(roof[dom], degree=dom_p, traffic=None),
+        "per_degree": per_p,
+    }
+    print(json.dumps(out))
 
 
 def patch_work(sp, steps):
@@ -155,6 +222,9 @@ def main():
 
     import doubleadapparabolicfem_amd as apf
 
+    if args.config == "c3":
+        run_c3(args)
+        return
     W = workload(args.config)
     sp = apf.Space(W["mesh"], W["p"], coarse=W["coarse"])
     ctx = apf.Context(0)

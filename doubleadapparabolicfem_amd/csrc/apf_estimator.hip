@@ -903,6 +903,255 @@ __global__ void __launch_bounds__(NW * 64)
     }
 }
 
+// Phase 1 for large patches (order > 128 with a diagonal block beyond the
+// Schur kernel, padded order <= 2048): Gauss-Jordan inversion blocked in
+// 16-column panels across launches, batched over every such patch.  Each
+// member g owns a workspace region at ws + off[g] (doubles):
+//   T    [np][np]  FT  the equilibrated matrix D K D, inverted in place;
+//   d    [np]          d_i = 1 / sqrt(max_j |K_ij|) (1 on padding rows);
+//   piv  [np]      int pivot row of column k (implicit pivoting: no swaps);
+//   mark [np]      int panel that took row i as pivot, -1 while unused.
+// Padding rows/columns n..np-1 carry an identity block, so every column has a
+// pivot.  Panel c0 (k_big_panel) eliminates columns c0..c0+15 with partial
+// pivoting over the unused rows; T's panel columns then hold P = M [e_r0..e_r15]
+// for the panel's Gauss-Jordan transform M = I + (P - E_R) E_R^T, and the
+// other columns take  T[:,j] <- (rows R zeroed) T[:,j] + P T[R,j]
+// (k_big_update, MFMA).  k_big_finish undoes the implicit pivoting and the
+// scaling: K^-1[k][r_k'] = d_k T[r_k][k'] d_r_k'.
+struct BigPatches {
+    const int* q;        // slot of member g
+    const int64_t* off;  // workspace offset (doubles) of member g
+    double* ws;
+};
+
+struct BigView {
+    int np;
+    void* T;
+    double* d;
+    int* piv;
+    int* mark;
+};
+
+__device__ inline BigView big_view(const PatchSlots& ps, const BigPatches& bp, int g)
+{
+    BigView v;
+    v.np = ps.np[bp.q[g]];
+    double* base = bp.ws + bp.off[g];
+    v.T = base;
+    v.d = base + static_cast<int64_t>(v.np) * v.np;
+    v.piv = reinterpret_cast<int*>(v.d + v.np);
+    v.mark = v.piv + v.np;
+    return v;
+}
+
+template <typename FT>
+__global__ void __launch_bounds__(PATCH_THREADS)
+    k_big_prepare(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, BigPatches bp, const double* verts,
+                  const int* cells, const double* Mcell, double* Kd)
+{
+    const int q = bp.q[blockIdx.x];
+    const int a = topo.order[q], n = ps.n[q], np = ps.np[q];
+    double* K = Kd + ps.off2[q];
+    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
+    const BigView v = big_view(ps, bp, blockIdx.x);
+    FT* A = static_cast<FT*>(v.T);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int i = wave; i < np; i += nw) {
+        double mx = 0.0;
+        if (i < n)
+            for (int j = lane; j < n; j += 64) mx = fmax(mx, fabs(K[static_cast<int64_t>(i) * np + j]));
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+        if (lane == 0) {
+            v.d[i] = (i < n && mx > 0.0) ? 1.0 / sqrt(mx) : 1.0;
+            v.mark[i] = -1;
+            v.piv[i] = i;
+        }
+    }
+    __syncthreads();
+    for (int i = wave; i < np; i += nw) {
+        const double di = v.d[i];
+        const double* Kr = K + static_cast<int64_t>(i) * np;
+        FT* Ar = A + static_cast<int64_t>(i) * np;
+        for (int j = lane; j < np; j += 64)
+            Ar[j] = (i < n && j < n) ? static_cast<FT>(di * Kr[j] * v.d[j]) : static_cast<FT>(i == j ? 1 : 0);
+    }
+}
+
+// one workgroup of 512 threads per member; thread t owns rows t + 512 s (s < RPT)
+// of the 16-column panel in registers
+template <typename FT, int RPT>
+__global__ void __launch_bounds__(512) k_big_panel(PatchSlots ps, BigPatches bp, int c0)
+{
+    const BigView v = big_view(ps, bp, blockIdx.x);
+    const int np = v.np;
+    if (c0 >= np) return;
+    FT* A = static_cast<FT*>(v.T);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ double s_best[8];
+    __shared__ int s_row[8];
+    __shared__ FT s_prow[16];
+    FT x[RPT][16];
+    bool used[RPT];
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+        const int i = tid + 512 * s;
+        used[s] = true;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[s][j] = static_cast<FT>(0);
+        if (i < np) {
+            used[s] = v.mark[i] >= 0;
+            const FT* Ar = A + static_cast<int64_t>(i) * np + c0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) x[s][j] = Ar[j];
+        }
+    }
+    const int panel = c0 >> 4;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        // largest |x| over the unused rows, ties to the lower row; a NaN wins so
+        // that some row is always taken
+        double best = -1.0;
+        int br = 0x7fffffff;
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) {
+            const double av = fabs(static_cast<double>(x[s][k]));
+            if (!used[s] && !(av <= best)) {
+                best = av;
+                br = tid + 512 * s;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off, 64);
+            const int orow = __shfl_xor(br, off, 64);
+            if (!(ob <= best) || (ob == best && orow < br)) {
+                best = ob;
+                br = orow;
+            }
+        }
+        if (lane == 0) {
+            s_best[wave] = best;
+            s_row[wave] = br;
+        }
+        __syncthreads();
+        best = s_best[0];
+        br = s_row[0];
+        for (int w = 1; w < 8; ++w)
+            if (!(s_best[w] <= best) || (s_best[w] == best && s_row[w] < br)) {
+                best = s_best[w];
+                br = s_row[w];
+            }
+        const int r = min(max(br, 0), np - 1);
+#pragma unroll
+        for (int s = 0; s < RPT; ++s)
+            if (tid + 512 * s == r) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) s_prow[j] = x[s][j];
+                used[s] = true;
+                v.mark[r] = panel;
+                v.piv[c0 + k] = r;
+            }
+        __syncthreads();
+        const FT inv = static_cast<FT>(1) / s_prow[k];
+        FT pr[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) pr[j] = s_prow[j] * inv;
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) {
+            if (tid + 512 * s == r) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) x[s][j] = (j == k) ? inv : pr[j];
+            } else {
+                const FT f = x[s][k];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) x[s][j] = (j == k) ? -f * inv : x[s][j] - f * pr[j];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+        const int i = tid + 512 * s;
+        if (i < np) {
+            FT* Ar = A + static_cast<int64_t>(i) * np + c0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) Ar[j] = x[s][j];
+        }
+    }
+}
+
+// workgroup (member, 64-column strip): T[:, strip] <- (rows R zeroed) + P T[R, strip],
+// the panel's own columns excepted; 4 waves over the 16-row tiles
+template <typename FT>
+__global__ void __launch_bounds__(256) k_big_update(PatchSlots ps, BigPatches bp, int c0, int nstrips)
+{
+    using V4 = typename Vec4<FT>::type;
+    const int g = blockIdx.x / nstrips, strip = blockIdx.x - g * nstrips;
+    const BigView v = big_view(ps, bp, g);
+    const int np = v.np, j0 = strip * 64;
+    if (c0 >= np || j0 >= np) return;
+    FT* A = static_cast<FT*>(v.T);
+    __shared__ FT s_ar[16][64];
+    __shared__ int s_r[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    if (tid < 16) s_r[tid] = v.piv[c0 + tid];
+    __syncthreads();
+    for (int e = tid; e < 16 * 64; e += blockDim.x) {
+        const int k = e >> 6, c = e & 63, j = j0 + c;
+        s_ar[k][c] = (j < np) ? A[static_cast<int64_t>(s_r[k]) * np + j] : static_cast<FT>(0);
+    }
+    __syncthreads();
+    const int panel = c0 >> 4;
+    for (int rt = wave; rt < (np >> 4); rt += 4) {
+        V4 acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = V4{0, 0, 0, 0};
+        const FT* Pr = A + static_cast<int64_t>(rt * 16 + li) * np + c0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const FT a = Pr[4 * t + lk];
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = mfma_16x16x4(a, s_ar[4 * t + lk][ct * 16 + li], acc[ct]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rt * 16 + drow<FT>(lk, r);
+            const bool in_r = v.mark[row] == panel;
+            FT* Ar = A + static_cast<int64_t>(row) * np;
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const int j = j0 + ct * 16 + li;
+                if (j < np && (j < c0 || j >= c0 + 16)) Ar[j] = (in_r ? static_cast<FT>(0) : Ar[j]) + acc[ct][r];
+            }
+        }
+    }
+}
+
+// workgroup (member, 16 output rows): K^-1[k][r_k'] = d_k T[r_k][k'] d_r_k' into Ki,
+// zero on the padding
+template <typename FT>
+__global__ void __launch_bounds__(256) k_big_finish(PatchSlots ps, BigPatches bp, int nrb, FT* Ki)
+{
+    const int g = blockIdx.x / nrb, rb = blockIdx.x - g * nrb;
+    const int q = bp.q[g];
+    const BigView v = big_view(ps, bp, g);
+    const int np = v.np, n = ps.n[q];
+    if (rb * 16 >= np) return;
+    const FT* A = static_cast<const FT*>(v.T);
+    FT* out = Ki + ps.off2[q];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = rb * 16 + wave; k < min(np, rb * 16 + 16); k += 4) {
+        const FT* src = A + static_cast<int64_t>(v.piv[k]) * np;
+        FT* dst = out + static_cast<int64_t>(k) * np;
+        const double dk = v.d[k];
+        for (int c = lane; c < np; c += 64) {
+            const int pc = v.piv[c];
+            dst[pc] = (k < n && pc < n) ? static_cast<FT>(dk * static_cast<double>(src[c]) * v.d[pc])
+                                        : static_cast<FT>(0);
+        }
+    }
+}
+
 // Phase 2, one workgroup per (patch, group of sg slabs): right-hand sides of
 // the patch problems of every slab and time mode in the group
 // (vec_f_2d.hpp:27-78, vec_g_2d.hpp:30-95), in the Legendre time basis,
@@ -1539,11 +1788,27 @@ void estimator_alloc(System* sys)
         o2 += static_cast<int64_t>(snp[q]) * snp[q];
         o1 += snp[q];
         const int nc = P.cell_ptr[a + 1] - P.cell_ptr[a];
-        if (B.buckets.empty() || B.buckets.back().np != snp[q]) B.buckets.push_back({q, 0, snp[q], 0, 0});
+        if (B.buckets.empty() || B.buckets.back().np != snp[q]) B.buckets.push_back({q, 0, snp[q], 0, 0, false});
         B.buckets.back().count++;
         B.buckets.back().nc_max = std::max(B.buckets.back().nc_max, nc);
         B.buckets.back().blk_max = std::max({B.buckets.back().blk_max, P.n_flux[a], pn[a] - P.n_flux[a]});
     }
+    // buckets beyond the single-workgroup factor kernels (order > 128 with a
+    // diagonal block > 128) and within the 512 x 4 rows of the panel kernel
+    std::vector<int> bq;
+    std::vector<int64_t> boff;
+    int64_t bo = 0;
+    for (auto& bk : B.buckets) {
+        bk.big = bk.np > 128 && bk.blk_max > 128 && bk.np <= 2048;
+        if (!bk.big) continue;
+        for (int q = bk.q0; q < bk.q0 + bk.count; ++q) {
+            bq.push_back(q);
+            boff.push_back(bo);
+            bo += static_cast<int64_t>(snp[q]) * snp[q] + 2 * snp[q];
+        }
+        B.big_npmax = std::max(B.big_npmax, bk.np);
+    }
+    B.big_count = static_cast<int>(bq.size());
     B.s_n = upi(sn);
     B.s_np = upi(snp);
     auto up64 = [&](const std::vector<int64_t>& v) {
@@ -1553,6 +1818,11 @@ void estimator_alloc(System* sys)
     };
     B.s_off2 = up64(off2);
     B.s_off1 = up64(off1);
+    if (B.big_count) {
+        B.big_q = upi(bq);
+        B.big_off = up64(boff);
+        B.big_ws = dalloc<double>(sys, static_cast<size_t>(bo));
+    }
     B.nmax = P.n_patches ? snp[0] : 0;
     B.Kd = dalloc<double>(sys, static_cast<size_t>(o2));
     B.Ki = dalloc<double>(sys, static_cast<size_t>(o2));
@@ -1660,6 +1930,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             using FT = decltype(tag);
             FT* Ki = std::is_same<FT, float>::value ? reinterpret_cast<FT*>(B.Ki32) : reinterpret_cast<FT*>(B.Ki);
             for (const auto& bk : B.buckets) {
+                if (bk.big) continue;
                 // one register-resident Gauss-Jordan where the whole matrix fits the register file,
                 // else block elimination with two of them
                 if constexpr (std::is_same<FT, float>::value) {
@@ -1700,6 +1971,26 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                 set_lds(reinterpret_cast<const void*>(&k_patch_factor<FT>), lf);
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor<FT>), dim3(bk.count), dim3(PATCH_THREADS), lf, st, P,
                                    Tb, topo, ps, bk.q0, in_lds, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
+            }
+            if (B.big_count) {
+                // large patches: blocked Gauss-Jordan, one panel per pair of launches
+                const BigPatches bp{B.big_q, B.big_off, B.big_ws};
+                const int G = B.big_count, npm = B.big_npmax;
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_prepare<FT>), dim3(G), dim3(PATCH_THREADS), 0, st, P, Tb,
+                                   topo, ps, bp, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd);
+                const int nstrips = (npm + 63) / 64, rpt = (npm + 511) / 512;
+                for (int c0 = 0; c0 < npm; c0 += 16) {
+                    switch (rpt) {
+                        case 1: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<FT, 1>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
+                        case 2: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<FT, 2>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
+                        case 3: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<FT, 3>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
+                        default: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<FT, 4>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
+                    }
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_update<FT>), dim3(G * nstrips), dim3(256), 0, st, ps, bp,
+                                       c0, nstrips);
+                }
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_finish<FT>), dim3(G * (npm / 16)), dim3(256), 0, st, ps, bp,
+                                   npm / 16, Ki);
             }
             HIP_TRY(hipEventRecord(sys->evp[1], st));
             for (const auto& bk : B.buckets) {

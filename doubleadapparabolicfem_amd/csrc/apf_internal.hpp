@@ -160,9 +160,15 @@ struct EstimatorBuffers {
     struct Bucket {
         int q0, count, np, nc_max;
         int blk_max;  // largest flux block or Schur block order in the bucket
+        bool big;     // factored by the blocked multi-launch Gauss-Jordan
     };
     std::vector<Bucket> buckets;
     int nmax = 0;
+    // patches of the big buckets: slot, workspace offset (doubles), workspace
+    int* big_q = nullptr;
+    int64_t* big_off = nullptr;
+    double* big_ws = nullptr;
+    int big_count = 0, big_npmax = 0;
 };
 
 struct System {

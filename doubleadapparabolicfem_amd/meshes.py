@@ -209,3 +209,16 @@ def indicator_initial() -> Coefficients:
         inside = (x[:, 0] >= 0.25) & (x[:, 0] <= 0.75) & (x[:, 1] >= 0.25) & (x[:, 1] <= 0.75)
         return inside.astype(np.float64)
     return Coefficients(None, 0.0, u0)
+
+
+def fan_hierarchy(q: int):
+    """Nested coarser meshes of fan_mesh(q, ...) for the multigrid
+    preconditioner, fine to coarse: every fan triangle lies in the triangle
+    (two corners of its square side, centre) of criss_cross_mesh(q), which lies
+    in a half of union_jack_mesh(q); then union-jack halvings down to 2."""
+    out = [criss_cross_mesh(q, [0.0, 1.0])]
+    n = q
+    while n >= 2:
+        out.append(union_jack_mesh(n, [0.0, 1.0]))
+        n //= 2
+    return out

@@ -25,7 +25,7 @@ def test_fan_patches_match_oracle(ctx, name):
     fx = np.load(os.path.join(GOLDEN, name + ".npz"))
     p, q, seed, smin, smax, lmin, lmax = (int(v) for v in fx["params"])
     m = meshes.fan_mesh(q, meshes.graded_times(lmin, lmax), seed, smin, smax)
-    sp = apf.Space(m, p)
+    sp = apf.Space(m, p, coarse=meshes.fan_hierarchy(q))
     nf, ns_, nc, bd = sp.patch_sizes()
     order = nf + ns_ + (bd == 0)
     print(f"{name}: n={sp.n} patches={order.size} max cells={nc.max()} max order={order.max()}", flush=True)
