@@ -634,57 +634,59 @@ __global__ void __launch_bounds__(256)
                            const int* cells, const double* Mcell, double* Kd, float* Ki)
 {
     constexpr int NPM = 64 * RPL;  // largest padded order
-    __shared__ float A[NPM * NPM];
-    __shared__ float Rb[16 * NPM];
+    // LDS row strides: S = np + 4 (= 4 mod 16) spreads the MFMA tile accesses
+    // (rows 4 (l / 16) + r, columns l % 16) and the row-tile operand reads over
+    // distinct banks; RS = 16 mod 64 does the same for the pivot-row operand.
+    // The scaling vector lives in the pivot-row buffer while the matrix is
+    // scaled, then in the stride padding of each row (columns np..np+3, never
+    // read by the elimination).
+    __shared__ __align__(16) float A[NPM * (NPM + 4)];
+    __shared__ __align__(16) float Rb[16 * (NPM + 16)];
     __shared__ int pivrow[NPM], rinv[NPM];
-    __shared__ double dsc[NPM];
+    double* dsc = reinterpret_cast<double*>(Rb);
     const int q = q0 + blockIdx.x;
     const int a = topo.order[q];
     const int n = ps.n[q], np = ps.np[q];
+    const int S = np + 4, RS = np + ((16 - np) & 63);
     double* K = Kd + ps.off2[q];
     build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
-    // symmetric equilibration D K D, d_i = 1 / sqrt(max_j |K_ij|): the flux,
-    // divergence and mean rows differ in scale by powers of h, and an FP32
-    // Gauss-Jordan inverse of the scaled matrix is several times more accurate
-    // 8 loads in flight per thread
-    for (int e0 = threadIdx.x; e0 < np * np; e0 += 8 * blockDim.x) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * blockDim.x;
-            v[u] = (e < np * np) ? K[e] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * blockDim.x;
-            if (e < np * np) A[e] = static_cast<float>(fabs(v[u]));
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < np; i += blockDim.x) {
-        float mx = 0.0f;
-        for (int j = 0; j < np; ++j) mx = fmaxf(mx, A[i * np + j]);
-        dsc[i] = (mx > 0.0f) ? 1.0 / sqrt(static_cast<double>(mx)) : 1.0;
-        rinv[i] = i;
-    }
-    __syncthreads();
-    for (int e0 = threadIdx.x; e0 < np * np; e0 += 8 * blockDim.x) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * blockDim.x;
-            v[u] = (e < np * np) ? K[e] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * blockDim.x;
-            if (e < np * np) A[e] = static_cast<float>(dsc[e / np] * v[u] * dsc[e % np]);
-        }
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int nrt = np >> 4;
+    // K into LDS in FP32 (one row per wave step, lanes along the row) and the
+    // symmetric equilibration D K D, d_i = 1 / sqrt(max_j |K_ij|): the flux,
+    // divergence and mean rows differ in scale by powers of h, and an FP32
+    // Gauss-Jordan inverse of the scaled matrix is several times more accurate
+#pragma unroll 4
+    for (int i = wave; i < np; i += 4) {
+        float mx = 0.0f;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int j = lane + 64 * t;
+            if (j < np) {
+                const float v = static_cast<float>(K[static_cast<int64_t>(i) * np + j]);
+                A[i * S + j] = v;
+                mx = fmaxf(mx, fabsf(v));
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        if (lane == 0) {
+            dsc[i] = (mx > 0.0f) ? 1.0 / sqrt(static_cast<double>(mx)) : 1.0;
+            rinv[i] = i;
+        }
+    }
+    __syncthreads();
+    auto dkeep = [&](int i) -> double& { return *reinterpret_cast<double*>(&A[i * S + np]); };
+    for (int i = wave; i < np; i += 4) {
+        const double di = dsc[i];
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int j = lane + 64 * t;
+            if (j < np) A[i * S + j] = static_cast<float>(di * static_cast<double>(A[i * S + j]) * dsc[j]);
+        }
+        if (lane == 0) dkeep(i) = di;
+    }
+    __syncthreads();
     unsigned used = 0;  // wave 0: bit s = row lane + 64 s used as pivot (or >= n)
 #pragma unroll
     for (int s = 0; s < RPL; ++s)
@@ -692,57 +694,61 @@ __global__ void __launch_bounds__(256)
     for (int k0 = 0; k0 < n; k0 += 16) {
         const int kc = min(16, n - k0);
         if (wave == 0) {
+            // wave 0 eliminates the panel in registers: lane l owns rows l + 64 s
             float Pn[RPL][16];
 #pragma unroll
             for (int s = 0; s < RPL; ++s)
 #pragma unroll
-                for (int c = 0; c < 16; ++c) Pn[s][c] = A[(lane + 64 * s) * np + k0 + c];
-            for (int kk = 0; kk < kc; ++kk) {
+                for (int c = 0; c < 16; ++c) Pn[s][c] = A[(lane + 64 * s) * S + k0 + c];
+            // fully unrolled: every column index below is a compile-time constant
+#pragma clang loop unroll(full)
+            for (int kk = 0; kk < 16; ++kk) {
+                if (kk >= kc) continue;
                 // pivot: largest |A[i][k]| over unused rows, as one 32-bit key
                 // per candidate (|v| with its 7 low mantissa bits replaced by
                 // 127 - i: a pivot within 1 - 2^-16 of the largest is as good,
                 // ties go to the lowest row, a NaN wins) and one DPP max
                 unsigned key = 0u;
-                float cur[RPL];
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) {
-                    float v = 0.0f;
-#pragma unroll
-                    for (int c = 0; c < 16; ++c)
-                        if (c == kk) v = Pn[s][c];
-                    cur[s] = v;
-                    const unsigned kv = (__float_as_uint(fabsf(v)) & ~0x7Fu) | static_cast<unsigned>(127 - (lane + 64 * s));
+                    const unsigned kv =
+                        (__float_as_uint(fabsf(Pn[s][kk])) & ~0x7Fu) | static_cast<unsigned>(127 - (lane + 64 * s));
                     if (!(used & (1u << s))) key = max(key, kv);
                 }
                 key = wave_umax(key);
                 const int r = min(max(127 - static_cast<int>(key & 0x7Fu), 0), n - 1);
                 const int owner = r & 63, sr = r >> 6;
-                float pvl = 0.0f;
-#pragma unroll
-                for (int s = 0; s < RPL; ++s)
-                    if (s == sr) pvl = cur[s];
-                const float inv = 1.0f / __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pvl), owner));
+                // the pivot row, scaled by the reciprocal pivot (hardware
+                // reciprocal: the FP64 refinement absorbs its last-bit error)
                 float prow[16];
-#pragma unroll
-                for (int s = 0; s < RPL; ++s)
-                    if (s == sr) {
-#pragma unroll
-                        for (int c = 0; c < 16; ++c) {
-                            const float v = ((c == kk) ? 1.0f : Pn[s][c]) * inv;
-                            if (lane == owner) Pn[s][c] = v;
-                            prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), owner));
-                        }
-                    }
-#pragma unroll
-                for (int s = 0; s < RPL; ++s) {
-                    const int i = lane + 64 * s;
-                    if (i == r || i >= n) continue;
-                    float f = 0.0f;
+                if (RPL == 1 || sr == 0) {
 #pragma unroll
                     for (int c = 0; c < 16; ++c)
-                        if (c == kk) f = Pn[s][c];
+                        prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Pn[0][c]), owner));
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                        prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Pn[RPL - 1][c]), owner));
+                }
+                const float inv = __builtin_amdgcn_rcpf(prow[kk]);
+#pragma unroll
+                for (int c = 0; c < 16; ++c) prow[c] = (c == kk) ? inv : prow[c] * inv;
+                // every row eliminates (padding rows are zero and stay zero),
+                // then the owner lane puts the scaled pivot row in place
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) {
+                    const float f = Pn[s][kk];
 #pragma unroll
                     for (int c = 0; c < 16; ++c) Pn[s][c] = ((c == kk) ? 0.0f : Pn[s][c]) - f * prow[c];
+                }
+                if (lane == owner) {
+                    if (RPL == 1 || sr == 0) {
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) Pn[0][c] = prow[c];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) Pn[RPL - 1][c] = prow[c];
+                    }
                 }
                 if (lane == owner) used |= 1u << sr;
                 if (lane == 0) {
@@ -753,44 +759,61 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int s = 0; s < RPL; ++s)
 #pragma unroll
-                for (int c = 0; c < 16; ++c) A[(lane + 64 * s) * np + k0 + c] = Pn[s][c];
+                for (int c = 0; c < 16; ++c) A[(lane + 64 * s) * S + k0 + c] = Pn[s][c];
         }
         __syncthreads();
-        // pivot rows outside the panel, before the update
-        for (int e = threadIdx.x; e < 16 * np; e += blockDim.x) {
-            const int c = e / np, j = e % np;
-            Rb[e] = (c < kc && (j < k0 || j >= k0 + 16)) ? A[pivrow[k0 + c] * np + j] : 0.0f;
-        }
-        __syncthreads();
-        // A[:, T] += (P - I_R) Rb on MFMA; the panel's own column tile is left alone
-        const int pt = k0 >> 4;
-        for (int tile = wave; tile < nrt * nrt; tile += 4) {
-            const int rt = tile % nrt, ct = tile / nrt;
-            if (ct == pt) continue;
-            v4f acc;
+        // pivot rows outside the panel, before the update (4 panel rows per wave)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[r] = A[(rt * 16 + drow<float>(lk, r)) * np + ct * 16 + li];
+        for (int u = 0; u < 4; ++u) {
+            const int c = wave + 4 * u;
+            const int pr = (c < kc) ? pivrow[k0 + c] : 0;
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int j = lane + 64 * t;
+                if (j < np) Rb[c * RS + j] = (c < kc && (j < k0 || j >= k0 + 16)) ? A[pr * S + j] : 0.0f;
+            }
+        }
+        __syncthreads();
+        // A[:, T] += (P - I_R) Rb on MFMA; the panel's own column tile is left
+        // alone.  Each wave keeps the (P - I_R) operand of its row tile.
+        const int pt = k0 >> 4;
+        for (int rt = wave; rt < nrt; rt += 4) {
+            float av[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int c = 4 * t + lk, i = rt * 16 + li;
-                float av = (c < kc) ? A[i * np + k0 + c] : 0.0f;
-                if (c < kc && i == pivrow[k0 + c]) av -= 1.0f;
-                acc = mfma_16x16x4(av, Rb[c * np + ct * 16 + li], acc);
+                av[t] = (c < kc) ? A[i * S + k0 + c] - ((i == pivrow[k0 + c]) ? 1.0f : 0.0f) : 0.0f;
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) A[(rt * 16 + drow<float>(lk, r)) * np + ct * 16 + li] = acc[r];
+            for (int ct = 0; ct < NPM / 16; ++ct) {
+                if (ct >= nrt || ct == pt) continue;
+                v4f acc;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[r] = A[(rt * 16 + drow<float>(lk, r)) * S + ct * 16 + li];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc = mfma_16x16x4(av[t], Rb[(4 * t + lk) * RS + ct * 16 + li], acc);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) A[(rt * 16 + drow<float>(lk, r)) * S + ct * 16 + li] = acc[r];
+            }
         }
         __syncthreads();
     }
-    // K^-1 = D (D K D)^-1 D, with the pivot permutation folded in
+    // K^-1 = D (D K D)^-1 D, with the pivot permutation folded in; rows by
+    // wave, columns by lane
     float* Kg = Ki + ps.off2[q];
-    for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
-        const int i = e / np, j = e % np;
-        if (i < n && j < n) {
-            const int r = rinv[i], c = pivrow[j];
-            Kg[static_cast<int64_t>(r) * np + c] = static_cast<float>(dsc[r] * static_cast<double>(A[e]) * dsc[c]);
-        } else {
-            Kg[e] = 0.0f;
+    for (int i = wave; i < np; i += 4) {
+        const int r = (i < n) ? rinv[i] : 0;
+        const double dr = (i < n) ? dkeep(r) : 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int j = lane + 64 * t;
+            if (j >= np) continue;
+            if (i < n && j < n) {
+                const int c = pivrow[j];
+                Kg[static_cast<int64_t>(r) * np + c] = static_cast<float>(dr * static_cast<double>(A[i * S + j]) * dkeep(c));
+            } else {
+                Kg[static_cast<int64_t>(i) * np + j] = 0.0f;
+            }
         }
     }
 }
