@@ -1295,30 +1295,57 @@ __global__ void __launch_bounds__(PATCH_THREADS)
     }
 }
 
-constexpr int SOLVE_THREADS = 512;
+// Solve workgroups: 8 waves, or 16 when the column tiles split in two halves
+// (one wave per row tile and half), so that each wave fits 128 VGPRs and four
+// waves share a SIMD.
+__host__ __device__ constexpr int solve_threads(int CT) { return (CT % 2 == 0) ? 1024 : 512; }
 
-// acc[ct] += M(rows rt*16.., k) B(k, cols ct*16..) over k < np, M read as
-// M[k][i] (symmetric), B from LDS with row stride ldb.  The A operands of 16
-// k-steps are loaded together so one memory latency covers 16 * CT MFMAs.
+// acc[ct] += M(rows rt*16.., k) B(k, cols ct*16..) over k < np.  mrow points
+// at row rt*16 + l%16 of M.  The k order is permuted: at MFMA step t, lane
+// group g = l/16 supplies k = kb + 16 g + t for both operands, so each lane
+// reads 16 consecutive entries of its row as vector loads (one address, one
+// memory latency for 16 * CT MFMAs).  A group past np supplies zeros.  B
+// comes from LDS (or the workspace) with row stride ldb.
+__device__ inline void load16(const float* p, float (&v)[16])
+{
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const float4 w = reinterpret_cast<const float4*>(p)[u];
+        v[4 * u] = w.x;
+        v[4 * u + 1] = w.y;
+        v[4 * u + 2] = w.z;
+        v[4 * u + 3] = w.w;
+    }
+}
+__device__ inline void load16(const double* p, double (&v)[16])
+{
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const double2 w = reinterpret_cast<const double2*>(p)[u];
+        v[2 * u] = w.x;
+        v[2 * u + 1] = w.y;
+    }
+}
+
 template <int CT, typename MT, typename BT, typename V4>
-__device__ inline void tile_product(const MT* mp, int np, const BT* Bs, int ldb, int li, int lk, V4 (&acc)[CT],
+__device__ inline void tile_product(const MT* mrow, int np, const BT* Bs, int ldb, int li, int lk, V4 (&acc)[CT],
                                     bool negate)
 {
     for (int kb = 0; kb < np; kb += 64) {
+        const int k0 = kb + 16 * lk;
         MT av[16];
+        if (k0 < np) {
+            load16(mrow + k0, av);
+        } else {
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const int k = kb + 4 * t + lk;
-            av[t] = (kb + 4 * t < np) ? mp[static_cast<int64_t>(k) * np] : static_cast<MT>(0);
+            for (int t = 0; t < 16; ++t) av[t] = static_cast<MT>(0);
         }
+        const BT* brow = Bs + (k0 < np ? k0 : 0) * ldb + li;
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
-            if (kb + 4 * t >= np) break;
-            const int k = kb + 4 * t + lk;
             const MT a = negate ? -av[t] : av[t];
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
-                acc[ct] = mfma_16x16x4(a, static_cast<MT>(Bs[k * ldb + ct * 16 + li]), acc[ct]);
+            for (int ct = 0; ct < CT; ++ct) acc[ct] = mfma_16x16x4(a, static_cast<MT>(brow[t * ldb + ct * 16]), acc[ct]);
         }
     }
 }
@@ -1327,18 +1354,24 @@ __device__ inline void tile_product(const MT* mp, int np, const BT* Bs, int ldb,
 // slabs): X = Ki B on MFMA in FT, then refine_steps rounds of R = B - Kd X
 // (FP64 MFMA) and X += Ki R (FT); the residual of the final X gives the
 // acceptance statistic |K x - b|_inf / max(1, |b|_inf) per (slab, vertex)
-// problem (dense_local_error_blocks_2d.hpp:923-949).  Each wave owns row tiles
-// rt = wave + 8 j.  The flux coefficients go to sig.  X and R live in LDS, or,
-// with GWS (patches whose tiles exceed the LDS), in this workgroup's slice of
-// the global workspace ws (np x CW doubles, then np x CW FT).
+// problem (dense_local_error_blocks_2d.hpp:923-949).  A wave item is one row
+// tile and one group of CT / CS column tiles (CS = 2 with 16 waves for even CT).
+// The flux coefficients go to sig.  X and R live in LDS, or, with GWS
+// (patches whose tiles exceed the LDS), in this workgroup's slice of the
+// global workspace ws (np x CW doubles, then np x LR FT).
 template <typename FT, int CT, bool GWS>
-__global__ void __launch_bounds__(SOLVE_THREADS)
+__global__ void __launch_bounds__(solve_threads(CT))
     k_patch_solve(EstParams P, PatchTopo topo, PatchSlots ps, int q0, int count, int ncg, int cpw, const double* Kd,
                   const FT* Ki, const double* RHS, double* sig, double* patch_res, int* extra_rounds, double* ws,
                   int64_t ws_stride)
 {
     using V4 = typename Vec4<FT>::type;
     constexpr int CW = 16 * CT;
+    constexpr int CS = (CT % 2 == 0) ? 2 : 1;  // column-tile groups per row tile, one wave each
+    constexpr int CTW = CT / CS;
+    // FP32 residual tile row stride = 2 mod 4 words: the B-operand reads of
+    // lane groups 0 and 1 (rows 16 apart) then fall in different bank halves
+    constexpr int LR = std::is_same<FT, float>::value ? CW + 2 : CW;
     extern __shared__ double lds[];
     __shared__ unsigned long long colmax[2 * CW];  // [0, CW): |R| per column, [CW, 2 CW): |b| per column
     const int Nf = P.Nf, Nt = P.Nt;
@@ -1358,7 +1391,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
     const double* Bq = RHS + ps.off1[q] * ncols + c0;
     double* Xs = lds;  // [np][CW]
     if constexpr (GWS) Xs = ws + (static_cast<int64_t>(qb) * ncg + slot % ncg) * ws_stride;
-    FT* Rs = reinterpret_cast<FT*>(Xs + static_cast<int64_t>(np) * CW);  // [np][CW]
+    FT* Rs = reinterpret_cast<FT*>(Xs + static_cast<int64_t>(np) * CW);  // [np][LR]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int nrt = np >> 4;
@@ -1373,20 +1406,22 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = e0 + u * blockDim.x;
-            if (e < np * CW) Rs[e] = static_cast<FT>(v[u]);
+            if (e < np * CW) Rs[(e / CW) * LR + e % CW] = static_cast<FT>(v[u]);
         }
     }
     __syncthreads();
-    // X = Ki B
-    for (int rt = wave; rt < nrt; rt += nw) {
-        V4 acc[CT];
+    // X = Ki B; wave item w = (row tile, column-tile group)
+    for (int w = wave; w < nrt * CS; w += nw) {
+        const int rt = w / CS, cg = (w % CS) * CTW;
+        V4 acc[CTW];
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct) acc[ct] = V4{0, 0, 0, 0};
-        tile_product<CT>(Iq + rt * 16 + li, np, Rs, CW, li, lk, acc, false);
+        for (int ct = 0; ct < CTW; ++ct) acc[ct] = V4{0, 0, 0, 0};
+        tile_product<CTW>(Iq + static_cast<int64_t>(rt * 16 + li) * np, np, Rs + cg * 16, LR, li, lk, acc, false);
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
+        for (int ct = 0; ct < CTW; ++ct)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Xs[(rt * 16 + drow<FT>(lk, r)) * CW + ct * 16 + li] = static_cast<double>(acc[ct][r]);
+            for (int r = 0; r < 4; ++r)
+                Xs[(rt * 16 + drow<FT>(lk, r)) * CW + (cg + ct) * 16 + li] = static_cast<double>(acc[ct][r]);
     }
     __syncthreads();
     // refine_steps rounds, then more (at most MAX_EXTRA) while the acceptance
@@ -1403,12 +1438,13 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
             __syncthreads();
         }
         // R = B - Kd X in FP64
-        for (int rt = wave; rt < nrt; rt += nw) {
-            v4d acc[CT];
-            double bm[CT];
+        for (int w = wave; w < nrt * CS; w += nw) {
+            const int rt = w / CS, cg = (w % CS) * CTW;
+            v4d acc[CTW];
+            double bm[CTW];
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-                const int col = ct * 16 + li;
+            for (int ct = 0; ct < CTW; ++ct) {
+                const int col = (cg + ct) * 16 + li;
                 bm[ct] = 0.0;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -1417,10 +1453,10 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
                     bm[ct] = fmax(bm[ct], fabs(acc[ct][r]));
                 }
             }
-            tile_product<CT>(Kq + rt * 16 + li, np, Xs, CW, li, lk, acc, true);
+            tile_product<CTW>(Kq + static_cast<int64_t>(rt * 16 + li) * np, np, Xs + cg * 16, CW, li, lk, acc, true);
             if (check) {
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct) {
+                for (int ct = 0; ct < CTW; ++ct) {
                     double rm = 0.0, bb = bm[ct];
 #pragma unroll
                     for (int r = 0; r < 4; ++r) rm = fmax(rm, fabs(acc[ct][r]));
@@ -1430,17 +1466,18 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
                     }
                     if (lk == 0) {
                         // non-negative doubles order like their bit patterns
-                        atomicMax(&colmax[ct * 16 + li], static_cast<unsigned long long>(__double_as_longlong(rm)));
-                        atomicMax(&colmax[CW + ct * 16 + li],
+                        atomicMax(&colmax[(cg + ct) * 16 + li],
+                                  static_cast<unsigned long long>(__double_as_longlong(rm)));
+                        atomicMax(&colmax[CW + (cg + ct) * 16 + li],
                                   static_cast<unsigned long long>(__double_as_longlong(bb)));
                     }
                 }
             }
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
+            for (int ct = 0; ct < CTW; ++ct)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    Rs[(rt * 16 + drow<double>(lk, r)) * CW + ct * 16 + li] = static_cast<FT>(acc[ct][r]);
+                    Rs[(rt * 16 + drow<double>(lk, r)) * LR + (cg + ct) * 16 + li] = static_cast<FT>(acc[ct][r]);
         }
         __syncthreads();
         if (check) {
@@ -1463,16 +1500,17 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
             ++extra;
         }
         // X += Ki R in FT
-        for (int rt = wave; rt < nrt; rt += nw) {
-            V4 acc[CT];
+        for (int w = wave; w < nrt * CS; w += nw) {
+            const int rt = w / CS, cg = (w % CS) * CTW;
+            V4 acc[CTW];
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct) acc[ct] = V4{0, 0, 0, 0};
-            tile_product<CT>(Iq + rt * 16 + li, np, Rs, CW, li, lk, acc, false);
+            for (int ct = 0; ct < CTW; ++ct) acc[ct] = V4{0, 0, 0, 0};
+            tile_product<CTW>(Iq + static_cast<int64_t>(rt * 16 + li) * np, np, Rs + cg * 16, LR, li, lk, acc, false);
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
+            for (int ct = 0; ct < CTW; ++ct)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    Xs[(rt * 16 + drow<FT>(lk, r)) * CW + ct * 16 + li] += static_cast<double>(acc[ct][r]);
+                    Xs[(rt * 16 + drow<FT>(lk, r)) * CW + (cg + ct) * 16 + li] += static_cast<double>(acc[ct][r]);
         }
         __syncthreads();
     }
@@ -1730,7 +1768,11 @@ struct SolveShape {
 SolveShape solve_shape(int np, int ncols, int Nt, size_t ft_bytes)
 {
     const size_t cap = 160 * 1024 - 4096;
-    auto tiles = [&](int ct) { return static_cast<size_t>(np) * 16 * ct * (sizeof(double) + ft_bytes); };
+    // X tile [np][16 ct] doubles, R tile [np][LR] (LR = 16 ct + 2 in FP32, 16 ct in FP64)
+    auto tiles = [&](int ct) {
+        const size_t cw = 16 * ct, lr = (ft_bytes == sizeof(float)) ? cw + 2 : cw;
+        return static_cast<size_t>(np) * (cw * sizeof(double) + lr * ft_bytes);
+    };
     SolveShape sh{};
     int CT = 6;
     if (const char* e = std::getenv("APF_SOLVE_CT_MAX")) CT = std::min(CT, std::max(1, std::atoi(e)));
@@ -2049,7 +2091,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                     throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
                 auto solve = [&](auto kern) {
                     if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
-                    hipLaunchKernelGGL(kern, dim3(grid), dim3(SOLVE_THREADS), sh.lds_bytes, st, P, topo, ps, bk.q0,
+                    hipLaunchKernelGGL(kern, dim3(grid), dim3(solve_threads(sh.CT)), sh.lds_bytes, st, P, topo, ps, bk.q0,
                                        bk.count, sh.ncg, sh.cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res, B.extra_rounds,
                                        B.solve_ws, sh.ws_stride);
                 };
