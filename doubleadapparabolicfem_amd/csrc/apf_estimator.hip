@@ -273,15 +273,18 @@ constexpr int PATCH_THREADS = 256;
 
 // FP64 patch matrix of one time mode (flux mass, divergence coupling,
 // mean-zero row; dense_local_error_blocks_2d.hpp:719-768, 812-879) of patch a
-// into K (np x np, zero padded, exactly symmetric).  Ends with a barrier.
+// into K (np x np, zero padded, exactly symmetric).  Only the entries of the
+// patch's pattern are written: the slot was zeroed when it was allocated and
+// no other entry is ever written.  With Alds, the same contributions also go to
+// an FP32 copy in LDS (row stride S).  Ends with a barrier.
 __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const PatchTopo& topo, int a, int n, int np,
-                                   const double* verts, const int* cells, const double* Mcell, double* K)
+                                   const double* verts, const int* cells, const double* Mcell, double* K,
+                                   float* Alds = nullptr, int S = 0)
 {
     const int Nf = P.Nf, Nq = P.Nq, dim = P.dim;
     const int nF = topo.nflux[a];
     const bool interior = topo.boundary[a] == 0;
     const int cb = topo.cell_ptr[a], ce = topo.cell_ptr[a + 1];
-    const int64_t nn = static_cast<int64_t>(np) * np;
     // cell metrics once per cell (patches of up to 64 cells; larger ones recompute)
     constexpr int GMAX = 64;
     __shared__ double gmH[GMAX][5];
@@ -291,7 +294,18 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
         for (int u = 0; u < 4; ++u) gmH[ci][u] = g.H[u];
         gmH[ci][4] = g.det;
     }
-    for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) K[e] = 0.0;
+    // zero the flux-flux entries the atomic sums below accumulate into
+    for (int e = threadIdx.x; e < nc * Nf * Nf; e += blockDim.x) {
+        const int ci = cb + e / (Nf * Nf), w = e % (Nf * Nf);
+        const int i = w / Nf, j = w % Nf;
+        if (j < i) continue;
+        const int fi = topo.fmap[static_cast<int64_t>(ci) * Nf + i], fj = topo.fmap[static_cast<int64_t>(ci) * Nf + j];
+        if (fi < 0 || fj < 0) continue;
+        K[static_cast<int64_t>(fi) * np + fj] = 0.0;
+        K[static_cast<int64_t>(fj) * np + fi] = 0.0;
+    }
+    if (Alds)
+        for (int e = threadIdx.x; e < np * S; e += blockDim.x) Alds[e] = 0.0f;
     __syncthreads();
     // every cell at once: a flux-flux entry gets at most two contributions (the
     // cells of an edge), so the FP64 atomic sums are exact-order independent
@@ -319,6 +333,10 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
             v *= static_cast<double>(fs[i] * fs[j]);
             atomicAdd(&K[static_cast<int64_t>(fi) * np + fj], v);
             if (i != j) atomicAdd(&K[static_cast<int64_t>(fj) * np + fi], v);
+            if (Alds) {
+                atomicAdd(&Alds[fi * S + fj], static_cast<float>(v));
+                if (i != j) atomicAdd(&Alds[fj * S + fi], static_cast<float>(v));
+            }
         } else if (w < Nf * Nf + Nq * Nf) {
             // divergence coupling int q_i div s_j (orthonormal scalars)
             const int w2 = w - Nf * Nf;
@@ -329,6 +347,10 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
             const double v = fs[j] * T.Bhat[qq * Nf + j];
             K[static_cast<int64_t>(r0 + qq) * np + fj] = v;
             K[static_cast<int64_t>(fj) * np + r0 + qq] = v;
+            if (Alds) {
+                Alds[(r0 + qq) * S + fj] = static_cast<float>(v);
+                Alds[fj * S + r0 + qq] = static_cast<float>(v);
+            }
         } else if (interior) {
             // mean-zero row
             const int qq = w - Nf * Nf - Nq * Nf;
@@ -337,6 +359,10 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
             const double v = det * T.qbar[qq];
             K[static_cast<int64_t>(r0 + qq) * np + (n - 1)] = v;
             K[static_cast<int64_t>(n - 1) * np + r0 + qq] = v;
+            if (Alds) {
+                Alds[(r0 + qq) * S + (n - 1)] = static_cast<float>(v);
+                Alds[(n - 1) * S + r0 + qq] = static_cast<float>(v);
+            }
         }
     }
     __syncthreads();
@@ -649,25 +675,20 @@ __global__ void __launch_bounds__(256)
     const int n = ps.n[q], np = ps.np[q];
     const int S = np + 4, RS = np + ((16 - np) & 63);
     double* K = Kd + ps.off2[q];
-    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K);
+    // the FP64 matrix into its slot and its FP32 copy into LDS in one pass
+    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K, A, S);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int nrt = np >> 4;
-    // K into LDS in FP32 (one row per wave step, lanes along the row) and the
     // symmetric equilibration D K D, d_i = 1 / sqrt(max_j |K_ij|): the flux,
     // divergence and mean rows differ in scale by powers of h, and an FP32
     // Gauss-Jordan inverse of the scaled matrix is several times more accurate
-#pragma unroll 4
     for (int i = wave; i < np; i += 4) {
         float mx = 0.0f;
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const int j = lane + 64 * t;
-            if (j < np) {
-                const float v = static_cast<float>(K[static_cast<int64_t>(i) * np + j]);
-                A[i * S + j] = v;
-                mx = fmaxf(mx, fabsf(v));
-            }
+            if (j < np) mx = fmaxf(mx, fabsf(A[i * S + j]));
         }
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
         if (lane == 0) {
