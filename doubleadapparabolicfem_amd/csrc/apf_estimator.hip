@@ -327,9 +327,11 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
                 const Metric2 g = metric_of(dim, topo.cells[ci], verts, cells, Mcell);
                 for (int u = 0; u < 4; ++u) H[u] = g.H[u];
             }
+            // (ab / dim) * 2 + ab % dim = ab for dim = 2, and dim = 1 has ab = 0 only
             double v = 0.0;
-            for (int ab = 0; ab < dim * dim; ++ab)
-                v += H[(ab / dim) * 2 + (ab % dim)] * T.RTab[(static_cast<int64_t>(ab) * Nf + i) * Nf + j];
+#pragma unroll
+            for (int ab = 0; ab < 4; ++ab)
+                if (ab < dim * dim) v += H[ab] * T.RTab[(static_cast<int64_t>(ab) * Nf + i) * Nf + j];
             v *= static_cast<double>(fs[i] * fs[j]);
             atomicAdd(&K[static_cast<int64_t>(fi) * np + fj], v);
             if (i != j) atomicAdd(&K[static_cast<int64_t>(fj) * np + fi], v);
