@@ -1802,6 +1802,9 @@ SolveShape solve_shape(int np, int ncols, int Nt, size_t ft_bytes)
     sh.gws = tiles(1) > cap;
     if (!sh.gws)
         while (CT > 1 && tiles(CT) > cap) --CT;
+    // workspace tiles: one column tile per workgroup, so that the few large
+    // patches spread over as many CUs as they have column tiles
+    if (sh.gws) CT = 1;
     CT = std::min(CT, (ncols + 15) / 16);
     sh.CT = CT;
     sh.cpw = std::max(Nt, (16 * CT / Nt) * Nt);
