@@ -1205,8 +1205,10 @@ __global__ void __launch_bounds__(256) k_big_finish(PatchSlots ps, BigPatches bp
 // col = slab * NT + kk.  The time projections Theta W and U W' of every
 // (cell, slab) are formed once while staging; flux rows shared by two cells
 // accumulate with LDS atomics; rows n..np-1 stay zero.
+constexpr int RHS_THREADS = 256;
+
 template <int DIM, int PP>
-__global__ void __launch_bounds__(PATCH_THREADS)
+__global__ void __launch_bounds__(RHS_THREADS)
     k_patch_rhs(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, int sg, int nsg, int nc_max,
                 const double* verts, const int* cells, const double* Mcell, const int* cell_free, const double* times,
                 const double* lt, const double* x, double* RHS)
@@ -1215,6 +1217,8 @@ __global__ void __launch_bounds__(PATCH_THREADS)
     constexpr int NS = L::NS, NT = L::NT, NF = L::NF, NQ = L::NQ, NV = L::NV;
     __shared__ double Rt[NV * NF * NS], Qt[NV * NQ * NS], Et[DIM * NQ * NS], Pt[NV * NQ];
     __shared__ double Wt[NT * NT], Wdt[NT * NT], Ln[NT];
+    constexpr int RMAX = 32;  // slabs per workgroup with staged step sizes
+    __shared__ double s_h[RMAX], s_scale[RMAX * NT];
     extern __shared__ double lds[];
     const int q = q0 + blockIdx.x / nsg;
     const int k0 = (blockIdx.x % nsg) * sg;
@@ -1237,6 +1241,13 @@ __global__ void __launch_bounds__(PATCH_THREADS)
         Wdt[e] = T.Wd[e];
     }
     for (int e = threadIdx.x; e < NT; e += blockDim.x) Ln[e] = T.Lnorm[e];
+    // step size and mode scale 1 / (h |L_kk|^2) per slab of the group
+    for (int e = threadIdx.x; e < nsl * NT; e += blockDim.x) {
+        const int sl = e / NT, kk = e - sl * NT;
+        const double h = times[k0 + sl + 1] - times[k0 + sl];
+        if (kk == 0) s_h[sl] = h;
+        s_scale[e] = 1.0 / (h * T.Lnorm[kk]);
+    }
     for (int e = threadIdx.x; e < np * wcol; e += blockDim.x) acc[e] = 0.0;
     for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) {
         const int c = topo.cells[cb + ci];
@@ -1274,39 +1285,54 @@ __global__ void __launch_bounds__(PATCH_THREADS)
     }
     __syncthreads();
     constexpr int ROWS = (NF + NQ) * NT;
-    for (int e = threadIdx.x; e < nc * nsl * ROWS; e += blockDim.x) {
-        const int ci = e / (nsl * ROWS), r = e % (nsl * ROWS);
-        const int sl = r / ROWS, r2 = r % ROWS;
-        const int row = r2 / NT, kk = r2 % NT;
-        const int v = topo.lv[cb + ci];
-        const int k = k0 + sl;
-        const double h = times[k + 1] - times[k];
-        const double scale = 1.0 / (h * Ln[kk]);
-        const double* st = stage + (static_cast<int64_t>(ci) * sg + sl) * NS * NT * 2;
-        if (row < NF) {
-            const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * NF + row];
-            if (fi < 0) continue;
-            // F = -h R_v (Theta W)
-            double sum = 0.0;
+    // thread t owns local (row, mode) rk = t % 64 (rows = flux dofs, then
+    // scalar dofs) for every (cell, slab) pair t / 64 + (waves) j: its indices
+    // and table rows are fixed, the pair advances without divisions
+    static_assert(ROWS <= 64 * 4, "local rows x modes: at most four rows per lane");
+    for (int rk = threadIdx.x % 64; rk < ROWS; rk += 64) {
+        const int row = rk / NT, kk = rk - (rk / NT) * NT;
+        int ci = 0, sl = threadIdx.x / 64;
+        while (sl >= nsl && ci < nc) {
+            sl -= nsl;
+            ++ci;
+        }
+        for (; ci < nc;) {
+            const int v = topo.lv[cb + ci];
+            const double h = s_h[sl];
+            const double scale = s_scale[sl * NT + kk];
+            const double* st = stage + (static_cast<int64_t>(ci) * sg + sl) * NS * NT * 2;
+            if (row < NF) {
+                const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * NF + row];
+                if (fi >= 0) {
+                    // F = -h R_v (Theta W)
+                    double sum = 0.0;
 #pragma unroll
-            for (int mm = 0; mm < NS; ++mm) sum += Rt[(v * NF + row) * NS + mm] * st[(mm * NT + kk) * 2];
-            const int8_t sgn = topo.fsign[static_cast<int64_t>(cb + ci) * NF + row];
-            atomicAdd(&acc[fi * wcol + sl * NT + kk], -h * sgn * sum * scale);
-        } else {
-            const int qq = row - NF;
-            const double det = geo[ci * 4 + 0];
-            double sum = (kk == 0) ? P.ell * det * h * Pt[v * NQ + qq] : 0.0;
-            double du = 0.0, dth = 0.0;
+                    for (int mm = 0; mm < NS; ++mm) sum += Rt[(v * NF + row) * NS + mm] * st[(mm * NT + kk) * 2];
+                    const int8_t sgn = topo.fsign[static_cast<int64_t>(cb + ci) * NF + row];
+                    atomicAdd(&acc[fi * wcol + sl * NT + kk], -h * sgn * sum * scale);
+                }
+            } else {
+                const int qq = row - NF;
+                const double det = geo[ci * 4 + 0];
+                double sum = (kk == 0) ? P.ell * det * h * Pt[v * NQ + qq] : 0.0;
+                double du = 0.0, dth = 0.0;
 #pragma unroll
-            for (int mm = 0; mm < NS; ++mm) {
-                du += Qt[(v * NQ + qq) * NS + mm] * st[(mm * NT + kk) * 2 + 1];
-                double eb = 0.0;
+                for (int mm = 0; mm < NS; ++mm) {
+                    du += Qt[(v * NQ + qq) * NS + mm] * st[(mm * NT + kk) * 2 + 1];
+                    double eb = 0.0;
 #pragma unroll
-                for (int b = 0; b < DIM; ++b) eb += geo[ci * 4 + 1 + b] * Et[(b * NQ + qq) * NS + mm];
-                dth += eb * st[(mm * NT + kk) * 2];
+                    for (int b = 0; b < DIM; ++b) eb += geo[ci * 4 + 1 + b] * Et[(b * NQ + qq) * NS + mm];
+                    dth += eb * st[(mm * NT + kk) * 2];
+                }
+                sum += -det * du - h * dth;
+                acc[(nF + ci * NQ + qq) * wcol + sl * NT + kk] = sum * scale;
             }
-            sum += -det * du - h * dth;
-            acc[(nF + ci * NQ + qq) * wcol + sl * NT + kk] = sum * scale;
+            // next (cell, slab) pair of this thread: one per wave further on
+            sl += blockDim.x / 64;
+            while (sl >= nsl && ci < nc) {
+                sl -= nsl;
+                ++ci;
+            }
         }
     }
     __syncthreads();
@@ -2095,7 +2121,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                 const int nsg = (P.m + sg - 1) / sg;
                 auto rhs = [&](auto kern) {
                     set_lds(reinterpret_cast<const void*>(kern), lr(sg));
-                    hipLaunchKernelGGL(kern, dim3(bk.count * nsg), dim3(PATCH_THREADS), lr(sg), st, P, Tb, topo, ps,
+                    hipLaunchKernelGGL(kern, dim3(bk.count * nsg), dim3(RHS_THREADS), lr(sg), st, P, Tb, topo, ps,
                                        bk.q0, sg, nsg, bk.nc_max, sys->d_verts, sys->d_cells, sys->d_Mcell,
                                        sys->d_cell_free, sys->d_times, B.lt, sys->d_x, B.RHS);
                 };
