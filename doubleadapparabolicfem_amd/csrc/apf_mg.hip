@@ -19,31 +19,28 @@ namespace apf {
 
 namespace {
 
-// Y = (K + s_c M) X   (mode 0)   or   Y = B - (K + s_c M) X   (mode 1), columns [0, nc).
-// One wavefront per row; the row's entries are loaded once into registers
-// (chunks of 32) and reused for every column pass.
-template <typename R>
-__global__ void __launch_bounds__(256) k_mg_apply(int nf, const int* __restrict__ ptr, const int* __restrict__ col,
-                                                  const R* __restrict__ K, const R* __restrict__ M,
-                                                  const R* __restrict__ sig, int nc, int ld, const R* __restrict__ X,
-                                                  const R* __restrict__ B, R* __restrict__ Y, int mode)
+// One wavefront per row of (K + s_c M) over columns [0, nc): the row's entries
+// are staged in registers, 64 at a time (one per lane), and broadcast to every
+// column pass; lanes run along the columns.  row_apply hands the product of
+// each column to the epilogue f(c, value).
+constexpr int MG_NP = 4;  // column passes of 64 handled together
+
+template <typename R, typename F>
+__device__ inline void row_apply(int row, const int* __restrict__ ptr, const int* __restrict__ col,
+                                 const R* __restrict__ K, const R* __restrict__ M, const R* __restrict__ sig, int nc,
+                                 int ld, const R* __restrict__ X, int lane, F f)
 {
-    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
-    const int lane = threadIdx.x % 64;
-    if (row >= nf) return;
     const int b = ptr[row], e = ptr[row + 1];
-    constexpr int NP = 4;  // column passes of 64 handled together
-    for (int cg = 0; cg < nc; cg += NP * 64) {
-        R acc[NP];
-        R sg[NP];
+    for (int cg = 0; cg < nc; cg += MG_NP * 64) {
+        R acc[MG_NP];
+        R sg[MG_NP];
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
+        for (int q = 0; q < MG_NP; ++q) {
             acc[q] = 0;
             const int c = cg + q * 64 + lane;
             sg[q] = (c < nc) ? sig[c] : R(0);
         }
         for (int e0 = b; e0 < e; e0 += 64) {
-            // stage up to 64 entries of the row in registers, one per lane
             const int my = e0 + lane;
             const int cj = (my < e) ? col[my] : 0;
             const R kv = (my < e) ? K[my] : R(0);
@@ -55,33 +52,78 @@ __global__ void __launch_bounds__(256) k_mg_apply(int nf, const int* __restrict_
                 const R mm = __shfl(mv, t, 64);
                 const R* xr = X + static_cast<int64_t>(j) * ld;
 #pragma unroll
-                for (int q = 0; q < NP; ++q) {
+                for (int q = 0; q < MG_NP; ++q) {
                     const int c = cg + q * 64 + lane;
                     if (c < nc) acc[q] += (kk + sg[q] * mm) * xr[c];
                 }
             }
         }
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
+        for (int q = 0; q < MG_NP; ++q) {
             const int c = cg + q * 64 + lane;
-            if (c < nc) {
-                const int64_t o = static_cast<int64_t>(row) * ld + c;
-                Y[o] = mode ? B[o] - acc[q] : acc[q];
-            }
+            if (c < nc) f(c, acc[q]);
         }
     }
 }
 
-// Chebyshev start: r = B - A x (given as r), d = D^-1 r / theta, x = x0 (+ nothing)
+// Y = (K + s_c M) X   (mode 0)   or   Y = B - (K + s_c M) X   (mode 1), columns [0, nc).
 template <typename R>
-__global__ void k_cheb_start(int nf, int nc, int ld, const R* __restrict__ dK, const R* __restrict__ dM,
-                             const R* __restrict__ sig, R inv_theta, const R* __restrict__ r, R* __restrict__ d)
+__global__ void __launch_bounds__(256) k_mg_apply(int nf, const int* __restrict__ ptr, const int* __restrict__ col,
+                                                  const R* __restrict__ K, const R* __restrict__ M,
+                                                  const R* __restrict__ sig, int nc, int ld, const R* __restrict__ X,
+                                                  const R* __restrict__ B, R* __restrict__ Y, int mode)
 {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= static_cast<int64_t>(nf) * nc) return;
-    const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
-    const int64_t o = static_cast<int64_t>(i) * ld + c;
-    d[o] = r[o] * inv_theta / (dK[i] + sig[c] * dM[i]);
+    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+    const int lane = threadIdx.x % 64;
+    if (row >= nf) return;
+    row_apply(row, ptr, col, K, M, sig, nc, ld, X, lane, [&](int c, R v) {
+        const int64_t o = static_cast<int64_t>(row) * ld + c;
+        Y[o] = mode ? B[o] - v : v;
+    });
+}
+
+// Chebyshev start from x: r = B - A x, d = D^-1 r / theta (one pass)
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_mg_resid_start(int nf, const int* __restrict__ ptr, const int* __restrict__ col, const R* __restrict__ K,
+                     const R* __restrict__ M, const R* __restrict__ sig, int nc, int ld, const R* __restrict__ X,
+                     const R* __restrict__ B, R* __restrict__ r, R* __restrict__ d, const R* __restrict__ dK,
+                     const R* __restrict__ dM, R inv_theta)
+{
+    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+    const int lane = threadIdx.x % 64;
+    if (row >= nf) return;
+    const R dk = dK[row], dm = dM[row];
+    row_apply(row, ptr, col, K, M, sig, nc, ld, X, lane, [&](int c, R v) {
+        const int64_t o = static_cast<int64_t>(row) * ld + c;
+        const R rv = B[o] - v;
+        r[o] = rv;
+        d[o] = rv * inv_theta / (dk + sig[c] * dm);
+    });
+}
+
+// One Chebyshev step with its product: x += d ; r -= A d ; dn = c1 d + c2 D^-1 r.
+// A d reads d of the neighbouring rows, so the new direction goes to a second
+// buffer dn (the caller swaps them).
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_mg_cheb_fused(int nf, const int* __restrict__ ptr, const int* __restrict__ col, const R* __restrict__ K,
+                    const R* __restrict__ M, const R* __restrict__ sig, int nc, int ld, const R* __restrict__ d,
+                    R* __restrict__ x, R* __restrict__ r, R* __restrict__ dn, const R* __restrict__ dK,
+                    const R* __restrict__ dM, R c1, R c2, int last)
+{
+    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+    const int lane = threadIdx.x % 64;
+    if (row >= nf) return;
+    const R dk = dK[row], dm = dM[row];
+    row_apply(row, ptr, col, K, M, sig, nc, ld, d, lane, [&](int c, R v) {
+        const int64_t o = static_cast<int64_t>(row) * ld + c;
+        const R dv = d[o];
+        x[o] += dv;
+        const R rv = r[o] - v;
+        r[o] = rv;
+        if (!last) dn[o] = c1 * dv + c2 * rv / (dk + sig[c] * dm);
+    });
 }
 
 // zero start: x = 0, r = B, d = D^-1 B / theta (no memset/memcpy nodes: graph safe)
@@ -109,23 +151,6 @@ __global__ void k_block_copy(int nf, int nc, int ld, const R* __restrict__ X, R*
     const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
     const int64_t o = static_cast<int64_t>(i) * ld + c;
     Y[o] = X[o];
-}
-
-// x += d ; r -= Ad ; d = c1 d + c2 D^-1 r
-template <typename R>
-__global__ void k_cheb_step(int nf, int nc, int ld, const R* __restrict__ dK, const R* __restrict__ dM,
-                            const R* __restrict__ sig, R c1, R c2, R* __restrict__ x, R* __restrict__ r,
-                            R* __restrict__ d, const R* __restrict__ Ad, int last)
-{
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= static_cast<int64_t>(nf) * nc) return;
-    const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
-    const int64_t o = static_cast<int64_t>(i) * ld + c;
-    const R dv = d[o];
-    x[o] += dv;
-    const R rv = r[o] - Ad[o];
-    r[o] = rv;
-    if (!last) d[o] = c1 * dv + c2 * rv / (dK[i] + sig[c] * dM[i]);
 }
 
 // Y = A X for a plain CSR (prolongation / restriction), optionally accumulated
@@ -395,19 +420,19 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start_zero<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc,
                            ld, d.dK, d.dM, sig, static_cast<R>(1) / theta, d.B, d.X, d.Rr, d.D);
     } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col, d.K,
-                           d.M, sig, nc, ld, d.X, d.B, d.Rr, 1);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc, ld,
-                           d.dK, d.dM, sig, static_cast<R>(1) / theta, d.Rr, d.D);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_resid_start<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col,
+                           d.K, d.M, sig, nc, ld, d.X, d.B, d.Rr, d.D, d.dK, d.dM, static_cast<R>(1) / theta);
     }
+    // the direction alternates between D and Tm
+    R* dcur = d.D;
+    R* dnext = d.Tm;
     R rho = 1 / sigma1;
     for (int k = 0; k < nu; ++k) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col, d.K,
-                           d.M, sig, nc, ld, d.D, d.D, d.Tm, 0);
         const R rho_n = 1 / (2 * sigma1 - rho);
         const R c1 = rho_n * rho, c2 = 2 * rho_n / delta;
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_step<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc, ld,
-                           d.dK, d.dM, sig, c1, c2, d.X, d.Rr, d.D, d.Tm, (k == nu - 1) ? 1 : 0);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_cheb_fused<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col,
+                           d.K, d.M, sig, nc, ld, dcur, d.X, d.Rr, dnext, d.dK, d.dM, c1, c2, (k == nu - 1) ? 1 : 0);
+        std::swap(dcur, dnext);
         rho = rho_n;
     }
 }

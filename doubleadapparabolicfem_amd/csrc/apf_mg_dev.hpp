@@ -26,7 +26,7 @@ struct MGDevice {
         R* B = nullptr;   // right-hand side block
         R* Rr = nullptr;  // residual
         R* D = nullptr;   // Chebyshev direction
-        R* Tm = nullptr;  // A D
+        R* Tm = nullptr;  // second Chebyshev direction (the fused step alternates D and Tm)
         int Pn = 0, Rn = 0;
         int* Pptr = nullptr;
         int* Pcol = nullptr;
