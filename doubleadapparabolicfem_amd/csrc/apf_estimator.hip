@@ -1232,6 +1232,10 @@ __global__ void __launch_bounds__(RHS_THREADS)
     double* acc = lds;                                                  // [np][sg * NT]
     double* stage = acc + static_cast<int64_t>(np) * wcol;             // [nc][sg][NS][NT][2]: Theta W, U W'
     double* geo = stage + static_cast<int64_t>(nc_max) * sg * NS * NT * 2;  // [nc][4]: det, gv
+    // per cell: local vertex, flux map and flux signs of the patch
+    int* c_lv = reinterpret_cast<int*>(geo + nc_max * 4);                 // [nc]
+    int* c_fmap = c_lv + nc_max;                                          // [nc][NF]
+    int* c_fsign = c_fmap + nc_max * NF;                                  // [nc][NF]
     for (int e = threadIdx.x; e < NV * NF * NS; e += blockDim.x) Rt[e] = T.R[e];
     for (int e = threadIdx.x; e < NV * NQ * NS; e += blockDim.x) Qt[e] = T.Q[e];
     for (int e = threadIdx.x; e < DIM * NQ * NS; e += blockDim.x) Et[e] = T.E[e];
@@ -1249,6 +1253,11 @@ __global__ void __launch_bounds__(RHS_THREADS)
         s_scale[e] = 1.0 / (h * T.Lnorm[kk]);
     }
     for (int e = threadIdx.x; e < np * wcol; e += blockDim.x) acc[e] = 0.0;
+    for (int e = threadIdx.x; e < nc * NF; e += blockDim.x) {
+        c_fmap[e] = topo.fmap[static_cast<int64_t>(cb) * NF + e];
+        c_fsign[e] = topo.fsign[static_cast<int64_t>(cb) * NF + e];
+    }
+    for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) c_lv[ci] = topo.lv[cb + ci];
     for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) {
         const int c = topo.cells[cb + ci];
         const int v = topo.lv[cb + ci];
@@ -1297,18 +1306,18 @@ __global__ void __launch_bounds__(RHS_THREADS)
             ++ci;
         }
         for (; ci < nc;) {
-            const int v = topo.lv[cb + ci];
+            const int v = c_lv[ci];
             const double h = s_h[sl];
             const double scale = s_scale[sl * NT + kk];
             const double* st = stage + (static_cast<int64_t>(ci) * sg + sl) * NS * NT * 2;
             if (row < NF) {
-                const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * NF + row];
+                const int fi = c_fmap[ci * NF + row];
                 if (fi >= 0) {
                     // F = -h R_v (Theta W)
                     double sum = 0.0;
 #pragma unroll
                     for (int mm = 0; mm < NS; ++mm) sum += Rt[(v * NF + row) * NS + mm] * st[(mm * NT + kk) * 2];
-                    const int8_t sgn = topo.fsign[static_cast<int64_t>(cb + ci) * NF + row];
+                    const int sgn = c_fsign[ci * NF + row];
                     atomicAdd(&acc[fi * wcol + sl * NT + kk], -h * sgn * sum * scale);
                 }
             } else {
@@ -2114,7 +2123,8 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                 int sg = std::min(8, P.m);
                 auto lr = [&](int g) {
                     return (static_cast<size_t>(bk.np) * g * P.Nt +
-                            static_cast<size_t>(bk.nc_max) * g * P.Ns * P.Nt * 2 + bk.nc_max * 4) * sizeof(double);
+                            static_cast<size_t>(bk.nc_max) * g * P.Ns * P.Nt * 2 + bk.nc_max * 4) * sizeof(double) +
+                           static_cast<size_t>(bk.nc_max) * (1 + 2 * P.Nf) * sizeof(int);
                 };
                 while (sg > 1 && lr(sg) > 64 * 1024) --sg;
                 if (lr(sg) > lds_cap) throw std::runtime_error("patch right-hand side block exceeds LDS");
