@@ -1577,15 +1577,30 @@ __global__ void __launch_bounds__(solve_threads(CT))
         if (extra) atomicAdd(extra_rounds, extra);
     }
     // flux coefficients per (spatial cell, local vertex, flux dof) and column:
-    // one (cell, flux dof) row per wave step, lanes along the contiguous columns
+    // one (cell, flux dof) row per wave step, lanes along the contiguous
+    // columns.  Lane l first fetches the map of the wave's l-th row, so the
+    // rows' topology loads are in flight together.
     const int cb = topo.cell_ptr[a], nc = topo.cell_ptr[a + 1] - cb;
-    for (int row = wave; row < nc * Nf; row += nw) {
-        const int ci = row / Nf, i = row - ci * Nf;
-        const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + i];
-        const double sg = topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i];
-        const int64_t sc = topo.cells[cb + ci];
-        double* out = sig + ((sc * P.NV + topo.lv[cb + ci]) * Nf + i) * ncols + c0;
-        for (int c = lane; c < ncw; c += 64) out[c] = (fi >= 0) ? sg * Xs[fi * CW + c] : 0.0;
+    const int nrows = nc * Nf;
+    for (int base = wave; base < nrows; base += nw * 64) {
+        const int my = base + nw * lane;
+        int my_fi = -1;
+        double my_sg = 0.0;
+        long long my_off = 0;
+        if (my < nrows) {
+            const int ci = my / Nf, i = my - ci * Nf;
+            my_fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + i];
+            my_sg = topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i];
+            const int64_t sc = topo.cells[cb + ci];
+            my_off = ((sc * P.NV + topo.lv[cb + ci]) * Nf + i) * ncols + c0;
+        }
+        const int cnt = min(64, (nrows - base + nw - 1) / nw);
+        for (int t = 0; t < cnt; ++t) {
+            const int fi = __shfl(my_fi, t, 64);
+            const double sg = __shfl(my_sg, t, 64);
+            double* out = sig + __shfl(my_off, t, 64);
+            for (int c = lane; c < ncw; c += 64) out[c] = (fi >= 0) ? sg * Xs[fi * CW + c] : 0.0;
+        }
     }
 }
 
