@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <future>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -334,12 +335,18 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
                 if (lv.scol[e] == i) { lv.dK[i] = lv.K[e]; lv.dM[i] = lv.M[e]; }
         for (int i = 0; i < lv.nf; ++i)
             if (!(lv.dK[i] > 0.0) || !(lv.dM[i] > 0.0)) throw std::runtime_error("setup_mg: non-positive diagonal");
-        lv.lmax = std::max(lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK),
-                           lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM));
         if (l + 1 < L) {
             lv.P = H.P[l];
             lv.R = transpose(H.P[l]);
         }
+    }
+    // smoother bounds of every level and matrix, on their own threads while
+    // the time factors and coarsest inverses are formed below
+    std::vector<std::future<double>> lm;
+    for (int l = 0; l < L; ++l) {
+        const MGLevelHost& lv = out.levels[l];
+        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK); }));
+        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM); }));
     }
     // time factors
     const int p = S.p, Nt = S.Nt, m = S.mesh.m();
@@ -465,6 +472,7 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
             if (!invert_dense(b, A)) throw std::runtime_error("setup_mg: singular coarsest operator");
             std::copy(A.begin(), A.end(), out.coarse_inv.begin() + static_cast<size_t>(q) * out.blk_total + out.blk_off[c]);
         }
+    for (int l = 0; l < L; ++l) out.levels[l].lmax = std::max(lm[2 * l].get(), lm[2 * l + 1].get());
 }
 
 }  // namespace apf
