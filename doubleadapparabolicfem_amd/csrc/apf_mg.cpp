@@ -312,6 +312,10 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
 {
     const int L = static_cast<int>(H.coarse.size()) + 1;
     out.levels.assign(L, MGLevelHost());
+    // smoother bounds of every level and matrix run on their own threads, each
+    // started as soon as its level exists (the Galerkin products of the coarser
+    // levels, the time factors and the coarsest inverses proceed meanwhile)
+    std::vector<std::future<double>> lm;
     for (int l = 0; l < L; ++l) {
         MGLevelHost& lv = out.levels[l];
         const Space& sp = (l == 0) ? S : H.coarse[l - 1];
@@ -335,18 +339,12 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
                 if (lv.scol[e] == i) { lv.dK[i] = lv.K[e]; lv.dM[i] = lv.M[e]; }
         for (int i = 0; i < lv.nf; ++i)
             if (!(lv.dK[i] > 0.0) || !(lv.dM[i] > 0.0)) throw std::runtime_error("setup_mg: non-positive diagonal");
+        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK); }));
+        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM); }));
         if (l + 1 < L) {
             lv.P = H.P[l];
             lv.R = transpose(H.P[l]);
         }
-    }
-    // smoother bounds of every level and matrix, on their own threads while
-    // the time factors and coarsest inverses are formed below
-    std::vector<std::future<double>> lm;
-    for (int l = 0; l < L; ++l) {
-        const MGLevelHost& lv = out.levels[l];
-        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK); }));
-        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM); }));
     }
     // time factors
     const int p = S.p, Nt = S.Nt, m = S.mesh.m();
