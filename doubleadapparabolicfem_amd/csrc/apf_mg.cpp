@@ -329,8 +329,11 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
             const MGLevelHost& f = out.levels[l - 1];
             const CSR& P = H.P[l - 1];
             const CSR R = transpose(P);
+            // the K and M products at once
+            auto gm = std::async(std::launch::async,
+                                 [&] { return galerkin(R, f.sptr, f.scol, f.M, P, lv.sptr, lv.scol); });
             lv.K = galerkin(R, f.sptr, f.scol, f.K, P, lv.sptr, lv.scol);
-            lv.M = galerkin(R, f.sptr, f.scol, f.M, P, lv.sptr, lv.scol);
+            lv.M = gm.get();
         }
         lv.dK.assign(lv.nf, 0.0);
         lv.dM.assign(lv.nf, 0.0);
