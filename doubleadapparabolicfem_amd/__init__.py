@@ -113,6 +113,7 @@ EXPORTED = {
     "apf_system_destroy": (None, [ctypes.c_void_p]),
     "apf_assemble_main": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Coeff), ctypes.c_int]),
     "apf_system_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_system_download_pattern": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "apf_system_set_solution": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_minres_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(_MinresStats)]),
@@ -300,6 +301,13 @@ class System:
         b = np.zeros(self.space.n) if rhs else None
         _check(lib().apf_system_download(self.handle, _ptr(v), _ptr(b)), self.ctx.handle)
         return v, b
+
+    def download_pattern(self):
+        """K's CSR pattern as held on the device (generated there): (rowptr, col)."""
+        rowptr = np.zeros(self.space.n + 1, dtype=np.int64)
+        col = np.zeros(self.space.nnz, dtype=np.int32)
+        _check(lib().apf_system_download_pattern(self.handle, _ptr(rowptr), _ptr(col)), self.ctx.handle)
+        return rowptr, col
 
     def set_solution(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

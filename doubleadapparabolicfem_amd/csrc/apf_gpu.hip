@@ -167,6 +167,69 @@ __global__ void __launch_bounds__(256) k_fill_kron(int nY, int n, int nf, int m,
     }
 }
 
+// Column indices of K's pattern, with the row-block layout of k_fill_kron:
+// segment g of row r is spatial row s's columns shifted by a column base
+//   Y row (k, l, s):  g < Nt  -> (k Nt + g) nf        (A_y, same slab)
+//                     g < 2Nt -> nY + (k p + g - Nt) nf  (B^T, the slab's X nodes)
+//   X row (tn, s):    interval k0 + blk, mode l -> ((k0 + blk) Nt + l) nf,
+//                     C block mode tq -> nY + tq nf  (tn <= p)
+// (the order of two_pass_full_saddle_2d.hpp:326-459 and :780-789, as on the host).
+template <int NT>
+__global__ void __launch_bounds__(256) k_fill_pattern(int nY, int n, int nf, int m, const int64_t* __restrict__ rowptr,
+                                                      const int* __restrict__ sptr, const int* __restrict__ scol,
+                                                      int* __restrict__ col)
+{
+    constexpr int P = NT - 1, SEG = 3 * NT;
+    __shared__ int s_start[FILL_ROWS + 1], s_sb[FILL_ROWS], s_deg[FILL_ROWS], s_cb[FILL_ROWS * SEG];
+    __shared__ float s_rcp[FILL_ROWS];
+    const int r0 = blockIdx.x * FILL_ROWS;
+    const int nrows = min(FILL_ROWS, n - r0);
+    const int64_t base = rowptr[r0];
+    for (int t = threadIdx.x; t <= nrows; t += blockDim.x) s_start[t] = static_cast<int>(rowptr[r0 + t] - base);
+    for (int t = threadIdx.x; t < nrows * SEG; t += blockDim.x) {
+        const int i = t / SEG, g = t - i * SEG;
+        const int row = r0 + i;
+        int s, cbase = 0;
+        if (row < nY) {
+            s = row % nf;
+            const int kl = row / nf, k = kl / NT;
+            cbase = (g < NT) ? (k * NT + g) * nf : nY + (k * P + g - NT) * nf;
+        } else {
+            const int x = row - nY, tn = x / nf;
+            s = x - tn * nf;
+            int k0, nk;
+            time_node_intervals(tn, P, m, k0, nk);
+            const int blk = g / NT, l = g - blk * NT;
+            cbase = (blk < nk) ? ((k0 + blk) * NT + l) * nf : nY + l * nf;
+        }
+        s_cb[t] = cbase;
+        if (g == 0) {
+            const int sb = sptr[s], deg = sptr[s + 1] - sb;
+            s_sb[i] = sb;
+            s_deg[i] = deg;
+            s_rcp[i] = 1.0f / static_cast<float>(deg);
+        }
+    }
+    __syncthreads();
+    const int len = s_start[nrows];
+    int* __restrict__ out = col + base;
+    int cur = 0;
+    for (int e = threadIdx.x; e < len; e += blockDim.x) {
+        while (e >= s_start[cur + 1]) ++cur;
+        const int pos = e - s_start[cur], deg = s_deg[cur];
+        int g = static_cast<int>(static_cast<float>(pos) * s_rcp[cur]);
+        int r = pos - g * deg;
+        if (r < 0) {
+            --g;
+            r += deg;
+        } else if (r >= deg) {
+            ++g;
+            r -= deg;
+        }
+        out[e] = s_cb[cur * SEG + g] + scol[s_sb[cur] + r];
+    }
+}
+
 __global__ void k_fill_rhs(int64_t nY, int64_t n, int nf, int p, double ell, const double* __restrict__ times,
                            TimeTables tt, const double* __restrict__ msp, const double* __restrict__ gsp,
                            double* __restrict__ b)
@@ -845,7 +908,7 @@ int apf_space_pattern(const apf_space* s, int64_t* rowptr, int* col)
     if (!s) return set_error(nullptr, APF_BAD_ARG, "apf_space_pattern: null space");
     if (!s->has_pattern) return set_error(nullptr, APF_BAD_ARG, "apf_space_pattern: space built without pattern");
     if (rowptr) std::copy(s->S.rowptr.begin(), s->S.rowptr.end(), rowptr);
-    if (col) std::copy(s->S.col.begin(), s->S.col.end(), col);
+    if (col) pattern_columns(s->S, col);
     return APF_OK;
 }
 
@@ -936,9 +999,33 @@ int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
         sys->d_gsp = dalloc<double>(sys.get(), S.nf);
         sys->d_rowptr = dalloc<int64_t>(sys.get(), S.rowptr.size());
         upload(sys->d_rowptr, S.rowptr.data(), S.rowptr.size() * sizeof(int64_t));
-        sys->d_col = dalloc<int>(sys.get(), S.col.size());
-        upload(sys->d_col, S.col.data(), S.col.size() * sizeof(int));
-        sys->d_val = dalloc<double>(sys.get(), S.col.size());
+        // the column indices are generated on the device (no host copy of the pattern)
+        sys->d_col = dalloc<int>(sys.get(), static_cast<size_t>(S.nnz));
+        {
+            const dim3 grid(static_cast<unsigned>((S.n + FILL_ROWS - 1) / FILL_ROWS)), block(256);
+            const int nY = static_cast<int>(S.nY), n = static_cast<int>(S.n), m = S.mesh.m();
+            switch (S.Nt) {
+            case 2:
+                hipLaunchKernelGGL(k_fill_pattern<2>, grid, block, 0, sys->stream, nY, n, S.nf, m, sys->d_rowptr,
+                                   sys->d_sptr, sys->d_scol, sys->d_col);
+                break;
+            case 3:
+                hipLaunchKernelGGL(k_fill_pattern<3>, grid, block, 0, sys->stream, nY, n, S.nf, m, sys->d_rowptr,
+                                   sys->d_sptr, sys->d_scol, sys->d_col);
+                break;
+            case 4:
+                hipLaunchKernelGGL(k_fill_pattern<4>, grid, block, 0, sys->stream, nY, n, S.nf, m, sys->d_rowptr,
+                                   sys->d_sptr, sys->d_scol, sys->d_col);
+                break;
+            default:
+                hipLaunchKernelGGL(k_fill_pattern<5>, grid, block, 0, sys->stream, nY, n, S.nf, m, sys->d_rowptr,
+                                   sys->d_sptr, sys->d_scol, sys->d_col);
+                break;
+            }
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(sys->stream));
+        }
+        sys->d_val = dalloc<double>(sys.get(), static_cast<size_t>(S.nnz));
         sys->d_b = dalloc<double>(sys.get(), S.n);
         sys->d_x = dalloc<double>(sys.get(), S.n);
         HIP_TRY(hipMemset(sys->d_x, 0, S.n * sizeof(double)));
@@ -1046,7 +1133,7 @@ int apf_assemble_main(apf_system* sys, const apf_coeff* coeff, int mode)
                 break;
             }
         } else {
-            HIP_TRY(hipMemsetAsync(sys->d_val, 0, S.col.size() * sizeof(double), st));
+            HIP_TRY(hipMemsetAsync(sys->d_val, 0, static_cast<size_t>(S.nnz) * sizeof(double), st));
             const int nloc = S.Ns * S.Nt;
             const int64_t total = static_cast<int64_t>(S.mesh.m()) * ns * nloc * nloc;
             const int block = 256;
@@ -1081,11 +1168,29 @@ int apf_system_download(apf_system* sys, double* K_values, double* b)
     try {
         HIP_TRY(hipSetDevice(sys->ctx->device));
         HIP_TRY(hipStreamSynchronize(sys->stream));
-        if (K_values) HIP_TRY(hipMemcpy(K_values, sys->d_val, sys->space->col.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (K_values)
+            HIP_TRY(hipMemcpy(K_values, sys->d_val, static_cast<size_t>(sys->space->nnz) * sizeof(double),
+                              hipMemcpyDeviceToHost));
         if (b) HIP_TRY(hipMemcpy(b, sys->d_b, sys->space->n * sizeof(double), hipMemcpyDeviceToHost));
         return APF_OK;
     } catch (const HipFailure& e) {
         return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_system_download: ") + e.what());
+    }
+}
+
+int apf_system_download_pattern(apf_system* sys, int64_t* rowptr, int* col)
+{
+    if (!sys) return set_error(nullptr, APF_BAD_ARG, "apf_system_download_pattern: null system");
+    try {
+        HIP_TRY(hipSetDevice(sys->ctx->device));
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+        const Space& S = *sys->space;
+        if (rowptr)
+            HIP_TRY(hipMemcpy(rowptr, sys->d_rowptr, (S.n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (col) HIP_TRY(hipMemcpy(col, sys->d_col, static_cast<size_t>(S.nnz) * sizeof(int), hipMemcpyDeviceToHost));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_system_download_pattern: ") + e.what());
     }
 }
 

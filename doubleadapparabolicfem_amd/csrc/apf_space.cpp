@@ -293,9 +293,14 @@ void build_pattern(Space& S)
     }
     S.nnz = S.rowptr[S.n];
     if (S.n > 2147483647LL) throw std::runtime_error("build_pattern: more than 2^31 unknowns");
-    S.col.resize(S.nnz);
+}
+
+void pattern_columns(const Space& S, int* col)
+{
+    const int p = S.p, Nt = S.Nt, nf = S.nf;
+    const int m = S.mesh.m();
     for (int64_t r = 0; r < S.n; ++r) {
-        int* out = &S.col[S.rowptr[r]];
+        int* out = &col[S.rowptr[r]];
         if (r < S.nY) {
             const int s = static_cast<int>(r % nf);
             const int k = static_cast<int>(r / nf) / Nt;

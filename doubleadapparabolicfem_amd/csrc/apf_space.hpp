@@ -74,8 +74,7 @@ struct Space {
     std::vector<int16_t> rank;        // [ns][Ns][Ns]
     // global sizes and pattern of K
     int64_t nY = 0, nX = 0, n = 0, nnz = 0;
-    std::vector<int64_t> rowptr;      // [n+1]
-    std::vector<int> col;             // [nnz]
+    std::vector<int64_t> rowptr;      // [n+1]; the columns are generated where needed (pattern_columns, or on the device)
     PatchTopology patches;
 };
 
@@ -83,6 +82,8 @@ struct Space {
 void build_space(Space& S, const TensorMesh& mesh, int p);
 // Build the CSR pattern of K = [A_y B^T; B C] (full element-coupling pattern).
 void build_pattern(Space& S);
+// column indices of K's pattern in row order, out[nnz]
+void pattern_columns(const Space& S, int* out);
 // Row-structure helper: number of intervals touching time node tn and the first one.
 APF_HD inline void time_node_intervals(int tn, int p, int m, int& k0, int& nk)
 {

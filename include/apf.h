@@ -155,6 +155,11 @@ void apf_system_destroy(apf_system* sys);
 int apf_assemble_main(apf_system* sys, const apf_coeff* coeff, int mode);
 /* Copy K values [nnz] and b = [f; g] [n] to the host (either may be NULL). */
 int apf_system_download(apf_system* sys, double* K_values, double* b);
+/* Copy the device CSR pattern of K (row pointer [n+1], columns [nnz]) to the
+ * host (either may be NULL).  The column indices are generated on the device
+ * when the system is created.  Replaces reading the compressed index arrays of
+ * the assembled SparseMatrix (src/linear_algebra/eigen_backend/sparse_matrix.hpp:29-37). */
+int apf_system_download_pattern(apf_system* sys, int64_t* rowptr, int* col);
 
 /* Preconditioned MINRES on K x = b (x0 may be NULL).  The preconditioner is
  * block diagonal: diag-in-space / exact-in-time blocks of A_y and of the

@@ -114,3 +114,30 @@ def test_multigrid_minres_converges_and_matches(ctx, dim, p, outer, prec):
     S = OA.assemble(stm, od, prob)
     xd = OE.main_solve(S)
     assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+def _pattern_case(name):
+    if name == "uj-p1":
+        return meshes.uniform_mesh(2, 2), 1
+    if name == "uj-p3":
+        return meshes.uniform_mesh(2, 1), 3
+    if name == "uj-p4":
+        return meshes.union_jack_mesh(4, np.linspace(0.0, 1.0, 3)), 4
+    if name == "interval-p2":
+        return meshes.uniform_mesh(1, 3), 2
+    if name == "fan-graded-p2":
+        return meshes.fan_mesh(2, meshes.graded_times(1, 4), seed=5), 2
+    return meshes.segment_batch_mesh([8, 16, 24], np.linspace(0.0, 1.0, 4)), 1
+
+
+@pytest.mark.parametrize("name", ["uj-p1", "uj-p3", "uj-p4", "interval-p2", "fan-graded-p2", "batch-p1"])
+def test_device_pattern_equals_host_pattern(ctx, name):
+    """The column indices generated on the device are bit-identical to the host pattern."""
+    m, p = _pattern_case(name)
+    print(f"pattern {name}", flush=True)
+    sp = apf.Space(m, p)
+    sysm = apf.System(ctx, sp)
+    rp_d, col_d = sysm.download_pattern()
+    rp_h, col_h = sp.pattern()
+    assert np.array_equal(rp_d, rp_h)
+    assert np.array_equal(col_d, col_h)
