@@ -1828,9 +1828,15 @@ __global__ void k_max_reduce(int n, const double* a, double* out)
 
 namespace apf {
 
-// Column blocking of the patch solve at padded order np: the widest block of
-// 16 CT columns (CT <= 6, whole slabs) whose X / R tiles fit the LDS.  When
-// even 16 columns do not fit, the tiles go to a global workspace instead.
+// Column blocking of the patch solve at padded order np: blocks of 16 CT
+// columns (CT <= 6, whole slabs) whose X / R tiles fit the LDS.  The best
+// width depends on the order and the precision (two workgroups per CU and more
+// column groups against fewer re-reads of the matrices: at 2+1D p=2, np = 128,
+// CT = 3 beats CT = 6 in FP32; at p=3, np = 224, CT = 1 beats CT = 3), so the
+// estimator times the candidates once per system (solve_candidates) and passes
+// the winner as `forced`.  Without one, the widest block that fits is used.
+// When even 16 columns do not fit, the tiles go to a global workspace, one
+// column tile per workgroup.
 struct SolveShape {
     int CT, cpw, ncg;
     bool gws;
@@ -1838,31 +1844,60 @@ struct SolveShape {
     int64_t ws_stride;  // workspace doubles per workgroup (gws only)
 };
 
-SolveShape solve_shape(int np, int ncols, int Nt, size_t ft_bytes)
+namespace {
+constexpr size_t SOLVE_LDS_CAP = 160 * 1024 - 4096;
+
+// X tile [np][16 ct] doubles, R tile [np][LR] (LR = 16 ct + 2 in FP32, 16 ct in FP64)
+size_t solve_tiles(int np, int ct, size_t ft_bytes)
 {
-    const size_t cap = 160 * 1024 - 4096;
-    // X tile [np][16 ct] doubles, R tile [np][LR] (LR = 16 ct + 2 in FP32, 16 ct in FP64)
-    auto tiles = [&](int ct) {
-        const size_t cw = 16 * ct, lr = (ft_bytes == sizeof(float)) ? cw + 2 : cw;
-        return static_cast<size_t>(np) * (cw * sizeof(double) + lr * ft_bytes);
-    };
+    const size_t cw = 16 * ct, lr = (ft_bytes == sizeof(float)) ? cw + 2 : cw;
+    return static_cast<size_t>(np) * (cw * sizeof(double) + lr * ft_bytes);
+}
+
+int solve_ct_max(int ncols)
+{
+    int ctmax = 6;
+    if (const char* e = std::getenv("APF_SOLVE_CT_MAX")) ctmax = std::min(ctmax, std::max(1, std::atoi(e)));
+    return std::min(ctmax, (ncols + 15) / 16);
+}
+
+int cols_per_group(int ct, int Nt) { return std::max(Nt, (16 * ct / Nt) * Nt); }
+}  // namespace
+
+SolveShape solve_shape(int np, int ncols, int Nt, size_t ft_bytes, int forced = 0)
+{
     SolveShape sh{};
-    int CT = 6;
-    if (const char* e = std::getenv("APF_SOLVE_CT_MAX")) CT = std::min(CT, std::max(1, std::atoi(e)));
-    sh.gws = tiles(1) > cap;
-    if (!sh.gws)
-        while (CT > 1 && tiles(CT) > cap) --CT;
-    // workspace tiles: one column tile per workgroup, so that the few large
-    // patches spread over as many CUs as they have column tiles
-    if (sh.gws) CT = 1;
-    CT = std::min(CT, (ncols + 15) / 16);
+    sh.gws = solve_tiles(np, 1, ft_bytes) > SOLVE_LDS_CAP;
+    int CT = solve_ct_max(ncols);
+    if (sh.gws) {
+        CT = 1;
+    } else if (forced > 0 && solve_tiles(np, forced, ft_bytes) <= SOLVE_LDS_CAP) {
+        CT = forced;
+    } else {
+        while (CT > 1 && solve_tiles(np, CT, ft_bytes) > SOLVE_LDS_CAP) --CT;
+    }
     sh.CT = CT;
-    sh.cpw = std::max(Nt, (16 * CT / Nt) * Nt);
+    sh.cpw = cols_per_group(CT, Nt);
     if (sh.cpw > 16 * CT) throw std::runtime_error("patch solve: a slab's modes exceed the column block");
     sh.ncg = (ncols + sh.cpw - 1) / sh.cpw;
-    sh.lds_bytes = sh.gws ? 0 : tiles(CT);
-    sh.ws_stride = sh.gws ? static_cast<int64_t>((tiles(CT) + sizeof(double) - 1) / sizeof(double)) : 0;
+    sh.lds_bytes = sh.gws ? 0 : solve_tiles(np, CT, ft_bytes);
+    sh.ws_stride = sh.gws ? static_cast<int64_t>((solve_tiles(np, CT, ft_bytes) + sizeof(double) - 1) / sizeof(double))
+                          : 0;
     return sh;
+}
+
+// Widths worth timing: tiles within the LDS and column groups that split the
+// columns evenly (the last group at least half full).
+std::vector<int> solve_candidates(int np, int ncols, int Nt, size_t ft_bytes)
+{
+    std::vector<int> out;
+    if (solve_tiles(np, 1, ft_bytes) > SOLVE_LDS_CAP) return out;
+    for (int ct = solve_ct_max(ncols); ct >= 1; --ct) {
+        if (solve_tiles(np, ct, ft_bytes) > SOLVE_LDS_CAP) continue;
+        const int cpw = cols_per_group(ct, Nt), rest = ncols % cpw;
+        if (rest == 0 || 2 * rest >= cpw) out.push_back(ct);
+    }
+    return out;
 }
 
 void estimator_alloc(System* sys)
@@ -1994,6 +2029,7 @@ void estimator_alloc(System* sys)
     B.lt = dalloc<double>(sys, S.nY);
     B.sums = dalloc<double>(sys, 8);
     B.extra_rounds = dalloc<int>(sys, 1);
+    B.tune_extra = dalloc<int>(sys, 1);
 }
 
 }  // namespace apf
@@ -2162,35 +2198,74 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             }
             HIP_TRY(hipEventRecord(sys->evp[2], st));
             for (const auto& bk : B.buckets) {
-                const SolveShape sh = solve_shape(bk.np, ncols, P.Nt, sizeof(FT));
-                const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * sh.ncg);
-                if (sh.gws && static_cast<int64_t>(bk.count) * sh.ncg * sh.ws_stride > B.solve_ws_doubles)
-                    throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
-                auto solve = [&](auto kern) {
-                    if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
-                    hipLaunchKernelGGL(kern, dim3(grid), dim3(solve_threads(sh.CT)), sh.lds_bytes, st, P, topo, ps, bk.q0,
-                                       bk.count, sh.ncg, sh.cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res, B.extra_rounds,
-                                       B.solve_ws, sh.ws_stride);
-                };
-                if (sh.gws) {
-                    switch (sh.CT) {
-                        case 1: solve(k_patch_solve<FT, 1, true>); break;
-                        case 2: solve(k_patch_solve<FT, 2, true>); break;
-                        case 3: solve(k_patch_solve<FT, 3, true>); break;
-                        case 4: solve(k_patch_solve<FT, 4, true>); break;
-                        case 5: solve(k_patch_solve<FT, 5, true>); break;
-                        default: solve(k_patch_solve<FT, 6, true>); break;
+                // one solve of the bucket with column blocking sh; `extra` counts refinement rounds
+                auto run = [&](const SolveShape& sh, int* extra) {
+                    const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * sh.ncg);
+                    if (sh.gws && static_cast<int64_t>(bk.count) * sh.ncg * sh.ws_stride > B.solve_ws_doubles)
+                        throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
+                    auto solve = [&](auto kern) {
+                        if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
+                        hipLaunchKernelGGL(kern, dim3(grid), dim3(solve_threads(sh.CT)), sh.lds_bytes, st, P, topo, ps,
+                                           bk.q0, bk.count, sh.ncg, sh.cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res, extra,
+                                           B.solve_ws, sh.ws_stride);
+                    };
+                    if (sh.gws) {
+                        switch (sh.CT) {
+                            case 1: solve(k_patch_solve<FT, 1, true>); break;
+                            case 2: solve(k_patch_solve<FT, 2, true>); break;
+                            case 3: solve(k_patch_solve<FT, 3, true>); break;
+                            case 4: solve(k_patch_solve<FT, 4, true>); break;
+                            case 5: solve(k_patch_solve<FT, 5, true>); break;
+                            default: solve(k_patch_solve<FT, 6, true>); break;
+                        }
+                        return;
                     }
-                    continue;
+                    switch (sh.CT) {
+                        case 1: solve(k_patch_solve<FT, 1, false>); break;
+                        case 2: solve(k_patch_solve<FT, 2, false>); break;
+                        case 3: solve(k_patch_solve<FT, 3, false>); break;
+                        case 4: solve(k_patch_solve<FT, 4, false>); break;
+                        case 5: solve(k_patch_solve<FT, 5, false>); break;
+                        default: solve(k_patch_solve<FT, 6, false>); break;
+                    }
+                };
+                // the column blocking of this (order, precision): timed once per system
+                // (each candidate twice, the faster run counts); the trial solves
+                // compute the same result and count their rounds elsewhere
+                const std::pair<int, int> key{bk.np, static_cast<int>(sizeof(FT))};
+                auto it = sys->solve_ct.find(key);
+                if (it == sys->solve_ct.end()) {
+                    const std::vector<int> cand = solve_candidates(bk.np, ncols, P.Nt, sizeof(FT));
+                    const char* tune_env = std::getenv("APF_SOLVE_TUNE");
+                    int best = 0;
+                    if (cand.size() > 1 && !(tune_env && std::atoi(tune_env) == 0)) {
+                        hipEvent_t ta, tb;
+                        HIP_TRY(hipEventCreate(&ta));
+                        HIP_TRY(hipEventCreate(&tb));
+                        float best_ms = 0.0f;
+                        for (int ct : cand) {
+                            const SolveShape shc = solve_shape(bk.np, ncols, P.Nt, sizeof(FT), ct);
+                            float ms = 0.0f;
+                            for (int rep = 0; rep < 2; ++rep) {
+                                HIP_TRY(hipEventRecord(ta, st));
+                                run(shc, B.tune_extra);
+                                HIP_TRY(hipEventRecord(tb, st));
+                                HIP_TRY(hipEventSynchronize(tb));
+                                float t = 0.0f;
+                                HIP_TRY(hipEventElapsedTime(&t, ta, tb));
+                                ms = (rep == 0) ? t : std::min(ms, t);
+                            }
+                            if (best == 0 || ms < best_ms) {
+                                best = ct;
+                                best_ms = ms;
+                            }
+                        }
+                        (void)hipEventDestroy(ta);
+                        (void)hipEventDestroy(tb);
+                    }
+                    it = sys->solve_ct.emplace(key, best).first;
                 }
-                switch (sh.CT) {
-                    case 1: solve(k_patch_solve<FT, 1, false>); break;
-                    case 2: solve(k_patch_solve<FT, 2, false>); break;
-                    case 3: solve(k_patch_solve<FT, 3, false>); break;
-                    case 4: solve(k_patch_solve<FT, 4, false>); break;
-                    case 5: solve(k_patch_solve<FT, 5, false>); break;
-                    default: solve(k_patch_solve<FT, 6, false>); break;
-                }
+                run(solve_shape(bk.np, ncols, P.Nt, sizeof(FT), it->second), B.extra_rounds);
             }
         };
         HIP_TRY(hipEventRecord(sys->ev0, st));

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -144,6 +145,7 @@ struct EstimatorBuffers {
     double* lt = nullptr;         // slab-reconstructed lambda
     double* sums = nullptr;
     int* extra_rounds = nullptr;  // refinement rounds beyond refine_steps (all blocks)
+    int* tune_extra = nullptr;    // the same count for the column-blocking trial launches (discarded)
     // patch slots in processing order (largest first) and size buckets
     int* s_n = nullptr;
     int* s_np = nullptr;
@@ -223,6 +225,8 @@ struct System {
     double t_mg_setup_s = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evp[4] = {nullptr, nullptr, nullptr, nullptr};  // estimator phase boundaries
+    // patch-solve column blocking chosen by timing, per (padded order, bytes of the inverse's type)
+    std::map<std::pair<int, int>, int> solve_ct;
     float t_assembly_ms = 0.f, t_patch_ms = 0.f, t_diag_ms = 0.f, t_spmv_ms = 0.f;
     float t_factor_ms = 0.f, t_rhs_ms = 0.f, t_solve_ms = 0.f;
     bool assembled = false;
