@@ -2171,30 +2171,68 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             }
             HIP_TRY(hipEventRecord(sys->evp[1], st));
             for (const auto& bk : B.buckets) {
-                int sg = std::min(8, P.m);
                 auto lr = [&](int g) {
                     return (static_cast<size_t>(bk.np) * g * P.Nt +
                             static_cast<size_t>(bk.nc_max) * g * P.Ns * P.Nt * 2 + bk.nc_max * 4) * sizeof(double) +
                            static_cast<size_t>(bk.nc_max) * (1 + 2 * P.Nf) * sizeof(int);
                 };
-                while (sg > 1 && lr(sg) > 64 * 1024) --sg;
-                if (lr(sg) > lds_cap) throw std::runtime_error("patch right-hand side block exceeds LDS");
-                const int nsg = (P.m + sg - 1) / sg;
-                auto rhs = [&](auto kern) {
-                    set_lds(reinterpret_cast<const void*>(kern), lr(sg));
-                    hipLaunchKernelGGL(kern, dim3(bk.count * nsg), dim3(RHS_THREADS), lr(sg), st, P, Tb, topo, ps,
-                                       bk.q0, sg, nsg, bk.nc_max, sys->d_verts, sys->d_cells, sys->d_Mcell,
-                                       sys->d_cell_free, sys->d_times, B.lt, sys->d_x, B.RHS);
+                // one pass of the bucket with sg slabs per workgroup
+                auto run = [&](int sg) {
+                    if (lr(sg) > lds_cap) throw std::runtime_error("patch right-hand side block exceeds LDS");
+                    const int nsg = (P.m + sg - 1) / sg;
+                    auto rhs = [&](auto kern) {
+                        set_lds(reinterpret_cast<const void*>(kern), lr(sg));
+                        hipLaunchKernelGGL(kern, dim3(bk.count * nsg), dim3(RHS_THREADS), lr(sg), st, P, Tb, topo, ps,
+                                           bk.q0, sg, nsg, bk.nc_max, sys->d_verts, sys->d_cells, sys->d_Mcell,
+                                           sys->d_cell_free, sys->d_times, B.lt, sys->d_x, B.RHS);
+                    };
+                    switch (S.dim * 10 + S.p) {
+                        case 11: rhs(k_patch_rhs<1, 1>); break;
+                        case 12: rhs(k_patch_rhs<1, 2>); break;
+                        case 13: rhs(k_patch_rhs<1, 3>); break;
+                        case 21: rhs(k_patch_rhs<2, 1>); break;
+                        case 22: rhs(k_patch_rhs<2, 2>); break;
+                        case 23: rhs(k_patch_rhs<2, 3>); break;
+                        default: throw std::runtime_error("patch right-hand sides: unsupported (dim, p)");
+                    }
                 };
-                switch (S.dim * 10 + S.p) {
-                    case 11: rhs(k_patch_rhs<1, 1>); break;
-                    case 12: rhs(k_patch_rhs<1, 2>); break;
-                    case 13: rhs(k_patch_rhs<1, 3>); break;
-                    case 21: rhs(k_patch_rhs<2, 1>); break;
-                    case 22: rhs(k_patch_rhs<2, 2>); break;
-                    case 23: rhs(k_patch_rhs<2, 3>); break;
-                    default: throw std::runtime_error("patch right-hand sides: unsupported (dim, p)");
+                // slabs per workgroup (1, 2, 4, 8 within 64 KiB of LDS): timed once per
+                // system and padded order, each candidate twice (the passes only write RHS)
+                auto it = sys->rhs_sg.find(bk.np);
+                if (it == sys->rhs_sg.end()) {
+                    std::vector<int> cand;
+                    for (int g = 8; g >= 1; g /= 2)
+                        if (g <= P.m && (lr(g) <= 64 * 1024 || g == 1)) cand.push_back(g);
+                    const char* tune_env = std::getenv("APF_SOLVE_TUNE");
+                    int best = cand.empty() ? 1 : cand.front();
+                    if (cand.size() > 1 && !(tune_env && std::atoi(tune_env) == 0)) {
+                        hipEvent_t ta, tb;
+                        HIP_TRY(hipEventCreate(&ta));
+                        HIP_TRY(hipEventCreate(&tb));
+                        float best_ms = 0.0f;
+                        best = 0;
+                        for (int g : cand) {
+                            float ms = 0.0f;
+                            for (int rep = 0; rep < 2; ++rep) {
+                                HIP_TRY(hipEventRecord(ta, st));
+                                run(g);
+                                HIP_TRY(hipEventRecord(tb, st));
+                                HIP_TRY(hipEventSynchronize(tb));
+                                float t = 0.0f;
+                                HIP_TRY(hipEventElapsedTime(&t, ta, tb));
+                                ms = (rep == 0) ? t : std::min(ms, t);
+                            }
+                            if (best == 0 || ms < best_ms) {
+                                best = g;
+                                best_ms = ms;
+                            }
+                        }
+                        (void)hipEventDestroy(ta);
+                        (void)hipEventDestroy(tb);
+                    }
+                    it = sys->rhs_sg.emplace(bk.np, best).first;
                 }
+                run(it->second);
             }
             HIP_TRY(hipEventRecord(sys->evp[2], st));
             for (const auto& bk : B.buckets) {

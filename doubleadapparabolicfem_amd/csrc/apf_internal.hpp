@@ -227,6 +227,8 @@ struct System {
     hipEvent_t evp[4] = {nullptr, nullptr, nullptr, nullptr};  // estimator phase boundaries
     // patch-solve column blocking chosen by timing, per (padded order, bytes of the inverse's type)
     std::map<std::pair<int, int>, int> solve_ct;
+    // patch right-hand sides: slabs per workgroup chosen by timing, per padded order
+    std::map<int, int> rhs_sg;
     float t_assembly_ms = 0.f, t_patch_ms = 0.f, t_diag_ms = 0.f, t_spmv_ms = 0.f;
     float t_factor_ms = 0.f, t_rhs_ms = 0.f, t_solve_ms = 0.f;
     bool assembled = false;
