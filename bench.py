@@ -31,7 +31,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors a
 # plain copy kernel of known size reads) from the rocprofv3 --pmc runs of
 # `bench.py --profile`, summarised in profiles/.
-MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9, "solve": 2.37e9, "factor": 0.33e9}}
+MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9, "solve": 2.28e9, "factor": 0.33e9}}
 
 
 def workload(name):
@@ -99,6 +99,7 @@ def run_c3(args):
                 wall += time.perf_counter() - t0
                 rec.append((t_asm, t_minres, st["iterations"], (est["seconds_patches"] + est["seconds_diagnostics"]) * 1e3,
                             sysm.phase_timings(), est))
+        sysm.estimate(apf.APF_FP64, 0, per_cell=False)  # untimed: the first call tunes the tiles
         est64 = sysm.estimate(apf.APF_FP64, 0, per_cell=False)
         ph64 = sysm.phase_timings()
         t_asm = float(np.mean([r[0] for r in rec]))
@@ -212,8 +213,8 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="one assembly, MINRES solve, mixed and FP64 estimate, 20 SpMVs and a 1 GiB device "
-                         "copy (for counter runs)")
+                    help="one assembly, MINRES solve, two mixed and two FP64 estimates, 20 SpMVs and a 1 GiB "
+                         "device copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
@@ -235,8 +236,12 @@ def main():
         # counter runs: one pass of each hot-path kernel, then the calibration copy
         sysm.assemble(W["coeff"])
         sysm.minres(tol=1e-10, max_it=5000, check_interval=10)
-        sysm.estimate(apf.APF_FP32_REFINE, 2, per_cell=False)
-        sysm.estimate(apf.APF_FP64, 0, per_cell=False)
+        # each estimate twice: a system's first call in a precision also times its tile
+        # candidates, the second runs the chosen tiles only
+        for _ in range(2):
+            sysm.estimate(apf.APF_FP32_REFINE, 2, per_cell=False)
+        for _ in range(2):
+            sysm.estimate(apf.APF_FP64, 0, per_cell=False)
         ms = sysm.bench_spmv(20)
         gbs = ctx.bench_stream(1 << 30, 3)
         print(json.dumps({"spmv_ms": ms, "spmv_algorithmic_bytes": 12 * sp.nnz + 20 * sp.n, "nnz": sp.nnz,
@@ -267,7 +272,9 @@ def main():
     t_est = np.mean([r[3] for r in rec])
     its = int(np.mean([r[2]["iterations"] for r in rec]))
     est_mixed = rec[-1][4]
-    # FP64 estimator path on the same solution for the mixed-vs-FP64 error and speed ratio
+    # FP64 estimator path on the same solution for the mixed-vs-FP64 error and speed ratio (one
+    # untimed call first: a system's first estimate in a precision times its tile candidates)
+    sysm.estimate(apf.APF_FP64, 0, per_cell=False)
     est64 = sysm.estimate(apf.APF_FP64, 0, per_cell=False)
     t_est64 = (est64["seconds_patches"] + est64["seconds_diagnostics"]) * 1e3
     rel = {k: abs(est_mixed[k] - est64[k]) / abs(est64[k]) for k in ("flux2", "eta2", "lambda_y2", "y_est2")}
