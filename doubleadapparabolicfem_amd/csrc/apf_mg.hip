@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "apf_internal.hpp"
@@ -19,47 +20,92 @@ namespace apf {
 
 namespace {
 
-// One wavefront per row of (K + s_c M) over columns [0, nc): the row's entries
-// are staged in registers, 64 at a time (one per lane), and broadcast to every
-// column pass; lanes run along the columns.  row_apply hands the product of
-// each column to the epilogue f(c, value).
-constexpr int MG_NP = 4;  // column passes of 64 handled together
+// Value of lane t (wave-uniform) of v, through scalar registers.
+__device__ inline int lane_value(int v, int t) { return __builtin_amdgcn_readlane(v, t); }
+__device__ inline float lane_value(float v, int t)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), t));
+}
+__device__ inline double lane_value(double v, int t)
+{
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffull), t));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), t));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
 
-template <typename R, typename F>
+// One wavefront per sparse row, lanes along the columns of a block X[n][ld]:
+// NP passes of 64 columns are handled together (cc holds each lane's column
+// per pass).  The row's entries (column and NV values) are staged 64 at a time,
+// one per lane, and read back through scalar registers; the column loads of
+// MG_U entries are issued before their FMAs, so a row costs a few memory
+// latencies instead of one per entry.  coef(v, x, acc) applies one entry with
+// values v to the loaded x, in the row's entry order.  Entries past the row's
+// end repeat its last column with zero values.
+constexpr int MG_U = 8;
+
+template <int NP, int NV, typename R, typename Coef>
+__device__ inline void row_entries(int b, int e, const int* __restrict__ col, const R* const (&vals)[NV],
+                                   const R* __restrict__ X, int ld, const int (&cc)[NP], R (&acc)[NP], Coef coef)
+{
+    const int lane = threadIdx.x % 64;
+    for (int e0 = b; e0 < e; e0 += 64) {
+        const int my = e0 + lane;
+        const int cnt = min(64, e - e0);
+        const int cj = col[min(my, e - 1)];
+        R v[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = (my < e) ? vals[k][my] : R(0);
+        auto step = [&](int t0, auto width) {
+            constexpr int U = decltype(width)::value;
+            R xv[U][NP];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const R* xr = X + static_cast<int64_t>(lane_value(cj, t0 + u)) * ld;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) xv[u][q] = xr[cc[q]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                R w[NV];
+#pragma unroll
+                for (int k = 0; k < NV; ++k) w[k] = lane_value(v[k], t0 + u);
+                coef(w, xv[u], acc);
+            }
+        };
+        int t0 = 0;
+        for (; t0 + MG_U <= cnt; t0 += MG_U) step(t0, std::integral_constant<int, MG_U>{});
+        for (; t0 < cnt; t0 += 4) step(t0, std::integral_constant<int, 4>{});
+    }
+}
+
+// Rows of (K + s_c M) X over columns [0, nc); row_apply hands the product of
+// each column to the epilogue f(c, value).  Lanes past the last column read
+// column nc - 1 and are not handed on.
+template <int NP, typename R, typename F>
 __device__ inline void row_apply(int row, const int* __restrict__ ptr, const int* __restrict__ col,
                                  const R* __restrict__ K, const R* __restrict__ M, const R* __restrict__ sig, int nc,
-                                 int ld, const R* __restrict__ X, int lane, F f)
+                                 int ld, const R* __restrict__ X, F f)
 {
+    const int lane = threadIdx.x % 64;
     const int b = ptr[row], e = ptr[row + 1];
-    for (int cg = 0; cg < nc; cg += MG_NP * 64) {
-        R acc[MG_NP];
-        R sg[MG_NP];
+    const R* const vals[2] = {K, M};
+    for (int cg = 0; cg < nc; cg += NP * 64) {
+        R acc[NP];
+        R sg[NP];
+        int cc[NP];
 #pragma unroll
-        for (int q = 0; q < MG_NP; ++q) {
+        for (int q = 0; q < NP; ++q) {
             acc[q] = 0;
-            const int c = cg + q * 64 + lane;
-            sg[q] = (c < nc) ? sig[c] : R(0);
+            cc[q] = min(cg + q * 64 + lane, nc - 1);
+            sg[q] = sig[cc[q]];
         }
-        for (int e0 = b; e0 < e; e0 += 64) {
-            const int my = e0 + lane;
-            const int cj = (my < e) ? col[my] : 0;
-            const R kv = (my < e) ? K[my] : R(0);
-            const R mv = (my < e) ? M[my] : R(0);
-            const int cnt = min(64, e - e0);
-            for (int t = 0; t < cnt; ++t) {
-                const int j = __shfl(cj, t, 64);
-                const R kk = __shfl(kv, t, 64);
-                const R mm = __shfl(mv, t, 64);
-                const R* xr = X + static_cast<int64_t>(j) * ld;
+        row_entries<NP, 2>(b, e, col, vals, X, ld, cc, acc, [&](const R(&w)[2], const R(&x)[NP], R(&a)[NP]) {
 #pragma unroll
-                for (int q = 0; q < MG_NP; ++q) {
-                    const int c = cg + q * 64 + lane;
-                    if (c < nc) acc[q] += (kk + sg[q] * mm) * xr[c];
-                }
-            }
-        }
+            for (int q = 0; q < NP; ++q) a[q] += (w[0] + sg[q] * w[1]) * x[q];
+        });
 #pragma unroll
-        for (int q = 0; q < MG_NP; ++q) {
+        for (int q = 0; q < NP; ++q) {
             const int c = cg + q * 64 + lane;
             if (c < nc) f(c, acc[q]);
         }
@@ -67,23 +113,22 @@ __device__ inline void row_apply(int row, const int* __restrict__ ptr, const int
 }
 
 // Y = (K + s_c M) X   (mode 0)   or   Y = B - (K + s_c M) X   (mode 1), columns [0, nc).
-template <typename R>
+template <typename R, int NP>
 __global__ void __launch_bounds__(256) k_mg_apply(int nf, const int* __restrict__ ptr, const int* __restrict__ col,
                                                   const R* __restrict__ K, const R* __restrict__ M,
                                                   const R* __restrict__ sig, int nc, int ld, const R* __restrict__ X,
                                                   const R* __restrict__ B, R* __restrict__ Y, int mode)
 {
     const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
-    const int lane = threadIdx.x % 64;
     if (row >= nf) return;
-    row_apply(row, ptr, col, K, M, sig, nc, ld, X, lane, [&](int c, R v) {
+    row_apply<NP>(row, ptr, col, K, M, sig, nc, ld, X, [&](int c, R v) {
         const int64_t o = static_cast<int64_t>(row) * ld + c;
         Y[o] = mode ? B[o] - v : v;
     });
 }
 
 // Chebyshev start from x: r = B - A x, d = D^-1 r / theta (one pass)
-template <typename R>
+template <typename R, int NP>
 __global__ void __launch_bounds__(256)
     k_mg_resid_start(int nf, const int* __restrict__ ptr, const int* __restrict__ col, const R* __restrict__ K,
                      const R* __restrict__ M, const R* __restrict__ sig, int nc, int ld, const R* __restrict__ X,
@@ -91,10 +136,9 @@ __global__ void __launch_bounds__(256)
                      const R* __restrict__ dM, R inv_theta)
 {
     const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
-    const int lane = threadIdx.x % 64;
     if (row >= nf) return;
     const R dk = dK[row], dm = dM[row];
-    row_apply(row, ptr, col, K, M, sig, nc, ld, X, lane, [&](int c, R v) {
+    row_apply<NP>(row, ptr, col, K, M, sig, nc, ld, X, [&](int c, R v) {
         const int64_t o = static_cast<int64_t>(row) * ld + c;
         const R rv = B[o] - v;
         r[o] = rv;
@@ -105,7 +149,7 @@ __global__ void __launch_bounds__(256)
 // One Chebyshev step with its product: x += d ; r -= A d ; dn = c1 d + c2 D^-1 r.
 // A d reads d of the neighbouring rows, so the new direction goes to a second
 // buffer dn (the caller swaps them).
-template <typename R>
+template <typename R, int NP>
 __global__ void __launch_bounds__(256)
     k_mg_cheb_fused(int nf, const int* __restrict__ ptr, const int* __restrict__ col, const R* __restrict__ K,
                     const R* __restrict__ M, const R* __restrict__ sig, int nc, int ld, const R* __restrict__ d,
@@ -113,10 +157,9 @@ __global__ void __launch_bounds__(256)
                     const R* __restrict__ dM, R c1, R c2, int last)
 {
     const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
-    const int lane = threadIdx.x % 64;
     if (row >= nf) return;
     const R dk = dK[row], dm = dM[row];
-    row_apply(row, ptr, col, K, M, sig, nc, ld, d, lane, [&](int c, R v) {
+    row_apply<NP>(row, ptr, col, K, M, sig, nc, ld, d, [&](int c, R v) {
         const int64_t o = static_cast<int64_t>(row) * ld + c;
         const R dv = d[o];
         x[o] += dv;
@@ -154,22 +197,36 @@ __global__ void k_block_copy(int nf, int nc, int ld, const R* __restrict__ X, R*
 }
 
 // Y = A X for a plain CSR (prolongation / restriction), optionally accumulated
-template <typename R>
+template <typename R, int NP>
 __global__ void __launch_bounds__(256) k_csr_mm(int nrows, const int* __restrict__ ptr, const int* __restrict__ col,
                                                 const R* __restrict__ val, int nc, int ldx, const R* __restrict__ X,
                                                 int ldy, R* __restrict__ Y, int accumulate)
 {
-    const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
     const int lane = threadIdx.x % 64;
     if (row >= nrows) return;
     const int b = ptr[row], e = ptr[row + 1];
-    for (int c0 = 0; c0 < nc; c0 += 64) {
-        const int c = c0 + lane;
-        if (c >= nc) break;
-        R acc = 0;
-        for (int q = b; q < e; ++q) acc += val[q] * X[static_cast<int64_t>(col[q]) * ldx + c];
-        const int64_t o = static_cast<int64_t>(row) * ldy + c;
-        Y[o] = accumulate ? Y[o] + acc : acc;
+    const R* const vals[1] = {val};
+    for (int cg = 0; cg < nc; cg += NP * 64) {
+        R acc[NP];
+        int cc[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            acc[q] = 0;
+            cc[q] = min(cg + q * 64 + lane, nc - 1);
+        }
+        row_entries<NP, 1>(b, e, col, vals, X, ldx, cc, acc, [&](const R(&w)[1], const R(&x)[NP], R(&a)[NP]) {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) a[q] += w[0] * x[q];
+        });
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const int c = cg + q * 64 + lane;
+            if (c < nc) {
+                const int64_t o = static_cast<int64_t>(row) * ldy + c;
+                Y[o] = accumulate ? Y[o] + acc[q] : acc[q];
+            }
+        }
     }
 }
 
@@ -191,78 +248,117 @@ __global__ void k_coarse_solve(int n, int nc, int ld, const R* __restrict__ inv,
     X[static_cast<int64_t>(i) * ld + c] = acc;
 }
 
+constexpr int MG_TT = 1024;  // threads of the transpose blocks
+constexpr int MG_LB = 4;     // staging loads in flight per thread
+
 // gather: Z[s][i] = sum_tn r_X[tn nf + s] W[tn][i]  (i < T)  and  Z[s][T + c] = r_Y[c nf + s].
-// A block owns 64 consecutive spatial rows: it reads the vector coalesced along
-// s into LDS and writes Z rows coalesced along the columns.
-template <typename R>
-__global__ void __launch_bounds__(256) k_mg_gather(int nf, int T, int nyc, int ld, int64_t nY, int th,
+// A block of MG_TT threads owns th consecutive spatial rows: it reads the
+// vector coalesced along s into an LDS tile (row stride th + 1), keeps W in LDS
+// too when WL, and writes Z rows coalesced along the columns.
+template <typename R, bool WL>
+__global__ void __launch_bounds__(MG_TT) k_mg_gather(int nf, int T, int nyc, int ld, int64_t nY, int th,
                                                    const double* __restrict__ r, const double* __restrict__ W,
                                                    R* __restrict__ Z)
 {
     extern __shared__ double lds_g[];
-    double* tile = lds_g;  // [ncol][th]
-    const int ncol = T + nyc;
+    const int ncol = T + nyc, ts = th + 1;
+    double* tile = lds_g;            // [ncol][th + 1]
+    double* Wl = lds_g + ncol * ts;  // [T][T] (WL)
     const int s0 = blockIdx.x * th;
     const int ns = min(th, nf - s0);
-    for (int e = threadIdx.x; e < ncol * th; e += blockDim.x) {
-        const int c = e / th, sl = e % th;
-        double v = 0.0;
-        if (sl < ns) v = (c < T) ? r[nY + static_cast<int64_t>(c) * nf + s0 + sl]
-                                 : r[static_cast<int64_t>(c - T) * nf + s0 + sl];
-        tile[c * th + sl] = v;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < ns * ncol; e += blockDim.x) {
-        const int sl = e / ncol, c = e % ncol;
-        double v;
-        if (c < T) {
-            v = 0.0;
-            for (int tn = 0; tn < T; ++tn) v += tile[tn * th + sl] * W[tn * T + c];
-        } else {
-            v = tile[c * th + sl];
+    // MG_LB elements per thread at a time: the loads are unconditional (indices
+    // clamped into range) so that they are all issued before the first store;
+    // rows past the last one store zero
+    const int nst = ncol * th;
+    for (int e0 = threadIdx.x; e0 < nst; e0 += MG_LB * MG_TT) {
+        double v[MG_LB];
+#pragma unroll
+        for (int u = 0; u < MG_LB; ++u) {
+            const int e = min(e0 + u * MG_TT, nst - 1);
+            const int c = e / th, sl = min(e % th, ns - 1);
+            v[u] = r[(c < T ? nY + static_cast<int64_t>(c) * nf : static_cast<int64_t>(c - T) * nf) + s0 + sl];
         }
+#pragma unroll
+        for (int u = 0; u < MG_LB; ++u) {
+            const int e = e0 + u * MG_TT;
+            if (e < nst) tile[(e / th) * ts + e % th] = (e % th < ns) ? v[u] : 0.0;
+        }
+    }
+    if (WL)
+        for (int e = threadIdx.x; e < T * T; e += blockDim.x) Wl[e] = W[e];
+    __syncthreads();
+    const double* Wp = WL ? Wl : W;
+    for (int e = threadIdx.x; e < ns * T; e += blockDim.x) {
+        const int sl = e / T, c = e % T;
+        double v = 0.0;
+#pragma unroll 8
+        for (int tn = 0; tn < T; ++tn) v += tile[tn * ts + sl] * Wp[tn * T + c];
         Z[static_cast<int64_t>(s0 + sl) * ld + c] = static_cast<R>(v);
+    }
+    for (int e = threadIdx.x; e < ns * nyc; e += blockDim.x) {
+        const int sl = e / nyc, c = e % nyc;
+        Z[static_cast<int64_t>(s0 + sl) * ld + T + c] = static_cast<R>(tile[(T + c) * ts + sl]);
     }
 }
 
 // scatter back: z_X[tn nf + s] = sum_i Z[s][i] W[tn][i] ; z_Y[(k Nt + l) nf + s] = sum_l' Minv_k[l][l'] Z[s][T + k Nt + l'],
-// and dot(z, r) partials (one per block).
-template <typename R>
-__global__ void __launch_bounds__(256) k_mg_scatter(int nf, int T, int Nt, int m, int ld, int64_t nY, int th,
+// and dot(z, r) partials (one per block).  Tiles as in k_mg_gather; the entries
+// of r are loaded before the products that they multiply.
+template <typename R, bool WL>
+__global__ void __launch_bounds__(MG_TT) k_mg_scatter(int nf, int T, int Nt, int m, int ld, int64_t nY, int th,
                                                     const R* __restrict__ Z, const double* __restrict__ W,
                                                     const double* __restrict__ MtInvH, const double* __restrict__ r,
                                                     double* __restrict__ z, double* __restrict__ part)
 {
     extern __shared__ double lds_s[];
-    const int ncol = T + m * Nt;
-    double* tile = lds_s;  // [ncol][th]
+    const int ncol = T + m * Nt, ts = th + 1;
+    double* tile = lds_s;            // [ncol][th + 1]
+    double* Wl = lds_s + ncol * ts;  // [T][T] (WL)
     const int s0 = blockIdx.x * th;
     const int ns = min(th, nf - s0);
-    for (int e = threadIdx.x; e < ns * ncol; e += blockDim.x) {
-        const int sl = e / ncol, c = e % ncol;
-        tile[c * th + sl] = static_cast<double>(Z[static_cast<int64_t>(s0 + sl) * ld + c]);
-    }
-    __syncthreads();
-    double dot = 0.0;
-    for (int e = threadIdx.x; e < ncol * th; e += blockDim.x) {
-        const int c = e / th, sl = e % th;
-        if (sl >= ns) continue;
-        int64_t idx;
-        double v = 0.0;
-        if (c < T) {
-            idx = nY + static_cast<int64_t>(c) * nf + s0 + sl;  // c plays tn
-            for (int i = 0; i < T; ++i) v += tile[i * th + sl] * W[c * T + i];
-        } else {
-            const int kl = c - T;
-            const int k = kl / Nt, l = kl % Nt;
-            idx = static_cast<int64_t>(kl) * nf + s0 + sl;
-            const double* Mi = MtInvH + (static_cast<int64_t>(k) * Nt + l) * Nt;
-            for (int q = 0; q < Nt; ++q) v += Mi[q] * tile[(T + k * Nt + q) * th + sl];
+    const int nst = ns * ncol;
+    for (int e0 = threadIdx.x; e0 < nst; e0 += MG_LB * MG_TT) {
+        double v[MG_LB];
+#pragma unroll
+        for (int u = 0; u < MG_LB; ++u) {
+            const int e = min(e0 + u * MG_TT, nst - 1);
+            v[u] = static_cast<double>(Z[static_cast<int64_t>(s0 + e / ncol) * ld + e % ncol]);
         }
-        z[idx] = v;
-        dot += v * r[idx];
+#pragma unroll
+        for (int u = 0; u < MG_LB; ++u) {
+            const int e = e0 + u * MG_TT;
+            if (e < nst) tile[(e % ncol) * ts + e / ncol] = v[u];
+        }
     }
-    __shared__ double sh[4];
+    if (WL)
+        for (int e = threadIdx.x; e < T * T; e += blockDim.x) Wl[e] = W[e];
+    __syncthreads();
+    const double* Wp = WL ? Wl : W;
+    double dot = 0.0;
+    for (int e = threadIdx.x; e < T * th; e += blockDim.x) {
+        const int c = e / th, sl = e % th;  // c plays tn
+        if (sl >= ns) continue;
+        const int64_t idx = nY + static_cast<int64_t>(c) * nf + s0 + sl;
+        const double rv = r[idx];
+        double v = 0.0;
+#pragma unroll 8
+        for (int i = 0; i < T; ++i) v += tile[i * ts + sl] * Wp[c * T + i];
+        z[idx] = v;
+        dot += v * rv;
+    }
+    for (int e = threadIdx.x; e < m * Nt * th; e += blockDim.x) {
+        const int kl = e / th, sl = e % th;
+        if (sl >= ns) continue;
+        const int k = kl / Nt, l = kl % Nt;
+        const int64_t idx = static_cast<int64_t>(kl) * nf + s0 + sl;
+        const double rv = r[idx];
+        const double* Mi = MtInvH + (static_cast<int64_t>(k) * Nt + l) * Nt;
+        double v = 0.0;
+        for (int q = 0; q < Nt; ++q) v += Mi[q] * tile[(T + k * Nt + q) * ts + sl];
+        z[idx] = v;
+        dot += v * rv;
+    }
+    __shared__ double sh[MG_TT / 64];
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = dot;
     __syncthreads();
@@ -271,6 +367,16 @@ __global__ void __launch_bounds__(256) k_mg_scatter(int nf, int T, int Nt, int m
         for (int w = 0; w < static_cast<int>(blockDim.x / 64); ++w) s += sh[w];
         part[blockIdx.x] = s;
     }
+}
+
+// Launch f with NP, the passes of 64 columns that cover nc columns (at most 4 per round).
+template <typename F>
+void with_passes(int nc, F f)
+{
+    if (nc <= 64) f(std::integral_constant<int, 1>{});
+    else if (nc <= 128) f(std::integral_constant<int, 2>{});
+    else if (nc <= 192) f(std::integral_constant<int, 3>{});
+    else f(std::integral_constant<int, 4>{});
 }
 
 template <typename R>
@@ -420,8 +526,11 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start_zero<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc,
                            ld, d.dK, d.dM, sig, static_cast<R>(1) / theta, d.B, d.X, d.Rr, d.D);
     } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_resid_start<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col,
-                           d.K, d.M, sig, nc, ld, d.X, d.B, d.Rr, d.D, d.dK, d.dM, static_cast<R>(1) / theta);
+        with_passes(nc, [&](auto np) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_resid_start<R, decltype(np)::value>), dim3(rows_grid), dim3(256), 0,
+                               st, d.nf, d.ptr, d.col, d.K, d.M, sig, nc, ld, d.X, d.B, d.Rr, d.D, d.dK, d.dM,
+                               static_cast<R>(1) / theta);
+        });
     }
     // the direction alternates between D and Tm
     R* dcur = d.D;
@@ -430,8 +539,11 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_
     for (int k = 0; k < nu; ++k) {
         const R rho_n = 1 / (2 * sigma1 - rho);
         const R c1 = rho_n * rho, c2 = 2 * rho_n / delta;
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_cheb_fused<R>), dim3(rows_grid), dim3(256), 0, st, d.nf, d.ptr, d.col,
-                           d.K, d.M, sig, nc, ld, dcur, d.X, d.Rr, dnext, d.dK, d.dM, c1, c2, (k == nu - 1) ? 1 : 0);
+        with_passes(nc, [&](auto np) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_cheb_fused<R, decltype(np)::value>), dim3(rows_grid), dim3(256), 0,
+                               st, d.nf, d.ptr, d.col, d.K, d.M, sig, nc, ld, dcur, d.X, d.Rr, dnext, d.dK, d.dM, c1, c2,
+                               (k == nu - 1) ? 1 : 0);
+        });
         std::swap(dcur, dnext);
         rho = rho_n;
     }
@@ -451,11 +563,15 @@ void MGDevice<R>::vcycle(hipStream_t st, int l, int nc, const R* sig, const int*
     }
     Level& c = levels[l + 1];
     smooth(st, l, nc, sig, true);  // leaves r = B - A x in Rr
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R>), dim3(grid_for(d.Rn, 4)), dim3(256), 0, st, d.Rn, d.Rptr, d.Rcol,
-                       d.Rval, nc, ld, d.Rr, ld, c.B, 0);
+    with_passes(nc, [&](auto np) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R, decltype(np)::value>), dim3(grid_for(d.Rn, 4)), dim3(256), 0, st,
+                           d.Rn, d.Rptr, d.Rcol, d.Rval, nc, ld, d.Rr, ld, c.B, 0);
+    });
     vcycle(st, l + 1, nc, sig, sidx);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R>), dim3(grid_for(d.Pn, 4)), dim3(256), 0, st, d.Pn, d.Pptr, d.Pcol,
-                       d.Pval, nc, ld, c.X, ld, d.X, 1);
+    with_passes(nc, [&](auto np) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R, decltype(np)::value>), dim3(grid_for(d.Pn, 4)), dim3(256), 0, st,
+                           d.Pn, d.Pptr, d.Pcol, d.Pval, nc, ld, c.X, ld, d.X, 1);
+    });
     smooth(st, l, nc, sig, false);
 }
 
@@ -468,29 +584,42 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
     const int64_t nel = static_cast<int64_t>(nf) * ncol;
     const int th = tile_rows();
     const unsigned tiles = static_cast<unsigned>((nf + th - 1) / th);
-    const size_t lds = static_cast<size_t>(ncol) * th * sizeof(double);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R>), dim3(tiles), dim3(256), lds, st, nf, T, nyc, ld, S.nY, th, r,
-                       d_W, f.B);
+    // the transpose tile, and W when it fits in 40 KiB
+    const bool wl = static_cast<size_t>(T) * T * sizeof(double) <= 40 * 1024;
+    const size_t lds = (static_cast<size_t>(ncol) * (th + 1) + (wl ? static_cast<size_t>(T) * T : 0)) * sizeof(double);
+    if (wl)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R, true>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, nyc, ld, S.nY,
+                           th, r, d_W, f.B);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R, false>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, nyc, ld,
+                           S.nY, th, r, d_W, f.B);
     // stage 1: all columns, Y columns with shift 0, X columns with shift s_i
     vcycle(st, 0, ncol, d_sig, d_sidx);
     // stage 2 (X columns): apply K, then the shifted V-cycle again
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, ncol, ld, f.X,
                        d_Z);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R>), dim3(grid_for(nf, 4)), dim3(256), 0, st, nf, f.ptr, f.col, f.K,
-                       f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
+    with_passes(T, [&](auto np) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R, decltype(np)::value>), dim3(grid_for(nf, 4)), dim3(256), 0, st,
+                           nf, f.ptr, f.col, f.K, f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
+    });
     vcycle(st, 0, T, d_sig, d_sidx);
     // X columns from the second V-cycle, Y columns from the first
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(static_cast<int64_t>(nf) * T, 256)), dim3(256), 0,
                        st, nf, T, ld, f.X, d_Z);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R>), dim3(tiles), dim3(256), lds, st, nf, T, S.Nt, S.mesh.m(),
-                       ld, S.nY, th, d_Z, d_W, d_MtInvH, r, z, part);
+    if (wl)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R, true>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, S.Nt,
+                           S.mesh.m(), ld, S.nY, th, d_Z, d_W, d_MtInvH, r, z, part);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R, false>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, S.Nt,
+                           S.mesh.m(), ld, S.nY, th, d_Z, d_W, d_MtInvH, r, z, part);
 }
 
 template <typename R>
 int MGDevice<R>::tile_rows() const
 {
-    // rows per transpose tile: the LDS tile ncol x th doubles stays within 64 KiB
-    int th = 65536 / (ncol * static_cast<int>(sizeof(double)));
+    // rows per transpose tile: the LDS tile ncol x (th + 1) doubles stays within
+    // 40 KiB, so with W beside it two blocks share a CU
+    int th = 40 * 1024 / (ncol * static_cast<int>(sizeof(double))) - 1;
     th = std::max(1, std::min(64, th));
     return th;
 }
