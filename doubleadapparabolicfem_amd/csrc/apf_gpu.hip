@@ -310,53 +310,65 @@ __global__ void k_element_scatter(int dim, int ns, int Ns, int p, int m, int nf,
 }
 
 // ============================================================== SpMV
-// CSR SpMV y = K x, LANES lanes per row, U independent (value, column, x)
-// load triples in flight per lane; values and column indices are streamed once
-// (non-temporal), x is gathered through the cache.
-// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8);
-// the remap gives every XCD one contiguous range of rows, so the x entries a
-// range gathers (a few spatial blocks of 8 nf bytes) stay in that XCD's L2.
-__device__ inline unsigned xcd_block(unsigned b, unsigned nblocks)
-{
-    const unsigned per = nblocks / 8u;  // nblocks is a multiple of 8
-    return (b % 8u) * per + b / 8u;
-}
-
-template <int LANES, int U, bool NT = true, bool XCD = true>
+// CSR SpMV y = K x.  A group of LANES lanes owns RP rows (rows gi, gi + G, ...
+// of the block's G*RP rows, G = groups per block); the first LANES*U entries of
+// all RP rows are loaded together, U (value, column) pairs per lane and row,
+// before any x is gathered, so a wave has RP row fetches in flight instead of
+// one.  Longer rows continue in LANES*U chunks.  Values and column indices are
+// streamed once, x is gathered through the cache.
+template <int LANES, int U, int RP>
 __global__ void __launch_bounds__(256) k_spmv(int64_t n, const int64_t* __restrict__ rowptr,
                                               const int* __restrict__ col, const double* __restrict__ val,
                                               const double* __restrict__ x, double* __restrict__ y)
 {
-    const unsigned lb = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int64_t row = (static_cast<int64_t>(lb) * blockDim.x + threadIdx.x) / LANES;
-    const int lane = threadIdx.x % LANES;
-    double sum = 0.0;
-    if (row < n) {
-        const int64_t b = rowptr[row];
-        const int len = static_cast<int>(rowptr[row + 1] - b);
-        const double* vr = val + b;
-        const int* cr = col + b;
-        for (int j0 = 0; j0 < len; j0 += LANES * U) {
-            double a[U];
-            int c[U];
+    constexpr int G = 256 / LANES;
+    const int lane = threadIdx.x % LANES, gi = threadIdx.x / LANES;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * (G * RP) + gi;
+    int64_t b[RP];
+    int len[RP];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int j = j0 + lane + u * LANES;
-                if (NT) {
-                    a[u] = (j < len) ? __builtin_nontemporal_load(vr + j) : 0.0;
-                    c[u] = (j < len) ? __builtin_nontemporal_load(cr + j) : 0;
-                } else {
-                    a[u] = (j < len) ? vr[j] : 0.0;
-                    c[u] = (j < len) ? cr[j] : 0;
-                }
-            }
+    for (int k = 0; k < RP; ++k) {
+        const int64_t row = base + k * G;
+        const int64_t rc = row < n ? row : n - 1;  // rows past the end: length 0
+        b[k] = rowptr[rc];
+        len[k] = row < n ? static_cast<int>(rowptr[rc + 1] - b[k]) : 0;
+    }
+    double sum[RP];
+    double a[RP][U];
+    int c[RP][U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) sum += a[u] * x[c[u]];
+    for (int k = 0; k < RP; ++k) {
+        sum[k] = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = lane + u * LANES;
+            a[k][u] = (j < len[k]) ? val[b[k] + j] : 0.0;
+            c[k][u] = (j < len[k]) ? col[b[k] + j] : 0;
         }
     }
 #pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, LANES);
-    if (row < n && lane == 0) y[row] = sum;
+    for (int k = 0; k < RP; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u) sum[k] += a[k][u] * x[c[k][u]];
+#pragma unroll
+    for (int k = 0; k < RP; ++k) {
+        for (int j0 = LANES * U; j0 < len[k]; j0 += LANES * U) {
+            double a2[U];
+            int c2[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = j0 + lane + u * LANES;
+                a2[u] = (j < len[k]) ? val[b[k] + j] : 0.0;
+                c2[u] = (j < len[k]) ? col[b[k] + j] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) sum[k] += a2[u] * x[c2[u]];
+        }
+#pragma unroll
+        for (int off = LANES / 2; off > 0; off >>= 1) sum[k] += __shfl_down(sum[k], off, LANES);
+        const int64_t row = base + k * G;
+        if (row < n && lane == 0) y[row] = sum[k];
+    }
 }
 
 // ============================================================== reductions
@@ -732,17 +744,17 @@ void System::spmv(const double* x, double* y)
 {
     const int64_t n = space->n;
     const int block = 256;
-    auto launch = [&](auto kern, int lanes) {
-        int64_t grid = (n * lanes + block - 1) / block;
-        grid = (grid + 7) / 8 * 8;  // whole XCD rounds for xcd_block
+    auto launch = [&](auto kern, int lanes, int rp) {
+        const int64_t rows_per_block = static_cast<int64_t>(block / lanes) * rp;
+        const int64_t grid = (n + rows_per_block - 1) / rows_per_block;
         hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, n, d_rowptr, d_col, d_val,
                            x, y);
     };
-    // 16 lanes per row and 4 independent loads in flight per lane measured
-    // fastest on gfx950 (plain loads; non-temporal hints and an XCD row remap
-    // were slower); 32 lanes is kept for rows that are much longer.
-    if (spmv_lanes == 32) launch(k_spmv<32, 4, false, false>, 32);
-    else launch(k_spmv<16, 4, false, false>, 16);
+    // Measured at 2+1D p=2 (61 entries per row on average): 16 lanes x 4 loads
+    // per row, two rows per lane group 0.606 ms; one row 0.625 ms; four rows
+    // 0.79 ms (registers); 8 lanes x 8 loads 0.71 ms.  Non-temporal loads and an
+    // XCD-contiguous row order were slower.
+    launch(k_spmv<16, 4, 2>, 16, 2);
 }
 
 }  // namespace apf
@@ -958,7 +970,6 @@ int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
         sys->ctx = ctx;
         sys->space = &s->S;
         sys->tables = &s->T;
-        if (const char* e = std::getenv("APF_SPMV_LANES")) sys->spmv_lanes = std::atoi(e);
         if (const char* e = std::getenv("APF_MINRES_GRAPH")) sys->use_graph = std::atoi(e) != 0;
         sys->H = s->H.coarse.empty() ? nullptr : &s->H;
         sys->stream = ctx->stream;

@@ -233,7 +233,6 @@ struct System {
     float t_factor_ms = 0.f, t_rhs_ms = 0.f, t_solve_ms = 0.f;
     bool assembled = false;
     bool use_graph = true;   // capture the MINRES iteration in a hipGraph
-    int spmv_lanes = 16;     // lanes per CSR row in k_spmv
 
     void spmv(const double* x, double* y);
 };
