@@ -123,6 +123,7 @@ EXPORTED = {
     "apf_system_mg_setup_seconds": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "apf_system_use_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "apf_bench_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "apf_bench_store": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_bench_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -258,6 +259,12 @@ class Context:
         """Copy-kernel bandwidth in GB/s (read + write)."""
         v = ctypes.c_double()
         _check(lib().apf_bench_stream(self.handle, int(nbytes), reps, ctypes.byref(v)), self.handle)
+        return v.value
+
+    def bench_store(self, nbytes=1 << 30, reps=10):
+        """Store-only kernel bandwidth in GB/s (bytes written)."""
+        v = ctypes.c_double()
+        _check(lib().apf_bench_store(self.handle, int(nbytes), reps, ctypes.byref(v)), self.handle)
         return v.value
 
     def __del__(self):

@@ -1488,6 +1488,47 @@ int apf_bench_stream(apf_ctx* ctx, int64_t bytes, int reps, double* gbs)
     }
 }
 
+__global__ void k_stream_store(int64_t n, double v, double* __restrict__ b)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        b[i] = v;
+}
+
+int apf_bench_store(apf_ctx* ctx, int64_t bytes, int reps, double* gbs)
+{
+    if (!ctx || !gbs || bytes < 8 || reps < 1) return set_error(ctx, APF_BAD_ARG, "apf_bench_store: bad argument");
+    double* b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int64_t n = bytes / 8;
+        HIP_TRY(hipMalloc(&b, n * sizeof(double)));
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        int cus = 0;
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        const unsigned grid = static_cast<unsigned>(cus) * 8u;
+        hipLaunchKernelGGL(k_stream_store, dim3(grid), dim3(256), 0, ctx->stream, n, 1.0, b);
+        HIP_TRY(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_store, dim3(grid), dim3(256), 0, ctx->stream, n, 1.0, b);
+        HIP_TRY(hipEventRecord(e1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *gbs = static_cast<double>(n) * 8.0 * reps / (ms * 1e-3) / 1e9;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(b);
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (b) (void)hipFree(b);
+        return set_error(ctx, APF_HIP_ERROR, std::string("apf_bench_store: ") + e.what());
+    }
+}
+
 int apf_system_use_graph(apf_system* sys, int on)
 {
     if (!sys) return set_error(nullptr, APF_BAD_ARG, "apf_system_use_graph: null system");

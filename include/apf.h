@@ -181,6 +181,10 @@ int apf_bench_spmv(apf_system* sys, int reps, double* ms_avg);
  * over `bytes` bytes, `reps` launches: the measured HBM ceiling and the
  * calibration kernel for counter-based traffic. */
 int apf_bench_stream(apf_ctx* ctx, int64_t bytes, int reps, double* gbs);
+/* Device store bandwidth (bytes written / s, in GB/s) of a plain store-only
+ * kernel over `bytes` bytes, `reps` launches: the measured ceiling of kernels
+ * whose only large stream is written, like the matrix fill of the assembly. */
+int apf_bench_store(apf_ctx* ctx, int64_t bytes, int reps, double* gbs);
 /* Capture the MINRES iteration in a hipGraph (default on) or launch eagerly. */
 int apf_system_use_graph(apf_system* sys, int on);
 /* Host seconds of the last multigrid setup (Galerkin operators, time eigenmodes). */
