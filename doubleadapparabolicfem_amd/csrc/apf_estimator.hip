@@ -666,12 +666,12 @@ __global__ void __launch_bounds__(256)
     // (rows 4 (l / 16) + r, columns l % 16) and the row-tile operand reads over
     // distinct banks; RS = 16 mod 64 does the same for the pivot-row operand.
     // The scaling vector lives in the pivot-row buffer while the matrix is
-    // scaled, then in the stride padding of each row (columns np..np+3, never
-    // read by the elimination).
+    // scaled, then in the stride padding of each row (column np, never read by
+    // the elimination).
     __shared__ __align__(16) float A[NPM * (NPM + 4)];
     __shared__ __align__(16) float Rb[16 * (NPM + 16)];
     __shared__ int pivrow[NPM], rinv[NPM];
-    double* dsc = reinterpret_cast<double*>(Rb);
+    float* dsc = Rb;
     const int q = q0 + blockIdx.x;
     const int a = topo.order[q];
     const int n = ps.n[q], np = ps.np[q];
@@ -682,30 +682,45 @@ __global__ void __launch_bounds__(256)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int nrt = np >> 4;
-    // symmetric equilibration D K D, d_i = 1 / sqrt(max_j |K_ij|): the flux,
-    // divergence and mean rows differ in scale by powers of h, and an FP32
-    // Gauss-Jordan inverse of the scaled matrix is several times more accurate
-    for (int i = wave; i < np; i += 4) {
-        float mx = 0.0f;
+    // symmetric equilibration D K D with d_i = 2^-floor(e_i / 2), e_i the binary
+    // exponent of max_j |K_ij| (so d_i^2 max_j |K_ij| lies in [1/2, 2)): the
+    // flux, divergence and mean rows differ in scale by powers of h, and an FP32
+    // Gauss-Jordan inverse of the scaled matrix is several times more accurate.
+    // Power-of-two factors make the scaling and the final unscaling exact in
+    // FP32.  Each wave reduces four rows (wave + 4u of a group of 16) at a time.
+    for (int i0 = wave; i0 < np; i0 += 16) {  // np is a multiple of 16
+        float mx[4];
 #pragma unroll
-        for (int t = 0; t < RPL; ++t) {
-            const int j = lane + 64 * t;
-            if (j < np) mx = fmaxf(mx, fabsf(A[i * S + j]));
+        for (int u = 0; u < 4; ++u) {
+            mx[u] = 0.0f;
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int j = lane + 64 * t;
+                if (j < np) mx[u] = fmaxf(mx[u], fabsf(A[(i0 + 4 * u) * S + j]));
+            }
         }
-        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) mx[u] = fmaxf(mx[u], __shfl_xor(mx[u], off, 64));
         if (lane == 0) {
-            dsc[i] = (mx > 0.0f) ? 1.0 / sqrt(static_cast<double>(mx)) : 1.0;
-            rinv[i] = i;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                int e = 0;
+                (void)frexpf(mx[u], &e);
+                dsc[i0 + 4 * u] = (mx[u] > 0.0f) ? ldexpf(1.0f, -(e >> 1)) : 1.0f;
+                rinv[i0 + 4 * u] = i0 + 4 * u;
+            }
         }
     }
     __syncthreads();
-    auto dkeep = [&](int i) -> double& { return *reinterpret_cast<double*>(&A[i * S + np]); };
+    auto dkeep = [&](int i) -> float& { return A[i * S + np]; };
     for (int i = wave; i < np; i += 4) {
-        const double di = dsc[i];
+        const float di = dsc[i];
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const int j = lane + 64 * t;
-            if (j < np) A[i * S + j] = static_cast<float>(di * static_cast<double>(A[i * S + j]) * dsc[j]);
+            if (j < np) A[i * S + j] *= di * dsc[j];
         }
         if (lane == 0) dkeep(i) = di;
     }
@@ -826,14 +841,14 @@ __global__ void __launch_bounds__(256)
     float* Kg = Ki + ps.off2[q];
     for (int i = wave; i < np; i += 4) {
         const int r = (i < n) ? rinv[i] : 0;
-        const double dr = (i < n) ? dkeep(r) : 0.0;
+        const float dr = (i < n) ? dkeep(r) : 0.0f;
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const int j = lane + 64 * t;
             if (j >= np) continue;
             if (i < n && j < n) {
                 const int c = pivrow[j];
-                Kg[static_cast<int64_t>(r) * np + c] = static_cast<float>(dr * static_cast<double>(A[i * S + j]) * dkeep(c));
+                Kg[static_cast<int64_t>(r) * np + c] = A[i * S + j] * (dr * dkeep(c));
             } else {
                 Kg[static_cast<int64_t>(i) * np + j] = 0.0f;
             }
