@@ -738,10 +738,8 @@ __global__ void __launch_bounds__(256)
             for (int s = 0; s < RPL; ++s)
 #pragma unroll
                 for (int c = 0; c < 16; ++c) Pn[s][c] = A[(lane + 64 * s) * S + k0 + c];
-            // fully unrolled: every column index below is a compile-time constant
-#pragma clang loop unroll(full)
-            for (int kk = 0; kk < 16; ++kk) {
-                if (kk >= kc) continue;
+            // one pivot step; kk is a compile-time constant in the unrolled loops below
+            auto pivot_step = [&](const int kk) __attribute__((always_inline)) {
                 // pivot: largest |A[i][k]| over unused rows, as one 32-bit key
                 // per candidate (|v| with its 7 low mantissa bits replaced by
                 // 127 - i: a pivot within 1 - 2^-16 of the largest is as good,
@@ -793,6 +791,17 @@ __global__ void __launch_bounds__(256)
                     pivrow[k0 + kk] = r;
                     rinv[r] = k0 + kk;
                 }
+            };
+            // full panels without per-step guards (straight-line code: the
+            // guarded loop costs ~40 register moves per step); the last panel
+            // of an order that is not a multiple of 16 is guarded
+            if (kc == 16) {
+#pragma clang loop unroll(full)
+                for (int kk = 0; kk < 16; ++kk) pivot_step(kk);
+            } else {
+#pragma clang loop unroll(full)
+                for (int kk = 0; kk < 16; ++kk)
+                    if (kk < kc) pivot_step(kk);
             }
 #pragma unroll
             for (int s = 0; s < RPL; ++s)
