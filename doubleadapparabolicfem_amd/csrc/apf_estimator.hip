@@ -277,6 +277,11 @@ constexpr int PATCH_THREADS = 256;
 // patch's pattern are written: the slot was zeroed when it was allocated and
 // no other entry is ever written.  With Alds, the same contributions also go to
 // an FP32 copy in LDS (row stride S).  Ends with a barrier.
+//
+// A flux-flux entry has two contributions only when both of its unknowns lie on
+// one edge of the patch (the two cells of that edge); two cells share at most
+// one edge, so every other entry comes from one cell and is stored directly.
+// Only the same-edge entries are zeroed first and summed with atomics.
 __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const PatchTopo& topo, int a, int n, int np,
                                    const double* verts, const int* cells, const double* Mcell, double* K,
                                    float* Alds = nullptr, int S = 0)
@@ -294,10 +299,13 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
         for (int u = 0; u < 4; ++u) gmH[ci][u] = g.H[u];
         gmH[ci][4] = g.det;
     }
-    // zero the flux-flux entries the atomic sums below accumulate into
-    for (int e = threadIdx.x; e < nc * Nf * Nf; e += blockDim.x) {
-        const int ci = cb + e / (Nf * Nf), w = e % (Nf * Nf);
-        const int i = w / Nf, j = w % Nf;
+    // local flux unknowns on the cell's edges: face i / Nfm for i < Nfc Nfm
+    const int nedge = P.Nfc * P.Nfm;
+    auto same_edge = [&](int i, int j) { return i < nedge && j < nedge && i / P.Nfm == j / P.Nfm; };
+    // zero the same-edge entries the atomic sums below accumulate into
+    for (int e = threadIdx.x; e < nc * nedge * P.Nfm; e += blockDim.x) {
+        const int ci = cb + e / (nedge * P.Nfm), w = e % (nedge * P.Nfm);
+        const int i = w / P.Nfm, j = (i / P.Nfm) * P.Nfm + w % P.Nfm;
         if (j < i) continue;
         const int fi = topo.fmap[static_cast<int64_t>(ci) * Nf + i], fj = topo.fmap[static_cast<int64_t>(ci) * Nf + j];
         if (fi < 0 || fj < 0) continue;
@@ -307,8 +315,8 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
     if (Alds)
         for (int e = threadIdx.x; e < np * S; e += blockDim.x) Alds[e] = 0.0f;
     __syncthreads();
-    // every cell at once: a flux-flux entry gets at most two contributions (the
-    // cells of an edge), so the FP64 atomic sums are exact-order independent
+    // every cell at once: a same-edge entry gets at most two contributions (the
+    // cells of the edge), so its FP64 atomic sum is exact-order independent
     const int per_cell = Nf * Nf + Nq * Nf + Nq;
     for (int e = threadIdx.x; e < nc * per_cell; e += blockDim.x) {
         const int ci = cb + e / per_cell, w = e % per_cell;
@@ -333,11 +341,20 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
             for (int ab = 0; ab < 4; ++ab)
                 if (ab < dim * dim) v += H[ab] * T.RTab[(static_cast<int64_t>(ab) * Nf + i) * Nf + j];
             v *= static_cast<double>(fs[i] * fs[j]);
-            atomicAdd(&K[static_cast<int64_t>(fi) * np + fj], v);
-            if (i != j) atomicAdd(&K[static_cast<int64_t>(fj) * np + fi], v);
-            if (Alds) {
-                atomicAdd(&Alds[fi * S + fj], static_cast<float>(v));
-                if (i != j) atomicAdd(&Alds[fj * S + fi], static_cast<float>(v));
+            if (same_edge(i, j)) {
+                atomicAdd(&K[static_cast<int64_t>(fi) * np + fj], v);
+                if (i != j) atomicAdd(&K[static_cast<int64_t>(fj) * np + fi], v);
+                if (Alds) {
+                    atomicAdd(&Alds[fi * S + fj], static_cast<float>(v));
+                    if (i != j) atomicAdd(&Alds[fj * S + fi], static_cast<float>(v));
+                }
+            } else {
+                K[static_cast<int64_t>(fi) * np + fj] = v;
+                K[static_cast<int64_t>(fj) * np + fi] = v;
+                if (Alds) {
+                    Alds[fi * S + fj] = static_cast<float>(v);
+                    Alds[fj * S + fi] = static_cast<float>(v);
+                }
             }
         } else if (w < Nf * Nf + Nq * Nf) {
             // divergence coupling int q_i div s_j (orthonormal scalars)
