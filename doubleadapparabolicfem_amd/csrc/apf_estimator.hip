@@ -755,7 +755,10 @@ __global__ void __launch_bounds__(256)
             for (int s = 0; s < RPL; ++s)
 #pragma unroll
                 for (int c = 0; c < 16; ++c) Pn[s][c] = A[(lane + 64 * s) * S + k0 + c];
-            // one pivot step; kk is a compile-time constant in the unrolled loops below
+            // one pivot step, branch-free so that the unrolled panel is one basic
+            // block (step kk + 1's pivot search can overlap step kk's updates); kk
+            // is a compile-time constant in the unrolled loops below
+            int prs[16];  // pivot row of each step (wave-uniform)
             auto pivot_step = [&](const int kk) __attribute__((always_inline)) {
                 // pivot: largest |A[i][k]| over unused rows, as one 32-bit key
                 // per candidate (|v| with its 7 low mantissa bits replaced by
@@ -766,7 +769,7 @@ __global__ void __launch_bounds__(256)
                 for (int s = 0; s < RPL; ++s) {
                     const unsigned kv =
                         (__float_as_uint(fabsf(Pn[s][kk])) & ~0x7Fu) | static_cast<unsigned>(127 - (lane + 64 * s));
-                    if (!(used & (1u << s))) key = max(key, kv);
+                    key = (used & (1u << s)) ? key : max(key, kv);
                 }
                 key = wave_umax(key);
                 const int r = min(max(127 - static_cast<int>(key & 0x7Fu), 0), n - 1);
@@ -774,40 +777,29 @@ __global__ void __launch_bounds__(256)
                 // the pivot row, scaled by the reciprocal pivot (hardware
                 // reciprocal: the FP64 refinement absorbs its last-bit error)
                 float prow[16];
-                if (RPL == 1 || sr == 0) {
 #pragma unroll
-                    for (int c = 0; c < 16; ++c)
-                        prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Pn[0][c]), owner));
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 16; ++c)
-                        prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Pn[RPL - 1][c]), owner));
+                for (int c = 0; c < 16; ++c) {
+                    const float src = (RPL == 1 || sr == 0) ? Pn[0][c] : Pn[RPL - 1][c];
+                    prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src), owner));
                 }
                 const float inv = __builtin_amdgcn_rcpf(prow[kk]);
 #pragma unroll
                 for (int c = 0; c < 16; ++c) prow[c] = (c == kk) ? inv : prow[c] * inv;
                 // every row eliminates (padding rows are zero and stay zero),
                 // then the owner lane puts the scaled pivot row in place
+                const bool mine = lane == owner;
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) {
                     const float f = Pn[s][kk];
+                    const bool w = mine && (RPL == 1 || s == sr);
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) Pn[s][c] = ((c == kk) ? 0.0f : Pn[s][c]) - f * prow[c];
-                }
-                if (lane == owner) {
-                    if (RPL == 1 || sr == 0) {
-#pragma unroll
-                        for (int c = 0; c < 16; ++c) Pn[0][c] = prow[c];
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < 16; ++c) Pn[RPL - 1][c] = prow[c];
+                    for (int c = 0; c < 16; ++c) {
+                        const float e = ((c == kk) ? 0.0f : Pn[s][c]) - f * prow[c];
+                        Pn[s][c] = w ? prow[c] : e;
                     }
                 }
-                if (lane == owner) used |= 1u << sr;
-                if (lane == 0) {
-                    pivrow[k0 + kk] = r;
-                    rinv[r] = k0 + kk;
-                }
+                used |= (mine ? 1u : 0u) << sr;
+                prs[kk] = r;
             };
             // full panels without per-step guards (straight-line code: the
             // guarded loop costs ~40 register moves per step); the last panel
@@ -819,6 +811,15 @@ __global__ void __launch_bounds__(256)
 #pragma clang loop unroll(full)
                 for (int kk = 0; kk < 16; ++kk)
                     if (kk < kc) pivot_step(kk);
+            }
+            // lane kk records step kk's pivot row
+            int myr = 0;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                if (kk < kc) myr = (lane == kk) ? prs[kk] : myr;
+            if (lane < kc) {
+                pivrow[k0 + lane] = myr;
+                rinv[myr] = k0 + lane;
             }
 #pragma unroll
             for (int s = 0; s < RPL; ++s)
