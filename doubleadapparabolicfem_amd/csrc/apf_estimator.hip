@@ -1278,28 +1278,69 @@ __global__ void __launch_bounds__(RHS_THREADS)
     int* c_lv = reinterpret_cast<int*>(geo + nc_max * 4);                 // [nc]
     int* c_fmap = c_lv + nc_max;                                          // [nc][NF]
     int* c_fsign = c_fmap + nc_max * NF;                                  // [nc][NF]
-    for (int e = threadIdx.x; e < NV * NF * NS; e += blockDim.x) Rt[e] = T.R[e];
-    for (int e = threadIdx.x; e < NV * NQ * NS; e += blockDim.x) Qt[e] = T.Q[e];
-    for (int e = threadIdx.x; e < DIM * NQ * NS; e += blockDim.x) Et[e] = T.E[e];
-    for (int e = threadIdx.x; e < NV * NQ; e += blockDim.x) Pt[e] = T.Pv[e];
-    for (int e = threadIdx.x; e < NT * NT; e += blockDim.x) {
-        Wt[e] = T.W[e];
-        Wdt[e] = T.Wd[e];
+    // the reference tables and the step sizes: every load is issued before the
+    // first store (indices clamped, sizes known at compile time), so the
+    // prologue costs one memory latency instead of one per table
+    {
+        constexpr int NR = NV * NF * NS, NQS = NV * NQ * NS, NE = DIM * NQ * NS, NPV = NV * NQ, NW = NT * NT;
+        constexpr int UR = (NR + RHS_THREADS - 1) / RHS_THREADS, UQ = (NQS + RHS_THREADS - 1) / RHS_THREADS;
+        constexpr int UE = (NE + RHS_THREADS - 1) / RHS_THREADS;
+        static_assert(NPV <= RHS_THREADS && NW <= RHS_THREADS && RMAX * NT <= RHS_THREADS, "one entry per thread");
+        const int t = threadIdx.x;
+        double vR[UR], vQ[UQ], vE[UE];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) vR[u] = T.R[min(t + u * RHS_THREADS, NR - 1)];
+#pragma unroll
+        for (int u = 0; u < UQ; ++u) vQ[u] = T.Q[min(t + u * RHS_THREADS, NQS - 1)];
+#pragma unroll
+        for (int u = 0; u < UE; ++u) vE[u] = T.E[min(t + u * RHS_THREADS, NE - 1)];
+        const double vP = T.Pv[min(t, NPV - 1)];
+        const double vW = T.W[min(t, NW - 1)], vWd = T.Wd[min(t, NW - 1)];
+        // step size and mode scale 1 / (h |L_kk|^2) of slab sl = t / NT of the group
+        const int sl = min(t / NT, nsl - 1), kk = t % NT;
+        const double t0 = times[k0 + sl], t1 = times[k0 + sl + 1], ln = T.Lnorm[kk];
+#pragma unroll
+        for (int u = 0; u < UR; ++u)
+            if (t + u * RHS_THREADS < NR) Rt[t + u * RHS_THREADS] = vR[u];
+#pragma unroll
+        for (int u = 0; u < UQ; ++u)
+            if (t + u * RHS_THREADS < NQS) Qt[t + u * RHS_THREADS] = vQ[u];
+#pragma unroll
+        for (int u = 0; u < UE; ++u)
+            if (t + u * RHS_THREADS < NE) Et[t + u * RHS_THREADS] = vE[u];
+        if (t < NPV) Pt[t] = vP;
+        if (t < NW) {
+            Wt[t] = vW;
+            Wdt[t] = vWd;
+        }
+        if (t < NT) Ln[t] = ln;
+        if (t < nsl * NT) {
+            const double h = t1 - t0;
+            if (kk == 0) s_h[sl] = h;
+            s_scale[t] = 1.0 / (h * ln);
+        }
     }
-    for (int e = threadIdx.x; e < NT; e += blockDim.x) Ln[e] = T.Lnorm[e];
-    // step size and mode scale 1 / (h |L_kk|^2) per slab of the group
-    for (int e = threadIdx.x; e < nsl * NT; e += blockDim.x) {
-        const int sl = e / NT, kk = e - sl * NT;
-        const double h = times[k0 + sl + 1] - times[k0 + sl];
-        if (kk == 0) s_h[sl] = h;
-        s_scale[e] = 1.0 / (h * T.Lnorm[kk]);
+    // the patch topology: the first RHS_THREADS entries of each map in one
+    // batch of loads (indices clamped; a patch has at least one cell), the rest
+    // (patches of more than RHS_THREADS / NF cells) by the loops after it
+    {
+        const int t = threadIdx.x, nmap = nc * NF;
+        const int64_t mb = static_cast<int64_t>(cb) * NF;
+        const int fm = topo.fmap[mb + min(t, nmap - 1)];
+        const int fs = topo.fsign[mb + min(t, nmap - 1)];
+        const int lv = topo.lv[cb + min(t, nc - 1)];
+        for (int e = t; e < np * wcol; e += blockDim.x) acc[e] = 0.0;
+        if (t < nmap) {
+            c_fmap[t] = fm;
+            c_fsign[t] = fs;
+        }
+        if (t < nc) c_lv[t] = lv;
+        for (int e = t + RHS_THREADS; e < nmap; e += RHS_THREADS) {
+            c_fmap[e] = topo.fmap[mb + e];
+            c_fsign[e] = topo.fsign[mb + e];
+        }
+        for (int ci = t + RHS_THREADS; ci < nc; ci += RHS_THREADS) c_lv[ci] = topo.lv[cb + ci];
     }
-    for (int e = threadIdx.x; e < np * wcol; e += blockDim.x) acc[e] = 0.0;
-    for (int e = threadIdx.x; e < nc * NF; e += blockDim.x) {
-        c_fmap[e] = topo.fmap[static_cast<int64_t>(cb) * NF + e];
-        c_fsign[e] = topo.fsign[static_cast<int64_t>(cb) * NF + e];
-    }
-    for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) c_lv[ci] = topo.lv[cb + ci];
     for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) {
         const int c = topo.cells[cb + ci];
         const int v = topo.lv[cb + ci];
