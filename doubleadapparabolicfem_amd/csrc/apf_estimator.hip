@@ -1713,72 +1713,36 @@ __global__ void __launch_bounds__(128)
     const int tid = threadIdx.x;
     const int ncols = P.m * NT;
     const Metric2 g = metric_of(DIM, c, verts, cells, Mcell);
-    // the cell's blocks sum_ab metric_ab table_ab[e]: four entries per thread
-    // per round, all their table loads issued before the products (indices
-    // clamped), so that the prologue waits on few memory latencies
-    constexpr int D2 = DIM * DIM;
-    auto metric_block = [&](double* dst, const int nel, const double* coef, auto tab) __attribute__((always_inline)) {
-        for (int e0 = tid; e0 < nel; e0 += 4 * 128) {
-            double v[4][D2];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = min(e0 + u * 128, nel - 1);
-#pragma unroll
-                for (int ab = 0; ab < D2; ++ab) v[u][ab] = tab(e, ab);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = e0 + u * 128;
-                if (e < nel) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int ab = 0; ab < D2; ++ab) acc += coef[ab] * v[u][ab];
-                    dst[e] = acc;
-                }
-            }
-        }
-    };
-    double cH[D2], cG[D2];
-#pragma unroll
-    for (int ab = 0; ab < D2; ++ab) {
-        cH[ab] = g.H[(ab / DIM) * 2 + (ab % DIM)];
-        cG[ab] = g.G[(ab / DIM) * 2 + (ab % DIM)];
+    for (int e = tid; e < NF * NF; e += blockDim.x) {
+        const int i = e / NF, j = e % NF;
+        double a = 0.0;
+        for (int ab = 0; ab < DIM * DIM; ++ab)
+            a += g.H[(ab / DIM) * 2 + (ab % DIM)] * T.RTab[(static_cast<int64_t>(ab) * NF + i) * NF + j];
+        Ac[e] = a;
     }
-    metric_block(Ac, NF * NF, cH, [&](int e, int ab) {
-        return T.RTab[(static_cast<int64_t>(ab) * NF + e / NF) * NF + e % NF];
-    });
-    metric_block(IG, NF * NS, cG, [&](int e, int ab) {
-        return T.Igrad[(static_cast<int64_t>(e / NS) * D2 + ab) * NS + e % NS];
-    });
-    metric_block(Kc, NS * NS, cG, [&](int e, int ab) {
-        return T.Khat[(static_cast<int64_t>(ab) * NS + e / NS) * NS + e % NS];
-    });
-    // the small tables, one pass of loads before the stores
-    {
-        constexpr int NW = NT * NT, NPH = NQ * NS, NBH = NQ * NF;
-        constexpr int UPH = (NPH + 127) / 128, UBH = (NBH + 127) / 128;
-        static_assert(NW <= 128 && NQ <= 128, "one entry per thread");
-        const double vW = T.W[min(tid, NW - 1)], vWd = T.Wd[min(tid, NW - 1)], vMt = T.Mt[min(tid, NW - 1)];
-        const double vL = T.Lnorm[min(tid, NT - 1)], vq = T.qbar[min(tid, NQ - 1)];
-        double vP[UPH], vB[UBH];
-#pragma unroll
-        for (int u = 0; u < UPH; ++u) vP[u] = T.Phi[min(tid + u * 128, NPH - 1)];
-#pragma unroll
-        for (int u = 0; u < UBH; ++u) vB[u] = T.Bhat[min(tid + u * 128, NBH - 1)];
-        if (tid < NW) {
-            Wt[tid] = vW;
-            Wdt[tid] = vWd;
-            Mtt[tid] = vMt;
-        }
-        if (tid < NT) Ln[tid] = vL;
-        if (tid < NQ) qb[tid] = vq;
-#pragma unroll
-        for (int u = 0; u < UPH; ++u)
-            if (tid + u * 128 < NPH) Ph[tid + u * 128] = vP[u];
-#pragma unroll
-        for (int u = 0; u < UBH; ++u)
-            if (tid + u * 128 < NBH) Bh[tid + u * 128] = vB[u];
+    for (int e = tid; e < NF * NS; e += blockDim.x) {
+        const int i = e / NS, mm = e % NS;
+        double a = 0.0;
+        for (int ab = 0; ab < DIM * DIM; ++ab)
+            a += g.G[(ab / DIM) * 2 + (ab % DIM)] * T.Igrad[(static_cast<int64_t>(i) * DIM * DIM + ab) * NS + mm];
+        IG[e] = a;
     }
+    for (int e = tid; e < NS * NS; e += blockDim.x) {
+        const int i = e / NS, j = e % NS;
+        double a = 0.0;
+        for (int ab = 0; ab < DIM * DIM; ++ab)
+            a += g.G[(ab / DIM) * 2 + (ab % DIM)] * T.Khat[(static_cast<int64_t>(ab) * NS + i) * NS + j];
+        Kc[e] = a;
+    }
+    for (int e = tid; e < NT * NT; e += blockDim.x) {
+        Wt[e] = T.W[e];
+        Wdt[e] = T.Wd[e];
+        Mtt[e] = T.Mt[e];
+    }
+    for (int e = tid; e < NT; e += blockDim.x) Ln[e] = T.Lnorm[e];
+    for (int e = tid; e < NQ * NS; e += blockDim.x) Ph[e] = T.Phi[e];
+    for (int e = tid; e < NQ * NF; e += blockDim.x) Bh[e] = T.Bhat[e];
+    for (int e = tid; e < NQ; e += blockDim.x) qb[e] = T.qbar[e];
     for (int k0 = 0; k0 < P.m; k0 += KC) {
         const int nsl = min(KC, P.m - k0);
         __syncthreads();
