@@ -100,3 +100,24 @@ def test_slab_reconstruction_solve_matches_oracle(ctx, dim, p, outer):
         assert _rel(est[key], rv) <= 1e-10, (key, est[key], rv)
     rc = est["recon_by_cell"]
     assert np.all(np.abs(rc - ref.recon_by_cell) <= 1e-9 * np.abs(ref.recon_by_cell) + 1e-13 * ref.recon2)
+
+
+@pytest.mark.parametrize("prec", [apf.APF_FP64, apf.APF_FP32_REFINE], ids=["fp64", "fp32ref"])
+@pytest.mark.parametrize("dim,p,outer", [(1, 2, 3), (2, 2, 2), (2, 3, 1)])
+def test_repeated_estimates_are_identical(ctx, dim, p, outer, prec):
+    """A system's first estimate in a precision also times its tile candidates
+    (column blockings of the solve, slab groupings of the right-hand sides);
+    later estimates run the chosen tiles only.  All calls give the same per-cell
+    values bit for bit: the tiling changes no sum order."""
+    print(f"estimate overlap dim={dim} p={p} outer={outer} prec={prec}", flush=True)
+    m = meshes.uniform_mesh(dim, outer)
+    sp = apf.Space(m, p, coarse=meshes.uniform_hierarchy(dim, outer))
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(meshes.smooth_initial(dim))
+    sysm.minres(tol=1e-10, max_it=600, check_interval=10)
+    runs = [sysm.estimate(prec, 2) for _ in range(3)]
+    for r in runs[1:]:
+        for key in ("flux_by_cell", "resid_by_cell", "recon_by_cell"):
+            assert np.array_equal(r[key], runs[0][key]), key
+        for key in ("eta2", "flux2", "lambda_y2"):
+            assert r[key] == runs[0][key], key
