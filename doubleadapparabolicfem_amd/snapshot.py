@@ -1,0 +1,124 @@
+"""Binary snapshots in the reference's file formats.
+
+The writers below produce the files that the reference's readers and plotters
+load: the mesh file of ``load_mesh_binary`` (python/binary_readers.py:90-205,
+written by src/finite_element/fespace/io/write_mesh_binary.hpp:16-87), and the
+per-cell estimator files (src/finite_element/io/write_cellwise_error_binary.hpp:25-78
+and write_cellwise_error_components_binary.hpp:14-87).  Every file starts with an
+int32 header whose first entry is the format version
+(src/finite_element/io/detail/binary_format_versions.hpp:10-22,
+``make_versioned_header`` at :28-40); the blocks that follow are raw
+little-endian arrays, int32 for ids and float64 for coordinates and values.
+
+Cell ids: the space-time cell of slab k and spatial cell c has id k * ns + c,
+the order of this package's per-cell estimator arrays.  (The reference numbers
+the active cells of its refinement forest; on the tensor meshes here these ids
+take that role.)  Spatial face f of a triangle is the edge opposite vertex f,
+of an interval its vertex f; temporal face 0 is a cell's bottom (t_k) and face 1
+its top.  A face without a neighbour lists -1, as the reference's
+``flatten_neighbours`` does (write_mesh_binary_detail.hpp:149-186).
+"""
+import numpy as np
+
+MESH_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:10
+CELLWISE_ERROR_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:20
+CELLWISE_ERROR_COMPONENTS_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:21
+
+_TRI_FACE = ((1, 2), (2, 0), (0, 1))  # face f of a triangle: the edge opposite vertex f
+
+
+def _write(path, blocks):
+    with open(path, "wb") as f:
+        for b in blocks:
+            np.ascontiguousarray(b).tofile(f)
+
+
+def _mesh_arrays(mesh):
+    dim = int(mesh.dim)
+    V = np.asarray(mesh.vertices, dtype=np.float64).reshape(-1, dim)
+    C = np.asarray(mesh.cells, dtype=np.int32).reshape(-1, dim + 1)
+    T = np.asarray(mesh.times if mesh.times is not None else [0.0, 1.0], dtype=np.float64)
+    return dim, V, C, T
+
+
+def cell_ids(mesh):
+    """Ids of the space-time cells in this package's order (slab-major)."""
+    dim, V, C, T = _mesh_arrays(mesh)
+    return np.arange(len(C) * (len(T) - 1), dtype=np.int32)
+
+
+def _spatial_face_neighbours(dim, C):
+    """[ns][faces] index of the spatial cell across each face, -1 on the boundary."""
+    ns, nfc = len(C), dim + 1
+    key_of = {}
+    for c in range(ns):
+        for f in range(nfc):
+            key = (int(C[c, f]),) if dim == 1 else tuple(sorted((int(C[c, _TRI_FACE[f][0]]), int(C[c, _TRI_FACE[f][1]]))))
+            key_of.setdefault(key, []).append((c, f))
+    nb = np.full((ns, nfc), -1, dtype=np.int64)
+    for owners in key_of.values():
+        if len(owners) > 2:
+            raise ValueError("a face shared by more than two cells")
+        if len(owners) == 2:
+            (c1, f1), (c2, f2) = owners
+            nb[c1, f1] = c2
+            nb[c2, f2] = c1
+    return nb
+
+
+def write_mesh_binary(mesh, path):
+    """Mesh file (format version 1) of a tensor space-time mesh."""
+    dim, V, C, T = _mesh_arrays(mesh)
+    ns, m = len(C), len(T) - 1
+    ncell = ns * m
+    tp, ip, nfs, nft = dim + 1, 2, dim + 1, 2
+    nb = _spatial_face_neighbours(dim, C)
+    max_sn = 1 if (nb >= 0).any() else 0
+    max_tn = 1 if m > 1 else 0
+    k = np.repeat(np.arange(m), ns)
+    c = np.tile(np.arange(ns), m)
+    spatial_vids = C[c]
+    temporal_vids = np.stack([k, k + 1], axis=1)
+    sn = np.where(nb[c] >= 0, k[:, None] * ns + nb[c], -1).reshape(ncell, nfs, 1)[:, :, :max_sn]
+    below = np.where(k > 0, (k - 1) * ns + c, -1)
+    above = np.where(k < m - 1, (k + 1) * ns + c, -1)
+    tn = np.stack([below, above], axis=1).reshape(ncell, nft, 1)[:, :, :max_tn]
+    header = [MESH_BINARY_FORMAT_VERSION, len(V), m + 1, dim, 1, ncell, dim, tp, tp, ip, nfs, max_sn, nft, max_tn]
+    _write(path, [np.asarray(header, dtype=np.int32), V, T.reshape(-1, 1), np.arange(ncell, dtype=np.int32),
+                  spatial_vids.astype(np.int32), temporal_vids.astype(np.int32), sn.astype(np.int32),
+                  tn.astype(np.int32)])
+
+
+def _order(ids, total, sort_descending):
+    """Entry order: value descending, ties to the lower id
+    (CellwiseSquaredError::sorted_descending, time_slab_cellwise_errors.hpp:54-70);
+    otherwise ascending id (the reference iterates an unordered map there)."""
+    if sort_descending:
+        return np.lexsort((ids, -total))
+    return np.argsort(ids, kind="stable")
+
+
+def write_cellwise_error_binary(values, path, sort_descending=True, ids=None):
+    """Per-cell squared errors (format version 1): header [version, n], ids, values."""
+    values = np.asarray(values, dtype=np.float64).reshape(-1)
+    ids = np.arange(len(values), dtype=np.int64) if ids is None else np.asarray(ids, dtype=np.int64)
+    o = _order(ids, values, sort_descending)
+    _write(path, [np.asarray([CELLWISE_ERROR_BINARY_FORMAT_VERSION, len(values)], dtype=np.int32),
+                  ids[o].astype(np.int32), values[o]])
+
+
+def write_cellwise_error_components_binary(estimate, path, sort_descending=True):
+    """Per-cell estimator components (format version 1) of an ``System.estimate``
+    result with per-cell arrays: header [version, n, 4], ids, flux^2,
+    reconstruction^2, divergence residual^2 and the total flux^2 + reconstruction^2
+    (the combination of apf_combine_estimator), ordered by the total."""
+    from . import combine_estimator
+
+    flux = np.asarray(estimate["flux_by_cell"], dtype=np.float64)
+    recon = np.asarray(estimate["recon_by_cell"], dtype=np.float64)
+    resid = np.asarray(estimate["resid_by_cell"], dtype=np.float64)
+    total = combine_estimator(flux, recon, resid)
+    ids = np.arange(len(total), dtype=np.int64)
+    o = _order(ids, total, sort_descending)
+    _write(path, [np.asarray([CELLWISE_ERROR_COMPONENTS_BINARY_FORMAT_VERSION, len(total), 4], dtype=np.int32),
+                  ids[o].astype(np.int32), flux[o], recon[o], resid[o], total[o]])

@@ -121,3 +121,31 @@ def test_repeated_estimates_are_identical(ctx, dim, p, outer, prec):
             assert np.array_equal(r[key], runs[0][key]), key
         for key in ("eta2", "flux2", "lambda_y2"):
             assert r[key] == runs[0][key], key
+
+
+def test_estimate_snapshot_in_the_reference_format(ctx, tmp_path):
+    """The per-cell estimator components of a GPU estimate, written in the
+    reference's cellwise-components format and read back, sum to the
+    estimate's totals."""
+    from doubleadapparabolicfem_amd import snapshot
+
+    print("estimate snapshot", flush=True)
+    m = meshes.uniform_mesh(2, 1)
+    sp = apf.Space(m, 2, coarse=meshes.uniform_hierarchy(2, 1))
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(meshes.smooth_initial(2))
+    sysm.minres(tol=1e-10, max_it=600, check_interval=10)
+    est = sysm.estimate(apf.APF_FP32_REFINE, 2)
+    path = tmp_path / "cellwise_error_components.bin"
+    snapshot.write_cellwise_error_components_binary(est, path)
+    with open(path, "rb") as f:
+        h = np.fromfile(f, dtype=np.int32, count=3)
+        n = int(h[1])
+        ids = np.fromfile(f, dtype=np.int32, count=n)
+        flux, recon, resid, total = (np.fromfile(f, dtype=np.float64, count=n) for _ in range(4))
+    assert h[0] == 1 and h[2] == 4 and n == sp.n_space_cells * sp.n_intervals
+    assert sorted(ids.tolist()) == list(range(n))
+    assert _rel(flux.sum(), est["flux2"]) < 1e-12
+    assert _rel(recon.sum(), est["recon2"]) < 1e-12
+    assert _rel(total.sum(), est["y_est2"]) < 1e-12
+    assert np.all(np.diff(total) <= 0)

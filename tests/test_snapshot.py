@@ -1,0 +1,149 @@
+"""Binary snapshots in the reference's formats: headers, block sizes and the
+meaning of every block, read back the way the reference's reader parses them
+(python/binary_readers.py:42-63 versioned header, :90-205 mesh file; the
+per-cell files as write_cellwise_error_binary.hpp:54-77 and
+write_cellwise_error_components_binary.hpp:36-86 write them)."""
+import numpy as np
+import pytest
+
+from doubleadapparabolicfem_amd import combine_estimator, meshes, snapshot
+
+
+def _header(f, payload, version):
+    h = np.fromfile(f, dtype=np.int32, count=payload + 1)
+    assert h.size == payload + 1
+    assert h[0] == version
+    return h[1:]
+
+
+def _block(f, dtype, count, shape=None):
+    a = np.fromfile(f, dtype=dtype, count=count)
+    assert a.size == count
+    return a.reshape(shape) if shape is not None else a
+
+
+def read_mesh(path):
+    with open(path, "rb") as f:
+        h = _header(f, 13, 1)
+        nsv, ntv, ds, dt, nc, vsf, vtf, tp, ip, nfs, msn, nft, mtn = (int(v) for v in h)
+        out = dict(header=h,
+                   spatial_vertices=_block(f, np.float64, nsv * ds, (nsv, ds)),
+                   temporal_vertices=_block(f, np.float64, ntv * dt, (ntv, dt)),
+                   cell_ids=_block(f, np.int32, nc),
+                   spatial_ids=_block(f, np.int32, nc * tp, (nc, tp)),
+                   temporal_ids=_block(f, np.int32, nc * ip, (nc, ip)),
+                   spatial_neigh=_block(f, np.int32, nc * nfs * msn, (nc, nfs, msn)),
+                   temporal_neigh=_block(f, np.int32, nc * nft * mtn, (nc, nft, mtn)))
+        assert np.fromfile(f, dtype=np.uint8).size == 0  # no trailing bytes
+    return out
+
+
+def test_format_versions_are_the_reference_ones():
+    # binary_format_versions.hpp:10, :20, :21
+    assert snapshot.MESH_BINARY_FORMAT_VERSION == 1
+    assert snapshot.CELLWISE_ERROR_BINARY_FORMAT_VERSION == 1
+    assert snapshot.CELLWISE_ERROR_COMPONENTS_BINARY_FORMAT_VERSION == 1
+
+
+@pytest.mark.parametrize("case", ["uj2", "interval", "fan"])
+def test_mesh_file_describes_the_space_time_mesh(tmp_path, case):
+    if case == "uj2":
+        m = meshes.union_jack_mesh(2, np.linspace(0.0, 1.0, 4))
+    elif case == "interval":
+        m = meshes.uniform_mesh(1, 2)
+    else:
+        m = meshes.fan_mesh(2, meshes.graded_times(1, 3), seed=5)
+    path = tmp_path / "mesh.bin"
+    snapshot.write_mesh_binary(m, path)
+    d = read_mesh(path)
+    dim = m.dim
+    V = np.asarray(m.vertices).reshape(-1, dim)
+    C = np.asarray(m.cells).reshape(-1, dim + 1)
+    T = np.asarray(m.times)
+    ns, mm = len(C), len(T) - 1
+    nsv, ntv, ds, dt, nc, vsf, vtf, tp, ip, nfs, msn, nft, mtn = (int(v) for v in d["header"])
+    assert (nsv, ntv, ds, dt, nc) == (len(V), mm + 1, dim, 1, ns * mm)
+    assert (vsf, vtf, tp, ip, nfs, nft) == (dim, dim + 1, dim + 1, 2, dim + 1, 2)
+    assert np.array_equal(d["spatial_vertices"], V)
+    assert np.array_equal(d["temporal_vertices"][:, 0], T)
+    assert np.array_equal(d["cell_ids"], np.arange(ns * mm))
+    # cell k * ns + c: the vertices of spatial cell c over [t_k, t_k+1]
+    ids = np.arange(ns * mm)
+    k, c = ids // ns, ids % ns
+    assert np.array_equal(d["spatial_ids"], C[c])
+    assert np.array_equal(d["temporal_ids"], np.stack([k, k + 1], axis=1))
+    # temporal neighbours: the same spatial cell one slab below / above, -1 at t = 0 and T
+    tnb = d["temporal_neigh"][:, :, 0]
+    assert np.array_equal(tnb[:, 0], np.where(k > 0, ids - ns, -1))
+    assert np.array_equal(tnb[:, 1], np.where(k < mm - 1, ids + ns, -1))
+    # spatial neighbours: same slab, symmetric, sharing exactly the face's vertices
+    snb = d["spatial_neigh"][:, :, 0]
+    assert msn == 1
+    n_interior = 0
+    for cell in range(ns * mm):
+        for f in range(nfs):
+            other = snb[cell, f]
+            if other < 0:
+                continue
+            n_interior += 1
+            assert other // ns == cell // ns
+            assert cell in snb[other]
+            face = set(C[cell % ns]) - {C[cell % ns][f]} if dim == 2 else {C[cell % ns][f]}
+            assert face <= set(C[other % ns])
+    # each interior spatial facet is listed from both sides, in every slab
+    if dim == 2:
+        edges = {}
+        for tri in C:
+            for a, b in ((tri[1], tri[2]), (tri[2], tri[0]), (tri[0], tri[1])):
+                key = (min(a, b), max(a, b))
+                edges[key] = edges.get(key, 0) + 1
+        interior = sum(1 for v in edges.values() if v == 2)
+    else:
+        interior = ns - 1
+    assert n_interior == 2 * interior * mm
+
+
+def _cellwise(path, payload):
+    with open(path, "rb") as f:
+        h = _header(f, payload, 1)
+        n = int(h[0])
+        ids = _block(f, np.int32, n)
+        blocks = [_block(f, np.float64, n) for _ in range(1 if payload == 1 else int(h[1]))]
+        assert np.fromfile(f, dtype=np.uint8).size == 0
+    return h, ids, blocks
+
+
+def test_cellwise_error_file_order_and_ties(tmp_path):
+    values = np.array([0.5, 2.0, 0.5, 1.0, 2.0, 0.0, 0.5])
+    path = tmp_path / "cellwise_error.bin"
+    snapshot.write_cellwise_error_binary(values, path)
+    h, ids, (v,) = _cellwise(path, 1)
+    assert int(h[0]) == len(values)
+    # value descending, ties to the lower id (time_slab_cellwise_errors.hpp:54-70)
+    assert ids.tolist() == [1, 4, 3, 0, 2, 6, 5]
+    assert np.array_equal(v, values[ids])
+    snapshot.write_cellwise_error_binary(values, path, sort_descending=False)
+    h, ids, (v,) = _cellwise(path, 1)
+    assert ids.tolist() == list(range(len(values)))
+    assert np.array_equal(v, values)
+
+
+def test_cellwise_error_components_file(tmp_path):
+    rng = np.random.default_rng(7)
+    n = 50
+    est = dict(flux_by_cell=rng.random(n), recon_by_cell=rng.random(n) * 1e-3, resid_by_cell=rng.random(n))
+    est["flux_by_cell"][10] = est["flux_by_cell"][20]  # a tie in the total
+    est["recon_by_cell"][10] = est["recon_by_cell"][20]
+    path = tmp_path / "cellwise_error_components.bin"
+    snapshot.write_cellwise_error_components_binary(est, path)
+    h, ids, (flux, recon, resid, total) = _cellwise(path, 2)
+    assert (int(h[0]), int(h[1])) == (n, 4)
+    expect_total = combine_estimator(est["flux_by_cell"], est["recon_by_cell"])
+    assert np.array_equal(total, expect_total[ids])
+    assert np.array_equal(total, flux + recon)
+    assert np.array_equal(flux, est["flux_by_cell"][ids])
+    assert np.array_equal(recon, est["recon_by_cell"][ids])
+    assert np.array_equal(resid, est["resid_by_cell"][ids])
+    assert np.all(np.diff(total) <= 0)
+    i10, i20 = list(ids).index(10), list(ids).index(20)
+    assert i20 == i10 + 1
