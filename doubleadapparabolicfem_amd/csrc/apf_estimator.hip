@@ -2319,74 +2319,86 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             }
             HIP_TRY(hipEventRecord(sys->evp[2], st));
             for (const auto& bk : B.buckets) {
-                // one solve of the bucket with column blocking sh; `extra` counts refinement rounds
-                auto run = [&](const SolveShape& sh, int* extra) {
-                    const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * sh.ncg);
-                    if (sh.gws && static_cast<int64_t>(bk.count) * sh.ncg * sh.ws_stride > B.solve_ws_doubles)
-                        throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
-                    auto solve = [&](auto kern) {
-                        if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
-                        hipLaunchKernelGGL(kern, dim3(grid), dim3(solve_threads(sh.CT)), sh.lds_bytes, st, P, topo, ps,
-                                           bk.q0, bk.count, sh.ncg, sh.cpw, B.Kd, Ki, B.RHS, B.sig, B.patch_res, extra,
-                                           B.solve_ws, sh.ws_stride);
-                    };
-                    if (sh.gws) {
+                // a bucket whose inverse is FP64 in both precisions (block elimination
+                // writes it in FP64; its FP32 copy is only a rounding) is solved once in
+                // FP64: FP32 products and refinement rounds would only add work
+                const bool fp64_inverse = !bk.big && bk.np > 128 && bk.blk_max <= 128;
+                auto solve_bucket = [&](auto stag) {
+                    using ST = decltype(stag);
+                    ST* Ks = std::is_same<ST, float>::value ? reinterpret_cast<ST*>(B.Ki32) : reinterpret_cast<ST*>(B.Ki);
+                    EstParams Ps = P;
+                    if (!std::is_same<ST, FT>::value) Ps.refine_steps = 0;
+                    // one solve of the bucket with column blocking sh; `extra` counts refinement rounds
+                    auto run = [&](const SolveShape& sh, int* extra) {
+                        const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * sh.ncg);
+                        if (sh.gws && static_cast<int64_t>(bk.count) * sh.ncg * sh.ws_stride > B.solve_ws_doubles)
+                            throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
+                        auto solve = [&](auto kern) {
+                            if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
+                            hipLaunchKernelGGL(kern, dim3(grid), dim3(solve_threads(sh.CT)), sh.lds_bytes, st, Ps, topo, ps,
+                                               bk.q0, bk.count, sh.ncg, sh.cpw, B.Kd, Ks, B.RHS, B.sig, B.patch_res, extra,
+                                               B.solve_ws, sh.ws_stride);
+                        };
+                        if (sh.gws) {
+                            switch (sh.CT) {
+                                case 1: solve(k_patch_solve<ST, 1, true>); break;
+                                case 2: solve(k_patch_solve<ST, 2, true>); break;
+                                case 3: solve(k_patch_solve<ST, 3, true>); break;
+                                case 4: solve(k_patch_solve<ST, 4, true>); break;
+                                case 5: solve(k_patch_solve<ST, 5, true>); break;
+                                default: solve(k_patch_solve<ST, 6, true>); break;
+                            }
+                            return;
+                        }
                         switch (sh.CT) {
-                            case 1: solve(k_patch_solve<FT, 1, true>); break;
-                            case 2: solve(k_patch_solve<FT, 2, true>); break;
-                            case 3: solve(k_patch_solve<FT, 3, true>); break;
-                            case 4: solve(k_patch_solve<FT, 4, true>); break;
-                            case 5: solve(k_patch_solve<FT, 5, true>); break;
-                            default: solve(k_patch_solve<FT, 6, true>); break;
+                            case 1: solve(k_patch_solve<ST, 1, false>); break;
+                            case 2: solve(k_patch_solve<ST, 2, false>); break;
+                            case 3: solve(k_patch_solve<ST, 3, false>); break;
+                            case 4: solve(k_patch_solve<ST, 4, false>); break;
+                            case 5: solve(k_patch_solve<ST, 5, false>); break;
+                            default: solve(k_patch_solve<ST, 6, false>); break;
                         }
-                        return;
+                    };
+                    // the column blocking of this (order, precision): timed once per system
+                    // (each candidate twice, the faster run counts); the trial solves
+                    // compute the same result and count their rounds elsewhere
+                    const std::pair<int, int> key{bk.np, static_cast<int>(sizeof(ST))};
+                    auto it = sys->solve_ct.find(key);
+                    if (it == sys->solve_ct.end()) {
+                        const std::vector<int> cand = solve_candidates(bk.np, ncols, P.Nt, sizeof(ST));
+                        const char* tune_env = std::getenv("APF_SOLVE_TUNE");
+                        int best = 0;
+                        if (cand.size() > 1 && !(tune_env && std::atoi(tune_env) == 0)) {
+                            hipEvent_t ta, tb;
+                            HIP_TRY(hipEventCreate(&ta));
+                            HIP_TRY(hipEventCreate(&tb));
+                            float best_ms = 0.0f;
+                            for (int ct : cand) {
+                                const SolveShape shc = solve_shape(bk.np, ncols, P.Nt, sizeof(ST), ct);
+                                float ms = 0.0f;
+                                for (int rep = 0; rep < 2; ++rep) {
+                                    HIP_TRY(hipEventRecord(ta, st));
+                                    run(shc, B.tune_extra);
+                                    HIP_TRY(hipEventRecord(tb, st));
+                                    HIP_TRY(hipEventSynchronize(tb));
+                                    float t = 0.0f;
+                                    HIP_TRY(hipEventElapsedTime(&t, ta, tb));
+                                    ms = (rep == 0) ? t : std::min(ms, t);
+                                }
+                                if (best == 0 || ms < best_ms) {
+                                    best = ct;
+                                    best_ms = ms;
+                                }
+                            }
+                            (void)hipEventDestroy(ta);
+                            (void)hipEventDestroy(tb);
+                        }
+                        it = sys->solve_ct.emplace(key, best).first;
                     }
-                    switch (sh.CT) {
-                        case 1: solve(k_patch_solve<FT, 1, false>); break;
-                        case 2: solve(k_patch_solve<FT, 2, false>); break;
-                        case 3: solve(k_patch_solve<FT, 3, false>); break;
-                        case 4: solve(k_patch_solve<FT, 4, false>); break;
-                        case 5: solve(k_patch_solve<FT, 5, false>); break;
-                        default: solve(k_patch_solve<FT, 6, false>); break;
-                    }
+                    run(solve_shape(bk.np, ncols, P.Nt, sizeof(ST), it->second), B.extra_rounds);
                 };
-                // the column blocking of this (order, precision): timed once per system
-                // (each candidate twice, the faster run counts); the trial solves
-                // compute the same result and count their rounds elsewhere
-                const std::pair<int, int> key{bk.np, static_cast<int>(sizeof(FT))};
-                auto it = sys->solve_ct.find(key);
-                if (it == sys->solve_ct.end()) {
-                    const std::vector<int> cand = solve_candidates(bk.np, ncols, P.Nt, sizeof(FT));
-                    const char* tune_env = std::getenv("APF_SOLVE_TUNE");
-                    int best = 0;
-                    if (cand.size() > 1 && !(tune_env && std::atoi(tune_env) == 0)) {
-                        hipEvent_t ta, tb;
-                        HIP_TRY(hipEventCreate(&ta));
-                        HIP_TRY(hipEventCreate(&tb));
-                        float best_ms = 0.0f;
-                        for (int ct : cand) {
-                            const SolveShape shc = solve_shape(bk.np, ncols, P.Nt, sizeof(FT), ct);
-                            float ms = 0.0f;
-                            for (int rep = 0; rep < 2; ++rep) {
-                                HIP_TRY(hipEventRecord(ta, st));
-                                run(shc, B.tune_extra);
-                                HIP_TRY(hipEventRecord(tb, st));
-                                HIP_TRY(hipEventSynchronize(tb));
-                                float t = 0.0f;
-                                HIP_TRY(hipEventElapsedTime(&t, ta, tb));
-                                ms = (rep == 0) ? t : std::min(ms, t);
-                            }
-                            if (best == 0 || ms < best_ms) {
-                                best = ct;
-                                best_ms = ms;
-                            }
-                        }
-                        (void)hipEventDestroy(ta);
-                        (void)hipEventDestroy(tb);
-                    }
-                    it = sys->solve_ct.emplace(key, best).first;
-                }
-                run(solve_shape(bk.np, ncols, P.Nt, sizeof(FT), it->second), B.extra_rounds);
+                if (std::is_same<FT, float>::value && fp64_inverse) solve_bucket(double{});
+                else solve_bucket(FT{});
             }
         };
         HIP_TRY(hipEventRecord(sys->ev0, st));
