@@ -2189,48 +2189,60 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         auto launch = [&](auto tag) {
             using FT = decltype(tag);
             FT* Ki = std::is_same<FT, float>::value ? reinterpret_cast<FT*>(B.Ki32) : reinterpret_cast<FT*>(B.Ki);
-            for (const auto& bk : B.buckets) {
-                if (bk.big) continue;
-                // one register-resident Gauss-Jordan where the whole matrix fits the register file,
-                // else block elimination with two of them
-                if constexpr (std::is_same<FT, float>::value) {
-                    // FP32: blocked Gauss-Jordan with MFMA panel updates
+            // the factor of one bucket in precision F2 (FP32: blocked Gauss-Jordan with MFMA
+            // panel updates up to order 128; otherwise one register-resident Gauss-Jordan
+            // where the matrix fits the register file, else block elimination with two)
+            auto factor_bucket = [&](const EstimatorBuffers::Bucket& bk, auto ftag) {
+                using F2 = decltype(ftag);
+                F2* K2 = std::is_same<F2, float>::value ? reinterpret_cast<F2*>(B.Ki32) : reinterpret_cast<F2*>(B.Ki);
+                if constexpr (std::is_same<F2, float>::value) {
                     if (bk.np <= 64) {
                         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_blocked<1>), dim3(bk.count), dim3(256), 0,
                                            st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd,
-                                           Ki);
-                        continue;
+                                           K2);
+                        return;
                     }
                     if (bk.np <= 128) {
                         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_blocked<2>), dim3(bk.count), dim3(256), 0,
                                            st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd,
-                                           Ki);
-                        continue;
+                                           K2);
+                        return;
                     }
                 }
                 if (bk.np <= 64) {
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_full<FT, 4, 16, 1>), dim3(bk.count), dim3(256), 0,
-                                       st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
-                    continue;
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_full<F2, 4, 16, 1>), dim3(bk.count), dim3(256), 0,
+                                       st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, K2);
+                    return;
                 }
                 if (bk.np <= 128) {
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_full<FT, 4, 32, 2>), dim3(bk.count), dim3(256), 0,
-                                       st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
-                    continue;
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_full<F2, 4, 32, 2>), dim3(bk.count), dim3(256), 0,
+                                       st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, K2);
+                    return;
                 }
                 if (bk.blk_max <= 128) {
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_schur<FT, 8, 16, 2>), dim3(bk.count), dim3(512), 0,
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_schur<F2, 8, 16, 2>), dim3(bk.count), dim3(512), 0,
                                        st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, B.Ki,
                                        B.Ki32);
-                    continue;
+                    return;
                 }
                 const size_t npb = static_cast<size_t>(bk.np);
-                const size_t in_lds_bytes = npb * npb * sizeof(FT) + npb * sizeof(int);
+                const size_t in_lds_bytes = npb * npb * sizeof(F2) + npb * sizeof(int);
                 const int in_lds = in_lds_bytes <= lds_cap ? 1 : 0;
                 const size_t lf = in_lds ? in_lds_bytes : npb * sizeof(int);
-                set_lds(reinterpret_cast<const void*>(&k_patch_factor<FT>), lf);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor<FT>), dim3(bk.count), dim3(PATCH_THREADS), lf, st, P,
-                                   Tb, topo, ps, bk.q0, in_lds, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, Ki);
+                set_lds(reinterpret_cast<const void*>(&k_patch_factor<F2>), lf);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor<F2>), dim3(bk.count), dim3(PATCH_THREADS), lf, st, P,
+                                   Tb, topo, ps, bk.q0, in_lds, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd, K2);
+            };
+            // mixed path: a bucket whose FP64 factor and single solve measured faster
+            auto use_fp64 = [&](const EstimatorBuffers::Bucket& bk) {
+                if (!std::is_same<FT, float>::value || bk.big) return false;
+                const auto it = sys->mixed_fp64.find(bk.np);
+                return it != sys->mixed_fp64.end() && it->second;
+            };
+            for (const auto& bk : B.buckets) {
+                if (bk.big) continue;
+                if (use_fp64(bk)) factor_bucket(bk, double{});
+                else factor_bucket(bk, FT{});
             }
             if (B.big_count) {
                 // large patches: blocked Gauss-Jordan, one panel per pair of launches
@@ -2323,7 +2335,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                 // writes it in FP64; its FP32 copy is only a rounding) is solved once in
                 // FP64: FP32 products and refinement rounds would only add work
                 const bool fp64_inverse = !bk.big && bk.np > 128 && bk.blk_max <= 128;
-                auto solve_bucket = [&](auto stag) {
+                auto solve_bucket = [&](auto stag, int* counter) {
                     using ST = decltype(stag);
                     ST* Ks = std::is_same<ST, float>::value ? reinterpret_cast<ST*>(B.Ki32) : reinterpret_cast<ST*>(B.Ki);
                     EstParams Ps = P;
@@ -2395,10 +2407,52 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                         }
                         it = sys->solve_ct.emplace(key, best).first;
                     }
-                    run(solve_shape(bk.np, ncols, P.Nt, sizeof(ST), it->second), B.extra_rounds);
+                    run(solve_shape(bk.np, ncols, P.Nt, sizeof(ST), it->second), counter);
                 };
-                if (std::is_same<FT, float>::value && fp64_inverse) solve_bucket(double{});
-                else solve_bucket(FT{});
+                if (!std::is_same<FT, float>::value) {
+                    solve_bucket(FT{}, B.extra_rounds);
+                    continue;
+                }
+                if (fp64_inverse || use_fp64(bk)) {
+                    solve_bucket(double{}, B.extra_rounds);
+                    continue;
+                }
+                const char* tune_env = std::getenv("APF_SOLVE_TUNE");
+                if (bk.big || sys->mixed_fp64.count(bk.np) || (tune_env && std::atoi(tune_env) == 0)) {
+                    solve_bucket(float{}, B.extra_rounds);
+                    continue;
+                }
+                // first mixed call for this padded order: time the FP32 factor with the refined
+                // solve against the FP64 factor with one solve (each after an untimed pass, so
+                // that neither includes its tile trials; the trial solves count their rounds
+                // apart) and keep the faster; the chosen pair runs last, so the outputs and the
+                // round count are those of a later call
+                solve_bucket(float{}, B.tune_extra);
+                factor_bucket(bk, double{});
+                solve_bucket(double{}, B.tune_extra);
+                hipEvent_t e[3];
+                for (auto& ev : e) HIP_TRY(hipEventCreate(&ev));
+                HIP_TRY(hipEventRecord(e[0], st));
+                factor_bucket(bk, float{});
+                solve_bucket(float{}, B.tune_extra);
+                HIP_TRY(hipEventRecord(e[1], st));
+                factor_bucket(bk, double{});
+                solve_bucket(double{}, B.tune_extra);
+                HIP_TRY(hipEventRecord(e[2], st));
+                HIP_TRY(hipEventSynchronize(e[2]));
+                float t32 = 0.0f, t64 = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&t32, e[0], e[1]));
+                HIP_TRY(hipEventElapsedTime(&t64, e[1], e[2]));
+                for (auto& ev : e) (void)hipEventDestroy(ev);
+                const bool fp64 = t64 < t32;
+                sys->mixed_fp64[bk.np] = fp64;
+                if (fp64) {
+                    factor_bucket(bk, double{});
+                    solve_bucket(double{}, B.extra_rounds);
+                } else {
+                    factor_bucket(bk, float{});
+                    solve_bucket(float{}, B.extra_rounds);
+                }
             }
         };
         HIP_TRY(hipEventRecord(sys->ev0, st));

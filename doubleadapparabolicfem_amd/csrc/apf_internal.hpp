@@ -229,6 +229,9 @@ struct System {
     std::map<std::pair<int, int>, int> solve_ct;
     // patch right-hand sides: slabs per workgroup chosen by timing, per padded order
     std::map<int, int> rhs_sg;
+    // mixed path: buckets (by padded order) that run the FP64 factor and solve because that
+    // measured faster than the FP32 factor with refinement
+    std::map<int, bool> mixed_fp64;
     float t_assembly_ms = 0.f, t_patch_ms = 0.f, t_diag_ms = 0.f, t_spmv_ms = 0.f;
     float t_factor_ms = 0.f, t_rhs_ms = 0.f, t_solve_ms = 0.f;
     bool assembled = false;
