@@ -2244,25 +2244,38 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                 if (use_fp64(bk)) factor_bucket(bk, double{});
                 else factor_bucket(bk, FT{});
             }
-            if (B.big_count) {
-                // large patches: blocked Gauss-Jordan, one panel per pair of launches
+            // the large patches of all big buckets in precision F2: blocked Gauss-Jordan,
+            // one panel per pair of launches
+            auto big_factor = [&](auto ftag) {
+                using F2 = decltype(ftag);
+                F2* K2 = std::is_same<F2, float>::value ? reinterpret_cast<F2*>(B.Ki32) : reinterpret_cast<F2*>(B.Ki);
                 const BigPatches bp{B.big_q, B.big_off, B.big_ws};
                 const int G = B.big_count, npm = B.big_npmax;
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_prepare<FT>), dim3(G), dim3(PATCH_THREADS), 0, st, P, Tb,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_prepare<F2>), dim3(G), dim3(PATCH_THREADS), 0, st, P, Tb,
                                    topo, ps, bp, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd);
                 const int nstrips = (npm + 63) / 64, rpt = (npm + 511) / 512;
                 for (int c0 = 0; c0 < npm; c0 += 16) {
                     switch (rpt) {
-                        case 1: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<FT, 1>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
-                        case 2: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<FT, 2>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
-                        case 3: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<FT, 3>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
-                        default: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<FT, 4>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
+                        case 1: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<F2, 1>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
+                        case 2: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<F2, 2>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
+                        case 3: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<F2, 3>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
+                        default: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_panel<F2, 4>), dim3(G), dim3(512), 0, st, ps, bp, c0); break;
                     }
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_update<FT>), dim3(G * nstrips), dim3(256), 0, st, ps, bp,
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_update<F2>), dim3(G * nstrips), dim3(256), 0, st, ps, bp,
                                        c0, nstrips);
                 }
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_finish<FT>), dim3(G * (npm / 16)), dim3(256), 0, st, ps, bp,
-                                   npm / 16, Ki);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_big_finish<F2>), dim3(G * (npm / 16)), dim3(256), 0, st, ps, bp,
+                                   npm / 16, K2);
+            };
+            // mixed path: the big buckets as one group (their factor runs together)
+            auto big_fp64 = [&]() {
+                if (!std::is_same<FT, float>::value) return false;
+                const auto it = sys->mixed_fp64.find(-1);
+                return it != sys->mixed_fp64.end() && it->second;
+            };
+            if (B.big_count) {
+                if (big_fp64()) big_factor(double{});
+                else big_factor(FT{});
             }
             HIP_TRY(hipEventRecord(sys->evp[1], st));
             for (const auto& bk : B.buckets) {
@@ -2330,128 +2343,162 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                 run(it->second);
             }
             HIP_TRY(hipEventRecord(sys->evp[2], st));
-            for (const auto& bk : B.buckets) {
-                // a bucket whose inverse is FP64 in both precisions (block elimination
-                // writes it in FP64; its FP32 copy is only a rounding) is solved once in
-                // FP64: FP32 products and refinement rounds would only add work
-                const bool fp64_inverse = !bk.big && bk.np > 128 && bk.blk_max <= 128;
-                auto solve_bucket = [&](auto stag, int* counter) {
-                    using ST = decltype(stag);
-                    ST* Ks = std::is_same<ST, float>::value ? reinterpret_cast<ST*>(B.Ki32) : reinterpret_cast<ST*>(B.Ki);
-                    EstParams Ps = P;
-                    if (!std::is_same<ST, FT>::value) Ps.refine_steps = 0;
-                    // one solve of the bucket with column blocking sh; `extra` counts refinement rounds
-                    auto run = [&](const SolveShape& sh, int* extra) {
-                        const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * sh.ncg);
-                        if (sh.gws && static_cast<int64_t>(bk.count) * sh.ncg * sh.ws_stride > B.solve_ws_doubles)
-                            throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
-                        auto solve = [&](auto kern) {
-                            if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
-                            hipLaunchKernelGGL(kern, dim3(grid), dim3(solve_threads(sh.CT)), sh.lds_bytes, st, Ps, topo, ps,
-                                               bk.q0, bk.count, sh.ncg, sh.cpw, B.Kd, Ks, B.RHS, B.sig, B.patch_res, extra,
-                                               B.solve_ws, sh.ws_stride);
-                        };
-                        if (sh.gws) {
-                            switch (sh.CT) {
-                                case 1: solve(k_patch_solve<ST, 1, true>); break;
-                                case 2: solve(k_patch_solve<ST, 2, true>); break;
-                                case 3: solve(k_patch_solve<ST, 3, true>); break;
-                                case 4: solve(k_patch_solve<ST, 4, true>); break;
-                                case 5: solve(k_patch_solve<ST, 5, true>); break;
-                                default: solve(k_patch_solve<ST, 6, true>); break;
-                            }
-                            return;
-                        }
-                        switch (sh.CT) {
-                            case 1: solve(k_patch_solve<ST, 1, false>); break;
-                            case 2: solve(k_patch_solve<ST, 2, false>); break;
-                            case 3: solve(k_patch_solve<ST, 3, false>); break;
-                            case 4: solve(k_patch_solve<ST, 4, false>); break;
-                            case 5: solve(k_patch_solve<ST, 5, false>); break;
-                            default: solve(k_patch_solve<ST, 6, false>); break;
-                        }
+            // one solve of bucket bk in precision ST (inverse Ki32 or Ki); `counter` receives
+            // the refinement rounds beyond refine_steps
+            auto solve_bucket = [&](const EstimatorBuffers::Bucket& bk, auto stag, int* counter) {
+                using ST = decltype(stag);
+                ST* Ks = std::is_same<ST, float>::value ? reinterpret_cast<ST*>(B.Ki32) : reinterpret_cast<ST*>(B.Ki);
+                EstParams Ps = P;
+                if (!std::is_same<ST, FT>::value) Ps.refine_steps = 0;
+                // one solve of the bucket with column blocking sh; `extra` counts refinement rounds
+                auto run = [&](const SolveShape& sh, int* extra) {
+                    const unsigned grid = static_cast<unsigned>((bk.count + 7) / 8 * 8 * sh.ncg);
+                    if (sh.gws && static_cast<int64_t>(bk.count) * sh.ncg * sh.ws_stride > B.solve_ws_doubles)
+                        throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
+                    auto solve = [&](auto kern) {
+                        if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
+                        hipLaunchKernelGGL(kern, dim3(grid), dim3(solve_threads(sh.CT)), sh.lds_bytes, st, Ps, topo, ps,
+                                           bk.q0, bk.count, sh.ncg, sh.cpw, B.Kd, Ks, B.RHS, B.sig, B.patch_res, extra,
+                                           B.solve_ws, sh.ws_stride);
                     };
-                    // the column blocking of this (order, precision): timed once per system
-                    // (each candidate twice, the faster run counts); the trial solves
-                    // compute the same result and count their rounds elsewhere
-                    const std::pair<int, int> key{bk.np, static_cast<int>(sizeof(ST))};
-                    auto it = sys->solve_ct.find(key);
-                    if (it == sys->solve_ct.end()) {
-                        const std::vector<int> cand = solve_candidates(bk.np, ncols, P.Nt, sizeof(ST));
-                        const char* tune_env = std::getenv("APF_SOLVE_TUNE");
-                        int best = 0;
-                        if (cand.size() > 1 && !(tune_env && std::atoi(tune_env) == 0)) {
-                            hipEvent_t ta, tb;
-                            HIP_TRY(hipEventCreate(&ta));
-                            HIP_TRY(hipEventCreate(&tb));
-                            float best_ms = 0.0f;
-                            for (int ct : cand) {
-                                const SolveShape shc = solve_shape(bk.np, ncols, P.Nt, sizeof(ST), ct);
-                                float ms = 0.0f;
-                                for (int rep = 0; rep < 2; ++rep) {
-                                    HIP_TRY(hipEventRecord(ta, st));
-                                    run(shc, B.tune_extra);
-                                    HIP_TRY(hipEventRecord(tb, st));
-                                    HIP_TRY(hipEventSynchronize(tb));
-                                    float t = 0.0f;
-                                    HIP_TRY(hipEventElapsedTime(&t, ta, tb));
-                                    ms = (rep == 0) ? t : std::min(ms, t);
-                                }
-                                if (best == 0 || ms < best_ms) {
-                                    best = ct;
-                                    best_ms = ms;
-                                }
-                            }
-                            (void)hipEventDestroy(ta);
-                            (void)hipEventDestroy(tb);
+                    if (sh.gws) {
+                        switch (sh.CT) {
+                            case 1: solve(k_patch_solve<ST, 1, true>); break;
+                            case 2: solve(k_patch_solve<ST, 2, true>); break;
+                            case 3: solve(k_patch_solve<ST, 3, true>); break;
+                            case 4: solve(k_patch_solve<ST, 4, true>); break;
+                            case 5: solve(k_patch_solve<ST, 5, true>); break;
+                            default: solve(k_patch_solve<ST, 6, true>); break;
                         }
-                        it = sys->solve_ct.emplace(key, best).first;
+                        return;
                     }
-                    run(solve_shape(bk.np, ncols, P.Nt, sizeof(ST), it->second), counter);
+                    switch (sh.CT) {
+                        case 1: solve(k_patch_solve<ST, 1, false>); break;
+                        case 2: solve(k_patch_solve<ST, 2, false>); break;
+                        case 3: solve(k_patch_solve<ST, 3, false>); break;
+                        case 4: solve(k_patch_solve<ST, 4, false>); break;
+                        case 5: solve(k_patch_solve<ST, 5, false>); break;
+                        default: solve(k_patch_solve<ST, 6, false>); break;
+                    }
                 };
-                if (!std::is_same<FT, float>::value) {
-                    solve_bucket(FT{}, B.extra_rounds);
-                    continue;
+                // the column blocking of this (order, precision): timed once per system
+                // (each candidate twice, the faster run counts); the trial solves
+                // compute the same result and count their rounds elsewhere
+                const std::pair<int, int> key{bk.np, static_cast<int>(sizeof(ST))};
+                auto it = sys->solve_ct.find(key);
+                if (it == sys->solve_ct.end()) {
+                    const std::vector<int> cand = solve_candidates(bk.np, ncols, P.Nt, sizeof(ST));
+                    const char* tune_env = std::getenv("APF_SOLVE_TUNE");
+                    int best = 0;
+                    if (cand.size() > 1 && !(tune_env && std::atoi(tune_env) == 0)) {
+                        hipEvent_t ta, tb;
+                        HIP_TRY(hipEventCreate(&ta));
+                        HIP_TRY(hipEventCreate(&tb));
+                        float best_ms = 0.0f;
+                        for (int ct : cand) {
+                            const SolveShape shc = solve_shape(bk.np, ncols, P.Nt, sizeof(ST), ct);
+                            float ms = 0.0f;
+                            for (int rep = 0; rep < 2; ++rep) {
+                                HIP_TRY(hipEventRecord(ta, st));
+                                run(shc, B.tune_extra);
+                                HIP_TRY(hipEventRecord(tb, st));
+                                HIP_TRY(hipEventSynchronize(tb));
+                                float t = 0.0f;
+                                HIP_TRY(hipEventElapsedTime(&t, ta, tb));
+                                ms = (rep == 0) ? t : std::min(ms, t);
+                            }
+                            if (best == 0 || ms < best_ms) {
+                                best = ct;
+                                best_ms = ms;
+                            }
+                        }
+                        (void)hipEventDestroy(ta);
+                        (void)hipEventDestroy(tb);
+                    }
+                    it = sys->solve_ct.emplace(key, best).first;
                 }
-                if (fp64_inverse || use_fp64(bk)) {
-                    solve_bucket(double{}, B.extra_rounds);
-                    continue;
-                }
-                const char* tune_env = std::getenv("APF_SOLVE_TUNE");
-                if (bk.big || sys->mixed_fp64.count(bk.np) || (tune_env && std::atoi(tune_env) == 0)) {
-                    solve_bucket(float{}, B.extra_rounds);
-                    continue;
-                }
-                // first mixed call for this padded order: time the FP32 factor with the refined
-                // solve against the FP64 factor with one solve (each after an untimed pass, so
-                // that neither includes its tile trials; the trial solves count their rounds
-                // apart) and keep the faster; the chosen pair runs last, so the outputs and the
-                // round count are those of a later call
-                solve_bucket(float{}, B.tune_extra);
-                factor_bucket(bk, double{});
-                solve_bucket(double{}, B.tune_extra);
+                run(solve_shape(bk.np, ncols, P.Nt, sizeof(ST), it->second), counter);
+            };
+            // first mixed call for a bucket (or the group of big buckets): time the FP32
+            // factor with the refined solve against the FP64 factor with one solve, each
+            // after an untimed pass so that neither includes its tile trials (the trial
+            // solves count their rounds apart); true if the FP64 pair is faster.  The FP32
+            // factor of the first pass already ran in the factor phase.
+            auto faster_fp64 = [&](auto fact32, auto fact64, auto solves) {
+                solves(float{}, B.tune_extra);
+                fact64();
+                solves(double{}, B.tune_extra);
                 hipEvent_t e[3];
                 for (auto& ev : e) HIP_TRY(hipEventCreate(&ev));
                 HIP_TRY(hipEventRecord(e[0], st));
-                factor_bucket(bk, float{});
-                solve_bucket(float{}, B.tune_extra);
+                fact32();
+                solves(float{}, B.tune_extra);
                 HIP_TRY(hipEventRecord(e[1], st));
-                factor_bucket(bk, double{});
-                solve_bucket(double{}, B.tune_extra);
+                fact64();
+                solves(double{}, B.tune_extra);
                 HIP_TRY(hipEventRecord(e[2], st));
                 HIP_TRY(hipEventSynchronize(e[2]));
                 float t32 = 0.0f, t64 = 0.0f;
                 HIP_TRY(hipEventElapsedTime(&t32, e[0], e[1]));
                 HIP_TRY(hipEventElapsedTime(&t64, e[1], e[2]));
                 for (auto& ev : e) (void)hipEventDestroy(ev);
-                const bool fp64 = t64 < t32;
+                return t64 < t32;
+            };
+            const char* tune_env = std::getenv("APF_SOLVE_TUNE");
+            const bool tune = std::is_same<FT, float>::value && !(tune_env && std::atoi(tune_env) == 0);
+            const bool big_tune = tune && B.big_count && !sys->mixed_fp64.count(-1);
+            for (const auto& bk : B.buckets) {
+                if (!std::is_same<FT, float>::value) {
+                    solve_bucket(bk, FT{}, B.extra_rounds);
+                    continue;
+                }
+                if (bk.big) {
+                    // solved after the loop while the group's precision is being chosen
+                    if (!big_tune) {
+                        if (big_fp64()) solve_bucket(bk, double{}, B.extra_rounds);
+                        else solve_bucket(bk, float{}, B.extra_rounds);
+                    }
+                    continue;
+                }
+                // a bucket whose inverse is FP64 in both precisions (block elimination
+                // writes it in FP64; its FP32 copy is only a rounding) is solved once in
+                // FP64: FP32 products and refinement rounds would only add work
+                const bool fp64_inverse = bk.np > 128 && bk.blk_max <= 128;
+                if (fp64_inverse || use_fp64(bk)) {
+                    solve_bucket(bk, double{}, B.extra_rounds);
+                    continue;
+                }
+                if (!tune || sys->mixed_fp64.count(bk.np)) {
+                    solve_bucket(bk, float{}, B.extra_rounds);
+                    continue;
+                }
+                // the chosen pair runs last, so the outputs and the round count are those
+                // of a later call
+                auto solves = [&](auto stag, int* counter) { solve_bucket(bk, stag, counter); };
+                const bool fp64 = faster_fp64([&] { factor_bucket(bk, float{}); }, [&] { factor_bucket(bk, double{}); },
+                                              solves);
                 sys->mixed_fp64[bk.np] = fp64;
                 if (fp64) {
                     factor_bucket(bk, double{});
-                    solve_bucket(double{}, B.extra_rounds);
+                    solves(double{}, B.extra_rounds);
                 } else {
                     factor_bucket(bk, float{});
-                    solve_bucket(float{}, B.extra_rounds);
+                    solves(float{}, B.extra_rounds);
+                }
+            }
+            if (big_tune) {
+                auto big_solves = [&](auto stag, int* counter) {
+                    for (const auto& b2 : B.buckets)
+                        if (b2.big) solve_bucket(b2, stag, counter);
+                };
+                const bool fp64 = faster_fp64([&] { big_factor(float{}); }, [&] { big_factor(double{}); }, big_solves);
+                sys->mixed_fp64[-1] = fp64;
+                if (fp64) {
+                    big_factor(double{});
+                    big_solves(double{}, B.extra_rounds);
+                } else {
+                    big_factor(float{});
+                    big_solves(float{}, B.extra_rounds);
                 }
             }
         };
