@@ -2357,7 +2357,11 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                         throw std::runtime_error("patch solve: workspace smaller than the column blocking needs");
                     auto solve = [&](auto kern) {
                         if (!sh.gws) set_lds(reinterpret_cast<const void*>(kern), sh.lds_bytes);
-                        hipLaunchKernelGGL(kern, dim3(grid), dim3(solve_threads(sh.CT)), sh.lds_bytes, st, Ps, topo, ps,
+                        // one wave per item (row tile, column-tile group) up to the kernel's width:
+                        // the kernel loops over its waves, and small patches then leave none idle
+                        const int items = (bk.np / 16) * ((sh.CT % 2 == 0) ? 2 : 1);
+                        const int threads = std::min(solve_threads(sh.CT), std::max(1, items) * 64);
+                        hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), sh.lds_bytes, st, Ps, topo, ps,
                                            bk.q0, bk.count, sh.ncg, sh.cpw, B.Kd, Ks, B.RHS, B.sig, B.patch_res, extra,
                                            B.solve_ws, sh.ws_stride);
                     };
