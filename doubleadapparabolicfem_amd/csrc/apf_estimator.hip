@@ -1377,13 +1377,18 @@ __global__ void __launch_bounds__(RHS_THREADS)
     }
     __syncthreads();
     constexpr int ROWS = (NF + NQ) * NT;
-    // thread t owns local (row, mode) rk = t % 64 (rows = flux dofs, then
-    // scalar dofs) for every (cell, slab) pair t / 64 + (waves) j: its indices
-    // and table rows are fixed, the pair advances without divisions
+    // a wave holds PW (cell, slab) pairs at a time, lane = slot ROWS + rk: lane
+    // owns local (row, mode) rk (rows = flux dofs, then scalar dofs) of the pairs
+    // wave PW + slot + (waves PW) j.  With more than 32 rows x modes PW = 1 and
+    // the lane loops over rk = lane, lane + 64, ..  Its indices and table rows are
+    // fixed, and the pair advances without divisions.
     static_assert(ROWS <= 64 * 4, "local rows x modes: at most four rows per lane");
-    for (int rk = threadIdx.x % 64; rk < ROWS; rk += 64) {
+    constexpr int PW = (ROWS <= 32) ? 64 / ROWS : 1;
+    const int lane64 = threadIdx.x % 64, slot = (PW > 1) ? lane64 / ROWS : 0;
+    const int pstep = (blockDim.x / 64) * PW;
+    for (int rk = (PW > 1) ? lane64 % ROWS : lane64; rk < ROWS && slot < PW; rk += (PW > 1) ? ROWS : 64) {
         const int row = rk / NT, kk = rk - (rk / NT) * NT;
-        int ci = 0, sl = threadIdx.x / 64;
+        int ci = 0, sl = (threadIdx.x / 64) * PW + slot;
         while (sl >= nsl && ci < nc) {
             sl -= nsl;
             ++ci;
@@ -1419,8 +1424,8 @@ __global__ void __launch_bounds__(RHS_THREADS)
                 sum += -det * du - h * dth;
                 acc[(nF + ci * NQ + qq) * wcol + sl * NT + kk] = sum * scale;
             }
-            // next (cell, slab) pair of this thread: one per wave further on
-            sl += blockDim.x / 64;
+            // next (cell, slab) pair of this thread: PW per wave further on
+            sl += pstep;
             while (sl >= nsl && ci < nc) {
                 sl -= nsl;
                 ++ci;
@@ -2304,12 +2309,13 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                         default: throw std::runtime_error("patch right-hand sides: unsupported (dim, p)");
                     }
                 };
-                // slabs per workgroup (1, 2, 4, 8 within 64 KiB of LDS): timed once per
-                // system and padded order, each candidate twice (the passes only write RHS)
+                // slabs per workgroup (1, 2, 4, .., 32 within 64 KiB of LDS; small patches
+                // amortise the workgroup's table and map loads over more slabs): timed once
+                // per system and padded order, each candidate twice (the passes only write RHS)
                 auto it = sys->rhs_sg.find(bk.np);
                 if (it == sys->rhs_sg.end()) {
                     std::vector<int> cand;
-                    for (int g = 8; g >= 1; g /= 2)
+                    for (int g = 32; g >= 1; g /= 2)
                         if (g <= P.m && (lr(g) <= 64 * 1024 || g == 1)) cand.push_back(g);
                     const char* tune_env = std::getenv("APF_SOLVE_TUNE");
                     int best = cand.empty() ? 1 : cand.front();
