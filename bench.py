@@ -31,7 +31,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors a
 # plain copy kernel of known size reads) from the rocprofv3 --pmc runs of
 # `bench.py --profile`, summarised in profiles/.
-MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9, "solve": 2.28e9, "factor": 0.28e9}}
+MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9, "solve": 2.49e9, "factor": 0.28e9}}
 
 
 def workload(name):
