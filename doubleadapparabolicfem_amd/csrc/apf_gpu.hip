@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(256) k_fill_kron(int nY, int n, int nf, int m,
             r -= deg;
         }
         const int j = s_sb[cur] + r, t = cur * SEG + g;
-        out[e] = Msp[j] * s_aM[t] + (Ksp[j] * s_h[t]) * s_aK[t];
+        __builtin_nontemporal_store(Msp[j] * s_aM[t] + (Ksp[j] * s_h[t]) * s_aK[t], &out[e]);
     }
 }
 
