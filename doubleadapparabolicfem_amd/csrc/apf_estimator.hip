@@ -1437,7 +1437,7 @@ __global__ void __launch_bounds__(RHS_THREADS)
     double* out = RHS + ps.off1[q] * ncols + static_cast<int64_t>(k0) * NT;
     for (int e = threadIdx.x; e < np * ncw; e += blockDim.x) {
         const int row = e / ncw, c = e % ncw;
-        out[static_cast<int64_t>(row) * ncols + c] = acc[row * wcol + c];
+        __builtin_nontemporal_store(acc[row * wcol + c], &out[static_cast<int64_t>(row) * ncols + c]);
     }
 }
 
@@ -1687,7 +1687,7 @@ __global__ void __launch_bounds__(solve_threads(CT))
             const int fi = __shfl(my_fi, t, 64);
             const double sg = __shfl(my_sg, t, 64);
             double* out = sig + __shfl(my_off, t, 64);
-            for (int c = lane; c < ncw; c += 64) out[c] = (fi >= 0) ? sg * Xs[fi * CW + c] : 0.0;
+            for (int c = lane; c < ncw; c += 64) __builtin_nontemporal_store((fi >= 0) ? sg * Xs[fi * CW + c] : 0.0, &out[c]);
         }
     }
 }
