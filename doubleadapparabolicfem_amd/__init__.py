@@ -90,6 +90,13 @@ class _EstimateOut(ctypes.Structure):
                 ("seconds_slab", ctypes.c_double)]
 
 
+class _GEstimateOut(ctypes.Structure):
+    _fields_ = [("lambda_difference_squared", ctypes.c_double), ("lambda_difference", ctypes.c_double),
+                ("raw", ctypes.c_double), ("coefficient_difference_squared", ctypes.c_double),
+                ("solver_relative_residual", ctypes.c_double), ("solver_iterations", ctypes.c_int),
+                ("g_true_dofs", ctypes.c_int64), ("seconds", ctypes.c_double)]
+
+
 class _DoerflerDiag(ctypes.Structure):
     _fields_ = [("total", ctypes.c_double), ("target", ctypes.c_double), ("cutoff_value", ctypes.c_double),
                 ("cutoff_margin", ctypes.c_double), ("max_indicator", ctypes.c_double),
@@ -130,6 +137,8 @@ EXPORTED = {
     "apf_estimate_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(_EstimateOut), ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p]),
+    "apf_g_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                      ctypes.POINTER(_GEstimateOut)]),
     "apf_system_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_estimate_phase_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_combine_estimator": (ctypes.c_int, [ctypes.c_int64] + [ctypes.c_void_p] * 4),
@@ -351,6 +360,16 @@ class System:
         res = {f: getattr(out, f) for f, _ in _EstimateOut._fields_}
         res.update(flux_by_cell=fl, resid_by_cell=rs, recon_by_cell=rc)
         return res
+
+    def g_estimate(self, gsys, tol=1e-12, max_it=100):
+        """G-estimator of this system's solution (algorithm/adaptive_driver.hpp:2680-2905).
+        gsys: a degree p+1 system on the same mesh, assembled with the same
+        coefficients, with coarse meshes for its multigrid; its solution vector is
+        overwritten."""
+        out = _GEstimateOut()
+        _check(lib().apf_g_estimate(self.handle, gsys.handle, tol, max_it, ctypes.byref(out)), self.ctx.handle,
+               "apf_g_estimate")
+        return {f: getattr(out, f) for f, _ in _GEstimateOut._fields_}
 
     def use_graph(self, on=True):
         _check(lib().apf_system_use_graph(self.handle, 1 if on else 0), self.ctx.handle)

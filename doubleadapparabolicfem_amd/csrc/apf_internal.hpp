@@ -232,6 +232,16 @@ struct System {
     // mixed path: buckets (by padded order) that run the FP64 factor and solve because that
     // measured faster than the FP32 factor with refinement
     std::map<int, bool> mixed_fp64;
+    // G-estimator on this (degree p+1) system: nodal interpolation from the degree-p
+    // space g_src (spatial CSR over free nodes, time matrix [p+2][p+1]) and two
+    // Y-sized vectors (I lambda, lambda_G)
+    const Space* g_src = nullptr;
+    int* d_g_ptr = nullptr;
+    int* d_g_col = nullptr;
+    double* d_g_val = nullptr;
+    double* d_g_It = nullptr;
+    double* d_g_il = nullptr;
+    double* d_g_lt = nullptr;
     float t_assembly_ms = 0.f, t_patch_ms = 0.f, t_diag_ms = 0.f, t_spmv_ms = 0.f;
     float t_factor_ms = 0.f, t_rhs_ms = 0.f, t_solve_ms = 0.f;
     bool assembled = false;

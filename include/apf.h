@@ -215,6 +215,29 @@ int apf_system_timings(apf_system* sys, double* ms4);
  * [2] batched solves with refinement. */
 int apf_estimate_phase_timings(apf_system* sys, double* ms3);
 
+/* ---- G-estimator ---- */
+typedef struct {
+    double lambda_difference_squared;       /* G^2 = max(0, raw) (g_lambda_difference_squared) */
+    double lambda_difference;               /* G */
+    double raw;                             /* (lambda_G - I lambda)^T A_G (lambda_G - I lambda) */
+    double coefficient_difference_squared;  /* |lambda_G - I lambda|^2 over the coefficients */
+    double solver_relative_residual;        /* |rhs - A_G lambda_G| / |rhs| */
+    int solver_iterations;
+    int64_t g_true_dofs;                    /* dimension of G */
+    double seconds;
+} apf_g_estimate_out;
+/* G-estimator of the degree-p solution held by `sys` (after apf_minres_solve or
+ * apf_system_set_solution).  Replaces compute_g_estimator_for_accepted_inner_iteration_
+ * (algorithm/adaptive_driver.hpp:2680-2905).  `gsys` is a system of degree p+1
+ * on the same mesh, assembled with the same coefficients and given the coarse
+ * meshes of its multigrid (apf_space_add_coarse_mesh); G is its Y space.  Its
+ * solution vector is overwritten: [I lambda; I u] (nodal interpolation, as
+ * prolong_true_coefficients_nodal).  lambda_G solves A_G lambda_G = f_G - B^T I u
+ * to the relative residual `tol` (slab solve, at most `max_it` iterations).
+ * A raw value below -1000 eps max(1, |lambda_G - I lambda|^2) returns APF_BAD_ARG,
+ * as the reference's runtime_error at adaptive_driver.hpp:2869-2883. */
+int apf_g_estimate(apf_system* sys, apf_system* gsys, double tol, int max_it, apf_g_estimate_out* out);
+
 /* ---- marking (host; no GPU needed) ---- */
 typedef struct {
     double total, target;          /* Neumaier total over ids ascending, theta * total */

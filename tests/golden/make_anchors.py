@@ -5,6 +5,12 @@ Reads the reference's published CSV histories
 mode == "uniform_standard") and writes tests/golden/anchors.json.  The values
 are data (6 significant digits as published); no reference code is run.
 
+Also writes tests/golden/g_anchors.json: the G-estimator values
+(g_lambda_difference_squared) of the adaptive runs' first record on the initial
+mesh, identified by X true DOFs equal to the uniform run's outer iteration 0, G
+true DOFs equal to the degree-(p+1) uniform Y true DOFs at outer iteration 0, and
+the same eta^2 and lambda_y^2 as that uniform record.
+
 Usage:  python tests/golden/make_anchors.py [/root/reference]
 """
 import csv
@@ -36,6 +42,32 @@ def main():
     with open(dst, "w") as fh:
         json.dump(out, fh, indent=1)
     print(f"wrote {len(out)} anchors to {dst}")
+    first = {(a["dim"], a["p"]): a for a in out if a["outer"] == 0}
+    gout = []
+    for dim in ("1d", "2d"):
+        path = os.path.join(REF, "algorithm_data_plot", dim, "smooth_initial.csv")
+        seen = set()
+        with open(path) as fh:
+            for i, row in enumerate(csv.DictReader(fh)):
+                if row["g_estimator_computed"] != "1" or not row["g_lambda_difference_squared"]:
+                    continue
+                d, p = int(dim[0]), int(row["p_int"])
+                u, uq = first.get((d, p)), first.get((d, p + 1))
+                if u is None or uq is None or (d, p) in seen:
+                    continue
+                if (int(float(row["x_true_dofs"])) != u["x_true_dofs"]
+                        or int(float(row["g_true_dofs"])) != uq["y_true_dofs_after"]
+                        or float(row["eta_squared"]) != u["eta_squared"]
+                        or float(row["lambda_y_squared"]) != u["lambda_y_squared"]):
+                    continue
+                seen.add((d, p))
+                gout.append({"dim": d, "p": p, "outer": 0, "source": f"algorithm_data_plot/{dim}/smooth_initial.csv:{i + 2}",
+                             "mode": row["mode"], "g_true_dofs": int(float(row["g_true_dofs"])),
+                             "g_lambda_difference_squared": float(row["g_lambda_difference_squared"])})
+    gdst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "g_anchors.json")
+    with open(gdst, "w") as fh:
+        json.dump(gout, fh, indent=1)
+    print(f"wrote {len(gout)} G anchors to {gdst}")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,99 @@
+"""GPU parity of the G-estimator against the oracle (through the C ABI).
+
+The published G values are on the initial meshes, which have no coarser mesh
+for the multigrid of the slab solve; the oracle carries those anchors
+(test_oracle_g_estimator.py) and the GPU path is held to the oracle on refined
+meshes, with the same degree-p solution on both sides."""
+import numpy as np
+import pytest
+
+import doubleadapparabolicfem_amd as apf
+from doubleadapparabolicfem_amd import meshes
+from oracle import assembly as OA, estimator as OE, g_estimator as OG
+
+from gpu_helpers import oracle_setup
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return apf.Context(0)
+
+
+def _systems(ctx, m, dim, outer, p, coeff):
+    sp = apf.Space(m, p)
+    sysp = apf.System(ctx, sp)
+    sysp.assemble(coeff)
+    sq = apf.Space(m, p + 1, coarse=meshes.uniform_hierarchy(dim, outer))
+    sysq = apf.System(ctx, sq)
+    sysq.assemble(coeff)
+    return sp, sysp, sq, sysq
+
+
+@pytest.mark.parametrize("kappa", [False, True], ids=["M=I", "checkerboard"])
+@pytest.mark.parametrize("dim,p,outer", [(1, 1, 2), (1, 2, 2), (1, 3, 1), (2, 1, 1), (2, 2, 1), (2, 1, 2)])
+def test_g_estimator_matches_oracle(ctx, dim, p, outer, kappa):
+    """G^2 on the oracle's direct solution: the slab solve to 1e-12 relative
+    residual against the oracle's direct solve agrees to 1e-8 relative."""
+    if kappa and dim == 1:
+        pytest.skip("the checkerboard coefficient is two-dimensional")
+    m = meshes.uniform_mesh(dim, outer)
+    coeff = meshes.smooth_initial(dim)
+    if kappa:
+        coeff = apf.Coefficients(meshes.checkerboard_kappa(m, 4, seed=3), 0.0, coeff.u0)
+    print(f"g-estimator dim={dim} p={p} outer={outer} kappa={kappa}", flush=True)
+    stm, od, prob = oracle_setup(m, p, coeff)
+    x = OE.main_solve(OA.assemble(stm, od, prob))
+    g2_ref, lG, _, _ = OG.g_estimator(stm, p, prob, x)
+    _, sysp, _, sysq = _systems(ctx, m, dim, outer, p, coeff)
+    sysp.set_solution(x)
+    g = sysp.g_estimate(sysq, tol=1e-12, max_it=200)
+    assert g["g_true_dofs"] == len(lG)
+    assert g["solver_relative_residual"] <= 1e-12
+    assert g2_ref > 0
+    assert abs(g["lambda_difference_squared"] - g2_ref) <= 1e-8 * g2_ref, (g, g2_ref)
+    assert g["lambda_difference"] == pytest.approx(np.sqrt(g["lambda_difference_squared"]), rel=1e-15)
+
+
+def test_g_estimator_after_minres_and_repeat(ctx):
+    """Full GPU pipeline (assembly, MINRES to 1e-10, G): close to the oracle on
+    its direct solution, and a second call on the same inputs is bit-identical."""
+    dim, p, outer = 2, 2, 2
+    m = meshes.uniform_mesh(dim, outer)
+    coeff = meshes.smooth_initial(dim)
+    print(f"g-estimator after minres dim={dim} p={p} outer={outer}", flush=True)
+    stm, od, prob = oracle_setup(m, p, coeff)
+    x = OE.main_solve(OA.assemble(stm, od, prob))
+    g2_ref = OG.g_estimator(stm, p, prob, x)[0]
+    sp = apf.Space(m, p, coarse=meshes.uniform_hierarchy(dim, outer))
+    sysp = apf.System(ctx, sp)
+    sysp.assemble(coeff)
+    sysp.minres(tol=1e-10, max_it=600, check_interval=10)
+    sq = apf.Space(m, p + 1, coarse=meshes.uniform_hierarchy(dim, outer))
+    sysq = apf.System(ctx, sq)
+    sysq.assemble(coeff)
+    a = sysp.g_estimate(sysq)
+    b = sysp.g_estimate(sysq)
+    assert a["lambda_difference_squared"] == b["lambda_difference_squared"]
+    assert abs(a["lambda_difference_squared"] - g2_ref) <= 1e-6 * g2_ref, (a, g2_ref)
+
+
+def test_g_estimator_rejects_mismatched_systems(ctx):
+    m = meshes.uniform_mesh(2, 1)
+    coeff = meshes.smooth_initial(2)
+    print("g-estimator argument checks", flush=True)
+    sp = apf.Space(m, 1)
+    sysp = apf.System(ctx, sp)
+    sysp.assemble(coeff)
+    same = apf.System(ctx, apf.Space(m, 1, coarse=meshes.uniform_hierarchy(2, 1)))
+    same.assemble(coeff)
+    with pytest.raises(apf.APFError, match="degree p\\+1"):
+        sysp.g_estimate(same)
+    other = apf.System(ctx, apf.Space(meshes.uniform_mesh(2, 2), 2, coarse=meshes.uniform_hierarchy(2, 2)))
+    other.assemble(coeff)
+    with pytest.raises(apf.APFError, match="different meshes"):
+        sysp.g_estimate(other)
+    unassembled = apf.System(ctx, apf.Space(m, 2, coarse=meshes.uniform_hierarchy(2, 1)))
+    with pytest.raises(apf.APFError, match="assemble"):
+        sysp.g_estimate(unassembled)
