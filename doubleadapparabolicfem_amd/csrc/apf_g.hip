@@ -232,7 +232,7 @@ extern "C" int apf_g_estimate(apf_system* sys, apf_system* gsys, double tol, int
         double* y = gsys->d_vec[3];
         const int nparts = gsys->nparts;
         hipLaunchKernelGGL(k_gdiff, dim3(nparts), dim3(256), 0, st, nYq, Sq.n, gsys->d_g_lt, gsys->d_g_il, t);
-        gsys->spmv(t, y);
+        gsys->spmv_rows(t, y, nYq);
         hipLaunchKernelGGL(k_gdot_parts, dim3(nparts), dim3(256), 0, st, nYq, t, y, gsys->d_part);
         std::vector<double> parts(2 * static_cast<size_t>(nparts));
         HIP_TRY(hipMemcpyAsync(parts.data(), gsys->d_part, parts.size() * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -715,14 +715,15 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     // |f - B^T u|: the lambda rows of K [0; u]
     HIP_TRY(hipMemsetAsync(t, 0, nY * sizeof(double), st));
     HIP_TRY(hipMemcpyAsync(t + nY, sys->d_x + nY, (n - nY) * sizeof(double), hipMemcpyDeviceToDevice, st));
-    sys->spmv(t, y);
+    // only the lambda rows of K are read below
+    sys->spmv_rows(t, y, nY);
     const double g2 = resid2(y);
     HIP_TRY(hipMemcpyAsync(lt, sys->d_x, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
     int it = 0;
     double rr = 0.0, prev = 0.0;
     for (;; ++it) {
         HIP_TRY(hipMemcpyAsync(t, lt, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
-        sys->spmv(t, y);
+        sys->spmv_rows(t, y, nY);
         const double r2 = resid2(y);
         rr = g2 > 0.0 ? std::sqrt(r2 / g2) : std::sqrt(r2);
         // stop at the tolerance, or at the round-off floor (less than 10% gained)
@@ -740,9 +741,11 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     *rel = rr;
 }
 
-void System::spmv(const double* x, double* y)
+void System::spmv(const double* x, double* y) { spmv_rows(x, y, space->n); }
+
+// y[0:n) = K[0:n, :] x (the first n rows only)
+void System::spmv_rows(const double* x, double* y, int64_t n)
 {
-    const int64_t n = space->n;
     const int block = 256;
     auto launch = [&](auto kern, int lanes, int rp) {
         const int64_t rows_per_block = static_cast<int64_t>(block / lanes) * rp;

@@ -248,6 +248,7 @@ struct System {
     bool use_graph = true;   // capture the MINRES iteration in a hipGraph
 
     void spmv(const double* x, double* y);
+    void spmv_rows(const double* x, double* y, int64_t rows);
 };
 
 void estimator_alloc(System* sys);
