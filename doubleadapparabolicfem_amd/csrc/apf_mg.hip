@@ -148,13 +148,14 @@ __global__ void __launch_bounds__(256)
 
 // One Chebyshev step with its product: x += d ; r -= A d ; dn = c1 d + c2 D^-1 r.
 // A d reads d of the neighbouring rows, so the new direction goes to a second
-// buffer dn (the caller swaps them).
+// buffer dn (the caller swaps them).  The first step from a zero start (r0 =
+// B, x0 = 0) reads B for r and writes x = d.
 template <typename R, int NP>
 __global__ void __launch_bounds__(256)
     k_mg_cheb_fused(int nf, const int* __restrict__ ptr, const int* __restrict__ col, const R* __restrict__ K,
                     const R* __restrict__ M, const R* __restrict__ sig, int nc, int ld, const R* __restrict__ d,
                     R* __restrict__ x, R* __restrict__ r, R* __restrict__ dn, const R* __restrict__ dK,
-                    const R* __restrict__ dM, R c1, R c2, int last)
+                    const R* __restrict__ dM, R c1, R c2, int last, const R* __restrict__ B0)
 {
     const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
     if (row >= nf) return;
@@ -162,27 +163,29 @@ __global__ void __launch_bounds__(256)
     row_apply<NP>(row, ptr, col, K, M, sig, nc, ld, d, [&](int c, R v) {
         const int64_t o = static_cast<int64_t>(row) * ld + c;
         const R dv = d[o];
-        x[o] += dv;
-        const R rv = r[o] - v;
+        R rv;
+        if (B0) {
+            x[o] = dv;
+            rv = B0[o] - v;
+        } else {
+            x[o] += dv;
+            rv = r[o] - v;
+        }
         r[o] = rv;
         if (!last) dn[o] = c1 * dv + c2 * rv / (dk + sig[c] * dm);
     });
 }
 
-// zero start: x = 0, r = B, d = D^-1 B / theta (no memset/memcpy nodes: graph safe)
+// zero start: d = D^-1 B / theta (x0 = 0 and r0 = B are folded into the first step)
 template <typename R>
 __global__ void k_cheb_start_zero(int nf, int nc, int ld, const R* __restrict__ dK, const R* __restrict__ dM,
-                                  const R* __restrict__ sig, R inv_theta, const R* __restrict__ B, R* __restrict__ x,
-                                  R* __restrict__ r, R* __restrict__ d)
+                                  const R* __restrict__ sig, R inv_theta, const R* __restrict__ B, R* __restrict__ d)
 {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<int64_t>(nf) * nc) return;
     const int i = static_cast<int>(t / nc), c = static_cast<int>(t % nc);
     const int64_t o = static_cast<int64_t>(i) * ld + c;
-    const R b = B[o];
-    x[o] = 0;
-    r[o] = b;
-    d[o] = b * inv_theta / (dK[i] + sig[c] * dM[i]);
+    d[o] = B[o] * inv_theta / (dK[i] + sig[c] * dM[i]);
 }
 
 // Y[:, 0:nc] = X[:, 0:nc]
@@ -524,7 +527,7 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_
     const unsigned rows_grid = grid_for(d.nf, 4);
     if (zero_start) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start_zero<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc,
-                           ld, d.dK, d.dM, sig, static_cast<R>(1) / theta, d.B, d.X, d.Rr, d.D);
+                           ld, d.dK, d.dM, sig, static_cast<R>(1) / theta, d.B, d.D);
     } else {
         with_passes(nc, [&](auto np) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_resid_start<R, decltype(np)::value>), dim3(rows_grid), dim3(256), 0,
@@ -542,7 +545,7 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_
         with_passes(nc, [&](auto np) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_cheb_fused<R, decltype(np)::value>), dim3(rows_grid), dim3(256), 0,
                                st, d.nf, d.ptr, d.col, d.K, d.M, sig, nc, ld, dcur, d.X, d.Rr, dnext, d.dK, d.dM, c1, c2,
-                               (k == nu - 1) ? 1 : 0);
+                               (k == nu - 1) ? 1 : 0, (zero_start && k == 0) ? d.B : nullptr);
         });
         std::swap(dcur, dnext);
         rho = rho_n;
