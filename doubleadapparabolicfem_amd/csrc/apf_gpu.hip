@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -649,6 +650,23 @@ __global__ void k_axpby(int64_t n, double a, const double* x, double b, double* 
         y[i] = a * x[i] + b * y[i];
 }
 
+__global__ void k_dot_parts(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                            double* __restrict__ part)
+{
+    double s = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        s += a[i] * b[i];
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_dot_final(const double* part, int nparts, double* out)
+{
+    const double s = reduce_partials(part, nparts);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
 // ============================================================== host side
 namespace apf {
 
@@ -689,9 +707,12 @@ void mg_prepare(System* sys, bool mg32)
 // Slab reconstruction (time_slabs/time_slab_reconstruction.hpp:584-735): on a
 // tensor mesh the slab space is Y restricted to the slab, so lambda_tilde
 // solves  A_y lt = f - B^T u  slab by slab (A_y is block diagonal over the
-// slabs).  Preconditioned Richardson from lt = lambda with
-// P_Y = K^-1 (x) (h_k Mt)^-1, one multigrid V-cycle per spatial inverse, for
-// all slabs at once, until |f - B^T u - A_y lt| <= tol |f - B^T u|.
+// slabs).  Preconditioned conjugate gradients from lt = lambda, for all slabs
+// at once, with P_Y = K^-1 (x) (h_k Mt)^-1 (one symmetric multigrid V-cycle per
+// spatial inverse), until the true residual |f - B^T u - A_y lt| <= tol |f - B^T u|.
+// A CG cycle ends when its recursive residual reaches tol / 10; the next one
+// restarts from the true residual, and the solve stops when a restart gains
+// less than 10% (the round-off floor).
 void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel)
 {
     const Space& S = *sys->space;
@@ -719,22 +740,48 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     sys->spmv_rows(t, y, nY);
     const double g2 = resid2(y);
     HIP_TRY(hipMemcpyAsync(lt, sys->d_x, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double* pd = sys->d_vec[4];  // search direction [p; 0]
+    auto dot = [&](const double* a, const double* b) {
+        hipLaunchKernelGGL(k_dot_parts, dim3(nparts), dim3(256), 0, st, nY, a, b, sys->d_part);
+        hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
+        double h = 0.0;
+        HIP_TRY(hipMemcpyAsync(&h, sys->d_res, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return h;
+    };
+    HIP_TRY(hipMemsetAsync(r + nY, 0, (n - nY) * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(pd + nY, 0, (n - nY) * sizeof(double), st));
+    const double scale = g2 > 0.0 ? g2 : 1.0;
     int it = 0;
-    double rr = 0.0, prev = 0.0;
-    for (;; ++it) {
+    double rr = 0.0, prev = std::numeric_limits<double>::infinity();
+    for (;;) {
+        // true residual r = [f - B^T u - A_y lt; 0]
         HIP_TRY(hipMemcpyAsync(t, lt, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         sys->spmv_rows(t, y, nY);
-        const double r2 = resid2(y);
-        rr = g2 > 0.0 ? std::sqrt(r2 / g2) : std::sqrt(r2);
-        // stop at the tolerance, or at the round-off floor (less than 10% gained)
-        if (!(rr > tol) || it >= max_it || (it >= 2 && rr > 0.9 * prev)) break;
+        rr = std::sqrt(resid2(y) / scale);
+        if (!(rr > tol) || it >= max_it || rr > 0.9 * prev) break;
         prev = rr;
-        // r = [f - B^T u - A_y lt; 0],  lt += P_Y r
         HIP_TRY(hipMemcpyAsync(r, sys->d_b, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -1.0, y, 1.0, r);
-        HIP_TRY(hipMemsetAsync(r + nY, 0, (n - nY) * sizeof(double), st));
-        sys->mg64->apply(st, S, r, z, sys->d_part, nparts);
-        hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, 1.0, z, 1.0, lt);
+        double rz_old = 0.0;
+        for (int k = 0; it < max_it; ++k) {
+            sys->mg64->apply(st, S, r, z, sys->d_part, nparts);
+            const double rz = dot(r, z);
+            if (!(rz > 0.0)) break;
+            if (k == 0)
+                HIP_TRY(hipMemcpyAsync(pd, z, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+            else
+                hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, 1.0, z, rz / rz_old, pd);
+            rz_old = rz;
+            sys->spmv_rows(pd, y, nY);  // A_y p
+            const double pap = dot(pd, y);
+            if (!(pap > 0.0)) break;
+            const double alpha = rz / pap;
+            hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, alpha, pd, 1.0, lt);
+            hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -alpha, y, 1.0, r);
+            ++it;
+            if (std::sqrt(dot(r, r) / scale) <= 0.1 * tol) break;
+        }
     }
     HIP_TRY(hipGetLastError());
     *its = it;
