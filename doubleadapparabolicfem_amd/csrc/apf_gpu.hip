@@ -765,7 +765,7 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
         hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -1.0, y, 1.0, r);
         double rz_old = 0.0;
         for (int k = 0; it < max_it; ++k) {
-            sys->mg64->apply(st, S, r, z, sys->d_part, nparts);
+            sys->mg64->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true);
             const double rz = dot(r, z);
             if (!(rz > 0.0)) break;
             if (k == 0)

@@ -580,7 +580,8 @@ void MGDevice<R>::vcycle(hipStream_t st, int l, int nc, const R* sig, const int*
 
 // z = P^-1 r for the saddle-point vector r = [r_Y; r_X]; writes dot(z, r) partials.
 template <typename R>
-void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts)
+void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts,
+                        bool y_only)
 {
     Level& f = levels[0];
     const int nf = f.nf;
@@ -601,14 +602,16 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
     // stage 2 (X columns): apply K, then the shifted V-cycle again
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, ncol, ld, f.X,
                        d_Z);
-    with_passes(T, [&](auto np) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R, decltype(np)::value>), dim3(grid_for(nf, 4)), dim3(256), 0, st,
-                           nf, f.ptr, f.col, f.K, f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
-    });
-    vcycle(st, 0, T, d_sig, d_sidx);
-    // X columns from the second V-cycle, Y columns from the first
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(static_cast<int64_t>(nf) * T, 256)), dim3(256), 0,
-                       st, nf, T, ld, f.X, d_Z);
+    if (!y_only) {
+        with_passes(T, [&](auto np) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R, decltype(np)::value>), dim3(grid_for(nf, 4)), dim3(256), 0,
+                               st, nf, f.ptr, f.col, f.K, f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
+        });
+        vcycle(st, 0, T, d_sig, d_sidx);
+        // X columns from the second V-cycle, Y columns from the first
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(static_cast<int64_t>(nf) * T, 256)), dim3(256),
+                           0, st, nf, T, ld, f.X, d_Z);
+    }
     if (wl)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R, true>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, S.Nt,
                            S.mesh.m(), ld, S.nY, th, d_Z, d_W, d_MtInvH, r, z, part);

@@ -59,7 +59,10 @@ struct MGDevice {
     void refresh(const MGSetupHost& H);
     void smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_start);
     void vcycle(hipStream_t st, int l, int nc, const R* sig, const int* sidx);
-    void apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts);
+    // y_only: the caller reads only z's Y part (r's X part is zero), so the second
+    // V-cycle over the X columns is skipped and z's X part is not the preconditioned value
+    void apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts,
+               bool y_only = false);
     int tile_rows() const;
     int n_partials(int nf) const;  // dot partials written by apply()
 };
