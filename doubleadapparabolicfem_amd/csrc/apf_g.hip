@@ -209,7 +209,9 @@ extern "C" int apf_g_estimate(apf_system* sys, apf_system* gsys, double tol, int
         HIP_TRY(hipSetDevice(sys->ctx->device));
         const auto t0 = std::chrono::steady_clock::now();
         HIP_TRY(hipStreamSynchronize(sys->stream));
-        if (gsys->g_src != &Sp) build_prolongation(gsys, Sp);
+        // The numbering of a space is a function of its mesh and degree, both checked
+        // above, so the interpolation built on the first call holds for every later one.
+        if (!gsys->g_src) build_prolongation(gsys, Sp);
         hipStream_t st = gsys->stream;
         const int m = Sq.mesh.m();
         const int64_t nXq = static_cast<int64_t>(m * Sq.p + 1) * Sq.nf;

@@ -233,8 +233,9 @@ struct System {
     // measured faster than the FP32 factor with refinement
     std::map<int, bool> mixed_fp64;
     // G-estimator on this (degree p+1) system: nodal interpolation from the degree-p
-    // space g_src (spatial CSR over free nodes, time matrix [p+2][p+1]) and two
-    // Y-sized vectors (I lambda, lambda_G)
+    // space on the same mesh (spatial CSR over free nodes, time matrix [p+2][p+1]),
+    // built on the first call (g_src: the space it was built from), and two Y-sized
+    // vectors (I lambda, lambda_G)
     const Space* g_src = nullptr;
     int* d_g_ptr = nullptr;
     int* d_g_col = nullptr;

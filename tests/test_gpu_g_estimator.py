@@ -69,7 +69,7 @@ def test_g_estimator_after_minres_and_repeat(ctx):
     sp = apf.Space(m, p, coarse=meshes.uniform_hierarchy(dim, outer))
     sysp = apf.System(ctx, sp)
     sysp.assemble(coeff)
-    sysp.minres(tol=1e-10, max_it=600, check_interval=10)
+    xs, _ = sysp.minres(tol=1e-10, max_it=600, check_interval=10)
     sq = apf.Space(m, p + 1, coarse=meshes.uniform_hierarchy(dim, outer))
     sysq = apf.System(ctx, sq)
     sysq.assemble(coeff)
@@ -77,6 +77,13 @@ def test_g_estimator_after_minres_and_repeat(ctx):
     b = sysp.g_estimate(sysq)
     assert a["lambda_difference_squared"] == b["lambda_difference_squared"]
     assert abs(a["lambda_difference_squared"] - g2_ref) <= 1e-6 * g2_ref, (a, g2_ref)
+    # another degree-p system on the same mesh with the same solution: the G system's
+    # interpolation, built on its first call, gives the same value bit for bit
+    sysp2 = apf.System(ctx, apf.Space(m, p))
+    sysp2.assemble(coeff)
+    sysp2.set_solution(xs)
+    c = sysp2.g_estimate(sysq)
+    assert c["lambda_difference_squared"] == a["lambda_difference_squared"]
 
 
 def test_g_estimator_rejects_mismatched_systems(ctx):
