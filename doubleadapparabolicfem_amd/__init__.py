@@ -34,6 +34,7 @@ APF_FP64, APF_FP32_REFINE = 0, 1
 APF_ASSEMBLE_KRON, APF_ASSEMBLE_SCATTER = 0, 1
 APF_PRECOND_DIAGONAL, APF_PRECOND_MULTIGRID = 0, 1
 APF_SLAB_FROM_SOLUTION, APF_SLAB_SOLVE = 0, 1
+APF_SLAB_STOP_NONE, APF_SLAB_STOP_TOL, APF_SLAB_STOP_FLOOR, APF_SLAB_STOP_MAX_IT = -1, 0, 1, 2
 
 _lib = None
 
@@ -87,14 +88,14 @@ class _EstimateOut(ctypes.Structure):
                 ("seconds_patches", ctypes.c_double), ("seconds_diagnostics", ctypes.c_double),
                 ("extra_refinements", ctypes.c_int), ("slab_iterations", ctypes.c_int),
                 ("slab_relative_residual", ctypes.c_double),
-                ("seconds_slab", ctypes.c_double)]
+                ("seconds_slab", ctypes.c_double), ("slab_stop", ctypes.c_int)]
 
 
 class _GEstimateOut(ctypes.Structure):
     _fields_ = [("lambda_difference_squared", ctypes.c_double), ("lambda_difference", ctypes.c_double),
                 ("raw", ctypes.c_double), ("coefficient_difference_squared", ctypes.c_double),
                 ("solver_relative_residual", ctypes.c_double), ("solver_iterations", ctypes.c_int),
-                ("g_true_dofs", ctypes.c_int64), ("seconds", ctypes.c_double)]
+                ("g_true_dofs", ctypes.c_int64), ("seconds", ctypes.c_double), ("solver_stop", ctypes.c_int)]
 
 
 class _DoerflerDiag(ctypes.Structure):

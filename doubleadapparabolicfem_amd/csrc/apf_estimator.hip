@@ -104,6 +104,13 @@ __device__ inline double block_max(double v, double* sh)
     return r;
 }
 
+// max of non-negative statistics in which a NaN counts as +infinity (fmax would drop it),
+// so that a non-finite patch solution or residual fails the acceptance test
+__device__ inline double max_nonfinite_wins(double a, double b)
+{
+    return (a != a || b != b) ? INFINITY : fmax(a, b);
+}
+
 __device__ inline void atomic_max_pos(double* addr, double v)
 {
     // v >= 0: the IEEE order of non-negative doubles matches their bit order
@@ -1596,7 +1603,7 @@ __global__ void __launch_bounds__(solve_threads(CT))
                 for (int r = 0; r < 4; ++r) {
                     const int row = rt * 16 + drow<double>(lk, r);
                     acc[ct][r] = (col < ncw) ? Bq[static_cast<int64_t>(row) * ncols + col] : 0.0;
-                    bm[ct] = fmax(bm[ct], fabs(acc[ct][r]));
+                    bm[ct] = max_nonfinite_wins(bm[ct], fabs(acc[ct][r]));
                 }
             }
             tile_product<CTW>(Kq + static_cast<int64_t>(rt * 16 + li) * np, np, Xs + cg * 16, CW, li, lk, acc, true);
@@ -1605,10 +1612,10 @@ __global__ void __launch_bounds__(solve_threads(CT))
                 for (int ct = 0; ct < CTW; ++ct) {
                     double rm = 0.0, bb = bm[ct];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) rm = fmax(rm, fabs(acc[ct][r]));
+                    for (int r = 0; r < 4; ++r) rm = max_nonfinite_wins(rm, fabs(acc[ct][r]));
                     for (int off = 16; off < 64; off <<= 1) {
-                        rm = fmax(rm, __shfl_xor(rm, off, 64));
-                        bb = fmax(bb, __shfl_xor(bb, off, 64));
+                        rm = max_nonfinite_wins(rm, __shfl_xor(rm, off, 64));
+                        bb = max_nonfinite_wins(bb, __shfl_xor(bb, off, 64));
                     }
                     if (lk == 0) {
                         // non-negative doubles order like their bit patterns
@@ -1636,7 +1643,10 @@ __global__ void __launch_bounds__(solve_threads(CT))
                         rm = fmax(rm, __longlong_as_double(static_cast<long long>(colmax[c])));
                         bm = fmax(bm, __longlong_as_double(static_cast<long long>(colmax[CW + c])));
                     }
-                    worst = fmax(worst, rm / fmax(1.0, bm));
+                    // non-finite residuals or right-hand sides fail, as the reference's
+                    // finiteness checks do (dense_local_error_blocks_2d.hpp:931-951)
+                    const double q = (bm < INFINITY) ? rm / fmax(1.0, bm) : INFINITY;
+                    worst = max_nonfinite_wins(worst, q);
                 }
                 s_worst = worst;
                 s_done = (worst <= 1e-10 || extra >= MAX_EXTRA) ? 1 : 0;
@@ -2177,9 +2187,11 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         // slab by slab; with the main solve converged it equals lambda.
         out->slab_iterations = 0;
         out->slab_relative_residual = -1.0;
+        out->slab_stop = APF_SLAB_STOP_NONE;
         const auto t_slab = std::chrono::steady_clock::now();
         if (slab_mode == APF_SLAB_SOLVE)
-            slab_reconstruct(sys, B.lt, 1e-13, 100, &out->slab_iterations, &out->slab_relative_residual);
+            slab_reconstruct(sys, B.lt, 1e-13, 100, &out->slab_iterations, &out->slab_relative_residual,
+                             &out->slab_stop);
         else
             HIP_TRY(hipMemcpyAsync(B.lt, sys->d_x, S.nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         out->seconds_slab = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_slab).count();

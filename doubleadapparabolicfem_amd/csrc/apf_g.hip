@@ -226,9 +226,10 @@ extern "C" int apf_g_estimate(apf_system* sys, apf_system* gsys, double tol, int
         HIP_TRY(hipMemcpyAsync(gsys->d_x, gsys->d_g_il, nYq * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIP_TRY(hipGetLastError());
         // lambda_G from I lambda: A_G lambda_G = f_G - B^T I u
-        int its = 0;
+        int its = 0, stop = APF_SLAB_STOP_NONE;
         double rel = 0.0;
-        slab_reconstruct(gsys, gsys->d_g_lt, tol, max_it, &its, &rel);
+        slab_reconstruct(gsys, gsys->d_g_lt, tol, max_it, &its, &rel, &stop);
+        out->solver_stop = stop;
         // d^T A_G d: the Y rows of K [d; 0]
         double* t = gsys->d_vec[2];
         double* y = gsys->d_vec[3];

@@ -700,6 +700,13 @@ void mg_prepare(System* sys, bool mg32)
         sys->mg64->build(sys, hs, nyc);
     }
     HIP_TRY(hipStreamSynchronize(st));
+    // apply() writes one dot partial per transpose tile into d_part: a fine level with many
+    // free nodes (e.g. the degree-(p+1) system of the G-estimator) needs more than 4 nparts
+    const int need = mg32 ? sys->mg32->n_partials(S.nf) : sys->mg64->n_partials(S.nf);
+    if (need > sys->part_cap) {
+        sys->d_part = dalloc<double>(sys, static_cast<size_t>(need));
+        sys->part_cap = need;
+    }
     ver = sys->assembly_version;
     sys->t_mg_setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
 }
@@ -714,7 +721,7 @@ void mg_prepare(System* sys, bool mg32)
 // restarted one (from the true residual) when that has dropped 10x or reached
 // tol / 10; the solve stops when a restart gains less than 10% of the true
 // residual (the round-off floor).
-void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel)
+void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel, int* stop)
 {
     const Space& S = *sys->space;
     if (!sys->H) throw std::runtime_error("slab reconstruction needs the nested coarse meshes of the multigrid");
@@ -760,12 +767,25 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
         HIP_TRY(hipMemcpyAsync(t, lt, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         sys->spmv_rows(t, y, nY);
         rr = std::sqrt(resid2(y) / scale);
-        if (!(rr > tol) || it >= max_it || rr > 0.9 * prev) break;
+        if (!(rr > tol)) {
+            *stop = APF_SLAB_STOP_TOL;
+            break;
+        }
+        if (it >= max_it) {
+            *stop = APF_SLAB_STOP_MAX_IT;
+            break;
+        }
+        if (rr > 0.9 * prev) {
+            *stop = APF_SLAB_STOP_FLOOR;
+            break;
+        }
         prev = rr;
         HIP_TRY(hipMemcpyAsync(r, sys->d_b, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -1.0, y, 1.0, r);
         double rz_old = 0.0;
         for (int k = 0; it < max_it; ++k) {
+            if (sys->mg64->n_partials(S.nf) > sys->part_cap)
+                throw std::runtime_error("too many multigrid transpose tiles");
             sys->mg64->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true);
             const double rz = dot(r, z);
             if (!(rz > 0.0)) break;
@@ -1096,6 +1116,7 @@ int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
         for (int i = 0; i < 8; ++i) sys->d_vec[i] = dalloc<double>(sys.get(), S.n);
         sys->nparts = 1024;
         sys->d_part = dalloc<double>(sys.get(), 4 * sys->nparts);
+        sys->part_cap = 4 * sys->nparts;
         sys->d_scal = dalloc<MinresScalars>(sys.get(), 1);
         sys->d_res = dalloc<double>(sys.get(), 4);
         const int Tn = S.mesh.m() * S.p + 1;
@@ -1309,7 +1330,7 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
         if (use_mg) {
             mg_prepare(sys, mg32);
             pparts = mg32 ? sys->mg32->n_partials(S.nf) : sys->mg64->n_partials(S.nf);
-            if (pparts > 4 * nparts) throw std::runtime_error("too many multigrid transpose tiles");
+            if (pparts > sys->part_cap) throw std::runtime_error("too many multigrid transpose tiles");
         } else {
             hipLaunchKernelGGL(k_precond_setup, dim3((S.nf + 127) / 128), dim3(128), 0, st, S.nf, S.p, S.mesh.m(),
                                sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_kdiag,

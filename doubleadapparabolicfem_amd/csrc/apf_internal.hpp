@@ -212,6 +212,7 @@ struct System {
     double* d_vec[8] = {};
     int nparts = 0;
     double* d_part = nullptr;
+    int64_t part_cap = 0;  // doubles in d_part (at least 4 nparts; grown to the multigrid's transpose tiles)
     MinresScalars* d_scal = nullptr;
     double* d_res = nullptr;
     double* d_kdiag = nullptr;
@@ -254,7 +255,9 @@ struct System {
 
 void estimator_alloc(System* sys);
 void mg_prepare(System* sys, bool mg32);
-void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel);
+// stop reason: APF_SLAB_STOP_TOL (relative residual <= tol), APF_SLAB_STOP_FLOOR (a restart
+// gained less than 10%: the FP64 round-off floor), APF_SLAB_STOP_MAX_IT
+void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel, int* stop);
 
 #define HIP_TRY(expr)                                                                    \
     do {                                                                                 \

@@ -13,8 +13,9 @@ little-endian arrays, int32 for ids and float64 for coordinates and values.
 Cell ids: the space-time cell of slab k and spatial cell c has id k * ns + c,
 the order of this package's per-cell estimator arrays.  (The reference numbers
 the active cells of its refinement forest; on the tensor meshes here these ids
-take that role.)  Spatial face f of a triangle is the edge opposite vertex f,
-of an interval its vertex f; temporal face 0 is a cell's bottom (t_k) and face 1
+take that role.)  Spatial face f of a triangle is the edge (v_f, v_{f+1 mod 3}), as
+fill_spatial_faces_2d sets it (src/mesh/topology/faces_2d.hpp:15-37), and face f
+of an interval is its vertex f (faces_1d.hpp:15-25); temporal face 0 is a cell's bottom (t_k) and face 1
 its top.  A face without a neighbour lists -1, as the reference's
 ``flatten_neighbours`` does (write_mesh_binary_detail.hpp:149-186).
 """
@@ -24,7 +25,7 @@ MESH_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:10
 CELLWISE_ERROR_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:20
 CELLWISE_ERROR_COMPONENTS_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:21
 
-_TRI_FACE = ((1, 2), (2, 0), (0, 1))  # face f of a triangle: the edge opposite vertex f
+_TRI_FACE = ((0, 1), (1, 2), (2, 0))  # face f of a triangle: edge (v_f, v_f+1) (faces_2d.hpp:15-37)
 
 
 def _write(path, blocks):

@@ -64,6 +64,12 @@ enum apf_precond_kind { APF_PRECOND_DIAGONAL = 0, APF_PRECOND_MULTIGRID = 1 };
 /* slab reconstruction: lambda_tilde = lambda of the current solution (exact when the
  * main solve is converged), or solved slab by slab to 1e-13 relative */
 enum apf_slab_mode { APF_SLAB_FROM_SOLUTION = 0, APF_SLAB_SOLVE = 1 };
+/* Why an iterative slab solve (APF_SLAB_SOLVE, apf_g_estimate) stopped: the tolerance is
+ * met, a restart from the true residual gained less than 10% (the FP64 round-off floor
+ * of the system, reached above the tolerance), or the iteration limit.  The reference
+ * solves these systems directly (time_slabs/time_slab_reconstruction_operator.hpp:810-814),
+ * so a caller that needs the tolerance checks for APF_SLAB_STOP_TOL. */
+enum apf_slab_stop { APF_SLAB_STOP_NONE = -1, APF_SLAB_STOP_TOL = 0, APF_SLAB_STOP_FLOOR = 1, APF_SLAB_STOP_MAX_IT = 2 };
 
 typedef struct apf_ctx apf_ctx;
 typedef struct apf_space apf_space;
@@ -122,6 +128,7 @@ typedef struct {
     int slab_iterations;                     /* APF_SLAB_SOLVE: preconditioned iterations */
     double slab_relative_residual;           /* |f - B^T u - A_y lt| / |f - B^T u| (-1: not solved) */
     double seconds_slab;
+    int slab_stop;                           /* enum apf_slab_stop; APF_SLAB_STOP_NONE when not solved */
 } apf_estimate_out;
 
 /* ---- host-side integer work (no GPU needed) ---- */
@@ -225,6 +232,7 @@ typedef struct {
     int solver_iterations;
     int64_t g_true_dofs;                    /* dimension of G */
     double seconds;
+    int solver_stop;                        /* apf_slab_stop: why the slab solve ended */
 } apf_g_estimate_out;
 /* G-estimator of the degree-p solution held by `sys` (after apf_minres_solve or
  * apf_system_set_solution).  Replaces compute_g_estimator_for_accepted_inner_iteration_

@@ -21,7 +21,10 @@ def ctx():
     return apf.Context(0)
 
 
-CASES = [(1, 1, 3), (1, 2, 2), (1, 3, 1), (2, 1, 0), (2, 1, 1), (2, 2, 0), (2, 2, 1), (2, 3, 0), (2, 1, 2)]
+# (2, 3, 1) with the checkerboard kappa is the C4 patch population: p=3 union-jack
+# interior patches of order 209 per time mode, factored by block elimination
+CASES = [(1, 1, 3), (1, 2, 2), (1, 3, 1), (2, 1, 0), (2, 1, 1), (2, 2, 0), (2, 2, 1), (2, 3, 0), (2, 1, 2),
+         (2, 3, 1)]
 
 
 def _rel(a, b):
@@ -75,6 +78,56 @@ def test_gpu_pipeline_reproduces_published_anchor(ctx, rec):
         assert _rel(est[key], rec[col]) <= 6e-6, (key, est[key], rec[col])
 
 
+def test_gpu_pipeline_reproduces_published_anchor_at_full_c2_size(ctx):
+    """The C2 workload itself: 2+1D p=2, union-jack 64x64 x 32 intervals
+    (2,596,769 unknowns), assembly -> multigrid MINRES (1e-10) -> mixed estimator,
+    with the slab reconstruction taken from the solution and solved, against the
+    published record algorithm_data_plot/2d/smooth_initial.csv:740."""
+    rec = [a for a in ANCHORS if (a["dim"], a["p"], a["outer"]) == (2, 2, 5)][0]
+    print("full C2 pipeline", flush=True)
+    m = meshes.uniform_mesh(2, 5)
+    sp = apf.Space(m, 2, coarse=meshes.uniform_hierarchy(2, 5))
+    assert (sp.nX, sp.nY) == (rec["x_true_dofs"], rec["y_true_dofs_after"])
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(meshes.smooth_initial(2))
+    _, st = sysm.minres(tol=1e-10, max_it=2000, check_interval=10, download=False)
+    assert st["true_relative_residual"] <= 1e-10
+    for slab in (apf.APF_SLAB_FROM_SOLUTION, apf.APF_SLAB_SOLVE):
+        est = sysm.estimate(apf.APF_FP32_REFINE, 2, per_cell=False, slab=slab)
+        assert est["max_patch_residual"] <= 1e-10
+        for key, col in (("lambda_y2", "lambda_y_squared"), ("eta2", "eta_squared"), ("flux2", "y_flux_squared"),
+                         ("y_est2", "y_estimator_squared")):
+            assert _rel(est[key], rec[col]) <= 6e-6, (slab, key, est[key], rec[col])
+    # the slab solve starts from lambda, which already meets the first block row to the
+    # MINRES tolerance: its reconstruction term stays at the level of that residual
+    assert est["slab_stop"] in (apf.APF_SLAB_STOP_TOL, apf.APF_SLAB_STOP_FLOOR)
+    assert est["recon2"] <= 1e-12 * est["lambda_y2"]
+
+
+def test_non_finite_patch_input_fails_the_acceptance(ctx):
+    """A NaN in one solution entry reaches the right-hand sides of the patches
+    around its cell: those patches fail the 1e-10 acceptance (the reference's
+    finiteness checks, dense_local_error_blocks_2d.hpp:931-951) and the estimate
+    returns APF_RESIDUAL_TOO_LARGE instead of NaN indicators."""
+    print("planted NaN in the estimator input", flush=True)
+    m = meshes.uniform_mesh(2, 1)
+    coeff = meshes.smooth_initial(2)
+    stm, od, prob = oracle_setup(m, 2, coeff)
+    x = OE.main_solve(OA.assemble(stm, od, prob))
+    sp = apf.Space(m, 2)
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(coeff)
+    for prec in (apf.APF_FP64, apf.APF_FP32_REFINE):
+        sysm.set_solution(x)
+        sysm.estimate(prec, 2)  # finite input: accepted
+        bad = x.copy()
+        bad[od.nY // 2] = np.nan
+        sysm.set_solution(bad)
+        with pytest.raises(apf.APFError) as e:
+            sysm.estimate(prec, 2)
+        assert e.value.code == apf.APF_RESIDUAL_TOO_LARGE, e.value
+
+
 @pytest.mark.parametrize("dim,p,outer", [(1, 1, 3), (1, 2, 2), (2, 2, 1), (2, 1, 2)])
 def test_slab_reconstruction_solve_matches_oracle(ctx, dim, p, outer):
     """lambda perturbed by 10%: the slab solve recovers the oracle's per-slab
@@ -95,6 +148,7 @@ def test_slab_reconstruction_solve_matches_oracle(ctx, dim, p, outer):
     sysm.set_solution(x)
     est = sysm.estimate(apf.APF_FP64, 0, slab=apf.APF_SLAB_SOLVE)
     assert est["slab_relative_residual"] <= 1e-12, est["slab_relative_residual"]
+    assert est["slab_stop"] in (apf.APF_SLAB_STOP_TOL, apf.APF_SLAB_STOP_FLOOR)
     assert ref.recon2 > 1e-6 * ref.lambda_y2
     for key, rv in (("recon2", ref.recon2), ("flux2", ref.flux2), ("y_est2", ref.y_est2)):
         assert _rel(est[key], rv) <= 1e-10, (key, est[key], rv)

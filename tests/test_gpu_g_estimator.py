@@ -34,8 +34,9 @@ def _systems(ctx, m, dim, outer, p, coeff):
 @pytest.mark.parametrize("kappa", [False, True], ids=["M=I", "checkerboard"])
 @pytest.mark.parametrize("dim,p,outer", [(1, 1, 2), (1, 2, 2), (1, 3, 1), (2, 1, 1), (2, 2, 1), (2, 1, 2)])
 def test_g_estimator_matches_oracle(ctx, dim, p, outer, kappa):
-    """G^2 on the oracle's direct solution: the slab solve to 1e-12 relative
-    residual against the oracle's direct solve agrees to 1e-8 relative."""
+    """G^2 on the oracle's direct solution: the slab solve run to its FP64
+    round-off floor (tol 1e-15) against the oracle's direct solve agrees to 1e-10
+    relative."""
     if kappa and dim == 1:
         pytest.skip("the checkerboard coefficient is two-dimensional")
     m = meshes.uniform_mesh(dim, outer)
@@ -48,11 +49,12 @@ def test_g_estimator_matches_oracle(ctx, dim, p, outer, kappa):
     g2_ref, lG, _, _ = OG.g_estimator(stm, p, prob, x)
     _, sysp, _, sysq = _systems(ctx, m, dim, outer, p, coeff)
     sysp.set_solution(x)
-    g = sysp.g_estimate(sysq, tol=1e-12, max_it=200)
+    g = sysp.g_estimate(sysq, tol=1e-15, max_it=300)
     assert g["g_true_dofs"] == len(lG)
-    assert g["solver_relative_residual"] <= 1e-12
     assert g2_ref > 0
-    assert abs(g["lambda_difference_squared"] - g2_ref) <= 1e-8 * g2_ref, (g, g2_ref)
+    assert abs(g["lambda_difference_squared"] - g2_ref) <= 1e-10 * g2_ref, (g, g2_ref)
+    assert g["solver_relative_residual"] <= 1e-12, g
+    assert g["solver_stop"] in (apf.APF_SLAB_STOP_TOL, apf.APF_SLAB_STOP_FLOOR), g
     assert g["lambda_difference"] == pytest.approx(np.sqrt(g["lambda_difference_squared"]), rel=1e-15)
 
 

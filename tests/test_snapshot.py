@@ -88,7 +88,10 @@ def test_mesh_file_describes_the_space_time_mesh(tmp_path, case):
             n_interior += 1
             assert other // ns == cell // ns
             assert cell in snb[other]
-            face = set(C[cell % ns]) - {C[cell % ns][f]} if dim == 2 else {C[cell % ns][f]}
+            # face f of a triangle is the edge (v_f, v_(f+1) mod 3) (src/mesh/topology/faces_2d.hpp:15-37),
+            # of an interval its vertex f (faces_1d.hpp:15-25)
+            tri = C[cell % ns]
+            face = {tri[f], tri[(f + 1) % 3]} if dim == 2 else {tri[f]}
             assert face <= set(C[other % ns])
     # each interior spatial facet is listed from both sides, in every slab
     if dim == 2:
