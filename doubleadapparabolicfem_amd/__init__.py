@@ -35,6 +35,7 @@ APF_ASSEMBLE_KRON, APF_ASSEMBLE_SCATTER = 0, 1
 APF_PRECOND_DIAGONAL, APF_PRECOND_MULTIGRID = 0, 1
 APF_SLAB_FROM_SOLUTION, APF_SLAB_SOLVE = 0, 1
 APF_SLAB_STOP_NONE, APF_SLAB_STOP_TOL, APF_SLAB_STOP_FLOOR, APF_SLAB_STOP_MAX_IT = -1, 0, 1, 2
+APF_CSR_FULL, APF_CSR_TRIANGLE = 0, 1
 
 _lib = None
 
@@ -148,6 +149,15 @@ EXPORTED = {
                                          ctypes.POINTER(_DoerflerDiag)]),
     "apf_y_estimator_threshold": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                  ctypes.POINTER(ctypes.c_double)]),
+    "apf_csr_matrix_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_csr_matrix_destroy": (None, [ctypes.c_void_p]),
+    "apf_csr_matrix_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64)]),
+    "apf_csr_minres_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(_MinresStats)]),
 }
 
 
@@ -395,6 +405,52 @@ class System:
         t = np.zeros(3)
         _check(lib().apf_estimate_phase_timings(self.handle, _ptr(t)))
         return dict(factor_ms=t[0], rhs_ms=t[1], solve_ms=t[2])
+
+
+class CsrMatrix:
+    """A symmetric sparse matrix on the device, from the compressed arrays of a caller's
+    Eigen::SparseMatrix<double, ColMajor, int> (outer [n+1], inner, values: int32, int32,
+    float64) or from a scipy.sparse matrix.  storage=APF_CSR_TRIANGLE reads only the
+    entries with inner >= outer and completes them symmetrically."""
+
+    def __init__(self, ctx: Context, A=None, outer=None, inner=None, values=None, storage=APF_CSR_FULL):
+        if A is not None:
+            A = A.tocsr()
+            A.sort_indices()
+            outer, inner, values = A.indptr, A.indices, A.data
+        outer = np.ascontiguousarray(outer, dtype=np.int32)
+        inner = np.ascontiguousarray(inner, dtype=np.int32)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().apf_csr_matrix_create(ctx.handle, len(outer) - 1, _ptr(outer), _ptr(inner), _ptr(values),
+                                           storage, ctypes.byref(h)), ctx.handle, "apf_csr_matrix_create")
+        self.handle = h
+        n, nnz = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().apf_csr_matrix_info(h, ctypes.byref(n), ctypes.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.apf_csr_matrix_destroy(self.handle)
+            self.handle = None
+
+    def minres(self, b, n_lambda, hhat=None, tol=1e-10, max_it=20000, check_interval=100, degree=8,
+               precond=APF_FP32_REFINE, x0=None, raise_on_fail=True):
+        """MINRES on this matrix (apf_csr_minres_solve); returns (x, stats)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        st = _MinresStats()
+        code = lib().apf_csr_minres_solve(self.ctx.handle, self.handle, n_lambda,
+                                          hhat.handle if hhat is not None else None, _ptr(b), _ptr(x), _ptr(x0),
+                                          tol, max_it, check_interval, degree, precond, ctypes.byref(st))
+        stats = {f: getattr(st, f) for f, _ in _MinresStats._fields_}
+        if code == APF_NOT_CONVERGED and raise_on_fail:
+            raise NotConverged(code, lib().apf_last_error(self.ctx.handle).decode(), stats)
+        if code not in (APF_OK, APF_NOT_CONVERGED):
+            _check(code, self.ctx.handle, "apf_csr_minres_solve")
+        return x, stats
 
 
 def combine_estimator(flux2, recon2, resid2=None):

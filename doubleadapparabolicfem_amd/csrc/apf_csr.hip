@@ -1,0 +1,639 @@
+// MINRES on a symmetric saddle-point matrix assembled by the caller.
+//
+// This is the boundary for a K that did not come from this library's own
+// assembly: the compressed arrays that SaddlePreconditionedSolveDispatcher<Backend>::solve
+// receives (src/linear_algebra/eigen_backend/preconditioners/solve_saddle_system.hpp:32-121;
+// Eigen compressed storage, eigen_backend/sparse_matrix.hpp:29-37), the size of the
+// lambda block (blocks.A.rows()) and optionally the graph-norm matrix Hhat of the
+// preconditioner context (schur_approximations.hpp:24-28, :233-287).
+//
+// Preconditioner.  The reference applies P^-1 = blockdiag(A_y^-1, Hhat^-1) with sparse
+// Cholesky factors (parabolic_graph_norm_minres.hpp:46-181).  Triangular solves are
+// sequential and do not map to a GPU, so each block is replaced by a fixed Chebyshev
+// polynomial in the Jacobi-scaled block:
+//     P_B^-1 = q(D^-1 B) D^-1,   B = A_y (the leading n_lambda rows and columns of K)
+//                                    or Hhat,
+// the Chebyshev semi-iteration (Saad, Iterative Methods for Sparse Linear Systems,
+// 2nd ed., Alg. 12.1) of `degree` steps from a zero start on the interval
+// [lambda_max / degree^2, lambda_max], with lambda_max the Gershgorin bound of D^-1 B
+// (an upper bound, so q(D^-1 B) D^-1 is symmetric positive definite).  Without Hhat
+// the X block takes the diagonal of the Schur complement -C + B diag(A_y)^-1 B^T.
+// The polynomial is applied in FP32 (APF_FP32_REFINE) or FP64; the SpMV, the Lanczos
+// recurrences and the inner products stay FP64.  The stopping rule is the reference's:
+// the true relative residual |b - K x| / |b| is checked every `check_interval` iterations
+// (iterative_residual_acceptance.hpp:117-216).
+//
+// apf_csr_minres_solve takes host vectors; its work buffers are allocated once per
+// matrix, on the first solve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/apf.h"
+#include "apf_internal.hpp"
+#include "apf_solver_dev.hpp"
+
+extern thread_local std::string g_last_error;
+
+using apf::HipFailure;
+using apf::MinresScalars;
+
+struct apf_csr_matrix {
+    apf_ctx* ctx = nullptr;
+    int64_t n = 0, nnz = 0;
+    int64_t* rowptr = nullptr;
+    int* col = nullptr;
+    double* val = nullptr;
+    // work buffers of apf_csr_minres_solve, allocated on its first call
+    void* work = nullptr;
+    size_t work_bytes = 0;
+};
+
+namespace {
+
+constexpr int kRowGrid = 1024;   // blocks of the grid-stride row kernels (= dot partials per block of rows)
+constexpr int kRowLanes = 16;    // lanes per matrix row
+constexpr size_t kPartDoubles = 4 * kRowGrid;
+
+int fail(apf_ctx* ctx, int code, const std::string& m)
+{
+    g_last_error = m;
+    if (ctx) ctx->error = m;
+    return code;
+}
+
+// Work buffers for n unknowns inside one allocation (offsets in bytes, 256-aligned).
+struct WorkLayout {
+    size_t r1, r2, t, y, v, w, w1, w2, dinv, cx0, cx1, cd, part, scal, res, lam, total;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+WorkLayout work_layout(int64_t n)
+{
+    WorkLayout L{};
+    size_t o = 0;
+    const size_t vec = align256(static_cast<size_t>(n) * sizeof(double));
+    L.r1 = o; o += vec;
+    L.r2 = o; o += vec;
+    L.t = o; o += vec;
+    L.y = o; o += vec;
+    L.v = o; o += vec;
+    L.w = o; o += vec;
+    L.w1 = o; o += vec;
+    L.w2 = o; o += vec;
+    L.dinv = o; o += vec;
+    L.cx0 = o; o += vec;  // Chebyshev iterates and direction (FP32 uses half of each)
+    L.cx1 = o; o += vec;
+    L.cd = o; o += vec;
+    L.part = o; o += align256(kPartDoubles * sizeof(double));
+    L.scal = o; o += align256(sizeof(MinresScalars));
+    L.res = o; o += align256(4 * sizeof(double));
+    L.lam = o; o += align256(4 * sizeof(double));
+    L.total = o;
+    return L;
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_fill(int64_t n, double v, double* __restrict__ a)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        a[i] = v;
+}
+
+__global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        b[i] = a[i];
+}
+
+__device__ inline double group_sum(double v)
+{
+    for (int off = kRowLanes / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kRowLanes);
+    return v;
+}
+
+__device__ inline double block_max_nn(double v)
+{
+    // max of non-negative values over the block (a NaN counts as +inf)
+    __shared__ double sh[16];
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(v, off, 64);
+        v = (v != v || o != o) ? INFINITY : fmax(v, o);
+    }
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < static_cast<int>(blockDim.x / 64); ++i) r = (sh[i] != sh[i]) ? INFINITY : fmax(r, sh[i]);
+    __syncthreads();
+    return r;
+}
+
+// Jacobi data of the block rows [0, rows) x columns [0, rows) of a CSR matrix:
+// dinv_i = 1 / B_ii, and the Gershgorin bound max_i sum_j |B_ij| / B_ii of D^-1 B
+// (block partial maxima).  A non-positive diagonal is reported as a partial of -1.
+__global__ void __launch_bounds__(256) k_block_jacobi(int64_t rows, const int64_t* __restrict__ rowptr,
+                                                      const int* __restrict__ col, const double* __restrict__ val,
+                                                      double* __restrict__ dinv, double* __restrict__ part)
+{
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+    const int64_t gs = static_cast<int64_t>(gridDim.x) * blockDim.x / kRowLanes;
+    double gmax = 0.0;
+    bool bad = false;
+    for (int64_t i = g0; i < rows; i += gs) {
+        double diag = 0.0, asum = 0.0;
+        for (int64_t e = rowptr[i] + lane; e < rowptr[i + 1]; e += kRowLanes) {
+            const int c = col[e];
+            if (c < rows) {
+                const double a = val[e];
+                asum += fabs(a);
+                if (c == i) diag += a;
+            }
+        }
+        diag = group_sum(diag);
+        asum = group_sum(asum);
+        if (lane == 0) {
+            if (!(diag > 0.0)) {
+                bad = true;
+                dinv[i] = 1.0;
+            } else {
+                dinv[i] = 1.0 / diag;
+                gmax = fmax(gmax, asum / diag);
+            }
+        }
+    }
+    double m = block_max_nn(gmax);
+    const int anybad = __syncthreads_or(bad ? 1 : 0);
+    if (threadIdx.x == 0) part[blockIdx.x] = anybad ? -1.0 : m;
+}
+
+// X-block diagonal of the Schur complement S = -C + B diag(A_y)^-1 B^T from the X rows
+// of K = [A_y B^T; B C]:  S_jj = sum_{c < nY} K_jc^2 / K_cc - K_jj.
+__global__ void __launch_bounds__(256) k_schur_diag(int64_t nY, int64_t n, const int64_t* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const double* __restrict__ val,
+                                                    const double* __restrict__ dinvY, double* __restrict__ dinvX,
+                                                    double* __restrict__ part)
+{
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+    const int64_t gs = static_cast<int64_t>(gridDim.x) * blockDim.x / kRowLanes;
+    bool bad = false;
+    for (int64_t j = nY + g0; j < n; j += gs) {
+        double s = 0.0;
+        for (int64_t e = rowptr[j] + lane; e < rowptr[j + 1]; e += kRowLanes) {
+            const int c = col[e];
+            const double a = val[e];
+            if (c < nY) s += a * a * dinvY[c];
+            else if (c == j) s -= a;
+        }
+        s = group_sum(s);
+        if (lane == 0) {
+            if (!(s > 0.0)) {
+                bad = true;
+                dinvX[j - nY] = 1.0;
+            } else {
+                dinvX[j - nY] = 1.0 / s;
+            }
+        }
+    }
+    const int anybad = __syncthreads_or(bad ? 1 : 0);
+    if (threadIdx.x == 0) part[blockIdx.x] = anybad ? -1.0 : 0.0;
+}
+
+// lam[slot] = max of the partials (a negative partial marks a non-positive diagonal)
+__global__ void k_max_partials(const double* __restrict__ part, int nparts, double* __restrict__ lam, int slot)
+{
+    double v = 0.0;
+    bool bad = false;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        const double p = part[i];
+        if (p < 0.0) bad = true;
+        else v = (p != p) ? INFINITY : fmax(v, p);
+    }
+    v = block_max_nn(v);
+    const int anybad = __syncthreads_or(bad ? 1 : 0);
+    if (threadIdx.x == 0) lam[slot] = anybad ? -1.0 : v;
+}
+
+// One step of the Chebyshev semi-iteration for B y = r on the block rows [0, rows)
+// (columns >= rows are not part of the block and are skipped), D = diag(B):
+//   step 0:  d = (1/theta) D^-1 r,                          y1 = d
+//   step k:  d = rho_k rho_{k-1} d + (2 rho_k / delta) D^-1 (r - B y_k),  y_{k+1} = y_k + d
+// theta and delta scale with lambda_max (lam[slot]), read on the device; the host passes
+// the lambda_max-free coefficients.  The last step writes z = y (FP64) and the block
+// partials of z . r for the MINRES scalars.
+template <typename CT>
+__global__ void __launch_bounds__(256) k_cheb_step(int64_t rows, const int64_t* __restrict__ rowptr,
+                                                   const int* __restrict__ col, const double* __restrict__ val,
+                                                   const double* __restrict__ dinv, const double* __restrict__ r,
+                                                   const CT* __restrict__ yo, CT* __restrict__ yn,
+                                                   CT* __restrict__ d, const double* __restrict__ lam, int slot,
+                                                   int step, double c_dir, double c_res, int last,
+                                                   double* __restrict__ z, double* __restrict__ part)
+{
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+    const int64_t gs = static_cast<int64_t>(gridDim.x) * blockDim.x / kRowLanes;
+    const CT cres = static_cast<CT>(c_res / lam[slot]);
+    const CT cdir = static_cast<CT>(c_dir);
+    double dot = 0.0;
+    for (int64_t i = g0; i < rows; i += gs) {
+        CT s = 0;
+        if (step > 0) {
+            for (int64_t e = rowptr[i] + lane; e < rowptr[i + 1]; e += kRowLanes) {
+                const int c = col[e];
+                if (c < rows) s += static_cast<CT>(val[e]) * yo[c];
+            }
+            for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kRowLanes);
+        }
+        if (lane == 0) {
+            const CT ri = static_cast<CT>(r[i]);
+            const CT di = static_cast<CT>(dinv[i]);
+            CT dn, yv;
+            if (step == 0) {
+                dn = cres * di * ri;
+                yv = dn;
+            } else {
+                dn = cdir * d[i] + cres * di * (ri - s);
+                yv = yo[i] + dn;
+            }
+            if (last) {
+                const double zi = static_cast<double>(yv);
+                z[i] = zi;
+                dot += zi * r[i];
+            } else {
+                d[i] = dn;
+                yn[i] = yv;
+            }
+        }
+    }
+    const double bs = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// z = D^-1 r on a diagonal block, with the partials of z . r
+__global__ void __launch_bounds__(256) k_diag_apply(int64_t rows, const double* __restrict__ dinv,
+                                                    const double* __restrict__ r, double* __restrict__ z,
+                                                    double* __restrict__ part)
+{
+    double dot = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < rows;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double zi = dinv[i] * r[i];
+        z[i] = zi;
+        dot += zi * r[i];
+    }
+    const double bs = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// ------------------------------------------------------------------ driver
+struct Dev {
+    int64_t n;
+    const int64_t* rowptr;
+    const int* col;
+    const double* val;
+};
+
+// The Chebyshev coefficients of Alg. 12.1 on [lmax / ratio, lmax] divided by lmax:
+// step 0: 1/theta; step k: rho_k rho_{k-1} and 2 rho_k / delta.
+struct Cheb {
+    std::vector<double> cdir, cres;
+};
+
+Cheb chebyshev_coefficients(int degree)
+{
+    const double ratio = std::max(4.0, static_cast<double>(degree) * degree);
+    const double th = 0.5 * (1.0 + 1.0 / ratio), de = 0.5 * (1.0 - 1.0 / ratio);
+    const double sigma = th / de;
+    Cheb c;
+    double rho = 1.0 / sigma;
+    c.cdir.push_back(0.0);
+    c.cres.push_back(1.0 / th);
+    for (int k = 1; k < degree; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        c.cdir.push_back(rn * rho);
+        c.cres.push_back(2.0 * rn / de);
+        rho = rn;
+    }
+    return c;
+}
+
+template <typename T>
+T* at(unsigned char* base, size_t off)
+{
+    return reinterpret_cast<T*>(base + off);
+}
+
+struct Outcome {
+    int iterations = 0;
+    bool converged = false;
+    double rel = 0.0;
+    double estimate = 0.0;
+};
+
+// d_x holds the initial guess when use_x0, else it is zeroed; on return it holds x.
+Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsigned char* work,
+                   const double* d_b, double* d_x, bool use_x0, double tol, int max_it, int check_interval,
+                   int degree, bool fp32)
+{
+    const int64_t n = K.n, nX = n - nY;
+    const WorkLayout L = work_layout(n);
+    double* r1 = at<double>(work, L.r1);
+    double* r2 = at<double>(work, L.r2);
+    double* t = at<double>(work, L.t);
+    double* y = at<double>(work, L.y);
+    double* v = at<double>(work, L.v);
+    double* w = at<double>(work, L.w);
+    double* w1 = at<double>(work, L.w1);
+    double* w2 = at<double>(work, L.w2);
+    double* dinv = at<double>(work, L.dinv);
+    double* part = at<double>(work, L.part);
+    MinresScalars* scal = at<MinresScalars>(work, L.scal);
+    double* res = at<double>(work, L.res);
+    double* lam = at<double>(work, L.lam);
+    const dim3 grid(kRowGrid), block(256);
+    const int G = kRowGrid;
+
+    // ---- preconditioner set-up: Jacobi data and Gershgorin bounds of both blocks
+    hipLaunchKernelGGL(k_block_jacobi, grid, block, 0, st, nY, K.rowptr, K.col, K.val, dinv, part);
+    hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, G, lam, 0);
+    if (H) {
+        hipLaunchKernelGGL(k_block_jacobi, grid, block, 0, st, nX, H->rowptr, H->col, H->val, dinv + nY, part);
+        hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, G, lam, 1);
+    } else {
+        hipLaunchKernelGGL(k_schur_diag, grid, block, 0, st, nY, n, K.rowptr, K.col, K.val, dinv, dinv + nY, part);
+        hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, G, lam, 1);
+    }
+    double hl[2];
+    HIP_TRY(hipMemcpyAsync(hl, lam, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (!(hl[0] > 0.0) || !std::isfinite(hl[0]))
+        throw std::runtime_error("the lambda block A_y is not positive definite (non-positive diagonal)");
+    if (!(hl[1] >= 0.0) || !std::isfinite(hl[1]))
+        throw std::runtime_error(H ? "Hhat is not positive definite (non-positive diagonal)"
+                                   : "the Schur complement diagonal of the X block is not positive");
+
+    const Cheb cc = chebyshev_coefficients(degree);
+    // z = P^-1 r with the partials of z . r in part[0, 2G)
+    auto cheb = [&](int64_t rows, const Dev& B, const double* dv, const double* r, double* z, double* pp, int slot) {
+        auto run = [&](auto tag) {
+            using CT = decltype(tag);
+            CT* cx0 = at<CT>(work, L.cx0);
+            CT* cx1 = at<CT>(work, L.cx1);
+            CT* cd = at<CT>(work, L.cd);
+            for (int k = 0; k < degree; ++k) {
+                CT* yo = (k % 2 == 0) ? cx1 : cx0;
+                CT* yn = (k % 2 == 0) ? cx0 : cx1;
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_step<CT>), grid, block, 0, st, rows, B.rowptr, B.col, B.val,
+                                   dv, r, yo, yn, cd, lam, slot, k, cc.cdir[k], cc.cres[k], k == degree - 1 ? 1 : 0,
+                                   z, pp);
+            }
+        };
+        if (fp32) run(float{});
+        else run(double{});
+    };
+    auto precond = [&](double* tt, const double* rr2, double* z) {
+        if (rr2) hipLaunchKernelGGL(k_minres_upd, grid, block, 0, st, n, scal, tt, rr2);
+        cheb(nY, K, dinv, tt, z, part, 0);
+        if (H) cheb(nX, *H, dinv + nY, tt + nY, z + nY, part + G, 1);
+        else hipLaunchKernelGGL(k_diag_apply, grid, block, 0, st, nX, dinv + nY, tt + nY, z + nY, part + G);
+    };
+    auto spmv = [&](const double* xv, double* yv) {
+        const int64_t rows_per_block = (256 / 16) * 2;
+        const int64_t g = (n + rows_per_block - 1) / rows_per_block;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spmv<16, 4, 2>), dim3(static_cast<unsigned>(g)), block, 0, st, n,
+                           K.rowptr, K.col, K.val, xv, yv);
+    };
+
+    if (!use_x0) hipLaunchKernelGGL(k_fill, grid, block, 0, st, n, 0.0, d_x);
+    spmv(d_x, r1);
+    hipLaunchKernelGGL(k_axpby, grid, block, 0, st, n, 1.0, d_b, -1.0, r1);
+    hipLaunchKernelGGL(k_copy, grid, block, 0, st, n, r1, r2);
+    hipLaunchKernelGGL(k_fill, grid, block, 0, st, n, 0.0, w);
+    hipLaunchKernelGGL(k_fill, grid, block, 0, st, n, 0.0, w2);
+    hipLaunchKernelGGL(k_copy, grid, block, 0, st, n, r1, t);
+    precond(t, nullptr, y);
+    hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(256), 0, st, scal, part, 2 * G);
+    auto true_residual = [&]() {
+        spmv(d_x, t);
+        hipLaunchKernelGGL(k_residual_parts, grid, block, 0, st, n, d_b, t, part);
+        hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, part, G, res);
+        double h[2];
+        HIP_TRY(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return (h[1] > 0.0) ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    };
+    Outcome o;
+    o.rel = true_residual();
+    o.converged = o.rel <= tol;
+    while (!o.converged && o.iterations < max_it) {
+        const int batch = std::min(check_interval, max_it - o.iterations);
+        for (int b = 0; b < batch; ++b) {
+            spmv(y, t);
+            hipLaunchKernelGGL(k_minres_vec1, grid, block, 0, st, n, scal, y, t, r1, v, part);
+            hipLaunchKernelGGL(k_minres_alfa, dim3(1), dim3(256), 0, st, scal, part, G);
+            precond(t, r2, y);
+            hipLaunchKernelGGL(k_minres_scal2, dim3(1), dim3(256), 0, st, scal, part, 2 * G);
+            std::swap(r1, r2);   // r1 <- r2, r2 <- t, t <- old r1
+            std::swap(r2, t);
+            hipLaunchKernelGGL(k_minres_vec3, grid, block, 0, st, n, scal, v, w2, w, w1, d_x);
+            double* wt = w1;
+            w1 = w2;
+            w2 = w;
+            w = wt;
+        }
+        o.iterations += batch;
+        HIP_TRY(hipGetLastError());
+        o.rel = true_residual();
+        o.converged = o.rel <= tol;
+        if (!std::isfinite(o.rel)) break;
+    }
+    MinresScalars hs{};
+    HIP_TRY(hipMemcpyAsync(&hs, scal, sizeof(hs), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    o.estimate = hs.beta1 > 0 ? hs.phibar / hs.beta1 : 0.0;
+    return o;
+}
+
+int check_solve_args(apf_ctx* ctx, const apf_csr_matrix* K, int64_t nY, const apf_csr_matrix* H, double tol,
+                     int max_it, int check_interval, int degree, int prec, const char* who)
+{
+    const std::string w(who);
+    if (!ctx || !K) return fail(ctx, APF_BAD_ARG, w + ": null argument");
+    if (K->ctx != ctx || (H && H->ctx != ctx)) return fail(ctx, APF_BAD_ARG, w + ": matrix of another context");
+    if (nY < 1 || nY >= K->n) return fail(ctx, APF_BAD_ARG, w + ": n_lambda must lie in [1, n)");
+    if (H && H->n != K->n - nY) return fail(ctx, APF_BAD_ARG, w + ": Hhat must have n - n_lambda rows");
+    if (!(tol > 0.0) || max_it < 1 || check_interval < 1)
+        return fail(ctx, APF_BAD_ARG, w + ": bad tolerance / iteration limits");
+    if (degree < 1 || degree > 64) return fail(ctx, APF_BAD_ARG, w + ": Chebyshev degree must be in [1, 64]");
+    if (prec != APF_FP64 && prec != APF_FP32_REFINE) return fail(ctx, APF_BAD_ARG, w + ": unknown precision");
+    return APF_OK;
+}
+
+void fill_stats(apf_minres_stats* stats, const Outcome& o, double secs)
+{
+    if (!stats) return;
+    stats->iterations = o.iterations;
+    stats->converged = o.converged ? 1 : 0;
+    stats->true_relative_residual = o.rel;
+    stats->estimate = o.estimate;
+    stats->seconds = secs;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int apf_csr_matrix_create(apf_ctx* ctx, int64_t n, const int* outer, const int* inner, const double* values,
+                          int storage, apf_csr_matrix** out)
+{
+    if (!ctx || !outer || !out || n < 1) return fail(ctx, APF_BAD_ARG, "apf_csr_matrix_create: bad argument");
+    if (storage != APF_CSR_FULL && storage != APF_CSR_TRIANGLE)
+        return fail(ctx, APF_BAD_ARG, "apf_csr_matrix_create: unknown storage");
+    try {
+        const int64_t nnz_in = outer[n] - outer[0];
+        if (outer[0] != 0 || nnz_in < 0) return fail(ctx, APF_BAD_ARG, "apf_csr_matrix_create: bad outer index array");
+        if (nnz_in > 0 && (!inner || !values)) return fail(ctx, APF_BAD_ARG, "apf_csr_matrix_create: null arrays");
+        for (int64_t i = 0; i < n; ++i)
+            if (outer[i + 1] < outer[i]) return fail(ctx, APF_BAD_ARG, "apf_csr_matrix_create: outer index decreases");
+        for (int64_t e = 0; e < nnz_in; ++e)
+            if (inner[e] < 0 || inner[e] >= n)
+                return fail(ctx, APF_BAD_ARG, "apf_csr_matrix_create: inner index out of range");
+        // rows of the matrix used: the stored entries, or the symmetric completion of those
+        // with inner >= outer (Eigen ColMajor storage: the lower triangle, Eigen::Lower)
+        std::vector<int64_t> rowptr(n + 1, 0);
+        std::vector<int> col;
+        std::vector<double> val;
+        if (storage == APF_CSR_FULL) {
+            for (int64_t i = 0; i <= n; ++i) rowptr[i] = outer[i];
+            col.assign(inner, inner + nnz_in);
+            val.assign(values, values + nnz_in);
+        } else {
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t e = outer[i]; e < outer[i + 1]; ++e) {
+                    const int j = inner[e];
+                    if (j < i) continue;
+                    ++rowptr[i + 1];
+                    if (j != i) ++rowptr[j + 1];
+                }
+            for (int64_t i = 0; i < n; ++i) rowptr[i + 1] += rowptr[i];
+            col.resize(rowptr[n]);
+            val.resize(rowptr[n]);
+            std::vector<int64_t> pos(rowptr.begin(), rowptr.end() - 1);
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t e = outer[i]; e < outer[i + 1]; ++e) {
+                    const int j = inner[e];
+                    if (j < i) continue;
+                    col[pos[i]] = j;
+                    val[pos[i]++] = values[e];
+                    if (j != i) {
+                        col[pos[j]] = static_cast<int>(i);
+                        val[pos[j]++] = values[e];
+                    }
+                }
+        }
+        auto M = std::make_unique<apf_csr_matrix>();
+        M->ctx = ctx;
+        M->n = n;
+        M->nnz = rowptr[n];
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipMalloc(&M->rowptr, (n + 1) * sizeof(int64_t)));
+        HIP_TRY(hipMalloc(&M->col, std::max<int64_t>(1, M->nnz) * sizeof(int)));
+        HIP_TRY(hipMalloc(&M->val, std::max<int64_t>(1, M->nnz) * sizeof(double)));
+        HIP_TRY(hipMemcpy(M->rowptr, rowptr.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        if (M->nnz) {
+            HIP_TRY(hipMemcpy(M->col, col.data(), M->nnz * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(M->val, val.data(), M->nnz * sizeof(double), hipMemcpyHostToDevice));
+        }
+        *out = M.release();
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_csr_matrix_create: ") + e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, APF_BAD_ARG, std::string("apf_csr_matrix_create: ") + e.what());
+    }
+}
+
+void apf_csr_matrix_destroy(apf_csr_matrix* M)
+{
+    if (!M) return;
+    (void)hipSetDevice(M->ctx->device);
+    (void)hipFree(M->rowptr);
+    (void)hipFree(M->col);
+    (void)hipFree(M->val);
+    if (M->work) (void)hipFree(M->work);
+    delete M;
+}
+
+int apf_csr_matrix_info(const apf_csr_matrix* M, int64_t* n, int64_t* nnz)
+{
+    if (!M) return fail(nullptr, APF_BAD_ARG, "apf_csr_matrix_info: null matrix");
+    if (n) *n = M->n;
+    if (nnz) *nnz = M->nnz;
+    return APF_OK;
+}
+
+int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, const apf_csr_matrix* Hhat,
+                         const double* b, double* x, const double* x0, double tol, int max_it, int check_interval,
+                         int cheb_degree, int precond_prec, apf_minres_stats* stats)
+{
+    if (int c = check_solve_args(ctx, K, n_lambda, Hhat, tol, max_it, check_interval, cheb_degree, precond_prec,
+                                 "apf_csr_minres_solve"))
+        return c;
+    if (!b || !x) return fail(ctx, APF_BAD_ARG, "apf_csr_minres_solve: null vector");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        hipStream_t st = ctx->stream;
+        const WorkLayout L = work_layout(K->n);
+        // work: the layout plus b and x
+        const size_t vec = align256(static_cast<size_t>(K->n) * sizeof(double));
+        const size_t need = L.total + 2 * vec;
+        if (K->work_bytes < need) {
+            if (K->work) HIP_TRY(hipFree(K->work));
+            K->work = nullptr;
+            K->work_bytes = 0;
+            HIP_TRY(hipMalloc(&K->work, need));
+            K->work_bytes = need;
+        }
+        unsigned char* work = static_cast<unsigned char*>(K->work);
+        double* d_b = at<double>(work, L.total);
+        double* d_x = at<double>(work, L.total + vec);
+        HIP_TRY(hipMemcpyAsync(d_b, b, K->n * sizeof(double), hipMemcpyHostToDevice, st));
+        if (x0) HIP_TRY(hipMemcpyAsync(d_x, x0, K->n * sizeof(double), hipMemcpyHostToDevice, st));
+        const Dev Kd{K->n, K->rowptr, K->col, K->val};
+        Dev Hd{};
+        if (Hhat) Hd = Dev{Hhat->n, Hhat->rowptr, Hhat->col, Hhat->val};
+        const Outcome o = run_minres(st, Kd, n_lambda, Hhat ? &Hd : nullptr, work, d_b, d_x, x0 != nullptr, tol,
+                                     max_it, check_interval, cheb_degree, precond_prec == APF_FP32_REFINE);
+        HIP_TRY(hipMemcpyAsync(x, d_x, K->n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        fill_stats(stats, o, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        if (!o.converged)
+            return fail(ctx, APF_NOT_CONVERGED,
+                        "apf_csr_minres_solve: not converged after " + std::to_string(o.iterations) +
+                            " iterations, true relative residual " + std::to_string(o.rel));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_csr_minres_solve: ") + e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, APF_BAD_ARG, std::string("apf_csr_minres_solve: ") + e.what());
+    }
+}
+
+}  // extern "C"
+

@@ -28,6 +28,8 @@
  *                            (src/linear_algebra/eigen_backend/preconditioners/
  *                             solve_saddle_system.hpp:32-121,
  *                             parabolic_graph_norm_minres.hpp:232-369)
+ *   apf_csr_minres_solve  -- the same dispatcher for a K, rhs and Hhat assembled by
+ *                            the caller (Eigen compressed arrays)
  *   apf_estimate          -- compute_adaptive_test_space_iteration, estimator part
  *                            (src/finite_element/time_slabs/
  *                             time_slab_adaptive_loop.hpp:833-2026):
@@ -221,6 +223,39 @@ int apf_system_timings(apf_system* sys, double* ms4);
  * milliseconds: [0] patch matrices and inverses, [1] right-hand sides,
  * [2] batched solves with refinement. */
 int apf_estimate_phase_timings(apf_system* sys, double* ms3);
+
+/* ---- a K assembled by the caller ---- */
+/* How the compressed arrays of apf_csr_matrix_create are read: every stored entry
+ * (a symmetric matrix in full storage; row- and column-major then coincide), or only
+ * the entries with inner index >= outer index, completed symmetrically (for Eigen's
+ * default ColMajor storage that is the lower triangle, the part Eigen::MINRES<...,
+ * Eigen::Lower, ...> reads, parabolic_graph_norm_minres.hpp:240-243). */
+enum apf_csr_storage { APF_CSR_FULL = 0, APF_CSR_TRIANGLE = 1 };
+typedef struct apf_csr_matrix apf_csr_matrix;
+/* Upload a symmetric sparse matrix given by the compressed arrays of an
+ * Eigen::SparseMatrix<double, ColMajor, int> after makeCompressed() (outerIndexPtr [n+1],
+ * innerIndexPtr, valuePtr: src/linear_algebra/eigen_backend/sparse_matrix.hpp:29-37).
+ * Replaces handing K (and the graph-norm Hhat) to the Eigen MINRES. */
+int apf_csr_matrix_create(apf_ctx* ctx, int64_t n, const int* outer, const int* inner, const double* values,
+                          int storage, apf_csr_matrix** out);
+void apf_csr_matrix_destroy(apf_csr_matrix* M);
+/* rows (= columns) and stored entries after the symmetric completion */
+int apf_csr_matrix_info(const apf_csr_matrix* M, int64_t* n, int64_t* nnz);
+/* MINRES on K x = b for a caller's K = [A_y B^T; B C] whose first n_lambda unknowns
+ * are lambda.  Replaces SaddlePreconditionedSolveDispatcher<Backend>::solve with the
+ * ParabolicGraphNorm MINRES (src/linear_algebra/eigen_backend/preconditioners/
+ * solve_saddle_system.hpp:32-121, parabolic_graph_norm_minres.hpp:46-369): K and rhs
+ * are its arguments, n_lambda = blocks.A.rows(), Hhat = context.graph_norm->hhat (may
+ * be NULL).  Preconditioner blockdiag(q(D^-1 A_y) D^-1, q(D^-1 Hhat) D^-1), q the
+ * Chebyshev polynomial of `cheb_degree` steps (1..64) on [lambda_max / degree^2,
+ * lambda_max] of the Jacobi-scaled block, applied in FP64 or FP32 (precond_prec);
+ * without Hhat the X block is the Schur-complement diagonal.  Accepts when the true
+ * relative residual <= tol, checked every check_interval iterations
+ * (iterative_residual_acceptance.hpp:117-216); APF_NOT_CONVERGED otherwise (x then
+ * holds the last iterate).  b, x, x0 (may be NULL) are host arrays of length n. */
+int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, const apf_csr_matrix* Hhat,
+                         const double* b, double* x, const double* x0, double tol, int max_it, int check_interval,
+                         int cheb_degree, int precond_prec, apf_minres_stats* stats);
 
 /* ---- G-estimator ---- */
 typedef struct {

@@ -1,0 +1,114 @@
+"""MINRES on a K handed over by the caller (apf_csr_minres_solve), through the C ABI:
+the oracle assembles K, f, g and the graph-norm matrix Hhat the way the reference's
+assembler does, and the GPU solve is held to the oracle's direct solve."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import doubleadapparabolicfem_amd as apf
+from doubleadapparabolicfem_amd import meshes
+from oracle import assembly as OA, estimator as OE
+
+from gpu_helpers import oracle_setup
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return apf.Context(0)
+
+
+def _system(dim, p, outer, kappa=False):
+    m = meshes.uniform_mesh(dim, outer)
+    coeff = meshes.smooth_initial(dim)
+    if kappa:
+        coeff = apf.Coefficients(meshes.checkerboard_kappa(m, 4, seed=11), 0.0, coeff.u0)
+    stm, od, prob = oracle_setup(m, p, coeff)
+    S = OA.assemble(stm, od, prob)
+    return S
+
+
+def graph_norm_hhat(S):
+    """Hhat = -C_signed + B_dt diag(A_y)^-1 B_dt^T + A_x, the reference's graph-norm
+    approximation (src/linear_algebra/preconditioners/schur_approximations.hpp:233-287),
+    formed by the caller."""
+    dA = 1.0 / S.A.diagonal()
+    H = -S.C + S.Bdt @ sp.diags(dA) @ S.Bdt.T + S.Ax
+    return sp.csr_matrix(H)
+
+
+@pytest.mark.parametrize("prec", [apf.APF_FP64, apf.APF_FP32_REFINE], ids=["pc64", "pc32"])
+@pytest.mark.parametrize("with_hhat", [True, False], ids=["hhat", "schur-diag"])
+@pytest.mark.parametrize("dim,p,outer,kappa", [(1, 1, 3, False), (1, 2, 2, False), (1, 3, 1, False),
+                                               (2, 1, 1, True), (2, 2, 1, False), (2, 1, 2, True)])
+def test_csr_minres_matches_direct_solve(ctx, dim, p, outer, kappa, with_hhat, prec):
+    print(f"csr minres dim={dim} p={p} outer={outer} kappa={kappa} hhat={with_hhat} prec={prec}", flush=True)
+    S = _system(dim, p, outer, kappa)
+    K = S.K()
+    b = S.rhs()
+    xd = OE.main_solve(S)
+    nY = S.A.shape[0]
+    Km = apf.CsrMatrix(ctx, K)
+    assert (Km.n, Km.nnz) == (K.shape[0], K.nnz)
+    H = apf.CsrMatrix(ctx, graph_norm_hhat(S)) if with_hhat else None
+    x, st = Km.minres(b, nY, hhat=H, tol=1e-10, max_it=20000, check_interval=20, degree=6, precond=prec)
+    assert st["converged"] == 1
+    assert st["true_relative_residual"] <= 1e-10
+    assert np.linalg.norm(K @ x - b) <= 1e-10 * np.linalg.norm(b) * (1 + 1e-6)
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+def test_csr_triangle_storage_is_the_symmetric_completion(ctx):
+    """Only the lower triangle of K in column-major storage (the part Eigen::MINRES with
+    Eigen::Lower reads) gives the same matrix and the same solution."""
+    print("csr triangle storage", flush=True)
+    S = _system(2, 2, 1)
+    K = S.K()
+    b = S.rhs()
+    nY = S.A.shape[0]
+    L = sp.tril(K).tocsc()
+    L.sort_indices()
+    full = apf.CsrMatrix(ctx, K)
+    tri = apf.CsrMatrix(ctx, outer=L.indptr, inner=L.indices, values=L.data, storage=apf.APF_CSR_TRIANGLE)
+    assert (tri.n, tri.nnz) == (full.n, full.nnz)
+    xf, sf = full.minres(b, nY, tol=1e-10, check_interval=20, degree=4)
+    xt, stt = tri.minres(b, nY, tol=1e-10, check_interval=20, degree=4)
+    assert stt["converged"] == 1
+    assert np.linalg.norm(xt - xf) <= 1e-8 * np.linalg.norm(xf)
+
+
+def test_csr_matches_the_system_solution_and_repeats_bitwise(ctx):
+    """K, b downloaded from a GPU-assembled system and handed back as a caller's matrix:
+    the CSR solve agrees with the system's multigrid MINRES, and a second solve of the
+    same input is bit-identical."""
+    print("csr vs system minres", flush=True)
+    m = meshes.uniform_mesh(2, 2)
+    sp_ = apf.Space(m, 2, coarse=meshes.uniform_hierarchy(2, 2))
+    sysm = apf.System(ctx, sp_)
+    sysm.assemble(meshes.smooth_initial(2))
+    vals, b = sysm.download()
+    rowptr, col = sysm.download_pattern()
+    xs, _ = sysm.minres(tol=1e-11, max_it=2000, check_interval=10)
+    K = sp.csr_matrix((vals, col, rowptr), shape=(sp_.n, sp_.n))
+    Km = apf.CsrMatrix(ctx, K)
+    x1, s1 = Km.minres(b, sp_.nY, tol=1e-11, check_interval=25, degree=8)
+    x2, s2 = Km.minres(b, sp_.nY, tol=1e-11, check_interval=25, degree=8)
+    assert np.array_equal(x1, x2) and s1["iterations"] == s2["iterations"]
+    assert np.linalg.norm(x1 - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_csr_rejects_bad_input(ctx):
+    print("csr argument checks", flush=True)
+    K = sp.identity(4, format="csr")
+    Km = apf.CsrMatrix(ctx, K)
+    with pytest.raises(apf.APFError, match="n_lambda"):
+        Km.minres(np.ones(4), 0)
+    with pytest.raises(apf.APFError, match="Chebyshev degree"):
+        Km.minres(np.ones(4), 2, degree=0)
+    with pytest.raises(apf.APFError, match="inner index"):
+        apf.CsrMatrix(ctx, outer=[0, 1, 2], inner=[0, 5], values=[1.0, 1.0])
+    # an indefinite lambda block is refused before any iteration
+    bad = sp.csr_matrix(np.diag([-1.0, 1.0, 1.0, 1.0]))
+    with pytest.raises(apf.APFError, match="not positive definite"):
+        apf.CsrMatrix(ctx, bad).minres(np.ones(4), 2)

@@ -106,3 +106,31 @@ def test_g_estimator_rejects_mismatched_systems(ctx):
     unassembled = apf.System(ctx, apf.Space(m, 2, coarse=meshes.uniform_hierarchy(2, 1)))
     with pytest.raises(apf.APFError, match="assemble"):
         sysp.g_estimate(unassembled)
+
+
+def test_g_estimator_with_more_multigrid_tiles_than_default_partials(ctx):
+    """1+1D p=1 on 2^19 cells x 2 intervals: the degree-2 G system has 1,048,575
+    free nodes, i.e. 16,384 multigrid transpose tiles of 64 rows, each writing one
+    dot partial, four times the 4 x 1024 partials a system starts with.  The
+    partial buffer grows with the multigrid at setup: the slab solve converges,
+    G is finite and positive, and a repeated call is bit-identical."""
+    n = 2 ** 19
+    print(f"g-estimator with {n} cells (large multigrid tile count)", flush=True)
+    m = meshes.interval_mesh(n, [0.0, 0.5, 1.0])
+    coarse = [meshes.interval_mesh(2 ** j, [0.0, 1.0]) for j in range(18, 0, -1)]
+    coeff = meshes.smooth_initial(1)
+    sp_ = apf.Space(m, 1, coarse=coarse)
+    sysp = apf.System(ctx, sp_)
+    sysp.assemble(coeff)
+    # any degree-p coefficient vector serves (the cells are 2e-6 x 0.5, too thin for the
+    # multigrid MINRES to reach 1e-10 in a test's time)
+    sysp.set_solution(np.random.default_rng(5).standard_normal(sp_.n))
+    sq = apf.Space(m, 2, coarse=coarse)
+    assert -(-sq.nf // 64) > 4 * 1024
+    sysq = apf.System(ctx, sq)
+    sysq.assemble(coeff)
+    a = sysp.g_estimate(sysq, tol=1e-12, max_it=300)
+    b = sysp.g_estimate(sysq, tol=1e-12, max_it=300)
+    assert np.isfinite(a["lambda_difference_squared"]) and a["lambda_difference_squared"] > 0
+    assert a["solver_stop"] in (apf.APF_SLAB_STOP_TOL, apf.APF_SLAB_STOP_FLOOR), a
+    assert a["lambda_difference_squared"] == b["lambda_difference_squared"]
