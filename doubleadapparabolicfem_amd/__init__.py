@@ -36,6 +36,7 @@ APF_PRECOND_DIAGONAL, APF_PRECOND_MULTIGRID = 0, 1
 APF_SLAB_FROM_SOLUTION, APF_SLAB_SOLVE = 0, 1
 APF_SLAB_STOP_NONE, APF_SLAB_STOP_TOL, APF_SLAB_STOP_FLOOR, APF_SLAB_STOP_MAX_IT = -1, 0, 1, 2
 APF_CSR_FULL, APF_CSR_TRIANGLE = 0, 1
+APF_NOALLOC_MAX_N = 262144  # include/apf.h
 
 _lib = None
 
@@ -158,6 +159,14 @@ EXPORTED = {
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(_MinresStats)]),
+    "apf_csr_minres_solve_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.POINTER(_MinresStats)]),
+    "apf_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_device_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_device_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "apf_device_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
 }
 
 
@@ -451,6 +460,47 @@ class CsrMatrix:
         if code not in (APF_OK, APF_NOT_CONVERGED):
             _check(code, self.ctx.handle, "apf_csr_minres_solve")
         return x, stats
+
+
+class DeviceArray:
+    """A float64 array in device memory (apf_device_alloc), for the device vectors of
+    apf_csr_minres_solve_noalloc."""
+
+    def __init__(self, ctx: Context, n: int):
+        self.ctx, self.n = ctx, int(n)
+        h = ctypes.c_void_p()
+        _check(lib().apf_device_alloc(ctx.handle, 8 * self.n, ctypes.byref(h)), ctx.handle, "apf_device_alloc")
+        self.ptr = h
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.n
+        _check(lib().apf_device_upload(self.ctx.handle, self.ptr, _ptr(a), 8 * self.n), self.ctx.handle)
+
+    def download(self):
+        out = np.empty(self.n)
+        _check(lib().apf_device_download(self.ctx.handle, _ptr(out), self.ptr, 8 * self.n), self.ctx.handle)
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.apf_device_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+
+def csr_minres_noalloc(K: "CsrMatrix", d_b: DeviceArray, d_x: DeviceArray, n_lambda, tol=1e-10, max_it=20000,
+                       check_interval=100, degree=8, precond=APF_FP32_REFINE, use_x0=False, raise_on_fail=True):
+    """apf_csr_minres_solve_noalloc on device vectors; returns the stats (x stays in d_x)."""
+    st = _MinresStats()
+    code = lib().apf_csr_minres_solve_noalloc(K.ctx.handle, K.handle, n_lambda, d_b.ptr, d_x.ptr,
+                                              1 if use_x0 else 0, tol, max_it, check_interval, degree, precond,
+                                              ctypes.byref(st))
+    stats = {f: getattr(st, f) for f, _ in _MinresStats._fields_}
+    if code == APF_NOT_CONVERGED and raise_on_fail:
+        raise NotConverged(code, lib().apf_last_error(K.ctx.handle).decode(), stats)
+    if code not in (APF_OK, APF_NOT_CONVERGED):
+        _check(code, K.ctx.handle, "apf_csr_minres_solve_noalloc")
+    return stats
 
 
 def combine_estimator(flux2, recon2, resid2=None):

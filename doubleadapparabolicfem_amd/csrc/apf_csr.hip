@@ -23,8 +23,10 @@
 // the true relative residual |b - K x| / |b| is checked every `check_interval` iterations
 // (iterative_residual_acceptance.hpp:117-216).
 //
-// apf_csr_minres_solve takes host vectors; its work buffers are allocated once per
-// matrix, on the first solve.
+// Two entry points share one driver and the same kernels, so they give bit-identical
+// results: apf_csr_minres_solve (host vectors; work buffers allocated once per matrix,
+// on its first solve) and apf_csr_minres_solve_noalloc (device vectors; work buffers
+// carved from the file-scope device array `scratch`, so the call allocates nothing).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +34,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -61,6 +64,17 @@ namespace {
 constexpr int kRowGrid = 1024;   // blocks of the grid-stride row kernels (= dot partials per block of rows)
 constexpr int kRowLanes = 16;    // lanes per matrix row
 constexpr size_t kPartDoubles = 4 * kRowGrid;
+
+// Scratch of the allocation-free entry point: one array per GPU, held until the process
+// ends (12 vectors of APF_NOALLOC_MAX_N doubles and the reduction partials, 25.2 MB).
+// Its layout for n unknowns is work_layout(n).  Host code never uses the name `scratch`
+// as an address: apf_internal_noalloc_init resolves the device address per device.
+constexpr size_t kScratchBytes = 12 * sizeof(double) * static_cast<size_t>(APF_NOALLOC_MAX_N) +
+                                 kPartDoubles * sizeof(double) + 4096;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[64] = {};   // device address of `scratch` per device (hipGetSymbolAddress)
 
 int fail(apf_ctx* ctx, int code, const std::string& m)
 {
@@ -637,3 +651,98 @@ int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, cons
 
 }  // extern "C"
 
+// Resolve the device address of `scratch` on the context's device (once per device).
+// Called from apf_ctx_create with that device current.
+int apf_internal_noalloc_init(int device)
+{
+    if (device < 0 || device >= 64) return APF_BAD_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return APF_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        g_last_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return APF_HIP_ERROR;
+    }
+    g_scratch_ptr[device] = ptr;
+    return APF_OK;
+}
+
+extern "C" int apf_csr_minres_solve_noalloc(apf_ctx* ctx, const apf_csr_matrix* K, int64_t n_lambda,
+                                            const double* d_b, double* d_x, int use_x0, double tol, int max_it,
+                                            int check_interval, int cheb_degree, int precond_prec,
+                                            apf_minres_stats* stats)
+{
+    if (int c = check_solve_args(ctx, K, n_lambda, nullptr, tol, max_it, check_interval, cheb_degree, precond_prec,
+                                 "apf_csr_minres_solve_noalloc"))
+        return c;
+    if (!d_b || !d_x) return fail(ctx, APF_BAD_ARG, "apf_csr_minres_solve_noalloc: null vector");
+    if (K->n > APF_NOALLOC_MAX_N)
+        return fail(ctx, APF_BAD_ARG,
+                    "apf_csr_minres_solve_noalloc: " + std::to_string(K->n) + " unknowns exceed the scratch limit of " +
+                        std::to_string(APF_NOALLOC_MAX_N));
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        base = (ctx->device >= 0 && ctx->device < 64) ? g_scratch_ptr[ctx->device] : nullptr;
+    }
+    if (!base) return fail(ctx, APF_HIP_ERROR, "apf_csr_minres_solve_noalloc: scratch not resolved for this device");
+    if (work_layout(K->n).total > kScratchBytes)
+        return fail(ctx, APF_BAD_ARG, "apf_csr_minres_solve_noalloc: work layout exceeds the scratch array");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        const Dev Kd{K->n, K->rowptr, K->col, K->val};
+        const Outcome o = run_minres(ctx->stream, Kd, n_lambda, nullptr, static_cast<unsigned char*>(base), d_b, d_x,
+                                     use_x0 != 0, tol, max_it, check_interval, cheb_degree,
+                                     precond_prec == APF_FP32_REFINE);
+        fill_stats(stats, o, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        if (!o.converged)
+            return fail(ctx, APF_NOT_CONVERGED,
+                        "apf_csr_minres_solve_noalloc: not converged after " + std::to_string(o.iterations) +
+                            " iterations, true relative residual " + std::to_string(o.rel));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_csr_minres_solve_noalloc: ") + e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, APF_BAD_ARG, std::string("apf_csr_minres_solve_noalloc: ") + e.what());
+    }
+}
+
+// ------------------------------------------------------------------ device buffers
+// Plain device memory for callers that hand device vectors to
+// apf_csr_minres_solve_noalloc (the allocation happens here, outside that call).
+extern "C" int apf_device_alloc(apf_ctx* ctx, int64_t bytes, void** out)
+{
+    if (!ctx || !out || bytes < 1) return fail(ctx, APF_BAD_ARG, "apf_device_alloc: bad argument");
+    *out = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess || hipMalloc(out, static_cast<size_t>(bytes)) != hipSuccess)
+        return fail(ctx, APF_HIP_ERROR, "apf_device_alloc: hipMalloc failed");
+    return APF_OK;
+}
+
+extern "C" void apf_device_free(apf_ctx* ctx, void* p)
+{
+    if (!ctx || !p) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(p);
+}
+
+extern "C" int apf_device_upload(apf_ctx* ctx, void* dst, const void* src, int64_t bytes)
+{
+    if (!ctx || !dst || !src || bytes < 0) return fail(ctx, APF_BAD_ARG, "apf_device_upload: bad argument");
+    if (hipSetDevice(ctx->device) != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(ctx, APF_HIP_ERROR, "apf_device_upload: copy failed");
+    return APF_OK;
+}
+
+extern "C" int apf_device_download(apf_ctx* ctx, void* dst, const void* src, int64_t bytes)
+{
+    if (!ctx || !dst || !src || bytes < 0) return fail(ctx, APF_BAD_ARG, "apf_device_download: bad argument");
+    if (hipSetDevice(ctx->device) != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(ctx, APF_HIP_ERROR, "apf_device_download: copy failed");
+    return APF_OK;
+}

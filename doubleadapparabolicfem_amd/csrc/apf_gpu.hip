@@ -23,6 +23,8 @@ using namespace apf;
 
 thread_local std::string g_last_error;
 
+int apf_internal_noalloc_init(int device);  // apf_csr.hip
+
 static int set_error(apf_ctx* ctx, int code, const std::string& msg)
 {
     g_last_error = msg;
@@ -796,6 +798,9 @@ int apf_ctx_create(apf_ctx** out, int device)
     if (hipSetDevice(device) != hipSuccess) return set_error(nullptr, APF_HIP_ERROR, "apf_ctx_create: hipSetDevice failed");
     if (hipStreamCreate(&ctx->stream) != hipSuccess)
         return set_error(nullptr, APF_HIP_ERROR, "apf_ctx_create: hipStreamCreate failed");
+    // device address of the allocation-free solver's scratch array on this device; on
+    // failure apf_csr_minres_solve_noalloc reports it and every other entry still works
+    (void)apf_internal_noalloc_init(device);
     *out = ctx.release();
     return APF_OK;
 }

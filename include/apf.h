@@ -257,6 +257,29 @@ int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, cons
                          const double* b, double* x, const double* x0, double tol, int max_it, int check_interval,
                          int cheb_degree, int precond_prec, apf_minres_stats* stats);
 
+/* Largest n accepted by apf_csr_minres_solve_noalloc: its work buffers live in one
+ * file-scope device array of 12 * 8 * APF_NOALLOC_MAX_N bytes plus 32 KiB (25.2 MB) per
+ * GPU, held until the process ends. */
+#define APF_NOALLOC_MAX_N 262144
+/* No counterpart in the reference.  The same solve as apf_csr_minres_solve (same kernels,
+ * bit-identical result) for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  It calls no
+ * hipMalloc / hipFree or allocating routine; its work buffers are the file-scope scratch
+ * array above (at most APF_NOALLOC_MAX_N unknowns, else APF_BAD_ARG).  d_b and d_x are
+ * device arrays of length n; with use_x0 != 0, d_x holds the initial guess on entry.
+ * Without Hhat (the X block takes the Schur-complement diagonal).  Every call on one GPU
+ * shares that GPU's scratch array and leaves its last contents there: calls that overlap
+ * on one GPU (two threads, or two contexts on one device) race. */
+int apf_csr_minres_solve_noalloc(apf_ctx* ctx, const apf_csr_matrix* K, int64_t n_lambda, const double* d_b,
+                                 double* d_x, int use_x0, double tol, int max_it, int check_interval,
+                                 int cheb_degree, int precond_prec, apf_minres_stats* stats);
+/* Device memory and copies for the device vectors of apf_csr_minres_solve_noalloc (no
+ * counterpart in the reference: the reference's vectors live in host memory). */
+int apf_device_alloc(apf_ctx* ctx, int64_t bytes, void** out);
+void apf_device_free(apf_ctx* ctx, void* p);
+int apf_device_upload(apf_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int apf_device_download(apf_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
 /* ---- G-estimator ---- */
 typedef struct {
     double lambda_difference_squared;       /* G^2 = max(0, raw) (g_lambda_difference_squared) */
