@@ -100,6 +100,18 @@ class _GEstimateOut(ctypes.Structure):
                 ("g_true_dofs", ctypes.c_int64), ("seconds", ctypes.c_double), ("solver_stop", ctypes.c_int)]
 
 
+class _CellSpaceDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("p", ctypes.c_int), ("n_vertices", ctypes.c_int64),
+                ("vertices", ctypes.c_void_p), ("n_cells", ctypes.c_int64), ("cells", ctypes.c_void_p),
+                ("times", ctypes.c_void_p), ("restr_ptr", ctypes.c_void_p), ("restr_dof", ctypes.c_void_p),
+                ("restr_weight", ctypes.c_void_p), ("n_dofs", ctypes.c_int64)]
+
+
+class _GeneralCoeff(ctypes.Structure):
+    _fields_ = [("grid_n", ctypes.c_int), ("M_grid", ctypes.c_void_p), ("ell", ctypes.c_double),
+                ("u0_bottom", ctypes.c_void_p)]
+
+
 class _DoerflerDiag(ctypes.Structure):
     _fields_ = [("total", ctypes.c_double), ("target", ctypes.c_double), ("cutoff_value", ctypes.c_double),
                 ("cutoff_margin", ctypes.c_double), ("max_indicator", ctypes.c_double),
@@ -163,6 +175,20 @@ EXPORTED = {
                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                     ctypes.POINTER(_MinresStats)]),
+    "apf_general_pattern": (ctypes.c_int, [ctypes.POINTER(_CellSpaceDesc), ctypes.POINTER(_CellSpaceDesc),
+                                           ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p,
+                                           ctypes.c_void_p]),
+    "apf_general_system_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_CellSpaceDesc),
+                                                 ctypes.POINTER(_CellSpaceDesc), ctypes.c_void_p,
+                                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_general_system_destroy": (None, [ctypes.c_void_p]),
+    "apf_general_system_info": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int64)] * 4
+                                + [ctypes.POINTER(ctypes.c_int)]),
+    "apf_general_bottom_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_general_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_GeneralCoeff),
+                                            ctypes.POINTER(ctypes.c_double)]),
+    "apf_general_download": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "apf_general_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "apf_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
     "apf_device_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_device_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -501,6 +527,91 @@ def csr_minres_noalloc(K: "CsrMatrix", d_b: DeviceArray, d_x: DeviceArray, n_lam
     if code not in (APF_OK, APF_NOT_CONVERGED):
         _check(code, K.ctx.handle, "apf_csr_minres_solve_noalloc")
     return stats
+
+
+def _cell_space_desc(cs):
+    keep = [np.ascontiguousarray(cs.vertices, dtype=np.float64), np.ascontiguousarray(cs.cells, dtype=np.int32),
+            np.ascontiguousarray(cs.times, dtype=np.float64), np.ascontiguousarray(cs.restr_ptr, dtype=np.int64),
+            np.ascontiguousarray(cs.restr_dof, dtype=np.int32), np.ascontiguousarray(cs.restr_w, dtype=np.float64)]
+    d = _CellSpaceDesc(cs.dim, cs.p, len(keep[0]), _ptr(keep[0]), len(keep[1]), _ptr(keep[1]), _ptr(keep[2]),
+                       _ptr(keep[3]), _ptr(keep[4]), _ptr(keep[5]), cs.n_dofs)
+    return d, keep
+
+
+def general_pattern(mesh):
+    """Host-side CSR pattern of K for a non-tensor mesh (layered.LayeredMesh or any object
+    with X, Y cell spaces and y_ancestor); no GPU needed."""
+    dx, kx = _cell_space_desc(mesh.X)
+    dy, ky = _cell_space_desc(mesh.Y)
+    anc = np.ascontiguousarray(mesh.y_ancestor, dtype=np.int32)
+    nnz = ctypes.c_int64()
+    _check(lib().apf_general_pattern(ctypes.byref(dx), ctypes.byref(dy), _ptr(anc), ctypes.byref(nnz), None, None),
+           None, "apf_general_pattern")
+    n = mesh.X.n_dofs + mesh.Y.n_dofs
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    col = np.zeros(nnz.value, dtype=np.int32)
+    _check(lib().apf_general_pattern(ctypes.byref(dx), ctypes.byref(dy), _ptr(anc), ctypes.byref(nnz), _ptr(rowptr),
+                                     _ptr(col)), None, "apf_general_pattern")
+    return rowptr, col
+
+
+class GeneralSystem:
+    """Saddle-point system of a space-time mesh that is not a tensor product: cells with
+    restriction rows (hanging nodes) and Y cells inside X ancestors (apf_general_*)."""
+
+    def __init__(self, ctx: Context, mesh):
+        self.ctx, self.mesh = ctx, mesh
+        dx, kx = _cell_space_desc(mesh.X)
+        dy, ky = _cell_space_desc(mesh.Y)
+        anc = np.ascontiguousarray(mesh.y_ancestor, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().apf_general_system_create(ctx.handle, ctypes.byref(dx), ctypes.byref(dy), _ptr(anc),
+                                               ctypes.byref(h)), ctx.handle, "apf_general_system_create")
+        self.handle = h
+        n, nl, nnz, nb = (ctypes.c_int64() for _ in range(4))
+        nq = ctypes.c_int()
+        _check(lib().apf_general_system_info(h, ctypes.byref(n), ctypes.byref(nl), ctypes.byref(nnz), ctypes.byref(nb),
+                                             ctypes.byref(nq)))
+        self.n, self.nY, self.nnz, self.n_bottom, self.n_bottom_points = n.value, nl.value, nnz.value, nb.value, nq.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.apf_general_system_destroy(self.handle)
+            self.handle = None
+
+    def bottom_points(self):
+        out = np.zeros((self.n_bottom, self.n_bottom_points, self.mesh.dim))
+        _check(lib().apf_general_bottom_points(self.handle, _ptr(out)))
+        return out
+
+    def assemble(self, u0, M_grid=None, grid_n=0, ell=0.0):
+        """M_grid: [grid_n^dim, dim, dim] tensors of a piecewise-constant grid (None: identity).
+        Returns the device time of the assembly in ms."""
+        pts = self.bottom_points()
+        u0b = np.ascontiguousarray(u0(pts.reshape(-1, self.mesh.dim)).reshape(self.n_bottom, -1), dtype=np.float64)
+        Mg = None if M_grid is None else np.ascontiguousarray(M_grid, dtype=np.float64).ravel()
+        c = _GeneralCoeff(grid_n if Mg is not None else 0, _ptr(Mg), ell, _ptr(u0b))
+        ms = ctypes.c_double()
+        _check(lib().apf_general_assemble(self.handle, ctypes.byref(c), ctypes.byref(ms)), self.ctx.handle,
+               "apf_general_assemble")
+        return ms.value
+
+    def download(self):
+        rowptr = np.zeros(self.n + 1, dtype=np.int64)
+        col = np.zeros(self.nnz, dtype=np.int32)
+        vals = np.zeros(self.nnz)
+        b = np.zeros(self.n)
+        _check(lib().apf_general_download(self.handle, _ptr(rowptr), _ptr(col), _ptr(vals), _ptr(b)), self.ctx.handle)
+        return rowptr, col, vals, b
+
+    def matrix(self):
+        """a CsrMatrix view of K (valid while this system lives)"""
+        h = ctypes.c_void_p()
+        _check(lib().apf_general_matrix(self.handle, ctypes.byref(h)), self.ctx.handle)
+        M = CsrMatrix.__new__(CsrMatrix)
+        M.ctx, M.handle, M.n, M.nnz = self.ctx, h, self.n, self.nnz
+        M._owner = self
+        return M
 
 
 def combine_estimator(flux2, recon2, resid2=None):

@@ -22,7 +22,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", 
              "-munsafe-fp-atomics"]
 
 SOURCES = ["apf_tables.cpp", "apf_space.cpp", "apf_mg.cpp", "apf_marking.cpp", "apf_gpu.hip", "apf_estimator.hip", "apf_g.hip",
-           "apf_mg.hip", "apf_csr.hip"]
+           "apf_mg.hip", "apf_csr.hip", "apf_general.hip"]
 
 
 def _compile(src):

@@ -48,17 +48,6 @@ extern thread_local std::string g_last_error;
 using apf::HipFailure;
 using apf::MinresScalars;
 
-struct apf_csr_matrix {
-    apf_ctx* ctx = nullptr;
-    int64_t n = 0, nnz = 0;
-    int64_t* rowptr = nullptr;
-    int* col = nullptr;
-    double* val = nullptr;
-    // work buffers of apf_csr_minres_solve, allocated on its first call
-    void* work = nullptr;
-    size_t work_bytes = 0;
-};
-
 namespace {
 
 constexpr int kRowGrid = 1024;   // blocks of the grid-stride row kernels (= dot partials per block of rows)
@@ -586,9 +575,11 @@ void apf_csr_matrix_destroy(apf_csr_matrix* M)
 {
     if (!M) return;
     (void)hipSetDevice(M->ctx->device);
-    (void)hipFree(M->rowptr);
-    (void)hipFree(M->col);
-    (void)hipFree(M->val);
+    if (M->owned) {
+        (void)hipFree(M->rowptr);
+        (void)hipFree(M->col);
+        (void)hipFree(M->val);
+    }
     if (M->work) (void)hipFree(M->work);
     delete M;
 }

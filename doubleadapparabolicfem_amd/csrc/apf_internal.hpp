@@ -23,6 +23,20 @@ struct apf_ctx {
     std::string error;
 };
 
+// A symmetric CSR matrix on the device (apf_csr.hip).  `owned`: the arrays belong to the
+// matrix; a view of another object's K (apf_general_matrix) borrows them.
+struct apf_csr_matrix {
+    apf_ctx* ctx = nullptr;
+    int64_t n = 0, nnz = 0;
+    int64_t* rowptr = nullptr;
+    int* col = nullptr;
+    double* val = nullptr;
+    bool owned = true;
+    // work buffers of apf_csr_minres_solve, allocated on its first call
+    void* work = nullptr;
+    size_t work_bytes = 0;
+};
+
 struct apf_space {
     apf::Space S;
     apf::RefTables T;

@@ -257,6 +257,66 @@ int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, cons
                          const double* b, double* x, const double* x0, double tol, int max_it, int check_interval,
                          int cheb_degree, int precond_prec, apf_minres_stats* stats);
 
+/* ---- space-time meshes that are not tensor products ---- */
+/* One space (X or Y) as the reference's FESpace hands it to the assembler: the active
+ * cells (prisms = triangle x [t0, t1], or rectangles in 1+1D) and each cell's restriction
+ * rows u_cell = C_c u_true (src/finite_element/assembly/constrained_dofs.hpp:36-56,
+ * build_cell_restriction :133-226): per (cell, local dof) a row of (true dof, weight),
+ * true dofs strictly increasing; [(d, 1.0)] for a free dof, empty for a Dirichlet dof,
+ * the masters and weights of a hanging dof.  Local dof = i_space * (p+1) + i_time
+ * (basis/functions/triangular_prism_lagrange.hpp:32-35), spatial nodes in the
+ * equidistant order (nodes/triangle_equidistant.hpp:39-53). */
+typedef struct {
+    int dim, p;                  /* 1 or 2; 1..3 */
+    int64_t n_vertices;
+    const double* vertices;      /* [n_vertices * dim] spatial vertex coordinates */
+    int64_t n_cells;
+    const int* cells;            /* [n_cells * (dim+1)] spatial vertices (counter-clockwise in 2D) */
+    const double* times;         /* [n_cells * 2] t0 < t1 */
+    const int64_t* restr_ptr;    /* [n_cells * n_local + 1], n_local = Ns * (p+1) */
+    const int* restr_dof;        /* true dof of each row entry */
+    const double* restr_weight;
+    int64_t n_dofs;              /* true dofs of the space */
+} apf_cell_space_desc;
+/* Coefficients of the non-tensor path.  M is evaluated at every quadrature point (the
+ * reference calls its M callback per point, dense_local_main_blocks_2d.hpp:234-240): the
+ * identity (grid_n = 0) or piecewise constant on a grid_n^dim grid of (0,1)^dim, tensor of
+ * grid cell (iy, ix) at (iy * grid_n + ix) * dim * dim.  u0 at the bottom rule points of
+ * the X cells with t0 = 0 (apf_general_bottom_points). */
+typedef struct {
+    int grid_n;
+    const double* M_grid;
+    double ell;
+    const double* u0_bottom;     /* [n_bottom_cells * n_bottom_points] */
+} apf_general_coeff;
+typedef struct apf_general_system apf_general_system;
+/* Host only (no GPU): the CSR pattern of K = [A_y B^T; B C] (row pointer [n+1], sorted
+ * columns) for the given spaces, y_ancestor[c] = the X cell containing Y cell c
+ * (assembly/detail/active_cell_locator.hpp).  Replaces the symbolic pass of the two-pass
+ * saddle assembly (assembly/main_system/two_pass_full_saddle_2d.hpp:326-459).  With rowptr
+ * and col NULL it returns nnz only. */
+int apf_general_pattern(const apf_cell_space_desc* X, const apf_cell_space_desc* Y, const int* y_ancestor,
+                        int64_t* nnz, int64_t* rowptr, int* col);
+/* Device system of a non-tensor mesh: uploads cells, restriction rows and the pattern. */
+int apf_general_system_create(apf_ctx* ctx, const apf_cell_space_desc* X, const apf_cell_space_desc* Y,
+                              const int* y_ancestor, apf_general_system** out);
+void apf_general_system_destroy(apf_general_system* sys);
+int apf_general_system_info(const apf_general_system* sys, int64_t* n, int64_t* n_lambda, int64_t* nnz,
+                            int64_t* n_bottom_cells, int* n_bottom_points);
+/* physical bottom rule points [n_bottom_cells * n_bottom_points * dim] */
+int apf_general_bottom_points(const apf_general_system* sys, double* pts);
+/* Element-local quadrature assembly of K and b = [f; g] on the device, with the X basis of
+ * each Y cell's ancestor evaluated at the mapped quadrature points.  Replaces
+ * assemble_main_system_blocks (system/assemble_saddle_problem.hpp:131-392) with
+ * assemble_main_y_dense_local_blocks_2d (assembly/main_system/dense_local_main_blocks_2d.hpp:48-640),
+ * the trace blocks and scatter_matrix (assembly/scatter.hpp:66-115).  ms (may be NULL):
+ * device time of the assembly kernels. */
+int apf_general_assemble(apf_general_system* sys, const apf_general_coeff* coeff, double* ms);
+/* Copy the pattern, values and b to the host (any may be NULL). */
+int apf_general_download(apf_general_system* sys, int64_t* rowptr, int* col, double* values, double* b);
+/* A view of the system's K for apf_csr_minres_solve (destroy it before the system). */
+int apf_general_matrix(apf_general_system* sys, apf_csr_matrix** out);
+
 /* Largest n accepted by apf_csr_minres_solve_noalloc: its work buffers live in one
  * file-scope device array of 12 * 8 * APF_NOALLOC_MAX_N bytes plus 32 KiB (25.2 MB) per
  * GPU, held until the process ends. */

@@ -1,0 +1,211 @@
+"""Saddle-point system on a non-tensor (layered) space-time mesh (oracle; test
+infrastructure only).
+
+The X and Y spaces of a layered mesh (doubleadapparabolicfem_amd.layered) are
+subspaces of the X and Y spaces of one tensor mesh: the finest spatial mesh of
+all layers times the time grid of all Y sub-intervals.  The bilinear forms of
+the reference (A_y, B, C; src/finite_element/system/assemble_saddle_problem.hpp:49-56)
+restricted to a subspace are P^T (form on the fine space) P, where P holds the
+values of the subspace's basis functions at the fine space's Lagrange nodes
+(nodal interpolation is exact on the subspace).  So
+
+    A = P_Y^T A_fine P_Y,  B = P_X^T B_fine P_Y,  C = P_X^T C_fine P_X,
+    f = P_Y^T f_fine,      g = P_X^T g_fine,
+
+with the fine forms from the tensor oracle (oracle.assembly, pinned by closed
+forms and the published anchors).  This is exact when M is constant on every
+cell of every layer, and the quadrature is exact for the polynomial integrands
+on the coarse cells (the rules of runner_case_execution.hpp:29-37 are).
+
+P is built from the layered mesh's cell restriction rows and its own basis
+evaluation: the layered basis function of true dof d is, on each cell, the sum
+over local dofs of (restriction weight of d) x (local Lagrange function).  A fine
+node on an interface between a fine and a coarse X layer is evaluated on the
+coarse side, where every interface node is a free dof; the hanging-node weights
+of the fine side do not enter, so a wrong weight makes the GPU assembly (which
+uses them) disagree with this oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import assembly as OA
+from . import mesh as OM
+from .basis import segment_lagrange, spatial_lagrange
+
+
+def _uniform_spatial(dim, n):
+    if dim == 1:
+        return OM.interval_mesh(n)
+    return OM.union_jack_mesh(n)
+
+
+def _locate(dim, n, x):
+    """cell of the uniform mesh (n cells per direction) containing x, and reference coords"""
+    m = _uniform_spatial(dim, n)
+    if dim == 1:
+        c = np.minimum((x[:, 0] * n).astype(np.int64), n - 1)
+        return c, ((x[:, 0] - m.vertices[m.cells[c, 0], 0]) * n)[:, None]
+    i = np.minimum((x[:, 0] * n).astype(np.int64), n - 1)
+    j = np.minimum((x[:, 1] * n).astype(np.int64), n - 1)
+    out_c = np.zeros(len(x), dtype=np.int64)
+    out_xi = np.zeros((len(x), 2))
+    best = np.full(len(x), -np.inf)
+    for h in (0, 1):
+        c = 2 * (j * n + i) + h
+        V = m.vertices[m.cells[c]]
+        J = np.stack([V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]], axis=2)
+        xi = np.linalg.solve(J, (x - V[:, 0])[:, :, None])[:, :, 0]
+        mn = np.minimum(np.minimum(xi[:, 0], xi[:, 1]), 1.0 - xi[:, 0] - xi[:, 1])
+        take = mn > best
+        out_c = np.where(take, c, out_c)
+        out_xi = np.where(take[:, None], xi, out_xi)
+        best = np.where(take, mn, best)
+    return out_c, out_xi
+
+
+def _cell_base(space, key):
+    """first global cell id of each layer / sub-interval"""
+    ids = np.asarray(key)
+    base = {}
+    for c, k in enumerate(ids):
+        base.setdefault(int(k), c)
+    return base
+
+
+def _prolongation(cs, n_fine_rows, pts, t, pick):
+    """P [n_fine_rows, cs.n_dofs]: basis functions of the cell space `cs` at the points
+    (pts[r], t[r]); pick(r) -> (layer key, resolution, t0, t1) of the cell to evaluate on."""
+    dim, p = cs.dim, cs.p
+    Nt = p + 1
+    base = _cell_base(cs, cs.layer)
+    rows, cols, vals = [], [], []
+    groups = {}
+    for r in range(n_fine_rows):
+        groups.setdefault(pick(r), []).append(r)
+    for (key, n, t0, t1), rr in groups.items():
+        rr = np.array(rr)
+        c, xi = _locate(dim, n, pts[rr])
+        gc = base[key] + c
+        sv, _ = spatial_lagrange(dim, p, xi)                   # [k, Ns]
+        tv, _ = segment_lagrange(p, (t[rr] - t0) / (t1 - t0))   # [k, Nt]
+        Ns = sv.shape[1]
+        for a in range(len(rr)):
+            for s in range(Ns):
+                for it in range(Nt):
+                    v = sv[a, s] * tv[a, it]
+                    if v == 0.0:
+                        continue
+                    lo = cs.restr_ptr[gc[a] * cs.n_loc + s * Nt + it]
+                    hi = cs.restr_ptr[gc[a] * cs.n_loc + s * Nt + it + 1]
+                    for e in range(lo, hi):
+                        rows.append(rr[a])
+                        cols.append(cs.restr_dof[e])
+                        vals.append(cs.restr_w[e] * v)
+    P = sp.coo_matrix((vals, (rows, cols)), shape=(n_fine_rows, cs.n_dofs)).tocsr()
+    P.sum_duplicates()
+    return P
+
+
+def fine_tensor(lm):
+    """the tensor mesh whose spaces contain the layered ones"""
+    N = max(int(r) for r in np.concatenate([lm.X.res, lm.Y.res]))
+    times = np.array(sorted({a for a, _, _, _ in lm.y_layers} | {1.0 * lm.y_layers[-1][1]}))
+    return OM.SpaceTimeMesh(_uniform_spatial(lm.dim, N), times)
+
+
+def prolongations(lm, stm, od):
+    dim, p = lm.dim, lm.p
+    sd = od.sd
+    free_nodes = np.argsort(np.where(sd.free_index >= 0, sd.free_index, np.iinfo(np.int64).max))[:sd.n_free]
+    xs = sd.node_coords[free_nodes]                             # [nf, dim]
+    nf = sd.n_free
+    times = stm.times
+    m = len(times) - 1
+    # X: fine X dof (time node tn, free node s) at t = times[k] + l h / p, tn = k p + l
+    tn_t = np.concatenate([times[k] + (times[k + 1] - times[k]) * np.arange(p) / p for k in range(m)] + [times[-1:]])
+    XL = lm.x_layers
+
+    def x_layer_of(tv):
+        for k, (t0, t1, n) in enumerate(XL):
+            if k > 0 and abs(tv - t0) < 1e-14:
+                # interface: evaluate on the coarser side
+                return k - 1 if XL[k - 1][2] < n else k
+            if t0 <= tv < t1 or (k == len(XL) - 1 and abs(tv - t1) < 1e-14):
+                return k
+        raise ValueError(tv)
+
+    nXf = len(tn_t) * nf
+    ptsX = np.tile(xs, (len(tn_t), 1))
+    tX = np.repeat(tn_t, nf)
+    keyX = []
+    for tv in tn_t:
+        k = x_layer_of(tv)
+        keyX.append((k, XL[k][2], XL[k][0], XL[k][1]))
+    PX = _prolongation(lm.X, nXf, ptsX, tX, lambda r: keyX[r // nf])
+    assert nXf == od.nX
+    # Y: fine Y dof (interval k, time node l, free node s); fine intervals are the Y sub-intervals
+    YL = lm.y_layers
+    assert len(YL) == m
+    tY = np.concatenate([times[k] + (times[k + 1] - times[k]) * np.arange(p + 1) / p for k in range(m)])
+    ptsY = np.tile(xs, (len(tY), 1))
+    tYr = np.repeat(tY, nf)
+    PY = _prolongation(lm.Y, m * (p + 1) * nf, ptsY, tYr,
+                       lambda r: (r // ((p + 1) * nf), YL[r // ((p + 1) * nf)][2], YL[r // ((p + 1) * nf)][0],
+                                  YL[r // ((p + 1) * nf)][1]))
+    return PX, PY
+
+
+def grid_M(dim, centres, M_grid, k):
+    """M per cell from a piecewise-constant k^dim grid of tensors [k^dim, dim, dim]
+    (grid cell (iy, ix) at index iy * k + ix)"""
+    if M_grid is None:
+        return np.broadcast_to(np.eye(dim), (len(centres), dim, dim)).copy()
+    ix = np.minimum((centres[:, 0] * k).astype(np.int64), k - 1)
+    if dim == 1:
+        return np.asarray(M_grid).reshape(k, 1, 1)[ix]
+    iy = np.minimum((centres[:, 1] * k).astype(np.int64), k - 1)
+    return np.asarray(M_grid).reshape(k * k, 2, 2)[iy * k + ix]
+
+
+def assemble(lm, M_grid=None, grid_k=1, ell=0.0, u0=None):
+    """SaddleBlocks of the layered system (A, B, C, f, g; B_dt and A_x on the layered spaces too)."""
+    stm = fine_tensor(lm)
+    od = OM.spacetime_dofs(stm, lm.p)
+    sm = stm.space
+    M = grid_M(lm.dim, sm.vertices[sm.cells].mean(axis=1), M_grid, grid_k)
+    S = OA.assemble(stm, od, OA.Problem(M, ell, u0))
+    PX, PY = prolongations(lm, stm, od)
+    A = (PY.T @ S.A @ PY).tocsr()
+    B = (PX.T @ S.B @ PY).tocsr()
+    C = (PX.T @ S.C @ PX).tocsr()
+    Bdt = (PX.T @ S.Bdt @ PY).tocsr()
+    Ax = (PX.T @ S.Ax @ PX).tocsr()
+    return OA.SaddleBlocks(A, B, C, PY.T @ S.f, PX.T @ S.g, Bdt, Ax)
+
+
+def symbolic_pattern(lm):
+    """CSR pattern (row pointer, sorted columns) of K = [A_y B^T; B C] from the cell
+    restriction rows alone: every pair of true dofs that appear in the rows of one Y
+    cell (A_y), of one Y cell and its X ancestor (B, B^T), or of one bottom X cell (C)."""
+    nY, nX = lm.Y.n_dofs, lm.X.n_dofs
+
+    def incidence(cs, cells, offset):
+        lo = cs.restr_ptr[cells * cs.n_loc]
+        hi = cs.restr_ptr[(cells + 1) * cs.n_loc]
+        r = np.repeat(np.arange(len(cells)), hi - lo)
+        idx = np.concatenate([np.arange(a, b) for a, b in zip(lo, hi)]) if len(cells) else np.zeros(0, np.int64)
+        E = sp.coo_matrix((np.ones(len(idx)), (r, cs.restr_dof[idx] + offset)), shape=(len(cells), nY + nX))
+        E = E.tocsr()
+        E.data[:] = 1.0
+        return E
+
+    ycells = np.arange(lm.Y.n_cells)
+    EY = incidence(lm.Y, ycells, 0)
+    EXa = incidence(lm.X, lm.y_ancestor.astype(np.int64), nY)
+    EXb = incidence(lm.X, lm.x_bottom.astype(np.int64), nY)
+    P = sp.csr_matrix((EY.T @ EY) + (EY.T @ EXa) + (EXa.T @ EY) + (EXb.T @ EXb))
+    P.data[:] = 1.0
+    P.sort_indices()
+    return P.indptr.astype(np.int64), P.indices.astype(np.int32)
