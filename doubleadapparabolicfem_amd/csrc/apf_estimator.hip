@@ -2190,7 +2190,7 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         out->slab_stop = APF_SLAB_STOP_NONE;
         const auto t_slab = std::chrono::steady_clock::now();
         if (slab_mode == APF_SLAB_SOLVE)
-            slab_reconstruct(sys, B.lt, 1e-13, 100, &out->slab_iterations, &out->slab_relative_residual,
+            slab_reconstruct(sys, B.lt, 1e-12, 100, &out->slab_iterations, &out->slab_relative_residual,
                              &out->slab_stop);
         else
             HIP_TRY(hipMemcpyAsync(B.lt, sys->d_x, S.nY * sizeof(double), hipMemcpyDeviceToDevice, st));

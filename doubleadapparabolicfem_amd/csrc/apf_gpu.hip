@@ -444,6 +444,40 @@ __global__ void k_precond_apply(int nf, int p, int m, int64_t nY, const double* 
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
+// y = A_y x on the lambda rows of a tensor mesh without reading K:
+//   A_y((k,l,s),(k,l',s')) = (K(s,s') h_k) Mt[l][l']
+// (the products k_fill_kron writes).  One thread per (interval k, free node s)
+// computes the NT outputs (k, l, s); the spatial row and the NT x values of each
+// neighbour are read once.
+template <int NT>
+__global__ void __launch_bounds__(256) k_ay_apply(int nf, int m, const int* __restrict__ sptr,
+                                                  const int* __restrict__ scol, const double* __restrict__ Ksp,
+                                                  const double* __restrict__ times, TimeTables tt,
+                                                  const double* __restrict__ x, double* __restrict__ y)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<int64_t>(m) * nf) return;
+    const int k = static_cast<int>(i / nf), s = static_cast<int>(i - static_cast<int64_t>(k) * nf);
+    const double h = times[k + 1] - times[k];
+    const double* __restrict__ xk = x + static_cast<int64_t>(k) * NT * nf;
+    double acc[NT];
+#pragma unroll
+    for (int l = 0; l < NT; ++l) acc[l] = 0.0;
+    for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
+        const double kh = Ksp[e] * h;
+        const int c = scol[e];
+        double xv[NT];
+#pragma unroll
+        for (int lp = 0; lp < NT; ++lp) xv[lp] = xk[static_cast<int64_t>(lp) * nf + c];
+#pragma unroll
+        for (int l = 0; l < NT; ++l)
+#pragma unroll
+            for (int lp = 0; lp < NT; ++lp) acc[l] += (kh * tt.Mt[l * NT + lp]) * xv[lp];
+    }
+#pragma unroll
+    for (int l = 0; l < NT; ++l) y[(static_cast<int64_t>(k) * NT + l) * nf + s] = acc[l];
+}
+
 // ============================================================== host side
 namespace apf {
 
@@ -510,20 +544,24 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     double* y = sys->d_vec[3];
     double* r = sys->d_vec[0];
     double* z = sys->d_vec[1];
-    auto resid2 = [&](const double* Kt) {
-        hipLaunchKernelGGL(k_residual_parts, dim3(nparts), dim3(256), 0, st, nY, sys->d_b, Kt, sys->d_part);
+    // r0 = f - B^T u: the lambda rows of b - K [0; u] (the only product that reads K); every
+    // later product is A_y alone, applied matrix-free from the spatial stiffness
+    double* r0 = sys->d_vec[5];
+    HIP_TRY(hipMemsetAsync(t, 0, nY * sizeof(double), st));
+    HIP_TRY(hipMemcpyAsync(t + nY, sys->d_x + nY, (n - nY) * sizeof(double), hipMemcpyDeviceToDevice, st));
+    sys->spmv_rows(t, y, nY);
+    HIP_TRY(hipMemcpyAsync(r0, sys->d_b, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -1.0, y, 1.0, r0);
+    auto resid2 = [&](const double* Ay) {
+        hipLaunchKernelGGL(k_residual_parts, dim3(nparts), dim3(256), 0, st, nY, r0, Ay, sys->d_part);
         hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
         double h[2];
         HIP_TRY(hipMemcpyAsync(h, sys->d_res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         return h[0];
     };
-    // |f - B^T u|: the lambda rows of K [0; u]
-    HIP_TRY(hipMemsetAsync(t, 0, nY * sizeof(double), st));
-    HIP_TRY(hipMemcpyAsync(t + nY, sys->d_x + nY, (n - nY) * sizeof(double), hipMemcpyDeviceToDevice, st));
-    // only the lambda rows of K are read below
-    sys->spmv_rows(t, y, nY);
-    const double g2 = resid2(y);
+    HIP_TRY(hipMemsetAsync(y, 0, nY * sizeof(double), st));
+    const double g2 = resid2(y);  // |r0|^2
     HIP_TRY(hipMemcpyAsync(lt, sys->d_x, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
     double* pd = sys->d_vec[4];  // search direction [p; 0]
     auto dot = [&](const double* a, const double* b) {
@@ -540,9 +578,8 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     int it = 0, cycle = 0;
     double rr = 0.0, prev = std::numeric_limits<double>::infinity();
     for (;; ++cycle) {
-        // true residual r = [f - B^T u - A_y lt; 0]
-        HIP_TRY(hipMemcpyAsync(t, lt, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
-        sys->spmv_rows(t, y, nY);
+        // true residual r = [r0 - A_y lt; 0]
+        sys->ay_apply(lt, y);
         rr = std::sqrt(resid2(y) / scale);
         if (!(rr > tol)) {
             *stop = APF_SLAB_STOP_TOL;
@@ -557,7 +594,7 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
             break;
         }
         prev = rr;
-        HIP_TRY(hipMemcpyAsync(r, sys->d_b, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(r, r0, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -1.0, y, 1.0, r);
         double rz_old = 0.0;
         for (int k = 0; it < max_it; ++k) {
@@ -571,7 +608,7 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
             else
                 hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, 1.0, z, rz / rz_old, pd);
             rz_old = rz;
-            sys->spmv_rows(pd, y, nY);  // A_y p
+            sys->ay_apply(pd, y);  // A_y p
             const double pap = dot(pd, y);
             if (!(pap > 0.0)) break;
             const double alpha = rz / pap;
@@ -589,6 +626,20 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
 }
 
 void System::spmv(const double* x, double* y) { spmv_rows(x, y, space->n); }
+
+void System::ay_apply(const double* x, double* y)
+{
+    const Space& S = *space;
+    const int m = S.mesh.m();
+    const int64_t threads = static_cast<int64_t>(m) * S.nf;
+    const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);
+    switch (S.Nt) {
+    case 2: hipLaunchKernelGGL(k_ay_apply<2>, grid, block, 0, stream, S.nf, m, d_sptr, d_scol, d_Ksp, d_times, tt, x, y); break;
+    case 3: hipLaunchKernelGGL(k_ay_apply<3>, grid, block, 0, stream, S.nf, m, d_sptr, d_scol, d_Ksp, d_times, tt, x, y); break;
+    case 4: hipLaunchKernelGGL(k_ay_apply<4>, grid, block, 0, stream, S.nf, m, d_sptr, d_scol, d_Ksp, d_times, tt, x, y); break;
+    default: hipLaunchKernelGGL(k_ay_apply<5>, grid, block, 0, stream, S.nf, m, d_sptr, d_scol, d_Ksp, d_times, tt, x, y); break;
+    }
+}
 
 // y[0:n) = K[0:n, :] x (the first n rows only)
 void System::spmv_rows(const double* x, double* y, int64_t n)

@@ -64,7 +64,7 @@ enum apf_precision { APF_FP64 = 0, APF_FP32_REFINE = 1 };
 enum apf_assembly_mode { APF_ASSEMBLE_KRON = 0, APF_ASSEMBLE_SCATTER = 1 };
 enum apf_precond_kind { APF_PRECOND_DIAGONAL = 0, APF_PRECOND_MULTIGRID = 1 };
 /* slab reconstruction: lambda_tilde = lambda of the current solution (exact when the
- * main solve is converged), or solved slab by slab to 1e-13 relative */
+ * main solve is converged), or solved slab by slab to 1e-12 relative */
 enum apf_slab_mode { APF_SLAB_FROM_SOLUTION = 0, APF_SLAB_SOLVE = 1 };
 /* Why an iterative slab solve (APF_SLAB_SOLVE, apf_g_estimate) stopped: the tolerance is
  * met, a restart from the true residual gained less than 10% (the FP64 round-off floor
