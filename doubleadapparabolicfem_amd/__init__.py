@@ -189,6 +189,13 @@ EXPORTED = {
                                             ctypes.POINTER(ctypes.c_double)]),
     "apf_general_download": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
     "apf_general_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_general_estimator_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_general_estimator_destroy": (None, [ctypes.c_void_p]),
+    "apf_general_estimator_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_Coeff)]),
+    "apf_general_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(_EstimateOut), ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
     "apf_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
     "apf_device_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_device_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -612,6 +619,64 @@ class GeneralSystem:
         M.ctx, M.handle, M.n, M.nnz = self.ctx, h, self.n, self.nnz
         M._owner = self
         return M
+
+
+class GeneralEstimator:
+    """Estimator of a GeneralSystem's solution, slab group by slab group
+    (apf_general_estimator_*).  groups: [(tensor Mesh of the group, coarse meshes for the
+    slab solve, first Y cell)]."""
+
+    def __init__(self, gs: GeneralSystem, groups):
+        self.gs = gs
+        self.spaces = [Space(m, gs.mesh.p, coarse=coarse) for m, coarse, _ in groups]
+        arr = (ctypes.c_void_p * len(groups))(*[sp.handle.value for sp in self.spaces])
+        first = np.ascontiguousarray([f for _, _, f in groups], dtype=np.int64)
+        h = ctypes.c_void_p()
+        _check(lib().apf_general_estimator_create(gs.handle, len(groups), arr, _ptr(first), ctypes.byref(h)),
+               gs.ctx.handle, "apf_general_estimator_create")
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.apf_general_estimator_destroy(self.handle)
+            self.handle = None
+
+    def assemble(self, u0, M_grid=None, grid_n=0, ell=0.0):
+        """each group's tensor system: M per cell from the grid (cells lie inside grid cells),
+        u0 at the bottom points of the group that starts at t = 0"""
+        dim = self.gs.mesh.dim
+        self._keep = []
+        for g, sp in enumerate(self.spaces):
+            m = sp.mesh
+            if M_grid is None:
+                M = None
+            else:
+                cen = m.vertices[m.cells].mean(axis=1)
+                ix = np.minimum((cen[:, 0] * grid_n).astype(np.int64), grid_n - 1)
+                idx = ix if dim == 1 else np.minimum((cen[:, 1] * grid_n).astype(np.int64), grid_n - 1) * grid_n + ix
+                M = np.ascontiguousarray(np.asarray(M_grid).reshape(-1, dim * dim)[idx], dtype=np.float64)
+            pts = sp.bottom_points()
+            u0b = (u0(pts.reshape(-1, dim)).reshape(pts.shape[0], pts.shape[1]) if m.times[0] == 0.0
+                   else np.zeros(pts.shape[:2]))
+            u0b = np.ascontiguousarray(u0b, dtype=np.float64)
+            self._keep += [M, u0b]
+            dp = ctypes.POINTER(ctypes.c_double)
+            c = _Coeff(M.ctypes.data_as(dp) if M is not None else None, float(ell), u0b.ctypes.data_as(dp))
+            _check(lib().apf_general_estimator_assemble(self.handle, g, ctypes.byref(c)), self.gs.ctx.handle,
+                   "apf_general_estimator_assemble")
+
+    def estimate(self, x, prec=APF_FP32_REFINE, refine_steps=2, per_cell=True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = _EstimateOut()
+        n = self.gs.mesh.Y.n_cells
+        fl = np.zeros(n) if per_cell else None
+        rs = np.zeros(n) if per_cell else None
+        rc = np.zeros(n) if per_cell else None
+        _check(lib().apf_general_estimate(self.handle, _ptr(x), prec, refine_steps, ctypes.byref(out), _ptr(fl),
+                                          _ptr(rs), _ptr(rc)), self.gs.ctx.handle, "apf_general_estimate")
+        res = {f: getattr(out, f) for f, _ in _EstimateOut._fields_}
+        res.update(flux_by_cell=fl, resid_by_cell=rs, recon_by_cell=rc)
+        return res
 
 
 def combine_estimator(flux2, recon2, resid2=None):

@@ -605,6 +605,7 @@ void prepare(const apf_cell_space_desc* Xd, const apf_cell_space_desc* Yd, const
 
 struct apf_general_system {
     apf_ctx* ctx = nullptr;
+    Prepared host;                   // cells and restriction rows (the estimator builds its maps from them)
     int dim = 0, p = 0;
     int64_t nX = 0, nY = 0, n = 0, nnz = 0, nyc = 0, nxc = 0, nbot = 0;
     int nqs = 0;
@@ -675,6 +676,7 @@ int apf_general_system_create(apf_ctx* ctx, const apf_cell_space_desc* Xd, const
         sys->nyc = P.Y.nc;
         sys->nxc = P.X.nc;
         sys->nbot = static_cast<int64_t>(P.bottom.size());
+        sys->host = P;
         std::vector<int64_t> rp;
         std::vector<int> cl;
         build_pattern(P.X, P.Y, P.anc, P.bottom, rp, cl);
@@ -913,6 +915,328 @@ int apf_general_matrix(apf_general_system* sys, apf_csr_matrix** out)
     M->owned = false;
     *out = M.release();
     return APF_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== estimator on slab groups
+// The estimator works slab by slab (time slabs between the global time endpoints,
+// time_slabs/time_slab_builder.hpp:53-120): the slab's Y space, its reconstruction
+// lambda_tilde (A_slab lt = f_slab - (B^T u)_slab, time_slab_reconstruction.hpp:584-735),
+// the (slab, vertex) patch problems and the per-cell indicators.  Slabs that share one
+// spatial mesh form a tensor mesh (that mesh x the slabs' time nodes), on which Y is the
+// tensor Y space and u_delta, the X function restricted to those slabs, lies in the tensor
+// X space (every X mesh of those slabs is that mesh or a coarser nested one).  So each
+// group is a tensor system of this library: lambda restricted to the group's Y dofs and
+// the nodal interpolant of u_delta go into its solution vector, and the tensor estimator
+// (with the slab solve) runs on it; totals and per-cell values are summed over the groups.
+namespace {
+
+__global__ void k_gather(int64_t n, const int* __restrict__ map, const double* __restrict__ src,
+                         double* __restrict__ dst)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[map[i]];
+}
+
+__global__ void k_interp(int64_t n, const int64_t* __restrict__ ptr, const int* __restrict__ col,
+                         const double* __restrict__ w, const double* __restrict__ src, double* __restrict__ dst)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double v = 0.0;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) v += w[e] * src[col[e]];
+        dst[i] = v;
+    }
+}
+
+// X basis of cell `a` (local index i_s * (p+1) + i_t) at the physical point (x, t)
+void x_basis_at(const CellSpaceHost& X, int64_t a, const double* x, double t, std::vector<double>& out)
+{
+    const int dim = X.dim, p = X.p, Ns = dim == 1 ? p + 1 : (p + 1) * (p + 2) / 2, Nt = p + 1;
+    const double* g = &X.geo[a * kGeo];
+    double xi0, xi1 = 0.0;
+    if (dim == 1) {
+        xi0 = (x[0] - g[0]) / (g[1] - g[0]);
+    } else {
+        const double A = g[2] - g[0], B = g[4] - g[0], C = g[3] - g[1], D = g[5] - g[1];
+        const double det = A * D - B * C, dx = x[0] - g[0], dy = x[1] - g[1];
+        xi0 = (D * dx - B * dy) / det;
+        xi1 = (-C * dx + A * dy) / det;
+    }
+    std::vector<double> sv(Ns), tv(Nt), d1(Ns), d2(Ns), dt(Nt);
+    if (dim == 1) apf::segment_lagrange(p, xi0, sv.data(), d1.data());
+    else apf::triangle_lagrange(p, xi0, xi1, sv.data(), d1.data(), d2.data());
+    apf::segment_lagrange(p, (t - g[kGeo - 2]) / (g[kGeo - 1] - g[kGeo - 2]), tv.data(), dt.data());
+    out.assign(static_cast<size_t>(Ns) * Nt, 0.0);
+    for (int is = 0; is < Ns; ++is)
+        for (int it = 0; it < Nt; ++it) out[is * Nt + it] = sv[is] * tv[it];
+}
+
+// physical position of local spatial node i_s of a tensor-space cell
+void node_position(const apf::TensorMesh& mesh, int p, int c, int is, double* x)
+{
+    if (mesh.dim == 1) {
+        const double x0 = mesh.vertices[mesh.cells[2 * c]], x1 = mesh.vertices[mesh.cells[2 * c + 1]];
+        x[0] = x0 + (x1 - x0) * is / p;
+        return;
+    }
+    int k = 0;
+    for (int cc = 0; cc <= p; ++cc)
+        for (int b = 0; b <= p - cc; ++b, ++k)
+            if (k == is) {
+                const int a = p - b - cc;
+                const int* V = &mesh.cells[3 * c];
+                for (int d = 0; d < 2; ++d)
+                    x[d] = (a * mesh.vertices[2 * V[0] + d] + b * mesh.vertices[2 * V[1] + d] +
+                            cc * mesh.vertices[2 * V[2] + d]) / p;
+                return;
+            }
+}
+
+struct EstGroup {
+    apf_space* space = nullptr;
+    apf_system* sys = nullptr;
+    int64_t first_y_cell = 0;
+    int64_t nYg = 0, nXg = 0, ncells = 0;
+    int* d_ymap = nullptr;       // group Y dof -> general Y dof
+    int64_t* d_iptr = nullptr;   // group X dof -> interpolation row over general X dofs
+    int* d_icol = nullptr;
+    double* d_iw = nullptr;
+    bool assembled = false;
+};
+
+}  // namespace
+
+struct apf_general_estimator {
+    apf_general_system* gsys = nullptr;   // used by the calls, never by destroy (it may be gone)
+    int device = 0;
+    std::vector<EstGroup> groups;
+    double* d_x = nullptr;           // the general solution [lambda; u]
+    std::vector<void*> allocs;
+};
+
+extern "C" {
+
+int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_space* const* spaces,
+                                 const int64_t* first_y_cell, apf_general_estimator** out)
+{
+    if (!gsys || !spaces || !first_y_cell || !out || n_groups < 1)
+        return fail(gsys ? gsys->ctx : nullptr, APF_BAD_ARG, "apf_general_estimator_create: bad argument");
+    apf_ctx* ctx = gsys->ctx;
+    auto ge = std::make_unique<apf_general_estimator>();
+    ge->gsys = gsys;
+    ge->device = ctx->device;
+    auto cleanup = [&]() {
+        for (auto& g : ge->groups)
+            if (g.sys) apf_system_destroy(g.sys);
+        for (void* q : ge->allocs) (void)hipFree(q);
+    };
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const CellSpaceHost& X = gsys->host.X;
+        const CellSpaceHost& Y = gsys->host.Y;
+        const int nlx = n_local(X.dim, X.p);
+        std::vector<int64_t> covered(static_cast<size_t>(Y.nc), 0);
+        auto dev = [&](auto& v) {
+            using T = typename std::decay_t<decltype(v)>::value_type;
+            void* q = nullptr;
+            HIP_TRY(hipMalloc(&q, std::max<size_t>(1, v.size()) * sizeof(T)));
+            ge->allocs.push_back(q);
+            if (!v.empty()) HIP_TRY(hipMemcpy(q, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+            return static_cast<T*>(q);
+        };
+        for (int gi = 0; gi < n_groups; ++gi) {
+            EstGroup G;
+            G.space = spaces[gi];
+            if (!G.space) throw std::invalid_argument("null group space");
+            const apf::Space& S = G.space->S;
+            if (S.dim != Y.dim || S.p != Y.p) throw std::invalid_argument("group space differs in dim or p");
+            if (!G.space->has_pattern) throw std::invalid_argument("group space needs its K pattern");
+            if (G.space->H.coarse.empty())
+                throw std::invalid_argument("group space needs coarse meshes (the slab solve's multigrid)");
+            const int ns = S.mesh.ns(), m = S.mesh.m(), p = S.p, Ns = S.Ns, Nt = S.Nt, nf = S.nf;
+            G.first_y_cell = first_y_cell[gi];
+            G.ncells = static_cast<int64_t>(ns) * m;
+            if (G.first_y_cell < 0 || G.first_y_cell + G.ncells > Y.nc)
+                throw std::invalid_argument("group cells outside the Y cells");
+            G.nYg = S.nY;
+            G.nXg = S.nX;
+            // Y map: the group's Y cells must be its mesh's cells slab by slab with identity rows
+            std::vector<int> ymap(static_cast<size_t>(S.nY), -1);
+            for (int k = 0; k < m; ++k)
+                for (int c = 0; c < ns; ++c) {
+                    const int64_t yc = G.first_y_cell + static_cast<int64_t>(k) * ns + c;
+                    if (covered[yc]++) throw std::invalid_argument("a Y cell in two groups");
+                    const double* gy = &Y.geo[yc * kGeo];
+                    if (std::fabs(gy[kGeo - 2] - S.mesh.times[k]) > 1e-13 ||
+                        std::fabs(gy[kGeo - 1] - S.mesh.times[k + 1]) > 1e-13)
+                        throw std::invalid_argument("group slab times differ from its Y cells'");
+                    for (int v = 0; v <= S.dim; ++v)
+                        for (int d = 0; d < S.dim; ++d)
+                            if (std::fabs(gy[v * S.dim + d] -
+                                          S.mesh.vertices[S.mesh.cells[c * (S.dim + 1) + v] * S.dim + d]) > 1e-13)
+                                throw std::invalid_argument("group cell geometry differs from its Y cell's");
+                    for (int is = 0; is < Ns; ++is)
+                        for (int it = 0; it < Nt; ++it) {
+                            const int loc = is * Nt + it;
+                            const int64_t r0 = Y.ptr[yc * (Ns * Nt) + loc], r1 = Y.ptr[yc * (Ns * Nt) + loc + 1];
+                            const int f = S.cell_free[static_cast<size_t>(c) * Ns + is];
+                            if (f < 0) {
+                                if (r1 != r0) throw std::invalid_argument("Dirichlet node with a non-empty Y row");
+                                continue;
+                            }
+                            if (r1 - r0 != 1 || Y.w[r0] != 1.0)
+                                throw std::invalid_argument("constrained Y dofs inside a slab are not supported");
+                            const int64_t td = (static_cast<int64_t>(k) * Nt + it) * nf + f;
+                            if (ymap[td] >= 0 && ymap[td] != Y.dof[r0])
+                                throw std::invalid_argument("inconsistent Y numbering in a slab");
+                            ymap[td] = Y.dof[r0];
+                        }
+                }
+            for (int v : ymap)
+                if (v < 0) throw std::invalid_argument("group Y dof without a general Y dof");
+            // X interpolation: each group X node evaluated on the X ancestor of a Y cell that holds it
+            std::vector<int> node_cell(nf, -1), node_loc(nf, -1);
+            for (int c = 0; c < ns; ++c)
+                for (int is = 0; is < Ns; ++is) {
+                    const int f = S.cell_free[static_cast<size_t>(c) * Ns + is];
+                    if (f >= 0 && node_cell[f] < 0) {
+                        node_cell[f] = c;
+                        node_loc[f] = is;
+                    }
+                }
+            std::vector<int64_t> iptr(1, 0);
+            std::vector<int> icol;
+            std::vector<double> iw, bv;
+            const int ntn = m * p + 1;
+            for (int tn = 0; tn < ntn; ++tn) {
+                const int k = std::min(tn / p, m - 1), l = tn - k * p;
+                const double t = S.mesh.times[k] + (S.mesh.times[k + 1] - S.mesh.times[k]) * l / p;
+                for (int f = 0; f < nf; ++f) {
+                    double x[2];
+                    node_position(S.mesh, p, node_cell[f], node_loc[f], x);
+                    const int64_t yc = G.first_y_cell + static_cast<int64_t>(k) * ns + node_cell[f];
+                    const int64_t a = gsys->host.anc[yc];
+                    x_basis_at(X, a, x, t, bv);
+                    std::vector<std::pair<int, double>> acc;
+                    for (int loc = 0; loc < nlx; ++loc) {
+                        const double vv = bv[loc];
+                        if (vv == 0.0) continue;
+                        for (int64_t e = X.ptr[a * nlx + loc]; e < X.ptr[a * nlx + loc + 1]; ++e)
+                            acc.emplace_back(X.dof[e], X.w[e] * vv);
+                    }
+                    std::sort(acc.begin(), acc.end());
+                    for (size_t q = 0; q < acc.size(); ++q) {
+                        if (!icol.empty() && static_cast<int64_t>(icol.size()) > iptr.back() && icol.back() == acc[q].first)
+                            iw.back() += acc[q].second;
+                        else {
+                            icol.push_back(acc[q].first);
+                            iw.push_back(acc[q].second);
+                        }
+                    }
+                    iptr.push_back(static_cast<int64_t>(icol.size()));
+                }
+            }
+            G.d_ymap = dev(ymap);
+            G.d_iptr = dev(iptr);
+            G.d_icol = dev(icol);
+            G.d_iw = dev(iw);
+            apf_system* sys_g = nullptr;
+            if (int c = apf_system_create(ctx, G.space, &sys_g)) throw std::runtime_error(g_last_error + " (code " + std::to_string(c) + ")");
+            G.sys = sys_g;
+            ge->groups.push_back(G);
+        }
+        for (int64_t c = 0; c < Y.nc; ++c)
+            if (!covered[c]) throw std::invalid_argument("Y cell " + std::to_string(c) + " in no group");
+        void* q = nullptr;
+        HIP_TRY(hipMalloc(&q, gsys->n * sizeof(double)));
+        ge->allocs.push_back(q);
+        ge->d_x = static_cast<double*>(q);
+        *out = ge.release();
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        cleanup();
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_general_estimator_create: ") + e.what());
+    } catch (const std::exception& e) {
+        cleanup();
+        return fail(ctx, APF_BAD_ARG, std::string("apf_general_estimator_create: ") + e.what());
+    }
+}
+
+void apf_general_estimator_destroy(apf_general_estimator* ge)
+{
+    if (!ge) return;
+    (void)hipSetDevice(ge->device);
+    for (auto& g : ge->groups)
+        if (g.sys) apf_system_destroy(g.sys);
+    for (void* q : ge->allocs) (void)hipFree(q);
+    delete ge;
+}
+
+int apf_general_estimator_assemble(apf_general_estimator* ge, int group, const apf_coeff* coeff)
+{
+    if (!ge || !coeff || group < 0 || group >= static_cast<int>(ge->groups.size()))
+        return fail(ge ? ge->gsys->ctx : nullptr, APF_BAD_ARG, "apf_general_estimator_assemble: bad argument");
+    if (int c = apf_assemble_main(ge->groups[group].sys, coeff, APF_ASSEMBLE_KRON)) return c;
+    ge->groups[group].assembled = true;
+    return APF_OK;
+}
+
+int apf_general_estimate(apf_general_estimator* ge, const double* x, int prec, int refine_steps, apf_estimate_out* out,
+                         double* flux_by_cell, double* resid_by_cell, double* recon_by_cell)
+{
+    if (!ge || !x || !out) return fail(ge ? ge->gsys->ctx : nullptr, APF_BAD_ARG, "apf_general_estimate: null argument");
+    apf_ctx* ctx = ge->gsys->ctx;
+    for (auto& g : ge->groups)
+        if (!g.assembled) return fail(ctx, APF_BAD_ARG, "apf_general_estimate: assemble every group first");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const int64_t nY = ge->gsys->nY;
+        HIP_TRY(hipMemcpyAsync(ge->d_x, x, ge->gsys->n * sizeof(double), hipMemcpyHostToDevice, st));
+        apf_estimate_out tot{};
+        tot.slab_stop = APF_SLAB_STOP_TOL;
+        double div2 = 0.0;
+        for (auto& g : ge->groups) {
+            apf::System* sg = g.sys;
+            hipLaunchKernelGGL(k_gather, dim3(1024), dim3(256), 0, st, g.nYg, g.d_ymap, ge->d_x, sg->d_x);
+            hipLaunchKernelGGL(k_interp, dim3(1024), dim3(256), 0, st, g.nXg, g.d_iptr, g.d_icol, g.d_iw, ge->d_x + nY,
+                               sg->d_x + g.nYg);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(st));
+            apf_estimate_out o{};
+            const int64_t off = g.first_y_cell;
+            const int code = apf_estimate_ex(g.sys, prec, refine_steps, APF_SLAB_SOLVE, &o,
+                                             flux_by_cell ? flux_by_cell + off : nullptr,
+                                             resid_by_cell ? resid_by_cell + off : nullptr,
+                                             recon_by_cell ? recon_by_cell + off : nullptr);
+            if (code != APF_OK) return code;
+            tot.lambda_y2 += o.lambda_y2;
+            // ||u0 - u(0)||^2 lives on the group that starts at t = 0 (a later group's
+            // bottom is an interior time)
+            if (g.space->S.mesh.times[0] == 0.0) tot.trace2 += o.trace2;
+            tot.flux2 += o.flux2;
+            tot.recon2 += o.recon2;
+            div2 += o.div_residual_l2 * o.div_residual_l2;
+            tot.max_patch_residual = std::max(tot.max_patch_residual, o.max_patch_residual);
+            tot.seconds_patches += o.seconds_patches;
+            tot.seconds_diagnostics += o.seconds_diagnostics;
+            tot.seconds_slab += o.seconds_slab;
+            tot.extra_refinements += o.extra_refinements;
+            tot.slab_iterations = std::max(tot.slab_iterations, o.slab_iterations);
+            tot.slab_relative_residual = std::max(tot.slab_relative_residual, o.slab_relative_residual);
+            if (o.slab_stop != APF_SLAB_STOP_TOL) tot.slab_stop = o.slab_stop;
+        }
+        tot.eta2 = tot.lambda_y2 + tot.trace2;
+        tot.y_est2 = tot.flux2 + tot.recon2;
+        tot.div_residual_l2 = std::sqrt(div2);
+        *out = tot;
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_general_estimate: ") + e.what());
+    }
 }
 
 }  // extern "C"

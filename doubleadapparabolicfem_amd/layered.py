@@ -357,3 +357,19 @@ def graded_layers(n_top, levels):
     for j in range(levels, 0, -1):
         layers.append((2.0 ** -j, 2.0 ** -(j - 1), n_top * 2 ** j))
     return layers
+
+
+def slab_groups(lm):
+    """Runs of consecutive Y sub-intervals with one spatial mesh: per group the number
+    of cells per direction, the time nodes and the first Y cell (the Y cells of a group
+    are its mesh's cells, sub-interval by sub-interval)."""
+    groups = []
+    first = 0
+    ncell = lambda n: n if lm.dim == 1 else 2 * n * n
+    for (a, b, n, _) in lm.y_layers:
+        if groups and groups[-1]["n"] == n:
+            groups[-1]["times"].append(b)
+        else:
+            groups.append({"n": n, "times": [a, b], "first_y_cell": first})
+        first += ncell(n)
+    return groups

@@ -317,6 +317,29 @@ int apf_general_download(apf_general_system* sys, int64_t* rowptr, int* col, dou
 /* A view of the system's K for apf_csr_minres_solve (destroy it before the system). */
 int apf_general_matrix(apf_general_system* sys, apf_csr_matrix** out);
 
+/* Estimator on a non-tensor mesh.  The estimator works per time slab (the slabs between
+ * the global time endpoints, time_slabs/time_slab_builder.hpp:53-120); slabs that share one
+ * spatial mesh form a group, given as a tensor space (apf_space_create on that mesh and the
+ * group's time nodes, with apf_space_add_coarse_mesh for the slab solve) whose space-time
+ * cells are the Y cells first_y_cell[g] + k * n_space_cells + c (slab k, spatial cell c, the
+ * general system's Y cells in that order, no constrained Y dofs inside a slab).  On each
+ * group lambda is restricted and u_delta interpolated through the X ancestors, and the
+ * estimator with the slab solve runs on it (the per-slab reconstruction of
+ * time_slab_reconstruction.hpp:584-735, the patches and indicators of
+ * time_slab_adaptive_loop.hpp:833-2026). */
+typedef struct apf_general_estimator apf_general_estimator;
+int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_space* const* spaces,
+                                 const int64_t* first_y_cell, apf_general_estimator** out);
+void apf_general_estimator_destroy(apf_general_estimator* ge);
+/* coefficients of group g's tensor system (M per spatial cell of the group mesh, u0 at
+ * its bottom points; only the group that starts at t = 0 uses u0) */
+int apf_general_estimator_assemble(apf_general_estimator* ge, int group, const apf_coeff* coeff);
+/* estimate on the solution x [n] (host) of the general system; totals as apf_estimate_out
+ * (slab iterations and residual: the largest over the groups); per-cell outputs (may be
+ * NULL) are indexed by the general system's Y cells. */
+int apf_general_estimate(apf_general_estimator* ge, const double* x, int prec, int refine_steps, apf_estimate_out* out,
+                         double* flux_by_cell, double* resid_by_cell, double* recon_by_cell);
+
 /* Largest n accepted by apf_csr_minres_solve_noalloc: its work buffers live in one
  * file-scope device array of 12 * 8 * APF_NOALLOC_MAX_N bytes plus 32 KiB (25.2 MB) per
  * GPU, held until the process ends. */

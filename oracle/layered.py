@@ -209,3 +209,88 @@ def symbolic_pattern(lm):
     P.data[:] = 1.0
     P.sort_indices()
     return P.indptr.astype(np.int64), P.indices.astype(np.int32)
+
+
+def slab_groups(lm):
+    """runs of consecutive Y sub-intervals with one spatial resolution: (n, time nodes, first Y cell)"""
+    out = []
+    first = 0
+    for (a, b, n, _) in lm.y_layers:
+        if out and out[-1][0] == n:
+            out[-1][1].append(b)
+        else:
+            out.append((n, [a, b], first))
+        first += n if lm.dim == 1 else 2 * n * n
+    return out
+
+
+def estimate(lm, x, M_grid=None, grid_k=1, ell=0.0, u0=None):
+    """The estimator of the layered solution x, slab group by slab group: on each group
+    (slabs of one spatial mesh = a tensor mesh) the tensor oracle estimator
+    (oracle.estimator.estimate, with its per-slab direct reconstruction solves) on
+    [lambda restricted to the group; u_delta interpolated at the group's X nodes, on the
+    coarse side of hanging interfaces]; totals summed, the trace term from the group
+    at t = 0.  Returns a dict of totals and per-Y-cell arrays."""
+    from . import estimator as OE
+
+    dim, p = lm.dim, lm.p
+    nY = lm.Y.n_dofs
+    Nt = p + 1
+    tot = dict(flux2=0.0, recon2=0.0, lambda_y2=0.0, trace2=0.0)
+    ncy = lm.Y.n_cells
+    per = {k: np.zeros(ncy) for k in ("flux_by_cell", "resid_by_cell", "recon_by_cell")}
+    XL = lm.x_layers
+    for (n, times, first) in slab_groups(lm):
+        sm = _uniform_spatial(dim, n)
+        stm = OM.SpaceTimeMesh(sm, np.array(times))
+        od = OM.spacetime_dofs(stm, p)
+        M = grid_M(dim, sm.vertices[sm.cells].mean(axis=1), M_grid, grid_k)
+        prob = OA.Problem(M, ell, u0)
+        S = OA.assemble(stm, od, prob)
+        ns, m = sm.n_cells, stm.n_intervals
+        # lambda: the group's tensor Y dofs are the general Y dofs of the same cells (identity rows)
+        lam = np.zeros(od.nY)
+        for c in range(ns * m):
+            gc = first + c
+            for i in range(od.y_map.shape[1]):
+                t = od.y_map[c, i]
+                if t < 0:
+                    continue
+                d_, w_ = lm.Y.rows(gc, i)
+                assert len(d_) == 1 and w_[0] == 1.0
+                lam[t] = x[d_[0]]
+        # u: the layered X function at the group's X nodes
+        sd = od.sd
+        nf = sd.n_free
+        free_nodes = np.argsort(np.where(sd.free_index >= 0, sd.free_index, np.iinfo(np.int64).max))[:nf]
+        xs = sd.node_coords[free_nodes]
+        tn_t = np.concatenate([stm.times[k] + (stm.times[k + 1] - stm.times[k]) * np.arange(p) / p
+                               for k in range(m)] + [stm.times[-1:]])
+
+        def x_layer_of(tv):
+            for k, (t0, t1, nn) in enumerate(XL):
+                if k > 0 and abs(tv - t0) < 1e-14:
+                    return k - 1 if XL[k - 1][2] < nn else k
+                if t0 <= tv < t1 or (k == len(XL) - 1 and abs(tv - t1) < 1e-14):
+                    return k
+            raise ValueError(tv)
+
+        keys = []
+        for tv in tn_t:
+            k = x_layer_of(tv)
+            keys.append((k, XL[k][2], XL[k][0], XL[k][1]))
+        PX = _prolongation(lm.X, len(tn_t) * nf, np.tile(xs, (len(tn_t), 1)), np.repeat(tn_t, nf),
+                           lambda r: keys[r // nf])
+        u = PX @ x[nY:]
+        r = OE.estimate(stm, od, prob, S, np.concatenate([lam, u]))
+        tot["flux2"] += r.flux2
+        tot["recon2"] += r.recon2
+        tot["lambda_y2"] += r.lambda_y2
+        if times[0] == 0.0:
+            tot["trace2"] += r.trace2
+        for k in per:
+            per[k][first:first + ns * m] = getattr(r, k)
+    tot["eta2"] = tot["lambda_y2"] + tot["trace2"]
+    tot["y_est2"] = tot["flux2"] + tot["recon2"]
+    tot.update(per)
+    return tot

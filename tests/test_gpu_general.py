@@ -114,3 +114,50 @@ def test_minres_on_the_general_system(ctx, case):
     assert st["converged"] == 1
     xd = OE.main_solve(S)
     assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+def _groups(lm):
+    """slab groups of a layered mesh as tensor meshes with their multigrid hierarchies"""
+    out = []
+    for g in L.slab_groups(lm):
+        n, times = g["n"], g["times"]
+        mk = meshes.interval_mesh if lm.dim == 1 else meshes.union_jack_mesh
+        coarse, k = [], n // 2
+        while k >= 2:
+            coarse.append(mk(k, [0.0, 1.0]))
+            k //= 2
+        out.append((mk(n, times), coarse, g["first_y_cell"]))
+    return out
+
+
+EST_CASES = [CASES[0], CASES[1], CASES[3], CASES[5], CASES[6]]
+
+
+@pytest.mark.parametrize("prec", [apf.APF_FP64, apf.APF_FP32_REFINE], ids=["fp64", "fp32ref"])
+@pytest.mark.parametrize("case", EST_CASES, ids=_ids)
+def test_general_estimator_matches_oracle(ctx, case, prec):
+    """The estimator on a non-tensor mesh (patches per (slab, vertex) on each slab's own
+    mesh, u_delta through the X ancestors, slab reconstruction from that u) against the
+    oracle's slab-group decomposition with direct slab solves, on the oracle's direct
+    solution with lambda perturbed by 10% (lambda_tilde depends on u only, so the
+    reconstruction term is far from zero): every total and per-cell value within 1e-10
+    relative."""
+    print(f"general estimator {_ids(case)} prec={prec}", flush=True)
+    dim, p, xl, yr, kap = case
+    lm, gs, S = _assemble(ctx, case, ell=0.3)
+    x = OE.main_solve(S)
+    nY = lm.Y.n_dofs
+    x[:nY] *= 1.0 + 0.1 * np.random.default_rng(9).standard_normal(nY)
+    k = 4
+    Mg = kappa_grid(dim, k, seed=17) if kap else None
+    ge = apf.GeneralEstimator(gs, _groups(lm))
+    ge.assemble(poly_u0, Mg, k, 0.3)
+    est = ge.estimate(x, prec, 2)
+    ref = OL.estimate(lm, x, Mg, k, 0.3, poly_u0)
+    assert est["max_patch_residual"] <= 1e-10
+    assert ref["recon2"] > 1e-8 * ref["y_est2"]  # the slabs' reconstruction is not lambda itself
+    for key in ("flux2", "recon2", "lambda_y2", "trace2", "eta2", "y_est2"):
+        assert abs(est[key] - ref[key]) <= 1e-10 * abs(ref[key]), (key, est[key], ref[key])
+    for key in ("flux_by_cell", "recon_by_cell"):
+        a, b = est[key], ref[key]
+        assert np.all(np.abs(a - b) <= 1e-9 * np.abs(b) + 1e-13 * np.abs(b).sum()), key
