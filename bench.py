@@ -12,7 +12,7 @@ synchronises per iteration); MINRES seconds per solve are reported beside it.
 Inputs are resident in HBM when the timed region starts (the mesh, pattern and
 coefficients are uploaded once).
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5]
+    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c3g|c4|c5]
 
 Prints one JSON line.
 """
@@ -139,6 +139,89 @@ def run_c3(args):
                                "u0 = indicator of [0.25,0.75]^2; p = 1, 2, 3 in turn",
                    "space_cells": int(m.cells.shape[0]), "intervals": int(m.times.size - 1)},
         "roofline": dict(roof[dom], degree=dom_p, traffic=None),
+        "per_degree": per_p,
+    }
+    print(json.dumps(out))
+
+
+def run_c3_graded(args):
+    """The graded half of C3: the reference's split policy applied 5 times (generations 2..10)
+    to the cells with t0 = 0 of a 2x2 union-jack mesh on [0, 1]: time layers [1/2, 1] at
+    4x4 squares down to [0, 1/32] and [1/32, 1/16] at 64x64, hanging X nodes at every layer
+    interface; u0 = indicator of [0.25, 0.75]^2; p = 1, 2, 3 in turn.  A step: element-
+    quadrature assembly (apf_general_assemble) and the estimator over the slab groups (slab
+    solve, patches, indicators) on the solution of one MINRES solve per degree (caller's-CSR
+    path, Chebyshev blocks; timed once, outside the steps).  value = sum of DOFs / sum of
+    assembly + estimator times."""
+    import doubleadapparabolicfem_amd as apf
+    from doubleadapparabolicfem_amd import layered as L, meshes
+
+    xl = L.graded_layers(2, 5)
+    u0 = meshes.indicator_initial().u0
+    ctx = apf.Context(0)
+    per_p, tot_dofs, tot_ms, wall = {}, 0, 0.0, 0.0
+    warm = max(args.warmup, 1)
+    for p in (1, 2, 3):
+        t0 = time.perf_counter()
+        lm = L.layered_mesh(2, p, xl)
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gs = apf.GeneralSystem(ctx, lm)
+        t_create = time.perf_counter() - t0
+        groups = []
+        for g in L.slab_groups(lm):
+            coarse, k = [], g["n"] // 2
+            while k >= 2:
+                coarse.append(meshes.union_jack_mesh(k, [0.0, 1.0]))
+                k //= 2
+            groups.append((meshes.union_jack_mesh(g["n"], g["times"]), coarse, g["first_y_cell"]))
+        ge = apf.GeneralEstimator(gs, groups)
+        ge.assemble(u0)
+        K = gs.matrix()
+        gs.assemble(u0)
+        b = gs.rhs()
+        print(f"c3g p={p}: {gs.n} unknowns, nnz {gs.nnz}; MINRES ...", file=sys.stderr, flush=True)
+        t1 = time.perf_counter()
+        x, st = K.minres(b, gs.nY, tol=1e-10, max_it=40000, check_interval=50, degree=16, precond=apf.APF_FP64)
+        t_minres = time.perf_counter() - t1
+        print(f"c3g p={p}: MINRES {st['iterations']} iterations, {t_minres:.2f} s", file=sys.stderr, flush=True)
+        rec = []
+        for i in range(warm + args.steps):
+            t0 = time.perf_counter()
+            t_asm = gs.assemble(u0)
+            est = ge.estimate(x, apf.APF_FP32_REFINE, 2, per_cell=False)
+            if i >= warm:
+                wall += time.perf_counter() - t0
+                rec.append((t_asm, est))
+        print(f"c3g p={p}: steps done", file=sys.stderr, flush=True)
+        t_asm = float(np.mean([r[0] for r in rec]))
+        t_est = float(np.mean([(r[1]["seconds_slab"] + r[1]["seconds_patches"] + r[1]["seconds_diagnostics"]) * 1e3
+                               for r in rec]))
+        est = rec[-1][1]
+        n_dofs = gs.n
+        tot_dofs += n_dofs
+        tot_ms += t_asm + t_est
+        hanging = int(np.sum(np.diff(lm.X.restr_ptr) > 1))
+        per_p[f"p{p}"] = {
+            "n_X": lm.X.n_dofs, "n_Y": lm.Y.n_dofs, "nnz_K": gs.nnz, "prisms": int(lm.Y.n_cells),
+            "hanging_x_rows": hanging, "slab_groups": len(groups),
+            "assembly_ms": t_asm, "assembly_kernel": "k_gen_cells (element quadrature) + k_gen_bottom",
+            "estimator_ms_mixed": t_est,
+            "slab_solve_ms": float(np.mean([r[1]["seconds_slab"] for r in rec])) * 1e3,
+            "minres_s_one_solve": t_minres, "minres_iterations": int(st["iterations"]),
+            "minres_true_rel_residual": st["true_relative_residual"],
+            "minres_preconditioner": ("Chebyshev blocks of degree 16, FP64: Jacobi on A_y, and on the X block "
+                                      "the Schur complement -C + B diag(A_y)^-1 B^T applied from K"),
+            "host_mesh_build_s": t_build, "host_pattern_and_upload_s": t_create,
+            "estimator": {k: est[k] for k in ("lambda_y2", "eta2", "flux2", "recon2", "y_est2", "max_patch_residual")}}
+    out = {
+        "metric": METRIC, "value": tot_dofs / (tot_ms * 1e-3), "unit": "DOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": warm, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "2+1D graded toward t=0 by 5 double rounds of the split policy from a 2x2 "
+                               "union-jack mesh: 6 time layers (1/2..1 at 4x4 squares .. 0..1/32 at 64x64), "
+                               "hanging X nodes at the layer interfaces; u0 = indicator of [0.25,0.75]^2; "
+                               "p = 1, 2, 3 in turn"},
         "per_degree": per_p,
     }
     print(json.dumps(out))
@@ -289,6 +372,9 @@ def main():
 
     if args.config == "c3":
         run_c3(args)
+        return
+    if args.config == "c3g":
+        run_c3_graded(args)
         return
     W = workload(args.config)
     sp = apf.Space(W["mesh"], W["p"], coarse=W["coarse"])

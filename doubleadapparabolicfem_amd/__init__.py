@@ -611,6 +611,11 @@ class GeneralSystem:
         _check(lib().apf_general_download(self.handle, _ptr(rowptr), _ptr(col), _ptr(vals), _ptr(b)), self.ctx.handle)
         return rowptr, col, vals, b
 
+    def rhs(self):
+        b = np.zeros(self.n)
+        _check(lib().apf_general_download(self.handle, None, None, None, _ptr(b)), self.ctx.handle)
+        return b
+
     def matrix(self):
         """a CsrMatrix view of K (valid while this system lives)"""
         h = ctypes.c_void_p()

@@ -17,7 +17,9 @@
 // 2nd ed., Alg. 12.1) of `degree` steps from a zero start on the interval
 // [lambda_max / degree^2, lambda_max], with lambda_max the Gershgorin bound of D^-1 B
 // (an upper bound, so q(D^-1 B) D^-1 is symmetric positive definite).  Without Hhat
-// the X block takes the diagonal of the Schur complement -C + B diag(A_y)^-1 B^T.
+// the X block is the same polynomial in D_S^-1 S for the Schur complement
+// S = -C + B diag(A_y)^-1 B^T, applied from K's blocks without forming S (D_S = diag(S),
+// lambda_max from a Gershgorin bound of |B| diag(A_y)^-1 |B^T| + |C|).
 // The polynomial is applied in FP32 (APF_FP32_REFINE) or FP64; the SpMV, the Lanczos
 // recurrences and the inner products stay FP64.  The stopping rule is the reference's:
 // the true relative residual |b - K x| / |b| is checked every `check_interval` iterations
@@ -55,10 +57,10 @@ constexpr int kRowLanes = 16;    // lanes per matrix row
 constexpr size_t kPartDoubles = 4 * kRowGrid;
 
 // Scratch of the allocation-free entry point: one array per GPU, held until the process
-// ends (12 vectors of APF_NOALLOC_MAX_N doubles and the reduction partials, 25.2 MB).
+// ends (13 vectors of APF_NOALLOC_MAX_N doubles and the reduction partials, 27.3 MB).
 // Its layout for n unknowns is work_layout(n).  Host code never uses the name `scratch`
 // as an address: apf_internal_noalloc_init resolves the device address per device.
-constexpr size_t kScratchBytes = 12 * sizeof(double) * static_cast<size_t>(APF_NOALLOC_MAX_N) +
+constexpr size_t kScratchBytes = 13 * sizeof(double) * static_cast<size_t>(APF_NOALLOC_MAX_N) +
                                  kPartDoubles * sizeof(double) + 4096;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -74,7 +76,7 @@ int fail(apf_ctx* ctx, int code, const std::string& m)
 
 // Work buffers for n unknowns inside one allocation (offsets in bytes, 256-aligned).
 struct WorkLayout {
-    size_t r1, r2, t, y, v, w, w1, w2, dinv, cx0, cx1, cd, part, scal, res, lam, total;
+    size_t r1, r2, t, y, v, w, w1, w2, dinv, cx0, cx1, cd, cw, part, scal, res, lam, total;
 };
 
 size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
@@ -96,6 +98,7 @@ WorkLayout work_layout(int64_t n)
     L.cx0 = o; o += vec;  // Chebyshev iterates and direction (FP32 uses half of each)
     L.cx1 = o; o += vec;
     L.cd = o; o += vec;
+    L.cw = o; o += vec;   // D_Y^-1 B^T v inside a Schur-complement product
     L.part = o; o += align256(kPartDoubles * sizeof(double));
     L.scal = o; o += align256(sizeof(MinresScalars));
     L.res = o; o += align256(4 * sizeof(double));
@@ -286,17 +289,117 @@ __global__ void __launch_bounds__(256) k_cheb_step(int64_t rows, const int64_t* 
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
-// z = D^-1 r on a diagonal block, with the partials of z . r
-__global__ void __launch_bounds__(256) k_diag_apply(int64_t rows, const double* __restrict__ dinv,
-                                                    const double* __restrict__ r, double* __restrict__ z,
-                                                    double* __restrict__ part)
+// r_i = sum over the B^T entries (columns >= nY) of |K_ik|, Y rows
+__global__ void __launch_bounds__(256) k_bt_absrow(int64_t nY, const int64_t* __restrict__ rowptr,
+                                                   const int* __restrict__ col, const double* __restrict__ val,
+                                                   double* __restrict__ out)
 {
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+    const int64_t gs = static_cast<int64_t>(gridDim.x) * blockDim.x / kRowLanes;
+    for (int64_t i = g0; i < nY; i += gs) {
+        double a = 0.0;
+        for (int64_t e = rowptr[i] + lane; e < rowptr[i + 1]; e += kRowLanes)
+            if (col[e] >= nY) a += fabs(val[e]);
+        a = group_sum(a);
+        if (lane == 0) out[i] = a;
+    }
+}
+
+// Gershgorin bound of D_S^-1 S on the X rows, S = -K_XX + K_XY D_Y^-1 K_YX:
+//   sum_k |S_jk| <= sum_{i < nY} |K_ji| r_i / K_ii + sum_{k >= nY} |K_jk|   (block partial maxima)
+__global__ void __launch_bounds__(256) k_schur_gersh(int64_t nY, int64_t n, const int64_t* __restrict__ rowptr,
+                                                     const int* __restrict__ col, const double* __restrict__ val,
+                                                     const double* __restrict__ dinvY, const double* __restrict__ r,
+                                                     const double* __restrict__ dinvX, double* __restrict__ part)
+{
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+    const int64_t gs = static_cast<int64_t>(gridDim.x) * blockDim.x / kRowLanes;
+    double gmax = 0.0;
+    for (int64_t j = nY + g0; j < n; j += gs) {
+        double a = 0.0;
+        for (int64_t e = rowptr[j] + lane; e < rowptr[j + 1]; e += kRowLanes) {
+            const int c = col[e];
+            a += (c < nY) ? fabs(val[e]) * r[c] * dinvY[c] : fabs(val[e]);
+        }
+        a = group_sum(a);
+        if (lane == 0) gmax = fmax(gmax, a * dinvX[j - nY]);
+    }
+    const double m = block_max_nn(gmax);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// w_i = D_Y^-1 (K_YX v)_i on the Y rows; v holds X values at the column indices [nY, n)
+template <typename CT>
+__global__ void __launch_bounds__(256) k_schur_w(int64_t nY, const int64_t* __restrict__ rowptr,
+                                                 const int* __restrict__ col, const double* __restrict__ val,
+                                                 const double* __restrict__ dinvY, const CT* __restrict__ v,
+                                                 CT* __restrict__ w)
+{
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+    const int64_t gs = static_cast<int64_t>(gridDim.x) * blockDim.x / kRowLanes;
+    for (int64_t i = g0; i < nY; i += gs) {
+        CT s = 0;
+        for (int64_t e = rowptr[i] + lane; e < rowptr[i + 1]; e += kRowLanes) {
+            const int c = col[e];
+            if (c >= nY) s += static_cast<CT>(val[e]) * v[c];
+        }
+        for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kRowLanes);
+        if (lane == 0) w[i] = s * static_cast<CT>(dinvY[i]);
+    }
+}
+
+// One Chebyshev step on the X rows for S y = r with S y = K_XY w - K_XX y, w = D_Y^-1 K_YX y
+// (k_schur_w of the same iterate); otherwise as k_cheb_step.  Vectors are indexed by K's
+// row / column index (X part at [nY, n)).
+template <typename CT>
+__global__ void __launch_bounds__(256) k_cheb_x_step(int64_t nY, int64_t n, const int64_t* __restrict__ rowptr,
+                                                     const int* __restrict__ col, const double* __restrict__ val,
+                                                     const double* __restrict__ dinvX, const double* __restrict__ r,
+                                                     const CT* __restrict__ w, const CT* __restrict__ yo,
+                                                     CT* __restrict__ yn, CT* __restrict__ d,
+                                                     const double* __restrict__ lam, int slot, int step, double c_dir,
+                                                     double c_res, int last, double* __restrict__ z,
+                                                     double* __restrict__ part)
+{
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+    const int64_t gs = static_cast<int64_t>(gridDim.x) * blockDim.x / kRowLanes;
+    const CT cres = static_cast<CT>(c_res / lam[slot]);
+    const CT cdir = static_cast<CT>(c_dir);
     double dot = 0.0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < rows;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const double zi = dinv[i] * r[i];
-        z[i] = zi;
-        dot += zi * r[i];
+    for (int64_t j = nY + g0; j < n; j += gs) {
+        CT s = 0;
+        if (step > 0) {
+            for (int64_t e = rowptr[j] + lane; e < rowptr[j + 1]; e += kRowLanes) {
+                const int c = col[e];
+                const CT a = static_cast<CT>(val[e]);
+                s += (c < nY) ? a * w[c] : -a * yo[c];
+            }
+            for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kRowLanes);
+        }
+        if (lane == 0) {
+            const CT rj = static_cast<CT>(r[j]);
+            const CT dj = static_cast<CT>(dinvX[j - nY]);
+            CT dn, yv;
+            if (step == 0) {
+                dn = cres * dj * rj;
+                yv = dn;
+            } else {
+                dn = cdir * d[j] + cres * dj * (rj - s);
+                yv = yo[j] + dn;
+            }
+            if (last) {
+                const double zj = static_cast<double>(yv);
+                z[j] = zj;
+                dot += zj * r[j];
+            } else {
+                d[j] = dn;
+                yn[j] = yv;
+            }
+        }
     }
     const double bs = block_sum(dot);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
@@ -378,16 +481,22 @@ Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsig
         hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, G, lam, 1);
     } else {
         hipLaunchKernelGGL(k_schur_diag, grid, block, 0, st, nY, n, K.rowptr, K.col, K.val, dinv, dinv + nY, part);
-        hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, G, lam, 1);
+        hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, G, lam, 1);  // flags a bad diagonal
+        hipLaunchKernelGGL(k_bt_absrow, grid, block, 0, st, nY, K.rowptr, K.col, K.val, t);
+        hipLaunchKernelGGL(k_schur_gersh, grid, block, 0, st, nY, n, K.rowptr, K.col, K.val, dinv, t, dinv + nY,
+                           part + G);
+        hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part + G, G, lam, 2);
     }
-    double hl[2];
-    HIP_TRY(hipMemcpyAsync(hl, lam, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    double hl[3] = {0.0, 0.0, 0.0};
+    HIP_TRY(hipMemcpyAsync(hl, lam, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (!(hl[0] > 0.0) || !std::isfinite(hl[0]))
         throw std::runtime_error("the lambda block A_y is not positive definite (non-positive diagonal)");
     if (!(hl[1] >= 0.0) || !std::isfinite(hl[1]))
         throw std::runtime_error(H ? "Hhat is not positive definite (non-positive diagonal)"
                                    : "the Schur complement diagonal of the X block is not positive");
+    if (!H && !(hl[2] > 0.0 && std::isfinite(hl[2])))
+        throw std::runtime_error("the Schur complement of the X block has no positive Gershgorin bound");
 
     const Cheb cc = chebyshev_coefficients(degree);
     // z = P^-1 r with the partials of z . r in part[0, 2G)
@@ -408,11 +517,33 @@ Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsig
         if (fp32) run(float{});
         else run(double{});
     };
+    // X block without Hhat: the Chebyshev polynomial in D_S^-1 S, S applied from K's blocks
+    auto cheb_schur = [&](const double* r, double* z, double* pp) {
+        auto run = [&](auto tag) {
+            using CT = decltype(tag);
+            CT* cx0 = at<CT>(work, L.cx0);
+            CT* cx1 = at<CT>(work, L.cx1);
+            CT* cd = at<CT>(work, L.cd);
+            CT* cw = at<CT>(work, L.cw);
+            for (int k = 0; k < degree; ++k) {
+                CT* yo = (k % 2 == 0) ? cx1 : cx0;
+                CT* yn = (k % 2 == 0) ? cx0 : cx1;
+                if (k > 0)
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_schur_w<CT>), grid, block, 0, st, nY, K.rowptr, K.col, K.val,
+                                       dinv, yo, cw);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_x_step<CT>), grid, block, 0, st, nY, n, K.rowptr, K.col,
+                                   K.val, dinv + nY, r, cw, yo, yn, cd, lam, 2, k, cc.cdir[k], cc.cres[k],
+                                   k == degree - 1 ? 1 : 0, z, pp);
+            }
+        };
+        if (fp32) run(float{});
+        else run(double{});
+    };
     auto precond = [&](double* tt, const double* rr2, double* z) {
         if (rr2) hipLaunchKernelGGL(k_minres_upd, grid, block, 0, st, n, scal, tt, rr2);
         cheb(nY, K, dinv, tt, z, part, 0);
         if (H) cheb(nX, *H, dinv + nY, tt + nY, z + nY, part + G, 1);
-        else hipLaunchKernelGGL(k_diag_apply, grid, block, 0, st, nX, dinv + nY, tt + nY, z + nY, part + G);
+        else cheb_schur(tt, z, part + G);
     };
     auto spmv = [&](const double* xv, double* yv) {
         const int64_t rows_per_block = (256 / 16) * 2;

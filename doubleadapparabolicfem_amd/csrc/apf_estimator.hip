@@ -2205,7 +2205,6 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         };
         auto launch = [&](auto tag) {
             using FT = decltype(tag);
-            FT* Ki = std::is_same<FT, float>::value ? reinterpret_cast<FT*>(B.Ki32) : reinterpret_cast<FT*>(B.Ki);
             // the factor of one bucket in precision F2 (FP32: blocked Gauss-Jordan with MFMA
             // panel updates up to order 128; otherwise one register-resident Gauss-Jordan
             // where the matrix fits the register file, else block elimination with two)
