@@ -251,6 +251,157 @@ __global__ void k_coarse_solve(int n, int nc, int ld, const R* __restrict__ inv,
     X[static_cast<int64_t>(i) * ld + c] = acc;
 }
 
+// ---- fused coarse tail of the V-cycle
+// Levels tail_from..L-1 of one column per workgroup: the right-hand side and
+// solution of every level and three work vectors of the first level stay in
+// LDS; the level matrices are read from global memory (L2-resident: a few
+// hundred KB shared by all columns).  The arithmetic is that of the per-level
+// kernels (Chebyshev start, fused steps, restriction, coarsest dense solve,
+// prolongation, post-smoothing) in the same order per row.
+constexpr int MG_TAIL_THREADS = 1024;
+constexpr int MG_TAIL_MAX_NU = 8;
+constexpr size_t MG_TAIL_LDS_CAP = 60 * 1024;  // within the default dynamic LDS limit
+
+template <typename R>
+struct MGTailLevel {
+    int nf;
+    int off;  // LDS offset of b (x follows at off + nf)
+    const int* ptr;
+    const int* col;
+    const R* K;
+    const R* M;
+    const R* dK;
+    const R* dM;
+    const int* Rptr;  // restriction to the next level (rows: the next level's nodes)
+    const int* Rcol;
+    const R* Rval;
+    const int* Pptr;  // prolongation from the next level (rows: this level's nodes)
+    const int* Pcol;
+    const R* Pval;
+    R inv_theta;
+    R c1[MG_TAIL_MAX_NU];
+    R c2[MG_TAIL_MAX_NU];
+};
+
+template <typename R>
+__device__ inline R tail_row(int i, const int* __restrict__ ptr, const int* __restrict__ col,
+                             const R* __restrict__ K, const R* __restrict__ M, R s, const R* x)
+{
+    R acc = 0;
+    const int b = ptr[i], e = ptr[i + 1];
+#pragma unroll 4
+    for (int k = b; k < e; ++k) acc += (K[k] + s * M[k]) * x[col[k]];
+    return acc;
+}
+
+template <typename R>
+__device__ inline R tail_csr_row(int i, const int* __restrict__ ptr, const int* __restrict__ col,
+                                 const R* __restrict__ val, const R* x)
+{
+    R acc = 0;
+    const int b = ptr[i], e = ptr[i + 1];
+#pragma unroll 4
+    for (int k = b; k < e; ++k) acc += val[k] * x[col[k]];
+    return acc;
+}
+
+// Chebyshev-Jacobi smoothing of one level in LDS (as MGDevice::smooth); leaves
+// r = b - A x in r
+template <typename R>
+__device__ void tail_smooth(const MGTailLevel<R>& L, R s, R* sm, R* r, R* d, R* dn, int nu, bool zero_start)
+{
+    const int nf = L.nf;
+    const R* b = sm + L.off;
+    R* x = sm + L.off + nf;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+        const R dg = L.dK[i] + s * L.dM[i];
+        if (zero_start) {
+            d[i] = b[i] * L.inv_theta / dg;
+        } else {
+            const R rv = b[i] - tail_row(i, L.ptr, L.col, L.K, L.M, s, x);
+            r[i] = rv;
+            d[i] = rv * L.inv_theta / dg;
+        }
+    }
+    __syncthreads();
+    R* dc = d;
+    R* dx = dn;
+    for (int k = 0; k < nu; ++k) {
+        const bool last = k == nu - 1;
+        const R c1 = L.c1[k], c2 = L.c2[k];
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+            const R v = tail_row(i, L.ptr, L.col, L.K, L.M, s, dc);
+            const R dv = dc[i];
+            R rv;
+            if (zero_start && k == 0) {
+                x[i] = dv;
+                rv = b[i] - v;
+            } else {
+                x[i] += dv;
+                rv = r[i] - v;
+            }
+            r[i] = rv;
+            if (!last) dx[i] = c1 * dv + c2 * rv / (L.dK[i] + s * L.dM[i]);
+        }
+        __syncthreads();
+        R* tmp = dc;
+        dc = dx;
+        dx = tmp;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(MG_TAIL_THREADS)
+    k_mg_tail(const MGTailLevel<R>* __restrict__ lv, int nl, int nu, int ld, const R* __restrict__ sig,
+              const int* __restrict__ sidx, const R* __restrict__ inv, int64_t blk_total,
+              const int* __restrict__ comp_start, const int* __restrict__ node_comp,
+              const int64_t* __restrict__ blk_off, const R* __restrict__ Bin, R* __restrict__ Xout, int tmp_off)
+{
+    extern __shared__ unsigned char tail_lds[];
+    R* sm = reinterpret_cast<R*>(tail_lds);
+    const int c = blockIdx.x;
+    const R s = sig[c];
+    const int nf0 = lv[0].nf;
+    R* r = sm + tmp_off;
+    R* d = r + nf0;
+    R* dn = d + nf0;
+    for (int i = threadIdx.x; i < nf0; i += blockDim.x) sm[lv[0].off + i] = Bin[static_cast<int64_t>(i) * ld + c];
+    __syncthreads();
+    for (int j = 0; j + 1 < nl; ++j) {
+        const MGTailLevel<R>& F = lv[j];
+        const MGTailLevel<R>& C = lv[j + 1];
+        tail_smooth(F, s, sm, r, d, dn, nu, true);
+        for (int i = threadIdx.x; i < C.nf; i += blockDim.x)
+            sm[C.off + i] = tail_csr_row(i, F.Rptr, F.Rcol, F.Rval, r);
+        __syncthreads();
+    }
+    {
+        const MGTailLevel<R>& C = lv[nl - 1];
+        const R* A0 = inv + static_cast<int64_t>(sidx[c]) * blk_total;
+        const R* b = sm + C.off;
+        for (int i = threadIdx.x; i < C.nf; i += blockDim.x) {
+            const int comp = node_comp[i];
+            const int st = comp_start[comp], bs = comp_start[comp + 1] - st;
+            const R* A = A0 + blk_off[comp] + static_cast<int64_t>(i - st) * bs;
+            R acc = 0;
+            for (int j = 0; j < bs; ++j) acc += A[j] * b[st + j];
+            sm[C.off + C.nf + i] = acc;
+        }
+        __syncthreads();
+    }
+    for (int j = nl - 2; j >= 0; --j) {
+        const MGTailLevel<R>& F = lv[j];
+        const MGTailLevel<R>& C = lv[j + 1];
+        const R* xc = sm + C.off + C.nf;
+        R* x = sm + F.off + F.nf;
+        for (int i = threadIdx.x; i < F.nf; i += blockDim.x) x[i] += tail_csr_row(i, F.Pptr, F.Pcol, F.Pval, xc);
+        __syncthreads();
+        tail_smooth(F, s, sm, r, d, dn, nu, false);
+    }
+    for (int i = threadIdx.x; i < nf0; i += blockDim.x)
+        Xout[static_cast<int64_t>(i) * ld + c] = sm[lv[0].off + nf0 + i];
+}
+
 constexpr int MG_TT = 1024;  // threads of the transpose blocks
 constexpr int MG_LB = 4;     // staging loads in flight per thread
 
@@ -491,7 +642,73 @@ void MGDevice<R>::build(System* sys, const MGSetupHost& H, int nyc_)
     up(d_MtInvH, H.MtInvH);
     d_Z = mg_alloc<R>(sys, static_cast<size_t>(levels[0].nf) * ld);
     d_Z2 = mg_alloc<R>(sys, static_cast<size_t>(levels[0].nf) * ld);
+    setup_tail(sys);
     ready = true;
+}
+
+// Choose the first level from which the rest of the V-cycle fits one workgroup's
+// LDS (b and x of every level plus three work vectors of the first, at most
+// MG_TAIL_LDS_CAP) and upload the level descriptors.  APF_MG_TAIL=0 keeps the
+// per-level launches.
+template <typename R>
+void MGDevice<R>::setup_tail(System* sys)
+{
+    const int L = static_cast<int>(levels.size());
+    tail_from = -1;
+    if (const char* e = std::getenv("APF_MG_TAIL"))
+        if (std::atoi(e) == 0) return;
+    if (nu > MG_TAIL_MAX_NU || L < 2) return;
+    int from = -1;
+    size_t bytes = 0;
+    for (int l = 0; l + 1 < L; ++l) {
+        size_t words = 3 * static_cast<size_t>(levels[l].nf);
+        for (int j = l; j < L; ++j) words += 2 * static_cast<size_t>(levels[j].nf);
+        if (words * sizeof(R) <= MG_TAIL_LDS_CAP) {
+            from = l;
+            bytes = words * sizeof(R);
+            break;
+        }
+    }
+    if (from < 0) return;
+    std::vector<MGTailLevel<R>> h(L - from);
+    int off = 0;
+    for (int l = from; l < L; ++l) {
+        const Level& d = levels[l];
+        MGTailLevel<R>& t = h[l - from];
+        t = MGTailLevel<R>{};
+        t.nf = d.nf;
+        t.off = off;
+        off += 2 * d.nf;
+        t.ptr = d.ptr;
+        t.col = d.col;
+        t.K = d.K;
+        t.M = d.M;
+        t.dK = d.dK;
+        t.dM = d.dM;
+        t.Rptr = d.Rptr;
+        t.Rcol = d.Rcol;
+        t.Rval = d.Rval;
+        t.Pptr = d.Pptr;
+        t.Pcol = d.Pcol;
+        t.Pval = d.Pval;
+        // the coefficients of smooth(), in the same type and order
+        const R lmax = static_cast<R>(d.lmax);
+        const R lmin = lmax / static_cast<R>(cheb_ratio);
+        const R theta = (lmax + lmin) / 2, delta = (lmax - lmin) / 2;
+        const R sigma1 = theta / delta;
+        t.inv_theta = static_cast<R>(1) / theta;
+        R rho = 1 / sigma1;
+        for (int k = 0; k < nu; ++k) {
+            const R rho_n = 1 / (2 * sigma1 - rho);
+            t.c1[k] = rho_n * rho;
+            t.c2[k] = 2 * rho_n / delta;
+            rho = rho_n;
+        }
+    }
+    if (!d_tail) d_tail = mg_alloc<MGTailLevel<R>>(sys, h.size());
+    up(static_cast<MGTailLevel<R>*>(d_tail), h);
+    tail_from = from;
+    tail_lds = bytes;
 }
 
 template <typename R>
@@ -512,12 +729,18 @@ void MGDevice<R>::refresh(const MGSetupHost& H)
     up(d_cinv, conv<R>(H.coarse_inv));
     up(d_W, H.W);
     up(d_MtInvH, H.MtInvH);
+    // the smoothing bounds may have changed: rewrite the tail descriptors (d_tail is
+    // already allocated, so no System is needed)
+    if (d_tail) setup_tail(nullptr);
 }
 
 template <typename R>
-void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_start)
+void MGDevice<R>::smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bool zero_start)
 {
     Level& d = levels[l];
+    R* X = d.X + c0;
+    R* B = d.B + c0;
+    R* Rr = d.Rr + c0;
     const R lmax = static_cast<R>(d.lmax);
     const R lmin = lmax / static_cast<R>(cheb_ratio);
     const R theta = (lmax + lmin) / 2, delta = (lmax - lmin) / 2;
@@ -527,55 +750,62 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_
     const unsigned rows_grid = grid_for(d.nf, 4);
     if (zero_start) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_start_zero<R>), dim3(grid_for(nel, blk)), dim3(blk), 0, st, d.nf, nc,
-                           ld, d.dK, d.dM, sig, static_cast<R>(1) / theta, d.B, d.D);
+                           ld, d.dK, d.dM, sig, static_cast<R>(1) / theta, B, d.D + c0);
     } else {
         with_passes(nc, [&](auto np) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_resid_start<R, decltype(np)::value>), dim3(rows_grid), dim3(256), 0,
-                               st, d.nf, d.ptr, d.col, d.K, d.M, sig, nc, ld, d.X, d.B, d.Rr, d.D, d.dK, d.dM,
+                               st, d.nf, d.ptr, d.col, d.K, d.M, sig, nc, ld, X, B, Rr, d.D + c0, d.dK, d.dM,
                                static_cast<R>(1) / theta);
         });
     }
     // the direction alternates between D and Tm
-    R* dcur = d.D;
-    R* dnext = d.Tm;
+    R* dcur = d.D + c0;
+    R* dnext = d.Tm + c0;
     R rho = 1 / sigma1;
     for (int k = 0; k < nu; ++k) {
         const R rho_n = 1 / (2 * sigma1 - rho);
         const R c1 = rho_n * rho, c2 = 2 * rho_n / delta;
         with_passes(nc, [&](auto np) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_cheb_fused<R, decltype(np)::value>), dim3(rows_grid), dim3(256), 0,
-                               st, d.nf, d.ptr, d.col, d.K, d.M, sig, nc, ld, dcur, d.X, d.Rr, dnext, d.dK, d.dM, c1, c2,
-                               (k == nu - 1) ? 1 : 0, (zero_start && k == 0) ? d.B : nullptr);
+                               st, d.nf, d.ptr, d.col, d.K, d.M, sig, nc, ld, dcur, X, Rr, dnext, d.dK, d.dM, c1, c2,
+                               (k == nu - 1) ? 1 : 0, (zero_start && k == 0) ? B : nullptr);
         });
         std::swap(dcur, dnext);
         rho = rho_n;
     }
 }
 
-// V-cycle on level l: B_l -> X_l (columns [0, nc))
+// V-cycle on level l: B_l -> X_l (columns [c0, c0 + nc); sig and sidx already offset)
 template <typename R>
-void MGDevice<R>::vcycle(hipStream_t st, int l, int nc, const R* sig, const int* sidx)
+void MGDevice<R>::vcycle(hipStream_t st, int l, int c0, int nc, const R* sig, const int* sidx)
 {
     Level& d = levels[l];
     const int L = static_cast<int>(levels.size());
+    if (l == tail_from) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_tail<R>), dim3(nc), dim3(MG_TAIL_THREADS), tail_lds, st,
+                           static_cast<const MGTailLevel<R>*>(d_tail), L - tail_from, nu, ld, sig, sidx, d_cinv,
+                           blk_total, d_comp_start, d_node_comp, d_blk_off, d.B + c0, d.X + c0,
+                           static_cast<int>(tail_lds / sizeof(R)) - 3 * d.nf);
+        return;
+    }
     if (l == L - 1) {
         const int64_t nel = static_cast<int64_t>(d.nf) * nc;
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_coarse_solve<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, d.nf, nc, ld,
-                           d_cinv, blk_total, d_comp_start, d_node_comp, d_blk_off, sidx, d.B, d.X);
+                           d_cinv, blk_total, d_comp_start, d_node_comp, d_blk_off, sidx, d.B + c0, d.X + c0);
         return;
     }
     Level& c = levels[l + 1];
-    smooth(st, l, nc, sig, true);  // leaves r = B - A x in Rr
+    smooth(st, l, c0, nc, sig, true);  // leaves r = B - A x in Rr
     with_passes(nc, [&](auto np) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R, decltype(np)::value>), dim3(grid_for(d.Rn, 4)), dim3(256), 0, st,
-                           d.Rn, d.Rptr, d.Rcol, d.Rval, nc, ld, d.Rr, ld, c.B, 0);
+                           d.Rn, d.Rptr, d.Rcol, d.Rval, nc, ld, d.Rr + c0, ld, c.B + c0, 0);
     });
-    vcycle(st, l + 1, nc, sig, sidx);
+    vcycle(st, l + 1, c0, nc, sig, sidx);
     with_passes(nc, [&](auto np) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R, decltype(np)::value>), dim3(grid_for(d.Pn, 4)), dim3(256), 0, st,
-                           d.Pn, d.Pptr, d.Pcol, d.Pval, nc, ld, c.X, ld, d.X, 1);
+                           d.Pn, d.Pptr, d.Pcol, d.Pval, nc, ld, c.X + c0, ld, d.X + c0, 1);
     });
-    smooth(st, l, nc, sig, false);
+    smooth(st, l, c0, nc, sig, false);
 }
 
 // z = P^-1 r for the saddle-point vector r = [r_Y; r_X]; writes dot(z, r) partials.
@@ -597,8 +827,10 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R, false>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, nyc, ld,
                            S.nY, th, r, d_W, f.B);
-    // stage 1: all columns, Y columns with shift 0, X columns with shift s_i
-    vcycle(st, 0, ncol, d_sig, d_sidx);
+    // stage 1: all columns, Y columns with shift 0, X columns with shift s_i (only the Y
+    // columns when the caller reads only z's Y part)
+    if (y_only) vcycle(st, 0, T, nyc, d_sig + T, d_sidx + T);
+    else vcycle(st, 0, 0, ncol, d_sig, d_sidx);
     // stage 2 (X columns): apply K, then the shifted V-cycle again
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, ncol, ld, f.X,
                        d_Z);
@@ -607,7 +839,7 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R, decltype(np)::value>), dim3(grid_for(nf, 4)), dim3(256), 0,
                                st, nf, f.ptr, f.col, f.K, f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
         });
-        vcycle(st, 0, T, d_sig, d_sidx);
+        vcycle(st, 0, 0, T, d_sig, d_sidx);
         // X columns from the second V-cycle, Y columns from the first
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(static_cast<int64_t>(nf) * T, 256)), dim3(256),
                            0, st, nf, T, ld, f.X, d_Z);

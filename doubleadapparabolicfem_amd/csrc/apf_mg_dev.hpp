@@ -54,11 +54,18 @@ struct MGDevice {
     R* d_Z = nullptr;
     R* d_Z2 = nullptr;
     bool ready = false;
+    // fused coarse tail: levels tail_from..L-1 run in one launch (k_mg_tail), one
+    // workgroup per column with the level vectors in LDS; -1 = off
+    int tail_from = -1;
+    size_t tail_lds = 0;
+    void* d_tail = nullptr;  // [L - tail_from] level descriptors on the device
 
     void build(System* sys, const MGSetupHost& H, int nyc);
     void refresh(const MGSetupHost& H);
-    void smooth(hipStream_t st, int l, int nc, const R* sig, bool zero_start);
-    void vcycle(hipStream_t st, int l, int nc, const R* sig, const int* sidx);
+    void setup_tail(System* sys);
+    // the block pointers of every level are offset by c0 columns (c0 = T: the Y columns only)
+    void smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bool zero_start);
+    void vcycle(hipStream_t st, int l, int c0, int nc, const R* sig, const int* sidx);
     // y_only: the caller reads only z's Y part (r's X part is zero), so the second
     // V-cycle over the X columns is skipped and z's X part is not the preconditioned value
     void apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts,

@@ -249,7 +249,8 @@ int apf_csr_matrix_info(const apf_csr_matrix* M, int64_t* n, int64_t* nnz);
  * be NULL).  Preconditioner blockdiag(q(D^-1 A_y) D^-1, q(D^-1 Hhat) D^-1), q the
  * Chebyshev polynomial of `cheb_degree` steps (1..64) on [lambda_max / degree^2,
  * lambda_max] of the Jacobi-scaled block, applied in FP64 or FP32 (precond_prec);
- * without Hhat the X block is the Schur-complement diagonal.  Accepts when the true
+ * without Hhat the X block is the same polynomial in D_S^-1 S for the Schur complement
+ * S = -C + B diag(A_y)^-1 B^T, applied from K's blocks (D_S = diag(S)).  Accepts when the true
  * relative residual <= tol, checked every check_interval iterations
  * (iterative_residual_acceptance.hpp:117-216); APF_NOT_CONVERGED otherwise (x then
  * holds the last iterate).  b, x, x0 (may be NULL) are host arrays of length n. */
@@ -341,7 +342,7 @@ int apf_general_estimate(apf_general_estimator* ge, const double* x, int prec, i
                          double* flux_by_cell, double* resid_by_cell, double* recon_by_cell);
 
 /* Largest n accepted by apf_csr_minres_solve_noalloc: its work buffers live in one
- * file-scope device array of 12 * 8 * APF_NOALLOC_MAX_N bytes plus 32 KiB (25.2 MB) per
+ * file-scope device array of 13 * 8 * APF_NOALLOC_MAX_N bytes plus 36 KiB (27.3 MB) per
  * GPU, held until the process ends. */
 #define APF_NOALLOC_MAX_N 262144
 /* No counterpart in the reference.  The same solve as apf_csr_minres_solve (same kernels,
@@ -350,7 +351,7 @@ int apf_general_estimate(apf_general_estimator* ge, const double* x, int prec, i
  * hipMalloc / hipFree or allocating routine; its work buffers are the file-scope scratch
  * array above (at most APF_NOALLOC_MAX_N unknowns, else APF_BAD_ARG).  d_b and d_x are
  * device arrays of length n; with use_x0 != 0, d_x holds the initial guess on entry.
- * Without Hhat (the X block takes the Schur-complement diagonal).  Every call on one GPU
+ * Without Hhat (the X block is the Schur-complement polynomial).  Every call on one GPU
  * shares that GPU's scratch array and leaves its last contents there: calls that overlap
  * on one GPU (two threads, or two contexts on one device) race. */
 int apf_csr_minres_solve_noalloc(apf_ctx* ctx, const apf_csr_matrix* K, int64_t n_lambda, const double* d_b,
