@@ -478,6 +478,112 @@ __global__ void __launch_bounds__(256) k_ay_apply(int nf, int m, const int* __re
     for (int l = 0; l < NT; ++l) y[(static_cast<int64_t>(k) * NT + l) * nf + s] = acc[l];
 }
 
+// ---- conjugate gradients of the slab reconstruction with device-side scalars
+// cg[]: rz (current z . r), rz_old, pap, alpha, beta, rr, bad (breakdown flag)
+enum { CG_RZ, CG_RZ_OLD, CG_PAP, CG_ALPHA, CG_BETA, CG_RR, CG_BAD, CG_N };
+
+// rz from the preconditioner's dot partials; beta = rz / rz_old (0 on a cycle's first step)
+__global__ void k_cg_beta(double* __restrict__ cg, const double* __restrict__ part, int nparts, int first)
+{
+    const double rz = reduce_partials(part, nparts);
+    if (threadIdx.x != 0) return;
+    if (first) cg[CG_BAD] = 0.0;
+    if (!(rz > 0.0) || !isfinite(rz)) cg[CG_BAD] = 1.0;
+    cg[CG_BETA] = (first || cg[CG_BAD] != 0.0) ? 0.0 : rz / cg[CG_RZ];
+    cg[CG_RZ_OLD] = cg[CG_RZ];
+    cg[CG_RZ] = rz;
+}
+
+// With beta from k_cg_beta:  p = z + beta p,  y = A_y z + beta y  (= A_y p, since y holds
+// A_y p of the previous step), and the block partials of p . y.  A_y is applied
+// matrix-free as in k_ay_apply; each thread owns (interval k, free node s) and reads
+// p and y only at its own entries, so the updates in place are race-free.
+template <int NT>
+__global__ void __launch_bounds__(256) k_ay_cg(int nf, int m, const int* __restrict__ sptr,
+                                               const int* __restrict__ scol, const double* __restrict__ Ksp,
+                                               const double* __restrict__ times, TimeTables tt,
+                                               const double* __restrict__ z, double* __restrict__ p,
+                                               double* __restrict__ y, const double* __restrict__ cg, int first,
+                                               double* __restrict__ part)
+{
+    const double beta = first ? 0.0 : cg[CG_BETA];
+    double dot = 0.0;
+    const int64_t total = static_cast<int64_t>(m) * nf;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int k = static_cast<int>(i / nf), s = static_cast<int>(i - static_cast<int64_t>(k) * nf);
+        const double h = times[k + 1] - times[k];
+        const double* __restrict__ zk = z + static_cast<int64_t>(k) * NT * nf;
+        double acc[NT];
+#pragma unroll
+        for (int l = 0; l < NT; ++l) acc[l] = 0.0;
+        for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
+            const double kh = Ksp[e] * h;
+            const int c = scol[e];
+            double zv[NT];
+#pragma unroll
+            for (int lp = 0; lp < NT; ++lp) zv[lp] = zk[static_cast<int64_t>(lp) * nf + c];
+#pragma unroll
+            for (int l = 0; l < NT; ++l)
+#pragma unroll
+                for (int lp = 0; lp < NT; ++lp) acc[l] += (kh * tt.Mt[l * NT + lp]) * zv[lp];
+        }
+#pragma unroll
+        for (int l = 0; l < NT; ++l) {
+            const int64_t o = (static_cast<int64_t>(k) * NT + l) * nf + s;
+            const double pv = first ? z[o] : z[o] + beta * p[o];
+            const double yv = first ? acc[l] : acc[l] + beta * y[o];
+            p[o] = pv;
+            y[o] = yv;
+            dot += pv * yv;
+        }
+    }
+    const double bs = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// alpha = rz / (p . A p)
+__global__ void k_cg_alpha(double* __restrict__ cg, const double* __restrict__ part, int nparts)
+{
+    const double pap = reduce_partials(part, nparts);
+    if (threadIdx.x != 0) return;
+    cg[CG_PAP] = pap;
+    if (!(pap > 0.0) || !isfinite(pap)) cg[CG_BAD] = 1.0;
+    cg[CG_ALPHA] = cg[CG_BAD] != 0.0 ? 0.0 : cg[CG_RZ] / pap;
+}
+
+// lt += alpha p ; r -= alpha y ; block partials of r . r
+__global__ void k_cg_update(int64_t n, const double* __restrict__ p, const double* __restrict__ y,
+                            double* __restrict__ lt, double* __restrict__ r, const double* __restrict__ cg,
+                            double* __restrict__ part)
+{
+    const double a = cg[CG_ALPHA];
+    double dot = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        lt[i] += a * p[i];
+        const double rv = r[i] - a * y[i];
+        r[i] = rv;
+        dot += rv * rv;
+    }
+    const double bs = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+__global__ void k_cg_rr(double* __restrict__ cg, const double* __restrict__ part, int nparts)
+{
+    const double rr = reduce_partials(part, nparts);
+    if (threadIdx.x == 0) cg[CG_RR] = rr;
+}
+
+// r = r0 - y (the true residual of a restart)
+__global__ void k_sub(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        out[i] = a[i] - b[i];
+}
+
 // ============================================================== host side
 namespace apf {
 
@@ -563,22 +669,26 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     HIP_TRY(hipMemsetAsync(y, 0, nY * sizeof(double), st));
     const double g2 = resid2(y);  // |r0|^2
     HIP_TRY(hipMemcpyAsync(lt, sys->d_x, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
-    double* pd = sys->d_vec[4];  // search direction [p; 0]
-    auto dot = [&](const double* a, const double* b) {
-        hipLaunchKernelGGL(k_dot_parts, dim3(nparts), dim3(256), 0, st, nY, a, b, sys->d_part);
-        hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
-        double h = 0.0;
-        HIP_TRY(hipMemcpyAsync(&h, sys->d_res, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        return h;
+    double* pd = sys->d_vec[4];  // search direction
+    if (!sys->d_cg) sys->d_cg = dalloc<double>(sys, CG_N);
+    double* cg = sys->d_cg;
+    const int tiles = sys->mg64->n_partials(S.nf);
+    if (tiles > sys->part_cap) throw std::runtime_error("too many multigrid transpose tiles");
+    const int m = S.mesh.m();
+    const dim3 grid(static_cast<unsigned>(nparts)), block(256);
+    auto ay_cg = [&](int first) {
+        switch (S.Nt) {
+        case 2: hipLaunchKernelGGL(k_ay_cg<2>, grid, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        case 3: hipLaunchKernelGGL(k_ay_cg<3>, grid, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        case 4: hipLaunchKernelGGL(k_ay_cg<4>, grid, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        default: hipLaunchKernelGGL(k_ay_cg<5>, grid, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        }
     };
-    HIP_TRY(hipMemsetAsync(r + nY, 0, (n - nY) * sizeof(double), st));
-    HIP_TRY(hipMemsetAsync(pd + nY, 0, (n - nY) * sizeof(double), st));
     const double scale = g2 > 0.0 ? g2 : 1.0;
     int it = 0, cycle = 0;
     double rr = 0.0, prev = std::numeric_limits<double>::infinity();
     for (;; ++cycle) {
-        // true residual r = [r0 - A_y lt; 0]
+        // true residual r = r0 - A_y lt
         sys->ay_apply(lt, y);
         rr = std::sqrt(resid2(y) / scale);
         if (!(rr > tol)) {
@@ -594,30 +704,23 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
             break;
         }
         prev = rr;
-        HIP_TRY(hipMemcpyAsync(r, r0, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -1.0, y, 1.0, r);
-        double rz_old = 0.0;
+        hipLaunchKernelGGL(k_sub, grid, block, 0, st, nY, r0, y, r);
+        // one host read per step: |r|^2 and the breakdown flag
         for (int k = 0; it < max_it; ++k) {
-            if (sys->mg64->n_partials(S.nf) > sys->part_cap)
-                throw std::runtime_error("too many multigrid transpose tiles");
-            sys->mg64->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true);
-            const double rz = dot(r, z);
-            if (!(rz > 0.0)) break;
-            if (k == 0)
-                HIP_TRY(hipMemcpyAsync(pd, z, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
-            else
-                hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, 1.0, z, rz / rz_old, pd);
-            rz_old = rz;
-            sys->ay_apply(pd, y);  // A_y p
-            const double pap = dot(pd, y);
-            if (!(pap > 0.0)) break;
-            const double alpha = rz / pap;
-            hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, alpha, pd, 1.0, lt);
-            hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -alpha, y, 1.0, r);
+            sys->mg64->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true);  // z . r partials per tile
+            hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, st, cg, sys->d_part, tiles, k == 0 ? 1 : 0);
+            ay_cg(k == 0 ? 1 : 0);
+            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, cg, sys->d_part, nparts);
+            hipLaunchKernelGGL(k_cg_update, grid, block, 0, st, nY, pd, y, lt, r, cg, sys->d_part);
+            hipLaunchKernelGGL(k_cg_rr, dim3(1), dim3(256), 0, st, cg, sys->d_part, nparts);
+            double h[CG_N];
+            HIP_TRY(hipMemcpyAsync(h, cg, CG_N * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (h[CG_BAD] != 0.0) break;  // no update was made
             ++it;
             // the first cycle runs to tol / 10; a restart only has to show a 10x gain, so that
             // a solve at its round-off floor stops after a few iterations
-            if (std::sqrt(dot(r, r) / scale) <= std::max(0.1 * tol, cycle > 0 ? 0.1 * rr : 0.0)) break;
+            if (std::sqrt(h[CG_RR] / scale) <= std::max(0.1 * tol, cycle > 0 ? 0.1 * rr : 0.0)) break;
         }
     }
     HIP_TRY(hipGetLastError());

@@ -229,6 +229,7 @@ struct System {
     int64_t part_cap = 0;  // doubles in d_part (at least 4 nparts; grown to the multigrid's transpose tiles)
     MinresScalars* d_scal = nullptr;
     double* d_res = nullptr;
+    double* d_cg = nullptr;  // scalars of the slab reconstruction's conjugate gradients
     double* d_kdiag = nullptr;
     double* d_band = nullptr;
     float* d_band32 = nullptr;

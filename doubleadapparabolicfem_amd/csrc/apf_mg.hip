@@ -409,8 +409,9 @@ constexpr int MG_LB = 4;     // staging loads in flight per thread
 // A block of MG_TT threads owns th consecutive spatial rows: it reads the
 // vector coalesced along s into an LDS tile (row stride th + 1), keeps W in LDS
 // too when WL, and writes Z rows coalesced along the columns.
+// With yo (Y only) the X columns are neither read nor written.
 template <typename R, bool WL>
-__global__ void __launch_bounds__(MG_TT) k_mg_gather(int nf, int T, int nyc, int ld, int64_t nY, int th,
+__global__ void __launch_bounds__(MG_TT) k_mg_gather(int nf, int T, int nyc, int ld, int64_t nY, int th, int yo,
                                                    const double* __restrict__ r, const double* __restrict__ W,
                                                    R* __restrict__ Z)
 {
@@ -423,26 +424,27 @@ __global__ void __launch_bounds__(MG_TT) k_mg_gather(int nf, int T, int nyc, int
     // MG_LB elements per thread at a time: the loads are unconditional (indices
     // clamped into range) so that they are all issued before the first store;
     // rows past the last one store zero
-    const int nst = ncol * th;
+    const int cb = yo ? T : 0;
+    const int nst = (ncol - cb) * th;
     for (int e0 = threadIdx.x; e0 < nst; e0 += MG_LB * MG_TT) {
         double v[MG_LB];
 #pragma unroll
         for (int u = 0; u < MG_LB; ++u) {
             const int e = min(e0 + u * MG_TT, nst - 1);
-            const int c = e / th, sl = min(e % th, ns - 1);
+            const int c = cb + e / th, sl = min(e % th, ns - 1);
             v[u] = r[(c < T ? nY + static_cast<int64_t>(c) * nf : static_cast<int64_t>(c - T) * nf) + s0 + sl];
         }
 #pragma unroll
         for (int u = 0; u < MG_LB; ++u) {
             const int e = e0 + u * MG_TT;
-            if (e < nst) tile[(e / th) * ts + e % th] = (e % th < ns) ? v[u] : 0.0;
+            if (e < nst) tile[(cb + e / th) * ts + e % th] = (e % th < ns) ? v[u] : 0.0;
         }
     }
-    if (WL)
+    if (WL && !yo)
         for (int e = threadIdx.x; e < T * T; e += blockDim.x) Wl[e] = W[e];
     __syncthreads();
     const double* Wp = WL ? Wl : W;
-    for (int e = threadIdx.x; e < ns * T; e += blockDim.x) {
+    for (int e = threadIdx.x; e < (yo ? 0 : ns * T); e += blockDim.x) {
         const int sl = e / T, c = e % T;
         double v = 0.0;
 #pragma unroll 8
@@ -459,7 +461,7 @@ __global__ void __launch_bounds__(MG_TT) k_mg_gather(int nf, int T, int nyc, int
 // and dot(z, r) partials (one per block).  Tiles as in k_mg_gather; the entries
 // of r are loaded before the products that they multiply.
 template <typename R, bool WL>
-__global__ void __launch_bounds__(MG_TT) k_mg_scatter(int nf, int T, int Nt, int m, int ld, int64_t nY, int th,
+__global__ void __launch_bounds__(MG_TT) k_mg_scatter(int nf, int T, int Nt, int m, int ld, int64_t nY, int th, int yo,
                                                     const R* __restrict__ Z, const double* __restrict__ W,
                                                     const double* __restrict__ MtInvH, const double* __restrict__ r,
                                                     double* __restrict__ z, double* __restrict__ part)
@@ -470,26 +472,27 @@ __global__ void __launch_bounds__(MG_TT) k_mg_scatter(int nf, int T, int Nt, int
     double* Wl = lds_s + ncol * ts;  // [T][T] (WL)
     const int s0 = blockIdx.x * th;
     const int ns = min(th, nf - s0);
-    const int nst = ns * ncol;
+    const int cb = yo ? T : 0, nce = ncol - cb;
+    const int nst = ns * nce;
     for (int e0 = threadIdx.x; e0 < nst; e0 += MG_LB * MG_TT) {
         double v[MG_LB];
 #pragma unroll
         for (int u = 0; u < MG_LB; ++u) {
             const int e = min(e0 + u * MG_TT, nst - 1);
-            v[u] = static_cast<double>(Z[static_cast<int64_t>(s0 + e / ncol) * ld + e % ncol]);
+            v[u] = static_cast<double>(Z[static_cast<int64_t>(s0 + e / nce) * ld + cb + e % nce]);
         }
 #pragma unroll
         for (int u = 0; u < MG_LB; ++u) {
             const int e = e0 + u * MG_TT;
-            if (e < nst) tile[(e % ncol) * ts + e / ncol] = v[u];
+            if (e < nst) tile[(cb + e % nce) * ts + e / nce] = v[u];
         }
     }
-    if (WL)
+    if (WL && !yo)
         for (int e = threadIdx.x; e < T * T; e += blockDim.x) Wl[e] = W[e];
     __syncthreads();
     const double* Wp = WL ? Wl : W;
     double dot = 0.0;
-    for (int e = threadIdx.x; e < T * th; e += blockDim.x) {
+    for (int e = threadIdx.x; e < (yo ? 0 : T * th); e += blockDim.x) {
         const int c = e / th, sl = e % th;  // c plays tn
         if (sl >= ns) continue;
         const int64_t idx = nY + static_cast<int64_t>(c) * nf + s0 + sl;
@@ -821,20 +824,22 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
     // the transpose tile, and W when it fits in 40 KiB
     const bool wl = static_cast<size_t>(T) * T * sizeof(double) <= 40 * 1024;
     const size_t lds = (static_cast<size_t>(ncol) * (th + 1) + (wl ? static_cast<size_t>(T) * T : 0)) * sizeof(double);
+    const int yo = y_only ? 1 : 0;
     if (wl)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R, true>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, nyc, ld, S.nY,
-                           th, r, d_W, f.B);
+                           th, yo, r, d_W, f.B);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_gather<R, false>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, nyc, ld,
-                           S.nY, th, r, d_W, f.B);
+                           S.nY, th, yo, r, d_W, f.B);
     // stage 1: all columns, Y columns with shift 0, X columns with shift s_i (only the Y
     // columns when the caller reads only z's Y part)
     if (y_only) vcycle(st, 0, T, nyc, d_sig + T, d_sidx + T);
     else vcycle(st, 0, 0, ncol, d_sig, d_sidx);
-    // stage 2 (X columns): apply K, then the shifted V-cycle again
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, ncol, ld, f.X,
-                       d_Z);
+    // stage 2 (X columns): apply K, then the shifted V-cycle again (with y_only the
+    // scatter reads the Y columns of stage 1 in place)
     if (!y_only) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, ncol, ld,
+                           f.X, d_Z);
         with_passes(T, [&](auto np) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R, decltype(np)::value>), dim3(grid_for(nf, 4)), dim3(256), 0,
                                st, nf, f.ptr, f.col, f.K, f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
@@ -844,12 +849,13 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(static_cast<int64_t>(nf) * T, 256)), dim3(256),
                            0, st, nf, T, ld, f.X, d_Z);
     }
+    const R* Zs = y_only ? f.X : d_Z;
     if (wl)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R, true>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, S.Nt,
-                           S.mesh.m(), ld, S.nY, th, d_Z, d_W, d_MtInvH, r, z, part);
+                           S.mesh.m(), ld, S.nY, th, yo, Zs, d_W, d_MtInvH, r, z, part);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R, false>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, S.Nt,
-                           S.mesh.m(), ld, S.nY, th, d_Z, d_W, d_MtInvH, r, z, part);
+                           S.mesh.m(), ld, S.nY, th, yo, Zs, d_W, d_MtInvH, r, z, part);
 }
 
 template <typename R>

@@ -66,8 +66,8 @@ struct MGDevice {
     // the block pointers of every level are offset by c0 columns (c0 = T: the Y columns only)
     void smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bool zero_start);
     void vcycle(hipStream_t st, int l, int c0, int nc, const R* sig, const int* sidx);
-    // y_only: the caller reads only z's Y part (r's X part is zero), so the second
-    // V-cycle over the X columns is skipped and z's X part is not the preconditioned value
+    // y_only: the caller reads only z's Y part, so r's X part is not read, only the Y
+    // columns are smoothed, z's X part is not written and the dot partials cover z_Y . r_Y
     void apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts,
                bool y_only = false);
     int tile_rows() const;
