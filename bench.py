@@ -360,7 +360,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-g", action="store_true", help="skip the G-estimator block (untimed, outside `value`)")
     ap.add_argument("--profile", action="store_true",
-                    help="one assembly, MINRES solve, two mixed and two FP64 estimates, 20 SpMVs and a 1 GiB "
+                    help="one assembly, MINRES solve, two mixed and two FP64 estimates (slab solve), 20 SpMVs and a 1 GiB "
                          "device copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -389,9 +389,9 @@ def main():
         # each estimate twice: a system's first call in a precision also times its tile
         # candidates, the second runs the chosen tiles only
         for _ in range(2):
-            sysm.estimate(apf.APF_FP32_REFINE, 2, per_cell=False)
+            sysm.estimate(apf.APF_FP32_REFINE, 2, per_cell=False, slab=apf.APF_SLAB_SOLVE)
         for _ in range(2):
-            sysm.estimate(apf.APF_FP64, 0, per_cell=False)
+            sysm.estimate(apf.APF_FP64, 0, per_cell=False, slab=apf.APF_SLAB_SOLVE)
         ms = sysm.bench_spmv(20)
         gbs = ctx.bench_stream(1 << 30, 3)
         print(json.dumps({"spmv_ms": ms, "spmv_algorithmic_bytes": 12 * sp.nnz + 20 * sp.n, "nnz": sp.nnz,

@@ -463,6 +463,7 @@ __global__ void __launch_bounds__(256) k_ay_apply(int nf, int m, const int* __re
     double acc[NT];
 #pragma unroll
     for (int l = 0; l < NT; ++l) acc[l] = 0.0;
+#pragma unroll 4
     for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
         const double kh = Ksp[e] * h;
         const int c = scol[e];
@@ -476,6 +477,54 @@ __global__ void __launch_bounds__(256) k_ay_apply(int nf, int m, const int* __re
     }
 #pragma unroll
     for (int l = 0; l < NT; ++l) y[(static_cast<int64_t>(k) * NT + l) * nf + s] = acc[l];
+}
+
+// r0 = f - B^T u on the lambda rows of a tensor mesh without reading K, with the
+// block partials of |r0|^2.  The B^T entries are those k_fill_kron writes:
+//   B^T((k,l,s),(tn = k p + it, s')) = Msp(s,s') Dt[it][l] + (Ksp(s,s') h_k) Mt[it][l],
+// X unknown (tn, s') at nY + tn nf + s'.  One thread per (interval k, free node s).
+template <int NT>
+__global__ void __launch_bounds__(256) k_bt_residual(int nf, int m, int64_t nY, const int* __restrict__ sptr,
+                                                     const int* __restrict__ scol, const double* __restrict__ Ksp,
+                                                     const double* __restrict__ Msp, const double* __restrict__ times,
+                                                     TimeTables tt, const double* __restrict__ b,
+                                                     const double* __restrict__ x, double* __restrict__ r0,
+                                                     double* __restrict__ part)
+{
+    constexpr int P = NT - 1;
+    double dot = 0.0;
+    const int64_t total = static_cast<int64_t>(m) * nf;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int k = static_cast<int>(i / nf), s = static_cast<int>(i - static_cast<int64_t>(k) * nf);
+        const double h = times[k + 1] - times[k];
+        const double* __restrict__ u = x + nY + static_cast<int64_t>(k) * P * nf;
+        double acc[NT];
+#pragma unroll
+        for (int l = 0; l < NT; ++l) acc[l] = 0.0;
+#pragma unroll 4
+        for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
+            const double kh = Ksp[e] * h, ms = Msp[e];
+            const int c = scol[e];
+            double uv[NT];
+#pragma unroll
+            for (int it = 0; it < NT; ++it) uv[it] = u[static_cast<int64_t>(it) * nf + c];
+#pragma unroll
+            for (int l = 0; l < NT; ++l)
+#pragma unroll
+                for (int it = 0; it < NT; ++it)
+                    acc[l] += (ms * tt.Dt[it * NT + l] + kh * tt.Mt[it * NT + l]) * uv[it];
+        }
+#pragma unroll
+        for (int l = 0; l < NT; ++l) {
+            const int64_t o = (static_cast<int64_t>(k) * NT + l) * nf + s;
+            const double rv = b[o] - acc[l];
+            r0[o] = rv;
+            dot += rv * rv;
+        }
+    }
+    const double bs = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // ---- conjugate gradients of the slab reconstruction with device-side scalars
@@ -517,6 +566,7 @@ __global__ void __launch_bounds__(256) k_ay_cg(int nf, int m, const int* __restr
         double acc[NT];
 #pragma unroll
         for (int l = 0; l < NT; ++l) acc[l] = 0.0;
+#pragma unroll 4
         for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
             const double kh = Ksp[e] * h;
             const int c = scol[e];
@@ -644,20 +694,34 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     if (!sys->H) throw std::runtime_error("slab reconstruction needs the nested coarse meshes of the multigrid");
     mg_prepare(sys, false);
     hipStream_t st = sys->stream;
-    const int64_t n = S.n, nY = S.nY;
+    const int64_t nY = S.nY;
     const int nparts = sys->nparts;
-    double* t = sys->d_vec[2];
     double* y = sys->d_vec[3];
     double* r = sys->d_vec[0];
     double* z = sys->d_vec[1];
     // r0 = f - B^T u: the lambda rows of b - K [0; u] (the only product that reads K); every
     // later product is A_y alone, applied matrix-free from the spatial stiffness
     double* r0 = sys->d_vec[5];
-    HIP_TRY(hipMemsetAsync(t, 0, nY * sizeof(double), st));
-    HIP_TRY(hipMemcpyAsync(t + nY, sys->d_x + nY, (n - nY) * sizeof(double), hipMemcpyDeviceToDevice, st));
-    sys->spmv_rows(t, y, nY);
-    HIP_TRY(hipMemcpyAsync(r0, sys->d_b, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_axpby, dim3(nparts), dim3(256), 0, st, nY, -1.0, y, 1.0, r0);
+    if (!sys->d_cg) sys->d_cg = dalloc<double>(sys, CG_N);
+    double* cg = sys->d_cg;
+    const int m = S.mesh.m();
+    // matrix-free passes over the (interval, free node) pairs: one item per thread, as
+    // many blocks as the partial buffer holds
+    const int64_t items = static_cast<int64_t>(m) * S.nf;
+    const int gi = static_cast<int>(std::min<int64_t>((items + 255) / 256, sys->part_cap));
+    {
+        const dim3 g(static_cast<unsigned>(gi)), bl(256);
+        switch (S.Nt) {
+        case 2: hipLaunchKernelGGL(k_bt_residual<2>, g, bl, 0, st, S.nf, m, nY, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
+        case 3: hipLaunchKernelGGL(k_bt_residual<3>, g, bl, 0, st, S.nf, m, nY, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
+        case 4: hipLaunchKernelGGL(k_bt_residual<4>, g, bl, 0, st, S.nf, m, nY, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
+        default: hipLaunchKernelGGL(k_bt_residual<5>, g, bl, 0, st, S.nf, m, nY, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
+        }
+    }
+    hipLaunchKernelGGL(k_cg_rr, dim3(1), dim3(256), 0, st, cg, sys->d_part, gi);
+    double g2 = 0.0;  // |r0|^2
+    HIP_TRY(hipMemcpyAsync(&g2, cg + CG_RR, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     auto resid2 = [&](const double* Ay) {
         hipLaunchKernelGGL(k_residual_parts, dim3(nparts), dim3(256), 0, st, nY, r0, Ay, sys->d_part);
         hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
@@ -666,22 +730,17 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
         HIP_TRY(hipStreamSynchronize(st));
         return h[0];
     };
-    HIP_TRY(hipMemsetAsync(y, 0, nY * sizeof(double), st));
-    const double g2 = resid2(y);  // |r0|^2
     HIP_TRY(hipMemcpyAsync(lt, sys->d_x, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
     double* pd = sys->d_vec[4];  // search direction
-    if (!sys->d_cg) sys->d_cg = dalloc<double>(sys, CG_N);
-    double* cg = sys->d_cg;
     const int tiles = sys->mg64->n_partials(S.nf);
     if (tiles > sys->part_cap) throw std::runtime_error("too many multigrid transpose tiles");
-    const int m = S.mesh.m();
-    const dim3 grid(static_cast<unsigned>(nparts)), block(256);
+    const dim3 grid(static_cast<unsigned>(nparts)), block(256), gitems(static_cast<unsigned>(gi));
     auto ay_cg = [&](int first) {
         switch (S.Nt) {
-        case 2: hipLaunchKernelGGL(k_ay_cg<2>, grid, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
-        case 3: hipLaunchKernelGGL(k_ay_cg<3>, grid, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
-        case 4: hipLaunchKernelGGL(k_ay_cg<4>, grid, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
-        default: hipLaunchKernelGGL(k_ay_cg<5>, grid, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        case 2: hipLaunchKernelGGL(k_ay_cg<2>, gitems, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        case 3: hipLaunchKernelGGL(k_ay_cg<3>, gitems, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        case 4: hipLaunchKernelGGL(k_ay_cg<4>, gitems, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        default: hipLaunchKernelGGL(k_ay_cg<5>, gitems, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
         }
     };
     const double scale = g2 > 0.0 ? g2 : 1.0;
@@ -710,7 +769,7 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
             sys->mg64->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true);  // z . r partials per tile
             hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, st, cg, sys->d_part, tiles, k == 0 ? 1 : 0);
             ay_cg(k == 0 ? 1 : 0);
-            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, cg, sys->d_part, nparts);
+            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, cg, sys->d_part, gi);
             hipLaunchKernelGGL(k_cg_update, grid, block, 0, st, nY, pd, y, lt, r, cg, sys->d_part);
             hipLaunchKernelGGL(k_cg_rr, dim3(1), dim3(256), 0, st, cg, sys->d_part, nparts);
             double h[CG_N];
