@@ -306,7 +306,8 @@ __device__ inline R tail_csr_row(int i, const int* __restrict__ ptr, const int* 
 }
 
 // Chebyshev-Jacobi smoothing of one level in LDS (as MGDevice::smooth); leaves
-// r = b - A x in r
+// r = b - A x in r.  (Keeping each thread's row in registers for a phase's three
+// passes, and 256-thread workgroups, were both measured slower at 2+1D p=2.)
 template <typename R>
 __device__ void tail_smooth(const MGTailLevel<R>& L, R s, R* sm, R* r, R* d, R* dn, int nu, bool zero_start)
 {
@@ -579,6 +580,9 @@ void MGDevice<R>::build(System* sys, const MGSetupHost& H, int nyc_)
         Level& d = levels[l];
         d.nf = h.nf;
         d.lmax = h.lmax;
+        d.nnz = static_cast<int64_t>(h.K.size());
+        d.Pnnz = static_cast<int64_t>(h.P.val.size());
+        d.Rnnz = static_cast<int64_t>(h.R.val.size());
         d.ptr = mg_alloc<int>(sys, h.sptr.size());
         up(d.ptr, h.sptr);
         d.col = mg_alloc<int>(sys, h.scol.size());
@@ -631,6 +635,9 @@ void MGDevice<R>::build(System* sys, const MGSetupHost& H, int nyc_)
     up(d_zidx, zidx);
     nc_coarse = H.levels.back().nf;
     blk_total = H.blk_total;
+    cinv_size = static_cast<int64_t>(H.coarse_inv.size());
+    W_size = static_cast<int64_t>(H.W.size());
+    MtInvH_size = static_cast<int64_t>(H.MtInvH.size());
     d_cinv = mg_alloc<R>(sys, H.coarse_inv.size());
     up(d_cinv, conv<R>(H.coarse_inv));
     d_comp_start = mg_alloc<int>(sys, H.comp_start.size());
@@ -735,6 +742,52 @@ void MGDevice<R>::refresh(const MGSetupHost& H)
     // the smoothing bounds may have changed: rewrite the tail descriptors (d_tail is
     // already allocated, so no System is needed)
     if (d_tail) setup_tail(nullptr);
+}
+
+template <typename S, typename R>
+__global__ void k_convert(int64_t n, const S* __restrict__ a, R* __restrict__ b)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        b[i] = static_cast<R>(a[i]);
+}
+
+template <typename R>
+template <typename S>
+void MGDevice<R>::refresh_from(const MGDevice<S>& src, hipStream_t st)
+{
+    if (!ready || !src.ready || src.levels.size() != levels.size() || src.ncol != ncol || src.blk_total != blk_total ||
+        src.cinv_size != cinv_size)
+        throw std::runtime_error("multigrid hierarchies of different structure");
+    auto cv = [&](int64_t n, const S* a, R* b) {
+        if (n > 0)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_convert<S, R>), dim3(static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 1024))),
+                               dim3(256), 0, st, n, a, b);
+    };
+    for (size_t l = 0; l < levels.size(); ++l) {
+        Level& d = levels[l];
+        const auto& e = src.levels[l];
+        if (e.nf != d.nf || e.nnz != d.nnz || e.Pnnz != d.Pnnz || e.Rnnz != d.Rnnz)
+            throw std::runtime_error("multigrid hierarchies of different structure");
+        d.lmax = e.lmax;
+        cv(d.nnz, e.K, d.K);
+        cv(d.nnz, e.M, d.M);
+        cv(d.nf, e.dK, d.dK);
+        cv(d.nf, e.dM, d.dM);
+        cv(d.Pnnz, e.Pval, d.Pval);
+        cv(d.Rnnz, e.Rval, d.Rval);
+    }
+    cv(ncol, src.d_sig, d_sig);
+    cv(cinv_size, src.d_cinv, d_cinv);
+    if (W_size) HIP_TRY(hipMemcpyAsync(d_W, src.d_W, W_size * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (MtInvH_size)
+        HIP_TRY(hipMemcpyAsync(d_MtInvH, src.d_MtInvH, MtInvH_size * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipGetLastError());
+    // the smoothing coefficients follow the new bounds (d_tail is already allocated)
+    if (d_tail) {
+        HIP_TRY(hipStreamSynchronize(st));
+        setup_tail(nullptr);
+    }
 }
 
 template <typename R>
@@ -877,5 +930,6 @@ int MGDevice<R>::n_partials(int nf) const
 
 template struct MGDevice<double>;
 template struct MGDevice<float>;
+template void MGDevice<float>::refresh_from<double>(const MGDevice<double>&, hipStream_t);
 
 }  // namespace apf

@@ -27,6 +27,7 @@ struct MGDevice {
         R* Rr = nullptr;  // residual
         R* D = nullptr;   // Chebyshev direction
         R* Tm = nullptr;  // second Chebyshev direction (the fused step alternates D and Tm)
+        int64_t nnz = 0, Pnnz = 0, Rnnz = 0;  // value counts of K / M, P, R
         int Pn = 0, Rn = 0;
         int* Pptr = nullptr;
         int* Pcol = nullptr;
@@ -60,8 +61,14 @@ struct MGDevice {
     size_t tail_lds = 0;
     void* d_tail = nullptr;  // [L - tail_from] level descriptors on the device
 
+    int64_t cinv_size = 0, W_size = 0, MtInvH_size = 0;
+
     void build(System* sys, const MGSetupHost& H, int nyc);
     void refresh(const MGSetupHost& H);
+    // the values of another precision's hierarchy of the same space (built alike), converted
+    // on the device: no host setup
+    template <typename S>
+    void refresh_from(const MGDevice<S>& src, hipStream_t st);
     void setup_tail(System* sys);
     // the block pointers of every level are offset by c0 columns (c0 = T: the Y columns only)
     void smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bool zero_start);
