@@ -1503,6 +1503,188 @@ __device__ inline void tile_product(const MT* mrow, int np, const BT* Bs, int ld
     }
 }
 
+// ---- full-pivot retry of patches that miss the acceptance
+// (solve_dense_local_error_scalar_constraint_system_2d, dense_local_error_blocks_2d.hpp:1014-1048:
+// after a failed PartialPivLU solve the reference refactors with FullPivLU at threshold
+// 1e-12 and solves again, and throws if the residual still fails.)
+
+// slots q (processing order) whose patch missed the acceptance (or all, with force)
+__global__ void k_fullpiv_list(int n_patches, const int* __restrict__ order, const double* __restrict__ patch_res,
+                               int force, int* __restrict__ list, int* __restrict__ count)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n_patches; q += gridDim.x * blockDim.x) {
+        const double r = patch_res[order[q]];
+        if (force || !(r <= 1e-10)) list[atomicAdd(count, 1)] = q;
+    }
+}
+
+constexpr int FP_THREADS = 1024;
+
+// block-wide argmax of (|v|, index): NaN wins, ties go to the lower index
+__device__ inline void argmax_merge(double& v, int& i, double v2, int i2)
+{
+    const bool n1 = v != v, n2 = v2 != v2;
+    bool take;
+    if (n1 || n2) take = n2 && (!n1 || i2 < i);
+    else take = v2 > v || (v2 == v && i2 < i);
+    if (take) {
+        v = v2;
+        i = i2;
+    }
+}
+
+__device__ inline void block_argmax(double& v, int& i, double* sv, int* si)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off, 64);
+        const int i2 = __shfl_xor(i, off, 64);
+        argmax_merge(v, i, v2, i2);
+    }
+    const int w = threadIdx.x / 64, nw = blockDim.x / 64;
+    __syncthreads();
+    if (threadIdx.x % 64 == 0) {
+        sv[w] = v;
+        si[w] = i;
+    }
+    __syncthreads();
+    v = sv[0];
+    i = si[0];
+    for (int k = 1; k < nw; ++k) argmax_merge(v, i, sv[k], si[k]);
+}
+
+// One workgroup per listed patch: Gauss-Jordan elimination with complete pivoting of
+// [K | B] (the patch's FP64 matrix of order n and all its slabs x modes right-hand
+// sides) in a global workspace; a pivot at most 1e-12 times the first one ends the
+// elimination (rank r), and the unknowns of the remaining columns are zero, as the
+// rank-revealing solve returns them.  Then the residual of the original K gives the
+// acceptance statistic per (slab, vertex) problem (written to patch_res), and the flux
+// coefficients go to sig as in k_patch_solve.
+__global__ void __launch_bounds__(FP_THREADS)
+    k_patch_fullpiv(EstParams P, PatchTopo topo, PatchSlots ps, const int* __restrict__ list, const double* Kd,
+                    const double* RHS, double* ws, int64_t ws_stride, int* perm_ws, int perm_stride,
+                    double* __restrict__ sig, double* __restrict__ patch_res)
+{
+    __shared__ double sv[FP_THREADS / 64];
+    __shared__ int si[FP_THREADS / 64];
+    __shared__ int s_rank;
+    const int q = list[blockIdx.x];
+    const int a = topo.order[q];
+    const int n = ps.n[q], np = ps.np[q];
+    const int Nt = P.Nt, Nf = P.Nf;
+    const int ncols = P.m * Nt;
+    const int ld = n + ncols;
+    const double* Kq = Kd + ps.off2[q];
+    const double* Bq = RHS + ps.off1[q] * ncols;
+    double* W = ws + static_cast<int64_t>(blockIdx.x) * ws_stride;  // [n][ld]
+    double* X = W + static_cast<int64_t>(n) * ld;                     // [n][ncols]
+    unsigned long long* colmax = reinterpret_cast<unsigned long long*>(X + static_cast<int64_t>(n) * ncols);
+    int* perm = perm_ws + static_cast<int64_t>(blockIdx.x) * perm_stride;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int64_t e = tid; e < static_cast<int64_t>(n) * ld; e += nt) {
+        const int i = static_cast<int>(e / ld), j = static_cast<int>(e % ld);
+        W[e] = (j < n) ? Kq[static_cast<int64_t>(i) * np + j] : Bq[static_cast<int64_t>(i) * ncols + (j - n)];
+    }
+    for (int j = tid; j < n; j += nt) perm[j] = j;
+    for (int c = tid; c < 2 * ncols; c += nt) colmax[c] = 0ull;
+    if (tid == 0) s_rank = n;
+    __syncthreads();
+    double first = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const int m = n - k;
+        double bv = -1.0;
+        int bi = 0x7fffffff;
+        for (int64_t e = tid; e < static_cast<int64_t>(m) * m; e += nt) {
+            const int i = k + static_cast<int>(e / m), j = k + static_cast<int>(e % m);
+            argmax_merge(bv, bi, fabs(W[static_cast<int64_t>(i) * ld + j]), i * n + j);
+        }
+        block_argmax(bv, bi, sv, si);
+        if (k == 0) first = bv;
+        if (!(bv > 1e-12 * first)) {
+            if (tid == 0) s_rank = k;
+            break;
+        }
+        const int pr = bi / n, pc = bi % n;
+        // rows k <-> pr over all columns, columns k <-> pc over all rows
+        if (pr != k)
+            for (int j = tid; j < ld; j += nt) {
+                const double t = W[static_cast<int64_t>(k) * ld + j];
+                W[static_cast<int64_t>(k) * ld + j] = W[static_cast<int64_t>(pr) * ld + j];
+                W[static_cast<int64_t>(pr) * ld + j] = t;
+            }
+        __syncthreads();
+        if (pc != k) {
+            for (int i = tid; i < n; i += nt) {
+                const double t = W[static_cast<int64_t>(i) * ld + k];
+                W[static_cast<int64_t>(i) * ld + k] = W[static_cast<int64_t>(i) * ld + pc];
+                W[static_cast<int64_t>(i) * ld + pc] = t;
+            }
+            if (tid == 0) {
+                const int t = perm[k];
+                perm[k] = perm[pc];
+                perm[pc] = t;
+            }
+        }
+        __syncthreads();
+        const double piv = W[static_cast<int64_t>(k) * ld + k];
+        __syncthreads();
+        for (int j = k + 1 + tid; j < ld; j += nt) W[static_cast<int64_t>(k) * ld + j] /= piv;
+        __syncthreads();
+        // eliminate column k from every other row (columns > k; column k is not read again)
+        const int w = ld - k - 1;
+        for (int64_t e = tid; e < static_cast<int64_t>(n) * w; e += nt) {
+            const int i = static_cast<int>(e / w), j = k + 1 + static_cast<int>(e % w);
+            if (i == k) continue;
+            W[static_cast<int64_t>(i) * ld + j] -= W[static_cast<int64_t>(i) * ld + k] * W[static_cast<int64_t>(k) * ld + j];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    const int rank = s_rank;
+    // x[perm[i]] = (i < rank) ? W[i][n + c] : 0
+    for (int64_t e = tid; e < static_cast<int64_t>(n) * ncols; e += nt) {
+        const int i = static_cast<int>(e / ncols), c = static_cast<int>(e % ncols);
+        X[static_cast<int64_t>(perm[i]) * ncols + c] = (i < rank) ? W[static_cast<int64_t>(i) * ld + n + c] : 0.0;
+    }
+    __syncthreads();
+    // residual b - K x of the original matrix, |r| and |b| maxima per column
+    for (int64_t e = tid; e < static_cast<int64_t>(n) * ncols; e += nt) {
+        const int i = static_cast<int>(e / ncols), c = static_cast<int>(e % ncols);
+        const double b = Bq[static_cast<int64_t>(i) * ncols + c];
+        double r = b;
+        for (int j = 0; j < n; ++j) r -= Kq[static_cast<int64_t>(i) * np + j] * X[static_cast<int64_t>(j) * ncols + c];
+        const double ar = (r != r) ? INFINITY : fabs(r);
+        const double ab = (b != b) ? INFINITY : fabs(b);
+        atomicMax(&colmax[c], static_cast<unsigned long long>(__double_as_longlong(ar)));
+        atomicMax(&colmax[ncols + c], static_cast<unsigned long long>(__double_as_longlong(ab)));
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double worst = 0.0;
+        for (int s0 = 0; s0 < ncols; s0 += Nt) {
+            double rm = 0.0, bm = 0.0;
+            for (int c = s0; c < s0 + Nt; ++c) {
+                rm = fmax(rm, __longlong_as_double(static_cast<long long>(colmax[c])));
+                bm = fmax(bm, __longlong_as_double(static_cast<long long>(colmax[ncols + c])));
+            }
+            const double qv = (bm < INFINITY) ? rm / fmax(1.0, bm) : INFINITY;
+            worst = max_nonfinite_wins(worst, qv);
+        }
+        patch_res[a] = worst;
+    }
+    // flux coefficients of the patch's cells (as k_patch_solve writes them)
+    const int cb = topo.cell_ptr[a], nc = topo.cell_ptr[a + 1] - cb;
+    for (int64_t e = tid; e < static_cast<int64_t>(nc) * Nf * ncols; e += nt) {
+        const int c = static_cast<int>(e % ncols);
+        const int row = static_cast<int>(e / ncols);
+        const int ci = row / Nf, i = row - ci * Nf;
+        const int fi = topo.fmap[static_cast<int64_t>(cb + ci) * Nf + i];
+        const double sg = topo.fsign[static_cast<int64_t>(cb + ci) * Nf + i];
+        const int64_t sc = topo.cells[cb + ci];
+        const int64_t off = ((sc * P.NV + topo.lv[cb + ci]) * Nf + i) * ncols;
+        sig[off + c] = (fi >= 0) ? sg * X[static_cast<int64_t>(fi) * ncols + c] : 0.0;
+    }
+}
+
 // Phase 3, one workgroup per (patch, block of 16 CT columns holding whole
 // slabs): X = Ki B on MFMA in FT, then refine_steps rounds of R = B - Kd X
 // (FP64 MFMA) and X += Ki R (FT); the residual of the final X gives the
@@ -2532,8 +2714,12 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         hipEvent_t e2, e3;
         HIP_TRY(hipEventCreate(&e2));
         HIP_TRY(hipEventCreate(&e3));
-        HIP_TRY(hipEventRecord(e2, st));
-        {
+        double h[6];
+        int extra_rounds = 0;
+        float ms2 = 0.f;
+        // diagnostics, sums and the acceptance maximum (run again after a full-pivot retry)
+        auto finish = [&]() {
+            HIP_TRY(hipEventRecord(e2, st));
             auto diag = [&](auto kern) {
                 hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(P.ns)), dim3(128), 0, st, P, Tb, sys->d_verts,
                                    sys->d_cells, sys->d_Mcell, sys->d_cell_free, sys->d_times, sys->d_x, B.lt, sys->d_x,
@@ -2549,26 +2735,63 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
                 case 23: diag(k_diag<2, 3>); break;
                 default: throw std::runtime_error("estimator diagnostics: unsupported (dim, p)");
             }
-        }
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(e3, st));
-        const int nparts = 256;
-        hipLaunchKernelGGL(k_sum5, dim3(nparts), dim3(256), 0, st, ncell, B.flux, B.resid, B.recon, B.lam2, B.trace,
-                           sys->d_part);
-        hipLaunchKernelGGL(k_sum5_final, dim3(1), dim3(64), 0, st, sys->d_part, nparts, B.sums);
-        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, st, S.patches.n_patches, B.patch_res, B.sums + 5);
-        HIP_TRY(hipGetLastError());
-        double h[6];
-        int extra_rounds = 0;
-        HIP_TRY(hipMemcpyAsync(h, B.sums, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(&extra_rounds, B.extra_rounds, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        float ms1 = 0.f, ms2 = 0.f;
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(e3, st));
+            const int nparts = 256;
+            hipLaunchKernelGGL(k_sum5, dim3(nparts), dim3(256), 0, st, ncell, B.flux, B.resid, B.recon, B.lam2, B.trace,
+                               sys->d_part);
+            hipLaunchKernelGGL(k_sum5_final, dim3(1), dim3(64), 0, st, sys->d_part, nparts, B.sums);
+            hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, st, S.patches.n_patches, B.patch_res, B.sums + 5);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(h, B.sums, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(&extra_rounds, B.extra_rounds, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            HIP_TRY(hipEventElapsedTime(&ms2, e2, e3));
+        };
+        finish();
+        float ms1 = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms1, sys->ev0, sys->ev1));
         HIP_TRY(hipEventElapsedTime(&sys->t_factor_ms, sys->ev0, sys->evp[1]));
         HIP_TRY(hipEventElapsedTime(&sys->t_rhs_ms, sys->evp[1], sys->evp[2]));
         HIP_TRY(hipEventElapsedTime(&sys->t_solve_ms, sys->evp[2], sys->ev1));
-        HIP_TRY(hipEventElapsedTime(&ms2, e2, e3));
+        // patches that missed the acceptance: solve again with complete pivoting
+        // (APF_PATCH_FULLPIV=1 sends every patch through this path)
+        bool force = false;
+        if (const char* e = std::getenv("APF_PATCH_FULLPIV")) force = std::atoi(e) != 0;
+        int fullpiv = 0;
+        double fullpiv_s = 0.0;
+        if (force || !(h[5] <= 1e-10)) {
+            const auto t_fp = std::chrono::steady_clock::now();
+            const int npat = S.patches.n_patches;
+            if (!B.fp_list) {
+                B.fp_list = dalloc<int>(sys, static_cast<size_t>(npat) + 1);
+                B.fp_count = dalloc<int>(sys, 1);
+            }
+            HIP_TRY(hipMemsetAsync(B.fp_count, 0, sizeof(int), st));
+            hipLaunchKernelGGL(k_fullpiv_list, dim3(static_cast<unsigned>(std::min(1024, (npat + 255) / 256 + 1))),
+                               dim3(256), 0, st, npat, B.order, B.patch_res, force ? 1 : 0, B.fp_list, B.fp_count);
+            HIP_TRY(hipMemcpyAsync(&fullpiv, B.fp_count, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (fullpiv > 0) {
+                const int ncols = P.m * P.Nt, nm = B.nmax;
+                const int64_t per = static_cast<int64_t>(nm) * (nm + ncols) + static_cast<int64_t>(nm) * ncols +
+                                    2 * static_cast<int64_t>(ncols);
+                const int batch = static_cast<int>(
+                    std::max<int64_t>(1, std::min<int64_t>(fullpiv, (int64_t(1) << 29) / (per * 8))));
+                if (static_cast<int64_t>(batch) * per > B.fp_ws_doubles) {
+                    B.fp_ws = dalloc<double>(sys, static_cast<size_t>(batch) * per);
+                    B.fp_perm = dalloc<int>(sys, static_cast<size_t>(batch) * nm);
+                    B.fp_ws_doubles = static_cast<int64_t>(batch) * per;
+                }
+                for (int b0 = 0; b0 < fullpiv; b0 += batch)
+                    hipLaunchKernelGGL(k_patch_fullpiv, dim3(static_cast<unsigned>(std::min(batch, fullpiv - b0))),
+                                       dim3(FP_THREADS), 0, st, P, topo, ps, B.fp_list + b0, B.Kd, B.RHS, B.fp_ws, per,
+                                       B.fp_perm, nm, B.sig, B.patch_res);
+                HIP_TRY(hipGetLastError());
+                finish();
+            }
+            fullpiv_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_fp).count();
+        }
         (void)hipEventDestroy(e2);
         (void)hipEventDestroy(e3);
         sys->t_patch_ms = ms1;
@@ -2582,7 +2805,8 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         out->y_est2 = h[0] + h[2];
         out->max_patch_residual = h[5];
         out->extra_refinements = extra_rounds;
-        out->seconds_patches = ms1 * 1e-3;
+        out->fullpiv_patches = fullpiv;
+        out->seconds_patches = ms1 * 1e-3 + fullpiv_s;
         out->seconds_diagnostics = ms2 * 1e-3;
         if (flux_by_cell) HIP_TRY(hipMemcpy(flux_by_cell, B.flux, ncell * sizeof(double), hipMemcpyDeviceToHost));
         if (resid_by_cell) HIP_TRY(hipMemcpy(resid_by_cell, B.resid, ncell * sizeof(double), hipMemcpyDeviceToHost));

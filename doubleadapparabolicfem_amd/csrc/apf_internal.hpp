@@ -185,6 +185,12 @@ struct EstimatorBuffers {
     int64_t* big_off = nullptr;
     double* big_ws = nullptr;
     int big_count = 0, big_npmax = 0;
+    // full-pivot retry of patches that miss the acceptance: slot list, count, workspace
+    int* fp_list = nullptr;
+    int* fp_count = nullptr;
+    double* fp_ws = nullptr;
+    int* fp_perm = nullptr;
+    int64_t fp_ws_doubles = 0;
 };
 
 struct System {

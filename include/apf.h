@@ -131,6 +131,8 @@ typedef struct {
     double slab_relative_residual;           /* |f - B^T u - A_y lt| / |f - B^T u| (-1: not solved) */
     double seconds_slab;
     int slab_stop;                           /* enum apf_slab_stop; APF_SLAB_STOP_NONE when not solved */
+    int fullpiv_patches;                     /* patches solved again with complete pivoting after missing the
+                                                acceptance (dense_local_error_blocks_2d.hpp:1014-1048) */
 } apf_estimate_out;
 
 /* ---- host-side integer work (no GPU needed) ---- */

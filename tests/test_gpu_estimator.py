@@ -61,6 +61,39 @@ def test_estimator_matches_oracle(ctx, dim, p, outer, prec):
     assert est["recon2"] <= 1e-25
 
 
+@pytest.mark.parametrize("dim,p,outer", [(1, 2, 2), (2, 1, 1), (2, 2, 1), (2, 3, 1)])
+def test_full_pivot_retry_matches_oracle(ctx, dim, p, outer, monkeypatch):
+    """The retry of patches that miss the acceptance (FP64 Gauss-Jordan with complete
+    pivoting, rank threshold 1e-12, as the reference's FullPivLU retry), forced on every
+    patch: the same values as the oracle to 1e-10 and every patch accepted.  Without the
+    switch no patch takes the retry."""
+    m = meshes.uniform_mesh(dim, outer)
+    coeff = meshes.smooth_initial(dim)
+    if dim == 2:
+        coeff = apf.Coefficients(meshes.checkerboard_kappa(m, 4, seed=3), 0.0, coeff.u0)
+    print(f"full-pivot retry dim={dim} p={p} outer={outer}", flush=True)
+    stm, od, prob = oracle_setup(m, p, coeff)
+    S = OA.assemble(stm, od, prob)
+    x = OE.main_solve(S)
+    ref = OE.estimate(stm, od, prob, S, x)
+    sp = apf.Space(m, p)
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(coeff)
+    sysm.set_solution(x)
+    plain = sysm.estimate(apf.APF_FP32_REFINE, 2)
+    assert plain["fullpiv_patches"] == 0
+    monkeypatch.setenv("APF_PATCH_FULLPIV", "1")
+    est = sysm.estimate(apf.APF_FP32_REFINE, 2)
+    assert est["fullpiv_patches"] == len(sp.patch_sizes()[0])
+    assert est["max_patch_residual"] <= 1e-10
+    for key, rv in (("flux2", ref.flux2), ("lambda_y2", ref.lambda_y2), ("trace2", ref.trace2),
+                    ("eta2", ref.eta2), ("y_est2", ref.y_est2)):
+        assert _rel(est[key], rv) <= 1e-10, (key, est[key], rv)
+    fc = est["flux_by_cell"]
+    assert np.all(np.abs(fc - ref.flux_by_cell) <= 1e-10 * np.abs(ref.flux_by_cell) + 1e-14 * ref.flux2)
+    assert est["div_residual_l2"] <= 1e-11
+
+
 @pytest.mark.parametrize("rec", [a for a in ANCHORS if (a["dim"], a["p"], a["outer"]) in
                                  {(1, 1, 3), (1, 2, 3), (1, 3, 2), (2, 1, 2), (2, 2, 2), (2, 3, 1)}],
                          ids=lambda a: f"{a['dim']}d-p{a['p']}-o{a['outer']}")
