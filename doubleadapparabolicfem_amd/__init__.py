@@ -147,6 +147,8 @@ EXPORTED = {
     "apf_bench_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_bench_store": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_bench_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "apf_bench_kron_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "apf_system_matvec": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "apf_estimate_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -431,6 +433,19 @@ class System:
         v = ctypes.c_double()
         _check(lib().apf_bench_spmv(self.handle, reps, ctypes.byref(v)), self.ctx.handle)
         return v.value
+
+    def bench_kron_apply(self, reps=20):
+        v = ctypes.c_double()
+        _check(lib().apf_bench_kron_apply(self.handle, reps, ctypes.byref(v)), self.ctx.handle)
+        return v.value
+
+    def matvec(self, x, matrix_free=False):
+        """y = K x: the assembled CSR K, or the same entries from K's Kronecker factors"""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check(lib().apf_system_matvec(self.handle, _ptr(x), _ptr(y), 1 if matrix_free else 0), self.ctx.handle,
+               "apf_system_matvec")
+        return y
 
     def mg_setup_seconds(self):
         v = ctypes.c_double()

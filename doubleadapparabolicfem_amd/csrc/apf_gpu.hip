@@ -527,6 +527,85 @@ __global__ void __launch_bounds__(256) k_bt_residual(int nf, int m, int64_t nY, 
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
+// y = K x on a tensor mesh without reading K: each entry is formed as k_fill_kron
+// writes it, Msp(s,s') aM + (Ksp(s,s') h_k) aK with the time-table factors
+//   A_y((k,l,s),(k,l',s'))        : aK = Mt[l][l'],  aM = 0
+//   B^T((k,l,s),(k P + it, s'))    : aK = Mt[it][l],  aM = Dt[it][l]   (B the transpose)
+//   C((0,s),(0,s'))                : aM = -1
+// One thread per (interval k, free node s) for the lambda rows, then one per (time node
+// tn, s) for the X rows (the one or two intervals at tn, and the t = 0 trace block).
+template <int NT>
+__global__ void __launch_bounds__(256) k_kron_apply(int nf, int m, int64_t nY, const int* __restrict__ sptr,
+                                                    const int* __restrict__ scol, const double* __restrict__ Ksp,
+                                                    const double* __restrict__ Msp, const double* __restrict__ times,
+                                                    TimeTables tt, const double* __restrict__ x,
+                                                    double* __restrict__ y)
+{
+    constexpr int P = NT - 1;
+    const int64_t nl = static_cast<int64_t>(m) * nf;
+    const int64_t nx = (static_cast<int64_t>(m) * P + 1) * nf;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nl) {
+        const int k = static_cast<int>(i / nf), s = static_cast<int>(i - static_cast<int64_t>(k) * nf);
+        const double h = times[k + 1] - times[k];
+        const double* __restrict__ xy = x + static_cast<int64_t>(k) * NT * nf;
+        const double* __restrict__ xx = x + nY + static_cast<int64_t>(k) * P * nf;
+        double acc[NT];
+#pragma unroll
+        for (int l = 0; l < NT; ++l) acc[l] = 0.0;
+#pragma unroll 2
+        for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
+            const double kh = Ksp[e] * h, ms = Msp[e];
+            const int c = scol[e];
+            double yv[NT], xv[NT];
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                yv[q] = xy[static_cast<int64_t>(q) * nf + c];
+                xv[q] = xx[static_cast<int64_t>(q) * nf + c];
+            }
+#pragma unroll
+            for (int l = 0; l < NT; ++l) {
+#pragma unroll
+                for (int q = 0; q < NT; ++q) acc[l] += (kh * tt.Mt[l * NT + q]) * yv[q];
+#pragma unroll
+                for (int q = 0; q < NT; ++q) acc[l] += (ms * tt.Dt[q * NT + l] + kh * tt.Mt[q * NT + l]) * xv[q];
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < NT; ++l) y[(static_cast<int64_t>(k) * NT + l) * nf + s] = acc[l];
+    } else if (i < nl + nx) {
+        const int64_t j = i - nl;
+        const int tn = static_cast<int>(j / nf), s = static_cast<int>(j - static_cast<int64_t>(tn) * nf);
+        double acc = 0.0;
+        // the intervals at time node tn, with tn's index it inside each
+        int ks[2], its[2], nk = 0;
+        if (tn % P == 0) {
+            if (tn / P - 1 >= 0) { ks[nk] = tn / P - 1; its[nk] = P; ++nk; }
+            if (tn / P < m) { ks[nk] = tn / P; its[nk] = 0; ++nk; }
+        } else {
+            ks[0] = tn / P;
+            its[0] = tn % P;
+            nk = 1;
+        }
+        for (int b = 0; b < nk; ++b) {
+            const int k = ks[b], it = its[b];
+            const double h = times[k + 1] - times[k];
+            const double* __restrict__ xy = x + static_cast<int64_t>(k) * NT * nf;
+#pragma unroll 2
+            for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
+                const double kh = Ksp[e] * h, ms = Msp[e];
+                const int c = scol[e];
+#pragma unroll
+                for (int l = 0; l < NT; ++l)
+                    acc += (ms * tt.Dt[it * NT + l] + kh * tt.Mt[it * NT + l]) * xy[static_cast<int64_t>(l) * nf + c];
+            }
+        }
+        if (tn == 0)
+            for (int e = sptr[s]; e < sptr[s + 1]; ++e) acc += -Msp[e] * x[nY + scol[e]];
+        y[nY + j] = acc;
+    }
+}
+
 // ---- conjugate gradients of the slab reconstruction with device-side scalars
 // cg[]: rz (current z . r), rz_old, pap, alpha, beta, rr, bad (breakdown flag)
 enum { CG_RZ, CG_RZ_OLD, CG_PAP, CG_ALPHA, CG_BETA, CG_RR, CG_BAD, CG_N };
@@ -802,6 +881,20 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
 }
 
 void System::spmv(const double* x, double* y) { spmv_rows(x, y, space->n); }
+
+void System::kron_apply(const double* x, double* y)
+{
+    const Space& S = *space;
+    const int m = S.mesh.m();
+    const int64_t threads = static_cast<int64_t>(m) * S.nf + (static_cast<int64_t>(m) * S.p + 1) * S.nf;
+    const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);
+    switch (S.Nt) {
+    case 2: hipLaunchKernelGGL(k_kron_apply<2>, grid, block, 0, stream, S.nf, m, S.nY, d_sptr, d_scol, d_Ksp, d_Msp, d_times, tt, x, y); break;
+    case 3: hipLaunchKernelGGL(k_kron_apply<3>, grid, block, 0, stream, S.nf, m, S.nY, d_sptr, d_scol, d_Ksp, d_Msp, d_times, tt, x, y); break;
+    case 4: hipLaunchKernelGGL(k_kron_apply<4>, grid, block, 0, stream, S.nf, m, S.nY, d_sptr, d_scol, d_Ksp, d_Msp, d_times, tt, x, y); break;
+    default: hipLaunchKernelGGL(k_kron_apply<5>, grid, block, 0, stream, S.nf, m, S.nY, d_sptr, d_scol, d_Ksp, d_Msp, d_times, tt, x, y); break;
+    }
+}
 
 void System::ay_apply(const double* x, double* y)
 {
@@ -1401,9 +1494,15 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
         // One MINRES iteration; the buffer rotations have period 3, so three
         // iterations form a fixed launch sequence that is captured once in a
         // hipGraph and replayed (no host synchronisation inside).
+        // The Lanczos products use K matrix-free from its Kronecker factors (the same entries;
+        // APF_MINRES_MATFREE=0: the CSR product); the true residuals that decide acceptance
+        // use the assembled K.
+        bool matfree = true;
+        if (const char* e = std::getenv("APF_MINRES_MATFREE")) matfree = std::atoi(e) != 0;
         auto iteration = [&]() {
             // t = K y ; v = y / beta ; t = t / beta - (beta / oldb) r1 ; alfa = v . t
-            sys->spmv(y, t);
+            if (matfree) sys->kron_apply(y, t);
+            else sys->spmv(y, t);
             hipLaunchKernelGGL(k_minres_vec1, dim3(vgrid), dim3(block), 0, st, n, sys->d_scal, y, t, r1, v,
                                sys->d_part);
             hipLaunchKernelGGL(k_minres_alfa, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, nparts);
@@ -1503,6 +1602,28 @@ int apf_space_add_coarse_mesh(apf_space* s, const apf_mesh_desc* md)
     }
 }
 
+int apf_system_matvec(apf_system* sys, const double* x, double* y, int matrix_free)
+{
+    if (!sys || !x || !y) return set_error(nullptr, APF_BAD_ARG, "apf_system_matvec: null argument");
+    if (!sys->assembled) return set_error(sys->ctx, APF_BAD_ARG, "apf_system_matvec: assemble first");
+    try {
+        HIP_TRY(hipSetDevice(sys->ctx->device));
+        const int64_t n = sys->space->n;
+        hipStream_t st = sys->stream;
+        double* dx = sys->d_vec[2];
+        double* dy = sys->d_vec[3];
+        HIP_TRY(hipMemcpyAsync(dx, x, n * sizeof(double), hipMemcpyHostToDevice, st));
+        if (matrix_free) sys->kron_apply(dx, dy);
+        else sys->spmv(dx, dy);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(y, dy, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_system_matvec: ") + e.what());
+    }
+}
+
 int apf_bench_spmv(apf_system* sys, int reps, double* ms_avg)
 {
     if (!sys || !ms_avg || reps < 1) return set_error(nullptr, APF_BAD_ARG, "apf_bench_spmv: bad argument");
@@ -1521,6 +1642,27 @@ int apf_bench_spmv(apf_system* sys, int reps, double* ms_avg)
         return APF_OK;
     } catch (const HipFailure& e) {
         return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_bench_spmv: ") + e.what());
+    }
+}
+
+int apf_bench_kron_apply(apf_system* sys, int reps, double* ms_avg)
+{
+    if (!sys || !ms_avg || reps < 1) return set_error(nullptr, APF_BAD_ARG, "apf_bench_kron_apply: bad argument");
+    if (!sys->assembled) return set_error(sys->ctx, APF_BAD_ARG, "apf_bench_kron_apply: assemble first");
+    try {
+        HIP_TRY(hipSetDevice(sys->ctx->device));
+        hipStream_t st = sys->stream;
+        sys->kron_apply(sys->d_x, sys->d_vec[2]);  // warm
+        HIP_TRY(hipEventRecord(sys->ev0, st));
+        for (int r = 0; r < reps; ++r) sys->kron_apply(sys->d_x, sys->d_vec[2 + (r & 1)]);
+        HIP_TRY(hipEventRecord(sys->ev1, st));
+        HIP_TRY(hipEventSynchronize(sys->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+        *ms_avg = static_cast<double>(ms) / reps;
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_bench_kron_apply: ") + e.what());
     }
 }
 

@@ -274,6 +274,7 @@ struct System {
     void spmv_rows(const double* x, double* y, int64_t rows);
     // y[0:nY) = A_y x[0:nY) without reading K (tensor mesh: K_sp (x) h_k Mt per slab)
     void ay_apply(const double* x, double* y);
+    void kron_apply(const double* x, double* y);  // y = K x from the Kronecker factors
 };
 
 void estimator_alloc(System* sys);

@@ -188,6 +188,14 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
 /* Average device time of `reps` back-to-back SpMVs y = K x (HIP events on the
  * system stream), for the HBM roofline of the MINRES inner loop. */
 int apf_bench_spmv(apf_system* sys, int reps, double* ms_avg);
+/* y = K x for host arrays x, y [n]: the assembled CSR K (matrix_free = 0), or the same
+ * entries formed from K's Kronecker factors (spatial K, M and the time tables) on the
+ * device (matrix_free = 1: the product of the MINRES iterations; the true residuals that
+ * decide acceptance use the CSR K).  Replaces the Eigen sparse product K * x
+ * (eigen_backend/sparse_matrix.hpp) on the hot path's operator. */
+int apf_system_matvec(apf_system* sys, const double* x, double* y, int matrix_free);
+/* Average device time of `reps` matrix-free products (as apf_bench_spmv). */
+int apf_bench_kron_apply(apf_system* sys, int reps, double* ms_avg);
 /* Device copy bandwidth (read + write bytes / s, in GB/s) of a plain copy kernel
  * over `bytes` bytes, `reps` launches: the measured HBM ceiling and the
  * calibration kernel for counter-based traffic. */

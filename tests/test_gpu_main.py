@@ -141,3 +141,42 @@ def test_device_pattern_equals_host_pattern(ctx, name):
     rp_h, col_h = sp.pattern()
     assert np.array_equal(rp_d, rp_h)
     assert np.array_equal(col_d, col_h)
+
+
+@pytest.mark.parametrize("name", ["uj-p1", "uj-p3", "uj-p4", "interval-p2", "fan-graded-p2", "batch-p1"])
+def test_matrix_free_product_equals_csr(ctx, name):
+    """y = K x from K's Kronecker factors (the product of the MINRES iterations) against
+    the assembled CSR K on random x: every entry within 1e-14 of (|K| |x|)_i."""
+    import scipy.sparse as sps
+    m, p = _pattern_case(name)
+    print(f"matrix-free product {name}", flush=True)
+    sp = apf.Space(m, p)
+    sysm = apf.System(ctx, sp)
+    coeff = meshes.smooth_initial(m.dim)
+    if m.dim == 2:
+        coeff = apf.Coefficients(meshes.checkerboard_kappa(m, 4, seed=5), 0.0, coeff.u0)
+    sysm.assemble(coeff)
+    vals, _ = sysm.download()
+    rowptr, col = sp.pattern()
+    Kabs = sps.csr_matrix((np.abs(vals), col, rowptr), shape=(sp.n, sp.n))
+    x = np.random.default_rng(3).standard_normal(sp.n)
+    y_csr = sysm.matvec(x, matrix_free=False)
+    y_mf = sysm.matvec(x, matrix_free=True)
+    bound = Kabs @ np.abs(x)
+    assert np.all(np.abs(y_mf - y_csr) <= 1e-14 * bound + 1e-300), np.max(np.abs(y_mf - y_csr) / (bound + 1e-300))
+
+
+def test_minres_matrix_free_and_csr_products_agree(ctx, monkeypatch):
+    """MINRES with the matrix-free Lanczos products and with the CSR products: both
+    accepted on the assembled K's true residual, the same iterations (to the check
+    interval) and solutions within 1e-8."""
+    m = meshes.uniform_mesh(2, 3)
+    sp = apf.Space(m, 2, coarse=meshes.uniform_hierarchy(2, 3))
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(meshes.smooth_initial(2))
+    x_mf, s_mf = sysm.minres(tol=1e-10, max_it=600, check_interval=10)
+    monkeypatch.setenv("APF_MINRES_MATFREE", "0")
+    x_csr, s_csr = sysm.minres(tol=1e-10, max_it=600, check_interval=10)
+    assert s_mf["converged"] and s_csr["converged"]
+    assert abs(s_mf["iterations"] - s_csr["iterations"]) <= 10
+    assert np.linalg.norm(x_mf - x_csr) <= 1e-8 * np.linalg.norm(x_csr)
