@@ -370,7 +370,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-g", action="store_true", help="skip the G-estimator block (untimed, outside `value`)")
     ap.add_argument("--profile", action="store_true",
-                    help="one assembly, MINRES solve, two mixed and two FP64 estimates (slab solve), 20 SpMVs and a 1 GiB "
+                    help="one assembly, MINRES solve, two mixed and two FP64 estimates (slab solve), 20 launches each of "
+                         "the CSR SpMV, the matrix-free product and the multigrid step, and a 1 GiB "
                          "device copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -403,6 +404,8 @@ def main():
         for _ in range(2):
             sysm.estimate(apf.APF_FP64, 0, per_cell=False, slab=apf.APF_SLAB_SOLVE)
         ms = sysm.bench_spmv(20)
+        sysm.bench_kron_apply(20)
+        sysm.bench_mg_step(20)
         gbs = ctx.bench_stream(1 << 30, 3)
         print(json.dumps({"spmv_ms": ms, "spmv_algorithmic_bytes": 12 * sp.nnz + 20 * sp.n, "nnz": sp.nnz,
                           "n": sp.n, "copy_kernel_bytes_read": 1 << 30, "copy_kernel_bytes_written": 1 << 30,
@@ -448,11 +451,21 @@ def main():
     ph64 = sysm.phase_timings()
     roof = phase_rooflines(sp, ph_mixed, 2, True)
     roof64 = phase_rooflines(sp, ph64, 0, False)
-    # HBM roofline of the MINRES SpMV: algorithmic bytes 12 nnz + 20 n
+    # HBM roofline of the CSR SpMV (MINRES true residuals, the general path's operator):
+    # algorithmic bytes 12 nnz + 20 n
     spmv_ms = sysm.bench_spmv(50)
     stream_gbs = ctx.bench_stream(1 << 30, 10)
     spmv_bytes = 12 * sp.nnz + 20 * sp.n
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    # the MINRES Lanczos product, matrix-free: x read and y written once, the spatial
+    # pattern (int column, K and M values) and row pointer
+    kron_ms = sysm.bench_kron_apply(50)
+    nnz_sp, nf_sp = sp.nnz_spatial, sp.nf
+    kron_bytes = 16 * sp.n + 20 * nnz_sp + 4 * nf_sp
+    kron_gbs = kron_bytes / (kron_ms * 1e-3) / 1e9
+    # the dominant kernel of a step: the multigrid's fine-level Chebyshev step (FP64)
+    mg_ms, mg_bytes = sysm.bench_mg_step(50)
+    mg_gbs = mg_bytes / (mg_ms * 1e-3) / 1e9
     # assembly: the value array (8 nnz bytes) is its only large stream; a plain
     # store-only kernel over the same bytes gives the write ceiling beside the peak
     asm_gbs = 8 * sp.nnz / (t_asm * 1e-3) / 1e9
@@ -470,18 +483,31 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+f32",
-        "dtype_note": "FP64 assembly, MINRES (SpMV, Lanczos, inner products) and slab solve; patch inverses "
-                      "and products FP32 with FP64 residual refinement",
+        "dtype_note": "FP64 assembly, MINRES (matrix-free Lanczos products, CSR true residuals, multigrid, inner "
+                      "products) and slab solve (FP32 V-cycles in its preconditioner); patch inverses and products "
+                      "FP32 with FP64 residual refinement",
         "data": "synthetic",
         "config": {"workload": W["desc"], "p": W["p"], "n_X": sp.nX, "n_Y": sp.nY, "nnz_K": sp.nnz,
                    "space_time_cells": sp.n_space_cells * sp.n_intervals,
                    "vertex_patches": sp.n_patches * sp.n_intervals},
-        # the kernel that dominates a step: the MINRES SpMV (about 5 launches per iteration)
-        "roofline": {"bound": "hbm", "kernel": "k_spmv (MINRES CSR SpMV, FP64 values, int32 columns)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get("spmv"),
-                     "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms,
-                     "measured_copy_GBs": stream_gbs, "frac_of_measured_copy": achieved / stream_gbs},
+        # the kernel that dominates a step: the multigrid's fine-level Chebyshev step (16 launches
+        # per MINRES iteration over 161 or 65 columns, plus the slab solve's)
+        "roofline": {"bound": "hbm", "kernel": "k_mg_cheb_fused (fine-level Chebyshev step of the multigrid, "
+                                               "FP64, all columns)",
+                     "achieved": mg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": mg_gbs / HBM_PEAK_GBS,
+                     "traffic": traffic.get("mg_step"), "algorithmic_bytes_per_launch": mg_bytes,
+                     "avg_launch_ms": mg_ms, "measured_copy_GBs": stream_gbs,
+                     "frac_of_measured_copy": mg_gbs / stream_gbs},
+        "roofline_csr_spmv": {"bound": "hbm", "kernel": "k_spmv (CSR SpMV, FP64 values, int32 columns: MINRES "
+                                                        "true residuals, the caller's-CSR and general paths)",
+                              "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get("spmv"),
+                              "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms,
+                              "frac_of_measured_copy": achieved / stream_gbs},
+        "roofline_minres_product": {"bound": "hbm", "kernel": "k_kron_apply (K x from the Kronecker factors)",
+                                    "achieved": kron_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": kron_gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": kron_bytes,
+                                    "avg_launch_ms": kron_ms, "csr_spmv_ms": spmv_ms},
         "roofline_patch_phases_mixed": roof,
         "roofline_patch_phases_fp64": roof64,
         "roofline_assembly": {"bound": "hbm", "kernel": "assembly (k_spatial_elements + k_fill_kron + k_fill_rhs)",

@@ -148,6 +148,8 @@ EXPORTED = {
     "apf_bench_store": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_bench_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_bench_kron_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "apf_bench_mg_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]),
     "apf_system_matvec": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -438,6 +440,12 @@ class System:
         v = ctypes.c_double()
         _check(lib().apf_bench_kron_apply(self.handle, reps, ctypes.byref(v)), self.ctx.handle)
         return v.value
+
+    def bench_mg_step(self, reps=20):
+        """(ms per launch, algorithmic bytes per launch) of the fine-level Chebyshev step"""
+        v, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().apf_bench_mg_step(self.handle, reps, ctypes.byref(v), ctypes.byref(b)), self.ctx.handle)
+        return v.value, b.value
 
     def matvec(self, x, matrix_free=False):
         """y = K x: the assembled CSR K, or the same entries from K's Kronecker factors"""

@@ -236,6 +236,8 @@ struct System {
     MinresScalars* d_scal = nullptr;
     double* d_res = nullptr;
     double* d_cg = nullptr;  // scalars of the slab reconstruction's conjugate gradients
+    int2* d_kron_halo = nullptr;  // neighbour range of each block of the matrix-free product
+    int kron_span_cap = 0;
     double* d_kdiag = nullptr;
     double* d_band = nullptr;
     float* d_band32 = nullptr;
@@ -275,6 +277,7 @@ struct System {
     // y[0:nY) = A_y x[0:nY) without reading K (tensor mesh: K_sp (x) h_k Mt per slab)
     void ay_apply(const double* x, double* y);
     void kron_apply(const double* x, double* y);  // y = K x from the Kronecker factors
+    void kron_prepare();
 };
 
 void estimator_alloc(System* sys);

@@ -912,6 +912,38 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
 }
 
 template <typename R>
+void MGDevice<R>::bench_step(hipStream_t st, int reps, double* ms, double* bytes)
+{
+    Level& d = levels[0];
+    const int nc = ncol;
+    const unsigned rows_grid = grid_for(d.nf, 4);
+    auto launch = [&]() {
+        with_passes(nc, [&](auto np) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_cheb_fused<R, decltype(np)::value>), dim3(rows_grid), dim3(256), 0,
+                               st, d.nf, d.ptr, d.col, d.K, d.M, d_sig, nc, ld, d.D, d.X, d.Rr, d.Tm, d.dK, d.dM,
+                               static_cast<R>(0.5), static_cast<R>(0.5), 0, static_cast<const R*>(nullptr));
+        });
+    };
+    launch();  // warm
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) launch();
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = static_cast<double>(t) / reps;
+    // d, x, r read; x, r, dn written (nf x nc each); the level's pattern, K, M, diagonals
+    *bytes = 6.0 * sizeof(R) * static_cast<double>(d.nf) * nc +
+             static_cast<double>(d.nnz) * (sizeof(int) + 2 * sizeof(R)) +
+             static_cast<double>(d.nf) * (sizeof(int) + 2 * sizeof(R));
+}
+
+template <typename R>
 int MGDevice<R>::tile_rows() const
 {
     // rows per transpose tile: the LDS tile ncol x (th + 1) doubles stays within

@@ -77,6 +77,9 @@ struct MGDevice {
     // columns are smoothed, z's X part is not written and the dot partials cover z_Y . r_Y
     void apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts,
                bool y_only = false);
+    // `reps` launches of one fine-level Chebyshev step over all columns (timing only: it
+    // changes the level's work vectors); average ms and the algorithmic bytes of a launch
+    void bench_step(hipStream_t st, int reps, double* ms, double* bytes);
     int tile_rows() const;
     int n_partials(int nf) const;  // dot partials written by apply()
 };
