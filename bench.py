@@ -34,7 +34,8 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors a
 # plain copy kernel of known size reads) from the rocprofv3 --pmc runs of
 # `bench.py --profile`, summarised in profiles/.
-MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9, "solve": 2.49e9, "factor": 0.28e9}}
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a copy kernel)
+MEASURED_TRAFFIC = {"c2": {"spmv": 2.11e9, "mg_step": 1.799e8, "solve": 2.49e9, "factor": 0.28e9}}
 
 
 def workload(name):
