@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <limits>
@@ -640,6 +641,16 @@ __global__ void __launch_bounds__(KR) k_kron_apply(int nf, int m, int64_t nY, co
     }
 }
 
+// one thread copies n doubles to the pinned mailbox, then publishes the sequence number
+__global__ void k_post_mail(const double* __restrict__ src, int n, System::Mailbox* mb, unsigned long long seq)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i < n; ++i) mb->v[i] = src[i];
+    __threadfence_system();
+    mb->seq = seq;
+    __threadfence_system();
+}
+
 // ---- conjugate gradients of the slab reconstruction with device-side scalars
 // cg[]: rz (current z . r), rz_old, pap, alpha, beta, rr, bad (breakdown flag)
 enum { CG_RZ, CG_RZ_OLD, CG_PAP, CG_ALPHA, CG_BETA, CG_RR, CG_BAD, CG_N };
@@ -817,6 +828,11 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     // from a current FP64 one when there is one.
     bool mg32 = true;
     if (const char* e = std::getenv("APF_SLAB_MG")) mg32 = std::atoi(e) != 64;
+    // The slab block is a spatial inverse per column, whose V-cycle gains from more smoothing
+    // (C2: 6 CG iterations with 2 Chebyshev steps per smoothing, 3 with 4), while the MINRES
+    // preconditioner does not (45 iterations either way); APF_SLAB_MG_DEGREE sets it.
+    int sdeg = 4;
+    if (const char* e = std::getenv("APF_SLAB_MG_DEGREE")) sdeg = std::max(1, std::atoi(e));
     mg_prepare(sys, mg32);
     hipStream_t st = sys->stream;
     const int64_t nY = S.nY;
@@ -845,14 +861,12 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     }
     hipLaunchKernelGGL(k_cg_rr, dim3(1), dim3(256), 0, st, cg, sys->d_part, gi);
     double g2 = 0.0;  // |r0|^2
-    HIP_TRY(hipMemcpyAsync(&g2, cg + CG_RR, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    sys->post_and_wait(cg + CG_RR, 1, &g2);
     auto resid2 = [&](const double* Ay) {
         hipLaunchKernelGGL(k_residual_parts, dim3(nparts), dim3(256), 0, st, nY, r0, Ay, sys->d_part);
         hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
         double h[2];
-        HIP_TRY(hipMemcpyAsync(h, sys->d_res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
+        sys->post_and_wait(sys->d_res, 2, h);
         return h[0];
     };
     HIP_TRY(hipMemcpyAsync(lt, sys->d_x, nY * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -892,16 +906,15 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
         // one host read per step: |r|^2 and the breakdown flag
         for (int k = 0; it < max_it; ++k) {
             // z . r partials per tile
-            if (mg32) sys->mg32->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true);
-            else sys->mg64->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true);
+            if (mg32) sys->mg32->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true, sdeg);
+            else sys->mg64->apply(st, S, r, z, sys->d_part, nparts, /*y_only=*/true, sdeg);
             hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, st, cg, sys->d_part, tiles, k == 0 ? 1 : 0);
             ay_cg(k == 0 ? 1 : 0);
             hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, cg, sys->d_part, gi);
             hipLaunchKernelGGL(k_cg_update, grid, block, 0, st, nY, pd, y, lt, r, cg, sys->d_part);
             hipLaunchKernelGGL(k_cg_rr, dim3(1), dim3(256), 0, st, cg, sys->d_part, nparts);
             double h[CG_N];
-            HIP_TRY(hipMemcpyAsync(h, cg, CG_N * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
+            sys->post_and_wait(cg, CG_N, h);
             if (h[CG_BAD] != 0.0) break;  // no update was made
             ++it;
             // the first cycle runs to tol / 10; a restart only has to show a 10x gain, so that
@@ -912,6 +925,36 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     HIP_TRY(hipGetLastError());
     *its = it;
     *rel = rr;
+}
+
+void System::post_and_wait(const double* src, int n, double* out)
+{
+    if (n > 8) throw std::runtime_error("mailbox: at most 8 values");
+    if (!h_mail) {
+        void* p = nullptr;
+        HIP_TRY(hipHostMalloc(&p, sizeof(Mailbox), hipHostMallocCoherent | hipHostMallocMapped));
+        h_mail = static_cast<Mailbox*>(p);
+        h_mail->seq = 0;
+        mail_seq = 0;
+        void* dp = nullptr;
+        HIP_TRY(hipHostGetDevicePointer(&dp, p, 0));
+        d_mail = static_cast<Mailbox*>(dp);
+    }
+    const unsigned long long want = ++mail_seq;
+    hipLaunchKernelGGL(k_post_mail, dim3(1), dim3(64), 0, stream, src, n, d_mail, want);
+    HIP_TRY(hipGetLastError());
+    volatile unsigned long long* seq = &h_mail->seq;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (*seq != want) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+            // a slow or failed stream: synchronise (reports a device error), then read again
+            HIP_TRY(hipStreamSynchronize(stream));
+            if (*seq != want) throw std::runtime_error("mailbox: value not posted");
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    volatile double* v = h_mail->v;
+    for (int i = 0; i < n; ++i) out[i] = v[i];
 }
 
 void System::spmv(const double* x, double* y) { spmv_rows(x, y, space->n); }
@@ -1308,6 +1351,7 @@ void apf_system_destroy(apf_system* sys)
     (void)hipSetDevice(sys->ctx->device);
     (void)hipStreamSynchronize(sys->stream);
     for (void* p : sys->allocs) (void)hipFree(p);
+    if (sys->h_mail) (void)hipHostFree(sys->h_mail);
     if (sys->ev0) (void)hipEventDestroy(sys->ev0);
     if (sys->ev1) (void)hipEventDestroy(sys->ev1);
     for (auto& e : sys->evp)
@@ -1547,8 +1591,7 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
             hipLaunchKernelGGL(k_residual_parts, dim3(vgrid), dim3(block), 0, st, n, sys->d_b, t, sys->d_part);
             hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
             double h[2];
-            HIP_TRY(hipMemcpyAsync(h, sys->d_res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
+            sys->post_and_wait(sys->d_res, 2, h);
             rel = (h[1] > 0.0) ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
         };
         double rel = 0.0;

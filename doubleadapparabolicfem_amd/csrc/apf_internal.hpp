@@ -236,6 +236,17 @@ struct System {
     MinresScalars* d_scal = nullptr;
     double* d_res = nullptr;
     double* d_cg = nullptr;  // scalars of the slab reconstruction's conjugate gradients
+    // pinned host mailbox: a kernel posts a few doubles and a sequence number, the host
+    // polls it (cheaper than a device-to-host copy and a stream synchronisation)
+    struct Mailbox {
+        double v[8];
+        unsigned long long seq;
+    };
+    Mailbox* h_mail = nullptr;  // host address
+    Mailbox* d_mail = nullptr;  // the same memory as the device addresses it
+    unsigned long long mail_seq = 0;
+    // post n <= 8 doubles from device memory and wait for them on the host
+    void post_and_wait(const double* src, int n, double* out);
     int2* d_kron_halo = nullptr;  // neighbour range of each block of the matrix-free product
     int kron_span_cap = 0;
     double* d_kdiag = nullptr;

@@ -667,7 +667,7 @@ void MGDevice<R>::setup_tail(System* sys)
     tail_from = -1;
     if (const char* e = std::getenv("APF_MG_TAIL"))
         if (std::atoi(e) == 0) return;
-    if (nu > MG_TAIL_MAX_NU || L < 2) return;
+    if (L < 2) return;
     int from = -1;
     size_t bytes = 0;
     for (int l = 0; l + 1 < L; ++l) {
@@ -708,7 +708,8 @@ void MGDevice<R>::setup_tail(System* sys)
         const R sigma1 = theta / delta;
         t.inv_theta = static_cast<R>(1) / theta;
         R rho = 1 / sigma1;
-        for (int k = 0; k < nu; ++k) {
+        // the step coefficients do not depend on the degree (only the last step differs)
+        for (int k = 0; k < MG_TAIL_MAX_NU; ++k) {
             const R rho_n = 1 / (2 * sigma1 - rho);
             t.c1[k] = rho_n * rho;
             t.c2[k] = 2 * rho_n / delta;
@@ -791,7 +792,7 @@ void MGDevice<R>::refresh_from(const MGDevice<S>& src, hipStream_t st)
 }
 
 template <typename R>
-void MGDevice<R>::smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bool zero_start)
+void MGDevice<R>::smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bool zero_start, int deg)
 {
     Level& d = levels[l];
     R* X = d.X + c0;
@@ -818,13 +819,13 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bo
     R* dcur = d.D + c0;
     R* dnext = d.Tm + c0;
     R rho = 1 / sigma1;
-    for (int k = 0; k < nu; ++k) {
+    for (int k = 0; k < deg; ++k) {
         const R rho_n = 1 / (2 * sigma1 - rho);
         const R c1 = rho_n * rho, c2 = 2 * rho_n / delta;
         with_passes(nc, [&](auto np) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_cheb_fused<R, decltype(np)::value>), dim3(rows_grid), dim3(256), 0,
                                st, d.nf, d.ptr, d.col, d.K, d.M, sig, nc, ld, dcur, X, Rr, dnext, d.dK, d.dM, c1, c2,
-                               (k == nu - 1) ? 1 : 0, (zero_start && k == 0) ? B : nullptr);
+                               (k == deg - 1) ? 1 : 0, (zero_start && k == 0) ? B : nullptr);
         });
         std::swap(dcur, dnext);
         rho = rho_n;
@@ -833,13 +834,13 @@ void MGDevice<R>::smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bo
 
 // V-cycle on level l: B_l -> X_l (columns [c0, c0 + nc); sig and sidx already offset)
 template <typename R>
-void MGDevice<R>::vcycle(hipStream_t st, int l, int c0, int nc, const R* sig, const int* sidx)
+void MGDevice<R>::vcycle(hipStream_t st, int l, int c0, int nc, const R* sig, const int* sidx, int deg)
 {
     Level& d = levels[l];
     const int L = static_cast<int>(levels.size());
-    if (l == tail_from) {
+    if (l == tail_from && deg <= MG_TAIL_MAX_NU) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_tail<R>), dim3(nc), dim3(MG_TAIL_THREADS), tail_lds, st,
-                           static_cast<const MGTailLevel<R>*>(d_tail), L - tail_from, nu, ld, sig, sidx, d_cinv,
+                           static_cast<const MGTailLevel<R>*>(d_tail), L - tail_from, deg, ld, sig, sidx, d_cinv,
                            blk_total, d_comp_start, d_node_comp, d_blk_off, d.B + c0, d.X + c0,
                            static_cast<int>(tail_lds / sizeof(R)) - 3 * d.nf);
         return;
@@ -851,24 +852,25 @@ void MGDevice<R>::vcycle(hipStream_t st, int l, int c0, int nc, const R* sig, co
         return;
     }
     Level& c = levels[l + 1];
-    smooth(st, l, c0, nc, sig, true);  // leaves r = B - A x in Rr
+    smooth(st, l, c0, nc, sig, true, deg);  // leaves r = B - A x in Rr
     with_passes(nc, [&](auto np) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R, decltype(np)::value>), dim3(grid_for(d.Rn, 4)), dim3(256), 0, st,
                            d.Rn, d.Rptr, d.Rcol, d.Rval, nc, ld, d.Rr + c0, ld, c.B + c0, 0);
     });
-    vcycle(st, l + 1, c0, nc, sig, sidx);
+    vcycle(st, l + 1, c0, nc, sig, sidx, deg);
     with_passes(nc, [&](auto np) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_csr_mm<R, decltype(np)::value>), dim3(grid_for(d.Pn, 4)), dim3(256), 0, st,
                            d.Pn, d.Pptr, d.Pcol, d.Pval, nc, ld, c.X + c0, ld, d.X + c0, 1);
     });
-    smooth(st, l, c0, nc, sig, false);
+    smooth(st, l, c0, nc, sig, false, deg);
 }
 
 // z = P^-1 r for the saddle-point vector r = [r_Y; r_X]; writes dot(z, r) partials.
 template <typename R>
 void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts,
-                        bool y_only)
+                        bool y_only, int degree)
 {
+    const int deg = degree > 0 ? degree : nu;
     Level& f = levels[0];
     const int nf = f.nf;
     const int64_t nel = static_cast<int64_t>(nf) * ncol;
@@ -886,8 +888,8 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
                            S.nY, th, yo, r, d_W, f.B);
     // stage 1: all columns, Y columns with shift 0, X columns with shift s_i (only the Y
     // columns when the caller reads only z's Y part)
-    if (y_only) vcycle(st, 0, T, nyc, d_sig + T, d_sidx + T);
-    else vcycle(st, 0, 0, ncol, d_sig, d_sidx);
+    if (y_only) vcycle(st, 0, T, nyc, d_sig + T, d_sidx + T, deg);
+    else vcycle(st, 0, 0, ncol, d_sig, d_sidx, deg);
     // stage 2 (X columns): apply K, then the shifted V-cycle again (with y_only the
     // scatter reads the Y columns of stage 1 in place)
     if (!y_only) {
@@ -897,7 +899,7 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R, decltype(np)::value>), dim3(grid_for(nf, 4)), dim3(256), 0,
                                st, nf, f.ptr, f.col, f.K, f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
         });
-        vcycle(st, 0, 0, T, d_sig, d_sidx);
+        vcycle(st, 0, 0, T, d_sig, d_sidx, deg);
         // X columns from the second V-cycle, Y columns from the first
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(static_cast<int64_t>(nf) * T, 256)), dim3(256),
                            0, st, nf, T, ld, f.X, d_Z);

@@ -71,12 +71,13 @@ struct MGDevice {
     void refresh_from(const MGDevice<S>& src, hipStream_t st);
     void setup_tail(System* sys);
     // the block pointers of every level are offset by c0 columns (c0 = T: the Y columns only)
-    void smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bool zero_start);
-    void vcycle(hipStream_t st, int l, int c0, int nc, const R* sig, const int* sidx);
+    void smooth(hipStream_t st, int l, int c0, int nc, const R* sig, bool zero_start, int deg);
+    void vcycle(hipStream_t st, int l, int c0, int nc, const R* sig, const int* sidx, int deg);
     // y_only: the caller reads only z's Y part, so r's X part is not read, only the Y
     // columns are smoothed, z's X part is not written and the dot partials cover z_Y . r_Y
+    // degree: Chebyshev steps per smoothing (0: nu)
     void apply(hipStream_t st, const Space& S, const double* r, double* z, double* part, int nparts,
-               bool y_only = false);
+               bool y_only = false, int degree = 0);
     // `reps` launches of one fine-level Chebyshev step over all columns (timing only: it
     // changes the level's work vectors); average ms and the algorithmic bytes of a launch
     void bench_step(hipStream_t st, int reps, double* ms, double* bytes);
