@@ -90,7 +90,8 @@ class _EstimateOut(ctypes.Structure):
                 ("seconds_patches", ctypes.c_double), ("seconds_diagnostics", ctypes.c_double),
                 ("extra_refinements", ctypes.c_int), ("slab_iterations", ctypes.c_int),
                 ("slab_relative_residual", ctypes.c_double),
-                ("seconds_slab", ctypes.c_double), ("slab_stop", ctypes.c_int), ("fullpiv_patches", ctypes.c_int)]
+                ("seconds_slab", ctypes.c_double), ("slab_stop", ctypes.c_int), ("fullpiv_patches", ctypes.c_int),
+                ("seconds_total", ctypes.c_double)]
 
 
 class _GEstimateOut(ctypes.Structure):

@@ -1226,6 +1226,7 @@ int apf_general_estimate(apf_general_estimator* ge, const double* x, int prec, i
             tot.seconds_slab += o.seconds_slab;
             tot.extra_refinements += o.extra_refinements;
             tot.fullpiv_patches += o.fullpiv_patches;
+            tot.seconds_total += o.seconds_total;
             tot.slab_iterations = std::max(tot.slab_iterations, o.slab_iterations);
             tot.slab_relative_residual = std::max(tot.slab_relative_residual, o.slab_relative_residual);
             if (o.slab_stop != APF_SLAB_STOP_TOL) tot.slab_stop = o.slab_stop;

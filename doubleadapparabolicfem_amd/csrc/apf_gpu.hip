@@ -818,22 +818,35 @@ void mg_prepare(System* sys, bool mg32)
 // restarted one (from the true residual) when that has dropped 10x or reached
 // tol / 10; the solve stops when a restart gains less than 10% of the true
 // residual (the round-off floor).
+static bool slab_mg32()
+{
+    if (const char* e = std::getenv("APF_SLAB_MG")) return std::atoi(e) != 64;
+    return true;
+}
+
+void slab_prepare(System* sys)
+{
+    if (!sys->H) throw std::runtime_error("slab reconstruction needs the nested coarse meshes of the multigrid");
+    mg_prepare(sys, slab_mg32());
+    if (!sys->d_cg) sys->d_cg = dalloc<double>(sys, CG_N);
+    sys->ensure_mailbox();
+}
+
 void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel, int* stop)
 {
     const Space& S = *sys->space;
     if (!sys->H) throw std::runtime_error("slab reconstruction needs the nested coarse meshes of the multigrid");
+    slab_prepare(sys);
     // The preconditioner's V-cycles run in FP32 (APF_SLAB_MG=64: FP64): the CG and its
     // stopping test on the true residual stay FP64, so the precision changes only the speed
     // (C2: same 6 iterations, 2.87 -> 2.41 ms).  The FP32 hierarchy is converted on the device
     // from a current FP64 one when there is one.
-    bool mg32 = true;
-    if (const char* e = std::getenv("APF_SLAB_MG")) mg32 = std::atoi(e) != 64;
+    const bool mg32 = slab_mg32();
     // The slab block is a spatial inverse per column, whose V-cycle gains from more smoothing
     // (C2: 6 CG iterations with 2 Chebyshev steps per smoothing, 3 with 4), while the MINRES
     // preconditioner does not (45 iterations either way); APF_SLAB_MG_DEGREE sets it.
     int sdeg = 4;
     if (const char* e = std::getenv("APF_SLAB_MG_DEGREE")) sdeg = std::max(1, std::atoi(e));
-    mg_prepare(sys, mg32);
     hipStream_t st = sys->stream;
     const int64_t nY = S.nY;
     const int nparts = sys->nparts;
@@ -843,7 +856,6 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     // r0 = f - B^T u: the lambda rows of b - K [0; u] (the only product that reads K); every
     // later product is A_y alone, applied matrix-free from the spatial stiffness
     double* r0 = sys->d_vec[5];
-    if (!sys->d_cg) sys->d_cg = dalloc<double>(sys, CG_N);
     double* cg = sys->d_cg;
     const int m = S.mesh.m();
     // matrix-free passes over the (interval, free node) pairs: one item per thread, as
@@ -927,9 +939,8 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     *rel = rr;
 }
 
-void System::post_and_wait(const double* src, int n, double* out)
+void System::ensure_mailbox()
 {
-    if (n > 8) throw std::runtime_error("mailbox: at most 8 values");
     if (!h_mail) {
         void* p = nullptr;
         HIP_TRY(hipHostMalloc(&p, sizeof(Mailbox), hipHostMallocCoherent | hipHostMallocMapped));
@@ -940,6 +951,12 @@ void System::post_and_wait(const double* src, int n, double* out)
         HIP_TRY(hipHostGetDevicePointer(&dp, p, 0));
         d_mail = static_cast<Mailbox*>(dp);
     }
+}
+
+void System::post_and_wait(const double* src, int n, double* out)
+{
+    if (n > 8) throw std::runtime_error("mailbox: at most 8 values");
+    ensure_mailbox();
     const unsigned long long want = ++mail_seq;
     hipLaunchKernelGGL(k_post_mail, dim3(1), dim3(64), 0, stream, src, n, d_mail, want);
     HIP_TRY(hipGetLastError());
@@ -1253,6 +1270,7 @@ int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
         if (const char* e = std::getenv("APF_MINRES_GRAPH")) sys->use_graph = std::atoi(e) != 0;
         sys->H = s->H.coarse.empty() ? nullptr : &s->H;
         sys->stream = ctx->stream;
+        HIP_TRY(hipStreamCreate(&sys->stream2));  // default priority, as every stream here
         const Space& S = s->S;
         const RefTables& T = s->T;
         const int ns = S.mesh.ns(), d = S.dim, Ns = S.Ns;
@@ -1352,6 +1370,7 @@ void apf_system_destroy(apf_system* sys)
     (void)hipStreamSynchronize(sys->stream);
     for (void* p : sys->allocs) (void)hipFree(p);
     if (sys->h_mail) (void)hipHostFree(sys->h_mail);
+    if (sys->stream2) (void)hipStreamDestroy(sys->stream2);
     if (sys->ev0) (void)hipEventDestroy(sys->ev0);
     if (sys->ev1) (void)hipEventDestroy(sys->ev1);
     for (auto& e : sys->evp)

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <map>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -236,6 +237,8 @@ struct System {
     MinresScalars* d_scal = nullptr;
     double* d_res = nullptr;
     double* d_cg = nullptr;  // scalars of the slab reconstruction's conjugate gradients
+    hipStream_t stream2 = nullptr;  // second stream of the estimate (factor beside the slab solve)
+    std::set<int> est_done;         // precisions with a completed estimate (tiles chosen)
     // pinned host mailbox: a kernel posts a few doubles and a sequence number, the host
     // polls it (cheaper than a device-to-host copy and a stream synchronisation)
     struct Mailbox {
@@ -247,6 +250,7 @@ struct System {
     unsigned long long mail_seq = 0;
     // post n <= 8 doubles from device memory and wait for them on the host
     void post_and_wait(const double* src, int n, double* out);
+    void ensure_mailbox();
     int2* d_kron_halo = nullptr;  // neighbour range of each block of the matrix-free product
     int kron_span_cap = 0;
     double* d_kdiag = nullptr;
@@ -296,6 +300,8 @@ void mg_prepare(System* sys, bool mg32);
 // stop reason: APF_SLAB_STOP_TOL (relative residual <= tol), APF_SLAB_STOP_FLOOR (a restart
 // gained less than 10%: the FP64 round-off floor), APF_SLAB_STOP_MAX_IT
 void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its, double* rel, int* stop);
+// host set-up of the slab reconstruction (multigrid hierarchy, scalars, mailbox); idempotent
+void slab_prepare(System* sys);
 
 #define HIP_TRY(expr)                                                                    \
     do {                                                                                 \

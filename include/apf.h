@@ -133,6 +133,8 @@ typedef struct {
     int slab_stop;                           /* enum apf_slab_stop; APF_SLAB_STOP_NONE when not solved */
     int fullpiv_patches;                     /* patches solved again with complete pivoting after missing the
                                                 acceptance (dense_local_error_blocks_2d.hpp:1014-1048) */
+    double seconds_total;                    /* device span of the whole estimate: slab reconstruction,
+                                                patches and indicators, the phases possibly overlapping */
 } apf_estimate_out;
 
 /* ---- host-side integer work (no GPU needed) ---- */

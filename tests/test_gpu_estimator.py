@@ -236,3 +236,24 @@ def test_estimate_snapshot_in_the_reference_format(ctx, tmp_path):
     assert _rel(recon.sum(), est["recon2"]) < 1e-12
     assert _rel(total.sum(), est["y_est2"]) < 1e-12
     assert np.all(np.diff(total) <= 0)
+
+
+@pytest.mark.parametrize("prec", [apf.APF_FP64, apf.APF_FP32_REFINE], ids=["fp64", "fp32ref"])
+def test_overlapped_factor_and_slab_solve_match_sequential(ctx, prec, monkeypatch):
+    """The factor phase on a second stream beside the slab reconstruction (forced on for
+    both precisions) gives bit-identical estimates to the phases in turn."""
+    m = meshes.uniform_mesh(2, 3)
+    sp = apf.Space(m, 2, coarse=meshes.uniform_hierarchy(2, 3))
+    sysm = apf.System(ctx, sp)
+    sysm.assemble(meshes.smooth_initial(2))
+    sysm.minres(tol=1e-10, max_it=2000, check_interval=10)
+    sysm.estimate(prec, 2, slab=apf.APF_SLAB_SOLVE)  # first call: tiles and precisions chosen
+    monkeypatch.setenv("APF_EST_OVERLAP", "0")
+    seq = sysm.estimate(prec, 2, slab=apf.APF_SLAB_SOLVE)
+    monkeypatch.setenv("APF_EST_OVERLAP", "1")
+    ovl = sysm.estimate(prec, 2, slab=apf.APF_SLAB_SOLVE)
+    for key in ("flux2", "recon2", "eta2", "y_est2", "max_patch_residual"):
+        assert ovl[key] == seq[key], key
+    for key in ("flux_by_cell", "resid_by_cell", "recon_by_cell"):
+        assert np.array_equal(ovl[key], seq[key]), key
+    assert ovl["seconds_total"] > 0
