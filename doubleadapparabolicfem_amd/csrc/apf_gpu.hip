@@ -814,9 +814,9 @@ void mg_prepare(System* sys, bool mg32)
 // slabs).  Preconditioned conjugate gradients from lt = lambda, for all slabs
 // at once, with P_Y = K^-1 (x) (h_k Mt)^-1 (one symmetric multigrid V-cycle per
 // spatial inverse), until the true residual |f - B^T u - A_y lt| <= tol |f - B^T u|.
-// The first CG cycle ends when its recursive residual reaches tol / 10, a
+// The first CG cycle ends when its recursive residual reaches tol / 2, a
 // restarted one (from the true residual) when that has dropped 10x or reached
-// tol / 10; the solve stops when a restart gains less than 10% of the true
+// tol / 2; the solve stops when a restart gains less than 10% of the true
 // residual (the round-off floor).
 static bool slab_mg32()
 {
@@ -842,6 +842,12 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     // (C2: same 6 iterations, 2.87 -> 2.41 ms).  The FP32 hierarchy is converted on the device
     // from a current FP64 one when there is one.
     const bool mg32 = slab_mg32();
+    // The first CG cycle runs until its recursive residual reaches tol / 2 (APF_SLAB_FIRST sets
+    // the fraction).  Below the true residual's round-off floor (about 9e-13 relative at 2+1D
+    // p=2) a smaller target only adds iterations: tol / 10 took 4 and tol / 2 takes 3, both
+    // ending at a true residual of 9e-13.
+    double first_frac = 0.5;
+    if (const char* e = std::getenv("APF_SLAB_FIRST")) first_frac = std::atof(e);
     // The slab block is a spatial inverse per column, whose V-cycle gains from more smoothing
     // (C2: 6 CG iterations with 2 Chebyshev steps per smoothing, 3 with 4), while the MINRES
     // preconditioner does not (45 iterations either way); APF_SLAB_MG_DEGREE sets it.
@@ -929,9 +935,9 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
             sys->post_and_wait(cg, CG_N, h);
             if (h[CG_BAD] != 0.0) break;  // no update was made
             ++it;
-            // the first cycle runs to tol / 10; a restart only has to show a 10x gain, so that
+            // the first cycle runs to first_frac * tol; a restart only has to show a 10x gain, so that
             // a solve at its round-off floor stops after a few iterations
-            if (std::sqrt(h[CG_RR] / scale) <= std::max(0.1 * tol, cycle > 0 ? 0.1 * rr : 0.0)) break;
+            if (std::sqrt(h[CG_RR] / scale) <= std::max(first_frac * tol, cycle > 0 ? 0.1 * rr : 0.0)) break;
         }
     }
     HIP_TRY(hipGetLastError());
