@@ -149,7 +149,7 @@ EXPORTED = {
     "apf_bench_store": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_bench_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "apf_bench_kron_apply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
-    "apf_bench_mg_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+    "apf_bench_mg_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]),
     "apf_system_matvec": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "apf_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_EstimateOut),
@@ -442,10 +442,10 @@ class System:
         _check(lib().apf_bench_kron_apply(self.handle, reps, ctypes.byref(v)), self.ctx.handle)
         return v.value
 
-    def bench_mg_step(self, reps=20):
+    def bench_mg_step(self, reps=20, prec=APF_FP64):
         """(ms per launch, algorithmic bytes per launch) of the fine-level Chebyshev step"""
         v, b = ctypes.c_double(), ctypes.c_double()
-        _check(lib().apf_bench_mg_step(self.handle, reps, ctypes.byref(v), ctypes.byref(b)), self.ctx.handle)
+        _check(lib().apf_bench_mg_step(self.handle, reps, prec, ctypes.byref(v), ctypes.byref(b)), self.ctx.handle)
         return v.value, b.value
 
     def matvec(self, x, matrix_free=False):

@@ -1799,15 +1799,17 @@ int apf_bench_kron_apply(apf_system* sys, int reps, double* ms_avg)
     }
 }
 
-int apf_bench_mg_step(apf_system* sys, int reps, double* ms_avg, double* bytes)
+int apf_bench_mg_step(apf_system* sys, int reps, int prec, double* ms_avg, double* bytes)
 {
     if (!sys || !ms_avg || !bytes || reps < 1) return set_error(nullptr, APF_BAD_ARG, "apf_bench_mg_step: bad argument");
     if (!sys->assembled) return set_error(sys->ctx, APF_BAD_ARG, "apf_bench_mg_step: assemble first");
     if (!sys->H) return set_error(sys->ctx, APF_BAD_ARG, "apf_bench_mg_step: no coarse meshes");
     try {
         HIP_TRY(hipSetDevice(sys->ctx->device));
-        mg_prepare(sys, false);
-        sys->mg64->bench_step(sys->stream, reps, ms_avg, bytes);
+        const bool mg32 = prec == APF_FP32_REFINE;
+        mg_prepare(sys, mg32);
+        if (mg32) sys->mg32->bench_step(sys->stream, reps, ms_avg, bytes);
+        else sys->mg64->bench_step(sys->stream, reps, ms_avg, bytes);
         return APF_OK;
     } catch (const HipFailure& e) {
         return set_error(sys->ctx, APF_HIP_ERROR, std::string("apf_bench_mg_step: ") + e.what());

@@ -199,9 +199,10 @@ int apf_system_matvec(apf_system* sys, const double* x, double* y, int matrix_fr
 /* Average device time of `reps` matrix-free products (as apf_bench_spmv). */
 int apf_bench_kron_apply(apf_system* sys, int reps, double* ms_avg);
 /* Average device time of `reps` launches of the multigrid's fine-level Chebyshev step
- * over all columns (the FP64 preconditioner's dominant kernel), and its algorithmic
- * bytes per launch.  Timing only: it overwrites the multigrid's work vectors. */
-int apf_bench_mg_step(apf_system* sys, int reps, double* ms_avg, double* bytes);
+ * over all columns (the preconditioner's dominant kernel) in precision prec (APF_FP64 or
+ * APF_FP32_REFINE: FP32 V-cycles), and its algorithmic bytes per launch.  Timing only: it
+ * overwrites the multigrid's work vectors. */
+int apf_bench_mg_step(apf_system* sys, int reps, int prec, double* ms_avg, double* bytes);
 /* Device copy bandwidth (read + write bytes / s, in GB/s) of a plain copy kernel
  * over `bytes` bytes, `reps` launches: the measured HBM ceiling and the
  * calibration kernel for counter-based traffic. */
