@@ -539,9 +539,10 @@ __global__ void __launch_bounds__(256) k_bt_residual(int nf, int m, int64_t nY, 
 // the neighbour range [lo_b, hi_b] of its rows (precomputed); the range of NT time
 // planes at a time is staged in LDS with coalesced loads and gathered from there.
 // Blocks whose range exceeds the LDS tile gather from global memory instead.
-// (At 2+1D p=2 this takes 0.235 ms against 0.256 ms with global gathers only; also
-// staging the rows' pattern once for 8 jobs per block was slower, 0.288 ms: the larger
-// tile halves the resident blocks.)
+// The row loops issue four entries' loads together (0.235 -> 0.169 ms at 2+1D p=2;
+// eight entries need 119 VGPRs and took 0.186 ms).  Global gathers only took 0.256 ms,
+// and staging the rows' pattern once for 8 jobs per block 0.288 ms (the larger tile
+// halves the resident blocks).
 constexpr int KR = 256;
 
 template <int NT>
@@ -582,6 +583,7 @@ __global__ void __launch_bounds__(KR) k_kron_apply(int nf, int m, int64_t nY, co
 #pragma unroll
         for (int l = 0; l < NT; ++l) acc[l] = 0.0;
         stage(xy, NT);
+#pragma unroll 4
         for (int e = eb; e < ee; ++e) {
             const double kh = Ksp[e] * h;
             const int c = scol[e];
@@ -594,6 +596,7 @@ __global__ void __launch_bounds__(KR) k_kron_apply(int nf, int m, int64_t nY, co
                 for (int q = 0; q < NT; ++q) acc[l] += (kh * tt.Mt[l * NT + q]) * v[q];
         }
         stage(xx, NT);
+#pragma unroll 4
         for (int e = eb; e < ee; ++e) {
             const double kh = Ksp[e] * h, ms = Msp[e];
             const int c = scol[e];
@@ -626,7 +629,8 @@ __global__ void __launch_bounds__(KR) k_kron_apply(int nf, int m, int64_t nY, co
             const double h = times[k + 1] - times[k];
             const double* xy = x + static_cast<int64_t>(k) * NT * nf;
             stage(xy, NT);
-            for (int e = eb; e < ee; ++e) {
+    #pragma unroll 4
+        for (int e = eb; e < ee; ++e) {
                 const double kh = Ksp[e] * h, ms = Msp[e];
                 const int c = scol[e];
 #pragma unroll
@@ -635,6 +639,7 @@ __global__ void __launch_bounds__(KR) k_kron_apply(int nf, int m, int64_t nY, co
         }
         if (tn == 0) {
             stage(x + nY, 1);
+#pragma unroll 4
             for (int e = eb; e < ee; ++e) acc += -Msp[e] * val(x + nY, 0, scol[e]);
         }
         if (row) y[nY + static_cast<int64_t>(tn) * nf + s] = acc;
