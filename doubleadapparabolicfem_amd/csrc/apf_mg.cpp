@@ -148,15 +148,15 @@ CSR transpose(const CSR& A)
 
 // C = R A P restricted to the coarse adjacency pattern (sptr, scol) of the
 // coarse space; the Galerkin product of nested Lagrange spaces lives there.
-std::vector<double> galerkin(const CSR& R, const std::vector<int>& fptr, const std::vector<int>& fcol,
-                             const std::vector<double>& A, const CSR& P, const std::vector<int>& cptr,
-                             const std::vector<int>& ccol)
+// Rows [i0, i1) of the Galerkin product R A P on the coarse pattern (cptr, ccol).
+void galerkin_rows(const CSR& R, const std::vector<int>& fptr, const std::vector<int>& fcol,
+                   const std::vector<double>& A, const CSR& P, const std::vector<int>& cptr,
+                   const std::vector<int>& ccol, int i0, int i1, std::vector<double>& out)
 {
     const int nc = R.nrows;
-    std::vector<double> out(ccol.size(), 0.0);
     std::vector<double> acc(nc, 0.0);
     std::vector<int> mark(nc, -1), touched;
-    for (int i = 0; i < nc; ++i) {
+    for (int i = i0; i < i1; ++i) {
         touched.clear();
         for (int e = R.ptr[i]; e < R.ptr[i + 1]; ++e) {
             const int r = R.col[e];
@@ -178,6 +178,25 @@ std::vector<double> galerkin(const CSR& R, const std::vector<int>& fptr, const s
                 throw std::runtime_error("setup_mg: Galerkin product leaves the coarse pattern (meshes not nested?)");
         }
     }
+}
+
+// R A P on the coarse pattern; the rows are split over threads (each row's sums run in
+// the same order as on one thread, so the result does not depend on the split)
+std::vector<double> galerkin(const CSR& R, const std::vector<int>& fptr, const std::vector<int>& fcol,
+                             const std::vector<double>& A, const CSR& P, const std::vector<int>& cptr,
+                             const std::vector<int>& ccol, int threads)
+{
+    const int nc = R.nrows;
+    std::vector<double> out(ccol.size(), 0.0);
+    const int nt = std::max(1, std::min(threads, nc / 256));
+    std::vector<std::future<void>> parts;
+    for (int t = 1; t < nt; ++t)
+        parts.push_back(std::async(std::launch::async, [&, t] {
+            galerkin_rows(R, fptr, fcol, A, P, cptr, ccol, static_cast<int>(static_cast<int64_t>(nc) * t / nt),
+                          static_cast<int>(static_cast<int64_t>(nc) * (t + 1) / nt), out);
+        }));
+    galerkin_rows(R, fptr, fcol, A, P, cptr, ccol, 0, static_cast<int>(static_cast<int64_t>(nc) / nt), out);
+    for (auto& f : parts) f.get();
     return out;
 }
 
@@ -306,50 +325,11 @@ void build_mg_prolongations(const Space& fine, MGHierarchyHost& H)
     }
 }
 
-void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double>& Ksp,
-              const std::vector<double>& Msp, const std::vector<double>& Mt, const std::vector<double>& Dt,
-              MGSetupHost& out)
+// Time factors of the preconditioner (they depend on the time grid only): the generalised
+// eigenvectors W and shifts sqrt(theta) of T1 w = theta T3 w, and (h_k Mt)^-1.
+static void setup_time_factors(const Space& S, const std::vector<double>& Mt, const std::vector<double>& Dt,
+                               MGSetupHost& out)
 {
-    const int L = static_cast<int>(H.coarse.size()) + 1;
-    out.levels.assign(L, MGLevelHost());
-    // smoother bounds of every level and matrix run on their own threads, each
-    // started as soon as its level exists (the Galerkin products of the coarser
-    // levels, the time factors and the coarsest inverses proceed meanwhile)
-    std::vector<std::future<double>> lm;
-    for (int l = 0; l < L; ++l) {
-        MGLevelHost& lv = out.levels[l];
-        const Space& sp = (l == 0) ? S : H.coarse[l - 1];
-        lv.nf = sp.nf;
-        lv.sptr = sp.sptr;
-        lv.scol = sp.scol;
-        if (l == 0) {
-            lv.K = Ksp;
-            lv.M = Msp;
-        } else {
-            const MGLevelHost& f = out.levels[l - 1];
-            const CSR& P = H.P[l - 1];
-            const CSR R = transpose(P);
-            // the K and M products at once
-            auto gm = std::async(std::launch::async,
-                                 [&] { return galerkin(R, f.sptr, f.scol, f.M, P, lv.sptr, lv.scol); });
-            lv.K = galerkin(R, f.sptr, f.scol, f.K, P, lv.sptr, lv.scol);
-            lv.M = gm.get();
-        }
-        lv.dK.assign(lv.nf, 0.0);
-        lv.dM.assign(lv.nf, 0.0);
-        for (int i = 0; i < lv.nf; ++i)
-            for (int e = lv.sptr[i]; e < lv.sptr[i + 1]; ++e)
-                if (lv.scol[e] == i) { lv.dK[i] = lv.K[e]; lv.dM[i] = lv.M[e]; }
-        for (int i = 0; i < lv.nf; ++i)
-            if (!(lv.dK[i] > 0.0) || !(lv.dM[i] > 0.0)) throw std::runtime_error("setup_mg: non-positive diagonal");
-        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK); }));
-        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM); }));
-        if (l + 1 < L) {
-            lv.P = H.P[l];
-            lv.R = transpose(H.P[l]);
-        }
-    }
-    // time factors
     const int p = S.p, Nt = S.Nt, m = S.mesh.m();
     const int T = m * p + 1;
     out.T = T;
@@ -418,6 +398,54 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
         }
     out.shift.resize(T);
     for (int i = 0; i < T; ++i) out.shift[i] = std::sqrt(std::max(theta[i], 0.0));
+}
+
+void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double>& Ksp,
+              const std::vector<double>& Msp, const std::vector<double>& Mt, const std::vector<double>& Dt,
+              MGSetupHost& out)
+{
+    const int L = static_cast<int>(H.coarse.size()) + 1;
+    out.levels.assign(L, MGLevelHost());
+    auto tf = std::async(std::launch::async, [&] { setup_time_factors(S, Mt, Dt, out); });
+    // smoother bounds of every level and matrix run on their own threads, each
+    // started as soon as its level exists (the Galerkin products of the coarser
+    // levels, the time factors and the coarsest inverses proceed meanwhile)
+    std::vector<std::future<double>> lm;
+    for (int l = 0; l < L; ++l) {
+        MGLevelHost& lv = out.levels[l];
+        const Space& sp = (l == 0) ? S : H.coarse[l - 1];
+        lv.nf = sp.nf;
+        lv.sptr = sp.sptr;
+        lv.scol = sp.scol;
+        if (l == 0) {
+            lv.K = Ksp;
+            lv.M = Msp;
+        } else {
+            const MGLevelHost& f = out.levels[l - 1];
+            const CSR& P = H.P[l - 1];
+            const CSR R = transpose(P);
+            // the K and M products at once
+            auto gm = std::async(std::launch::async,
+                                 [&] { return galerkin(R, f.sptr, f.scol, f.M, P, lv.sptr, lv.scol, 4); });
+            lv.K = galerkin(R, f.sptr, f.scol, f.K, P, lv.sptr, lv.scol, 4);
+            lv.M = gm.get();
+        }
+        lv.dK.assign(lv.nf, 0.0);
+        lv.dM.assign(lv.nf, 0.0);
+        for (int i = 0; i < lv.nf; ++i)
+            for (int e = lv.sptr[i]; e < lv.sptr[i + 1]; ++e)
+                if (lv.scol[e] == i) { lv.dK[i] = lv.K[e]; lv.dM[i] = lv.M[e]; }
+        for (int i = 0; i < lv.nf; ++i)
+            if (!(lv.dK[i] > 0.0) || !(lv.dM[i] > 0.0)) throw std::runtime_error("setup_mg: non-positive diagonal");
+        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK); }));
+        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM); }));
+        if (l + 1 < L) {
+            lv.P = H.P[l];
+            lv.R = transpose(H.P[l]);
+        }
+    }
+    tf.get();  // the time factors (computed beside the levels)
+    const int T = out.T;
     // coarsest inverses for shift 0 and each time-mode shift, per connected
     // component when the components are contiguous node ranges
     const MGLevelHost& cl = out.levels.back();
