@@ -267,8 +267,8 @@ def phase_rooflines(sp, ph, steps, mixed):
     out = {}
     t = ph["factor_ms"] * 1e-3
     ach = w["factor_alg"] / t / 1e12
-    name = ("k_patch_factor_blocked (Gauss-Jordan: FP32 VALU panels, MFMA rank-16 updates)" if mixed
-            else "k_patch_factor_full (register Gauss-Jordan, FP64 VALU)")
+    name = ("k_patch_factor_blocked (Gauss-Jordan: FP%d VALU panels, MFMA rank-16 updates)"
+            % (32 if mixed else 64))
     out["factor"] = {"bound": "mfma", "kernel": name, "achieved": ach, "peak": ft_peak, "unit": "TFLOP/s",
                      "frac": ach / ft_peak, "algorithmic_flops_per_launch": w["factor_alg"],
                      "work_unit": "(2/3) n^3 per patch mode matrix",

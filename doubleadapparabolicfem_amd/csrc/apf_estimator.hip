@@ -283,15 +283,16 @@ constexpr int PATCH_THREADS = 256;
 // into K (np x np, zero padded, exactly symmetric).  Only the entries of the
 // patch's pattern are written: the slot was zeroed when it was allocated and
 // no other entry is ever written.  With Alds, the same contributions also go to
-// an FP32 copy in LDS (row stride S).  Ends with a barrier.
+// a copy in LDS (FP32 or FP64, row stride S).  Ends with a barrier.
 //
 // A flux-flux entry has two contributions only when both of its unknowns lie on
 // one edge of the patch (the two cells of that edge); two cells share at most
 // one edge, so every other entry comes from one cell and is stored directly.
 // Only the same-edge entries are zeroed first and summed with atomics.
+template <typename AT = float>
 __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const PatchTopo& topo, int a, int n, int np,
                                    const double* verts, const int* cells, const double* Mcell, double* K,
-                                   float* Alds = nullptr, int S = 0)
+                                   AT* Alds = nullptr, int S = 0)
 {
     const int Nf = P.Nf, Nq = P.Nq, dim = P.dim;
     const int nF = topo.nflux[a];
@@ -320,7 +321,7 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
         K[static_cast<int64_t>(fj) * np + fi] = 0.0;
     }
     if (Alds)
-        for (int e = threadIdx.x; e < np * S; e += blockDim.x) Alds[e] = 0.0f;
+        for (int e = threadIdx.x; e < np * S; e += blockDim.x) Alds[e] = static_cast<AT>(0);
     __syncthreads();
     // every cell at once: a same-edge entry gets at most two contributions (the
     // cells of the edge), so its FP64 atomic sum is exact-order independent
@@ -352,15 +353,15 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
                 atomicAdd(&K[static_cast<int64_t>(fi) * np + fj], v);
                 if (i != j) atomicAdd(&K[static_cast<int64_t>(fj) * np + fi], v);
                 if (Alds) {
-                    atomicAdd(&Alds[fi * S + fj], static_cast<float>(v));
-                    if (i != j) atomicAdd(&Alds[fj * S + fi], static_cast<float>(v));
+                    atomicAdd(&Alds[fi * S + fj], static_cast<AT>(v));
+                    if (i != j) atomicAdd(&Alds[fj * S + fi], static_cast<AT>(v));
                 }
             } else {
                 K[static_cast<int64_t>(fi) * np + fj] = v;
                 K[static_cast<int64_t>(fj) * np + fi] = v;
                 if (Alds) {
-                    Alds[fi * S + fj] = static_cast<float>(v);
-                    Alds[fj * S + fi] = static_cast<float>(v);
+                    Alds[fi * S + fj] = static_cast<AT>(v);
+                    Alds[fj * S + fi] = static_cast<AT>(v);
                 }
             }
         } else if (w < Nf * Nf + Nq * Nf) {
@@ -374,8 +375,8 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
             K[static_cast<int64_t>(r0 + qq) * np + fj] = v;
             K[static_cast<int64_t>(fj) * np + r0 + qq] = v;
             if (Alds) {
-                Alds[(r0 + qq) * S + fj] = static_cast<float>(v);
-                Alds[fj * S + r0 + qq] = static_cast<float>(v);
+                Alds[(r0 + qq) * S + fj] = static_cast<AT>(v);
+                Alds[fj * S + r0 + qq] = static_cast<AT>(v);
             }
         } else if (interior) {
             // mean-zero row
@@ -386,8 +387,8 @@ __device__ void build_patch_matrix(const EstParams& P, const EstTables& T, const
             K[static_cast<int64_t>(r0 + qq) * np + (n - 1)] = v;
             K[static_cast<int64_t>(n - 1) * np + r0 + qq] = v;
             if (Alds) {
-                Alds[(r0 + qq) * S + (n - 1)] = static_cast<float>(v);
-                Alds[(n - 1) * S + r0 + qq] = static_cast<float>(v);
+                Alds[(r0 + qq) * S + (n - 1)] = static_cast<AT>(v);
+                Alds[(n - 1) * S + r0 + qq] = static_cast<AT>(v);
             }
         }
     }
@@ -671,8 +672,30 @@ __device__ inline unsigned wave_umax(unsigned v)
     return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
-// Phase 1, blocked (FP32, padded order np <= 64 RPL): Gauss-Jordan in
-// 16-column panels on the matrix in LDS.  Wave 0 eliminates a panel in
+// Pivot-search key bits of |v| (a non-negative value orders like its bit
+// pattern): the FP32 word, or the high word of the FP64 value.
+__device__ inline unsigned pivot_bits(float v) { return __float_as_uint(fabsf(v)); }
+__device__ inline unsigned pivot_bits(double v) { return static_cast<unsigned>(__double2hiint(fabs(v))); }
+// 1 / v: the FP32 hardware reciprocal (the FP64 refinement of the solve absorbs
+// its last-bit error), the correctly rounded FP64 quotient
+__device__ inline float pivot_recip(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ inline double pivot_recip(double v) { return 1.0 / v; }
+// 2^-floor(e / 2) for the binary exponent e of v > 0 (1 for v = 0)
+__device__ inline float equil_scale(float v)
+{
+    int e = 0;
+    (void)frexpf(v, &e);
+    return (v > 0.0f) ? ldexpf(1.0f, -(e >> 1)) : 1.0f;
+}
+__device__ inline double equil_scale(double v)
+{
+    int e = 0;
+    (void)frexp(v, &e);
+    return (v > 0.0) ? ldexp(1.0, -(e >> 1)) : 1.0;
+}
+
+// Phase 1, blocked (padded order np <= 64 RPL, FT = FP32 or FP64): Gauss-Jordan
+// in 16-column panels on the matrix in LDS.  Wave 0 eliminates a panel in
 // registers (lane l owns rows l + 64 s, all 16 panel columns; partial pivoting
 // among unused rows, no row interchange), so a panel step needs no workgroup
 // barrier and every row's multiplier is in its own lane.  The panel's
@@ -680,11 +703,13 @@ __device__ inline unsigned wave_umax(unsigned v)
 // eliminated panel P holds) is then applied to the other columns by all waves
 // as one rank-16 MFMA update  A[:, T] += (P - I_R) A[R, T]  with A[R, T]
 // copied first.  Same arithmetic as the unblocked elimination, reordered.
-template <int RPL>
+// FP64 at np = 128 takes 157 KB of LDS: one workgroup per CU.
+template <typename FT, int RPL>
 __global__ void __launch_bounds__(256)
     k_patch_factor_blocked(EstParams P, EstTables T, PatchTopo topo, PatchSlots ps, int q0, const double* verts,
-                           const int* cells, const double* Mcell, double* Kd, float* Ki)
+                           const int* cells, const double* Mcell, double* Kd, FT* Ki)
 {
+    using V4 = typename Vec4<FT>::type;
     constexpr int NPM = 64 * RPL;  // largest padded order
     // LDS row strides: S = np + 4 (= 4 mod 16) spreads the MFMA tile accesses
     // (rows 4 (l / 16) + r, columns l % 16) and the row-tile operand reads over
@@ -692,17 +717,17 @@ __global__ void __launch_bounds__(256)
     // The scaling vector lives in the pivot-row buffer while the matrix is
     // scaled, then in the stride padding of each row (column np, never read by
     // the elimination).
-    __shared__ __align__(16) float A[NPM * (NPM + 4)];
-    __shared__ __align__(16) float Rb[16 * (NPM + 16)];
+    __shared__ __align__(16) FT A[NPM * (NPM + 4)];
+    __shared__ __align__(16) FT Rb[16 * (NPM + 16)];
     __shared__ int pivrow[NPM], rinv[NPM];
-    float* dsc = Rb;
+    FT* dsc = Rb;
     const int q = q0 + blockIdx.x;
     const int a = topo.order[q];
     const int n = ps.n[q], np = ps.np[q];
     const int S = np + 4, RS = np + ((16 - np) & 63);
     double* K = Kd + ps.off2[q];
-    // the FP64 matrix into its slot and its FP32 copy into LDS in one pass
-    build_patch_matrix(P, T, topo, a, n, np, verts, cells, Mcell, K, A, S);
+    // the FP64 matrix into its slot and its FT copy into LDS in one pass
+    build_patch_matrix<FT>(P, T, topo, a, n, np, verts, cells, Mcell, K, A, S);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int nrt = np >> 4;
@@ -710,37 +735,35 @@ __global__ void __launch_bounds__(256)
     // exponent of max_j |K_ij| (so d_i^2 max_j |K_ij| lies in [1/2, 2)): the
     // flux, divergence and mean rows differ in scale by powers of h, and an FP32
     // Gauss-Jordan inverse of the scaled matrix is several times more accurate.
-    // Power-of-two factors make the scaling and the final unscaling exact in
-    // FP32.  Each wave reduces four rows (wave + 4u of a group of 16) at a time.
+    // Power-of-two factors make the scaling and the final unscaling exact.
+    // Each wave reduces four rows (wave + 4u of a group of 16) at a time.
     for (int i0 = wave; i0 < np; i0 += 16) {  // np is a multiple of 16
-        float mx[4];
+        FT mx[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            mx[u] = 0.0f;
+            mx[u] = static_cast<FT>(0);
 #pragma unroll
             for (int t = 0; t < RPL; ++t) {
                 const int j = lane + 64 * t;
-                if (j < np) mx[u] = fmaxf(mx[u], fabsf(A[(i0 + 4 * u) * S + j]));
+                if (j < np) mx[u] = fmax(mx[u], fabs(A[(i0 + 4 * u) * S + j]));
             }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) mx[u] = fmaxf(mx[u], __shfl_xor(mx[u], off, 64));
+            for (int u = 0; u < 4; ++u) mx[u] = fmax(mx[u], __shfl_xor(mx[u], off, 64));
         if (lane == 0) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                int e = 0;
-                (void)frexpf(mx[u], &e);
-                dsc[i0 + 4 * u] = (mx[u] > 0.0f) ? ldexpf(1.0f, -(e >> 1)) : 1.0f;
+                dsc[i0 + 4 * u] = equil_scale(mx[u]);
                 rinv[i0 + 4 * u] = i0 + 4 * u;
             }
         }
     }
     __syncthreads();
-    auto dkeep = [&](int i) -> float& { return A[i * S + np]; };
+    auto dkeep = [&](int i) -> FT& { return A[i * S + np]; };
     for (int i = wave; i < np; i += 4) {
-        const float di = dsc[i];
+        const FT di = dsc[i];
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const int j = lane + 64 * t;
@@ -757,7 +780,7 @@ __global__ void __launch_bounds__(256)
         const int kc = min(16, n - k0);
         if (wave == 0) {
             // wave 0 eliminates the panel in registers: lane l owns rows l + 64 s
-            float Pn[RPL][16];
+            FT Pn[RPL][16];
 #pragma unroll
             for (int s = 0; s < RPL; ++s)
 #pragma unroll
@@ -768,28 +791,27 @@ __global__ void __launch_bounds__(256)
             int prs[16];  // pivot row of each step (wave-uniform)
             auto pivot_step = [&](const int kk) __attribute__((always_inline)) {
                 // pivot: largest |A[i][k]| over unused rows, as one 32-bit key
-                // per candidate (|v| with its 7 low mantissa bits replaced by
-                // 127 - i: a pivot within 1 - 2^-16 of the largest is as good,
-                // ties go to the lowest row, a NaN wins) and one DPP max
+                // per candidate (the key bits of |v| with their 7 low bits
+                // replaced by 127 - i: a pivot within 1 - 2^-16 (FP32) or
+                // 1 - 2^-13 (FP64) of the largest is as good, ties go to the
+                // lowest row, a NaN wins) and one DPP max
                 unsigned key = 0u;
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) {
-                    const unsigned kv =
-                        (__float_as_uint(fabsf(Pn[s][kk])) & ~0x7Fu) | static_cast<unsigned>(127 - (lane + 64 * s));
+                    const unsigned kv = (pivot_bits(Pn[s][kk]) & ~0x7Fu) | static_cast<unsigned>(127 - (lane + 64 * s));
                     key = (used & (1u << s)) ? key : max(key, kv);
                 }
                 key = wave_umax(key);
                 const int r = min(max(127 - static_cast<int>(key & 0x7Fu), 0), n - 1);
                 const int owner = r & 63, sr = r >> 6;
-                // the pivot row, scaled by the reciprocal pivot (hardware
-                // reciprocal: the FP64 refinement absorbs its last-bit error)
-                float prow[16];
+                // the pivot row, scaled by the reciprocal pivot
+                FT prow[16];
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
-                    const float src = (RPL == 1 || sr == 0) ? Pn[0][c] : Pn[RPL - 1][c];
-                    prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src), owner));
+                    const FT src = (RPL == 1 || sr == 0) ? Pn[0][c] : Pn[RPL - 1][c];
+                    prow[c] = lane_value(src, owner);
                 }
-                const float inv = __builtin_amdgcn_rcpf(prow[kk]);
+                const FT inv = pivot_recip(prow[kk]);
 #pragma unroll
                 for (int c = 0; c < 16; ++c) prow[c] = (c == kk) ? inv : prow[c] * inv;
                 // every row eliminates (padding rows are zero and stay zero),
@@ -797,11 +819,11 @@ __global__ void __launch_bounds__(256)
                 const bool mine = lane == owner;
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) {
-                    const float f = Pn[s][kk];
+                    const FT f = Pn[s][kk];
                     const bool w = mine && (RPL == 1 || s == sr);
 #pragma unroll
                     for (int c = 0; c < 16; ++c) {
-                        const float e = ((c == kk) ? 0.0f : Pn[s][c]) - f * prow[c];
+                        const FT e = ((c == kk) ? static_cast<FT>(0) : Pn[s][c]) - f * prow[c];
                         Pn[s][c] = w ? prow[c] : e;
                     }
                 }
@@ -842,7 +864,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int t = 0; t < RPL; ++t) {
                 const int j = lane + 64 * t;
-                if (j < np) Rb[c * RS + j] = (c < kc && (j < k0 || j >= k0 + 16)) ? A[pr * S + j] : 0.0f;
+                if (j < np) Rb[c * RS + j] = (c < kc && (j < k0 || j >= k0 + 16)) ? A[pr * S + j] : static_cast<FT>(0);
             }
         }
         __syncthreads();
@@ -850,32 +872,33 @@ __global__ void __launch_bounds__(256)
         // alone.  Each wave keeps the (P - I_R) operand of its row tile.
         const int pt = k0 >> 4;
         for (int rt = wave; rt < nrt; rt += 4) {
-            float av[4];
+            FT av[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int c = 4 * t + lk, i = rt * 16 + li;
-                av[t] = (c < kc) ? A[i * S + k0 + c] - ((i == pivrow[k0 + c]) ? 1.0f : 0.0f) : 0.0f;
+                av[t] = (c < kc) ? A[i * S + k0 + c] - ((i == pivrow[k0 + c]) ? static_cast<FT>(1) : static_cast<FT>(0))
+                                 : static_cast<FT>(0);
             }
 #pragma unroll
             for (int ct = 0; ct < NPM / 16; ++ct) {
                 if (ct >= nrt || ct == pt) continue;
-                v4f acc;
+                V4 acc;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[r] = A[(rt * 16 + drow<float>(lk, r)) * S + ct * 16 + li];
+                for (int r = 0; r < 4; ++r) acc[r] = A[(rt * 16 + drow<FT>(lk, r)) * S + ct * 16 + li];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) acc = mfma_16x16x4(av[t], Rb[(4 * t + lk) * RS + ct * 16 + li], acc);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) A[(rt * 16 + drow<float>(lk, r)) * S + ct * 16 + li] = acc[r];
+                for (int r = 0; r < 4; ++r) A[(rt * 16 + drow<FT>(lk, r)) * S + ct * 16 + li] = acc[r];
             }
         }
         __syncthreads();
     }
     // K^-1 = D (D K D)^-1 D, with the pivot permutation folded in; rows by
     // wave, columns by lane
-    float* Kg = Ki + ps.off2[q];
+    FT* Kg = Ki + ps.off2[q];
     for (int i = wave; i < np; i += 4) {
         const int r = (i < n) ? rinv[i] : 0;
-        const float dr = (i < n) ? dkeep(r) : 0.0f;
+        const FT dr = (i < n) ? dkeep(r) : static_cast<FT>(0);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const int j = lane + 64 * t;
@@ -884,7 +907,7 @@ __global__ void __launch_bounds__(256)
                 const int c = pivrow[j];
                 Kg[static_cast<int64_t>(r) * np + c] = A[i * S + j] * (dr * dkeep(c));
             } else {
-                Kg[static_cast<int64_t>(i) * np + j] = 0.0f;
+                Kg[static_cast<int64_t>(i) * np + j] = static_cast<FT>(0);
             }
         }
     }
@@ -2392,6 +2415,10 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         auto set_lds = [&](const void* fn, size_t bytes) {
             HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
         };
+        static const bool fp64_blocked = [] {
+            const char* e = std::getenv("APF_FP64_BLOCKED");
+            return !(e && std::atoi(e) == 0);
+        }();
         auto launch = [&](auto tag) {
             using FT = decltype(tag);
             // the factor of one bucket in precision F2 (FP32: blocked Gauss-Jordan with MFMA
@@ -2400,17 +2427,19 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
             auto factor_bucket = [&](const EstimatorBuffers::Bucket& bk, auto ftag) {
                 using F2 = decltype(ftag);
                 F2* K2 = std::is_same<F2, float>::value ? reinterpret_cast<F2*>(B.Ki32) : reinterpret_cast<F2*>(B.Ki);
-                if constexpr (std::is_same<F2, float>::value) {
+                // blocked Gauss-Jordan with MFMA panel updates up to order 128 (FP64 too
+                // unless APF_FP64_BLOCKED=0, which takes the register-resident one)
+                if (std::is_same<F2, float>::value || fp64_blocked) {
                     if (bk.np <= 64) {
-                        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_blocked<1>), dim3(bk.count), dim3(256), 0,
-                                           st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd,
-                                           K2);
+                        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_blocked<F2, 1>), dim3(bk.count), dim3(256),
+                                           0, st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell,
+                                           B.Kd, K2);
                         return;
                     }
                     if (bk.np <= 128) {
-                        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_blocked<2>), dim3(bk.count), dim3(256), 0,
-                                           st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell, B.Kd,
-                                           K2);
+                        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_patch_factor_blocked<F2, 2>), dim3(bk.count), dim3(256),
+                                           0, st, P, Tb, topo, ps, bk.q0, sys->d_verts, sys->d_cells, sys->d_Mcell,
+                                           B.Kd, K2);
                         return;
                     }
                 }
@@ -2717,12 +2746,12 @@ extern "C" int apf_estimate_ex(apf_system* sys, int prec, int refine_steps, int 
         // The patch matrices and their factors do not read lambda_tilde: after a system's
         // first estimate in this precision (which times its tile and precision candidates),
         // the factor phase can run on a second stream while the slab reconstruction, a chain
-        // of small latency-bound launches, runs on the system stream.  Measured at 2+1D p=2:
-        // the FP64 estimate gains (5.20 -> 4.63 ms: the register-resident FP64 factor leaves
-        // room), the mixed one loses (4.15 -> 5.22 ms: the FP32 factor's workgroups hold the
-        // LDS of every CU, and each slab-solve launch waits for them).  So the default
-        // overlaps FP64 estimates only; APF_EST_OVERLAP=1 / 0 forces either way.
-        bool overlap = slab_mode == APF_SLAB_SOLVE && sys->est_done.count(prec) > 0 && prec == APF_FP64;
+        // of small latency-bound launches, runs on the system stream.  Measured at 2+1D p=2
+        // (the slab solve in 2-3 CG steps with FP32 V-cycles): mixed 3.16 -> 2.92 ms, FP64
+        // 3.62 -> 3.06 ms; the slab solve stretches from 1.02 to 1.41 ms while the factor's
+        // workgroups hold the CUs' LDS.  (With the slower 6-step slab solve the mixed estimate
+        // lost: 4.15 -> 5.22 ms.)  APF_EST_OVERLAP=0 / 1 forces either way.
+        bool overlap = slab_mode == APF_SLAB_SOLVE && sys->est_done.count(prec) > 0;
         if (const char* e = std::getenv("APF_EST_OVERLAP"))
             overlap = slab_mode == APF_SLAB_SOLVE && sys->est_done.count(prec) > 0 && std::atoi(e) != 0;
         if (overlap) {
