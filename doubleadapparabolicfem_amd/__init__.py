@@ -205,6 +205,23 @@ EXPORTED = {
     "apf_device_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "apf_device_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "apf_device_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "apf_mesh1d_create": (ctypes.c_int, [ctypes.c_double] * 4 + [ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_mesh1d_destroy": (None, [ctypes.c_void_p]),
+    "apf_mesh1d_info": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int64)] * 3),
+    "apf_mesh1d_vertices": (ctypes.c_int, [ctypes.c_void_p] * 3),
+    "apf_mesh1d_cells": (ctypes.c_int, [ctypes.c_void_p] * 6),
+    "apf_fespace1d_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_fespace1d_destroy": (None, [ctypes.c_void_p]),
+    "apf_fespace1d_refine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "apf_fespace1d_info": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int64)] * 4),
+    "apf_fespace1d_active_cells": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_fespace1d_cell_dofs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_fespace1d_constraints": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "apf_fespace1d_restriction": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "apf_fespace1d_ancestors": (ctypes.c_int, [ctypes.c_void_p] * 3),
+    "apf_general_x_indicators": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(_GeneralCoeff),
+                                                ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 
@@ -639,6 +656,22 @@ class GeneralSystem:
         b = np.zeros(self.n)
         _check(lib().apf_general_download(self.handle, None, None, None, _ptr(b)), self.ctx.handle)
         return b
+
+    def x_indicators(self, x, u0, M_grid=None, grid_n=0):
+        """(lambda_Y^2 per Y cell, initial-trace mismatch^2 per X cell) of x = [lambda; u]
+        (apf_general_x_indicators)."""
+        pts = self.bottom_points()
+        u0b = np.ascontiguousarray(u0(pts.reshape(-1, self.mesh.dim)).reshape(self.n_bottom, -1), dtype=np.float64)
+        Mg = None if M_grid is None else np.ascontiguousarray(M_grid, dtype=np.float64).ravel()
+        c = _GeneralCoeff(grid_n if Mg is not None else 0, _ptr(Mg), 0.0, _ptr(u0b))
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        if xx.shape != (self.n,):
+            raise ValueError(f"x must have length {self.n}")
+        lam2 = np.zeros(len(self.mesh.y_ancestor))
+        tr2 = np.zeros(self.mesh.X.n_cells)
+        _check(lib().apf_general_x_indicators(self.handle, _ptr(xx), ctypes.byref(c), _ptr(lam2), _ptr(tr2)),
+               self.ctx.handle, "apf_general_x_indicators")
+        return lam2, tr2
 
     def matrix(self):
         """a CsrMatrix view of K (valid while this system lives)"""

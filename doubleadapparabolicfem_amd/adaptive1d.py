@@ -1,0 +1,202 @@
+"""1+1D adaptive space-time meshes, their X and Y spaces, and the adaptive X loop.
+
+Mesh1D and FESpace1D wrap the library's restatement of the reference's 1+1D cell tree,
+refinement closure and DoF numbering (apf_mesh1d_*, apf_fespace1d_*): cell ids, true-DoF
+numbers and restriction rows are the reference's.  An FESpace1D carries the arrays of
+apf_cell_space_desc, so a pair (X, Y) feeds GeneralSystem (the GPU element-quadrature
+assembly) directly.
+
+AdaptiveX1D runs the outer loop of the reference's adaptive driver
+(algorithm/adaptive_driver.hpp:130-1230) on that pair: X = FESpace(SpaceTimePolicy) on
+the root cell refined `x_uniform_levels` times, Y = FESpace(SpaceOnlyPolicy) initialised
+on X's active cells (algorithm/examples/space_time_1d/space_time_1d_example.hpp:113-132 and
+adaptive_driver.hpp:1129); per outer iteration the saddle system is assembled and solved on
+the GPU, the X indicator eta^2 = lambda_Y^2 + initial-trace^2 per X cell
+(indicator_utilities.hpp:435-473) is computed on the GPU, X is Doerfler-marked with
+theta_x (adaptive_driver.hpp:974-1012) and refined (x_space_.refine, :1117), and Y is
+re-initialised on X's active cells.  The inner Y loop is not run here: each outer
+iteration takes Y = X's mesh, which is the inner loop's first iterate and its result
+whenever the Y estimator meets its threshold at once.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import (APF_FP64, Context, GeneralSystem, _check, _ptr, doerfler_mark, lib, y_estimator_threshold)
+
+
+class Mesh1D:
+    """Root cell [x0, x1] x [t0, t1] and its refinement tree (apf_mesh1d_*)."""
+
+    def __init__(self, x0=0.0, x1=1.0, t0=0.0, t1=1.0):
+        h = ctypes.c_void_p()
+        _check(lib().apf_mesh1d_create(x0, x1, t0, t1, ctypes.byref(h)), None, "apf_mesh1d_create")
+        self.handle = h
+        self._destroy = lib().apf_mesh1d_destroy
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._destroy(self.handle)
+            self.handle = None
+
+    def info(self):
+        n, nx, nt = (ctypes.c_int64() for _ in range(3))
+        _check(lib().apf_mesh1d_info(self.handle, ctypes.byref(n), ctypes.byref(nx), ctypes.byref(nt)))
+        return n.value, nx.value, nt.value
+
+    def vertices(self):
+        _, nx, nt = self.info()
+        x, t = np.zeros(nx), np.zeros(nt)
+        _check(lib().apf_mesh1d_vertices(self.handle, _ptr(x), _ptr(t)))
+        return x, t
+
+    def cells(self):
+        """dict of per-cell arrays: parent, generation, is_leaf, vertex_ids [n, 4], box [n, 4]"""
+        n, _, _ = self.info()
+        out = dict(parent=np.zeros(n, np.int32), generation=np.zeros(n, np.int32), is_leaf=np.zeros(n, np.int32),
+                   vertex_ids=np.zeros((n, 4), np.int32), box=np.zeros((n, 4)))
+        _check(lib().apf_mesh1d_cells(self.handle, _ptr(out["parent"]), _ptr(out["generation"]), _ptr(out["is_leaf"]),
+                                      _ptr(out["vertex_ids"]), _ptr(out["box"])))
+        return out
+
+    def leaf_cells(self):
+        return np.flatnonzero(self.cells()["is_leaf"]).astype(np.int32)
+
+
+class FESpace1D:
+    """X (continuous_in_time) or Y space of degree p on a Mesh1D, with the arrays of
+    apf_cell_space_desc: vertices [nv, 1], cells [n_active, 2] spatial vertex ids,
+    times [n_active, 2], restr_ptr / restr_dof / restr_w, n_dofs (true DoFs)."""
+
+    dim = 1
+
+    def __init__(self, mesh: Mesh1D, p: int, continuous_in_time: bool, active_cells):
+        self.mesh, self.p, self.continuous_in_time = mesh, p, bool(continuous_in_time)
+        act = np.ascontiguousarray(active_cells, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().apf_fespace1d_create(mesh.handle, p, int(self.continuous_in_time), _ptr(act), len(act),
+                                          ctypes.byref(h)), None, "apf_fespace1d_create")
+        self.handle = h
+        self._destroy = lib().apf_fespace1d_destroy
+        self._refresh()
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._destroy(self.handle)
+            self.handle = None
+
+    @property
+    def n_loc(self):
+        return (self.p + 1) ** 2
+
+    @property
+    def n_cells(self):
+        return len(self.active)
+
+    def _refresh(self):
+        na, nt, nc, ne = (ctypes.c_int64() for _ in range(4))
+        _check(lib().apf_fespace1d_info(self.handle, ctypes.byref(na), ctypes.byref(nt), ctypes.byref(nc),
+                                        ctypes.byref(ne)), None, "apf_fespace1d_info")
+        self.n_dofs, self.n_constrained = nt.value, nc.value
+        self.active = np.zeros(na.value, np.int32)
+        _check(lib().apf_fespace1d_active_cells(self.handle, _ptr(self.active)))
+        self.cell_dofs = np.zeros((na.value, self.n_loc), np.int32)
+        _check(lib().apf_fespace1d_cell_dofs(self.handle, _ptr(self.cell_dofs)))
+        self.restr_ptr = np.zeros(na.value * self.n_loc + 1, np.int64)
+        self.restr_dof = np.zeros(ne.value, np.int32)
+        self.restr_w = np.zeros(ne.value)
+        _check(lib().apf_fespace1d_restriction(self.handle, _ptr(self.restr_ptr), _ptr(self.restr_dof),
+                                               _ptr(self.restr_w)), None, "apf_fespace1d_restriction")
+        self.con_ptr = np.zeros(nc.value + 1, np.int64)
+        nce = 0
+        if nc.value:
+            _check(lib().apf_fespace1d_constraints(self.handle, _ptr(self.con_ptr), None, None))
+            nce = int(self.con_ptr[-1])
+        self.con_masters = np.zeros(nce, np.int32)
+        self.con_weights = np.zeros(nce)
+        if nc.value:
+            _check(lib().apf_fespace1d_constraints(self.handle, _ptr(self.con_ptr), _ptr(self.con_masters),
+                                                   _ptr(self.con_weights)))
+        x, _ = self.mesh.vertices()
+        self.vertices = x[:, None].copy()
+        cl = self.mesh.cells()
+        self.cells = np.ascontiguousarray(cl["vertex_ids"][self.active, :2])
+        self.times = np.ascontiguousarray(cl["box"][self.active, 2:])
+        self.box = np.ascontiguousarray(cl["box"][self.active])
+        self.generation = cl["generation"][self.active]
+
+    def refine(self, marked):
+        """FESpace::refine of the marked active cells (in marking order)"""
+        m = np.ascontiguousarray(marked, dtype=np.int32)
+        _check(lib().apf_fespace1d_refine(self.handle, _ptr(m), len(m)), None, "apf_fespace1d_refine")
+        self._refresh()
+
+    def ancestors_of(self, fine: "FESpace1D"):
+        """position in this space's active list of each active cell's first active ancestor in `fine`"""
+        out = np.zeros(fine.n_cells, np.int32)
+        _check(lib().apf_fespace1d_ancestors(self.handle, fine.handle, _ptr(out)), None, "apf_fespace1d_ancestors")
+        return out
+
+
+class SpaceTimePair:
+    """X and Y spaces on one mesh, in the form GeneralSystem takes."""
+
+    dim = 1
+
+    def __init__(self, X: FESpace1D, Y: FESpace1D):
+        if X.p != Y.p:
+            raise ValueError("X and Y need the same degree")
+        self.X, self.Y, self.p = X, Y, X.p
+        self.y_ancestor = X.ancestors_of(Y)
+
+
+def smooth_initial_u0(x):
+    """u0 = sin(pi x) (algorithm/examples/space_time_1d/initial_data_1d_examples.hpp:9-63: M = 1, ell = 0)"""
+    return np.sin(np.pi * np.asarray(x)[..., 0])
+
+
+class AdaptiveX1D:
+    """Outer X loop of the reference's adaptive driver on 1+1D meshes, on the GPU."""
+
+    def __init__(self, ctx: Context, p=1, theta_x=0.4, rho=1.0, x_uniform_levels=1, u0=smooth_initial_u0,
+                 tol=1e-10, max_it=40000):
+        self.ctx, self.p, self.theta_x, self.rho, self.u0 = ctx, p, theta_x, rho, u0
+        self.tol, self.max_it = tol, max_it
+        self.mesh = Mesh1D(0.0, 1.0, 0.0, 1.0)
+        self.X = FESpace1D(self.mesh, p, True, self.mesh.leaf_cells())
+        for _ in range(x_uniform_levels):
+            self.X.refine(self.X.active)
+        self.Y = FESpace1D(self.mesh, p, False, self.X.active)
+        self.history = []
+
+    def step(self, refine=True):
+        """one outer iteration: assemble, solve, indicators, mark, refine X, re-initialise Y.
+        Returns the iteration's record."""
+        pair = SpaceTimePair(self.X, self.Y)
+        gs = GeneralSystem(self.ctx, pair)
+        ms = gs.assemble(self.u0)
+        K = gs.matrix()
+        x, stats = K.minres(gs.rhs(), gs.nY, tol=self.tol, max_it=self.max_it, precond=APF_FP64)
+        lam2_y, tr2_x = gs.x_indicators(x, self.u0)
+        lam2_x = np.zeros(self.X.n_cells)
+        np.add.at(lam2_x, pair.y_ancestor, lam2_y)
+        eta2_x = lam2_x + tr2_x
+        lam2, tr2 = float(lam2_y.sum()), float(tr2_x.sum())
+        rec = dict(x_true_dofs=self.X.n_dofs, y_true_dofs=self.Y.n_dofs, x_cells=self.X.n_cells,
+                   y_cells=self.Y.n_cells, lambda_y_squared=lam2, initial_trace_squared=tr2,
+                   eta_squared=float(eta2_x.sum()),
+                   y_estimator_threshold_squared=y_estimator_threshold(self.rho, lam2, tr2),
+                   minres_iterations=stats["iterations"], minres_rel_residual=stats["true_relative_residual"],
+                   assembly_ms=ms, solution=x, eta2_by_x_cell=eta2_x)
+        if refine:
+            marked, _ = doerfler_mark(eta2_x, self.theta_x, ids=self.X.active.astype(np.int64))
+            rec["marked_x_cells"] = np.asarray(marked)
+            self.X.refine(marked)
+            self.Y = FESpace1D(self.mesh, self.p, False, self.X.active)
+            rec["x_true_dofs_after"] = self.X.n_dofs
+        del K
+        del gs
+        self.history.append(rec)
+        return rec

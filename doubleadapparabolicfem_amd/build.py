@@ -21,7 +21,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics"]
 
-SOURCES = ["apf_tables.cpp", "apf_space.cpp", "apf_mg.cpp", "apf_marking.cpp", "apf_gpu.hip", "apf_estimator.hip", "apf_g.hip",
+SOURCES = ["apf_tables.cpp", "apf_space.cpp", "apf_mg.cpp", "apf_marking.cpp", "apf_mesh1d.cpp", "apf_gpu.hip", "apf_estimator.hip", "apf_g.hip",
            "apf_mg.hip", "apf_csr.hip", "apf_general.hip"]
 
 

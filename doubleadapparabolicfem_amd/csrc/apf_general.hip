@@ -448,6 +448,109 @@ __global__ void __launch_bounds__(256) k_gen_bottom(GenDev d)
     }
 }
 
+// X-marking indicator parts (algorithm/detail/indicator_utilities.hpp:435-473):
+// lambda_Y^2 of each Y cell, int_cell M grad_x lambda . grad_x lambda on the cell rule
+// (compute_spatial_energy_squared_by_active_cell, :109-171), one thread per Y cell.
+template <int DIM, int P>
+__global__ void __launch_bounds__(256) k_gen_lambda_energy(GenDev d, const double* __restrict__ x,
+                                                           double* __restrict__ out)
+{
+    using S = GS<DIM, P>;
+    constexpr int NS = S::NS, NT = S::NT, NL = S::NL, NQS = S::NQS, NQT = S::NQT, MD = S::MD;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= d.nyc) return;
+    double lam[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        double v = 0.0;
+        for (int64_t e = d.yptr[c * NL + l]; e < d.yptr[c * NL + l + 1]; ++e) v += d.yw[e] * x[d.ydof[e]];
+        lam[l] = v;
+    }
+    const double* gy = d.ygeo + c * kGeo;
+    double v0[DIM], Ji[MD];
+    const double det = cell_map<DIM>(gy, v0, Ji);
+    const double h = gy[kGeo - 1] - gy[kGeo - 2];
+    double acc = 0.0;
+    for (int q = 0; q < NQS; ++q) {
+        double xq[DIM];
+        if constexpr (DIM == 1) {
+            xq[0] = gy[0] + (gy[1] - gy[0]) * d.sq_x[q];
+        } else {
+            const double a = d.sq_x[2 * q], bb = d.sq_x[2 * q + 1];
+            xq[0] = gy[0] + (gy[2] - gy[0]) * a + (gy[4] - gy[0]) * bb;
+            xq[1] = gy[1] + (gy[3] - gy[1]) * a + (gy[5] - gy[1]) * bb;
+        }
+        double M[MD];
+        M_at<DIM>(d, xq, M);
+        double gp[NS][DIM];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const double* gr = d.glamY + (q * NS + s) * DIM;
+            if constexpr (DIM == 1) {
+                gp[s][0] = Ji[0] * gr[0];
+            } else {
+                gp[s][0] = Ji[0] * gr[0] + Ji[2] * gr[1];
+                gp[s][1] = Ji[1] * gr[0] + Ji[3] * gr[1];
+            }
+        }
+        for (int qt = 0; qt < NQT; ++qt) {
+            double g[DIM];
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) g[a] = 0.0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                double ls = 0.0;
+#pragma unroll
+                for (int j = 0; j < NT; ++j) ls += lam[s * NT + j] * d.ellY[qt * NT + j];
+#pragma unroll
+                for (int a = 0; a < DIM; ++a) g[a] += ls * gp[s][a];
+            }
+            double e = 0.0;
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) {
+                double mg = 0.0;
+#pragma unroll
+                for (int b = 0; b < DIM; ++b) mg += M[a * DIM + b] * g[b];
+                e += g[a] * mg;
+            }
+            acc += d.sq_w[q] * det * d.tq_w[qt] * h * e;
+        }
+    }
+    out[c] = acc;
+}
+
+// initial-trace mismatch int_{t=0} (u0 - u_delta)^2 of each X cell with t0 = 0
+// (compute_initial_trace_mismatch_squared_by_x_cell, indicator_utilities.hpp:333-396), one
+// thread per such cell; out[] (one entry per X cell) is zero elsewhere.
+template <int DIM, int P>
+__global__ void __launch_bounds__(256) k_gen_trace(GenDev d, const double* __restrict__ x, double* __restrict__ out)
+{
+    using S = GS<DIM, P>;
+    constexpr int NS = S::NS, NT = S::NT, NL = S::NL, NQS = S::NQS, MD = S::MD;
+    const int64_t bc = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (bc >= d.nbot) return;
+    const int64_t xc = d.bottom[bc];
+    double us[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int l = s * NT;  // bottom time node
+        double v = 0.0;
+        for (int64_t e = d.xptr[xc * NL + l]; e < d.xptr[xc * NL + l + 1]; ++e) v += d.xw[e] * x[d.nY + d.xdof[e]];
+        us[s] = v;
+    }
+    double v0[DIM], Ji[MD];
+    const double det = cell_map<DIM>(d.xgeo + xc * kGeo, v0, Ji);
+    double acc = 0.0;
+    for (int q = 0; q < NQS; ++q) {
+        double u = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) u += us[s] * d.lamY[q * NS + s];
+        const double diff = d.u0b[bc * NQS + q] - u;
+        acc += diff * diff * det * d.sq_w[q];
+    }
+    out[xc] = acc;
+}
+
 __global__ void k_zero(int64_t n, double* __restrict__ a)
 {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -899,6 +1002,79 @@ int apf_general_download(apf_general_system* sys, int64_t* rowptr, int* col, dou
         return APF_OK;
     } catch (const HipFailure& e) {
         return fail(sys->ctx, APF_HIP_ERROR, std::string("apf_general_download: ") + e.what());
+    }
+}
+
+int apf_general_x_indicators(apf_general_system* sys, const double* x, const apf_general_coeff* coeff,
+                             double* lambda_y2, double* trace2)
+{
+    if (!sys || !x || !coeff || !lambda_y2 || !trace2)
+        return fail(sys ? sys->ctx : nullptr, APF_BAD_ARG, "apf_general_x_indicators: null argument");
+    apf_ctx* ctx = sys->ctx;
+    if (coeff->grid_n < 0 || (coeff->grid_n > 0 && !coeff->M_grid))
+        return fail(ctx, APF_BAD_ARG, "apf_general_x_indicators: bad M grid");
+    if (sys->nbot > 0 && !coeff->u0_bottom) return fail(ctx, APF_BAD_ARG, "apf_general_x_indicators: null u0_bottom");
+    void* dx = nullptr;
+    void* dout = nullptr;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const int dim = sys->dim, p = sys->p;
+        GenDev d = sys->d;
+        d.grid_n = coeff->grid_n;
+        if (coeff->grid_n > 0) {
+            int64_t len = 1;
+            for (int a = 0; a < dim; ++a) len *= coeff->grid_n;
+            len *= dim * dim;
+            if (len > sys->Mgrid_len) {
+                sys->Mgrid = sys->alloc<double>(static_cast<size_t>(len));
+                sys->Mgrid_len = len;
+            }
+            HIP_TRY(hipMemcpyAsync(sys->Mgrid, coeff->M_grid, len * sizeof(double), hipMemcpyHostToDevice, st));
+            d.Mgrid = sys->Mgrid;
+        }
+        if (sys->nbot > 0)
+            HIP_TRY(hipMemcpyAsync(sys->u0b, coeff->u0_bottom, sys->nbot * sys->nqs * sizeof(double),
+                                   hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMalloc(&dx, std::max<int64_t>(1, sys->n) * sizeof(double)));
+        HIP_TRY(hipMalloc(&dout, std::max<int64_t>(1, sys->nyc + sys->nxc) * sizeof(double)));
+        HIP_TRY(hipMemcpyAsync(dx, x, sys->n * sizeof(double), hipMemcpyHostToDevice, st));
+        double* oy = static_cast<double*>(dout);
+        double* ox = oy + sys->nyc;
+        hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, st, sys->nxc, ox);
+        auto launch = [&](auto dimc, auto pc) {
+            constexpr int D = decltype(dimc)::value, PP = decltype(pc)::value;
+            const double* xs = static_cast<const double*>(dx);
+            if (sys->nyc > 0)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gen_lambda_energy<D, PP>),
+                                   dim3(static_cast<unsigned>((sys->nyc + 255) / 256)), dim3(256), 0, st, d, xs, oy);
+            if (sys->nbot > 0)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gen_trace<D, PP>),
+                                   dim3(static_cast<unsigned>((sys->nbot + 255) / 256)), dim3(256), 0, st, d, xs, ox);
+        };
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        if (dim == 1) {
+            if (p == 1) launch(I1{}, I1{});
+            else if (p == 2) launch(I1{}, I2{});
+            else launch(I1{}, I3{});
+        } else {
+            if (p == 1) launch(I2{}, I1{});
+            else if (p == 2) launch(I2{}, I2{});
+            else launch(I2{}, I3{});
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(lambda_y2, oy, sys->nyc * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(trace2, ox, sys->nxc * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipFree(dx));
+        HIP_TRY(hipFree(dout));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        if (dx) (void)hipFree(dx);
+        if (dout) (void)hipFree(dout);
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_general_x_indicators: ") + e.what());
     }
 }
 

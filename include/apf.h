@@ -334,6 +334,16 @@ int apf_general_assemble(apf_general_system* sys, const apf_general_coeff* coeff
 int apf_general_download(apf_general_system* sys, int64_t* rowptr, int* col, double* values, double* b);
 /* A view of the system's K for apf_csr_minres_solve (destroy it before the system). */
 int apf_general_matrix(apf_general_system* sys, apf_csr_matrix** out);
+/* The two parts of the X-marking indicator of the solution x = [lambda; u] (host, length n)
+ * (compute_x_marking_indicator_components, algorithm/detail/indicator_utilities.hpp:435-473):
+ * lambda_y2[c] = int_{Y cell c} M grad_x lambda . grad_x lambda on the cell rule
+ * (compute_spatial_energy_squared_by_active_cell, :109-171), and trace2[a] =
+ * int_{t=0} (u0 - u)^2 on X cell a if t0 = 0, else 0 (compute_initial_trace_mismatch_squared_
+ * by_x_cell, :333-396).  M and u0 (at apf_general_bottom_points) as for apf_general_assemble.
+ * The caller sums lambda_y2 into the Y cells' X ancestors (aggregate_to_x_active_cells,
+ * :398-433) and adds trace2 to get eta^2 per X cell. */
+int apf_general_x_indicators(apf_general_system* sys, const double* x, const apf_general_coeff* coeff,
+                             double* lambda_y2, double* trace2);
 
 /* Estimator on a non-tensor mesh.  The estimator works per time slab (the slabs between
  * the global time endpoints, time_slabs/time_slab_builder.hpp:53-120); slabs that share one
@@ -404,6 +414,53 @@ typedef struct {
  * A raw value below -1000 eps max(1, |lambda_G - I lambda|^2) returns APF_BAD_ARG,
  * as the reference's runtime_error at adaptive_driver.hpp:2869-2883. */
 int apf_g_estimate(apf_system* sys, apf_system* gsys, double tol, int max_it, apf_g_estimate_out* out);
+
+/* ---- 1+1D adaptive meshes and their X / Y spaces (host; no GPU needed) ---- */
+/* The reference's cell tree, vertex ids, refinement closure and DoF numbering for 1+1D
+ * rectangles, so that cell ids, true-DoF numbers and restriction rows are those of the
+ * reference.  A space references its mesh (the X and Y spaces share one mesh, as the
+ * reference's FESpaces do); destroy the spaces before the mesh. */
+typedef struct apf_mesh1d apf_mesh1d;
+typedef struct apf_fespace1d apf_fespace1d;
+/* Root cell [x0, x1] x [t0, t1] (Mesh(x0, x1, t0, t1), mesh/mesh.hpp:52-59 with
+ * mesh/initialization/root_cell.hpp:60-107); cell 0, spatial vertices 0, 1, temporal 0, 1. */
+int apf_mesh1d_create(double x0, double x1, double t0, double t1, apf_mesh1d** out);
+void apf_mesh1d_destroy(apf_mesh1d* m);
+int apf_mesh1d_info(const apf_mesh1d* m, int64_t* n_cells, int64_t* n_spatial_vertices, int64_t* n_temporal_vertices);
+/* vertex coordinates by id (either may be NULL) */
+int apf_mesh1d_vertices(const apf_mesh1d* m, double* x, double* t);
+/* per mesh cell (any may be NULL): parent (-1 for the root), generation, leaf flag,
+ * [x0 id, x1 id, t0 id, t1 id], box [x0, x1, t0, t1] (mesh/cell.hpp). */
+int apf_mesh1d_cells(const apf_mesh1d* m, int* parent, int* generation, int* is_leaf, int* vertex_ids, double* box);
+/* FE space of degree p (1..3 in space and time) on the given active cells: the X space
+ * (continuous_in_time != 0: SpaceTimePolicy) or the Y space (0: SpaceOnlyPolicy,
+ * fespace/policies.hpp:9-28).  FESpace::initialize (fespace/fespace.hpp:115-121):
+ * adjacency and distribute_dofs_1d (fespace/dofs/dof_distribution_1d.hpp:655-671). */
+int apf_fespace1d_create(apf_mesh1d* m, int p, int continuous_in_time, const int* active_cells, int64_t n_active,
+                         apf_fespace1d** out);
+void apf_fespace1d_destroy(apf_fespace1d* s);
+/* FESpace::refine (fespace/fespace.hpp:347-371 -> fespace/refinement/refine_1d.hpp:282-321):
+ * the marked active cells split in the given order (the Doerfler order), then the
+ * one-level grading closure; the DoFs are redistributed. */
+int apf_fespace1d_refine(apf_fespace1d* s, const int* marked, int64_t n_marked);
+/* active cells, true (free) DoFs, constrained DoFs, entries of the restriction rows */
+int apf_fespace1d_info(const apf_fespace1d* s, int64_t* n_active, int64_t* n_true_dofs, int64_t* n_constrained_dofs,
+                       int64_t* n_restriction_entries);
+/* active cell ids in the partition's order (fespace/partition_view.hpp) */
+int apf_fespace1d_active_cells(const apf_fespace1d* s, int* cells);
+/* [n_active][(p+1)^2] global DoF ids per local DoF (local = i_space * (p+1) + j_time):
+ * true DoFs 0..n_true-1, constrained DoFs after them, -1 for a Dirichlet DoF
+ * (DoFHandler::cell_dofs, fespace/dofs/dof_handler.hpp) */
+int apf_fespace1d_cell_dofs(const apf_fespace1d* s, int* gids);
+/* constrained DoF n_true + k: masters[ptr[k] .. ptr[k+1]) (true DoFs) and weights */
+int apf_fespace1d_constraints(const apf_fespace1d* s, int64_t* ptr, int* masters, double* weights);
+/* restriction rows of the active cells, in active order ([n_active * (p+1)^2 + 1] ptr):
+ * build_cell_restriction (assembly/constrained_dofs.hpp:133-226), the rows that
+ * apf_cell_space_desc takes */
+int apf_fespace1d_restriction(const apf_fespace1d* s, int64_t* ptr, int* dof, double* weight);
+/* for each active cell of `fine`, the position in coarse's active list of its first
+ * active ancestor (find_active_ancestor_cell, assembly/detail/active_cell_locator.hpp) */
+int apf_fespace1d_ancestors(const apf_fespace1d* coarse, const apf_fespace1d* fine, int* index);
 
 /* ---- marking (host; no GPU needed) ---- */
 typedef struct {

@@ -11,6 +11,12 @@ mesh, identified by X true DOFs equal to the uniform run's outer iteration 0, G
 true DOFs equal to the degree-(p+1) uniform Y true DOFs at outer iteration 0, and
 the same eta^2 and lambda_y^2 as that uniform record.
 
+Also writes tests/golden/adaptive1d_anchors.json: the 1+1D smooth_initial adaptive runs
+(mode adaptive_standard, p = 1, 2, 3), one record per outer iteration while the inner Y
+loop accepted its first iterate (inner iteration 0 with Y true DOFs unchanged and the
+next record starting the next outer iteration): X true DOFs before and after the X
+refinement, Y true DOFs, eta^2, lambda_Y^2, y_estimator^2 and its threshold.
+
 Usage:  python tests/golden/make_anchors.py [/root/reference]
 """
 import csv
@@ -68,6 +74,29 @@ def main():
     with open(gdst, "w") as fh:
         json.dump(gout, fh, indent=1)
     print(f"wrote {len(gout)} G anchors to {gdst}")
+    aout = []
+    path = os.path.join(REF, "algorithm_data_plot", "1d", "smooth_initial.csv")
+    with open(path) as fh:
+        rows = [(i + 2, row) for i, row in enumerate(csv.DictReader(fh))]
+    for p in (1, 2, 3):
+        inner = [(ln, r) for ln, r in rows
+                 if r["record_type"] == "inner" and r["mode"] == "adaptive_standard" and r["p_int"] == str(p)]
+        for k, (ln, r) in enumerate(inner):
+            if int(r["outer_iteration"]) != k or r["inner_iteration"] != "0":
+                break
+            if r["y_true_dofs_before"] != r["y_true_dofs_after"]:
+                break
+            aout.append({"dim": 1, "p": p, "outer": k, "source": f"algorithm_data_plot/1d/smooth_initial.csv:{ln}",
+                         "mode": r["mode"], "x_true_dofs": int(float(r["x_true_dofs"])),
+                         "x_true_dofs_after": int(float(r["x_true_dofs_after"])),
+                         "y_true_dofs": int(float(r["y_true_dofs_before"])),
+                         "eta_squared": float(r["eta_squared"]), "lambda_y_squared": float(r["lambda_y_squared"]),
+                         "y_estimator_squared": float(r["y_estimator_squared"]),
+                         "y_estimator_threshold_squared": float(r["y_estimator_threshold_squared"])})
+    adst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "adaptive1d_anchors.json")
+    with open(adst, "w") as fh:
+        json.dump(aout, fh, indent=1)
+    print(f"wrote {len(aout)} adaptive 1+1D anchors to {adst}")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,195 @@
+"""1+1D adaptive meshes and spaces (host; no GPU).
+
+* The library's cell tree and DoF numbering (apf_mesh1d_*, apf_fespace1d_*) agree bit for
+  bit with the oracle's independent restatement: cell ids, active order, per-cell global
+  DoFs, constraint masters and weights, restriction rows, ancestors, on the published
+  adaptive runs' marking sequences and on random ones (p = 1, 2, 3; X and Y).
+* The restriction rows give conforming spaces: every X function is continuous across
+  every face, every Y function across spatial faces.
+* The oracle's adaptive X loop reproduces the published X / Y true-DoF sequences and the
+  published eta^2, lambda_Y^2 (6 digits) of the 1+1D smooth_initial adaptive runs
+  (algorithm_data_plot/1d/smooth_initial.csv:336-352 for p = 1 and the p = 2, 3 rows in
+  tests/golden/adaptive1d_anchors.json), and the uniform counts 255 / 480 at 16 x 16
+  (smooth_initial.csv:442).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from doubleadapparabolicfem_amd import adaptive1d as A
+from oracle import adaptive1d as O
+from oracle.basis import segment_lagrange
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adaptive1d_anchors.json")
+
+
+def u0(x):
+    return np.sin(np.pi * x)
+
+
+def _same_space(lib_space, ora_space):
+    assert list(lib_space.active) == list(ora_space.active)
+    assert lib_space.n_dofs == ora_space.n_true
+    assert lib_space.n_constrained == len(ora_space.constraints)
+    np.testing.assert_array_equal(lib_space.cell_dofs, ora_space.cell_dofs)
+    for k, (m, w) in enumerate(ora_space.constraints):
+        a, b = lib_space.con_ptr[k], lib_space.con_ptr[k + 1]
+        assert list(lib_space.con_masters[a:b]) == m
+        assert list(lib_space.con_weights[a:b]) == w  # bit for bit
+    ptr, dof, w = ora_space.restriction()
+    np.testing.assert_array_equal(lib_space.restr_ptr, ptr)
+    np.testing.assert_array_equal(lib_space.restr_dof, dof)
+    np.testing.assert_array_equal(lib_space.restr_w, w)
+
+
+def _same_tree(mesh, tree):
+    c = mesh.cells()
+    assert len(c["parent"]) == len(tree.cells)
+    assert list(c["parent"]) == [t["parent"] for t in tree.cells]
+    assert list(c["generation"]) == [t["gen"] for t in tree.cells]
+    assert [list(v) for v in c["vertex_ids"]] == [t["sv"] + t["tv"] for t in tree.cells]
+    x, t = mesh.vertices()
+    assert list(x) == tree.xv and list(t) == tree.tv
+
+
+def test_initial_and_uniform_counts():
+    m = A.Mesh1D()
+    X = A.FESpace1D(m, 1, True, m.leaf_cells())
+    assert X.n_dofs == 0 and list(X.active) == [0]
+    X.refine(X.active)
+    assert list(X.active) == [1, 2, 3, 4] and X.n_dofs == 3
+    assert A.FESpace1D(m, 1, False, X.active).n_dofs == 4
+    for _ in range(3):
+        X.refine(X.active)
+    # 16 x 16: published X / Y = 255 / 480 (smooth_initial.csv:442)
+    assert (X.n_dofs, A.FESpace1D(m, 1, False, X.active).n_dofs) == (255, 480)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_library_numbering_matches_oracle_on_published_runs(p):
+    """drive both with the oracle's Doerfler marks of the smooth_initial run"""
+    steps = {1: 9, 2: 7, 3: 6}[p]
+    m, T = A.Mesh1D(), O.Tree()
+    X = A.FESpace1D(m, p, True, m.leaf_cells())
+    OX = O.Space(T, p, True, T.leaves())
+    X.refine(X.active)
+    OX.refine(list(OX.active))
+    for _ in range(steps):
+        Y = A.FESpace1D(m, p, False, X.active)
+        OY = O.Space(T, p, False, OX.active)
+        _same_tree(m, T)
+        _same_space(X, OX)
+        _same_space(Y, OY)
+        np.testing.assert_array_equal(X.ancestors_of(Y), OY.ancestors_in(OX))
+        K, b = O.assemble(OX, OY, u0)
+        x = O.solve(K, b)
+        l2, t2 = O.x_indicators(OX, OY, x, u0)
+        lx = np.zeros(len(OX.active))
+        np.add.at(lx, OY.ancestors_in(OX), l2)
+        marked, _ = O.doerfler_mark(lx + t2, 0.4, ids=OX.active)
+        X.refine(marked)
+        OX.refine(marked)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("p", [1, 2])
+def test_library_numbering_matches_oracle_on_random_marks(p, seed):
+    """random marks in the driver's pattern on one shared tree: Y refined (inner loop), then
+    X refined (reusing children Y created) and Y re-initialised on X's active cells"""
+    rng = np.random.default_rng(seed)
+    m, T = A.Mesh1D(), O.Tree()
+    X = A.FESpace1D(m, p, True, m.leaf_cells())
+    OX = O.Space(T, p, True, T.leaves())
+    X.refine(X.active)
+    OX.refine(list(OX.active))
+    Y = A.FESpace1D(m, p, False, X.active)
+    OY = O.Space(T, p, False, OX.active)
+    for _ in range(6):
+        for _inner in range(2):
+            marked = [int(c) for c in rng.choice(OY.active, size=max(1, len(OY.active) // 5), replace=False)]
+            Y.refine(marked)
+            OY.refine(marked)
+            _same_space(Y, OY)
+            np.testing.assert_array_equal(X.ancestors_of(Y), OY.ancestors_in(OX))
+        marked = [int(c) for c in rng.choice(OX.active, size=max(1, len(OX.active) // 5), replace=False)]
+        X.refine(marked)
+        OX.refine(marked)
+        Y = A.FESpace1D(m, p, False, X.active)
+        OY = O.Space(T, p, False, OX.active)
+        _same_tree(m, T)
+        _same_space(X, OX)
+        _same_space(Y, OY)
+        np.testing.assert_array_equal(X.ancestors_of(Y), OY.ancestors_in(OX))
+
+
+def _eval(space, coef, pts):
+    """values of the true-DoF vector `coef` at points (x, t), evaluated from every active cell
+    whose closure contains the point: list of arrays per point"""
+    p = space.p
+    ptr, dof, w = space.restr_ptr, space.restr_dof, space.restr_w
+    NL = (p + 1) ** 2
+    out = [[] for _ in pts]
+    for a in range(space.n_cells):
+        x0, x1, t0, t1 = space.box[a]
+        loc = np.array([np.dot(w[ptr[a * NL + l]:ptr[a * NL + l + 1]], coef[dof[ptr[a * NL + l]:ptr[a * NL + l + 1]]])
+                        for l in range(NL)])
+        for k, (x, t) in enumerate(pts):
+            if x0 - 1e-14 <= x <= x1 + 1e-14 and t0 - 1e-14 <= t <= t1 + 1e-14:
+                vs, _ = segment_lagrange(p, [(x - x0) / (x1 - x0)])
+                vt, _ = segment_lagrange(p, [(t - t0) / (t1 - t0)])
+                out[k].append(float(np.outer(vs[0], vt[0]).ravel() @ loc))
+    return out
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_spaces_are_conforming(p):
+    """every X basis function is continuous on all face points, every Y one across spatial
+    faces: the hanging-node weights are right"""
+    rng = np.random.default_rng(7)
+    m = A.Mesh1D()
+    X = A.FESpace1D(m, p, True, m.leaf_cells())
+    X.refine(X.active)
+    for _ in range(4):
+        X.refine(rng.choice(X.active, size=max(1, X.n_cells // 4), replace=False))
+    Y = A.FESpace1D(m, p, False, X.active)
+    Y.refine(rng.choice(Y.active, size=max(1, Y.n_cells // 4), replace=False))
+    assert X.n_constrained > 0 and Y.n_constrained > 0
+    s = np.linspace(0.0, 1.0, 2 * p + 3)
+    for space, across_time in ((X, True), (Y, False)):
+        pts = []
+        for x0, x1, t0, t1 in space.box:
+            for v in s[1:-1]:
+                pts += [(x0, t0 + v * (t1 - t0)), (x1, t0 + v * (t1 - t0))]  # spatial faces
+                if across_time:
+                    pts += [(x0 + v * (x1 - x0), t0), (x0 + v * (x1 - x0), t1)]
+        for d in rng.choice(space.n_dofs, size=min(space.n_dofs, 12), replace=False):
+            coef = np.zeros(space.n_dofs)
+            coef[d] = 1.0
+            for vals in _eval(space, coef, pts):
+                assert len(vals) >= 1
+                assert max(vals) - min(vals) < 1e-12
+
+
+@pytest.mark.parametrize("p,steps", [(1, 17), (2, 12), (3, 10)])
+def test_oracle_reproduces_published_adaptive_runs(p, steps):
+    gold = [a for a in json.load(open(GOLD)) if a["p"] == p][:steps]
+    assert len(gold) == steps
+    run = O.adaptive_x_run(p, steps)
+    for rec, a in zip(run, gold):
+        assert (rec["x_true_dofs"], rec["y_true_dofs"]) == (a["x_true_dofs"], a["y_true_dofs"]), a["source"]
+        assert rec["eta2"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
+        assert rec["lambda_y2"] == pytest.approx(a["lambda_y_squared"], rel=6e-6), a["source"]
+    for k in range(steps - 1):
+        assert gold[k]["x_true_dofs_after"] == run[k + 1]["x_true_dofs"]
+
+
+def test_oracle_matches_uniform_anchors():
+    """x_uniform_levels = L gives the uniform runs' meshes (published totals, anchors.json)"""
+    anchors = json.load(open(os.path.join(os.path.dirname(GOLD), "anchors.json")))
+    for a in [a for a in anchors if a["dim"] == 1 and a["p"] == 1 and a["outer"] <= 2]:
+        run = O.adaptive_x_run(1, 1, x_uniform_levels=a["outer"] + 1)
+        assert run[0]["x_true_dofs"] == a["x_true_dofs"]
+        assert run[0]["eta2"] == pytest.approx(a["eta_squared"], rel=6e-6)
+        assert run[0]["lambda_y2"] == pytest.approx(a["lambda_y_squared"], rel=6e-6)

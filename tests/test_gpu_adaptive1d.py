@@ -1,0 +1,70 @@
+"""The adaptive 1+1D X loop on the GPU (AdaptiveX1D: the library's reference numbering,
+GPU element-quadrature assembly, caller's-CSR MINRES, GPU X indicators, Doerfler marking)
+against the published adaptive runs and against the oracle on the same spaces:
+
+* every step's X / Y true-DoF counts equal the published ones, and eta^2, lambda_Y^2 match
+  them to their 6 published digits (algorithm_data_plot/1d/smooth_initial.csv:336-352 and
+  the p = 2, 3 rows of tests/golden/adaptive1d_anchors.json);
+* every step's K and b equal the oracle's element assembly on the same spaces to 1e-12
+  (relative to max |K|), and the GPU indicators of the GPU solution equal the oracle's
+  indicators of that solution to 1e-10 per cell.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import doubleadapparabolicfem_amd as apf
+from doubleadapparabolicfem_amd import adaptive1d as A
+from oracle import adaptive1d as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adaptive1d_anchors.json")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return apf.Context(0)
+
+
+def u0(x):
+    return np.sin(np.pi * x)
+
+
+@pytest.mark.parametrize("p,steps", [(1, 17), (2, 12), (3, 10)])
+def test_gpu_adaptive_run_reproduces_published_history(ctx, p, steps):
+    gold = [a for a in json.load(open(GOLD)) if a["p"] == p][:steps]
+    run = A.AdaptiveX1D(ctx, p=p, theta_x=0.4, tol=1e-10, max_it=40000)
+    T = O.Tree()
+    OX = O.Space(T, p, True, T.leaves())
+    OX.refine(list(OX.active))
+    for a in gold:
+        print(f"p={p} outer={a['outer']} X={run.X.n_dofs}", flush=True)
+        OY = O.Space(T, p, False, OX.active)
+        assert (run.X.n_dofs, run.Y.n_dofs) == (a["x_true_dofs"], a["y_true_dofs"]), a["source"]
+        assert list(run.X.active) == OX.active and list(run.Y.active) == OY.active
+        # assembly parity on these spaces
+        pair = A.SpaceTimePair(run.X, run.Y)
+        gs = apf.GeneralSystem(ctx, pair)
+        gs.assemble(u0)
+        rowptr, col, vals, b = gs.download()
+        Ko, bo = O.assemble(OX, OY, u0)
+        Kg = sp.csr_matrix((vals, col, rowptr), shape=Ko.shape)
+        scale = abs(Ko).max()
+        assert abs(Kg - Ko).max() <= 1e-12 * scale
+        assert np.abs(b - bo).max() <= 1e-12 * max(1.0, np.abs(bo).max())
+        del gs
+        rec = run.step()
+        assert rec["minres_rel_residual"] <= 1e-10
+        assert rec["eta_squared"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
+        assert rec["lambda_y_squared"] == pytest.approx(a["lambda_y_squared"], rel=6e-6), a["source"]
+        # indicators of the GPU solution, oracle vs GPU
+        l2o, t2o = O.x_indicators(OX, OY, rec["solution"], u0)
+        lxo = np.zeros(len(OX.active))
+        np.add.at(lxo, OY.ancestors_in(OX), l2o)
+        np.testing.assert_allclose(rec["eta2_by_x_cell"], lxo + t2o, rtol=1e-10, atol=1e-14)
+        assert run.X.n_dofs == a["x_true_dofs_after"], a["source"]
+        OX.refine([int(c) for c in rec["marked_x_cells"]])
