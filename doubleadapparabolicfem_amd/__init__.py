@@ -222,6 +222,9 @@ EXPORTED = {
     "apf_fespace1d_ancestors": (ctypes.c_int, [ctypes.c_void_p] * 3),
     "apf_general_x_indicators": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(_GeneralCoeff),
                                                 ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_general_graph_norm": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_GeneralCoeff),
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+    "apf_csr_matrix_download": (ctypes.c_int, [ctypes.c_void_p] * 4),
 }
 
 
@@ -518,6 +521,15 @@ class CsrMatrix:
             _lib.apf_csr_matrix_destroy(self.handle)
             self.handle = None
 
+    def download(self):
+        """(rowptr int64 [n+1], col int32, values) of the device matrix"""
+        rowptr = np.zeros(self.n + 1, np.int64)
+        col = np.zeros(self.nnz, np.int32)
+        val = np.zeros(self.nnz)
+        _check(lib().apf_csr_matrix_download(self.handle, _ptr(rowptr), _ptr(col), _ptr(val)), self.ctx.handle,
+               "apf_csr_matrix_download")
+        return rowptr, col, val
+
     def minres(self, b, n_lambda, hhat=None, tol=1e-10, max_it=20000, check_interval=100, degree=8,
                precond=APF_FP32_REFINE, x0=None, raise_on_fail=True):
         """MINRES on this matrix (apf_csr_minres_solve); returns (x, stats)."""
@@ -672,6 +684,21 @@ class GeneralSystem:
         _check(lib().apf_general_x_indicators(self.handle, _ptr(xx), ctypes.byref(c), _ptr(lam2), _ptr(tr2)),
                self.ctx.handle, "apf_general_x_indicators")
         return lam2, tr2
+
+    def graph_norm(self, M_grid=None, grid_n=0):
+        """Hhat = -C + B_dt diag(A_y)^-1 B_dt^T + A_x as a CsrMatrix (apf_general_graph_norm),
+        after assemble()"""
+        Mg = None if M_grid is None else np.ascontiguousarray(M_grid, dtype=np.float64).ravel()
+        c = _GeneralCoeff(grid_n if Mg is not None else 0, _ptr(Mg), 0.0, None)
+        h = ctypes.c_void_p()
+        _check(lib().apf_general_graph_norm(self.handle, ctypes.byref(c), ctypes.byref(h)), self.ctx.handle,
+               "apf_general_graph_norm")
+        M = CsrMatrix.__new__(CsrMatrix)
+        M.ctx, M.handle = self.ctx, h
+        n, nnz = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().apf_csr_matrix_info(h, ctypes.byref(n), ctypes.byref(nnz)))
+        M.n, M.nnz = n.value, nnz.value
+        return M
 
     def matrix(self):
         """a CsrMatrix view of K (valid while this system lives)"""

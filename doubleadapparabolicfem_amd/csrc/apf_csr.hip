@@ -723,6 +723,21 @@ int apf_csr_matrix_info(const apf_csr_matrix* M, int64_t* n, int64_t* nnz)
     return APF_OK;
 }
 
+int apf_csr_matrix_download(const apf_csr_matrix* M, int64_t* rowptr, int* col, double* values)
+{
+    if (!M) return fail(nullptr, APF_BAD_ARG, "apf_csr_matrix_download: null matrix");
+    try {
+        HIP_TRY(hipSetDevice(M->ctx->device));
+        HIP_TRY(hipStreamSynchronize(M->ctx->stream));
+        if (rowptr) HIP_TRY(hipMemcpy(rowptr, M->rowptr, (M->n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (col && M->nnz) HIP_TRY(hipMemcpy(col, M->col, M->nnz * sizeof(int), hipMemcpyDeviceToHost));
+        if (values && M->nnz) HIP_TRY(hipMemcpy(values, M->val, M->nnz * sizeof(double), hipMemcpyDeviceToHost));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return fail(M->ctx, APF_HIP_ERROR, std::string("apf_csr_matrix_download: ") + e.what());
+    }
+}
+
 int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, const apf_csr_matrix* Hhat,
                          const double* b, double* x, const double* x0, double tol, int max_it, int check_interval,
                          int cheb_degree, int precond_prec, apf_minres_stats* stats)

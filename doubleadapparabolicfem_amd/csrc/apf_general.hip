@@ -551,6 +551,194 @@ __global__ void __launch_bounds__(256) k_gen_trace(GenDev d, const double* __res
     out[xc] = acc;
 }
 
+// Parabolic graph-norm parts on each Y cell (assemble_parabolic_graph_norm_approximation,
+// assembly/main_system/parabolic_graph_norm.hpp:22-65): B_dt(i, j) = int dpsi_i/dt phi_j
+// (assemble_mat_B_dt, mat_B.hpp:250-273: the time-derivative part of B, with the X basis
+// of the cell's ancestor at the mapped points) into BT = B_dt^T (rows Y dofs), and
+// A_x(i, k) = int M grad_x psi_i . grad_x psi_k (assemble_mat_A on the X space, mat_A.hpp)
+// into Hhat, integrated over the Y cells that tile each X cell.  Local values with
+// |v| <= 1e-15 are skipped (scatter.hpp:92).
+struct GraphDev {
+    const int64_t* bt_rowptr;
+    const int* bt_col;
+    double* bt_val;
+    const int64_t* h_rowptr;
+    const int* h_col;
+    double* h_val;
+};
+
+__device__ inline void add_csr(const int64_t* rowptr, const int* col, double* val, int* err, int64_t r, int c,
+                               double v)
+{
+    const int64_t s = find_slot(rowptr, col, r, c);
+    if (s < 0) {
+        atomicAdd(err, 1);
+        return;
+    }
+    atomicAdd(&val[s], v);
+}
+
+template <int DIM, int P>
+__global__ void __launch_bounds__(256) k_gen_graph(GenDev d, GraphDev g)
+{
+    using S = GS<DIM, P>;
+    constexpr int NS = S::NS, NT = S::NT, NL = S::NL, NQS = S::NQS, NQT = S::NQT, MD = S::MD, CB = S::CB;
+    __shared__ double t_lam[NQS * NS], t_w[NQS], t_x[NQS * DIM];
+    __shared__ double t_tw[NQT], t_tx[NQT], t_ell[NQT * NT];
+    __shared__ double g_det[CB], g_t0[CB], g_h[CB];
+    __shared__ double gx_v0[CB][DIM], gx_Ji[CB][MD], gx_t0[CB], gx_h[CB];
+    __shared__ double q_M[CB * NQS * MD], q_mu[CB * NQS], q_lamX[CB * NQS * NS], q_gX[CB * NQS * NS * DIM];
+    __shared__ double MXY[CB * NS * NS], KXX[CB * NS * NS], DtXY[CB * NT * NT], MtXX[CB * NT * NT];
+
+    const int tid = threadIdx.x;
+    const int64_t cbase = static_cast<int64_t>(blockIdx.x) * CB;
+    const int ncb = static_cast<int>(min(static_cast<int64_t>(CB), d.nyc - cbase));
+    for (int e = tid; e < NQS * NS; e += blockDim.x) t_lam[e] = d.lamY[e];
+    for (int e = tid; e < NQS; e += blockDim.x) t_w[e] = d.sq_w[e];
+    for (int e = tid; e < NQS * DIM; e += blockDim.x) t_x[e] = d.sq_x[e];
+    for (int e = tid; e < NQT; e += blockDim.x) {
+        t_tw[e] = d.tq_w[e];
+        t_tx[e] = d.tq_x[e];
+    }
+    for (int e = tid; e < NQT * NT; e += blockDim.x) t_ell[e] = d.ellY[e];
+    if (tid < ncb) {
+        const int64_t c = cbase + tid;
+        const double* gy = d.ygeo + c * kGeo;
+        double v0[DIM], Ji[MD];
+        g_det[tid] = cell_map<DIM>(gy, v0, Ji);
+        g_t0[tid] = gy[kGeo - 2];
+        g_h[tid] = gy[kGeo - 1] - gy[kGeo - 2];
+        const double* gxp = d.xgeo + static_cast<int64_t>(d.anc[c]) * kGeo;
+        (void)cell_map<DIM>(gxp, gx_v0[tid], gx_Ji[tid]);
+        gx_t0[tid] = gxp[kGeo - 2];
+        gx_h[tid] = gxp[kGeo - 1] - gxp[kGeo - 2];
+    }
+    __syncthreads();
+    for (int e = tid; e < ncb * NQS; e += blockDim.x) {
+        const int cl = e / NQS, q = e % NQS;
+        double xq[DIM], xi[DIM];
+        const double* gy = d.ygeo + (cbase + cl) * kGeo;
+        if constexpr (DIM == 1) {
+            xq[0] = gy[0] + (gy[1] - gy[0]) * t_x[q];
+            xi[0] = gx_Ji[cl][0] * (xq[0] - gx_v0[cl][0]);
+        } else {
+            const double a = t_x[2 * q], bb = t_x[2 * q + 1];
+            xq[0] = gy[0] + (gy[2] - gy[0]) * a + (gy[4] - gy[0]) * bb;
+            xq[1] = gy[1] + (gy[3] - gy[1]) * a + (gy[5] - gy[1]) * bb;
+            const double dx = xq[0] - gx_v0[cl][0], dy = xq[1] - gx_v0[cl][1];
+            xi[0] = gx_Ji[cl][0] * dx + gx_Ji[cl][1] * dy;
+            xi[1] = gx_Ji[cl][2] * dx + gx_Ji[cl][3] * dy;
+        }
+        M_at<DIM>(d, xq, &q_M[e * MD]);
+        q_mu[e] = t_w[q] * g_det[cl];
+        double v[NS], gr[NS * DIM];
+        spatial_lagrange<DIM, P>(xi, v, gr);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            q_lamX[e * NS + s] = v[s];
+            if constexpr (DIM == 1) {
+                q_gX[e * NS + s] = gx_Ji[cl][0] * gr[s];
+            } else {
+                q_gX[(e * NS + s) * 2] = gx_Ji[cl][0] * gr[2 * s] + gx_Ji[cl][2] * gr[2 * s + 1];
+                q_gX[(e * NS + s) * 2 + 1] = gx_Ji[cl][1] * gr[2 * s] + gx_Ji[cl][3] * gr[2 * s + 1];
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < ncb * NS * NS; e += blockDim.x) {
+        const int cl = e / (NS * NS), s = (e / NS) % NS, sp = e % NS;
+        double kxx = 0.0, mxy = 0.0;
+        for (int q = 0; q < NQS; ++q) {
+            const int pe = cl * NQS + q;
+            const double mu = q_mu[pe];
+            const double* M = &q_M[pe * MD];
+            if constexpr (DIM == 1) {
+                kxx += mu * q_gX[pe * NS + s] * M[0] * q_gX[pe * NS + sp];
+            } else {
+                const double* ga = &q_gX[(pe * NS + s) * 2];
+                const double* gb = &q_gX[(pe * NS + sp) * 2];
+                kxx += mu * (ga[0] * (M[0] * gb[0] + M[1] * gb[1]) + ga[1] * (M[2] * gb[0] + M[3] * gb[1]));
+            }
+            mxy += mu * q_lamX[pe * NS + s] * t_lam[q * NS + sp];
+        }
+        KXX[e] = kxx;
+        MXY[e] = mxy;
+    }
+    for (int e = tid; e < ncb * NT * NT; e += blockDim.x) {
+        const int cl = e / (NT * NT), t = (e / NT) % NT, tp = e % NT;
+        const double h = g_h[cl], hx = gx_h[cl];
+        double dxy = 0.0, mxx = 0.0;
+        for (int q = 0; q < NQT; ++q) {
+            const double tx = (g_t0[cl] + h * t_tx[q] - gx_t0[cl]) / hx;
+            double lx[P + 1], dlx[P + 1];
+            seg_lagrange<P>(tx, lx, dlx);
+            const double w = h * t_tw[q];
+            dxy += w * (dlx[t] / hx) * t_ell[q * NT + tp];
+            mxx += w * lx[t] * lx[tp];
+        }
+        DtXY[e] = dxy;
+        MtXX[e] = mxx;
+    }
+    __syncthreads();
+    for (int e = tid; e < ncb * NL * NL; e += blockDim.x) {
+        const int cl = e / (NL * NL), i = (e / NL) % NL, j = e % NL;
+        const int s = i / NT, t = i % NT, sp = j / NT, tp = j % NT;
+        const int64_t c = cbase + cl, xc = d.anc[c];
+        const int m0 = cl * NS * NS + s * NS + sp, t0 = cl * NT * NT + t * NT + tp;
+        // B_dt(X local i of the ancestor, Y local j) into BT(Y, X)
+        const double bv = MXY[m0] * DtXY[t0];
+        if (fabs(bv) > kZeroTol) {
+            for (int64_t ei = d.xptr[xc * NL + i]; ei < d.xptr[xc * NL + i + 1]; ++ei)
+                for (int64_t ej = d.yptr[c * NL + j]; ej < d.yptr[c * NL + j + 1]; ++ej)
+                    add_csr(g.bt_rowptr, g.bt_col, g.bt_val, d.err, d.ydof[ej], d.xdof[ei], d.xw[ei] * d.yw[ej] * bv);
+        }
+        // A_x(X local i, X local j) over this Y cell
+        const double av = KXX[m0] * MtXX[t0];
+        if (fabs(av) > kZeroTol) {
+            for (int64_t ei = d.xptr[xc * NL + i]; ei < d.xptr[xc * NL + i + 1]; ++ei)
+                for (int64_t ej = d.xptr[xc * NL + j]; ej < d.xptr[xc * NL + j + 1]; ++ej)
+                    add_csr(g.h_rowptr, g.h_col, g.h_val, d.err, d.xdof[ei], d.xdof[ej], d.xw[ei] * d.xw[ej] * av);
+        }
+    }
+}
+
+// Hhat += -C_signed (K's X-X block) and, per Y dof j, the outer product of column j of B_dt
+// scaled by 1 / A_y(j, j) (diagonal_schur_approximation_with_diagnostics,
+// linear_algebra/preconditioners/schur_approximations.hpp:80-212).  A_y(j, j) <= 1e-14
+// sets the error flag (the reference throws "A_y diagonal must be positive").
+__global__ void k_hhat_schur(GenDev d, GraphDev g)
+{
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= d.nY) return;
+    const int64_t s = find_slot(d.rowptr, d.col, j, static_cast<int>(j));
+    const double diag = s >= 0 ? d.val[s] : 0.0;
+    if (!(diag > 1.0e-14)) {
+        atomicAdd(d.err + 1, 1);
+        return;
+    }
+    const double inv = 1.0 / diag;
+    for (int64_t a = g.bt_rowptr[j]; a < g.bt_rowptr[j + 1]; ++a) {
+        const double va = g.bt_val[a];
+        if (va == 0.0) continue;
+        for (int64_t b = g.bt_rowptr[j]; b < g.bt_rowptr[j + 1]; ++b) {
+            const double v = inv * va * g.bt_val[b];
+            if (v != 0.0) add_csr(g.h_rowptr, g.h_col, g.h_val, d.err, g.bt_col[a], g.bt_col[b], v);
+        }
+    }
+}
+
+__global__ void k_hhat_trace(GenDev d, GraphDev g, int64_t nX)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nX) return;
+    const int64_t r = d.nY + i;
+    for (int64_t e = d.rowptr[r]; e < d.rowptr[r + 1]; ++e) {
+        const int c = d.col[e];
+        if (c < d.nY || d.val[e] == 0.0) continue;
+        add_csr(g.h_rowptr, g.h_col, g.h_val, d.err, i, static_cast<int>(c - d.nY), -d.val[e]);
+    }
+}
+
 __global__ void k_zero(int64_t n, double* __restrict__ a)
 {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -702,6 +890,47 @@ void prepare(const apf_cell_space_desc* Xd, const apf_cell_space_desc* Yd, const
     }
     for (int64_t c = 0; c < P.X.nc; ++c)
         if (P.X.geo[c * kGeo + kGeo - 2] == 0.0) P.bottom.push_back(static_cast<int>(c));
+}
+
+// Patterns of B_dt^T (Y rows) and Hhat (X rows): the X dofs of each Y cell's ancestor
+// against the Y dofs of the cell, and the X-X pairs of the ancestors (A_x, C) and of the
+// columns of B_dt (the outer products).
+void graph_patterns(const Prepared& P, std::vector<int64_t>& bt_rp, std::vector<int>& bt_col,
+                    std::vector<int64_t>& h_rp, std::vector<int>& h_col)
+{
+    const int nl = n_local(P.X.dim, P.X.p);
+    std::vector<int> xs, ys;
+    std::vector<std::vector<int>> bt(P.Y.ndofs), hh(P.X.ndofs);
+    for (int64_t c = 0; c < P.Y.nc; ++c) {
+        cell_dofs(P.Y, c, ys);
+        cell_dofs(P.X, P.anc[c], xs);
+        for (int gy : ys) bt[gy].insert(bt[gy].end(), xs.begin(), xs.end());
+    }
+    for (auto& r : bt) {
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+    }
+    for (int64_t a = 0; a < P.X.nc; ++a) {
+        cell_dofs(P.X, a, xs);
+        for (int gi : xs) hh[gi].insert(hh[gi].end(), xs.begin(), xs.end());
+    }
+    for (const auto& r : bt)
+        for (int gi : r) hh[gi].insert(hh[gi].end(), r.begin(), r.end());
+    bt_rp.assign(1, 0);
+    bt_col.clear();
+    for (const auto& r : bt) {
+        bt_col.insert(bt_col.end(), r.begin(), r.end());
+        bt_rp.push_back(static_cast<int64_t>(bt_col.size()));
+    }
+    h_rp.assign(1, 0);
+    h_col.clear();
+    for (auto& r : hh) {
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        h_col.insert(h_col.end(), r.begin(), r.end());
+        h_rp.push_back(static_cast<int64_t>(h_col.size()));
+    }
+    (void)nl;
 }
 
 }  // namespace
@@ -1075,6 +1304,122 @@ int apf_general_x_indicators(apf_general_system* sys, const double* x, const apf
         if (dx) (void)hipFree(dx);
         if (dout) (void)hipFree(dout);
         return fail(ctx, APF_HIP_ERROR, std::string("apf_general_x_indicators: ") + e.what());
+    }
+}
+
+int apf_general_graph_norm(apf_general_system* sys, const apf_general_coeff* coeff, apf_csr_matrix** out)
+{
+    if (!sys || !coeff || !out) return fail(sys ? sys->ctx : nullptr, APF_BAD_ARG, "apf_general_graph_norm: null argument");
+    apf_ctx* ctx = sys->ctx;
+    if (!sys->assembled) return fail(ctx, APF_BAD_ARG, "apf_general_graph_norm: assemble the system first");
+    if (coeff->grid_n < 0 || (coeff->grid_n > 0 && !coeff->M_grid))
+        return fail(ctx, APF_BAD_ARG, "apf_general_graph_norm: bad M grid");
+    std::vector<void*> tmp;
+    auto M = std::make_unique<apf_csr_matrix>();
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        std::vector<int64_t> bt_rp, h_rp;
+        std::vector<int> bt_col, h_col;
+        graph_patterns(sys->host, bt_rp, bt_col, h_rp, h_col);
+        auto dalloc = [&](size_t bytes) {
+            void* q = nullptr;
+            HIP_TRY(hipMalloc(&q, std::max<size_t>(bytes, 8)));
+            return q;
+        };
+        GraphDev g{};
+        void* p_btrp = dalloc(bt_rp.size() * sizeof(int64_t));
+        tmp.push_back(p_btrp);
+        void* p_btcol = dalloc(bt_col.size() * sizeof(int));
+        tmp.push_back(p_btcol);
+        void* p_btval = dalloc(bt_col.size() * sizeof(double));
+        tmp.push_back(p_btval);
+        void* p_err = dalloc(2 * sizeof(int));
+        tmp.push_back(p_err);
+        M->ctx = ctx;
+        M->n = sys->nX;
+        M->nnz = static_cast<int64_t>(h_col.size());
+        M->rowptr = static_cast<int64_t*>(dalloc(h_rp.size() * sizeof(int64_t)));
+        M->col = static_cast<int*>(dalloc(std::max<size_t>(1, h_col.size()) * sizeof(int)));
+        M->val = static_cast<double*>(dalloc(std::max<size_t>(1, h_col.size()) * sizeof(double)));
+        M->owned = true;
+        HIP_TRY(hipMemcpyAsync(p_btrp, bt_rp.data(), bt_rp.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        if (!bt_col.empty())
+            HIP_TRY(hipMemcpyAsync(p_btcol, bt_col.data(), bt_col.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(M->rowptr, h_rp.data(), h_rp.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        if (!h_col.empty())
+            HIP_TRY(hipMemcpyAsync(M->col, h_col.data(), h_col.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        g.bt_rowptr = static_cast<int64_t*>(p_btrp);
+        g.bt_col = static_cast<int*>(p_btcol);
+        g.bt_val = static_cast<double*>(p_btval);
+        g.h_rowptr = M->rowptr;
+        g.h_col = M->col;
+        g.h_val = M->val;
+        GenDev d = sys->d;
+        d.err = static_cast<int*>(p_err);
+        d.grid_n = coeff->grid_n;
+        if (coeff->grid_n > 0) {
+            int64_t len = 1;
+            for (int a = 0; a < sys->dim; ++a) len *= coeff->grid_n;
+            len *= sys->dim * sys->dim;
+            if (len > sys->Mgrid_len) {
+                sys->Mgrid = sys->alloc<double>(static_cast<size_t>(len));
+                sys->Mgrid_len = len;
+            }
+            HIP_TRY(hipMemcpyAsync(sys->Mgrid, coeff->M_grid, len * sizeof(double), hipMemcpyHostToDevice, st));
+            d.Mgrid = sys->Mgrid;
+        }
+        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, st, static_cast<int64_t>(bt_col.size()), g.bt_val);
+        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, st, M->nnz, M->val);
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, static_cast<int*>(p_err));
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, static_cast<int*>(p_err) + 1);
+        auto launch = [&](auto dimc, auto pc) {
+            constexpr int D = decltype(dimc)::value, PP = decltype(pc)::value;
+            using S = GS<D, PP>;
+            if (sys->nyc > 0)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gen_graph<D, PP>),
+                                   dim3(static_cast<unsigned>((sys->nyc + S::CB - 1) / S::CB)), dim3(256), 0, st, d, g);
+        };
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        if (sys->dim == 1) {
+            if (sys->p == 1) launch(I1{}, I1{});
+            else if (sys->p == 2) launch(I1{}, I2{});
+            else launch(I1{}, I3{});
+        } else {
+            if (sys->p == 1) launch(I2{}, I1{});
+            else if (sys->p == 2) launch(I2{}, I2{});
+            else launch(I2{}, I3{});
+        }
+        if (sys->nY > 0)
+            hipLaunchKernelGGL(k_hhat_schur, dim3(static_cast<unsigned>((sys->nY + 255) / 256)), dim3(256), 0, st, d, g);
+        if (sys->nX > 0)
+            hipLaunchKernelGGL(k_hhat_trace, dim3(static_cast<unsigned>((sys->nX + 255) / 256)), dim3(256), 0, st, d, g,
+                               sys->nX);
+        HIP_TRY(hipGetLastError());
+        int err[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(err, p_err, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (void* q : tmp) (void)hipFree(q);
+        tmp.clear();
+        if (err[1]) {
+            apf_csr_matrix_destroy(M.release());
+            return fail(ctx, APF_BAD_ARG, "diagonal_schur_approximation: A_y diagonal must be positive.");
+        }
+        if (err[0]) {
+            apf_csr_matrix_destroy(M.release());
+            return fail(ctx, APF_BAD_ARG, "apf_general_graph_norm: " + std::to_string(err[0]) +
+                                              " contributions outside the pattern");
+        }
+        *out = M.release();
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        for (void* q : tmp) (void)hipFree(q);
+        if (M->rowptr) (void)hipFree(M->rowptr);
+        if (M->col) (void)hipFree(M->col);
+        if (M->val) (void)hipFree(M->val);
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_general_graph_norm: ") + e.what());
     }
 }
 

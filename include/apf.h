@@ -253,6 +253,9 @@ typedef struct apf_csr_matrix apf_csr_matrix;
  * Eigen::SparseMatrix<double, ColMajor, int> after makeCompressed() (outerIndexPtr [n+1],
  * innerIndexPtr, valuePtr: src/linear_algebra/eigen_backend/sparse_matrix.hpp:29-37).
  * Replaces handing K (and the graph-norm Hhat) to the Eigen MINRES. */
+/* Copy a matrix's CSR arrays (row pointer [n+1] as int64, sorted columns, values) to the
+ * host (any may be NULL). */
+int apf_csr_matrix_download(const apf_csr_matrix* M, int64_t* rowptr, int* col, double* values);
 int apf_csr_matrix_create(apf_ctx* ctx, int64_t n, const int* outer, const int* inner, const double* values,
                           int storage, apf_csr_matrix** out);
 void apf_csr_matrix_destroy(apf_csr_matrix* M);
@@ -344,6 +347,17 @@ int apf_general_matrix(apf_general_system* sys, apf_csr_matrix** out);
  * :398-433) and adds trace2 to get eta^2 per X cell. */
 int apf_general_x_indicators(apf_general_system* sys, const double* x, const apf_general_coeff* coeff,
                              double* lambda_y2, double* trace2);
+/* The parabolic graph-norm approximation Hhat = -C_signed + B_dt diag(A_y)^-1 B_dt^T + A_x
+ * (X x X, symmetric positive definite, full storage) of an assembled system, built on the
+ * device: B_dt = int dpsi/dt phi and A_x = int M grad_x psi . grad_x psi by element
+ * quadrature on the Y cells (X basis of each ancestor at the mapped points), the outer
+ * products of B_dt's columns scaled by 1 / A_y(j, j), and C from K.  Replaces
+ * assemble_parabolic_graph_norm_approximation (assembly/main_system/parabolic_graph_norm.hpp:
+ * 22-65) with diagonal_schur_approximation_with_diagnostics and
+ * parabolic_graph_norm_approximation (linear_algebra/preconditioners/schur_approximations.hpp:
+ * 80-287); APF_BAD_ARG with the reference's message if an A_y diagonal is <= 1e-14.  The
+ * result is apf_csr_minres_solve's Hhat (destroy it with apf_csr_matrix_destroy). */
+int apf_general_graph_norm(apf_general_system* sys, const apf_general_coeff* coeff, apf_csr_matrix** out);
 
 /* Estimator on a non-tensor mesh.  The estimator works per time slab (the slabs between
  * the global time endpoints, time_slabs/time_slab_builder.hpp:53-120); slabs that share one

@@ -587,3 +587,51 @@ def adaptive_x_run(p, steps, theta_x=0.4, x_uniform_levels=1, u0=lambda x: np.si
                         lambda_y2=float(lam2_y.sum()), trace2=float(tr2_x.sum()), marked=list(marked)))
         X.refine(marked)
     return out
+
+
+def graph_norm(X: Space, Y: Space, K, kappa=1.0):
+    """Hhat = -C_signed + B_dt diag(A_y)^-1 B_dt^T + A_x (assemble_parabolic_graph_norm_
+    approximation, assembly/main_system/parabolic_graph_norm.hpp:22-65, with
+    schur_approximations.hpp:80-287): B_dt = int dpsi/dt phi on the Y cells (X basis of the
+    ancestor, mat_B.hpp:250-273), A_x = int kappa psi_x psi_x on the X cells with their own
+    rule (mat_A.hpp), C_signed and A_y taken from K."""
+    p = X.p
+    nq = 2 * p + 1
+    qs, ws = gauss_legendre_01(nq)
+    S, Tq = np.meshgrid(qs, qs, indexing="ij")
+    S, Tq = S.ravel(), Tq.ravel()
+    W = np.outer(ws, ws).ravel()
+    VY, _, _ = _tensor_basis(p, S, Tq)
+    _, DxX0, _ = _tensor_basis(p, S, Tq)
+    nX, nY = X.n_true, Y.n_true
+    NL = (p + 1) ** 2
+    anc = Y.ancestors_in(X)
+    rX, rY = X.restriction(), Y.restriction()
+
+    def expand(r, cell):
+        ptr, dof, w = r
+        P = np.zeros((NL, max(nX, nY)))
+        for l in range(NL):
+            for a in range(ptr[cell * NL + l], ptr[cell * NL + l + 1]):
+                P[l, dof[a]] += w[a]
+        return P
+
+    Bdt = np.zeros((nX, nY))
+    for yc in range(len(Y.active)):
+        x0, x1, t0, t1 = Y.tree.box(Y.active[yc])
+        X0, X1, T0, T1 = X.tree.box(X.active[anc[yc]])
+        xs, ts = x0 + (x1 - x0) * S, t0 + (t1 - t0) * Tq
+        _, _, DtX = _tensor_basis(p, (xs - X0) / (X1 - X0), (ts - T0) / (T1 - T0))
+        loc = np.einsum("q,qi,qj->ij", W * (x1 - x0) * (t1 - t0), DtX / (T1 - T0), VY)
+        Bdt += expand(rX, anc[yc])[:, :nX].T @ loc @ expand(rY, yc)[:, :nY]
+    Ax = np.zeros((nX, nX))
+    for xc in range(len(X.active)):
+        X0, X1, T0, T1 = X.tree.box(X.active[xc])
+        g = DxX0 / (X1 - X0)
+        loc = np.einsum("q,qi,qj->ij", W * (X1 - X0) * (T1 - T0) * kappa, g, g)
+        E = expand(rX, xc)[:, :nX]
+        Ax += E.T @ loc @ E
+    Kd = K.toarray()
+    Ay_diag = np.diag(Kd[:nY, :nY])
+    C = Kd[nY:, nY:]
+    return -C + (Bdt / Ay_diag[None, :]) @ Bdt.T + Ax

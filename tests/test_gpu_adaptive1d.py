@@ -68,3 +68,35 @@ def test_gpu_adaptive_run_reproduces_published_history(ctx, p, steps):
         np.testing.assert_allclose(rec["eta2_by_x_cell"], lxo + t2o, rtol=1e-10, atol=1e-14)
         assert run.X.n_dofs == a["x_true_dofs_after"], a["source"]
         OX.refine([int(c) for c in rec["marked_x_cells"]])
+
+
+@pytest.mark.parametrize("p,steps", [(1, 8), (2, 6), (3, 5)])
+def test_gpu_graph_norm_matches_oracle_and_preconditions_minres(ctx, p, steps):
+    """Hhat = -C + B_dt diag(A_y)^-1 B_dt^T + A_x on the adaptive meshes of the published run,
+    GPU against the oracle (1e-12 relative to max |Hhat|); MINRES with Hhat as the X block
+    reaches the 1e-10 true residual and the direct solve's solution"""
+    run = A.AdaptiveX1D(ctx, p=p, theta_x=0.4, tol=1e-10, max_it=40000)
+    T = O.Tree()
+    OX = O.Space(T, p, True, T.leaves())
+    OX.refine(list(OX.active))
+    for k in range(steps):
+        OY = O.Space(T, p, False, OX.active)
+        pair = A.SpaceTimePair(run.X, run.Y)
+        gs = apf.GeneralSystem(ctx, pair)
+        gs.assemble(u0)
+        H = gs.graph_norm()
+        rowptr, col, vals = H.download()
+        Hg = sp.csr_matrix((vals, col, rowptr), shape=(run.X.n_dofs, run.X.n_dofs)).toarray()
+        Ko, bo = O.assemble(OX, OY, u0)
+        Ho = O.graph_norm(OX, OY, Ko)
+        scale = np.abs(Ho).max()
+        print(f"p={p} step={k} nX={run.X.n_dofs} |Hg-Ho|/max={np.abs(Hg - Ho).max() / scale:.2e}", flush=True)
+        assert np.abs(Hg - Ho).max() <= 1e-12 * scale
+        assert np.abs(Hg - Hg.T).max() <= 1e-13 * scale
+        x, st = gs.matrix().minres(gs.rhs(), gs.nY, hhat=H, tol=1e-10, max_it=40000, precond=apf.APF_FP64)
+        assert st["true_relative_residual"] <= 1e-10
+        xo = O.solve(Ko, bo)
+        assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
+        del H, gs
+        rec = run.step()
+        OX.refine([int(c) for c in rec["marked_x_cells"]])
