@@ -589,7 +589,7 @@ def adaptive_x_run(p, steps, theta_x=0.4, x_uniform_levels=1, u0=lambda x: np.si
     return out
 
 
-def graph_norm(X: Space, Y: Space, K, kappa=1.0):
+def graph_norm(X: Space, Y: Space, K, kappa=1.0, parts=False):
     """Hhat = -C_signed + B_dt diag(A_y)^-1 B_dt^T + A_x (assemble_parabolic_graph_norm_
     approximation, assembly/main_system/parabolic_graph_norm.hpp:22-65, with
     schur_approximations.hpp:80-287): B_dt = int dpsi/dt phi on the Y cells (X basis of the
@@ -634,4 +634,5 @@ def graph_norm(X: Space, Y: Space, K, kappa=1.0):
     Kd = K.toarray()
     Ay_diag = np.diag(Kd[:nY, :nY])
     C = Kd[nY:, nY:]
-    return -C + (Bdt / Ay_diag[None, :]) @ Bdt.T + Ax
+    H = -C + (Bdt / Ay_diag[None, :]) @ Bdt.T + Ax
+    return (H, Bdt, Ax) if parts else H

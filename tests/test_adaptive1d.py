@@ -193,3 +193,38 @@ def test_oracle_matches_uniform_anchors():
         assert run[0]["x_true_dofs"] == a["x_true_dofs"]
         assert run[0]["eta2"] == pytest.approx(a["eta_squared"], rel=6e-6)
         assert run[0]["lambda_y2"] == pytest.approx(a["lambda_y_squared"], rel=6e-6)
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_oracle_graph_norm_parts(p):
+    """the oracle's B_dt and A_x against the pinned assembly: B is affine in kappa,
+    B(kappa) = B_dt + kappa B_A, so B_dt = 2 B(1) - B(2); X lies in Y when both live on one
+    mesh, so A_x = E^T A_y E with E the nodal embedding of X into Y; Hhat is symmetric
+    positive definite"""
+    T = O.Tree()
+    X = O.Space(T, p, True, T.leaves())
+    X.refine(list(X.active))
+    for _ in range(3):
+        X.refine([X.active[0], X.active[-1]])
+    Y = O.Space(T, p, False, X.active)
+    nY, nX = Y.n_true, X.n_true
+    K1, _ = O.assemble(X, Y, u0, kappa=1.0)
+    K2, _ = O.assemble(X, Y, u0, kappa=2.0)
+    H, Bdt, Ax = O.graph_norm(X, Y, K1, parts=True)
+    B1, B2 = K1.toarray()[nY:, :nY], K2.toarray()[nY:, :nY]
+    np.testing.assert_allclose(Bdt, 2.0 * B1 - B2, atol=1e-13)
+    # E[j, i] = psi_i at Y dof j's node, from the restriction rows of the shared cells
+    E = np.zeros((nY, nX))
+    NL = (p + 1) ** 2
+    rY, rX = Y.restriction(), X.restriction()
+    for a in range(len(Y.active)):
+        for l in range(NL):
+            ya, yb = rY[0][a * NL + l], rY[0][a * NL + l + 1]
+            xa, xb = rX[0][a * NL + l], rX[0][a * NL + l + 1]
+            if yb - ya == 1 and rY[2][ya] == 1.0:  # a free Y dof: its value is X's local value there
+                E[rY[1][ya], :] = 0.0
+                for e in range(xa, xb):
+                    E[rY[1][ya], rX[1][e]] = rX[2][e]
+    Ay = K1.toarray()[:nY, :nY]
+    np.testing.assert_allclose(Ax, E.T @ Ay @ E, atol=1e-12)
+    assert np.abs(H - H.T).max() < 1e-14 and np.linalg.eigvalsh(H).min() > 0.0
