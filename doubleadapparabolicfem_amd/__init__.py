@@ -220,6 +220,8 @@ EXPORTED = {
     "apf_fespace1d_constraints": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "apf_fespace1d_restriction": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "apf_fespace1d_ancestors": (ctypes.c_int, [ctypes.c_void_p] * 3),
+    "apf_fespace1d_neighbours": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                ctypes.c_void_p]),
     "apf_general_x_indicators": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(_GeneralCoeff),
                                                 ctypes.c_void_p, ctypes.c_void_p]),
     "apf_general_graph_norm": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_GeneralCoeff),

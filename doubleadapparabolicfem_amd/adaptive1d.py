@@ -133,6 +133,23 @@ class FESpace1D:
         _check(lib().apf_fespace1d_refine(self.handle, _ptr(m), len(m)), None, "apf_fespace1d_refine")
         self._refresh()
 
+    def neighbours(self, temporal: bool):
+        """[n_active, 2, max] neighbour cell ids across the spatial or temporal faces, -1 padded
+        (the mesh file's layout)"""
+        mx = ctypes.c_int()
+        _check(lib().apf_fespace1d_neighbours(self.handle, int(temporal), ctypes.byref(mx), None))
+        out = np.zeros((self.n_cells, 2, mx.value), np.int32)
+        _check(lib().apf_fespace1d_neighbours(self.handle, int(temporal), ctypes.byref(mx), _ptr(out)))
+        return out
+
+    def local_coefficients(self, coef):
+        """[n_active, (p+1)^2] local values of the function with true-DoF vector coef"""
+        coef = np.asarray(coef, dtype=np.float64)
+        vals = np.zeros(len(self.restr_ptr) - 1)
+        seg = np.repeat(np.arange(len(vals)), np.diff(self.restr_ptr))
+        np.add.at(vals, seg, self.restr_w * coef[self.restr_dof])
+        return vals.reshape(self.n_cells, self.n_loc)
+
     def ancestors_of(self, fine: "FESpace1D"):
         """position in this space's active list of each active cell's first active ancestor in `fine`"""
         out = np.zeros(fine.n_cells, np.int32)

@@ -823,6 +823,30 @@ int apf_fespace1d_restriction(const apf_fespace1d* s, int64_t* ptr, int* dof, do
     return APF_OK;
 }
 
+int apf_fespace1d_neighbours(const apf_fespace1d* s, int temporal, int* max_per_face, int* flat)
+{
+    if (!s || !max_per_face) return bad("apf_fespace1d_neighbours: null argument");
+    // build_{spatial,temporal}_neighbour_lists + flatten_neighbours
+    // (fespace/io/detail/write_mesh_binary_detail.hpp:60-186): interfaces in adjacency order
+    const auto& ifs = temporal ? s->tif : s->sif;
+    std::vector<std::vector<int>> nb(s->active.size() * 2);
+    for (const Iface& f : ifs) {
+        if (f.boundary) continue;
+        if (!s->is_active(f.mc) || f.sc < 0 || !s->is_active(f.sc)) continue;
+        nb[s->pos[f.mc] * 2 + f.mf].push_back(f.sc);
+        nb[s->pos[f.sc] * 2 + f.sf].push_back(f.mc);
+    }
+    int mx = 0;
+    for (const auto& v : nb) mx = std::max(mx, static_cast<int>(v.size()));
+    *max_per_face = mx;
+    if (flat) {
+        size_t k = 0;
+        for (const auto& v : nb)
+            for (int e = 0; e < mx; ++e) flat[k++] = e < static_cast<int>(v.size()) ? v[e] : -1;
+    }
+    return APF_OK;
+}
+
 int apf_fespace1d_ancestors(const apf_fespace1d* coarse, const apf_fespace1d* fine, int* index)
 {
     if (!coarse || !fine || !index) return bad("apf_fespace1d_ancestors: null argument");

@@ -123,3 +123,95 @@ def write_cellwise_error_components_binary(estimate, path, sort_descending=True)
     o = _order(ids, total, sort_descending)
     _write(path, [np.asarray([CELLWISE_ERROR_COMPONENTS_BINARY_FORMAT_VERSION, len(total), 4], dtype=np.int32),
                   ids[o].astype(np.int32), flux[o], recon[o], resid[o], total[o]])
+
+
+# ---- 1+1D adaptive spaces (adaptive1d.FESpace1D): the reference's own cell ids and DoFs ----
+DOFS_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:11
+FUNCTION_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:12
+FUNCTION_TRUE_COEFFICIENTS_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:14
+
+
+def write_space_mesh_binary(space, path):
+    """Mesh file (format version 1) of an FE space's active cells, as
+    FESpace::write_mesh_binary writes it (fespace/io/write_mesh_binary.hpp:16-87): all mesh
+    vertices, the active cells' ids and vertex ids, and per face the neighbours across it
+    (several across a hanging face) in the adjacency's order, -1 padded."""
+    x, t = space.mesh.vertices()
+    sn = space.neighbours(temporal=False)
+    tn = space.neighbours(temporal=True)
+    cl = space.mesh.cells()
+    vids = cl["vertex_ids"][space.active]
+    header = [MESH_BINARY_FORMAT_VERSION, len(x), len(t), 1, 1, space.n_cells, 1, 2, 2, 2, 2, sn.shape[2], 2,
+              tn.shape[2]]
+    _write(path, [np.asarray(header, dtype=np.int32), x.astype(np.float64), t.astype(np.float64),
+                  space.active.astype(np.int32), vids[:, :2].astype(np.int32), vids[:, 2:].astype(np.int32),
+                  sn.astype(np.int32), tn.astype(np.int32)])
+
+
+def _local_reference_coords(p):
+    """(x, t) of local DoF i_space * (p+1) + j_time (tables/element_tables.hpp:182-236)"""
+    return np.array([(i / p, j / p) for i in range(p + 1) for j in range(p + 1)], dtype=np.float64)
+
+
+def write_dofs_binary(space, path):
+    """DoF file (format version 1) of FESpace::write_dofs_binary
+    (fespace/io/write_dofs_binary.hpp:12-72, detail/write_dofs_binary_detail.hpp:11-148):
+    header [version, n_cells, dofs_per_cell, n_dofs (true + constrained), dim_v = 2, p, p,
+    constraint entries], DoF coordinates, cell ids, cell-to-DoF table, local reference
+    coordinates, constrained flags (uint8), constraint offsets, masters, weights."""
+    p, nl = space.p, space.n_loc
+    n_dofs = space.n_dofs + space.n_constrained
+    ref = _local_reference_coords(p)
+    coords = np.zeros((n_dofs, 2))
+    seen = np.zeros(n_dofs, dtype=bool)
+    for a in range(space.n_cells):
+        x0, x1, t0, t1 = space.box[a]
+        for l in range(nl):
+            g = space.cell_dofs[a, l]
+            if g >= 0 and not seen[g]:
+                coords[g] = (x0 + (x1 - x0) * ref[l, 0], t0 + (t1 - t0) * ref[l, 1])
+                seen[g] = True
+    if not seen.all():
+        raise RuntimeError("write_dofs_binary: failed to determine coordinates for a global DoF.")
+    is_con = np.zeros(n_dofs, dtype=np.uint8)
+    is_con[space.n_dofs:] = 1
+    offsets = np.zeros(n_dofs + 1, dtype=np.int32)
+    offsets[space.n_dofs + 1:] = space.con_ptr[1:]
+    header = [DOFS_BINARY_FORMAT_VERSION, space.n_cells, nl, n_dofs, 2, p, p, len(space.con_masters)]
+    _write(path, [np.asarray(header, dtype=np.int32), coords, space.active.astype(np.int32),
+                  space.cell_dofs.astype(np.int32), ref, is_con, offsets, space.con_masters.astype(np.int32),
+                  space.con_weights.astype(np.float64)])
+
+
+def write_function_binary(space, coef, path, samples_x=32, samples_t=32):
+    """Function samples (format version 1, 1+1D) of write_function_binary
+    (fespace/io/write_function_binary.hpp:42-92): per active cell the values on a
+    samples_t x samples_x reference grid (time outer, space inner)."""
+    if samples_x < 2 or samples_t < 2:
+        raise ValueError("write_function_binary: need at least 2 samples per direction.")
+    p = space.p
+    loc = space.local_coefficients(coef)
+    xs = np.arange(samples_x) / (samples_x - 1)
+    ts = np.arange(samples_t) / (samples_t - 1)
+    nodes = np.arange(p + 1) / p
+
+    def lag(s):
+        out = np.ones((len(s), p + 1))
+        for i in range(p + 1):
+            for j in range(p + 1):
+                if j != i:
+                    out[:, i] *= (s - nodes[j]) / (nodes[i] - nodes[j])
+        return out
+
+    Lx, Lt = lag(xs), lag(ts)
+    # value(jt, ix) = sum_{i, j} c[i * (p+1) + j] Lx[ix, i] Lt[jt, j]
+    vals = np.einsum("cij,xi,tj->ctx", loc.reshape(-1, p + 1, p + 1), Lx, Lt)
+    header = [FUNCTION_BINARY_FORMAT_VERSION, space.n_cells, 1, 1, samples_x, samples_t, samples_x * samples_t]
+    _write(path, [np.asarray(header, dtype=np.int32), space.active.astype(np.int32), vals.reshape(-1)])
+
+
+def write_function_true_coefficients_binary(coef, path):
+    """True coefficients (format version 1): header [version, n], values
+    (write_function_true_coefficients_binary, fespace/io/write_function_binary.hpp:176-196)."""
+    coef = np.asarray(coef, dtype=np.float64).reshape(-1)
+    _write(path, [np.asarray([FUNCTION_TRUE_COEFFICIENTS_BINARY_FORMAT_VERSION, len(coef)], dtype=np.int32), coef])

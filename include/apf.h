@@ -472,6 +472,11 @@ int apf_fespace1d_constraints(const apf_fespace1d* s, int64_t* ptr, int* masters
  * build_cell_restriction (assembly/constrained_dofs.hpp:133-226), the rows that
  * apf_cell_space_desc takes */
 int apf_fespace1d_restriction(const apf_fespace1d* s, int64_t* ptr, int* dof, double* weight);
+/* Neighbours of each active cell across its spatial (temporal = 0) or temporal faces in
+ * the layout of the reference's mesh file (build_*_neighbour_lists + flatten_neighbours,
+ * fespace/io/detail/write_mesh_binary_detail.hpp:60-186): *max_per_face = the longest
+ * list; flat (may be NULL) [n_active][2][max_per_face] cell ids, -1 padded. */
+int apf_fespace1d_neighbours(const apf_fespace1d* s, int temporal, int* max_per_face, int* flat);
 /* for each active cell of `fine`, the position in coarse's active list of its first
  * active ancestor (find_active_ancestor_cell, assembly/detail/active_cell_locator.hpp) */
 int apf_fespace1d_ancestors(const apf_fespace1d* coarse, const apf_fespace1d* fine, int* index);

@@ -150,3 +150,126 @@ def test_cellwise_error_components_file(tmp_path):
     assert np.all(np.diff(total) <= 0)
     i10, i20 = list(ids).index(10), list(ids).index(20)
     assert i20 == i10 + 1
+
+
+# ---- 1+1D adaptive spaces with the reference's cell ids and DoFs ----
+def read_dofs(path):
+    """as load_dofs_binary (python/binary_readers.py:208-305)"""
+    with open(path, "rb") as f:
+        nc, dpc, nd, dv, ps, pt, nce = (int(v) for v in _header(f, 7, 1))
+        out = dict(n_cells=nc, dofs_per_cell=dpc, n_dofs=nd, dim_v=dv, p_space=ps, p_time=pt,
+                   dof_coords=_block(f, np.float64, nd * dv, (nd, dv)),
+                   cell_ids=_block(f, np.int32, nc),
+                   cell_to_dofs=_block(f, np.int32, nc * dpc, (nc, dpc)),
+                   local_reference_coords=_block(f, np.float64, dpc * dv, (dpc, dv)),
+                   is_constrained=_block(f, np.uint8, nd).astype(bool),
+                   constraint_offsets=_block(f, np.int32, nd + 1),
+                   constraint_masters=_block(f, np.int32, nce),
+                   constraint_weights=_block(f, np.float64, nce))
+        assert np.fromfile(f, dtype=np.uint8).size == 0
+    return out
+
+
+def _adaptive_space(p, seed=3):
+    from doubleadapparabolicfem_amd import adaptive1d as A
+
+    rng = np.random.default_rng(seed)
+    m = A.Mesh1D()
+    X = A.FESpace1D(m, p, True, m.leaf_cells())
+    X.refine(X.active)
+    for _ in range(4):
+        X.refine(rng.choice(X.active, size=max(1, X.n_cells // 4), replace=False))
+    return m, X
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_adaptive_dofs_file(tmp_path, p):
+    m, X = _adaptive_space(p)
+    assert X.n_constrained > 0
+    path = tmp_path / "dofs.bin"
+    snapshot.write_dofs_binary(X, path)
+    d = read_dofs(path)
+    assert (d["n_cells"], d["dofs_per_cell"], d["dim_v"], d["p_space"], d["p_time"]) == (X.n_cells, (p + 1) ** 2, 2, p, p)
+    assert d["n_dofs"] == X.n_dofs + X.n_constrained
+    np.testing.assert_array_equal(d["cell_ids"], X.active)
+    np.testing.assert_array_equal(d["cell_to_dofs"], X.cell_dofs)
+    # every cell's local DoFs sit at the coordinates the file gives their global DoF
+    for a in range(X.n_cells):
+        x0, x1, t0, t1 = X.box[a]
+        for l, (rx, rt) in enumerate(d["local_reference_coords"]):
+            g = d["cell_to_dofs"][a, l]
+            if g >= 0:
+                assert tuple(d["dof_coords"][g]) == (x0 + (x1 - x0) * rx, t0 + (t1 - t0) * rt)
+    # constrained DoFs: flag and masters / weights reproduce the restriction rows
+    assert not d["is_constrained"][:X.n_dofs].any() and d["is_constrained"][X.n_dofs:].all()
+    nl = (p + 1) ** 2
+    for a in range(X.n_cells):
+        for l in range(nl):
+            g = d["cell_to_dofs"][a, l]
+            lo, hi = X.restr_ptr[a * nl + l], X.restr_ptr[a * nl + l + 1]
+            if g < 0:
+                assert hi == lo
+            elif not d["is_constrained"][g]:
+                assert list(X.restr_dof[lo:hi]) == [g] and list(X.restr_w[lo:hi]) == [1.0]
+            else:
+                o0, o1 = d["constraint_offsets"][g], d["constraint_offsets"][g + 1]
+                assert list(d["constraint_masters"][o0:o1]) == list(X.restr_dof[lo:hi])
+                assert list(d["constraint_weights"][o0:o1]) == list(X.restr_w[lo:hi])
+
+
+def test_adaptive_mesh_file(tmp_path):
+    m, X = _adaptive_space(1)
+    path = tmp_path / "mesh.bin"
+    snapshot.write_space_mesh_binary(X, path)
+    d = read_mesh(path)
+    h = d["header"]
+    assert list(h[2:11]) == [1, 1, X.n_cells, 1, 2, 2, 2, 2, int(h[10])] and h[12] == 2
+    x, t = m.vertices()
+    np.testing.assert_array_equal(d["spatial_vertices"][:, 0], x)
+    np.testing.assert_array_equal(d["temporal_vertices"][:, 0], t)
+    np.testing.assert_array_equal(d["cell_ids"], X.active)
+    row = {int(c): r for r, c in enumerate(d["cell_ids"])}
+    sn, tn = d["spatial_neigh"], d["temporal_neigh"]
+    assert sn.shape[2] >= 2  # a hanging face lists both fine neighbours
+    for r, c in enumerate(d["cell_ids"]):
+        x0, x1 = x[d["spatial_ids"][r]]
+        t0, t1 = t[d["temporal_ids"][r]]
+        for f, xf in ((0, x0), (1, x1)):
+            for nbr in sn[r, f]:
+                if nbr < 0:
+                    continue
+                q = row[int(nbr)]
+                qx = x[d["spatial_ids"][q]]
+                qt = t[d["temporal_ids"][q]]
+                assert xf in qx and max(t0, qt[0]) < min(t1, qt[1])  # shares the face, overlaps in time
+                assert c in sn[q, 1 - f]
+        for f, tf in ((0, t0), (1, t1)):
+            for nbr in tn[r, f]:
+                if nbr < 0:
+                    continue
+                q = row[int(nbr)]
+                assert tf in t[d["temporal_ids"][q]]
+                assert c in tn[q, 1 - f]
+
+
+def test_adaptive_function_files(tmp_path):
+    m, X = _adaptive_space(2)
+    rng = np.random.default_rng(0)
+    coef = rng.standard_normal(X.n_dofs)
+    snapshot.write_function_binary(X, coef, tmp_path / "f.bin", samples_x=5, samples_t=4)
+    snapshot.write_function_true_coefficients_binary(coef, tmp_path / "c.bin")
+    with open(tmp_path / "f.bin", "rb") as f:
+        nc, ds, dt, nx, nt, vpc = (int(v) for v in _header(f, 6, 1))
+        assert (nc, ds, dt, nx, nt, vpc) == (X.n_cells, 1, 1, 5, 4, 20)
+        ids = _block(f, np.int32, nc)
+        vals = _block(f, np.float64, nc * vpc, (nc, nt, nx))
+        assert np.fromfile(f, dtype=np.uint8).size == 0
+    np.testing.assert_array_equal(ids, X.active)
+    loc = X.local_coefficients(coef)
+    # at the sample (xi, eta) = (0, 0) and (1, 1) the value is the local DoF there
+    np.testing.assert_allclose(vals[:, 0, 0], loc[:, 0], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(vals[:, -1, -1], loc[:, -1], rtol=0, atol=1e-13)
+    with open(tmp_path / "c.bin", "rb") as f:
+        n, = _header(f, 1, 1)
+        np.testing.assert_array_equal(_block(f, np.float64, int(n)), coef)
+        assert np.fromfile(f, dtype=np.uint8).size == 0
