@@ -451,8 +451,8 @@ __global__ void k_precond_apply(int nf, int p, int m, int64_t nY, const double* 
 // computes the NT outputs (k, l, s); the spatial row and the NT x values of each
 // neighbour are read once.
 template <int NT>
-__global__ void __launch_bounds__(256) k_ay_apply(int nf, int m, const int* __restrict__ sptr,
-                                                  const int* __restrict__ scol, const double* __restrict__ Ksp,
+__global__ void __launch_bounds__(256) k_ay_apply(int nf, int m, int w, const int* __restrict__ ecol,
+                                                  const double* __restrict__ eK,
                                                   const double* __restrict__ times, TimeTables tt,
                                                   const double* __restrict__ x, double* __restrict__ y)
 {
@@ -465,9 +465,10 @@ __global__ void __launch_bounds__(256) k_ay_apply(int nf, int m, const int* __re
 #pragma unroll
     for (int l = 0; l < NT; ++l) acc[l] = 0.0;
 #pragma unroll 4
-    for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
-        const double kh = Ksp[e] * h;
-        const int c = scol[e];
+    for (int j = 0; j < w; ++j) {
+        const int64_t e = static_cast<int64_t>(j) * nf + s;  // ELL entry j of row s
+        const double kh = eK[e] * h;
+        const int c = ecol[e];
         double xv[NT];
 #pragma unroll
         for (int lp = 0; lp < NT; ++lp) xv[lp] = xk[static_cast<int64_t>(lp) * nf + c];
@@ -485,9 +486,9 @@ __global__ void __launch_bounds__(256) k_ay_apply(int nf, int m, const int* __re
 //   B^T((k,l,s),(tn = k p + it, s')) = Msp(s,s') Dt[it][l] + (Ksp(s,s') h_k) Mt[it][l],
 // X unknown (tn, s') at nY + tn nf + s'.  One thread per (interval k, free node s).
 template <int NT>
-__global__ void __launch_bounds__(256) k_bt_residual(int nf, int m, int64_t nY, const int* __restrict__ sptr,
-                                                     const int* __restrict__ scol, const double* __restrict__ Ksp,
-                                                     const double* __restrict__ Msp, const double* __restrict__ times,
+__global__ void __launch_bounds__(256) k_bt_residual(int nf, int m, int64_t nY, int w, const int* __restrict__ ecol,
+                                                     const double* __restrict__ eK,
+                                                     const double* __restrict__ eM, const double* __restrict__ times,
                                                      TimeTables tt, const double* __restrict__ b,
                                                      const double* __restrict__ x, double* __restrict__ r0,
                                                      double* __restrict__ part)
@@ -504,9 +505,10 @@ __global__ void __launch_bounds__(256) k_bt_residual(int nf, int m, int64_t nY, 
 #pragma unroll
         for (int l = 0; l < NT; ++l) acc[l] = 0.0;
 #pragma unroll 4
-        for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
-            const double kh = Ksp[e] * h, ms = Msp[e];
-            const int c = scol[e];
+        for (int j = 0; j < w; ++j) {
+            const int64_t e = static_cast<int64_t>(j) * nf + s;  // ELL entry j of row s
+            const double kh = eK[e] * h, ms = eM[e];
+            const int c = ecol[e];
             double uv[NT];
 #pragma unroll
             for (int it = 0; it < NT; ++it) uv[it] = u[static_cast<int64_t>(it) * nf + c];
@@ -546,10 +548,10 @@ __global__ void __launch_bounds__(256) k_bt_residual(int nf, int m, int64_t nY, 
 constexpr int KR = 256;
 
 template <int NT>
-__global__ void __launch_bounds__(KR) k_kron_apply(int nf, int m, int64_t nY, const int* __restrict__ sptr,
-                                                   const int* __restrict__ scol, const double* __restrict__ Ksp,
-                                                   const double* __restrict__ Msp, const double* __restrict__ times,
-                                                   TimeTables tt, const int2* __restrict__ halo, int span_cap,
+__global__ void __launch_bounds__(KR) k_kron_apply(int nf, int m, int64_t nY, int w, const int* __restrict__ ecol,
+                                                   const double* __restrict__ eK, const double* __restrict__ eM,
+                                                   const double* __restrict__ times, TimeTables tt,
+                                                   const int2* __restrict__ halo, int span_cap,
                                                    const double* __restrict__ x, double* __restrict__ y)
 {
     extern __shared__ double kr_tile[];  // [NT][span_cap]
@@ -570,80 +572,105 @@ __global__ void __launch_bounds__(KR) k_kron_apply(int nf, int m, int64_t nY, co
             }
         __syncthreads();
     };
-    auto val = [&](const double* base, int q, int c) {
-        return staged ? kr_tile[q * span_cap + (c - lo)] : base[static_cast<int64_t>(q) * nf + c];
-    };
-    const int eb = row ? sptr[s] : 0, ee = row ? sptr[s + 1] : 0;
-    if (job < m) {
-        const int k = job;
-        const double h = times[k + 1] - times[k];
-        const double* xy = x + static_cast<int64_t>(k) * NT * nf;
-        const double* xx = x + nY + static_cast<int64_t>(k) * P * nf;
-        double acc[NT];
-#pragma unroll
-        for (int l = 0; l < NT; ++l) acc[l] = 0.0;
-        stage(xy, NT);
-#pragma unroll 4
-        for (int e = eb; e < ee; ++e) {
-            const double kh = Ksp[e] * h;
-            const int c = scol[e];
-            double v[NT];
-#pragma unroll
-            for (int q = 0; q < NT; ++q) v[q] = val(xy, q, c);
-#pragma unroll
-            for (int l = 0; l < NT; ++l)
-#pragma unroll
-                for (int q = 0; q < NT; ++q) acc[l] += (kh * tt.Mt[l * NT + q]) * v[q];
-        }
-        stage(xx, NT);
-#pragma unroll 4
-        for (int e = eb; e < ee; ++e) {
-            const double kh = Ksp[e] * h, ms = Msp[e];
-            const int c = scol[e];
-            double v[NT];
-#pragma unroll
-            for (int q = 0; q < NT; ++q) v[q] = val(xx, q, c);
-#pragma unroll
-            for (int l = 0; l < NT; ++l)
-#pragma unroll
-                for (int q = 0; q < NT; ++q) acc[l] += (ms * tt.Dt[q * NT + l] + kh * tt.Mt[q * NT + l]) * v[q];
-        }
-        if (row)
-#pragma unroll
-            for (int l = 0; l < NT; ++l) y[(static_cast<int64_t>(k) * NT + l) * nf + s] = acc[l];
-    } else {
-        const int tn = job - m;
-        double acc = 0.0;
-        // the intervals at time node tn, with tn's index it inside each
-        int ks[2], its[2], nk = 0;
-        if (tn % P == 0) {
-            if (tn / P - 1 >= 0) { ks[nk] = tn / P - 1; its[nk] = P; ++nk; }
-            if (tn / P < m) { ks[nk] = tn / P; its[nk] = 0; ++nk; }
-        } else {
-            ks[0] = tn / P;
-            its[0] = tn % P;
-            nk = 1;
-        }
-        for (int bk = 0; bk < nk; ++bk) {
-            const int k = ks[bk], it = its[bk];
+    // ELL rows: entry j of row s at j * nf + s (coalesced over the wave's rows)
+    const int ee = row ? w : 0;
+    // the job's work with x plane q at column c read by get(base, q, c): from the LDS tile
+    // or from global memory, two instantiations (so no pointer select reaches the loads)
+    auto body = [&](auto get) {
+        if (job < m) {
+            const int k = job;
             const double h = times[k + 1] - times[k];
             const double* xy = x + static_cast<int64_t>(k) * NT * nf;
-            stage(xy, NT);
-    #pragma unroll 4
-        for (int e = eb; e < ee; ++e) {
-                const double kh = Ksp[e] * h, ms = Msp[e];
-                const int c = scol[e];
+            const double* xx = x + nY + static_cast<int64_t>(k) * P * nf;
+            double acc[NT];
 #pragma unroll
-                for (int l = 0; l < NT; ++l) acc += (ms * tt.Dt[it * NT + l] + kh * tt.Mt[it * NT + l]) * val(xy, l, c);
-            }
-        }
-        if (tn == 0) {
-            stage(x + nY, 1);
+            for (int l = 0; l < NT; ++l) acc[l] = 0.0;
+            stage(xy, NT);
 #pragma unroll 4
-            for (int e = eb; e < ee; ++e) acc += -Msp[e] * val(x + nY, 0, scol[e]);
+            for (int j = 0; j < ee; ++j) {
+                const int64_t e = static_cast<int64_t>(j) * nf + s;
+                const double kh = eK[e] * h;
+                const int c = ecol[e];
+                double v[NT];
+#pragma unroll
+                for (int q = 0; q < NT; ++q) v[q] = get(xy, q, c);
+#pragma unroll
+                for (int l = 0; l < NT; ++l)
+#pragma unroll
+                    for (int q = 0; q < NT; ++q) acc[l] += (kh * tt.Mt[l * NT + q]) * v[q];
+            }
+            stage(xx, NT);
+#pragma unroll 4
+            for (int j = 0; j < ee; ++j) {
+                const int64_t e = static_cast<int64_t>(j) * nf + s;
+                const double kh = eK[e] * h, ms = eM[e];
+                const int c = ecol[e];
+                double v[NT];
+#pragma unroll
+                for (int q = 0; q < NT; ++q) v[q] = get(xx, q, c);
+#pragma unroll
+                for (int l = 0; l < NT; ++l)
+#pragma unroll
+                    for (int q = 0; q < NT; ++q) acc[l] += (ms * tt.Dt[q * NT + l] + kh * tt.Mt[q * NT + l]) * v[q];
+            }
+            if (row)
+#pragma unroll
+                for (int l = 0; l < NT; ++l) y[(static_cast<int64_t>(k) * NT + l) * nf + s] = acc[l];
+        } else {
+            const int tn = job - m;
+            double acc = 0.0;
+            // the intervals at time node tn, with tn's index it inside each
+            int ks[2], its[2], nk = 0;
+            if (tn % P == 0) {
+                if (tn / P - 1 >= 0) { ks[nk] = tn / P - 1; its[nk] = P; ++nk; }
+                if (tn / P < m) { ks[nk] = tn / P; its[nk] = 0; ++nk; }
+            } else {
+                ks[0] = tn / P;
+                its[0] = tn % P;
+                nk = 1;
+            }
+            for (int bk = 0; bk < nk; ++bk) {
+                const int k = ks[bk], it = its[bk];
+                const double h = times[k + 1] - times[k];
+                const double* xy = x + static_cast<int64_t>(k) * NT * nf;
+                stage(xy, NT);
+#pragma unroll 4
+                for (int j = 0; j < ee; ++j) {
+                    const int64_t e = static_cast<int64_t>(j) * nf + s;
+                    const double kh = eK[e] * h, ms = eM[e];
+                    const int c = ecol[e];
+#pragma unroll
+                    for (int l = 0; l < NT; ++l) acc += (ms * tt.Dt[it * NT + l] + kh * tt.Mt[it * NT + l]) * get(xy, l, c);
+                }
+            }
+            if (tn == 0) {
+                stage(x + nY, 1);
+#pragma unroll 4
+                for (int j = 0; j < ee; ++j) {
+                    const int64_t e = static_cast<int64_t>(j) * nf + s;
+                    acc += -eM[e] * get(x + nY, 0, ecol[e]);
+                }
+            }
+            if (row) y[nY + static_cast<int64_t>(tn) * nf + s] = acc;
         }
-        if (row) y[nY + static_cast<int64_t>(tn) * nf + s] = acc;
-    }
+    };
+    if (staged)
+        body([&](const double*, int q, int c) { return kr_tile[q * span_cap + (c - lo)]; });
+    else
+        body([&](const double* base, int q, int c) { return base[static_cast<int64_t>(q) * nf + c]; });
+}
+
+// ELL values of the spatial K and M: entry j of row s at j * nf + s, zero past the row's end
+__global__ void k_csr_to_ell(int nf, int w, const int* __restrict__ sptr, const double* __restrict__ K,
+                             const double* __restrict__ M, double* __restrict__ eK, double* __restrict__ eM)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<int64_t>(w) * nf) return;
+    const int j = static_cast<int>(i / nf), s = static_cast<int>(i - static_cast<int64_t>(j) * nf);
+    const int e = sptr[s] + j;
+    const bool in = e < sptr[s + 1];
+    eK[i] = in ? K[e] : 0.0;
+    eM[i] = in ? M[e] : 0.0;
 }
 
 // one thread copies n doubles to the pinned mailbox, then publishes the sequence number
@@ -677,8 +704,8 @@ __global__ void k_cg_beta(double* __restrict__ cg, const double* __restrict__ pa
 // matrix-free as in k_ay_apply; each thread owns (interval k, free node s) and reads
 // p and y only at its own entries, so the updates in place are race-free.
 template <int NT>
-__global__ void __launch_bounds__(256) k_ay_cg(int nf, int m, const int* __restrict__ sptr,
-                                               const int* __restrict__ scol, const double* __restrict__ Ksp,
+__global__ void __launch_bounds__(256) k_ay_cg(int nf, int m, int w, const int* __restrict__ ecol,
+                                               const double* __restrict__ eK,
                                                const double* __restrict__ times, TimeTables tt,
                                                const double* __restrict__ z, double* __restrict__ p,
                                                double* __restrict__ y, const double* __restrict__ cg, int first,
@@ -696,9 +723,10 @@ __global__ void __launch_bounds__(256) k_ay_cg(int nf, int m, const int* __restr
 #pragma unroll
         for (int l = 0; l < NT; ++l) acc[l] = 0.0;
 #pragma unroll 4
-        for (int e = sptr[s]; e < sptr[s + 1]; ++e) {
-            const double kh = Ksp[e] * h;
-            const int c = scol[e];
+        for (int j = 0; j < w; ++j) {
+            const int64_t e = static_cast<int64_t>(j) * nf + s;  // ELL entry j of row s
+            const double kh = eK[e] * h;
+            const int c = ecol[e];
             double zv[NT];
 #pragma unroll
             for (int lp = 0; lp < NT; ++lp) zv[lp] = zk[static_cast<int64_t>(lp) * nf + c];
@@ -791,8 +819,10 @@ void mg_prepare(System* sys, bool mg32)
     HIP_TRY(hipMemcpyAsync(K.data(), sys->d_Ksp, K.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(M.data(), sys->d_Msp, M.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    const auto t_a = std::chrono::steady_clock::now();
     MGSetupHost hs;
     setup_mg(S, *sys->H, K, M, sys->tables->Mt, sys->tables->Dt, hs);
+    const auto t_b = std::chrono::steady_clock::now();
     const int nyc = S.mesh.m() * S.Nt;
     if (mg32) {
         if (!sys->mg32) sys->mg32 = std::make_unique<MGDevice<float>>();
@@ -802,6 +832,12 @@ void mg_prepare(System* sys, bool mg32)
         sys->mg64->build(sys, hs, nyc);
     }
     HIP_TRY(hipStreamSynchronize(st));
+    if (std::getenv("APF_MG_TIMING")) {
+        const auto t_c = std::chrono::steady_clock::now();
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        fprintf(stderr, "mg_prepare: download %.3f ms, host setup %.3f ms, device build %.3f ms\n", ms(ts, t_a),
+                ms(t_a, t_b), ms(t_b, t_c));
+    }
     // apply() writes one dot partial per transpose tile into d_part: a fine level with many
     // free nodes (e.g. the degree-(p+1) system of the G-estimator) needs more than 4 nparts
     const int need = mg32 ? sys->mg32->n_partials(S.nf) : sys->mg64->n_partials(S.nf);
@@ -876,10 +912,10 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     {
         const dim3 g(static_cast<unsigned>(gi)), bl(256);
         switch (S.Nt) {
-        case 2: hipLaunchKernelGGL(k_bt_residual<2>, g, bl, 0, st, S.nf, m, nY, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
-        case 3: hipLaunchKernelGGL(k_bt_residual<3>, g, bl, 0, st, S.nf, m, nY, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
-        case 4: hipLaunchKernelGGL(k_bt_residual<4>, g, bl, 0, st, S.nf, m, nY, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
-        default: hipLaunchKernelGGL(k_bt_residual<5>, g, bl, 0, st, S.nf, m, nY, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_Msp, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
+        case 2: hipLaunchKernelGGL(k_bt_residual<2>, g, bl, 0, st, S.nf, m, nY, sys->ell_w, sys->d_ell_col, sys->d_ell_K, sys->d_ell_M, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
+        case 3: hipLaunchKernelGGL(k_bt_residual<3>, g, bl, 0, st, S.nf, m, nY, sys->ell_w, sys->d_ell_col, sys->d_ell_K, sys->d_ell_M, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
+        case 4: hipLaunchKernelGGL(k_bt_residual<4>, g, bl, 0, st, S.nf, m, nY, sys->ell_w, sys->d_ell_col, sys->d_ell_K, sys->d_ell_M, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
+        default: hipLaunchKernelGGL(k_bt_residual<5>, g, bl, 0, st, S.nf, m, nY, sys->ell_w, sys->d_ell_col, sys->d_ell_K, sys->d_ell_M, sys->d_times, sys->tt, sys->d_b, sys->d_x, r0, sys->d_part); break;
         }
     }
     hipLaunchKernelGGL(k_cg_rr, dim3(1), dim3(256), 0, st, cg, sys->d_part, gi);
@@ -899,10 +935,10 @@ void slab_reconstruct(System* sys, double* lt, double tol, int max_it, int* its,
     const dim3 grid(static_cast<unsigned>(nparts)), block(256), gitems(static_cast<unsigned>(gi));
     auto ay_cg = [&](int first) {
         switch (S.Nt) {
-        case 2: hipLaunchKernelGGL(k_ay_cg<2>, gitems, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
-        case 3: hipLaunchKernelGGL(k_ay_cg<3>, gitems, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
-        case 4: hipLaunchKernelGGL(k_ay_cg<4>, gitems, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
-        default: hipLaunchKernelGGL(k_ay_cg<5>, gitems, block, 0, st, S.nf, m, sys->d_sptr, sys->d_scol, sys->d_Ksp, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        case 2: hipLaunchKernelGGL(k_ay_cg<2>, gitems, block, 0, st, S.nf, m, sys->ell_w, sys->d_ell_col, sys->d_ell_K, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        case 3: hipLaunchKernelGGL(k_ay_cg<3>, gitems, block, 0, st, S.nf, m, sys->ell_w, sys->d_ell_col, sys->d_ell_K, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        case 4: hipLaunchKernelGGL(k_ay_cg<4>, gitems, block, 0, st, S.nf, m, sys->ell_w, sys->d_ell_col, sys->d_ell_K, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
+        default: hipLaunchKernelGGL(k_ay_cg<5>, gitems, block, 0, st, S.nf, m, sys->ell_w, sys->d_ell_col, sys->d_ell_K, sys->d_times, sys->tt, z, pd, y, cg, first, sys->d_part); break;
         }
     };
     const double scale = g2 > 0.0 ? g2 : 1.0;
@@ -1012,7 +1048,31 @@ void System::kron_prepare()
         kron_span_cap = std::max(1, std::min(widest, static_cast<int>(48 * 1024 / (sizeof(double) * S.Nt))));
         d_kron_halo = dalloc<int2>(this, nb);
         HIP_TRY(hipMemcpy(d_kron_halo, hb.data(), nb * sizeof(int2), hipMemcpyHostToDevice));
+        // ELL pattern: padding entries point at the row's own node (inside its block's range)
+        int w = 0;
+        for (int r = 0; r < S.nf; ++r) w = std::max(w, S.sptr[r + 1] - S.sptr[r]);
+        ell_w = w;
+        std::vector<int> ec(static_cast<size_t>(w) * S.nf);
+        for (int r = 0; r < S.nf; ++r)
+            for (int j = 0; j < w; ++j) {
+                const int e = S.sptr[r] + j;
+                ec[static_cast<size_t>(j) * S.nf + r] = e < S.sptr[r + 1] ? S.scol[e] : r;
+            }
+        d_ell_col = dalloc<int>(this, ec.size());
+        HIP_TRY(hipMemcpy(d_ell_col, ec.data(), ec.size() * sizeof(int), hipMemcpyHostToDevice));
+        d_ell_K = dalloc<double>(this, ec.size());
+        d_ell_M = dalloc<double>(this, ec.size());
     }
+}
+
+void System::ell_refresh()
+{
+    const Space& S = *space;
+    if (!d_ell_col) return;
+    const int64_t total = static_cast<int64_t>(ell_w) * S.nf;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_csr_to_ell, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, stream, S.nf,
+                       ell_w, d_sptr, d_Ksp, d_Msp, d_ell_K, d_ell_M);
 }
 
 void System::kron_apply(const double* x, double* y)
@@ -1024,10 +1084,10 @@ void System::kron_apply(const double* x, double* y)
     const dim3 grid(static_cast<unsigned>(nb), static_cast<unsigned>(m + m * S.p + 1)), block(KR);
     const size_t lds = static_cast<size_t>(S.Nt) * kron_span_cap * sizeof(double);
     switch (S.Nt) {
-    case 2: hipLaunchKernelGGL(k_kron_apply<2>, grid, block, lds, stream, S.nf, m, S.nY, d_sptr, d_scol, d_Ksp, d_Msp, d_times, tt, d_kron_halo, kron_span_cap, x, y); break;
-    case 3: hipLaunchKernelGGL(k_kron_apply<3>, grid, block, lds, stream, S.nf, m, S.nY, d_sptr, d_scol, d_Ksp, d_Msp, d_times, tt, d_kron_halo, kron_span_cap, x, y); break;
-    case 4: hipLaunchKernelGGL(k_kron_apply<4>, grid, block, lds, stream, S.nf, m, S.nY, d_sptr, d_scol, d_Ksp, d_Msp, d_times, tt, d_kron_halo, kron_span_cap, x, y); break;
-    default: hipLaunchKernelGGL(k_kron_apply<5>, grid, block, lds, stream, S.nf, m, S.nY, d_sptr, d_scol, d_Ksp, d_Msp, d_times, tt, d_kron_halo, kron_span_cap, x, y); break;
+    case 2: hipLaunchKernelGGL(k_kron_apply<2>, grid, block, lds, stream, S.nf, m, S.nY, ell_w, d_ell_col, d_ell_K, d_ell_M, d_times, tt, d_kron_halo, kron_span_cap, x, y); break;
+    case 3: hipLaunchKernelGGL(k_kron_apply<3>, grid, block, lds, stream, S.nf, m, S.nY, ell_w, d_ell_col, d_ell_K, d_ell_M, d_times, tt, d_kron_halo, kron_span_cap, x, y); break;
+    case 4: hipLaunchKernelGGL(k_kron_apply<4>, grid, block, lds, stream, S.nf, m, S.nY, ell_w, d_ell_col, d_ell_K, d_ell_M, d_times, tt, d_kron_halo, kron_span_cap, x, y); break;
+    default: hipLaunchKernelGGL(k_kron_apply<5>, grid, block, lds, stream, S.nf, m, S.nY, ell_w, d_ell_col, d_ell_K, d_ell_M, d_times, tt, d_kron_halo, kron_span_cap, x, y); break;
     }
 }
 
@@ -1038,10 +1098,10 @@ void System::ay_apply(const double* x, double* y)
     const int64_t threads = static_cast<int64_t>(m) * S.nf;
     const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);
     switch (S.Nt) {
-    case 2: hipLaunchKernelGGL(k_ay_apply<2>, grid, block, 0, stream, S.nf, m, d_sptr, d_scol, d_Ksp, d_times, tt, x, y); break;
-    case 3: hipLaunchKernelGGL(k_ay_apply<3>, grid, block, 0, stream, S.nf, m, d_sptr, d_scol, d_Ksp, d_times, tt, x, y); break;
-    case 4: hipLaunchKernelGGL(k_ay_apply<4>, grid, block, 0, stream, S.nf, m, d_sptr, d_scol, d_Ksp, d_times, tt, x, y); break;
-    default: hipLaunchKernelGGL(k_ay_apply<5>, grid, block, 0, stream, S.nf, m, d_sptr, d_scol, d_Ksp, d_times, tt, x, y); break;
+    case 2: hipLaunchKernelGGL(k_ay_apply<2>, grid, block, 0, stream, S.nf, m, ell_w, d_ell_col, d_ell_K, d_times, tt, x, y); break;
+    case 3: hipLaunchKernelGGL(k_ay_apply<3>, grid, block, 0, stream, S.nf, m, ell_w, d_ell_col, d_ell_K, d_times, tt, x, y); break;
+    case 4: hipLaunchKernelGGL(k_ay_apply<4>, grid, block, 0, stream, S.nf, m, ell_w, d_ell_col, d_ell_K, d_times, tt, x, y); break;
+    default: hipLaunchKernelGGL(k_ay_apply<5>, grid, block, 0, stream, S.nf, m, ell_w, d_ell_col, d_ell_K, d_times, tt, x, y); break;
     }
 }
 
@@ -1434,6 +1494,7 @@ int apf_assemble_main(apf_system* sys, const apf_coeff* coeff, int mode)
                                sys->d_rank, sys->d_sptr, sys->d_Khat, sys->d_Mhat, sys->d_mhat, T.nb, sys->d_bw,
                                sys->d_bphi, sys->d_u0b, sys->d_Ksp, sys->d_Msp, sys->d_msp, sys->d_gsp, nullptr);
         }
+        sys->ell_refresh();
         if (mode == APF_ASSEMBLE_KRON) {
             // row indices fit int (build_pattern rejects n >= 2^31)
             const dim3 grid(static_cast<unsigned>((S.n + FILL_ROWS - 1) / FILL_ROWS)), block(256);
@@ -1616,8 +1677,17 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
         HIP_TRY(hipMemcpyAsync(t, r1, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         precond(t, nullptr, y);
         hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(256), 0, st, sys->d_scal, sys->d_part, pparts);
-        auto true_residual = [&](double& rel) {
-            sys->spmv(x, t);
+        // true residual |b - K x| / |b|: with the assembled K (CSR), timed with events (the SpMV
+        // time apf_system_timings reports), or with the matrix-free product for the intermediate
+        // checks; acceptance is always confirmed with the assembled K
+        auto true_residual = [&](double& rel, bool csr = true) {
+            if (csr) {
+                HIP_TRY(hipEventRecord(sys->ev0, st));
+                sys->spmv(x, t);
+                HIP_TRY(hipEventRecord(sys->ev1, st));
+            } else {
+                sys->kron_apply(x, t);
+            }
             hipLaunchKernelGGL(k_residual_parts, dim3(vgrid), dim3(block), 0, st, n, sys->d_b, t, sys->d_part);
             hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, sys->d_part, nparts, sys->d_res);
             double h[2];
@@ -1686,19 +1756,21 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
             }
             it += batch;
             HIP_TRY(hipGetLastError());
-            true_residual(rel);
+            true_residual(rel, !matfree);
             conv = rel <= tol;
+            if (conv && matfree) {
+                true_residual(rel, true);
+                conv = rel <= tol;
+            }
             if (!std::isfinite(rel)) break;
         }
         HIP_TRY(hipMemcpy(&hs, sys->d_scal, sizeof(hs), hipMemcpyDeviceToHost));
         if (x_out) HIP_TRY(hipMemcpy(x_out, x, n * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_TRY(hipEventRecord(sys->ev0, st));
-        sys->spmv(x, t);
-        HIP_TRY(hipEventRecord(sys->ev1, st));
-        HIP_TRY(hipEventSynchronize(sys->ev1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
-        sys->t_spmv_ms = ms;
+        {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+            sys->t_spmv_ms = ms;
+        }
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) {
             stats->iterations = it;

@@ -253,6 +253,13 @@ struct System {
     void ensure_mailbox();
     int2* d_kron_halo = nullptr;  // neighbour range of each block of the matrix-free product
     int kron_span_cap = 0;
+    // the spatial K and M in ELL form ([entry j][free node s], rows padded with zero entries
+    // on their own diagonal column) for the matrix-free kernels' coalesced row loads
+    int ell_w = 0;
+    int* d_ell_col = nullptr;
+    double* d_ell_K = nullptr;
+    double* d_ell_M = nullptr;
+    void ell_refresh();  // values from d_Ksp / d_Msp (after each assembly)
     double* d_kdiag = nullptr;
     double* d_band = nullptr;
     float* d_band32 = nullptr;

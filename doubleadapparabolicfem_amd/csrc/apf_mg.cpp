@@ -2,6 +2,9 @@
 #include "apf_mg.hpp"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <future>
 #include <random>
@@ -404,6 +407,7 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
               const std::vector<double>& Msp, const std::vector<double>& Mt, const std::vector<double>& Dt,
               MGSetupHost& out)
 {
+    const auto t_start = std::chrono::steady_clock::now();
     const int L = static_cast<int>(H.coarse.size()) + 1;
     out.levels.assign(L, MGLevelHost());
     auto tf = std::async(std::launch::async, [&] { setup_time_factors(S, Mt, Dt, out); });
@@ -444,7 +448,9 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
             lv.R = transpose(H.P[l]);
         }
     }
+    const auto t_lv = std::chrono::steady_clock::now();
     tf.get();  // the time factors (computed beside the levels)
+    const auto t_tf = std::chrono::steady_clock::now();
     const int T = out.T;
     // coarsest inverses for shift 0 and each time-mode shift, per connected
     // component when the components are contiguous node ranges
@@ -501,7 +507,14 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
             if (!invert_dense(b, A)) throw std::runtime_error("setup_mg: singular coarsest operator");
             std::copy(A.begin(), A.end(), out.coarse_inv.begin() + static_cast<size_t>(q) * out.blk_total + out.blk_off[c]);
         }
+    const auto t_ci = std::chrono::steady_clock::now();
     for (int l = 0; l < L; ++l) out.levels[l].lmax = std::max(lm[2 * l].get(), lm[2 * l + 1].get());
+    if (std::getenv("APF_MG_TIMING")) {
+        const auto t_end = std::chrono::steady_clock::now();
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        fprintf(stderr, "setup_mg: levels %.3f, time factors wait %.3f, coarse inverses %.3f, lmax wait %.3f ms\n",
+                ms(t_start, t_lv), ms(t_lv, t_tf), ms(t_tf, t_ci), ms(t_ci, t_end));
+    }
 }
 
 }  // namespace apf
