@@ -6,7 +6,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
+#include <atomic>
 #include <future>
+#include <thread>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -201,6 +203,67 @@ std::vector<double> galerkin(const CSR& R, const std::vector<int>& fptr, const s
     galerkin_rows(R, fptr, fcol, A, P, cptr, ccol, 0, static_cast<int>(static_cast<int64_t>(nc) / nt), out);
     for (auto& f : parts) f.get();
     return out;
+}
+
+// The same bound with the rows of each power-iteration step split over `nt` threads (a
+// spin barrier between the phases; the norm partials are summed in thread order, so the
+// result does not depend on the timing).
+double lmax_bound_threads(int n, const std::vector<int>& ptr, const std::vector<int>& col,
+                          const std::vector<double>& A, const std::vector<double>& d, int nt)
+{
+    double gers = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) s += std::fabs(A[e]);
+        gers = std::max(gers, s / d[i]);
+    }
+    std::vector<double> x(n), y(n);
+    std::mt19937_64 rng(12345);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (auto& v : x) v = U(rng);
+    std::vector<double> part(nt, 0.0), xpart(nt, 0.0);
+    std::atomic<int> arrived{0}, generation{0};
+    auto barrier = [&]() {
+        const int g = generation.load(std::memory_order_acquire);
+        if (arrived.fetch_add(1, std::memory_order_acq_rel) == nt - 1) {
+            arrived.store(0, std::memory_order_relaxed);
+            generation.fetch_add(1, std::memory_order_acq_rel);
+        } else {
+            while (generation.load(std::memory_order_acquire) == g) std::this_thread::yield();
+        }
+    };
+    double lam = 0.0;
+    auto work = [&](int t) {
+        const int i0 = static_cast<int>(static_cast<int64_t>(n) * t / nt);
+        const int i1 = static_cast<int>(static_cast<int64_t>(n) * (t + 1) / nt);
+        for (int it = 0; it < 60; ++it) {
+            double nrm = 0.0, xn = 0.0;
+            for (int i = i0; i < i1; ++i) {
+                double s = 0.0;
+                for (int e = ptr[i]; e < ptr[i + 1]; ++e) s += A[e] * x[col[e]];
+                y[i] = s / d[i];
+                nrm += y[i] * y[i];
+                xn += x[i] * x[i];
+            }
+            part[t] = nrm;
+            xpart[t] = xn;
+            barrier();
+            double tn = 0.0, tx = 0.0;
+            for (int q = 0; q < nt; ++q) {
+                tn += part[q];
+                tx += xpart[q];
+            }
+            tn = std::sqrt(tn);
+            if (t == 0) lam = tn / std::sqrt(tx);
+            for (int i = i0; i < i1; ++i) x[i] = y[i] / tn;
+            barrier();
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& q : th) q.join();
+    return std::min(gers, 1.2 * lam);
 }
 
 double lmax_bound(int n, const std::vector<int>& ptr, const std::vector<int>& col, const std::vector<double>& A,
@@ -441,8 +504,16 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
                 if (lv.scol[e] == i) { lv.dK[i] = lv.K[e]; lv.dM[i] = lv.M[e]; }
         for (int i = 0; i < lv.nf; ++i)
             if (!(lv.dK[i] > 0.0) || !(lv.dM[i] > 0.0)) throw std::runtime_error("setup_mg: non-positive diagonal");
-        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK); }));
-        lm.push_back(std::async(std::launch::async, [&lv] { return lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM); }));
+        // the fine level's power iterations split their rows over 4 threads each
+        const int pt = (l == 0 && lv.nf >= 4096) ? 4 : 1;
+        lm.push_back(std::async(std::launch::async, [&lv, pt] {
+            return pt > 1 ? lmax_bound_threads(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK, pt)
+                          : lmax_bound(lv.nf, lv.sptr, lv.scol, lv.K, lv.dK);
+        }));
+        lm.push_back(std::async(std::launch::async, [&lv, pt] {
+            return pt > 1 ? lmax_bound_threads(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM, pt)
+                          : lmax_bound(lv.nf, lv.sptr, lv.scol, lv.M, lv.dM);
+        }));
         if (l + 1 < L) {
             lv.P = H.P[l];
             lv.R = transpose(H.P[l]);
