@@ -650,8 +650,6 @@ void MGDevice<R>::build(System* sys, const MGSetupHost& H, int nyc_)
     up(d_W, H.W);
     d_MtInvH = mg_alloc<double>(sys, H.MtInvH.size());
     up(d_MtInvH, H.MtInvH);
-    d_Z = mg_alloc<R>(sys, static_cast<size_t>(levels[0].nf) * ld);
-    d_Z2 = mg_alloc<R>(sys, static_cast<size_t>(levels[0].nf) * ld);
     setup_tail(sys);
     ready = true;
 }
@@ -873,7 +871,6 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
     const int deg = degree > 0 ? degree : nu;
     Level& f = levels[0];
     const int nf = f.nf;
-    const int64_t nel = static_cast<int64_t>(nf) * ncol;
     const int th = tile_rows();
     const unsigned tiles = static_cast<unsigned>((nf + th - 1) / th);
     // the transpose tile, and W when it fits in 40 KiB
@@ -890,21 +887,17 @@ void MGDevice<R>::apply(hipStream_t st, const Space& S, const double* r, double*
     // columns when the caller reads only z's Y part)
     if (y_only) vcycle(st, 0, T, nyc, d_sig + T, d_sidx + T, deg);
     else vcycle(st, 0, 0, ncol, d_sig, d_sidx, deg);
-    // stage 2 (X columns): apply K, then the shifted V-cycle again (with y_only the
-    // scatter reads the Y columns of stage 1 in place)
+    // stage 2 (X columns): apply K, then the shifted V-cycle again.  The second V-cycle
+    // writes only the X columns [0, T) of the fine level's X, so afterwards X holds the
+    // X columns of stage 2 and the Y columns of stage 1: the scatter reads it in place.
     if (!y_only) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(nel, 256)), dim3(256), 0, st, nf, ncol, ld,
-                           f.X, d_Z);
         with_passes(T, [&](auto np) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_apply<R, decltype(np)::value>), dim3(grid_for(nf, 4)), dim3(256), 0,
-                               st, nf, f.ptr, f.col, f.K, f.M, d_zero, T, ld, d_Z, d_Z, f.B, 0);
+                               st, nf, f.ptr, f.col, f.K, f.M, d_zero, T, ld, f.X, f.X, f.B, 0);
         });
         vcycle(st, 0, 0, T, d_sig, d_sidx, deg);
-        // X columns from the second V-cycle, Y columns from the first
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_block_copy<R>), dim3(grid_for(static_cast<int64_t>(nf) * T, 256)), dim3(256),
-                           0, st, nf, T, ld, f.X, d_Z);
     }
-    const R* Zs = y_only ? f.X : d_Z;
+    const R* Zs = f.X;
     if (wl)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mg_scatter<R, true>), dim3(tiles), dim3(MG_TT), lds, st, nf, T, S.Nt,
                            S.mesh.m(), ld, S.nY, th, yo, Zs, d_W, d_MtInvH, r, z, part);

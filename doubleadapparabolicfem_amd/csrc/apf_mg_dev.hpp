@@ -52,8 +52,6 @@ struct MGDevice {
     int64_t* d_blk_off = nullptr;
     double* d_W = nullptr;
     double* d_MtInvH = nullptr;
-    R* d_Z = nullptr;
-    R* d_Z2 = nullptr;
     bool ready = false;
     // fused coarse tail: levels tail_from..L-1 run in one launch (k_mg_tail), one
     // workgroup per column with the level vectors in LDS; -1 = off
