@@ -25,6 +25,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
+#include <thread>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -892,47 +894,98 @@ void prepare(const apf_cell_space_desc* Xd, const apf_cell_space_desc* Yd, const
         if (P.X.geo[c * kGeo + kGeo - 2] == 0.0) P.bottom.push_back(static_cast<int>(c));
 }
 
-// Patterns of B_dt^T (Y rows) and Hhat (X rows): the X dofs of each Y cell's ancestor
-// against the Y dofs of the cell, and the X-X pairs of the ancestors (A_x, C) and of the
-// columns of B_dt (the outer products).
+// Patterns of B_dt^T (Y rows) and Hhat (X rows).  Row j of B_dt^T holds the X dofs of the
+// ancestors of the Y cells that contain Y dof j; Hhat's pattern is that of B_dt B_dt^T (it
+// contains A_x's and C's: every X cell holds Y cells, whose dofs couple to all its X
+// dofs), row a the union of the B_dt^T rows of the Y dofs in row a of B_dt.  Both are built
+// with marker arrays, rows split over threads.
+template <typename F>
+void parallel_rows(int64_t n, F f)
+{
+    const int nt = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(16, n / 1024)));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(f, t, n * t / nt, n * (t + 1) / nt);
+    f(0, 0, n / nt);
+    for (auto& q : th) q.join();
+}
+
+void union_rows(int64_t nrows, int64_t ncols, const std::function<void(int64_t, std::vector<int>&, std::vector<int>&, int)>& gather,
+                std::vector<int64_t>& rp, std::vector<int>& col)
+{
+    const int nt = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(16, nrows / 1024)));
+    std::vector<std::vector<int>> cols(nt);
+    std::vector<std::vector<int64_t>> lens(nt);
+    parallel_rows(nrows, [&](int t, int64_t r0, int64_t r1) {
+        std::vector<int> mark(static_cast<size_t>(ncols), -1);
+        auto& out = cols[t];
+        auto& len = lens[t];
+        for (int64_t r = r0; r < r1; ++r) {
+            const size_t start = out.size();
+            gather(r, mark, out, static_cast<int>(r));
+            std::sort(out.begin() + start, out.end());
+            len.push_back(static_cast<int64_t>(out.size() - start));
+        }
+    });
+    rp.assign(1, 0);
+    col.clear();
+    size_t total = 0;
+    for (const auto& c : cols) total += c.size();
+    col.reserve(total);
+    for (int t = 0; t < nt; ++t) {
+        for (int64_t l : lens[t]) rp.push_back(rp.back() + l);
+        col.insert(col.end(), cols[t].begin(), cols[t].end());
+    }
+}
+
 void graph_patterns(const Prepared& P, std::vector<int64_t>& bt_rp, std::vector<int>& bt_col,
                     std::vector<int64_t>& h_rp, std::vector<int>& h_col)
 {
-    const int nl = n_local(P.X.dim, P.X.p);
-    std::vector<int> xs, ys;
-    std::vector<std::vector<int>> bt(P.Y.ndofs), hh(P.X.ndofs);
-    for (int64_t c = 0; c < P.Y.nc; ++c) {
-        cell_dofs(P.Y, c, ys);
-        cell_dofs(P.X, P.anc[c], xs);
-        for (int gy : ys) bt[gy].insert(bt[gy].end(), xs.begin(), xs.end());
+    const int64_t nY = P.Y.ndofs, nX = P.X.ndofs, nyc = P.Y.nc, nxc = P.X.nc;
+    // Y dof -> Y cells, and the dof lists of every cell
+    std::vector<std::vector<int>> ycd(nyc), xcd(nxc);
+    for (int64_t c = 0; c < nyc; ++c) cell_dofs(P.Y, c, ycd[c]);
+    for (int64_t a = 0; a < nxc; ++a) cell_dofs(P.X, a, xcd[a]);
+    std::vector<int64_t> yc_ptr(nY + 1, 0);
+    for (int64_t c = 0; c < nyc; ++c)
+        for (int j : ycd[c]) ++yc_ptr[j + 1];
+    for (int64_t j = 0; j < nY; ++j) yc_ptr[j + 1] += yc_ptr[j];
+    std::vector<int> yc_cell(yc_ptr.back());
+    {
+        std::vector<int64_t> pos(yc_ptr.begin(), yc_ptr.end() - 1);
+        for (int64_t c = 0; c < nyc; ++c)
+            for (int j : ycd[c]) yc_cell[pos[j]++] = static_cast<int>(c);
     }
-    for (auto& r : bt) {
-        std::sort(r.begin(), r.end());
-        r.erase(std::unique(r.begin(), r.end()), r.end());
+    union_rows(nY, nX, [&](int64_t j, std::vector<int>& mark, std::vector<int>& out, int tag) {
+        for (int64_t e = yc_ptr[j]; e < yc_ptr[j + 1]; ++e)
+            for (int g : xcd[P.anc[yc_cell[e]]])
+                if (mark[g] != tag) {
+                    mark[g] = tag;
+                    out.push_back(g);
+                }
+    }, bt_rp, bt_col);
+    // B_dt rows (X dof -> Y dofs): the transpose of B_dt^T's pattern
+    std::vector<int64_t> b_rp(nX + 1, 0);
+    for (int g : bt_col) ++b_rp[g + 1];
+    for (int64_t a = 0; a < nX; ++a) b_rp[a + 1] += b_rp[a];
+    std::vector<int> b_col(bt_col.size());
+    {
+        std::vector<int64_t> pos(b_rp.begin(), b_rp.end() - 1);
+        for (int64_t j = 0; j < nY; ++j)
+            for (int64_t e = bt_rp[j]; e < bt_rp[j + 1]; ++e) b_col[pos[bt_col[e]]++] = static_cast<int>(j);
     }
-    for (int64_t a = 0; a < P.X.nc; ++a) {
-        cell_dofs(P.X, a, xs);
-        for (int gi : xs) hh[gi].insert(hh[gi].end(), xs.begin(), xs.end());
-    }
-    for (const auto& r : bt)
-        for (int gi : r) hh[gi].insert(hh[gi].end(), r.begin(), r.end());
-    bt_rp.assign(1, 0);
-    bt_col.clear();
-    for (const auto& r : bt) {
-        bt_col.insert(bt_col.end(), r.begin(), r.end());
-        bt_rp.push_back(static_cast<int64_t>(bt_col.size()));
-    }
-    h_rp.assign(1, 0);
-    h_col.clear();
-    for (auto& r : hh) {
-        std::sort(r.begin(), r.end());
-        r.erase(std::unique(r.begin(), r.end()), r.end());
-        h_col.insert(h_col.end(), r.begin(), r.end());
-        h_rp.push_back(static_cast<int64_t>(h_col.size()));
-    }
-    (void)nl;
+    union_rows(nX, nX, [&](int64_t a, std::vector<int>& mark, std::vector<int>& out, int tag) {
+        for (int64_t e = b_rp[a]; e < b_rp[a + 1]; ++e) {
+            const int j = b_col[e];
+            for (int64_t f = bt_rp[j]; f < bt_rp[j + 1]; ++f) {
+                const int g = bt_col[f];
+                if (mark[g] != tag) {
+                    mark[g] = tag;
+                    out.push_back(g);
+                }
+            }
+        }
+    }, h_rp, h_col);
 }
-
 }  // namespace
 
 struct apf_general_system {
@@ -1321,7 +1374,9 @@ int apf_general_graph_norm(apf_general_system* sys, const apf_general_coeff* coe
         hipStream_t st = ctx->stream;
         std::vector<int64_t> bt_rp, h_rp;
         std::vector<int> bt_col, h_col;
+        const auto t_p0 = std::chrono::steady_clock::now();
         graph_patterns(sys->host, bt_rp, bt_col, h_rp, h_col);
+        const auto t_p1 = std::chrono::steady_clock::now();
         auto dalloc = [&](size_t bytes) {
             void* q = nullptr;
             HIP_TRY(hipMalloc(&q, std::max<size_t>(bytes, 8)));
@@ -1401,6 +1456,12 @@ int apf_general_graph_norm(apf_general_system* sys, const apf_general_coeff* coe
         int err[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(err, p_err, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+        if (std::getenv("APF_HHAT_TIMING")) {
+            const auto t_p2 = std::chrono::steady_clock::now();
+            auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+            fprintf(stderr, "graph_norm: patterns %.1f ms (nnz BT %zu, Hhat %zu), device %.1f ms\n", ms(t_p0, t_p1),
+                    bt_col.size(), h_col.size(), ms(t_p1, t_p2));
+        }
         for (void* q : tmp) (void)hipFree(q);
         tmp.clear();
         if (err[1]) {
