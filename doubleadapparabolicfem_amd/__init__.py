@@ -197,6 +197,9 @@ EXPORTED = {
     "apf_general_estimator_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.POINTER(ctypes.c_void_p)]),
     "apf_general_estimator_destroy": (None, [ctypes.c_void_p]),
+    "apf_general_estimator_create_sliced": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                           ctypes.c_void_p, ctypes.c_void_p,
+                                                           ctypes.POINTER(ctypes.c_void_p)]),
     "apf_general_estimator_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_Coeff)]),
     "apf_general_estimate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(_EstimateOut), ctypes.c_void_p, ctypes.c_void_p,
@@ -718,13 +721,24 @@ class GeneralEstimator:
     slab solve, first Y cell)]."""
 
     def __init__(self, gs: GeneralSystem, groups):
+        """groups: [(tensor Mesh, coarse meshes, first Y cell)], or, for Y cells sliced into
+        several slabs (apf_general_estimator_create_sliced), [(tensor Mesh, coarse meshes,
+        source Y cell of each group cell, slab-major)]"""
         self.gs = gs
         self.spaces = [Space(m, gs.mesh.p, coarse=coarse) for m, coarse, _ in groups]
         arr = (ctypes.c_void_p * len(groups))(*[sp.handle.value for sp in self.spaces])
-        first = np.ascontiguousarray([f for _, _, f in groups], dtype=np.int64)
         h = ctypes.c_void_p()
-        _check(lib().apf_general_estimator_create(gs.handle, len(groups), arr, _ptr(first), ctypes.byref(h)),
-               gs.ctx.handle, "apf_general_estimator_create")
+        if np.isscalar(groups[0][2]):
+            first = np.ascontiguousarray([f for _, _, f in groups], dtype=np.int64)
+            _check(lib().apf_general_estimator_create(gs.handle, len(groups), arr, _ptr(first), ctypes.byref(h)),
+                   gs.ctx.handle, "apf_general_estimator_create")
+        else:
+            src = [np.asarray(s, dtype=np.int32) for _, _, s in groups]
+            ptr = np.ascontiguousarray(np.concatenate([[0], np.cumsum([len(s) for s in src])]), dtype=np.int64)
+            cat = np.ascontiguousarray(np.concatenate(src), dtype=np.int32)
+            _check(lib().apf_general_estimator_create_sliced(gs.handle, len(groups), arr, _ptr(ptr), _ptr(cat),
+                                                             ctypes.byref(h)),
+                   gs.ctx.handle, "apf_general_estimator_create_sliced")
         self.handle = h
 
     def __del__(self):

@@ -24,7 +24,8 @@ import ctypes
 
 import numpy as np
 
-from . import (APF_FP64, Context, GeneralSystem, _check, _ptr, doerfler_mark, lib, y_estimator_threshold)
+from . import (APF_FP64, Context, GeneralEstimator, GeneralSystem, _check, _ptr, doerfler_mark, lib,
+               y_estimator_threshold)
 
 
 class Mesh1D:
@@ -169,6 +170,47 @@ class SpaceTimePair:
         self.y_ancestor = X.ancestors_of(Y)
 
 
+def slab_groups(Y: FESpace1D, max_direct=64):
+    """The estimator's time slabs (between consecutive time endpoints of Y's cells, each
+    cell sliced into the slabs it spans: time_slabs/time_slab_builder.hpp:53-120), runs of
+    slabs with one spatial partition grouped as tensor meshes, for GeneralEstimator:
+    [(Mesh of the group, coarse meshes, source Y cell of each group cell)].  A partition with
+    more than `max_direct` intervals gets coarse meshes of every other vertex (nested: each
+    coarse interval is a union of fine ones) for the slab solve's multigrid; smaller ones are
+    solved on one level."""
+    from . import Mesh
+
+    box = Y.box
+    times = np.unique(np.concatenate([box[:, 2], box[:, 3]]))
+    runs = []
+    for k in range(len(times) - 1):
+        t0, t1 = times[k], times[k + 1]
+        inside = np.flatnonzero((box[:, 2] <= t0) & (box[:, 3] >= t1))
+        inside = inside[np.argsort(box[inside, 0], kind="stable")]
+        part = tuple(map(tuple, box[inside, :2]))
+        if runs and runs[-1][0] == part:
+            runs[-1][1].append(t1)
+            runs[-1][2].append(inside)
+        else:
+            runs.append((part, [t0, t1], [inside]))
+    out = []
+    for part, ts, srcs in runs:
+        verts = np.array([part[0][0]] + [b for (_, b) in part])
+        ns = len(part)
+        mesh = Mesh(1, verts[:, None], np.stack([np.arange(ns), np.arange(1, ns + 1)], axis=1).astype(np.int32),
+                    np.array(ts))
+        coarse = []
+        v = verts
+        while len(v) - 1 > max_direct:
+            keep = v[::2] if (len(v) - 1) % 2 == 0 else np.concatenate([v[::2], v[-1:]])
+            v = keep
+            nc = len(v) - 1
+            coarse.append(Mesh(1, v[:, None], np.stack([np.arange(nc), np.arange(1, nc + 1)], axis=1).astype(np.int32),
+                               np.array([0.0, 1.0])))  # finest coarse mesh first
+        out.append((mesh, coarse, np.concatenate(srcs).astype(np.int32)))
+    return out
+
+
 def smooth_initial_u0(x):
     """u0 = sin(pi x) (algorithm/examples/space_time_1d/initial_data_1d_examples.hpp:9-63: M = 1, ell = 0)"""
     return np.sin(np.pi * np.asarray(x)[..., 0])
@@ -178,8 +220,9 @@ class AdaptiveX1D:
     """Outer X loop of the reference's adaptive driver on 1+1D meshes, on the GPU."""
 
     def __init__(self, ctx: Context, p=1, theta_x=0.4, rho=1.0, x_uniform_levels=1, u0=smooth_initial_u0,
-                 tol=1e-10, max_it=40000):
+                 tol=1e-10, max_it=40000, with_y_estimator=False, estimator_precision=APF_FP64):
         self.ctx, self.p, self.theta_x, self.rho, self.u0 = ctx, p, theta_x, rho, u0
+        self.with_y_estimator, self.estimator_precision = with_y_estimator, estimator_precision
         self.tol, self.max_it = tol, max_it
         self.mesh = Mesh1D(0.0, 1.0, 0.0, 1.0)
         self.X = FESpace1D(self.mesh, p, True, self.mesh.leaf_cells())
@@ -207,6 +250,13 @@ class AdaptiveX1D:
                    y_estimator_threshold_squared=y_estimator_threshold(self.rho, lam2, tr2),
                    minres_iterations=stats["iterations"], minres_rel_residual=stats["true_relative_residual"],
                    assembly_ms=ms, solution=x, eta2_by_x_cell=eta2_x)
+        if self.with_y_estimator:
+            ge = GeneralEstimator(gs, slab_groups(self.Y))
+            ge.assemble(self.u0)
+            est = ge.estimate(x, self.estimator_precision, 2, per_cell=True)
+            rec["y_estimator"] = est
+            rec["y_estimator_squared"] = est["y_est2"]
+            del ge
         if refine:
             marked, _ = doerfler_mark(eta2_x, self.theta_x, ids=self.X.active.astype(np.int64))
             rec["marked_x_cells"] = np.asarray(marked)

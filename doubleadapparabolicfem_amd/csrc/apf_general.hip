@@ -1583,6 +1583,11 @@ struct EstGroup {
     int64_t first_y_cell = 0;
     int64_t nYg = 0, nXg = 0, ncells = 0;
     int* d_ymap = nullptr;       // group Y dof -> general Y dof
+    bool sliced = false;         // Y cells cut into slabs: lambda interpolated, outputs summed
+    std::vector<int> src;        // sliced: group cell (slab k, spatial cell c) -> source Y cell
+    int64_t* d_lptr = nullptr;   // sliced: group Y dof -> interpolation row over general Y dofs
+    int* d_lcol = nullptr;
+    double* d_lw = nullptr;
     int64_t* d_iptr = nullptr;   // group X dof -> interpolation row over general X dofs
     int* d_icol = nullptr;
     double* d_iw = nullptr;
@@ -1601,10 +1606,18 @@ struct apf_general_estimator {
 
 extern "C" {
 
-int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_space* const* spaces,
-                                 const int64_t* first_y_cell, apf_general_estimator** out)
+}  // extern "C"
+
+namespace {
+
+// Groups of slabs as tensor systems.  Plain mode (first_y_cell): the group's cells are the
+// general system's Y cells first_y_cell[g] + k * ns + c.  Sliced mode (cell_ptr, cell_source):
+// cell (k, c) of group g is a slice of the Y cell cell_source[cell_ptr[g] + k * ns + c].
+int estimator_create(apf_general_system* gsys, int n_groups, apf_space* const* spaces, const int64_t* first_y_cell,
+                     const int64_t* cell_ptr, const int* cell_source, apf_general_estimator** out)
 {
-    if (!gsys || !spaces || !first_y_cell || !out || n_groups < 1)
+    const bool sliced = cell_source != nullptr;
+    if (!gsys || !spaces || !out || n_groups < 1 || (!sliced && !first_y_cell) || (sliced && !cell_ptr))
         return fail(gsys ? gsys->ctx : nullptr, APF_BAD_ARG, "apf_general_estimator_create: bad argument");
     apf_ctx* ctx = gsys->ctx;
     auto ge = std::make_unique<apf_general_estimator>();
@@ -1636,30 +1649,77 @@ int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_spa
             const apf::Space& S = G.space->S;
             if (S.dim != Y.dim || S.p != Y.p) throw std::invalid_argument("group space differs in dim or p");
             if (!G.space->has_pattern) throw std::invalid_argument("group space needs its K pattern");
-            if (G.space->H.coarse.empty())
+            if (G.space->H.coarse.empty() && !sliced)
                 throw std::invalid_argument("group space needs coarse meshes (the slab solve's multigrid)");
             const int ns = S.mesh.ns(), m = S.mesh.m(), p = S.p, Ns = S.Ns, Nt = S.Nt, nf = S.nf;
-            G.first_y_cell = first_y_cell[gi];
             G.ncells = static_cast<int64_t>(ns) * m;
-            if (G.first_y_cell < 0 || G.first_y_cell + G.ncells > Y.nc)
-                throw std::invalid_argument("group cells outside the Y cells");
             G.nYg = S.nY;
             G.nXg = S.nX;
+            G.sliced = sliced;
+            if (sliced) {
+                G.first_y_cell = 0;
+                G.src.assign(cell_source + cell_ptr[gi], cell_source + cell_ptr[gi] + G.ncells);
+                if (cell_ptr[gi + 1] - cell_ptr[gi] != G.ncells)
+                    throw std::invalid_argument("group " + std::to_string(gi) + ": cell_ptr range differs from its mesh");
+            } else {
+                G.first_y_cell = first_y_cell[gi];
+                if (G.first_y_cell < 0 || G.first_y_cell + G.ncells > Y.nc)
+                    throw std::invalid_argument("group cells outside the Y cells");
+                G.src.resize(static_cast<size_t>(G.ncells));
+                for (int64_t q = 0; q < G.ncells; ++q) G.src[q] = static_cast<int>(G.first_y_cell + q);
+            }
+            for (int v : G.src)
+                if (v < 0 || v >= Y.nc) throw std::invalid_argument("group cell source outside the Y cells");
             // Y map: the group's Y cells must be its mesh's cells slab by slab with identity rows
+            // (sliced: the source cell's polynomial evaluated at the slab's time nodes)
             std::vector<int> ymap(static_cast<size_t>(S.nY), -1);
+            std::vector<std::vector<std::pair<int, double>>> lrows(sliced ? static_cast<size_t>(S.nY) : 0);
+            std::vector<char> lset(sliced ? static_cast<size_t>(S.nY) : 0, 0);
+            std::vector<double> lv(Nt), ld(Nt);
             for (int k = 0; k < m; ++k)
                 for (int c = 0; c < ns; ++c) {
-                    const int64_t yc = G.first_y_cell + static_cast<int64_t>(k) * ns + c;
-                    if (covered[yc]++) throw std::invalid_argument("a Y cell in two groups");
+                    const int64_t yc = G.src[static_cast<size_t>(k) * ns + c];
+                    if (covered[yc]++ && !sliced) throw std::invalid_argument("a Y cell in two groups");
                     const double* gy = &Y.geo[yc * kGeo];
-                    if (std::fabs(gy[kGeo - 2] - S.mesh.times[k]) > 1e-13 ||
-                        std::fabs(gy[kGeo - 1] - S.mesh.times[k + 1]) > 1e-13)
+                    if (sliced) {
+                        if (S.mesh.times[k] < gy[kGeo - 2] - 1e-13 || S.mesh.times[k + 1] > gy[kGeo - 1] + 1e-13)
+                            throw std::invalid_argument("a slab outside its source Y cell's time interval");
+                    } else if (std::fabs(gy[kGeo - 2] - S.mesh.times[k]) > 1e-13 ||
+                               std::fabs(gy[kGeo - 1] - S.mesh.times[k + 1]) > 1e-13) {
                         throw std::invalid_argument("group slab times differ from its Y cells'");
+                    }
                     for (int v = 0; v <= S.dim; ++v)
                         for (int d = 0; d < S.dim; ++d)
                             if (std::fabs(gy[v * S.dim + d] -
                                           S.mesh.vertices[S.mesh.cells[c * (S.dim + 1) + v] * S.dim + d]) > 1e-13)
                                 throw std::invalid_argument("group cell geometry differs from its Y cell's");
+                    if (sliced) {
+                        for (int is = 0; is < Ns; ++is) {
+                            const int f = S.cell_free[static_cast<size_t>(c) * Ns + is];
+                            for (int it = 0; it < Nt; ++it) {
+                                const int loc = is * Nt + it;
+                                const int64_t r0 = Y.ptr[yc * (Ns * Nt) + loc], r1 = Y.ptr[yc * (Ns * Nt) + loc + 1];
+                                if (f < 0) {
+                                    if (r1 != r0) throw std::invalid_argument("Dirichlet node with a non-empty Y row");
+                                    continue;
+                                }
+                                const int64_t td = (static_cast<int64_t>(k) * Nt + it) * nf + f;
+                                if (lset[td]) continue;
+                                lset[td] = 1;
+                                const double t = S.mesh.times[k] + (S.mesh.times[k + 1] - S.mesh.times[k]) * it / p;
+                                apf::segment_lagrange(p, (t - gy[kGeo - 2]) / (gy[kGeo - 1] - gy[kGeo - 2]), lv.data(),
+                                                      ld.data());
+                                auto& row = lrows[td];
+                                for (int j = 0; j < Nt; ++j) {
+                                    if (lv[j] == 0.0) continue;
+                                    const int lj = is * Nt + j;
+                                    for (int64_t e = Y.ptr[yc * (Ns * Nt) + lj]; e < Y.ptr[yc * (Ns * Nt) + lj + 1]; ++e)
+                                        row.emplace_back(Y.dof[e], Y.w[e] * lv[j]);
+                                }
+                            }
+                        }
+                        continue;
+                    }
                     for (int is = 0; is < Ns; ++is)
                         for (int it = 0; it < Nt; ++it) {
                             const int loc = is * Nt + it;
@@ -1677,6 +1737,26 @@ int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_spa
                             ymap[td] = Y.dof[r0];
                         }
                 }
+            std::vector<int64_t> lptr(1, 0);
+            std::vector<int> lcol;
+            std::vector<double> lw;
+            if (sliced) {
+                for (int64_t td = 0; td < S.nY; ++td) {
+                    if (!lset[td]) throw std::invalid_argument("group Y dof without a source cell");
+                    auto& row = lrows[td];
+                    std::sort(row.begin(), row.end());
+                    for (size_t q = 0; q < row.size(); ++q) {
+                        if (static_cast<int64_t>(lcol.size()) > lptr.back() && lcol.back() == row[q].first)
+                            lw.back() += row[q].second;
+                        else {
+                            lcol.push_back(row[q].first);
+                            lw.push_back(row[q].second);
+                        }
+                    }
+                    lptr.push_back(static_cast<int64_t>(lcol.size()));
+                }
+                std::fill(ymap.begin(), ymap.end(), 0);
+            }
             for (int v : ymap)
                 if (v < 0) throw std::invalid_argument("group Y dof without a general Y dof");
             // X interpolation: each group X node evaluated on the X ancestor of a Y cell that holds it
@@ -1699,7 +1779,7 @@ int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_spa
                 for (int f = 0; f < nf; ++f) {
                     double x[2];
                     node_position(S.mesh, p, node_cell[f], node_loc[f], x);
-                    const int64_t yc = G.first_y_cell + static_cast<int64_t>(k) * ns + node_cell[f];
+                    const int64_t yc = G.src[static_cast<size_t>(k) * ns + node_cell[f]];
                     const int64_t a = gsys->host.anc[yc];
                     x_basis_at(X, a, x, t, bv);
                     std::vector<std::pair<int, double>> acc;
@@ -1722,11 +1802,18 @@ int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_spa
                 }
             }
             G.d_ymap = dev(ymap);
+            if (sliced) {
+                G.d_lptr = dev(lptr);
+                G.d_lcol = dev(lcol);
+                G.d_lw = dev(lw);
+            }
             G.d_iptr = dev(iptr);
             G.d_icol = dev(icol);
             G.d_iw = dev(iw);
             apf_system* sys_g = nullptr;
             if (int c = apf_system_create(ctx, G.space, &sys_g)) throw std::runtime_error(g_last_error + " (code " + std::to_string(c) + ")");
+            // a group mesh without coarser meshes: its multigrid is the one level, solved directly
+            if (sliced && !sys_g->H) sys_g->H = &G.space->H;
             G.sys = sys_g;
             ge->groups.push_back(G);
         }
@@ -1745,6 +1832,24 @@ int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_spa
         cleanup();
         return fail(ctx, APF_BAD_ARG, std::string("apf_general_estimator_create: ") + e.what());
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_space* const* spaces,
+                                 const int64_t* first_y_cell, apf_general_estimator** out)
+{
+    return estimator_create(gsys, n_groups, spaces, first_y_cell, nullptr, nullptr, out);
+}
+
+int apf_general_estimator_create_sliced(apf_general_system* gsys, int n_groups, apf_space* const* spaces,
+                                        const int64_t* cell_ptr, const int* cell_source, apf_general_estimator** out)
+{
+    if (!cell_ptr || !cell_source)
+        return fail(gsys ? gsys->ctx : nullptr, APF_BAD_ARG, "apf_general_estimator_create_sliced: null argument");
+    return estimator_create(gsys, n_groups, spaces, nullptr, cell_ptr, cell_source, out);
 }
 
 void apf_general_estimator_destroy(apf_general_estimator* ge)
@@ -1778,22 +1883,42 @@ int apf_general_estimate(apf_general_estimator* ge, const double* x, int prec, i
         hipStream_t st = ctx->stream;
         const int64_t nY = ge->gsys->nY;
         HIP_TRY(hipMemcpyAsync(ge->d_x, x, ge->gsys->n * sizeof(double), hipMemcpyHostToDevice, st));
+        const int64_t nyc = ge->gsys->nyc;
+        for (double* a : {flux_by_cell, resid_by_cell, recon_by_cell})
+            if (a) std::fill(a, a + nyc, 0.0);
         apf_estimate_out tot{};
         tot.slab_stop = APF_SLAB_STOP_TOL;
         double div2 = 0.0;
         for (auto& g : ge->groups) {
             apf::System* sg = g.sys;
-            hipLaunchKernelGGL(k_gather, dim3(1024), dim3(256), 0, st, g.nYg, g.d_ymap, ge->d_x, sg->d_x);
+            if (g.sliced)
+                hipLaunchKernelGGL(k_interp, dim3(1024), dim3(256), 0, st, g.nYg, g.d_lptr, g.d_lcol, g.d_lw, ge->d_x,
+                                   sg->d_x);
+            else
+                hipLaunchKernelGGL(k_gather, dim3(1024), dim3(256), 0, st, g.nYg, g.d_ymap, ge->d_x, sg->d_x);
             hipLaunchKernelGGL(k_interp, dim3(1024), dim3(256), 0, st, g.nXg, g.d_iptr, g.d_icol, g.d_iw, ge->d_x + nY,
                                sg->d_x + g.nYg);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(st));
             apf_estimate_out o{};
-            const int64_t off = g.first_y_cell;
-            const int code = apf_estimate_ex(g.sys, prec, refine_steps, APF_SLAB_SOLVE, &o,
-                                             flux_by_cell ? flux_by_cell + off : nullptr,
-                                             resid_by_cell ? resid_by_cell + off : nullptr,
-                                             recon_by_cell ? recon_by_cell + off : nullptr);
+            int code;
+            if (g.sliced) {
+                // per-slice values summed into the source cells (by_source_cell)
+                std::vector<double> fl(static_cast<size_t>(g.ncells)), rs(fl.size()), rc(fl.size());
+                code = apf_estimate_ex(g.sys, prec, refine_steps, APF_SLAB_SOLVE, &o, fl.data(), rs.data(), rc.data());
+                if (code == APF_OK)
+                    for (int64_t q = 0; q < g.ncells; ++q) {
+                        if (flux_by_cell) flux_by_cell[g.src[q]] += fl[q];
+                        if (resid_by_cell) resid_by_cell[g.src[q]] += rs[q];
+                        if (recon_by_cell) recon_by_cell[g.src[q]] += rc[q];
+                    }
+            } else {
+                const int64_t off = g.first_y_cell;
+                code = apf_estimate_ex(g.sys, prec, refine_steps, APF_SLAB_SOLVE, &o,
+                                       flux_by_cell ? flux_by_cell + off : nullptr,
+                                       resid_by_cell ? resid_by_cell + off : nullptr,
+                                       recon_by_cell ? recon_by_cell + off : nullptr);
+            }
             if (code != APF_OK) return code;
             tot.lambda_y2 += o.lambda_y2;
             // ||u0 - u(0)||^2 lives on the group that starts at t = 0 (a later group's

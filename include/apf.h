@@ -372,6 +372,16 @@ int apf_general_graph_norm(apf_general_system* sys, const apf_general_coeff* coe
 typedef struct apf_general_estimator apf_general_estimator;
 int apf_general_estimator_create(apf_general_system* gsys, int n_groups, apf_space* const* spaces,
                                  const int64_t* first_y_cell, apf_general_estimator** out);
+/* The same for Y cells that span several time slabs (adaptive meshes): the slabs lie between
+ * consecutive time endpoints of the Y cells and each Y cell is sliced into the slabs it spans
+ * (time_slab_builder.hpp:53-120).  Cell (slab k, spatial cell c) of group g is a slice of the
+ * general system's Y cell cell_source[cell_ptr[g] + k * n_space_cells + c] with the same
+ * spatial cell; lambda at the group's Y nodes is that cell's polynomial at the slab's time
+ * nodes, and the per-cell outputs of apf_general_estimate are summed into the source cells
+ * (by_source_cell).  A group space without coarse meshes solves its slab problems on one
+ * multigrid level (a dense inverse per time shift, at most 4096 coupled nodes). */
+int apf_general_estimator_create_sliced(apf_general_system* gsys, int n_groups, apf_space* const* spaces,
+                                        const int64_t* cell_ptr, const int* cell_source, apf_general_estimator** out);
 void apf_general_estimator_destroy(apf_general_estimator* ge);
 /* coefficients of group g's tensor system (M per spatial cell of the group mesh, u0 at
  * its bottom points; only the group that starts at t = 0 uses u0) */

@@ -636,3 +636,93 @@ def graph_norm(X: Space, Y: Space, K, kappa=1.0, parts=False):
     C = Kd[nY:, nY:]
     H = -C + (Bdt / Ay_diag[None, :]) @ Bdt.T + Ax
     return (H, Bdt, Ax) if parts else H
+
+
+def _value_on_cell(space: Space, local, cell, x, t):
+    """value at (x, t) of the function with local coefficients `local` [NL] on active cell
+    position `cell` (Lagrange basis, local dof i_space * (p+1) + j_time)"""
+    p = space.p
+    x0, x1, t0, t1 = space.tree.box(space.active[cell])
+    vs, _ = segment_lagrange(p, [(x - x0) / (x1 - x0)])
+    vt, _ = segment_lagrange(p, [(t - t0) / (t1 - t0)])
+    return float(np.outer(vs[0], vt[0]).ravel() @ local)
+
+
+def slab_groups(Y: Space):
+    """The time slabs between consecutive time endpoints of Y's cells
+    (time_slabs/time_slab_builder.hpp:53-120, each source cell sliced into the slabs it
+    spans), runs of consecutive slabs with one spatial partition grouped: per group the
+    interval endpoints, the slab times and, per slab, the source cell (position in Y's
+    active list) of each interval, left to right."""
+    boxes = [Y.tree.box(c) for c in Y.active]
+    times = sorted({b[2] for b in boxes} | {b[3] for b in boxes})
+    groups = []
+    for k in range(len(times) - 1):
+        t0, t1 = times[k], times[k + 1]
+        cells = sorted((b[0], b[1], a) for a, b in enumerate(boxes) if b[2] <= t0 and b[3] >= t1)
+        part = tuple((c[0], c[1]) for c in cells)
+        if groups and groups[-1]["part"] == part:
+            groups[-1]["times"].append(t1)
+            groups[-1]["src"].append([c[2] for c in cells])
+        else:
+            groups.append(dict(part=part, times=[t0, t1], src=[[c[2] for c in cells]]))
+    return groups
+
+
+def estimate_y(X: Space, Y: Space, x, u0, kappa=1.0, ell=0.0):
+    """The Y estimator of x = [lambda; u] on the adaptive pair, slab group by slab group:
+    on each group (slabs of one spatial partition = a tensor mesh) the tensor oracle
+    estimator (oracle.estimator.estimate, per-slab direct reconstruction solves, per
+    (slab, vertex) patches) on [lambda of the source cells at the group's Y nodes; u at its
+    X nodes]; per-cell values summed into the source cells
+    (time_slab_adaptive_loop.hpp:833-2026, by_source_cell)."""
+    from . import assembly as OA
+    from . import estimator as OE
+    from . import mesh as OM
+
+    p = Y.p
+    Nt = p + 1
+    lamloc = Y.local_values(x[:Y.n_true])
+    uloc = X.local_values(x[Y.n_true:])
+    xboxes = [X.tree.box(c) for c in X.active]
+    anc = Y.ancestors_in(X)
+    out = dict(flux2=0.0, recon2=0.0, flux_by_cell=np.zeros(len(Y.active)),
+               recon_by_cell=np.zeros(len(Y.active)), resid_by_cell=np.zeros(len(Y.active)))
+    for g in slab_groups(Y):
+        verts = np.array([g["part"][0][0]] + [b for (_, b) in g["part"]])
+        ns = len(g["part"])
+        sm = OM.SpatialMesh(1, verts[:, None], np.stack([np.arange(ns), np.arange(1, ns + 1)], axis=1))
+        stm = OM.SpaceTimeMesh(sm, np.array(g["times"]))
+        od = OM.spacetime_dofs(stm, p)
+        prob = OA.Problem(kappa * np.ones((ns, 1, 1)), ell, lambda pts: u0(pts[:, 0]))
+        S = OA.assemble(stm, od, prob)
+        m = stm.n_intervals
+        lam = np.zeros(od.nY)
+        u = np.zeros(od.nX)
+        for k in range(m):
+            tk, hk = stm.times[k], stm.times[k + 1] - stm.times[k]
+            for c in range(ns):
+                a = g["src"][k][c]
+                xa, xb = verts[c], verts[c + 1]
+                for i in range(p + 1):
+                    xi = xa + (xb - xa) * i / p
+                    for l in range(Nt):
+                        t = tk + hk * l / p
+                        yd = od.y_map[k * ns + c, i * Nt + l]
+                        if yd >= 0:
+                            lam[yd] = _value_on_cell(Y, lamloc[a], a, xi, t)
+                        xd = od.x_map[k * ns + c, i * Nt + l]
+                        if xd >= 0:
+                            xa_cell = anc[a]
+                            u[xd] = _value_on_cell(X, uloc[xa_cell], xa_cell, xi, t)
+        r = OE.estimate(stm, od, prob, S, np.concatenate([lam, u]))
+        out["flux2"] += r.flux2
+        out["recon2"] += r.recon2
+        for k in range(m):
+            for c in range(ns):
+                a = g["src"][k][c]
+                for key in ("flux_by_cell", "recon_by_cell", "resid_by_cell"):
+                    out[key][a] += getattr(r, key)[k * ns + c]
+    out["y_est2"] = out["flux2"] + out["recon2"]
+    del xboxes
+    return out

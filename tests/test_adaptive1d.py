@@ -228,3 +228,27 @@ def test_oracle_graph_norm_parts(p):
     Ay = K1.toarray()[:nY, :nY]
     np.testing.assert_allclose(Ax, E.T @ Ay @ E, atol=1e-12)
     assert np.abs(H - H.T).max() < 1e-14 and np.linalg.eigvalsh(H).min() > 0.0
+
+
+@pytest.mark.parametrize("p,steps", [(1, 6), (2, 4), (3, 3)])
+def test_oracle_y_estimator_reproduces_published_runs(p, steps):
+    """The oracle's Y estimator on the adaptive pairs (slab groups of one spatial partition,
+    tensor estimator per group, per-cell values summed into the source Y cells) against the
+    published y_estimator^2 of each step (6 printed digits; the published solve is direct,
+    the oracle's too)."""
+    gold = [a for a in json.load(open(GOLD)) if a["p"] == p][:steps]
+    T = O.Tree()
+    X = O.Space(T, p, True, T.leaves())
+    X.refine(list(X.active))
+    for a in gold:
+        Y = O.Space(T, p, False, X.active)
+        K, b = O.assemble(X, Y, u0)
+        x = O.solve(K, b)
+        e = O.estimate_y(X, Y, x, u0)
+        assert e["y_est2"] == pytest.approx(a["y_estimator_squared"], rel=6e-6), a["source"]
+        assert e["flux_by_cell"].sum() + e["recon_by_cell"].sum() == pytest.approx(e["y_est2"], rel=1e-12)
+        l2, t2 = O.x_indicators(X, Y, x, u0)
+        lx = np.zeros(len(X.active))
+        np.add.at(lx, Y.ancestors_in(X), l2)
+        marked, _ = O.doerfler_mark(lx + t2, 0.4, ids=X.active)
+        X.refine(marked)

@@ -100,3 +100,54 @@ def test_gpu_graph_norm_matches_oracle_and_preconditions_minres(ctx, p, steps):
         del H, gs
         rec = run.step()
         OX.refine([int(c) for c in rec["marked_x_cells"]])
+
+
+@pytest.mark.parametrize("p,steps,max_direct", [(1, 10, 64), (2, 7, 64), (3, 6, 64), (2, 6, 4)])
+def test_gpu_y_estimator_on_adaptive_meshes(ctx, p, steps, max_direct):
+    """The Y estimator on the reference's adaptive meshes (Y cells sliced into the time
+    slabs they span, apf_general_estimator_create_sliced): the published y_estimator^2 of
+    every step to its 6 digits, and the oracle's slab-group estimator on the same spaces and
+    the same GPU solution to 1e-10 (totals, and per source cell to 1e-10 of the largest cell value).  max_direct = 4 gives the
+    larger slab meshes coarse meshes, so the slab solve runs its multigrid."""
+    gold = [a for a in json.load(open(GOLD)) if a["p"] == p][:steps]
+    run = A.AdaptiveX1D(ctx, p=p, theta_x=0.4, tol=1e-10, max_it=40000)
+    T = O.Tree()
+    OX = O.Space(T, p, True, T.leaves())
+    OX.refine(list(OX.active))
+    for a in gold:
+        OY = O.Space(T, p, False, OX.active)
+        pair = A.SpaceTimePair(run.X, run.Y)
+        gs = apf.GeneralSystem(ctx, pair)
+        gs.assemble(u0)
+        x, st = gs.matrix().minres(gs.rhs(), gs.nY, tol=1e-10, max_it=40000, precond=apf.APF_FP64)
+        ge = apf.GeneralEstimator(gs, A.slab_groups(run.Y, max_direct=max_direct))
+        ge.assemble(A.smooth_initial_u0)
+        est = ge.estimate(x, apf.APF_FP64, 0, per_cell=True)
+        print(f"p={p} outer={a['outer']} X={run.X.n_dofs} y_est2={est['y_est2']:.6g} "
+              f"(published {a['y_estimator_squared']:.6g}), slab its {est['slab_iterations']}", flush=True)
+        assert est["y_est2"] == pytest.approx(a["y_estimator_squared"], rel=6e-6), a["source"]
+        eo = O.estimate_y(OX, OY, x, u0)
+        assert est["y_est2"] == pytest.approx(eo["y_est2"], rel=1e-10)
+        for key in ("flux_by_cell", "recon_by_cell"):
+            # per cell to 1e-10 of the largest cell value; reconstruction terms at round-off
+            # (the slabs of a uniform Y mesh) to 1e-12 of the total
+            diff = np.abs(est[key] - eo[key]).max()
+            assert diff <= 1e-10 * np.abs(eo[key]).max() + 1e-12 * eo["y_est2"], (key, diff)
+        del ge, gs
+        rec = run.step()
+        OX.refine([int(c) for c in rec["marked_x_cells"]])
+
+
+def test_gpu_adaptive_loop_reports_y_estimator(ctx):
+    """AdaptiveX1D(with_y_estimator=True): each step's record carries the Y estimator of the
+    step's solution (the published y_estimator^2 to its 6 digits) beside the X indicators."""
+    gold = [a for a in json.load(open(GOLD)) if a["p"] == 2][:5]
+    run = A.AdaptiveX1D(ctx, p=2, theta_x=0.4, tol=1e-10, max_it=40000, with_y_estimator=True)
+    for a in gold:
+        rec = run.step()
+        est = rec["y_estimator"]
+        print(f"outer={a['outer']} y_est2={rec['y_estimator_squared']:.6g} slab its {est['slab_iterations']}",
+              flush=True)
+        assert rec["y_estimator_squared"] == pytest.approx(a["y_estimator_squared"], rel=6e-6), a["source"]
+        assert rec["eta_squared"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
+        assert est["flux_by_cell"].sum() + est["recon_by_cell"].sum() == pytest.approx(est["y_est2"], rel=1e-12)
