@@ -21,6 +21,7 @@ whenever the Y estimator meets its threshold at once.
 from __future__ import annotations
 
 import ctypes
+import time
 
 import numpy as np
 
@@ -220,8 +221,13 @@ class AdaptiveX1D:
     """Outer X loop of the reference's adaptive driver on 1+1D meshes, on the GPU."""
 
     def __init__(self, ctx: Context, p=1, theta_x=0.4, rho=1.0, x_uniform_levels=1, u0=smooth_initial_u0,
-                 tol=1e-10, max_it=40000, with_y_estimator=False, estimator_precision=APF_FP64):
+                 tol=1e-10, max_it=40000, with_y_estimator=False, estimator_precision=APF_FP64, hhat=True,
+                 cheb_degree=16):
+        """hhat: precondition the X block with the graph norm Hhat formed on the device (the
+        reference's choice, parabolic_graph_norm_minres.hpp:46-181), else with the Schur
+        complement's diagonal; cheb_degree: Chebyshev steps of each block."""
         self.ctx, self.p, self.theta_x, self.rho, self.u0 = ctx, p, theta_x, rho, u0
+        self.hhat, self.cheb_degree = hhat, cheb_degree
         self.with_y_estimator, self.estimator_precision = with_y_estimator, estimator_precision
         self.tol, self.max_it = tol, max_it
         self.mesh = Mesh1D(0.0, 1.0, 0.0, 1.0)
@@ -238,7 +244,12 @@ class AdaptiveX1D:
         gs = GeneralSystem(self.ctx, pair)
         ms = gs.assemble(self.u0)
         K = gs.matrix()
-        x, stats = K.minres(gs.rhs(), gs.nY, tol=self.tol, max_it=self.max_it, precond=APF_FP64)
+        t0 = time.perf_counter()
+        H = gs.graph_norm() if self.hhat else None
+        x, stats = K.minres(gs.rhs(), gs.nY, hhat=H, tol=self.tol, max_it=self.max_it, degree=self.cheb_degree,
+                            precond=APF_FP64)
+        t_solve = time.perf_counter() - t0
+        del H
         lam2_y, tr2_x = gs.x_indicators(x, self.u0)
         lam2_x = np.zeros(self.X.n_cells)
         np.add.at(lam2_x, pair.y_ancestor, lam2_y)
@@ -249,7 +260,7 @@ class AdaptiveX1D:
                    eta_squared=float(eta2_x.sum()),
                    y_estimator_threshold_squared=y_estimator_threshold(self.rho, lam2, tr2),
                    minres_iterations=stats["iterations"], minres_rel_residual=stats["true_relative_residual"],
-                   assembly_ms=ms, solution=x, eta2_by_x_cell=eta2_x)
+                   minres_seconds=t_solve, assembly_ms=ms, solution=x, eta2_by_x_cell=eta2_x)
         if self.with_y_estimator:
             ge = GeneralEstimator(gs, slab_groups(self.Y))
             ge.assemble(self.u0)

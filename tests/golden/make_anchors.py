@@ -12,8 +12,8 @@ true DOFs equal to the degree-(p+1) uniform Y true DOFs at outer iteration 0, an
 the same eta^2 and lambda_y^2 as that uniform record.
 
 Also writes tests/golden/adaptive1d_anchors.json: the 1+1D smooth_initial adaptive runs
-(mode adaptive_standard, p = 1, 2, 3), one record per outer iteration while the inner Y
-loop accepted its first iterate (inner iteration 0 with Y true DOFs unchanged and the
+(modes adaptive_standard and adaptive_force_effective_rho_no_inner_refinement, p = 1, 2, 3),
+one record per outer iteration while the inner Y loop accepted its first iterate (inner iteration 0 with Y true DOFs unchanged and the
 next record starting the next outer iteration): X true DOFs before and after the X
 refinement, Y true DOFs, eta^2, lambda_Y^2, y_estimator^2 and its threshold.
 
@@ -78,9 +78,10 @@ def main():
     path = os.path.join(REF, "algorithm_data_plot", "1d", "smooth_initial.csv")
     with open(path) as fh:
         rows = [(i + 2, row) for i, row in enumerate(csv.DictReader(fh))]
-    for p in (1, 2, 3):
+    for mode, p in [(m, p) for m in ("adaptive_standard", "adaptive_force_effective_rho_no_inner_refinement")
+                    for p in (1, 2, 3)]:
         inner = [(ln, r) for ln, r in rows
-                 if r["record_type"] == "inner" and r["mode"] == "adaptive_standard" and r["p_int"] == str(p)]
+                 if r["record_type"] == "inner" and r["mode"] == mode and r["p_int"] == str(p)]
         for k, (ln, r) in enumerate(inner):
             if int(r["outer_iteration"]) != k or r["inner_iteration"] != "0":
                 break

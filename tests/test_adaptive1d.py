@@ -25,6 +25,11 @@ from oracle.basis import segment_lagrange
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adaptive1d_anchors.json")
 
 
+def published(p, mode="adaptive_standard"):
+    """published records of the 1+1D smooth_initial run of degree p in the given mode"""
+    return [a for a in json.load(open(GOLD)) if a["p"] == p and a["mode"] == mode]
+
+
 def u0(x):
     return np.sin(np.pi * x)
 
@@ -174,7 +179,7 @@ def test_spaces_are_conforming(p):
 
 @pytest.mark.parametrize("p,steps", [(1, 17), (2, 12), (3, 10)])
 def test_oracle_reproduces_published_adaptive_runs(p, steps):
-    gold = [a for a in json.load(open(GOLD)) if a["p"] == p][:steps]
+    gold = published(p)[:steps]
     assert len(gold) == steps
     run = O.adaptive_x_run(p, steps)
     for rec, a in zip(run, gold):
@@ -236,7 +241,7 @@ def test_oracle_y_estimator_reproduces_published_runs(p, steps):
     tensor estimator per group, per-cell values summed into the source Y cells) against the
     published y_estimator^2 of each step (6 printed digits; the published solve is direct,
     the oracle's too)."""
-    gold = [a for a in json.load(open(GOLD)) if a["p"] == p][:steps]
+    gold = published(p)[:steps]
     T = O.Tree()
     X = O.Space(T, p, True, T.leaves())
     X.refine(list(X.active))
@@ -252,3 +257,16 @@ def test_oracle_y_estimator_reproduces_published_runs(p, steps):
         np.add.at(lx, Y.ancestors_in(X), l2)
         marked, _ = O.doerfler_mark(lx + t2, 0.4, ids=X.active)
         X.refine(marked)
+
+
+def test_oracle_reproduces_published_no_inner_history():
+    """The published p = 1 run without inner Y refinement (mode
+    adaptive_force_effective_rho_no_inner_refinement: Y = X's mesh every outer iteration)
+    through outer iteration 19 (X = 4631): the same X marking as the standard run while that
+    one accepts its first Y iterate, and beyond it."""
+    gold = published(1, "adaptive_force_effective_rho_no_inner_refinement")[:20]
+    run = O.adaptive_x_run(1, 20)
+    for rec, a in zip(run, gold):
+        assert (rec["x_true_dofs"], rec["y_true_dofs"]) == (a["x_true_dofs"], a["y_true_dofs"]), a["source"]
+        assert rec["eta2"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
+        assert rec["lambda_y2"] == pytest.approx(a["lambda_y_squared"], rel=6e-6), a["source"]

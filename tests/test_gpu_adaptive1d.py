@@ -25,6 +25,11 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adaptive1d_anchors.json")
 
 
+def published(p, mode="adaptive_standard"):
+    """published records of the 1+1D smooth_initial run of degree p in the given mode"""
+    return [a for a in json.load(open(GOLD)) if a["p"] == p and a["mode"] == mode]
+
+
 @pytest.fixture(scope="module")
 def ctx():
     return apf.Context(0)
@@ -36,7 +41,7 @@ def u0(x):
 
 @pytest.mark.parametrize("p,steps", [(1, 17), (2, 12), (3, 10)])
 def test_gpu_adaptive_run_reproduces_published_history(ctx, p, steps):
-    gold = [a for a in json.load(open(GOLD)) if a["p"] == p][:steps]
+    gold = published(p)[:steps]
     run = A.AdaptiveX1D(ctx, p=p, theta_x=0.4, tol=1e-10, max_it=40000)
     T = O.Tree()
     OX = O.Space(T, p, True, T.leaves())
@@ -109,7 +114,7 @@ def test_gpu_y_estimator_on_adaptive_meshes(ctx, p, steps, max_direct):
     every step to its 6 digits, and the oracle's slab-group estimator on the same spaces and
     the same GPU solution to 1e-10 (totals, and per source cell to 1e-10 of the largest cell value).  max_direct = 4 gives the
     larger slab meshes coarse meshes, so the slab solve runs its multigrid."""
-    gold = [a for a in json.load(open(GOLD)) if a["p"] == p][:steps]
+    gold = published(p)[:steps]
     run = A.AdaptiveX1D(ctx, p=p, theta_x=0.4, tol=1e-10, max_it=40000)
     T = O.Tree()
     OX = O.Space(T, p, True, T.leaves())
@@ -141,7 +146,7 @@ def test_gpu_y_estimator_on_adaptive_meshes(ctx, p, steps, max_direct):
 def test_gpu_adaptive_loop_reports_y_estimator(ctx):
     """AdaptiveX1D(with_y_estimator=True): each step's record carries the Y estimator of the
     step's solution (the published y_estimator^2 to its 6 digits) beside the X indicators."""
-    gold = [a for a in json.load(open(GOLD)) if a["p"] == 2][:5]
+    gold = published(2)[:5]
     run = A.AdaptiveX1D(ctx, p=2, theta_x=0.4, tol=1e-10, max_it=40000, with_y_estimator=True)
     for a in gold:
         rec = run.step()
@@ -151,3 +156,27 @@ def test_gpu_adaptive_loop_reports_y_estimator(ctx):
         assert rec["y_estimator_squared"] == pytest.approx(a["y_estimator_squared"], rel=6e-6), a["source"]
         assert rec["eta_squared"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
         assert est["flux_by_cell"].sum() + est["recon_by_cell"].sum() == pytest.approx(est["y_est2"], rel=1e-12)
+
+
+FORCE = "adaptive_force_effective_rho_no_inner_refinement"
+
+
+@pytest.mark.parametrize("p,steps", [(1, 28), (2, 22), (3, 20)])
+def test_gpu_adaptive_run_reproduces_published_no_inner_history(ctx, p, steps):
+    """The published 1+1D runs without inner Y refinement (Y = X's mesh every outer
+    iteration, algorithm_data_plot/1d/smooth_initial.csv), as far as a step solves in a few
+    seconds: X up to 44,562 (p = 1), 31,379 (p = 2) and 15,356 (p = 3) true DoFs.  Every
+    step's X / Y true-DoF counts equal the published ones, and eta^2, lambda_Y^2 match their 6
+    published digits; MINRES (Hhat-preconditioned) reaches the reference's 1e-10 acceptance."""
+    gold = published(p, FORCE)[:steps]
+    assert len(gold) == steps
+    run = A.AdaptiveX1D(ctx, p=p, theta_x=0.4, tol=1e-10, max_it=100000)
+    for a in gold:
+        rec = run.step()
+        print(f"p={p} outer={a['outer']} X={rec['x_true_dofs']} its={rec['minres_iterations']} "
+              f"solve={rec['minres_seconds']:.2f}s", flush=True)
+        assert (rec["x_true_dofs"], rec["y_true_dofs"]) == (a["x_true_dofs"], a["y_true_dofs"]), a["source"]
+        assert rec["minres_rel_residual"] <= 1e-10
+        assert rec["eta_squared"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
+        assert rec["lambda_y_squared"] == pytest.approx(a["lambda_y_squared"], rel=6e-6), a["source"]
+        assert rec["x_true_dofs_after"] == a["x_true_dofs_after"], a["source"]
