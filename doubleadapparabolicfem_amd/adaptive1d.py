@@ -6,17 +6,17 @@ numbers and restriction rows are the reference's.  An FESpace1D carries the arra
 apf_cell_space_desc, so a pair (X, Y) feeds GeneralSystem (the GPU element-quadrature
 assembly) directly.
 
-AdaptiveX1D runs the outer loop of the reference's adaptive driver
-(algorithm/adaptive_driver.hpp:130-1230) on that pair: X = FESpace(SpaceTimePolicy) on
-the root cell refined `x_uniform_levels` times, Y = FESpace(SpaceOnlyPolicy) initialised
-on X's active cells (algorithm/examples/space_time_1d/space_time_1d_example.hpp:113-132 and
-adaptive_driver.hpp:1129); per outer iteration the saddle system is assembled and solved on
-the GPU, the X indicator eta^2 = lambda_Y^2 + initial-trace^2 per X cell
-(indicator_utilities.hpp:435-473) is computed on the GPU, X is Doerfler-marked with
-theta_x (adaptive_driver.hpp:974-1012) and refined (x_space_.refine, :1117), and Y is
-re-initialised on X's active cells.  The inner Y loop is not run here: each outer
-iteration takes Y = X's mesh, which is the inner loop's first iterate and its result
-whenever the Y estimator meets its threshold at once.
+AdaptiveX1D runs the reference's adaptive driver (algorithm/adaptive_driver.hpp:130-1230)
+on that pair: X = FESpace(SpaceTimePolicy) on the root cell refined `x_uniform_levels`
+times, Y = FESpace(SpaceOnlyPolicy) initialised on X's active cells
+(algorithm/examples/space_time_1d/space_time_1d_example.hpp:113-132 and
+adaptive_driver.hpp:1129).  Per outer iteration the saddle system is assembled and solved on
+the GPU and the X indicator eta^2 = lambda_Y^2 + initial-trace^2 per X cell
+(indicator_utilities.hpp:435-473) is computed on the GPU.  With the inner Y loop the Y
+estimator decides whether Y is accepted or Doerfler-marked and refined, and the solve
+repeats; without it Y is X's mesh.  X is then Doerfler-marked with theta_x
+(adaptive_driver.hpp:974-1012) and refined (x_space_.refine, :1117), and Y is
+re-initialised on X's active cells.
 """
 from __future__ import annotations
 
@@ -25,8 +25,8 @@ import time
 
 import numpy as np
 
-from . import (APF_FP64, Context, GeneralEstimator, GeneralSystem, _check, _ptr, doerfler_mark, lib,
-               y_estimator_threshold)
+from . import (APF_FP64, Context, GeneralEstimator, GeneralSystem, _check, _ptr, combine_estimator,
+               doerfler_mark, lib, y_estimator_threshold)
 
 
 class Mesh1D:
@@ -218,17 +218,27 @@ def smooth_initial_u0(x):
 
 
 class AdaptiveX1D:
-    """Outer X loop of the reference's adaptive driver on 1+1D meshes, on the GPU."""
+    """The reference's adaptive driver on 1+1D meshes, on the GPU (adaptive_driver.hpp:130-1230).
+
+    Each outer iteration solves on (X, Y), computes the X indicators and, with
+    inner_y_loop=True, runs the inner Y loop: the Y estimator by source cell (flux^2 +
+    reconstruction^2) against its threshold rho^2 (lambda_Y^2 + (1 + 1/(rho + 1/2)^2) trace^2);
+    above it the Y cells are Doerfler-marked with theta_y and Y is refined with its grading
+    closure (time_slab_adaptive_refinement.hpp:240-338, no local time-slab closure), and the
+    iteration repeats; an empty marking stops the loop.  Without the inner loop Y is X's mesh
+    (the published runs without inner refinement).  Then the X cells are marked with theta_x on
+    the last iteration's indicators, X is refined and Y re-initialised on X's active cells."""
 
     def __init__(self, ctx: Context, p=1, theta_x=0.4, rho=1.0, x_uniform_levels=1, u0=smooth_initial_u0,
                  tol=1e-10, max_it=40000, with_y_estimator=False, estimator_precision=APF_FP64, hhat=True,
-                 cheb_degree=16):
+                 cheb_degree=16, inner_y_loop=False, theta_y=0.4, max_inner=50):
         """hhat: precondition the X block with the graph norm Hhat formed on the device (the
         reference's choice, parabolic_graph_norm_minres.hpp:46-181), else with the Schur
         complement's diagonal; cheb_degree: Chebyshev steps of each block."""
         self.ctx, self.p, self.theta_x, self.rho, self.u0 = ctx, p, theta_x, rho, u0
         self.hhat, self.cheb_degree = hhat, cheb_degree
-        self.with_y_estimator, self.estimator_precision = with_y_estimator, estimator_precision
+        self.with_y_estimator, self.estimator_precision = with_y_estimator or inner_y_loop, estimator_precision
+        self.inner_y_loop, self.theta_y, self.max_inner = inner_y_loop, theta_y, max_inner
         self.tol, self.max_it = tol, max_it
         self.mesh = Mesh1D(0.0, 1.0, 0.0, 1.0)
         self.X = FESpace1D(self.mesh, p, True, self.mesh.leaf_cells())
@@ -237,9 +247,8 @@ class AdaptiveX1D:
         self.Y = FESpace1D(self.mesh, p, False, self.X.active)
         self.history = []
 
-    def step(self, refine=True):
-        """one outer iteration: assemble, solve, indicators, mark, refine X, re-initialise Y.
-        Returns the iteration's record."""
+    def _solve(self):
+        """assemble, solve, X indicators (and the Y estimator) on the current pair"""
         pair = SpaceTimePair(self.X, self.Y)
         gs = GeneralSystem(self.ctx, pair)
         ms = gs.assemble(self.u0)
@@ -262,19 +271,47 @@ class AdaptiveX1D:
                    minres_iterations=stats["iterations"], minres_rel_residual=stats["true_relative_residual"],
                    minres_seconds=t_solve, assembly_ms=ms, solution=x, eta2_by_x_cell=eta2_x)
         if self.with_y_estimator:
+            t0 = time.perf_counter()
             ge = GeneralEstimator(gs, slab_groups(self.Y))
             ge.assemble(self.u0)
             est = ge.estimate(x, self.estimator_precision, 2, per_cell=True)
             rec["y_estimator"] = est
             rec["y_estimator_squared"] = est["y_est2"]
+            rec["y_estimator_by_cell"] = combine_estimator(est["flux_by_cell"], est["recon_by_cell"])
+            rec["estimator_seconds"] = time.perf_counter() - t0
             del ge
+        del K
+        del gs
+        return rec
+
+    def step(self, refine=True):
+        """one outer iteration (with its inner Y loop when enabled); returns the record of the
+        accepted (last) inner iteration, with the list of all of them under "inner"."""
+        inner = []
+        for it in range(self.max_inner if self.inner_y_loop else 1):
+            rec = self._solve()
+            rec["inner_iteration"] = it
+            rec["y_true_dofs_before"] = self.Y.n_dofs
+            inner.append(rec)
+            if not self.inner_y_loop:
+                break
+            rec["accepted"] = rec["y_estimator_squared"] <= rec["y_estimator_threshold_squared"]
+            if rec["accepted"]:
+                break
+            marked_y, _ = doerfler_mark(rec["y_estimator_by_cell"], self.theta_y,
+                                        ids=self.Y.active.astype(np.int64))
+            rec["marked_y_cells"] = np.asarray(marked_y)
+            if len(marked_y) == 0 or it + 1 >= self.max_inner:
+                break
+            self.Y.refine(marked_y)
+            rec["y_true_dofs_after"] = self.Y.n_dofs
+        rec["inner"] = inner
+        rec.setdefault("y_true_dofs_after", self.Y.n_dofs)
         if refine:
-            marked, _ = doerfler_mark(eta2_x, self.theta_x, ids=self.X.active.astype(np.int64))
+            marked, _ = doerfler_mark(rec["eta2_by_x_cell"], self.theta_x, ids=self.X.active.astype(np.int64))
             rec["marked_x_cells"] = np.asarray(marked)
             self.X.refine(marked)
             self.Y = FESpace1D(self.mesh, self.p, False, self.X.active)
             rec["x_true_dofs_after"] = self.X.n_dofs
-        del K
-        del gs
         self.history.append(rec)
         return rec

@@ -44,7 +44,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from .basis import segment_lagrange
-from .marking import doerfler_mark
+from .marking import doerfler_mark, y_estimator_threshold
 from .quadrature import gauss_legendre_01
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -586,6 +586,56 @@ def adaptive_x_run(p, steps, theta_x=0.4, x_uniform_levels=1, u0=lambda x: np.si
         out.append(dict(x_true_dofs=X.n_true, y_true_dofs=Y.n_true, eta2=float(eta2.sum()),
                         lambda_y2=float(lam2_y.sum()), trace2=float(tr2_x.sum()), marked=list(marked)))
         X.refine(marked)
+    return out
+
+
+def adaptive_run(p, outer_steps, theta_x=0.4, theta_y=0.4, rho=1.0, max_inner=50, x_uniform_levels=1,
+                 u0=lambda x: np.sin(np.pi * x)):
+    """The adaptive driver with its inner Y loop (adaptive_driver.hpp:130-1230, the
+    parameters of the published adaptive_standard runs: rho = 1, theta_x = theta_y = 0.4,
+    deterministic reductions, no local time-slab closure, stop on an empty Y marking).  Per
+    inner iteration: solve on (X, Y), X indicators, the Y estimator by source cell (flux^2 +
+    reconstruction^2), its threshold rho^2 (lambda_Y^2 + (1 + 1/(rho + 1/2)^2) trace^2); accept
+    when the estimator is below it, else Doerfler-mark the Y cells with theta_y and refine Y
+    with its grading closure (time_slab_adaptive_refinement.hpp:240-338, closure disabled).
+    After the accepted iteration the X cells are marked with theta_x on that iteration's
+    indicators, X is refined and Y re-initialised on X's active cells (adaptive_driver.hpp:1129).
+    Returns one record per inner iteration."""
+    T = Tree()
+    X = Space(T, p, True, T.leaves())
+    for _ in range(x_uniform_levels):
+        X.refine(list(X.active))
+    Y = Space(T, p, False, X.active)
+    out = []
+    for outer in range(outer_steps):
+        for inner in range(max_inner):
+            K, b = assemble(X, Y, u0)
+            x = solve(K, b)
+            lam2_y, tr2_x = x_indicators(X, Y, x, u0)
+            lam2_x = np.zeros(len(X.active))
+            np.add.at(lam2_x, Y.ancestors_in(X), lam2_y)
+            eta2 = lam2_x + tr2_x
+            est = estimate_y(X, Y, x, u0)
+            per_cell = est["flux_by_cell"] + est["recon_by_cell"]
+            y2 = float(per_cell.sum())
+            thr = y_estimator_threshold(rho, float(lam2_y.sum()), float(tr2_x.sum()))
+            rec = dict(outer=outer, inner=inner, x_true_dofs=X.n_true, y_true_dofs_before=Y.n_true,
+                       eta2=float(eta2.sum()), lambda_y2=float(lam2_y.sum()), trace2=float(tr2_x.sum()),
+                       y_est2=y2, threshold2=thr, accepted=y2 <= thr)
+            out.append(rec)
+            if rec["accepted"]:
+                rec["y_true_dofs_after"] = Y.n_true
+                break
+            marked_y, _ = doerfler_mark(per_cell, theta_y, ids=Y.active)
+            rec["marked_y"] = list(marked_y)
+            if not marked_y or inner + 1 >= max_inner:
+                rec["y_true_dofs_after"] = Y.n_true
+                break
+            Y.refine(list(marked_y))
+            rec["y_true_dofs_after"] = Y.n_true
+        marked, _ = doerfler_mark(eta2, theta_x, ids=X.active)
+        X.refine(marked)
+        Y = Space(T, p, False, X.active)
     return out
 
 

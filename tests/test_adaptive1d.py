@@ -270,3 +270,20 @@ def test_oracle_reproduces_published_no_inner_history():
         assert (rec["x_true_dofs"], rec["y_true_dofs"]) == (a["x_true_dofs"], a["y_true_dofs"]), a["source"]
         assert rec["eta2"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
         assert rec["lambda_y2"] == pytest.approx(a["lambda_y_squared"], rel=6e-6), a["source"]
+
+
+def test_oracle_driver_with_inner_loop_reproduces_published_history():
+    """The oracle's driver with the inner Y loop (oracle.adaptive1d.adaptive_run) against every
+    inner-loop record of the published p = 1 adaptive_standard run through outer iteration 18,
+    which includes the first Y refinement (outer 17: Y 5868 -> 7520 true DoFs, accepted at the
+    second inner iteration): counts exactly, eta^2, y_estimator^2 and its threshold to 6 digits."""
+    gold = [a for a in json.load(open(os.path.join(os.path.dirname(GOLD), "adaptive1d_inner_anchors.json")))
+            if a["p"] == 1 and a["outer"] <= 18]
+    run = O.adaptive_run(1, 19)
+    assert len(run) == len(gold)
+    for rec, a in zip(run, gold):
+        assert (rec["outer"], rec["inner"], rec["x_true_dofs"], rec["y_true_dofs_before"], rec["y_true_dofs_after"]) \
+            == (a["outer"], a["inner"], a["x_true_dofs"], a["y_true_dofs_before"], a["y_true_dofs_after"]), a["source"]
+        assert rec["eta2"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
+        assert rec["y_est2"] == pytest.approx(a["y_estimator_squared"], rel=6e-6), a["source"]
+        assert rec["threshold2"] == pytest.approx(a["y_estimator_threshold_squared"], rel=6e-6), a["source"]

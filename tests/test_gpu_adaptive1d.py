@@ -180,3 +180,36 @@ def test_gpu_adaptive_run_reproduces_published_no_inner_history(ctx, p, steps):
         assert rec["eta_squared"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
         assert rec["lambda_y_squared"] == pytest.approx(a["lambda_y_squared"], rel=6e-6), a["source"]
         assert rec["x_true_dofs_after"] == a["x_true_dofs_after"], a["source"]
+
+
+INNER_GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adaptive1d_inner_anchors.json")
+
+
+@pytest.mark.parametrize("p,outer_steps", [(1, 26), (2, 25), (3, 21)])
+def test_gpu_adaptive_driver_with_inner_loop_reproduces_published_history(ctx, p, outer_steps):
+    """The whole adaptive driver on the GPU, inner Y loop included (Y estimator against its
+    threshold, Doerfler marking of the Y cells with theta_y = 0.4, Y refinement with its
+    grading closure), against every inner-loop record of the published adaptive_standard
+    runs through outer iteration 25 / 24 / 20 at p = 1 / 2 / 3 (X up to 29,431 / 52,615 /
+    15,356 true DoFs; Y refined inside 7 / 2 / 0 outer iterations): outer and inner indices, X
+    and Y true-DoF counts before and after each Y refinement exactly, eta^2, lambda_Y^2,
+    y_estimator^2 and its threshold to their 6 published digits."""
+    gold = [a for a in json.load(open(INNER_GOLD)) if a["p"] == p and a["outer"] < outer_steps]
+    run = A.AdaptiveX1D(ctx, p=p, theta_x=0.4, rho=1.0, tol=1e-10, max_it=100000, inner_y_loop=True,
+                        theta_y=0.4)
+    k = 0
+    for outer in range(outer_steps):
+        rec = run.step()
+        for r in rec["inner"]:
+            a = gold[k]
+            k += 1
+            print(f"p={p} outer={outer} inner={r['inner_iteration']} X={r['x_true_dofs']} "
+                  f"Y={r['y_true_dofs_before']}->{r.get('y_true_dofs_after', r['y_true_dofs'])} "
+                  f"its={r['minres_iterations']}", flush=True)
+            assert (a["outer"], a["inner"]) == (outer, r["inner_iteration"]), a["source"]
+            assert (r["x_true_dofs"], r["y_true_dofs_before"]) == (a["x_true_dofs"], a["y_true_dofs_before"]), \
+                a["source"]
+            assert r.get("y_true_dofs_after", r["y_true_dofs"]) == a["y_true_dofs_after"], a["source"]
+            for key in ("eta_squared", "lambda_y_squared", "y_estimator_squared", "y_estimator_threshold_squared"):
+                assert r[key] == pytest.approx(a[key], rel=6e-6), (key, a["source"])
+    assert k == len(gold)
