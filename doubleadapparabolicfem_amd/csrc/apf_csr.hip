@@ -451,9 +451,13 @@ struct Outcome {
 };
 
 // d_x holds the initial guess when use_x0, else it is zeroed; on return it holds x.
+// launch_opt: replay the iterations from a hipGraph and run the two preconditioner blocks
+// on two streams (the default; APF_CSR_GRAPH=0 / APF_CSR_FORK=0 switch either off).  The
+// allocation-free entry passes false: graphs, streams and events are resources it must not
+// create.
 Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsigned char* work,
                    const double* d_b, double* d_x, bool use_x0, double tol, int max_it, int check_interval,
-                   int degree, bool fp32)
+                   int degree, bool fp32, bool launch_opt)
 {
     const int64_t n = K.n, nX = n - nY;
     const WorkLayout L = work_layout(n);
@@ -500,16 +504,18 @@ Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsig
 
     const Cheb cc = chebyshev_coefficients(degree);
     // z = P^-1 r with the partials of z . r in part[0, 2G)
-    auto cheb = [&](int64_t rows, const Dev& B, const double* dv, const double* r, double* z, double* pp, int slot) {
+    // (the work arrays from element `off` on, on stream `sb`)
+    auto cheb = [&](int64_t rows, const Dev& B, const double* dv, const double* r, double* z, double* pp, int slot,
+                    int64_t off, hipStream_t sb) {
         auto run = [&](auto tag) {
             using CT = decltype(tag);
-            CT* cx0 = at<CT>(work, L.cx0);
-            CT* cx1 = at<CT>(work, L.cx1);
-            CT* cd = at<CT>(work, L.cd);
+            CT* cx0 = at<CT>(work, L.cx0) + off;
+            CT* cx1 = at<CT>(work, L.cx1) + off;
+            CT* cd = at<CT>(work, L.cd) + off;
             for (int k = 0; k < degree; ++k) {
                 CT* yo = (k % 2 == 0) ? cx1 : cx0;
                 CT* yn = (k % 2 == 0) ? cx0 : cx1;
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_step<CT>), grid, block, 0, st, rows, B.rowptr, B.col, B.val,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_step<CT>), grid, block, 0, sb, rows, B.rowptr, B.col, B.val,
                                    dv, r, yo, yn, cd, lam, slot, k, cc.cdir[k], cc.cres[k], k == degree - 1 ? 1 : 0,
                                    z, pp);
             }
@@ -518,7 +524,7 @@ Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsig
         else run(double{});
     };
     // X block without Hhat: the Chebyshev polynomial in D_S^-1 S, S applied from K's blocks
-    auto cheb_schur = [&](const double* r, double* z, double* pp) {
+    auto cheb_schur = [&](const double* r, double* z, double* pp, hipStream_t sb) {
         auto run = [&](auto tag) {
             using CT = decltype(tag);
             CT* cx0 = at<CT>(work, L.cx0);
@@ -529,9 +535,9 @@ Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsig
                 CT* yo = (k % 2 == 0) ? cx1 : cx0;
                 CT* yn = (k % 2 == 0) ? cx0 : cx1;
                 if (k > 0)
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_schur_w<CT>), grid, block, 0, st, nY, K.rowptr, K.col, K.val,
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_schur_w<CT>), grid, block, 0, sb, nY, K.rowptr, K.col, K.val,
                                        dinv, yo, cw);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_x_step<CT>), grid, block, 0, st, nY, n, K.rowptr, K.col,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_x_step<CT>), grid, block, 0, sb, nY, n, K.rowptr, K.col,
                                    K.val, dinv + nY, r, cw, yo, yn, cd, lam, 2, k, cc.cdir[k], cc.cres[k],
                                    k == degree - 1 ? 1 : 0, z, pp);
             }
@@ -539,11 +545,40 @@ Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsig
         if (fp32) run(float{});
         else run(double{});
     };
+    // The two blocks are independent: with a second stream the X block's chain runs beside
+    // the lambda block's (its work arrays at elements [nY, n), its partials at [G, 2G)).
+    struct Fork {
+        hipStream_t s2 = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Fork()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            if (s2) (void)hipStreamDestroy(s2);
+        }
+    } fk;
+    bool fork = launch_opt;
+    if (const char* e = std::getenv("APF_CSR_FORK")) fork = fork && std::atoi(e) != 0;
+    if (fork) {
+        HIP_TRY(hipStreamCreate(&fk.s2));  // default priority, as every stream here
+        HIP_TRY(hipEventCreateWithFlags(&fk.a, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&fk.b, hipEventDisableTiming));
+    }
     auto precond = [&](double* tt, const double* rr2, double* z) {
         if (rr2) hipLaunchKernelGGL(k_minres_upd, grid, block, 0, st, n, scal, tt, rr2);
-        cheb(nY, K, dinv, tt, z, part, 0);
-        if (H) cheb(nX, *H, dinv + nY, tt + nY, z + nY, part + G, 1);
-        else cheb_schur(tt, z, part + G);
+        hipStream_t sx = st;
+        if (fork) {
+            HIP_TRY(hipEventRecord(fk.a, st));
+            HIP_TRY(hipStreamWaitEvent(fk.s2, fk.a, 0));
+            sx = fk.s2;
+        }
+        if (H) cheb(nX, *H, dinv + nY, tt + nY, z + nY, part + G, 1, nY, sx);
+        else cheb_schur(tt, z, part + G, sx);
+        cheb(nY, K, dinv, tt, z, part, 0, 0, st);
+        if (fork) {
+            HIP_TRY(hipEventRecord(fk.b, fk.s2));
+            HIP_TRY(hipStreamWaitEvent(st, fk.b, 0));
+        }
     };
     auto spmv = [&](const double* xv, double* yv) {
         const int64_t rows_per_block = (256 / 16) * 2;
@@ -573,21 +608,51 @@ Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsig
     Outcome o;
     o.rel = true_residual();
     o.converged = o.rel <= tol;
+    auto iteration = [&]() {
+        spmv(y, t);
+        hipLaunchKernelGGL(k_minres_vec1, grid, block, 0, st, n, scal, y, t, r1, v, part);
+        hipLaunchKernelGGL(k_minres_alfa, dim3(1), dim3(256), 0, st, scal, part, G);
+        precond(t, r2, y);
+        hipLaunchKernelGGL(k_minres_scal2, dim3(1), dim3(256), 0, st, scal, part, 2 * G);
+        std::swap(r1, r2);   // r1 <- r2, r2 <- t, t <- old r1
+        std::swap(r2, t);
+        hipLaunchKernelGGL(k_minres_vec3, grid, block, 0, st, n, scal, v, w2, w, w1, d_x);
+        double* wt = w1;
+        w1 = w2;
+        w2 = w;
+        w = wt;
+    };
+    // The buffer rotations have period 3: three iterations are one fixed launch sequence,
+    // captured once per solve and replayed (kernel launches and stream-ordering events only).
+    struct GraphGuard {
+        hipGraphExec_t e = nullptr;
+        hipGraph_t g = nullptr;
+        ~GraphGuard()
+        {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } gg;
+    bool use_graph = launch_opt && !o.converged && max_it >= 3 && check_interval >= 3;
+    if (const char* e = std::getenv("APF_CSR_GRAPH")) use_graph = use_graph && std::atoi(e) != 0;
+    if (use_graph) {
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < 3; ++q) iteration();
+        HIP_TRY(hipStreamEndCapture(st, &gg.g));
+        HIP_TRY(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
+    }
     while (!o.converged && o.iterations < max_it) {
-        const int batch = std::min(check_interval, max_it - o.iterations);
-        for (int b = 0; b < batch; ++b) {
-            spmv(y, t);
-            hipLaunchKernelGGL(k_minres_vec1, grid, block, 0, st, n, scal, y, t, r1, v, part);
-            hipLaunchKernelGGL(k_minres_alfa, dim3(1), dim3(256), 0, st, scal, part, G);
-            precond(t, r2, y);
-            hipLaunchKernelGGL(k_minres_scal2, dim3(1), dim3(256), 0, st, scal, part, 2 * G);
-            std::swap(r1, r2);   // r1 <- r2, r2 <- t, t <- old r1
-            std::swap(r2, t);
-            hipLaunchKernelGGL(k_minres_vec3, grid, block, 0, st, n, scal, v, w2, w, w1, d_x);
-            double* wt = w1;
-            w1 = w2;
-            w2 = w;
-            w = wt;
+        int batch = std::min(check_interval, max_it - o.iterations);
+        if (gg.e && batch >= 3) {
+            batch = (batch / 3) * 3;
+            for (int b = 0; b < batch; b += 3) HIP_TRY(hipGraphLaunch(gg.e, st));
+        } else {
+            // eager iterations continue the rotation where the graph's left it (a multiple of 3)
+            if (gg.e) {
+                HIP_TRY(hipGraphExecDestroy(gg.e));
+                gg.e = nullptr;
+            }
+            for (int b = 0; b < batch; ++b) iteration();
         }
         o.iterations += batch;
         HIP_TRY(hipGetLastError());
@@ -770,7 +835,7 @@ int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, cons
         Dev Hd{};
         if (Hhat) Hd = Dev{Hhat->n, Hhat->rowptr, Hhat->col, Hhat->val};
         const Outcome o = run_minres(st, Kd, n_lambda, Hhat ? &Hd : nullptr, work, d_b, d_x, x0 != nullptr, tol,
-                                     max_it, check_interval, cheb_degree, precond_prec == APF_FP32_REFINE);
+                                     max_it, check_interval, cheb_degree, precond_prec == APF_FP32_REFINE, true);
         HIP_TRY(hipMemcpyAsync(x, d_x, K->n * sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         fill_stats(stats, o, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
@@ -832,7 +897,7 @@ extern "C" int apf_csr_minres_solve_noalloc(apf_ctx* ctx, const apf_csr_matrix* 
         const Dev Kd{K->n, K->rowptr, K->col, K->val};
         const Outcome o = run_minres(ctx->stream, Kd, n_lambda, nullptr, static_cast<unsigned char*>(base), d_b, d_x,
                                      use_x0 != 0, tol, max_it, check_interval, cheb_degree,
-                                     precond_prec == APF_FP32_REFINE);
+                                     precond_prec == APF_FP32_REFINE, false);
         fill_stats(stats, o, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         if (!o.converged)
             return fail(ctx, APF_NOT_CONVERGED,
