@@ -176,6 +176,12 @@ EXPORTED = {
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(_MinresStats)]),
+    "apf_csr_spmv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "apf_csr_block_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_double)]),
+    "apf_csr_block_pcg_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(_MinresStats)]),
     "apf_csr_minres_solve_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -550,6 +556,44 @@ class CsrMatrix:
             raise NotConverged(code, lib().apf_last_error(self.ctx.handle).decode(), stats)
         if code not in (APF_OK, APF_NOT_CONVERGED):
             _check(code, self.ctx.handle, "apf_csr_minres_solve")
+        return x, stats
+
+
+    def spmv(self, x):
+        """A x (apf_csr_spmv)"""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.n,):
+            raise ValueError("spmv: x must have n entries")
+        y = np.zeros(self.n)
+        _check(lib().apf_csr_spmv(self.ctx.handle, self.handle, _ptr(x), _ptr(y)), self.ctx.handle, "apf_csr_spmv")
+        return y
+
+    def block_energy(self, d, n_block):
+        """d^T A[:n_block, :n_block] d (apf_csr_block_energy)"""
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        if d.shape != (n_block,):
+            raise ValueError("block_energy: d must have n_block entries")
+        out = ctypes.c_double()
+        _check(lib().apf_csr_block_energy(self.ctx.handle, self.handle, n_block, _ptr(d), ctypes.byref(out)),
+               self.ctx.handle, "apf_csr_block_energy")
+        return out.value
+
+    def block_pcg(self, b, n_block, tol=1e-12, max_it=20000, check_interval=20, degree=8, x0=None,
+                  raise_on_fail=True):
+        """Conjugate gradients on the SPD leading block (apf_csr_block_pcg_solve); returns (x, stats)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.shape != (n_block,):
+            raise ValueError("block_pcg: b must have n_block entries")
+        x = np.zeros(n_block)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        st = _MinresStats()
+        code = lib().apf_csr_block_pcg_solve(self.ctx.handle, self.handle, n_block, _ptr(b), _ptr(x), _ptr(x0), tol,
+                                             max_it, check_interval, degree, ctypes.byref(st))
+        stats = {f: getattr(st, f) for f, _ in _MinresStats._fields_}
+        if code == APF_NOT_CONVERGED and raise_on_fail:
+            raise NotConverged(code, lib().apf_last_error(self.ctx.handle).decode(), stats)
+        if code not in (APF_OK, APF_NOT_CONVERGED):
+            _check(code, self.ctx.handle, "apf_csr_block_pcg_solve")
         return x, stats
 
 

@@ -25,8 +25,8 @@ import time
 
 import numpy as np
 
-from . import (APF_FP64, Context, GeneralEstimator, GeneralSystem, _check, _ptr, combine_estimator,
-               doerfler_mark, lib, y_estimator_threshold)
+from . import (APF_BAD_ARG, APF_FP64, APFError, Context, GeneralEstimator, GeneralSystem, _check, _ptr,
+               combine_estimator, doerfler_mark, lib, y_estimator_threshold)
 
 
 class Mesh1D:
@@ -212,6 +212,87 @@ def slab_groups(Y: FESpace1D, max_direct=64):
     return out
 
 
+def _lagrange_values(p, s):
+    """[len(s), p+1] values of the degree-p Lagrange basis on the equidistant nodes of [0, 1]"""
+    s = np.asarray(s, dtype=np.float64)
+    nodes = np.arange(p + 1) / p
+    V = np.ones((s.size, p + 1))
+    for a in range(p + 1):
+        for k in range(p + 1):
+            if k != a:
+                V[:, a] *= (s - nodes[k]) / (nodes[a] - nodes[k])
+    return V
+
+
+def prolong_nodal(src: FESpace1D, coef, dst: FESpace1D):
+    """True coefficients in `dst` of the `src` function with true coefficients `coef`: its
+    value at the node of each free dst DoF, taken on the first active dst cell (in active
+    order, local order) where that DoF is free, evaluated on the src cell containing it
+    (fespace/prolongation.hpp:15-97).  dst's active cells lie inside src's."""
+    anc = src.ancestors_of(dst)
+    loc = src.local_coefficients(coef)
+    p, q = src.p, dst.p
+    ii, jj = np.divmod(np.arange(dst.n_loc), q + 1)     # local = i_space * (q+1) + i_time
+    bd, bs = dst.box, src.box[anc]
+    x = bd[:, 0:1] + (bd[:, 1:2] - bd[:, 0:1]) * (ii / q)[None, :]
+    t = bd[:, 2:3] + (bd[:, 3:4] - bd[:, 2:3]) * (jj / q)[None, :]
+    sx = ((x - bs[:, 0:1]) / (bs[:, 1:2] - bs[:, 0:1])).ravel()
+    st = ((t - bs[:, 2:3]) / (bs[:, 3:4] - bs[:, 2:3])).ravel()
+    Lx, Lt = _lagrange_values(p, sx), _lagrange_values(p, st)
+    c = loc[np.repeat(np.arange(dst.n_cells), dst.n_loc)].reshape(-1, p + 1, p + 1)
+    vals = np.einsum("na,nab,nb->n", Lx, c, Lt)
+    # the free dofs: restriction rows [(d, 1.0)]
+    cnt = np.diff(dst.restr_ptr)
+    one = np.flatnonzero(cnt == 1)
+    free = one[dst.restr_w[dst.restr_ptr[one]] == 1.0]
+    dof = dst.restr_dof[dst.restr_ptr[free]]
+    first = np.full(dst.n_dofs, -1, np.int64)
+    order = np.argsort(dof, kind="stable")
+    ds = dof[order]
+    head = np.concatenate([[True], ds[1:] != ds[:-1]])
+    first[ds[head]] = free[order][head]
+    if (first < 0).any():
+        raise ValueError("prolong_nodal: a target true DoF is not free on any cell")
+    return vals[first]
+
+
+def g_estimate(ctx: Context, X: FESpace1D, Y: FESpace1D, x, u0=None, tol=1e-12, max_it=100000, degree=8):
+    """The G estimator of an accepted (or any) inner iteration on the 1+1D adaptive pair
+    (compute_g_estimator_for_accepted_inner_iteration_, adaptive_driver.hpp:2680-2905): G =
+    the Y space of degree p+1 in space and time on Y's cells; A_G lambda_G = f_G - B(X,G)^T
+    u_delta; G^2 = (lambda_G - I lambda_delta)^T A_G (lambda_G - I lambda_delta), with I the
+    nodal prolongation.  A_G, f_G and B(X_{p+1},G) come from the degree-(p+1) GeneralSystem
+    on (X_{p+1}, G); B(X,G)^T u = B(X_{p+1},G)^T (I u) since the prolongation of u is the
+    same function and both rules integrate the polynomial integrand exactly.  The G system
+    is solved by conjugate gradients on the device (apf_csr_block_pcg_solve) from I lambda."""
+    p = X.p
+    if p + 1 > 3:
+        raise ValueError("g_estimate: the degree-(p+1) system needs p <= 2 on 1+1D adaptive meshes")
+    u0 = smooth_initial_u0 if u0 is None else u0
+    t0 = time.perf_counter()
+    x = np.asarray(x, dtype=np.float64)
+    lam, u = x[:Y.n_dofs], x[Y.n_dofs:]
+    Xq = FESpace1D(X.mesh, p + 1, True, X.active)
+    G = FESpace1D(X.mesh, p + 1, False, Y.active)
+    Iu = prolong_nodal(X, u, Xq)
+    Il = prolong_nodal(Y, lam, G)
+    gs = GeneralSystem(ctx, SpaceTimePair(Xq, G))
+    gs.assemble(u0)
+    K = gs.matrix()
+    nG = gs.nY
+    rhs = gs.rhs()[:nG] - K.spmv(np.concatenate([np.zeros(nG), Iu]))[:nG]
+    lg, st = K.block_pcg(rhs, nG, tol=tol, max_it=max_it, degree=degree, x0=Il)
+    d = lg - Il
+    g2 = K.block_energy(d, nG)
+    coeff2 = float(d @ d)
+    if g2 < -1000.0 * np.finfo(float).eps * max(1.0, coeff2):
+        raise APFError(APF_BAD_ARG, f"g_estimate: negative G^2 ({g2})")
+    del K, gs
+    return dict(g_true_dofs=G.n_dofs, g_lambda_difference_squared=max(0.0, g2),
+                g_solver_iterations=st["iterations"], g_solver_relative_residual=st["true_relative_residual"],
+                g_seconds=time.perf_counter() - t0)
+
+
 def smooth_initial_u0(x):
     """u0 = sin(pi x) (algorithm/examples/space_time_1d/initial_data_1d_examples.hpp:9-63: M = 1, ell = 0)"""
     return np.sin(np.pi * np.asarray(x)[..., 0])
@@ -231,14 +312,16 @@ class AdaptiveX1D:
 
     def __init__(self, ctx: Context, p=1, theta_x=0.4, rho=1.0, x_uniform_levels=1, u0=smooth_initial_u0,
                  tol=1e-10, max_it=40000, with_y_estimator=False, estimator_precision=APF_FP64, hhat=True,
-                 cheb_degree=16, inner_y_loop=False, theta_y=0.4, max_inner=50):
+                 cheb_degree=16, inner_y_loop=False, theta_y=0.4, max_inner=50, with_g_estimator=False):
         """hhat: precondition the X block with the graph norm Hhat formed on the device (the
         reference's choice, parabolic_graph_norm_minres.hpp:46-181), else with the Schur
-        complement's diagonal; cheb_degree: Chebyshev steps of each block."""
+        complement's diagonal; cheb_degree: Chebyshev steps of each block; with_g_estimator: G^2
+        of every inner iteration (g_estimate, the published adaptive_g_effective_rho runs)."""
         self.ctx, self.p, self.theta_x, self.rho, self.u0 = ctx, p, theta_x, rho, u0
         self.hhat, self.cheb_degree = hhat, cheb_degree
         self.with_y_estimator, self.estimator_precision = with_y_estimator or inner_y_loop, estimator_precision
         self.inner_y_loop, self.theta_y, self.max_inner = inner_y_loop, theta_y, max_inner
+        self.with_g_estimator = with_g_estimator
         self.tol, self.max_it = tol, max_it
         self.mesh = Mesh1D(0.0, 1.0, 0.0, 1.0)
         self.X = FESpace1D(self.mesh, p, True, self.mesh.leaf_cells())
@@ -280,6 +363,8 @@ class AdaptiveX1D:
             rec["y_estimator_by_cell"] = combine_estimator(est["flux_by_cell"], est["recon_by_cell"])
             rec["estimator_seconds"] = time.perf_counter() - t0
             del ge
+        if self.with_g_estimator:
+            rec.update(g_estimate(self.ctx, self.X, self.Y, x, self.u0))
         del K
         del gs
         return rec

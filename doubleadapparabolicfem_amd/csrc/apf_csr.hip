@@ -405,6 +405,100 @@ __global__ void __launch_bounds__(256) k_cheb_x_step(int64_t nY, int64_t n, cons
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
+
+// ------------------------------------------------------------------ block SPD solve
+// y = B x and the block partials of x . y for the leading block rows [0, rows) x
+// columns [0, rows) of a CSR matrix (columns >= rows are skipped).
+__global__ void __launch_bounds__(256) k_block_spmv(int64_t rows, const int64_t* __restrict__ rowptr,
+                                                    const int* __restrict__ col, const double* __restrict__ val,
+                                                    const double* __restrict__ x, double* __restrict__ y,
+                                                    double* __restrict__ part)
+{
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t g0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kRowLanes;
+    const int64_t gs = static_cast<int64_t>(gridDim.x) * blockDim.x / kRowLanes;
+    double dot = 0.0;
+    for (int64_t i = g0; i < rows; i += gs) {
+        double s = 0.0;
+        for (int64_t e = rowptr[i] + lane; e < rowptr[i + 1]; e += kRowLanes) {
+            const int c = col[e];
+            if (c < rows) s += val[e] * x[c];
+        }
+        s = group_sum(s);
+        if (lane == 0) {
+            y[i] = s;
+            dot += x[i] * s;
+        }
+    }
+    const double bs = block_sum(dot);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// x += alpha p, r -= alpha q (alpha = rz / p.Ap from the device scalars), partials of r . r
+__global__ void k_pcg_update(int64_t n, const double* __restrict__ sc, const double* __restrict__ p,
+                             const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
+                             double* __restrict__ part)
+{
+    // a cycle that has reached round-off can meet p.Ap <= 0 (or 0 / 0): it stops moving, and
+    // the true residual after the batch decides
+    const double alpha = (sc[1] > 0.0 && sc[0] > 0.0) ? sc[0] / sc[1] : 0.0;
+    double rr = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        rr += ri * ri;
+    }
+    const double bs = block_sum(rr);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// p = z + beta p, beta = rz_new / rz_old; the first step writes p = z without reading p
+// (fresh memory may hold any bit pattern, NaN included, and 0 * NaN is NaN)
+__global__ void k_pcg_dir(int64_t n, const double* __restrict__ sc, int first, const double* __restrict__ z,
+                          double* __restrict__ p)
+{
+    const double beta = (first || !(sc[0] > 0.0)) ? 0.0 : sc[2] / sc[0];
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+// sc[slot] = sum of the partials (fixed order); slot 2 (new rz) moves to slot 0 after use by
+// the direction kernel when `shift` is set
+__global__ void k_scalar_from_partials(const double* __restrict__ part, int nparts, double* __restrict__ sc,
+                                       int slot)
+{
+    const double v = reduce_partials(part, nparts);
+    if (threadIdx.x == 0) sc[slot] = v;
+}
+
+__global__ void k_scalar_copy(double* __restrict__ sc, int from, int to)
+{
+    if (threadIdx.x == 0) sc[to] = sc[from];
+}
+
+// r = b - y and the partials of r . r and b . b (the true residual of the block solve)
+__global__ void k_block_residual(int64_t n, const double* __restrict__ b, const double* __restrict__ y,
+                                 double* __restrict__ r, double* __restrict__ part)
+{
+    double rr = 0.0, bb = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double ri = b[i] - y[i];
+        r[i] = ri;
+        rr += ri * ri;
+        bb += b[i] * b[i];
+    }
+    const double a = block_sum(rr);
+    const double c = block_sum(bb);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[gridDim.x + blockIdx.x] = c;
+    }
+}
+
 // ------------------------------------------------------------------ driver
 struct Dev {
     int64_t n;
@@ -852,6 +946,203 @@ int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, cons
 }
 
 }  // extern "C"
+
+
+// ------------------------------------------------------------------ block SPD solve: C ABI
+namespace {
+struct BlockWork {
+    double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr;
+    double *c0 = nullptr, *c1 = nullptr, *cd = nullptr, *part = nullptr, *sc = nullptr;
+    std::vector<void*> all;
+    ~BlockWork()
+    {
+        for (void* v : all) (void)hipFree(v);
+    }
+    double* get(size_t count)
+    {
+        void* v = nullptr;
+        HIP_TRY(hipMalloc(&v, std::max<size_t>(1, count) * sizeof(double)));
+        all.push_back(v);
+        return static_cast<double*>(v);
+    }
+};
+
+int check_block(apf_ctx* ctx, const apf_csr_matrix* A, int64_t n_block, const char* who)
+{
+    if (!ctx || !A) return fail(ctx, APF_BAD_ARG, std::string(who) + ": null argument");
+    if (A->ctx != ctx) return fail(ctx, APF_BAD_ARG, std::string(who) + ": matrix of another context");
+    if (n_block < 1 || n_block > A->n) return fail(ctx, APF_BAD_ARG, std::string(who) + ": n_block must lie in [1, n]");
+    return APF_OK;
+}
+}  // namespace
+
+extern "C" int apf_csr_spmv(apf_ctx* ctx, const apf_csr_matrix* A, const double* x, double* y)
+{
+    if (int c = check_block(ctx, A, A ? A->n : 1, "apf_csr_spmv")) return c;
+    if (!x || !y) return fail(ctx, APF_BAD_ARG, "apf_csr_spmv: null vector");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        BlockWork W;
+        double* dx = W.get(A->n);
+        double* dy = W.get(A->n);
+        HIP_TRY(hipMemcpyAsync(dx, x, A->n * sizeof(double), hipMemcpyHostToDevice, st));
+        const int64_t rows_per_block = (256 / 16) * 2;
+        const int64_t g = (A->n + rows_per_block - 1) / rows_per_block;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spmv<16, 4, 2>), dim3(static_cast<unsigned>(g)), dim3(256), 0, st, A->n,
+                           A->rowptr, A->col, A->val, dx, dy);
+        HIP_TRY(hipMemcpyAsync(y, dy, A->n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_csr_spmv: ") + e.what());
+    }
+}
+
+extern "C" int apf_csr_block_energy(apf_ctx* ctx, const apf_csr_matrix* A, int64_t n_block, const double* d,
+                                    double* out)
+{
+    if (int c = check_block(ctx, A, n_block, "apf_csr_block_energy")) return c;
+    if (!d || !out) return fail(ctx, APF_BAD_ARG, "apf_csr_block_energy: null argument");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        BlockWork W;
+        double* dd = W.get(n_block);
+        double* dy = W.get(n_block);
+        double* part = W.get(kRowGrid);
+        double* sc = W.get(4);
+        HIP_TRY(hipMemcpyAsync(dd, d, n_block * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_block_spmv, dim3(kRowGrid), dim3(256), 0, st, n_block, A->rowptr, A->col, A->val, dd,
+                           dy, part);
+        hipLaunchKernelGGL(k_scalar_from_partials, dim3(1), dim3(256), 0, st, part, kRowGrid, sc, 0);
+        HIP_TRY(hipMemcpyAsync(out, sc, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_csr_block_energy: ") + e.what());
+    }
+}
+
+extern "C" int apf_csr_block_pcg_solve(apf_ctx* ctx, const apf_csr_matrix* A, int64_t n_block, const double* b,
+                                       double* x, const double* x0, double tol, int max_it, int check_interval,
+                                       int cheb_degree, apf_minres_stats* stats)
+{
+    if (int c = check_block(ctx, A, n_block, "apf_csr_block_pcg_solve")) return c;
+    if (!b || !x) return fail(ctx, APF_BAD_ARG, "apf_csr_block_pcg_solve: null vector");
+    if (!(tol > 0.0) || max_it < 1 || check_interval < 1)
+        return fail(ctx, APF_BAD_ARG, "apf_csr_block_pcg_solve: bad tolerance / iteration limits");
+    if (cheb_degree < 1 || cheb_degree > 64)
+        return fail(ctx, APF_BAD_ARG, "apf_csr_block_pcg_solve: Chebyshev degree must be in [1, 64]");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        hipStream_t st = ctx->stream;
+        const int64_t n = n_block;
+        const int G = kRowGrid;
+        BlockWork W;
+        double* db = W.get(n);
+        double* dx = W.get(n);
+        double* r = W.get(n);
+        double* z = W.get(n);
+        double* p = W.get(n);
+        double* q = W.get(n);
+        double* dinv = W.get(n);
+        double* c0 = W.get(n);
+        double* c1 = W.get(n);
+        double* cd = W.get(n);
+        double* part = W.get(2 * G);
+        double* sc = W.get(8);   // [0] rz, [1] p.Ap, [2] new rz, [3] r.r, [4] lambda_max, [5..6] residual sums
+        HIP_TRY(hipMemcpyAsync(db, b, n * sizeof(double), hipMemcpyHostToDevice, st));
+        if (x0) HIP_TRY(hipMemcpyAsync(dx, x0, n * sizeof(double), hipMemcpyHostToDevice, st));
+        else hipLaunchKernelGGL(k_fill, dim3(G), dim3(256), 0, st, n, 0.0, dx);
+        // Jacobi data and the Gershgorin bound of D^-1 A
+        hipLaunchKernelGGL(k_block_jacobi, dim3(G), dim3(256), 0, st, n, A->rowptr, A->col, A->val, dinv, part);
+        hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, st, part, G, sc, 4);
+        double lm = 0.0;
+        HIP_TRY(hipMemcpyAsync(&lm, sc + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (!(lm > 0.0) || !std::isfinite(lm))
+            return fail(ctx, APF_BAD_ARG, "apf_csr_block_pcg_solve: the block is not positive definite "
+                                          "(non-positive diagonal)");
+        const Cheb cc = chebyshev_coefficients(cheb_degree);
+        // z = q(D^-1 A) D^-1 r, with the partials of z . r (new rz) into sc[2]
+        auto precond = [&]() {
+            for (int k = 0; k < cheb_degree; ++k) {
+                double* yo = (k % 2 == 0) ? c1 : c0;
+                double* yn = (k % 2 == 0) ? c0 : c1;
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cheb_step<double>), dim3(G), dim3(256), 0, st, n, A->rowptr,
+                                   A->col, A->val, dinv, r, yo, yn, cd, sc, 4, k, cc.cdir[k], cc.cres[k],
+                                   k == cheb_degree - 1 ? 1 : 0, z, part);
+            }
+            hipLaunchKernelGGL(k_scalar_from_partials, dim3(1), dim3(256), 0, st, part, G, sc, 2);
+        };
+        // r = b - A x; returns ||r|| / ||b||
+        auto true_residual = [&]() {
+            hipLaunchKernelGGL(k_block_spmv, dim3(G), dim3(256), 0, st, n, A->rowptr, A->col, A->val, dx, q, part);
+            hipLaunchKernelGGL(k_block_residual, dim3(G), dim3(256), 0, st, n, db, q, r, part);
+            hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, st, part, G, sc + 5);
+            double h[2];
+            HIP_TRY(hipMemcpyAsync(h, sc + 5, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            return (h[1] > 0.0) ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+        };
+        Outcome o;
+        o.rel = true_residual();
+        o.converged = o.rel <= tol;
+        int restarts = 0;
+        while (!o.converged && o.iterations < max_it) {
+            // one CG cycle from the current true residual
+            precond();
+            hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(256), 0, st, n, sc, 1, z, p);
+            hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(64), 0, st, sc, 2, 0);
+            double rr0 = -1.0;
+            bool cycle_done = false;
+            while (!cycle_done && o.iterations < max_it) {
+                const int batch = std::min(check_interval, max_it - o.iterations);
+                for (int it = 0; it < batch; ++it) {
+                    hipLaunchKernelGGL(k_block_spmv, dim3(G), dim3(256), 0, st, n, A->rowptr, A->col, A->val, p,
+                                       q, part);
+                    hipLaunchKernelGGL(k_scalar_from_partials, dim3(1), dim3(256), 0, st, part, G, sc, 1);
+                    hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, st, n, sc, p, q, dx, r, part + G);
+                    precond();
+                    hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(256), 0, st, n, sc, 0, z, p);
+                    hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(64), 0, st, sc, 2, 0);
+                }
+                o.iterations += batch;
+                hipLaunchKernelGGL(k_scalar_from_partials, dim3(1), dim3(256), 0, st, part + G, G, sc, 3);
+                double rr = 0.0;
+                HIP_TRY(hipMemcpyAsync(&rr, sc + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                if (!std::isfinite(rr)) throw std::runtime_error("the block solve diverged (non-finite residual)");
+                if (rr0 < 0.0) rr0 = rr;
+                // recursive residual at a tenth of the tolerance, or stalled: confirm on the true one
+                const double bn2 = [&] {
+                    double h[2];
+                    HIP_TRY(hipMemcpyAsync(h, sc + 5, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+                    HIP_TRY(hipStreamSynchronize(st));
+                    return h[1] > 0.0 ? h[1] : 1.0;
+                }();
+                if (std::sqrt(rr / bn2) <= 0.1 * tol || o.iterations >= max_it) cycle_done = true;
+            }
+            o.rel = true_residual();
+            o.converged = o.rel <= tol;
+            if (!o.converged && ++restarts > 8) break;
+        }
+        HIP_TRY(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        fill_stats(stats, o, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        if (!o.converged)
+            return fail(ctx, APF_NOT_CONVERGED,
+                        "apf_csr_block_pcg_solve: not converged after " + std::to_string(o.iterations) +
+                            " iterations, true relative residual " + std::to_string(o.rel));
+        return APF_OK;
+    } catch (const HipFailure& e) {
+        return fail(ctx, APF_HIP_ERROR, std::string("apf_csr_block_pcg_solve: ") + e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, APF_BAD_ARG, std::string("apf_csr_block_pcg_solve: ") + e.what());
+    }
+}
 
 // Resolve the device address of `scratch` on the context's device (once per device).
 // Called from apf_ctx_create with that device current.

@@ -277,6 +277,25 @@ int apf_csr_matrix_info(const apf_csr_matrix* M, int64_t* n, int64_t* nnz);
 int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, const apf_csr_matrix* Hhat,
                          const double* b, double* x, const double* x0, double tol, int max_it, int check_interval,
                          int cheb_degree, int precond_prec, apf_minres_stats* stats);
+/* y = A x (host vectors of length n).  Replaces la::ops::matvec / transpose_matvec on a
+ * symmetric SparseMatrix (src/linear_algebra/eigen_backend/operations.hpp:103-128); the
+ * G estimator forms its right-hand side f_G - B(X,G)^T u_delta with it
+ * (algorithm/adaptive_driver.hpp:2790-2798). */
+int apf_csr_spmv(apf_ctx* ctx, const apf_csr_matrix* A, const double* x, double* y);
+/* d^T B d for the leading block B = A[0:n_block, 0:n_block] (host d of length n_block),
+ * reduced in a fixed order.  Replaces la::ops::quadratic_form(A_g, d)
+ * (eigen_backend/operations.hpp:129-150; adaptive_driver.hpp:2860-2866). */
+int apf_csr_block_energy(apf_ctx* ctx, const apf_csr_matrix* A, int64_t n_block, const double* d, double* out);
+/* Conjugate gradients on B x = b for the SPD leading block B = A[0:n_block, 0:n_block],
+ * preconditioned by q(D^-1 B) D^-1 (the Chebyshev polynomial of cheb_degree steps, as in
+ * apf_csr_minres_solve).  Replaces the direct solve of the G system
+ * (solver_.compute(A_g); solver_.solve(rhs_g, lambda_g), adaptive_driver.hpp:2806-2814).
+ * The recursive residual is read every check_interval iterations; at a tenth of tol the
+ * true relative residual ||b - B x|| / ||b|| decides (restarting from it, at most 8 times);
+ * APF_NOT_CONVERGED when it stays above tol.  b, x, x0 (may be NULL) are host arrays. */
+int apf_csr_block_pcg_solve(apf_ctx* ctx, const apf_csr_matrix* A, int64_t n_block, const double* b, double* x,
+                            const double* x0, double tol, int max_it, int check_interval, int cheb_degree,
+                            apf_minres_stats* stats);
 
 /* ---- space-time meshes that are not tensor products ---- */
 /* One space (X or Y) as the reference's FESpace hands it to the assembler: the active

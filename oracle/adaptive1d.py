@@ -639,6 +639,46 @@ def adaptive_run(p, outer_steps, theta_x=0.4, theta_y=0.4, rho=1.0, max_inner=50
     return out
 
 
+def prolong_nodal(src: Space, coef, dst: Space):
+    """true coefficients in `dst` of the `src` function: its value at the node of each free
+    dst DoF, on the first active dst cell (active order, local order) where the DoF is free,
+    evaluated on the src cell containing it (fespace/prolongation.hpp:15-97)"""
+    anc = dst.ancestors_in(src)
+    loc = src.local_values(coef)
+    q = dst.p
+    out = np.full(dst.n_true, np.nan)
+    for cpos, c in enumerate(dst.active):
+        x0, x1, t0, t1 = dst.tree.box(c)
+        for l, g in enumerate(dst.cell_dofs[cpos]):
+            if 0 <= g < dst.n_true and np.isnan(out[g]):
+                i, j = divmod(l, q + 1)
+                a = anc[cpos]
+                out[g] = _value_on_cell(src, loc[a], a, x0 + (x1 - x0) * i / q, t0 + (t1 - t0) * j / q)
+    assert not np.isnan(out).any()
+    return out
+
+
+def g_estimate(X: Space, Y: Space, x, u0, kappa=1.0):
+    """G^2 of the solution x = [lambda; u] on the adaptive pair
+    (compute_g_estimator_for_accepted_inner_iteration_, adaptive_driver.hpp:2680-2905): G =
+    Y's cells with degree p+1 in space and time, A_G lambda_G = f_G - B(X,G)^T u (sparse
+    direct solve), G^2 = d^T A_G d for d = lambda_G - I lambda.  A_G, f_G and B come from the
+    degree-(p+1) assembly on (X_{p+1}, G), B applied to the prolongation I u."""
+    p = X.p
+    Xq = Space(X.tree, p + 1, True, X.active)
+    G = Space(X.tree, p + 1, False, Y.active)
+    K, b = assemble(Xq, G, u0, kappa)
+    nG = G.n_true
+    K = sp.csr_matrix(K)
+    A, Bt = K[:nG, :nG], K[:nG, nG:]
+    lam, u = x[:Y.n_true], x[Y.n_true:]
+    Iu = prolong_nodal(X, u, Xq)
+    Il = prolong_nodal(Y, lam, G)
+    lg = spla.spsolve(A.tocsc(), b[:nG] - Bt @ Iu)
+    d = lg - Il
+    return dict(g_true_dofs=nG, g_lambda_difference_squared=max(0.0, float(d @ (A @ d))))
+
+
 def graph_norm(X: Space, Y: Space, K, kappa=1.0, parts=False):
     """Hhat = -C_signed + B_dt diag(A_y)^-1 B_dt^T + A_x (assemble_parabolic_graph_norm_
     approximation, assembly/main_system/parabolic_graph_norm.hpp:22-65, with

@@ -14,7 +14,9 @@ the same eta^2 and lambda_y^2 as that uniform record.
 Also writes tests/golden/adaptive1d_anchors.json: the 1+1D smooth_initial adaptive runs
 (modes adaptive_standard and adaptive_force_effective_rho_no_inner_refinement, p = 1, 2, 3),
 one record per outer iteration while the inner Y loop accepted its first iterate; and
-tests/golden/adaptive1d_inner_anchors.json: every inner-loop record of the adaptive_standard runs (inner iteration 0 with Y true DOFs unchanged and the
+tests/golden/adaptive1d_inner_anchors.json: every inner-loop record of the adaptive_standard runs, and
+the G values (g_true_dofs, g_lambda_difference_squared) of the adaptive_g_effective_rho runs (p = 1, 2),
+whose meshes follow the same history (inner iteration 0 with Y true DOFs unchanged and the
 next record starting the next outer iteration): X true DOFs before and after the X
 refinement, Y true DOFs, eta^2, lambda_Y^2, y_estimator^2 and its threshold.
 
@@ -112,6 +114,19 @@ def main():
                          "eta_squared": float(r["eta_squared"]), "lambda_y_squared": float(r["lambda_y_squared"]),
                          "y_estimator_squared": float(r["y_estimator_squared"]),
                          "y_estimator_threshold_squared": float(r["y_estimator_threshold_squared"])})
+    for p in (1, 2):
+        for ln, r in rows:
+            if r["record_type"] != "inner" or r["mode"] != "adaptive_g_effective_rho" or r["p_int"] != str(p):
+                continue
+            if not r["g_lambda_difference_squared"]:
+                continue
+            iout.append({"dim": 1, "p": p, "outer": int(r["outer_iteration"]), "inner": int(r["inner_iteration"]),
+                         "source": f"algorithm_data_plot/1d/smooth_initial.csv:{ln}", "mode": r["mode"],
+                         "x_true_dofs": int(float(r["x_true_dofs"])),
+                         "y_true_dofs_before": int(float(r["y_true_dofs_before"])),
+                         "y_true_dofs_after": int(float(r["y_true_dofs_after"])),
+                         "g_true_dofs": int(float(r["g_true_dofs"])),
+                         "g_lambda_difference_squared": float(r["g_lambda_difference_squared"])})
     idst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "adaptive1d_inner_anchors.json")
     with open(idst, "w") as fh:
         json.dump(iout, fh, indent=1)

@@ -278,7 +278,7 @@ def test_oracle_driver_with_inner_loop_reproduces_published_history():
     which includes the first Y refinement (outer 17: Y 5868 -> 7520 true DoFs, accepted at the
     second inner iteration): counts exactly, eta^2, y_estimator^2 and its threshold to 6 digits."""
     gold = [a for a in json.load(open(os.path.join(os.path.dirname(GOLD), "adaptive1d_inner_anchors.json")))
-            if a["p"] == 1 and a["outer"] <= 18]
+            if a["p"] == 1 and a["mode"] == "adaptive_standard" and a["outer"] <= 18]
     run = O.adaptive_run(1, 19)
     assert len(run) == len(gold)
     for rec, a in zip(run, gold):
@@ -287,3 +287,29 @@ def test_oracle_driver_with_inner_loop_reproduces_published_history():
         assert rec["eta2"] == pytest.approx(a["eta_squared"], rel=6e-6), a["source"]
         assert rec["y_est2"] == pytest.approx(a["y_estimator_squared"], rel=6e-6), a["source"]
         assert rec["threshold2"] == pytest.approx(a["y_estimator_threshold_squared"], rel=6e-6), a["source"]
+
+
+@pytest.mark.parametrize("p,steps", [(1, 10), (2, 7)])
+def test_oracle_g_estimator_reproduces_published_values(p, steps):
+    """The oracle's G on the adaptive meshes (degree-(p+1) assembly on (X_{p+1}, G), sparse
+    direct G solve, nodal prolongations) against the published adaptive_g_effective_rho runs:
+    G true DoFs exactly, G^2 to 6 digits."""
+    gold = [a for a in json.load(open(os.path.join(os.path.dirname(GOLD), "adaptive1d_inner_anchors.json")))
+            if a["p"] == p and a["mode"] == "adaptive_g_effective_rho"][:steps]
+    T = O.Tree()
+    X = O.Space(T, p, True, T.leaves())
+    X.refine(list(X.active))
+    for a in gold:
+        assert a["inner"] == 0
+        Y = O.Space(T, p, False, X.active)
+        K, b = O.assemble(X, Y, u0)
+        x = O.solve(K, b)
+        g = O.g_estimate(X, Y, x, u0)
+        assert g["g_true_dofs"] == a["g_true_dofs"], a["source"]
+        assert g["g_lambda_difference_squared"] == pytest.approx(a["g_lambda_difference_squared"], rel=6e-6), \
+            a["source"]
+        l2, t2 = O.x_indicators(X, Y, x, u0)
+        lx = np.zeros(len(X.active))
+        np.add.at(lx, Y.ancestors_in(X), l2)
+        marked, _ = O.doerfler_mark(lx + t2, 0.4, ids=X.active)
+        X.refine(marked)

@@ -194,7 +194,8 @@ def test_gpu_adaptive_driver_with_inner_loop_reproduces_published_history(ctx, p
     15,356 true DoFs; Y refined inside 7 / 2 / 0 outer iterations): outer and inner indices, X
     and Y true-DoF counts before and after each Y refinement exactly, eta^2, lambda_Y^2,
     y_estimator^2 and its threshold to their 6 published digits."""
-    gold = [a for a in json.load(open(INNER_GOLD)) if a["p"] == p and a["outer"] < outer_steps]
+    gold = [a for a in json.load(open(INNER_GOLD))
+            if a["p"] == p and a["mode"] == "adaptive_standard" and a["outer"] < outer_steps]
     run = A.AdaptiveX1D(ctx, p=p, theta_x=0.4, rho=1.0, tol=1e-10, max_it=100000, inner_y_loop=True,
                         theta_y=0.4)
     k = 0
@@ -212,4 +213,42 @@ def test_gpu_adaptive_driver_with_inner_loop_reproduces_published_history(ctx, p
             assert r.get("y_true_dofs_after", r["y_true_dofs"]) == a["y_true_dofs_after"], a["source"]
             for key in ("eta_squared", "lambda_y_squared", "y_estimator_squared", "y_estimator_threshold_squared"):
                 assert r[key] == pytest.approx(a[key], rel=6e-6), (key, a["source"])
+    assert k == len(gold)
+
+
+@pytest.mark.parametrize("p,outer_steps", [(1, 22), (2, 21)])
+def test_gpu_g_estimator_on_adaptive_meshes(ctx, p, outer_steps):
+    """G^2 of every inner iteration on the adaptive meshes (degree-(p+1) element assembly on
+    (X_{p+1}, G), G solve by conjugate gradients on the device, energy of lambda_G - I lambda)
+    against the published adaptive_g_effective_rho runs, whose meshes follow the standard
+    runs' history: G true DoFs exactly and G^2 to its 6 digits through outer iteration 21 / 20
+    at p = 1 / 2, including the inner iterations that refine Y; and against the oracle's G
+    (sparse direct solve) on the same GPU solution to 1e-8 on the first steps."""
+    gold = [a for a in json.load(open(INNER_GOLD))
+            if a["p"] == p and a["mode"] == "adaptive_g_effective_rho" and a["outer"] < outer_steps]
+    run = A.AdaptiveX1D(ctx, p=p, tol=1e-10, max_it=100000, inner_y_loop=True, with_g_estimator=True)
+    T = O.Tree()
+    OX = O.Space(T, p, True, T.leaves())
+    OX.refine(list(OX.active))
+    k = 0
+    for outer in range(outer_steps):
+        if outer < 6:
+            OY = O.Space(T, p, False, OX.active)
+        rec = run.step()
+        for r in rec["inner"]:
+            a = gold[k]
+            k += 1
+            print(f"p={p} outer={outer} inner={r['inner_iteration']} G={r['g_true_dofs']} "
+                  f"g2={r['g_lambda_difference_squared']:.6g} (published {a['g_lambda_difference_squared']:.6g}) "
+                  f"its={r['g_solver_iterations']}", flush=True)
+            assert (a["outer"], a["inner"], a["x_true_dofs"]) == (outer, r["inner_iteration"], r["x_true_dofs"])
+            assert r["g_true_dofs"] == a["g_true_dofs"], a["source"]
+            assert r["g_lambda_difference_squared"] == pytest.approx(a["g_lambda_difference_squared"], rel=6e-6), \
+                a["source"]
+            if outer < 6:
+                go = O.g_estimate(OX, OY, r["solution"], u0)
+                assert go["g_true_dofs"] == r["g_true_dofs"]
+                assert r["g_lambda_difference_squared"] == pytest.approx(go["g_lambda_difference_squared"], rel=1e-8)
+        if outer < 6:
+            OX.refine([int(c) for c in rec["marked_x_cells"]])
     assert k == len(gold)
