@@ -312,7 +312,7 @@ class AdaptiveX1D:
 
     def __init__(self, ctx: Context, p=1, theta_x=0.4, rho=1.0, x_uniform_levels=1, u0=smooth_initial_u0,
                  tol=1e-10, max_it=40000, with_y_estimator=False, estimator_precision=APF_FP64, hhat=True,
-                 cheb_degree=16, inner_y_loop=False, theta_y=0.4, max_inner=50, with_g_estimator=False):
+                 cheb_degree=24, inner_y_loop=False, theta_y=0.4, max_inner=50, with_g_estimator=False):
         """hhat: precondition the X block with the graph norm Hhat formed on the device (the
         reference's choice, parabolic_graph_norm_minres.hpp:46-181), else with the Schur
         complement's diagonal; cheb_degree: Chebyshev steps of each block; with_g_estimator: G^2

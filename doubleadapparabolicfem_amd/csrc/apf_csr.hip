@@ -546,10 +546,10 @@ struct Outcome {
 
 // d_x holds the initial guess when use_x0, else it is zeroed; on return it holds x.
 // launch_opt: replay the iterations from a hipGraph and run the two preconditioner blocks
-// on two streams (the default; APF_CSR_GRAPH=0 / APF_CSR_FORK=0 switch either off).  The
+// on st and st2 (the default; APF_CSR_GRAPH=0 / APF_CSR_FORK=0 switch either off).  The
 // allocation-free entry passes false: graphs, streams and events are resources it must not
 // create.
-Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsigned char* work,
+Outcome run_minres(hipStream_t st, hipStream_t st2, const Dev& K, int64_t nY, const Dev* H, unsigned char* work,
                    const double* d_b, double* d_x, bool use_x0, double tol, int max_it, int check_interval,
                    int degree, bool fp32, bool launch_opt)
 {
@@ -648,13 +648,12 @@ Outcome run_minres(hipStream_t st, const Dev& K, int64_t nY, const Dev* H, unsig
         {
             if (a) (void)hipEventDestroy(a);
             if (b) (void)hipEventDestroy(b);
-            if (s2) (void)hipStreamDestroy(s2);
         }
     } fk;
-    bool fork = launch_opt;
+    bool fork = launch_opt && st2 != nullptr;
     if (const char* e = std::getenv("APF_CSR_FORK")) fork = fork && std::atoi(e) != 0;
     if (fork) {
-        HIP_TRY(hipStreamCreate(&fk.s2));  // default priority, as every stream here
+        fk.s2 = st2;
         HIP_TRY(hipEventCreateWithFlags(&fk.a, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&fk.b, hipEventDisableTiming));
     }
@@ -928,7 +927,8 @@ int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, cons
         const Dev Kd{K->n, K->rowptr, K->col, K->val};
         Dev Hd{};
         if (Hhat) Hd = Dev{Hhat->n, Hhat->rowptr, Hhat->col, Hhat->val};
-        const Outcome o = run_minres(st, Kd, n_lambda, Hhat ? &Hd : nullptr, work, d_b, d_x, x0 != nullptr, tol,
+        const Outcome o = run_minres(st, ctx->stream2, Kd, n_lambda, Hhat ? &Hd : nullptr, work, d_b, d_x,
+                                     x0 != nullptr, tol,
                                      max_it, check_interval, cheb_degree, precond_prec == APF_FP32_REFINE, true);
         HIP_TRY(hipMemcpyAsync(x, d_x, K->n * sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -1186,7 +1186,8 @@ extern "C" int apf_csr_minres_solve_noalloc(apf_ctx* ctx, const apf_csr_matrix* 
         HIP_TRY(hipSetDevice(ctx->device));
         const auto t0 = std::chrono::steady_clock::now();
         const Dev Kd{K->n, K->rowptr, K->col, K->val};
-        const Outcome o = run_minres(ctx->stream, Kd, n_lambda, nullptr, static_cast<unsigned char*>(base), d_b, d_x,
+        const Outcome o = run_minres(ctx->stream, nullptr, Kd, n_lambda, nullptr, static_cast<unsigned char*>(base),
+                                     d_b, d_x,
                                      use_x0 != 0, tol, max_it, check_interval, cheb_degree,
                                      precond_prec == APF_FP32_REFINE, false);
         fill_stats(stats, o, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());

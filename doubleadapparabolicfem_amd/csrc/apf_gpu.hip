@@ -1311,7 +1311,7 @@ int apf_ctx_create(apf_ctx** out, int device)
     auto ctx = std::make_unique<apf_ctx>();
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) return set_error(nullptr, APF_HIP_ERROR, "apf_ctx_create: hipSetDevice failed");
-    if (hipStreamCreate(&ctx->stream) != hipSuccess)
+    if (hipStreamCreate(&ctx->stream) != hipSuccess || hipStreamCreate(&ctx->stream2) != hipSuccess)
         return set_error(nullptr, APF_HIP_ERROR, "apf_ctx_create: hipStreamCreate failed");
     // device address of the allocation-free solver's scratch array on this device; on
     // failure apf_csr_minres_solve_noalloc reports it and every other entry still works
@@ -1325,6 +1325,7 @@ void apf_ctx_destroy(apf_ctx* ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
 }
 
@@ -1341,7 +1342,7 @@ int apf_system_create(apf_ctx* ctx, const apf_space* s, apf_system** out)
         if (const char* e = std::getenv("APF_MINRES_GRAPH")) sys->use_graph = std::atoi(e) != 0;
         sys->H = s->H.coarse.empty() ? nullptr : &s->H;
         sys->stream = ctx->stream;
-        HIP_TRY(hipStreamCreate(&sys->stream2));  // default priority, as every stream here
+        sys->stream2 = ctx->stream2;
         const Space& S = s->S;
         const RefTables& T = s->T;
         const int ns = S.mesh.ns(), d = S.dim, Ns = S.Ns;
@@ -1441,7 +1442,6 @@ void apf_system_destroy(apf_system* sys)
     (void)hipStreamSynchronize(sys->stream);
     for (void* p : sys->allocs) (void)hipFree(p);
     if (sys->h_mail) (void)hipHostFree(sys->h_mail);
-    if (sys->stream2) (void)hipStreamDestroy(sys->stream2);
     if (sys->ev0) (void)hipEventDestroy(sys->ev0);
     if (sys->ev1) (void)hipEventDestroy(sys->ev1);
     for (auto& e : sys->evp)

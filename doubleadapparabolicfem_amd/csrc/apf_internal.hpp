@@ -20,7 +20,12 @@
 
 struct apf_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;  // default priority (no priority streams)
+    hipStream_t stream = nullptr;   // default priority (no priority streams)
+    // second stream, shared by the context's systems and solves (calls on one context are
+    // sequential): the estimate's factor phase beside its slab solve, the caller's-CSR
+    // MINRES X-block chain beside the lambda block's.  Created once: a stream costs about
+    // a millisecond to create and destroy.
+    hipStream_t stream2 = nullptr;
     std::string error;
 };
 
@@ -237,7 +242,7 @@ struct System {
     MinresScalars* d_scal = nullptr;
     double* d_res = nullptr;
     double* d_cg = nullptr;  // scalars of the slab reconstruction's conjugate gradients
-    hipStream_t stream2 = nullptr;  // second stream of the estimate (factor beside the slab solve)
+    hipStream_t stream2 = nullptr;  // the context's second stream (factor beside the slab solve)
     std::set<int> est_done;         // precisions with a completed estimate (tiles chosen)
     // pinned host mailbox: a kernel posts a few doubles and a sequence number, the host
     // polls it (cheaper than a device-to-host copy and a stream synchronisation)
