@@ -216,6 +216,8 @@ EXPORTED = {
     "apf_device_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "apf_mesh1d_create": (ctypes.c_int, [ctypes.c_double] * 4 + [ctypes.POINTER(ctypes.c_void_p)]),
     "apf_mesh1d_destroy": (None, [ctypes.c_void_p]),
+    "apf_mesh1d_create_slab": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
+                                              ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
     "apf_mesh1d_info": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int64)] * 3),
     "apf_mesh1d_vertices": (ctypes.c_int, [ctypes.c_void_p] * 3),
     "apf_mesh1d_cells": (ctypes.c_int, [ctypes.c_void_p] * 6),

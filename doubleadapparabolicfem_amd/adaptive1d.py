@@ -32,11 +32,23 @@ from . import (APF_BAD_ARG, APF_FP64, APFError, Context, GeneralEstimator, Gener
 class Mesh1D:
     """Root cell [x0, x1] x [t0, t1] and its refinement tree (apf_mesh1d_*)."""
 
-    def __init__(self, x0=0.0, x1=1.0, t0=0.0, t1=1.0):
-        h = ctypes.c_void_p()
-        _check(lib().apf_mesh1d_create(x0, x1, t0, t1, ctypes.byref(h)), None, "apf_mesh1d_create")
+    def __init__(self, x0=0.0, x1=1.0, t0=0.0, t1=1.0, _handle=None):
+        h = _handle
+        if h is None:
+            h = ctypes.c_void_p()
+            _check(lib().apf_mesh1d_create(x0, x1, t0, t1, ctypes.byref(h)), None, "apf_mesh1d_create")
         self.handle = h
         self._destroy = lib().apf_mesh1d_destroy
+
+    @classmethod
+    def slab(cls, source: "Mesh1D", source_cells, t_begin, t_end):
+        """the mesh of one time slab: the source cells sliced to [t_begin, t_end]
+        (apf_mesh1d_create_slab)"""
+        c = np.ascontiguousarray(source_cells, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().apf_mesh1d_create_slab(source.handle, _ptr(c), len(c), float(t_begin), float(t_end),
+                                            ctypes.byref(h)), None, "apf_mesh1d_create_slab")
+        return cls(_handle=h)
 
     def __del__(self):
         if getattr(self, "handle", None):
@@ -212,6 +224,39 @@ def slab_groups(Y: FESpace1D, max_direct=64):
     return out
 
 
+class TimeSlabSpace:
+    """The reference's time-slab space of a Y space (TimeSlabBuilder::initialize,
+    time_slabs/time_slab_builder.hpp:53-200): slab times = the distinct time endpoints of the
+    active cells, ascending; slab k holds, in the source's active order, every active cell
+    spanning [t_k, t_{k+1}], sliced to it, on a mesh of its own with a Y space (continuous in
+    space, discontinuous in time) of the source's degree.  Attributes: times, slabs (list of
+    dict(mesh, space, source_cells))."""
+
+    def __init__(self, Y: FESpace1D):
+        box = Y.box
+        self.times = np.unique(np.concatenate([box[:, 2], box[:, 3]]))
+        self.slabs = []
+        for k in range(len(self.times) - 1):
+            t0, t1 = self.times[k], self.times[k + 1]
+            inside = np.flatnonzero((box[:, 2] <= t0) & (box[:, 3] >= t1))
+            src = Y.active[inside].astype(np.int32)
+            mesh = Mesh1D.slab(Y.mesh, src, t0, t1)
+            space = FESpace1D(mesh, Y.p, False, np.arange(len(src), dtype=np.int32))
+            self.slabs.append(dict(mesh=mesh, space=space, source_cells=src, t_begin=t0, t_end=t1))
+
+    @property
+    def n_slabs(self):
+        return len(self.slabs)
+
+    def restrict(self, Y: FESpace1D, coef):
+        """per slab, the true coefficients of the Y function `coef` restricted to the slab
+        (nodal values at the slab space's DoFs, each taken on its source cell: exact, the
+        slab cells being slices of the source cells at the same degree)"""
+        pos = {int(c): i for i, c in enumerate(Y.active)}
+        return [prolong_nodal(Y, coef, sl["space"], cell_map=np.array([pos[int(c)] for c in sl["source_cells"]]))
+                for sl in self.slabs]
+
+
 def _lagrange_values(p, s):
     """[len(s), p+1] values of the degree-p Lagrange basis on the equidistant nodes of [0, 1]"""
     s = np.asarray(s, dtype=np.float64)
@@ -224,12 +269,13 @@ def _lagrange_values(p, s):
     return V
 
 
-def prolong_nodal(src: FESpace1D, coef, dst: FESpace1D):
+def prolong_nodal(src: FESpace1D, coef, dst: FESpace1D, cell_map=None):
     """True coefficients in `dst` of the `src` function with true coefficients `coef`: its
     value at the node of each free dst DoF, taken on the first active dst cell (in active
     order, local order) where that DoF is free, evaluated on the src cell containing it
-    (fespace/prolongation.hpp:15-97).  dst's active cells lie inside src's."""
-    anc = src.ancestors_of(dst)
+    (fespace/prolongation.hpp:15-97).  dst's active cells lie inside src's, on the same mesh,
+    or cell_map gives the position in src's active list of each dst cell's container."""
+    anc = src.ancestors_of(dst) if cell_map is None else np.asarray(cell_map, dtype=np.int64)
     loc = src.local_coefficients(coef)
     p, q = src.p, dst.p
     ii, jj = np.divmod(np.arange(dst.n_loc), q + 1)     # local = i_space * (q+1) + i_time
@@ -239,7 +285,7 @@ def prolong_nodal(src: FESpace1D, coef, dst: FESpace1D):
     sx = ((x - bs[:, 0:1]) / (bs[:, 1:2] - bs[:, 0:1])).ravel()
     st = ((t - bs[:, 2:3]) / (bs[:, 3:4] - bs[:, 2:3])).ravel()
     Lx, Lt = _lagrange_values(p, sx), _lagrange_values(p, st)
-    c = loc[np.repeat(np.arange(dst.n_cells), dst.n_loc)].reshape(-1, p + 1, p + 1)
+    c = loc[np.repeat(anc, dst.n_loc)].reshape(-1, p + 1, p + 1)
     vals = np.einsum("na,nab,nb->n", Lx, c, Lt)
     # the free dofs: restriction rows [(d, 1.0)]
     cnt = np.diff(dst.restr_ptr)

@@ -679,6 +679,53 @@ int apf_mesh1d_create(double x0, double x1, double t0, double t1, apf_mesh1d** o
     return APF_OK;
 }
 
+int apf_mesh1d_create_slab(const apf_mesh1d* src, const int* source_cells, int64_t n, double t_begin, double t_end,
+                           apf_mesh1d** out)
+{
+    if (!src || !out || (n > 0 && !source_cells) || n < 0) return bad("apf_mesh1d_create_slab: null argument");
+    if (!(std::isfinite(t_begin) && std::isfinite(t_end)) || !(t_begin < t_end))
+        return bad("apf_mesh1d_create_slab: expected t_begin < t_end");
+    auto* m = new apf_mesh1d();
+    try {
+        for (int64_t i = 0; i < n; ++i) {
+            const int sc = source_cells[i];
+            if (sc < 0 || sc >= static_cast<int>(src->cells.size()))
+                throw std::invalid_argument("source cell id out of range");
+            const Cell1& c = src->cells[sc];
+            const double t0 = src->tv[c.tv[0]], t1 = src->tv[c.tv[1]];
+            const double tol = 1e-14;
+            if (t_begin < t0 - tol || t_end > t1 + tol)
+                throw std::invalid_argument("requested slice is not contained in source cell");
+            Cell1 s;
+            s.id = static_cast<int>(m->cells.size());
+            s.parent = -1;
+            s.leaf = true;
+            s.generation = c.generation;
+            s.slev = c.slev;
+            s.tlev = c.tlev;
+            s.sv[0] = apf_mesh1d::vertex(m->xv, m->xreg, src->xv[c.sv[0]]);
+            s.sv[1] = apf_mesh1d::vertex(m->xv, m->xreg, src->xv[c.sv[1]]);
+            s.tv[0] = apf_mesh1d::vertex(m->tv, m->treg, t_begin);
+            s.tv[1] = apf_mesh1d::vertex(m->tv, m->treg, t_end);
+            if (m->xv[s.sv[1]] < m->xv[s.sv[0]]) std::swap(s.sv[0], s.sv[1]);
+            s.sb[0] = c.sb[0];
+            s.sb[1] = c.sb[1];
+            s.tb[0] = s.tb[1] = true;   // both temporal faces bound the slab-local problem
+            for (int f = 0; f < 2; ++f)
+                if (s.sb[f] && std::find(m->xbnd.begin(), m->xbnd.end(), s.sv[f]) == m->xbnd.end())
+                    m->xbnd.push_back(s.sv[f]);
+            m->cells.push_back(s);
+        }
+        m->tbnd.clear();
+        for (int t = 0; t < static_cast<int>(m->tv.size()); ++t) m->tbnd.push_back(t);
+    } catch (const std::exception& e) {
+        delete m;
+        return bad(std::string("apf_mesh1d_create_slab: ") + e.what());
+    }
+    *out = m;
+    return APF_OK;
+}
+
 void apf_mesh1d_destroy(apf_mesh1d* m) { delete m; }
 
 int apf_mesh1d_info(const apf_mesh1d* m, int64_t* n_cells, int64_t* n_spatial_vertices, int64_t* n_temporal_vertices)

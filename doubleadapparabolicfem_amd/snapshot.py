@@ -129,6 +129,8 @@ def write_cellwise_error_components_binary(estimate, path, sort_descending=True)
 DOFS_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:11
 FUNCTION_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:12
 FUNCTION_TRUE_COEFFICIENTS_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:14
+TIME_SLAB_SPACE_METADATA_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:15
+TIME_SLAB_PROVENANCE_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:16
 
 
 def write_space_mesh_binary(space, path):
@@ -215,3 +217,50 @@ def write_function_true_coefficients_binary(coef, path):
     (write_function_true_coefficients_binary, fespace/io/write_function_binary.hpp:176-196)."""
     coef = np.asarray(coef, dtype=np.float64).reshape(-1)
     _write(path, [np.asarray([FUNCTION_TRUE_COEFFICIENTS_BINARY_FORMAT_VERSION, len(coef)], dtype=np.int32), coef])
+
+
+def _time_slab_dir(out_dir, k):
+    """slab_%04d (io/detail/time_slab_io_detail.hpp:9-22)"""
+    import os
+    d = os.path.join(out_dir, "slab_%04d" % k)
+    os.makedirs(d, exist_ok=True)
+    return d
+
+
+def write_time_slab_space_binary(tss, out_dir, metadata_filename="time_slab_space.bin", mesh_filename="mesh.bin",
+                                 dofs_filename="dofs.bin", provenance_filename="provenance.bin"):
+    """The files of write_time_slab_space_binary (io/write_time_slab_space_binary.hpp:14-118)
+    for an adaptive1d.TimeSlabSpace: the metadata file [version, n_slabs, n_slab_times] and
+    the slab times; per slab, in slab_%04d/, the slab space's mesh and DoF files and the
+    provenance file [version, slab_id, n_cells], [t_begin, t_end], the slab-local cell ids,
+    the source cell ids and each slice's time interval."""
+    import os
+    os.makedirs(out_dir, exist_ok=True)
+    _write(os.path.join(out_dir, metadata_filename),
+           [np.asarray([TIME_SLAB_SPACE_METADATA_BINARY_FORMAT_VERSION, tss.n_slabs, len(tss.times)], dtype=np.int32),
+            np.asarray(tss.times, dtype=np.float64)])
+    for k, sl in enumerate(tss.slabs):
+        d = _time_slab_dir(out_dir, k)
+        write_space_mesh_binary(sl["space"], os.path.join(d, mesh_filename))
+        write_dofs_binary(sl["space"], os.path.join(d, dofs_filename))
+        n = len(sl["source_cells"])
+        _write(os.path.join(d, provenance_filename),
+               [np.asarray([TIME_SLAB_PROVENANCE_BINARY_FORMAT_VERSION, k, n], dtype=np.int32),
+                np.asarray([sl["t_begin"], sl["t_end"]], dtype=np.float64),
+                np.arange(n, dtype=np.int32), np.asarray(sl["source_cells"], dtype=np.int32),
+                np.tile(np.asarray([sl["t_begin"], sl["t_end"]], dtype=np.float64), n)])
+
+
+def write_time_slab_function_binary(tss, slab_coefs, out_dir, samples_x=32, samples_t=32, **names):
+    """The files of write_time_slab_function_binary (io/write_time_slab_function_binary.hpp:10-53):
+    the time-slab space files, then per slab the function samples and true coefficients of
+    that slab's function (slab_coefs[k], e.g. TimeSlabSpace.restrict of a Y function)."""
+    import os
+    write_time_slab_space_binary(tss, out_dir, **{k: v for k, v in names.items() if k.endswith("_filename")
+                                                  and k not in ("function_filename", "coefficients_filename")})
+    for k, sl in enumerate(tss.slabs):
+        d = _time_slab_dir(out_dir, k)
+        write_function_binary(sl["space"], slab_coefs[k], os.path.join(d, names.get("function_filename", "function.bin")),
+                              samples_x, samples_t)
+        write_function_true_coefficients_binary(
+            slab_coefs[k], os.path.join(d, names.get("coefficients_filename", "true_coefficients.bin")))

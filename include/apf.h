@@ -468,6 +468,13 @@ typedef struct apf_fespace1d apf_fespace1d;
 /* Root cell [x0, x1] x [t0, t1] (Mesh(x0, x1, t0, t1), mesh/mesh.hpp:52-59 with
  * mesh/initialization/root_cell.hpp:60-107); cell 0, spatial vertices 0, 1, temporal 0, 1. */
 int apf_mesh1d_create(double x0, double x1, double t0, double t1, apf_mesh1d** out);
+/* The mesh of one time slab: the source cells (in the given order) sliced to
+ * [t_begin, t_end], each a new leaf cell (ids 0..n-1) with the source's generation and
+ * levels and spatial boundary flags, both temporal faces on the slab's boundary, vertices
+ * found or created by coordinate (TimeSlab::append_slice_from_source,
+ * time_slabs/time_slab.hpp:180-249; mesh/refinement/time_slicing/slice_cell_in_time_1d.hpp:14-113). */
+int apf_mesh1d_create_slab(const apf_mesh1d* source, const int* source_cells, int64_t n, double t_begin,
+                           double t_end, apf_mesh1d** out);
 void apf_mesh1d_destroy(apf_mesh1d* m);
 int apf_mesh1d_info(const apf_mesh1d* m, int64_t* n_cells, int64_t* n_spatial_vertices, int64_t* n_temporal_vertices);
 /* vertex coordinates by id (either may be NULL) */

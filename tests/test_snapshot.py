@@ -273,3 +273,126 @@ def test_adaptive_function_files(tmp_path):
         n, = _header(f, 1, 1)
         np.testing.assert_array_equal(_block(f, np.float64, int(n)), coef)
         assert np.fromfile(f, dtype=np.uint8).size == 0
+
+
+def _adaptive_pair_1d(p, steps):
+    """library X / Y spaces after `steps` outer iterations of the published marking (oracle
+    solve), with the oracle's Y space beside them"""
+    from doubleadapparabolicfem_amd import adaptive1d as A
+    from oracle import adaptive1d as O
+    u0 = lambda x: np.sin(np.pi * x)
+    T = O.Tree()
+    X = O.Space(T, p, True, T.leaves())
+    X.refine(list(X.active))
+    m = A.Mesh1D()
+    LX = A.FESpace1D(m, p, True, m.leaf_cells())
+    LX.refine(LX.active)
+    for _ in range(steps):
+        Y = O.Space(T, p, False, X.active)
+        K, b = O.assemble(X, Y, u0)
+        x = O.solve(K, b)
+        l2, t2 = O.x_indicators(X, Y, x, u0)
+        lx = np.zeros(len(X.active))
+        np.add.at(lx, Y.ancestors_in(X), l2)
+        mk, _ = O.doerfler_mark(lx + t2, 0.4, ids=X.active)
+        X.refine(mk)
+        LX.refine(mk)
+    Y = O.Space(T, p, False, X.active)
+    K, b = O.assemble(X, Y, u0)
+    return LX, A.FESpace1D(m, p, False, LX.active), Y, O.solve(K, b)
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_time_slab_space_and_files_1d(tmp_path, p):
+    """TimeSlabSpace of an adaptive Y (time_slab_builder.hpp:53-200): the slab times are the
+    cells' distinct time endpoints, slab k holds the active cells spanning it in active order
+    (the same cells, left to right, as the oracle's slab partition), each slab space numbers its
+    DoFs by first occurrence with the spatial boundary eliminated, (n_cells p - 1)(p + 1) true
+    DoFs; the restriction of a Y function is exact; the files carry the reference's layout."""
+    from doubleadapparabolicfem_amd import adaptive1d as A
+    from oracle import adaptive1d as O
+    LX, LY, OY, x = _adaptive_pair_1d(p, 8 if p == 1 else 6)
+    tss = A.TimeSlabSpace(LY)
+    og = O.slab_groups(OY)
+    oslabs = [src for g in og for src in g["src"]]
+    assert tss.n_slabs == len(oslabs) and tss.n_slabs > 4
+    pos = {int(c): i for i, c in enumerate(LY.active)}
+    for sl, osrc in zip(tss.slabs, oslabs):
+        ps = [pos[int(c)] for c in sl["source_cells"]]
+        assert ps == sorted(ps)
+        assert sorted(ps) == sorted(osrc)
+        sp_ = sl["space"]
+        n = len(ps)
+        assert sp_.n_dofs == (n * p - 1) * (p + 1) and sp_.n_constrained == 0
+        # first-occurrence numbering by node (x, time index); nodes on x = 0, 1 eliminated
+        seen, nxt = {}, 0
+        for i in range(n):
+            x0, x1 = sp_.box[i, :2]
+            for l in range(sp_.n_loc):
+                a, j = divmod(l, p + 1)
+                xx = x0 + (x1 - x0) * a / p
+                if xx in (0.0, 1.0):
+                    assert sp_.cell_dofs[i, l] == -1
+                    continue
+                key = (round(xx * 2**40), j)
+                if key not in seen:
+                    seen[key] = nxt
+                    nxt += 1
+                assert sp_.cell_dofs[i, l] == seen[key]
+    coefs = tss.restrict(LY, x[:LY.n_dofs])
+    lam_loc = LY.local_coefficients(x[:LY.n_dofs])
+    L = A._lagrange_values
+    rng = np.random.default_rng(1)
+    for k, sl in enumerate(tss.slabs):
+        loc = sl["space"].local_coefficients(coefs[k])
+        for i, c in enumerate(sl["source_cells"]):
+            a = pos[int(c)]
+            s_, t_ = rng.random(), rng.random()
+            bx, bs = sl["space"].box[i], LY.box[a]
+            xx, tt = bx[0] + (bx[1] - bx[0]) * s_, bx[2] + (bx[3] - bx[2]) * t_
+            v1 = (L(p, [s_]) @ loc[i].reshape(p + 1, p + 1) @ L(p, [t_]).T).item()
+            v2 = (L(p, [(xx - bs[0]) / (bs[1] - bs[0])]) @ lam_loc[a].reshape(p + 1, p + 1)
+                  @ L(p, [(tt - bs[2]) / (bs[3] - bs[2])]).T).item()
+            assert abs(v1 - v2) <= 1e-12 * max(1.0, abs(v2))
+    snapshot.write_time_slab_function_binary(tss, coefs, str(tmp_path))
+    meta = np.fromfile(tmp_path / "time_slab_space.bin", dtype=np.int32, count=3)
+    assert list(meta) == [1, tss.n_slabs, tss.n_slabs + 1]
+    times = np.fromfile(tmp_path / "time_slab_space.bin", dtype=np.float64, offset=12)
+    np.testing.assert_array_equal(times, tss.times)
+    for k in (0, tss.n_slabs - 1):
+        d = tmp_path / ("slab_%04d" % k)
+        sl = tss.slabs[k]
+        n = len(sl["source_cells"])
+        raw = (d / "provenance.bin").read_bytes()
+        head = np.frombuffer(raw[:12], dtype=np.int32)
+        assert list(head) == [1, k, n]
+        tb = np.frombuffer(raw[12:28], dtype=np.float64)
+        assert list(tb) == [sl["t_begin"], sl["t_end"]]
+        ids = np.frombuffer(raw[28:28 + 4 * n], dtype=np.int32)
+        src = np.frombuffer(raw[28 + 4 * n:28 + 8 * n], dtype=np.int32)
+        iv = np.frombuffer(raw[28 + 8 * n:], dtype=np.float64)
+        assert list(ids) == list(range(n)) and list(src) == list(sl["source_cells"])
+        assert len(iv) == 2 * n and np.all(iv[0::2] == sl["t_begin"]) and np.all(iv[1::2] == sl["t_end"])
+        for name in ("mesh.bin", "dofs.bin", "function.bin", "true_coefficients.bin"):
+            assert (d / name).stat().st_size > 0
+        tc = np.fromfile(d / "true_coefficients.bin", dtype=np.int32, count=2)
+        assert list(tc) == [1, len(coefs[k])]
+
+
+def test_prolong_nodal_onto_finer_cells_matches_oracle():
+    """adaptive1d.prolong_nodal from X onto a degree-(p+1) space on finer cells (Y refined
+    past X, the G space of an inner iteration that refined Y) equals the oracle's
+    prolongation (fespace/prolongation.hpp:15-97) to round-off."""
+    from doubleadapparabolicfem_amd import adaptive1d as A
+    from oracle import adaptive1d as O
+    p = 1
+    LX, LY, OY, x = _adaptive_pair_1d(p, 5)
+    T = OY.tree
+    OX = O.Space(T, p, True, OY.active)
+    marks = [int(c) for c in LY.active[::3]]
+    LY.refine(marks)
+    OY.refine(marks)
+    LG = A.FESpace1D(LY.mesh, p + 1, False, LY.active)
+    OG = O.Space(T, p + 1, False, OY.active)
+    u = x[-LX.n_dofs:]
+    np.testing.assert_allclose(A.prolong_nodal(LX, u, LG), O.prolong_nodal(OX, u, OG), rtol=1e-14, atol=1e-15)
