@@ -248,6 +248,38 @@ class TimeSlabSpace:
     def n_slabs(self):
         return len(self.slabs)
 
+    def edge_patches(self):
+        """The 1+1D patches of every slab (TimeSlabEdgePatchBuilder,
+        time_slabs/time_slab_edge_patch_builder.hpp:349-527): the slab's cells sorted by
+        (x_begin, x_end, slab cell id); a boundary patch on the first cell's left vertex, one
+        interior patch per shared vertex of consecutive cells, a boundary patch on the last
+        cell's right vertex; patch ids consecutive over the slabs.  Per patch: dict(patch_id,
+        slab_id, spatial_vertex_id (slab mesh), kind (0 interior, 1 boundary), x_vertex,
+        t_begin, t_end, cells = [dict(slab_cell_id, source_cell_id, left_vertex_id,
+        right_vertex_id, side (0 left of the vertex, 1 right of it), x_begin, x_end)])."""
+        out = []
+        for k, sl in enumerate(self.slabs):
+            sp = sl["space"]
+            vids = sp.cells
+            cells = [dict(slab_cell_id=i, source_cell_id=int(sl["source_cells"][i]), left_vertex_id=int(vids[i, 0]),
+                          right_vertex_id=int(vids[i, 1]), x_begin=float(sp.box[i, 0]), x_end=float(sp.box[i, 1]))
+                     for i in range(sp.n_cells)]
+            cells.sort(key=lambda c: (c["x_begin"], c["x_end"], c["slab_cell_id"]))
+            if not cells:
+                continue
+
+            def patch(kind, vid, xv, members):
+                return dict(patch_id=len(out), slab_id=k, spatial_vertex_id=vid, kind=kind, x_vertex=xv,
+                            t_begin=float(sl["t_begin"]), t_end=float(sl["t_end"]), cells=members)
+
+            out.append(patch(1, cells[0]["left_vertex_id"], cells[0]["x_begin"], [dict(cells[0], side=1)]))
+            for a, b in zip(cells[:-1], cells[1:]):
+                if a["right_vertex_id"] != b["left_vertex_id"] or abs(a["x_end"] - b["x_begin"]) > 1e-12:
+                    raise ValueError("edge_patches: adjacent slab cells do not share a spatial vertex")
+                out.append(patch(0, a["right_vertex_id"], a["x_end"], [dict(a, side=0), dict(b, side=1)]))
+            out.append(patch(1, cells[-1]["right_vertex_id"], cells[-1]["x_end"], [dict(cells[-1], side=0)]))
+        return out
+
     def restrict(self, Y: FESpace1D, coef):
         """per slab, the true coefficients of the Y function `coef` restricted to the slab
         (nodal values at the slab space's DoFs, each taken on its source cell: exact, the

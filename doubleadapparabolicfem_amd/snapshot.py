@@ -131,6 +131,7 @@ FUNCTION_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:12
 FUNCTION_TRUE_COEFFICIENTS_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:14
 TIME_SLAB_SPACE_METADATA_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:15
 TIME_SLAB_PROVENANCE_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:16
+TIME_SLAB_EDGE_PATCHES_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:17
 
 
 def write_space_mesh_binary(space, path):
@@ -264,3 +265,36 @@ def write_time_slab_function_binary(tss, slab_coefs, out_dir, samples_x=32, samp
                               samples_x, samples_t)
         write_function_true_coefficients_binary(
             slab_coefs[k], os.path.join(d, names.get("coefficients_filename", "true_coefficients.bin")))
+
+
+def write_time_slab_edge_patches_binary(tss, path):
+    """The 1+1D edge-patch file (format version 1, io/write_time_slab_edge_patches_binary.hpp:
+    14-155 with detail/time_slab_patch_io_detail.hpp:9-103) of TimeSlabSpace.edge_patches():
+    header [version, n_slabs, n_patches, 2]; the slab index (offsets [n_slabs+1], patch ids);
+    per patch its id, slab, spatial vertex, kind (0 interior, 1 boundary), cell count, vertex
+    x, t_begin, t_end; per patch cell (2 per patch, -1 / 0.0 padded) the slab cell, source
+    cell, left and right vertex ids, side (0 left of the vertex, 1 right) and x extent."""
+    patches = tss.edge_patches()
+    n, mc = len(patches), 2
+    offsets = [0]
+    ids = []
+    for k in range(tss.n_slabs):
+        ids += [q["patch_id"] for q in patches if q["slab_id"] == k]
+        offsets.append(len(ids))
+    ci = {key: np.full(n * mc, -1, dtype=np.int32) for key in ("slab_cell_id", "source_cell_id", "left_vertex_id",
+                                                                "right_vertex_id", "side")}
+    cx = {key: np.zeros(n * mc) for key in ("x_begin", "x_end")}
+    for q in patches:
+        for j, c in enumerate(q["cells"]):
+            for key in ci:
+                ci[key][q["patch_id"] * mc + j] = c[key]
+            for key in cx:
+                cx[key][q["patch_id"] * mc + j] = c[key]
+    col = lambda key, dt: np.asarray([q[key] for q in patches], dtype=dt)
+    _write(path, [np.asarray([TIME_SLAB_EDGE_PATCHES_BINARY_FORMAT_VERSION, tss.n_slabs, n, mc], dtype=np.int32),
+                  np.asarray(offsets, dtype=np.int32), np.asarray(ids, dtype=np.int32),
+                  col("patch_id", np.int32), col("slab_id", np.int32), col("spatial_vertex_id", np.int32),
+                  col("kind", np.int32), np.asarray([len(q["cells"]) for q in patches], dtype=np.int32),
+                  col("x_vertex", np.float64), col("t_begin", np.float64), col("t_end", np.float64),
+                  ci["slab_cell_id"], ci["source_cell_id"], ci["left_vertex_id"], ci["right_vertex_id"], ci["side"],
+                  cx["x_begin"], cx["x_end"]])

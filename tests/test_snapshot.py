@@ -396,3 +396,64 @@ def test_prolong_nodal_onto_finer_cells_matches_oracle():
     OG = O.Space(T, p + 1, False, OY.active)
     u = x[-LX.n_dofs:]
     np.testing.assert_allclose(A.prolong_nodal(LX, u, LG), O.prolong_nodal(OX, u, OG), rtol=1e-14, atol=1e-15)
+
+
+def test_time_slab_edge_patches_1d(tmp_path):
+    """TimeSlabSpace.edge_patches (time_slab_edge_patch_builder.hpp:349-527) and its file: per
+    slab n_cells + 1 patches (two boundary patches, one interior patch per shared vertex),
+    consecutive ids, every slab cell in one or two patches, the interior patch's vertex the
+    left cell's right vertex; the file parses back to the same arrays in the reference's
+    block order."""
+    from doubleadapparabolicfem_amd import adaptive1d as A
+    LX, LY, OY, x = _adaptive_pair_1d(2, 6)
+    tss = A.TimeSlabSpace(LY)
+    P = tss.edge_patches()
+    assert [q["patch_id"] for q in P] == list(range(len(P)))
+    for k, sl in enumerate(tss.slabs):
+        mine = [q for q in P if q["slab_id"] == k]
+        n = len(sl["source_cells"])
+        assert len(mine) == n + 1
+        assert [q["kind"] for q in mine] == [1] + [0] * (n - 1) + [1]
+        assert mine[0]["x_vertex"] == 0.0 and mine[-1]["x_vertex"] == 1.0
+        counts = np.zeros(n, int)
+        for q in mine:
+            for c in q["cells"]:
+                counts[c["slab_cell_id"]] += 1
+            if q["kind"] == 0:
+                a, b = q["cells"]
+                assert a["right_vertex_id"] == b["left_vertex_id"] == q["spatial_vertex_id"]
+                assert (a["side"], b["side"]) == (0, 1) and a["x_end"] == b["x_begin"] == q["x_vertex"]
+        assert np.all(counts == 2)
+        xs = [q["x_vertex"] for q in mine]
+        assert xs == sorted(xs)
+    path = tmp_path / "time_slab_edge_patches.bin"
+    snapshot.write_time_slab_edge_patches_binary(tss, str(path))
+    raw = path.read_bytes()
+    o = 0
+
+    def take(dt, count):
+        nonlocal o
+        a = np.frombuffer(raw, dtype=dt, count=count, offset=o)
+        o += a.nbytes
+        return a
+
+    head = take(np.int32, 4)
+    ns, npch = tss.n_slabs, len(P)
+    assert list(head) == [1, ns, npch, 2]
+    offs = take(np.int32, ns + 1)
+    assert offs[-1] == npch and list(take(np.int32, npch)) == list(range(npch))
+    assert list(take(np.int32, npch)) == list(range(npch))
+    assert list(take(np.int32, npch)) == [q["slab_id"] for q in P]
+    assert list(take(np.int32, npch)) == [q["spatial_vertex_id"] for q in P]
+    assert list(take(np.int32, npch)) == [q["kind"] for q in P]
+    assert list(take(np.int32, npch)) == [len(q["cells"]) for q in P]
+    assert list(take(np.float64, npch)) == [q["x_vertex"] for q in P]
+    take(np.float64, npch)
+    take(np.float64, npch)
+    slab_cell = take(np.int32, 2 * npch)
+    assert slab_cell[1] == -1 and slab_cell[0] == P[0]["cells"][0]["slab_cell_id"]
+    for _ in range(4):
+        take(np.int32, 2 * npch)
+    take(np.float64, 2 * npch)
+    take(np.float64, 2 * npch)
+    assert o == len(raw)
