@@ -715,38 +715,39 @@ Outcome run_minres(hipStream_t st, hipStream_t st2, const Dev& K, int64_t nY, co
         w2 = w;
         w = wt;
     };
-    // The buffer rotations have period 3: three iterations are one fixed launch sequence,
-    // captured once per solve and replayed (kernel launches and stream-ordering events only).
+    // The buffer rotations have period 3: three iterations from a given rotation phase are one
+    // fixed launch sequence.  One graph per phase (iterations done mod 3) is captured when first
+    // needed and replayed; a batch of check_interval iterations replays its phase's graph
+    // batch / 3 times and launches the remaining batch % 3 iterations eagerly, so the true
+    // residual is checked at the same iterations as without graphs.
     struct GraphGuard {
-        hipGraphExec_t e = nullptr;
-        hipGraph_t g = nullptr;
+        hipGraphExec_t e[3] = {nullptr, nullptr, nullptr};
+        hipGraph_t g[3] = {nullptr, nullptr, nullptr};
         ~GraphGuard()
         {
-            if (e) (void)hipGraphExecDestroy(e);
-            if (g) (void)hipGraphDestroy(g);
+            for (int q = 0; q < 3; ++q) {
+                if (e[q]) (void)hipGraphExecDestroy(e[q]);
+                if (g[q]) (void)hipGraphDestroy(g[q]);
+            }
         }
     } gg;
-    bool use_graph = launch_opt && !o.converged && max_it >= 3 && check_interval >= 3;
+    bool use_graph = launch_opt;
     if (const char* e = std::getenv("APF_CSR_GRAPH")) use_graph = use_graph && std::atoi(e) != 0;
-    if (use_graph) {
-        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int q = 0; q < 3; ++q) iteration();
-        HIP_TRY(hipStreamEndCapture(st, &gg.g));
-        HIP_TRY(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
-    }
     while (!o.converged && o.iterations < max_it) {
-        int batch = std::min(check_interval, max_it - o.iterations);
-        if (gg.e && batch >= 3) {
-            batch = (batch / 3) * 3;
-            for (int b = 0; b < batch; b += 3) HIP_TRY(hipGraphLaunch(gg.e, st));
-        } else {
-            // eager iterations continue the rotation where the graph's left it (a multiple of 3)
-            if (gg.e) {
-                HIP_TRY(hipGraphExecDestroy(gg.e));
-                gg.e = nullptr;
+        const int batch = std::min(check_interval, max_it - o.iterations);
+        int eager = batch;
+        if (use_graph && batch >= 3) {
+            const int ph = o.iterations % 3;
+            if (!gg.e[ph]) {
+                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                for (int q = 0; q < 3; ++q) iteration();
+                HIP_TRY(hipStreamEndCapture(st, &gg.g[ph]));
+                HIP_TRY(hipGraphInstantiate(&gg.e[ph], gg.g[ph], nullptr, nullptr, 0));
             }
-            for (int b = 0; b < batch; ++b) iteration();
+            for (int b = 0; b + 3 <= batch; b += 3) HIP_TRY(hipGraphLaunch(gg.e[ph], st));
+            eager = batch % 3;
         }
+        for (int b = 0; b < eager; ++b) iteration();
         o.iterations += batch;
         HIP_TRY(hipGetLastError());
         o.rel = true_residual();
