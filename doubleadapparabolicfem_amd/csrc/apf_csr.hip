@@ -1092,6 +1092,25 @@ extern "C" int apf_csr_block_pcg_solve(apf_ctx* ctx, const apf_csr_matrix* A, in
         o.rel = true_residual();
         o.converged = o.rel <= tol;
         int restarts = 0;
+        auto iteration = [&]() {
+            hipLaunchKernelGGL(k_block_spmv, dim3(G), dim3(256), 0, st, n, A->rowptr, A->col, A->val, p, q, part);
+            hipLaunchKernelGGL(k_scalar_from_partials, dim3(1), dim3(256), 0, st, part, G, sc, 1);
+            hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, st, n, sc, p, q, dx, r, part + G);
+            precond();
+            hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(256), 0, st, n, sc, 0, z, p);
+            hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(64), 0, st, sc, 2, 0);
+        };
+        struct GraphGuard {
+            hipGraphExec_t e = nullptr;
+            hipGraph_t g = nullptr;
+            ~GraphGuard()
+            {
+                if (e) (void)hipGraphExecDestroy(e);
+                if (g) (void)hipGraphDestroy(g);
+            }
+        } gg;
+        bool use_graph = true;
+        if (const char* e = std::getenv("APF_CSR_GRAPH")) use_graph = std::atoi(e) != 0;
         while (!o.converged && o.iterations < max_it) {
             // one CG cycle from the current true residual
             precond();
@@ -1101,14 +1120,18 @@ extern "C" int apf_csr_block_pcg_solve(apf_ctx* ctx, const apf_csr_matrix* A, in
             bool cycle_done = false;
             while (!cycle_done && o.iterations < max_it) {
                 const int batch = std::min(check_interval, max_it - o.iterations);
-                for (int it = 0; it < batch; ++it) {
-                    hipLaunchKernelGGL(k_block_spmv, dim3(G), dim3(256), 0, st, n, A->rowptr, A->col, A->val, p,
-                                       q, part);
-                    hipLaunchKernelGGL(k_scalar_from_partials, dim3(1), dim3(256), 0, st, part, G, sc, 1);
-                    hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(256), 0, st, n, sc, p, q, dx, r, part + G);
-                    precond();
-                    hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(256), 0, st, n, sc, 0, z, p);
-                    hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(64), 0, st, sc, 2, 0);
+                if (use_graph && batch == check_interval) {
+                    // a full batch: the same launches on the same buffers every time, captured
+                    // once and replayed
+                    if (!gg.e) {
+                        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                        for (int it = 0; it < batch; ++it) iteration();
+                        HIP_TRY(hipStreamEndCapture(st, &gg.g));
+                        HIP_TRY(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
+                    }
+                    HIP_TRY(hipGraphLaunch(gg.e, st));
+                } else {
+                    for (int it = 0; it < batch; ++it) iteration();
                 }
                 o.iterations += batch;
                 hipLaunchKernelGGL(k_scalar_from_partials, dim3(1), dim3(256), 0, st, part + G, G, sc, 3);

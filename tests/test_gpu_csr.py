@@ -4,6 +4,7 @@ assembler does, and the GPU solve is held to the oracle's direct solve."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg  # noqa: F401  (sp.linalg)
 
 import doubleadapparabolicfem_amd as apf
 from doubleadapparabolicfem_amd import meshes
@@ -138,3 +139,30 @@ def test_csr_graph_and_block_streams_match_eager_bitwise(ctx, monkeypatch, with_
     for key, (x, st) in out.items():
         assert st["iterations"] == s0["iterations"], key
         assert np.array_equal(x, x0), key
+
+
+def test_block_pcg_solves_the_leading_block_and_graph_matches_eager(ctx, monkeypatch):
+    """apf_csr_block_pcg_solve on K's SPD leading block A_y: the true residual meets the
+    tolerance, the solution equals scipy's direct solve to 1e-9, apf_csr_block_energy equals
+    d^T A_y d, apf_csr_spmv equals K x; batches replayed from a hipGraph give the eager
+    iterate bit for bit (APF_CSR_GRAPH=0)."""
+    print("csr block pcg", flush=True)
+    S = _system(2, 2, 1)
+    K = S.K().tocsr()
+    nY = S.A.shape[0]
+    A = K[:nY, :nY].tocsc()
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(nY)
+    Km = apf.CsrMatrix(ctx, K)
+    monkeypatch.setenv("APF_CSR_GRAPH", "1")
+    x1, s1 = Km.block_pcg(b, nY, tol=1e-12, check_interval=20, degree=8)
+    monkeypatch.setenv("APF_CSR_GRAPH", "0")
+    x0, s0 = Km.block_pcg(b, nY, tol=1e-12, check_interval=20, degree=8)
+    assert s1["converged"] == 1 and s1["true_relative_residual"] <= 1e-12
+    assert s1["iterations"] == s0["iterations"] and np.array_equal(x1, x0)
+    xd = sp.linalg.spsolve(A, b)
+    assert np.linalg.norm(x1 - xd) <= 1e-9 * np.linalg.norm(xd)
+    d = rng.standard_normal(nY)
+    assert Km.block_energy(d, nY) == pytest.approx(float(d @ (A @ d)), rel=1e-12)
+    v = rng.standard_normal(K.shape[0])
+    np.testing.assert_allclose(Km.spmv(v), K @ v, rtol=1e-12, atol=1e-12 * np.abs(K @ v).max())
