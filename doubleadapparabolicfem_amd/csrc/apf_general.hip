@@ -1812,6 +1812,7 @@ int estimator_create(apf_general_system* gsys, int n_groups, apf_space* const* s
             G.d_iw = dev(iw);
             apf_system* sys_g = nullptr;
             if (int c = apf_system_create(ctx, G.space, &sys_g)) throw std::runtime_error(g_last_error + " (code " + std::to_string(c) + ")");
+            sys_g->slab_only = true;   // slab solves only: no X-column shifts in its multigrid
             // a group mesh without coarser meshes: its multigrid is the one level, solved directly
             if (sliced && !sys_g->H) sys_g->H = &G.space->H;
             G.sys = sys_g;

@@ -821,7 +821,7 @@ void mg_prepare(System* sys, bool mg32)
     HIP_TRY(hipStreamSynchronize(st));
     const auto t_a = std::chrono::steady_clock::now();
     MGSetupHost hs;
-    setup_mg(S, *sys->H, K, M, sys->tables->Mt, sys->tables->Dt, hs);
+    setup_mg(S, *sys->H, K, M, sys->tables->Mt, sys->tables->Dt, hs, sys->slab_only);
     const auto t_b = std::chrono::steady_clock::now();
     const int nyc = S.mesh.m() * S.Nt;
     if (mg32) {
@@ -1609,6 +1609,8 @@ int apf_minres_solve_ex(apf_system* sys, double tol, int max_it, int check_inter
         return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: unknown preconditioner precision");
     if (precond_kind != APF_PRECOND_DIAGONAL && precond_kind != APF_PRECOND_MULTIGRID)
         return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: unknown preconditioner kind");
+    if (precond_kind == APF_PRECOND_MULTIGRID && sys->slab_only)
+        return set_error(sys->ctx, APF_BAD_ARG, "apf_minres_solve: a slab-group system has no MINRES multigrid");
     if (precond_kind == APF_PRECOND_MULTIGRID && !sys->H)
         return set_error(sys->ctx, APF_BAD_ARG,
                          "apf_minres_solve: the multigrid preconditioner needs nested coarse meshes "

@@ -298,6 +298,9 @@ struct System {
     float t_factor_ms = 0.f, t_rhs_ms = 0.f, t_solve_ms = 0.f;
     bool assembled = false;
     bool use_graph = true;   // capture the MINRES iteration in a hipGraph
+    // a system that only runs slab solves (the general estimator's slab groups): its multigrid
+    // serves the Y columns alone, so the coarsest inverses are formed for shift 0 only
+    bool slab_only = false;
 
     void spmv(const double* x, double* y);
     void spmv_rows(const double* x, double* y, int64_t rows);

@@ -468,7 +468,7 @@ static void setup_time_factors(const Space& S, const std::vector<double>& Mt, co
 
 void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double>& Ksp,
               const std::vector<double>& Msp, const std::vector<double>& Mt, const std::vector<double>& Dt,
-              MGSetupHost& out)
+              MGSetupHost& out, bool zero_shift_only)
 {
     const auto t_start = std::chrono::steady_clock::now();
     const int L = static_cast<int>(H.coarse.size()) + 1;
@@ -568,7 +568,7 @@ void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double
     for (int i = 0; i < T; ++i) out.shifts_all.push_back(out.shift[i]);
     const int nsft = static_cast<int>(out.shifts_all.size());
     out.coarse_inv.assign(static_cast<size_t>(nsft) * out.blk_total, 0.0);
-    for (int q = 0; q < nsft; ++q)
+    for (int q = 0; q < (zero_shift_only ? 1 : nsft); ++q)
         for (int c = 0; c < ncomp; ++c) {
             const int s = out.comp_start[c], b = out.comp_start[c + 1] - s;
             std::vector<double> A(static_cast<size_t>(b) * b, 0.0);

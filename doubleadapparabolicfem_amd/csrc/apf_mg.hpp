@@ -72,9 +72,11 @@ struct MGSetupHost {
 
 // Galerkin coarse operators, smoother bounds, time eigen-decomposition,
 // coarsest inverses.  Ksp/Msp are the finest spatial matrices on (S.sptr, S.scol).
+// zero_shift_only: the coarsest inverses of shift 0 alone (the slab solve's Y columns); the
+// other shifts' blocks stay zero.
 void setup_mg(const Space& S, const MGHierarchyHost& H, const std::vector<double>& Ksp,
               const std::vector<double>& Msp, const std::vector<double>& Mt, const std::vector<double>& Dt,
-              MGSetupHost& out);
+              MGSetupHost& out, bool zero_shift_only = false);
 
 // small dense symmetric eigen-decomposition (cyclic Jacobi), A = V diag(w) V^T
 void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V);
