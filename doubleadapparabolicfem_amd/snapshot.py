@@ -132,6 +132,7 @@ FUNCTION_TRUE_COEFFICIENTS_BINARY_FORMAT_VERSION = 1  # binary_format_versions.h
 TIME_SLAB_SPACE_METADATA_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:15
 TIME_SLAB_PROVENANCE_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:16
 TIME_SLAB_EDGE_PATCHES_BINARY_FORMAT_VERSION = 1  # binary_format_versions.hpp:17
+TIME_SLAB_VERTEX_PATCHES_BINARY_FORMAT_VERSION = 2  # binary_format_versions.hpp:18
 
 
 def write_space_mesh_binary(space, path):
@@ -298,3 +299,59 @@ def write_time_slab_edge_patches_binary(tss, path):
                   col("x_vertex", np.float64), col("t_begin", np.float64), col("t_end", np.float64),
                   ci["slab_cell_id"], ci["source_cell_id"], ci["left_vertex_id"], ci["right_vertex_id"], ci["side"],
                   cx["x_begin"], cx["x_end"]])
+
+
+def write_time_slab_vertex_patches_binary(mesh, path):
+    """The 2+1D vertex-patch file (format version 2, io/write_time_slab_vertex_patches_binary.hpp:
+    10-216 with detail/time_slab_patch_io_detail.hpp:9-103) of a tensor mesh T_h x {t_k}: slab k
+    is the interval [t_k, t_k+1] over the spatial mesh, one patch per (slab, spatial vertex)
+    (vertices ascending, the triangles holding the vertex ascending), kind 1 on the boundary.
+    Cell ids are this package's (slab cell = spatial triangle c, source cell k * ns + c), as in
+    write_mesh_binary.  Blocks: header [version, n_slabs, n_patches, max cells]; slab index;
+    per patch id, slab, vertex, kind, cell count, vertex (x, y), t_begin, t_end; per patch cell
+    (max cells per patch, -1 / 0.0 padded) slab cell, source cell, local vertex index (twice),
+    source vertex, the triangle's vertex ids (twice), its vertex coordinates and time interval."""
+    dim, V, C, T = _mesh_arrays(mesh)
+    if dim != 2:
+        raise ValueError("write_time_slab_vertex_patches_binary: 2+1D meshes only")
+    ns, m, nv = len(C), len(T) - 1, len(V)
+    cells_of = [[] for _ in range(nv)]
+    for c in range(ns):
+        for a in range(3):
+            cells_of[int(C[c, a])].append(c)
+    edge_count = {}
+    for c in range(ns):
+        for a, b in _TRI_FACE:
+            key = tuple(sorted((int(C[c, a]), int(C[c, b]))))
+            edge_count[key] = edge_count.get(key, 0) + 1
+    boundary = np.zeros(nv, dtype=bool)
+    for (a, b), n in edge_count.items():
+        if n == 1:
+            boundary[a] = boundary[b] = True
+    mc = max(len(x) for x in cells_of)
+    verts = [v for v in range(nv) if cells_of[v]]
+    n = m * len(verts)
+    pid = np.arange(n, dtype=np.int32)
+    slab = np.repeat(np.arange(m, dtype=np.int32), len(verts))
+    vid = np.tile(np.asarray(verts, dtype=np.int32), m)
+    kind = boundary[vid].astype(np.int32)
+    ncell = np.asarray([len(cells_of[v]) for v in vid], dtype=np.int32)
+    xv = V[vid].reshape(-1)
+    tb, te = T[slab].astype(np.float64), T[slab + 1].astype(np.float64)
+    flat = n * mc
+    sc, src, lv, sv = (np.full(flat, -1, dtype=np.int32) for _ in range(4))
+    tri = np.full(flat * 3, -1, dtype=np.int32)
+    xy = np.zeros(flat * 6)
+    ti = np.zeros(flat * 2)
+    for q in range(n):
+        v, k = int(vid[q]), int(slab[q])
+        for j, c in enumerate(sorted(cells_of[v])):
+            e = q * mc + j
+            sc[e], src[e], sv[e] = c, k * ns + c, v
+            lv[e] = int(np.flatnonzero(C[c] == v)[0])
+            tri[3 * e:3 * e + 3] = C[c]
+            xy[6 * e:6 * e + 6] = V[C[c]].reshape(-1)
+            ti[2 * e:2 * e + 2] = (T[k], T[k + 1])
+    offsets = np.arange(0, n + 1, len(verts), dtype=np.int32)
+    _write(path, [np.asarray([TIME_SLAB_VERTEX_PATCHES_BINARY_FORMAT_VERSION, m, n, mc], dtype=np.int32),
+                  offsets, pid, pid, slab, vid, kind, ncell, xv, tb, te, sc, src, lv, lv, sv, tri, tri, xy, ti])

@@ -457,3 +457,58 @@ def test_time_slab_edge_patches_1d(tmp_path):
     take(np.float64, 2 * npch)
     take(np.float64, 2 * npch)
     assert o == len(raw)
+
+
+def test_time_slab_vertex_patches_2d(tmp_path):
+    """write_time_slab_vertex_patches_binary on a tensor 2+1D mesh: one patch per (slab, vertex),
+    the vertex's triangles with the vertex at the recorded local index, boundary vertices of the
+    unit square marked, interior patches of the union-jack mesh holding 4 or 8 triangles; the file
+    parses back in the reference's block order."""
+    m = meshes.uniform_mesh(2, 1)
+    dim, V, C, T = snapshot._mesh_arrays(m)
+    path = tmp_path / "time_slab_vertex_patches.bin"
+    snapshot.write_time_slab_vertex_patches_binary(m, str(path))
+    raw = path.read_bytes()
+    o = 0
+
+    def take(dt, count):
+        nonlocal o
+        a = np.frombuffer(raw, dtype=dt, count=count, offset=o)
+        o += a.nbytes
+        return a
+
+    head = take(np.int32, 4)
+    ns_, nslab, nv = len(C), len(T) - 1, len(V)
+    assert list(head[:3]) == [2, nslab, nslab * nv]
+    n, mc = int(head[2]), int(head[3])
+    offs = take(np.int32, nslab + 1)
+    assert list(offs) == [k * nv for k in range(nslab + 1)]
+    take(np.int32, n)
+    assert list(take(np.int32, n)) == list(range(n))
+    slab = take(np.int32, n)
+    vid = take(np.int32, n)
+    kind = take(np.int32, n)
+    ncell = take(np.int32, n)
+    xv = take(np.float64, 2 * n).reshape(-1, 2)
+    tb, te = take(np.float64, n), take(np.float64, n)
+    sc, src, lv, lv2, sv = (take(np.int32, n * mc) for _ in range(5))
+    tri, tri2 = take(np.int32, 3 * n * mc), take(np.int32, 3 * n * mc)
+    xy, ti = take(np.float64, 6 * n * mc), take(np.float64, 2 * n * mc)
+    assert o == len(raw)
+    on_bnd = (np.abs(V[:, 0]) < 1e-14) | (np.abs(V[:, 0] - 1) < 1e-14) | (np.abs(V[:, 1]) < 1e-14) | \
+        (np.abs(V[:, 1] - 1) < 1e-14)
+    assert np.array_equal(kind.astype(bool), on_bnd[vid])
+    assert set(ncell[kind == 0]) <= {4, 8}
+    np.testing.assert_array_equal(xv, V[vid])
+    np.testing.assert_array_equal(tb, T[slab]) and np.testing.assert_array_equal(te, T[slab + 1])
+    for q in range(n):
+        for j in range(mc):
+            e = q * mc + j
+            if j >= ncell[q]:
+                assert sc[e] == -1
+                continue
+            c = sc[e]
+            assert src[e] == slab[q] * ns_ + c and C[c, lv[e]] == vid[q] and sv[e] == vid[q]
+            assert list(tri[3 * e:3 * e + 3]) == list(C[c]) and list(tri2[3 * e:3 * e + 3]) == list(C[c])
+            np.testing.assert_array_equal(xy[6 * e:6 * e + 6], V[C[c]].reshape(-1))
+            assert list(ti[2 * e:2 * e + 2]) == [T[slab[q]], T[slab[q] + 1]]
