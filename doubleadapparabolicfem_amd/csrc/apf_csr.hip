@@ -406,6 +406,38 @@ __global__ void __launch_bounds__(256) k_cheb_x_step(int64_t nY, int64_t n, cons
 }
 
 
+// ------------------------------------------------------------------ compact lambda block
+// entries of the leading block (columns < rows) of each of the first `rows` rows
+__global__ void k_blk_count(int64_t rows, const int64_t* __restrict__ rowptr, const int* __restrict__ col,
+                            int* __restrict__ cnt)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < rows;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int n = 0;
+        for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) n += (col[e] < rows) ? 1 : 0;
+        cnt[i] = n;
+    }
+}
+
+// copy those entries, in their order, into the compact CSR (bptr from the counts' prefix sums)
+__global__ void k_blk_fill(int64_t rows, const int64_t* __restrict__ rowptr, const int* __restrict__ col,
+                           const double* __restrict__ val, const int64_t* __restrict__ bptr, int* __restrict__ bcol,
+                           double* __restrict__ bval)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < rows;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int64_t o = bptr[i];
+        for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+            const int c = col[e];
+            if (c < rows) {
+                bcol[o] = c;
+                bval[o] = val[e];
+                ++o;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ block SPD solve
 // y = B x and the block partials of x . y for the leading block rows [0, rows) x
 // columns [0, rows) of a CSR matrix (columns >= rows are skipped).
@@ -549,9 +581,11 @@ struct Outcome {
 // on st and st2 (the default; APF_CSR_GRAPH=0 / APF_CSR_FORK=0 switch either off).  The
 // allocation-free entry passes false: graphs, streams and events are resources it must not
 // create.
+// Ay (may be NULL): K's leading lambda block as a compact CSR, read by the lambda block's
+// Chebyshev steps in place of K's lambda rows (which also hold B^T).
 Outcome run_minres(hipStream_t st, hipStream_t st2, const Dev& K, int64_t nY, const Dev* H, unsigned char* work,
                    const double* d_b, double* d_x, bool use_x0, double tol, int max_it, int check_interval,
-                   int degree, bool fp32, bool launch_opt)
+                   int degree, bool fp32, bool launch_opt, const Dev* Ay = nullptr)
 {
     const int64_t n = K.n, nX = n - nY;
     const WorkLayout L = work_layout(n);
@@ -667,7 +701,7 @@ Outcome run_minres(hipStream_t st, hipStream_t st2, const Dev& K, int64_t nY, co
         }
         if (H) cheb(nX, *H, dinv + nY, tt + nY, z + nY, part + G, 1, nY, sx);
         else cheb_schur(tt, z, part + G, sx);
-        cheb(nY, K, dinv, tt, z, part, 0, 0, st);
+        cheb(nY, Ay ? *Ay : K, dinv, tt, z, part, 0, 0, st);
         if (fork) {
             HIP_TRY(hipEventRecord(fk.b, fk.s2));
             HIP_TRY(hipStreamWaitEvent(st, fk.b, 0));
@@ -928,9 +962,46 @@ int apf_csr_minres_solve(apf_ctx* ctx, apf_csr_matrix* K, int64_t n_lambda, cons
         const Dev Kd{K->n, K->rowptr, K->col, K->val};
         Dev Hd{};
         if (Hhat) Hd = Dev{Hhat->n, Hhat->rowptr, Hhat->col, Hhat->val};
+        // K's lambda block as a compact CSR for the lambda block's Chebyshev steps: K's lambda rows
+        // also hold B^T, which those steps would read and skip (APF_CSR_COMPACT=0: read K's rows)
+        struct Blk {
+            int64_t* ptr = nullptr;
+            int* col = nullptr;
+            double* val = nullptr;
+            int* cnt = nullptr;
+            ~Blk()
+            {
+                if (ptr) (void)hipFree(ptr);
+                if (col) (void)hipFree(col);
+                if (val) (void)hipFree(val);
+                if (cnt) (void)hipFree(cnt);
+            }
+        } blk;
+        bool compact = true;
+        if (const char* e = std::getenv("APF_CSR_COMPACT")) compact = std::atoi(e) != 0;
+        Dev Ad{};
+        if (compact) {
+            const int64_t nY = n_lambda;
+            HIP_TRY(hipMalloc(&blk.cnt, nY * sizeof(int)));
+            hipLaunchKernelGGL(k_blk_count, dim3(kRowGrid), dim3(256), 0, st, nY, K->rowptr, K->col, blk.cnt);
+            std::vector<int> cnt(nY);
+            HIP_TRY(hipMemcpyAsync(cnt.data(), blk.cnt, nY * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            std::vector<int64_t> bptr(nY + 1, 0);
+            for (int64_t i = 0; i < nY; ++i) bptr[i + 1] = bptr[i] + cnt[i];
+            const int64_t nnzA = bptr[nY];
+            HIP_TRY(hipMalloc(&blk.ptr, (nY + 1) * sizeof(int64_t)));
+            HIP_TRY(hipMalloc(&blk.col, std::max<int64_t>(1, nnzA) * sizeof(int)));
+            HIP_TRY(hipMalloc(&blk.val, std::max<int64_t>(1, nnzA) * sizeof(double)));
+            HIP_TRY(hipMemcpyAsync(blk.ptr, bptr.data(), (nY + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_blk_fill, dim3(kRowGrid), dim3(256), 0, st, nY, K->rowptr, K->col, K->val, blk.ptr,
+                               blk.col, blk.val);
+            HIP_TRY(hipGetLastError());
+            Ad = Dev{nY, blk.ptr, blk.col, blk.val};
+        }
         const Outcome o = run_minres(st, ctx->stream2, Kd, n_lambda, Hhat ? &Hd : nullptr, work, d_b, d_x,
-                                     x0 != nullptr, tol,
-                                     max_it, check_interval, cheb_degree, precond_prec == APF_FP32_REFINE, true);
+                                     x0 != nullptr, tol, max_it, check_interval, cheb_degree,
+                                     precond_prec == APF_FP32_REFINE, true, compact ? &Ad : nullptr);
         HIP_TRY(hipMemcpyAsync(x, d_x, K->n * sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         fill_stats(stats, o, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
